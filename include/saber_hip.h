@@ -1,0 +1,157 @@
+/*
+ * saber_hip.h -- C ABI of the MI355X (gfx950) candidate-scoring library.
+ *
+ * This is the boundary a SABER host (the reference is a Rust crate) binds with
+ * an `extern "C"` block: plain pointers and sizes, caller-owned outputs, int
+ * status codes. It replaces, one entry point each:
+ *
+ *   saber_hip_score_windows  <- trait ScoreBackend::score_windows
+ *                               (src/gpu/mod.rs:45-59; CUDA impl
+ *                               src/gpu/cuda.rs:115-155, CPU twin
+ *                               src/gpu/cpu_fallback.rs:22-44)
+ *   saber_hip_score_bands    <- score_banded_many (src/algorithm/band_simd8.rs:78-185)
+ *                               and banded_score (src/algorithm/banded.rs:408-449),
+ *                               called from Engine::score_windows
+ *                               (src/search/engine.rs:1055, 1081)
+ *   saber_hip_score_full     <- score_many (src/algorithm/sw_simd.rs:95-153) with
+ *                               the end cell of sw_reference
+ *                               (src/algorithm/reference.rs:15-52)
+ *   saber_hip_xdrop_batch    <- xdrop_cat / xdrop_avx2 / ungapped_xdrop_prof
+ *                               (src/search/engine/scan.rs:859-904, 1024-1031;
+ *                               src/search/engine.rs:2425-2455), called from
+ *                               visit_scan (scan.rs:2414-2420)
+ *
+ * Conventions (src/gpu/mod.rs:9-31): residues are alignment codes 0..23
+ * (ARNDCQEGHILKMFPSTWYVBZX*), any code >= 24 is scored as X (22); a gap of
+ * length k costs gap_open + (k-1)*gap_extend; bias[i], when given, is added to
+ * every aligned column of query position i. Every score is the exact int32
+ * value of the reference CPU path. A call that cannot deliver exact results
+ * returns a nonzero status (the host then falls back, as
+ * Engine::score_windows_simd does on a backend error, engine.rs:1107-1116);
+ * saber_hip_last_error() describes the last failure of the calling thread.
+ *
+ * Threading: a context serialises its own launches (like the Mutex in
+ * src/gpu/cuda.rs:37, 144), so one context may be shared by threads.
+ * Device memory is owned by the context and its batches; outputs are caller
+ * owned host buffers.
+ */
+#ifndef SABER_HIP_H
+#define SABER_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SABER_HIP_OK 0
+#define SABER_HIP_EINVAL 1   /* bad argument (lengths, indices, null pointers) */
+#define SABER_HIP_EDEVICE 2  /* HIP runtime / device failure */
+#define SABER_HIP_ENOMEM 3   /* device allocation failed */
+#define SABER_HIP_ESTATE 4   /* call out of order (e.g. read before run) */
+
+/* Context options. */
+#define SABER_HIP_FORCE_EXACT32 1u /* score every band job with the int32 kernel (testing) */
+typedef struct saber_hip_opts {
+    uint32_t flags;
+    uint32_t reserved[7];
+} saber_hip_opts;
+
+/* Substitution matrix (24x24, row-major by query code) and SABER gap costs
+ * (GapCosts, src/scoring/stats.rs:59-103). */
+typedef struct saber_hip_scheme {
+    int32_t matrix[576];
+    int32_t gap_open;
+    int32_t gap_extend;
+} saber_hip_scheme;
+
+/* A set of sequences: residues concatenated, offsets[count] + 1 entries.
+ * bias (queries only) is NULL or holds one int8 per residue. */
+typedef struct saber_hip_seqset {
+    const uint8_t* residues;
+    const uint64_t* offsets;
+    uint32_t count;
+    const int8_t* bias;
+} saber_hip_seqset;
+
+/* One banded job: query `query` against the window [win_lo, win_hi) of
+ * subject `subject`, restricted to diagonals band_lo <= j - i <= band_hi in
+ * window coordinates (BandJob, src/algorithm/band_simd8.rs:29-33; Band,
+ * src/algorithm/banded.rs:30-54). */
+typedef struct saber_hip_band_job {
+    uint32_t query;
+    uint32_t subject;
+    uint32_t win_lo;
+    uint32_t win_hi;
+    int32_t band_lo;
+    int32_t band_hi;
+} saber_hip_band_job;
+
+typedef struct saber_hip_ctx saber_hip_ctx;
+typedef struct saber_hip_band_batch saber_hip_band_batch;
+
+/* Library / context. device is a HIP device ordinal. */
+const char* saber_hip_version(void);
+const char* saber_hip_last_error(void);
+int saber_hip_device_count(int* count);
+int saber_hip_create(int device, const saber_hip_opts* opts, saber_hip_ctx** out);
+void saber_hip_destroy(saber_hip_ctx* ctx);
+
+/* Make a subject database device resident (the residues/subject_offsets of
+ * SeedIndex, src/search/index.rs:644-703). Replaces any previous upload. */
+int saber_hip_upload_db(saber_hip_ctx* ctx, const saber_hip_seqset* subjects);
+
+/* ScoreBackend::score_windows: exact full-window SW scores of one query
+ * against n windows (windows[k] has lens[k] residues), in input order.
+ * bias may be NULL. An empty query gives all zeros. */
+int saber_hip_score_windows(saber_hip_ctx* ctx, const uint8_t* query, uint32_t qlen,
+                            const int8_t* bias, const uint8_t* const* windows,
+                            const uint32_t* lens, uint32_t n,
+                            const saber_hip_scheme* scheme, int32_t* out);
+
+/* Banded scores with banded_score's end cell and cell count: per job the best
+ * score, exclusive ends (q_end, s_end) of the first row-major optimal cell in
+ * window coordinates (0, 0 when the score is 0) and the band cells inside the
+ * rectangle. subjects == NULL selects the uploaded database. Any of q_end,
+ * s_end, cells may be NULL. */
+int saber_hip_score_bands(saber_hip_ctx* ctx, const saber_hip_seqset* queries,
+                          const saber_hip_seqset* subjects, const saber_hip_band_job* jobs,
+                          uint64_t njobs, const saber_hip_scheme* scheme, int32_t* score,
+                          uint32_t* q_end, uint32_t* s_end, uint64_t* cells);
+
+/* Device-resident batch: create uploads the inputs and plans the work once;
+ * run enqueues the whole scoring pass (asynchronous, on the context stream;
+ * may be repeated); wait blocks until it is done; read copies the results. */
+int saber_hip_band_batch_create(saber_hip_ctx* ctx, const saber_hip_seqset* queries,
+                                const saber_hip_seqset* subjects,
+                                const saber_hip_band_job* jobs, uint64_t njobs,
+                                const saber_hip_scheme* scheme, saber_hip_band_batch** out);
+int saber_hip_band_batch_run(saber_hip_band_batch* b);
+int saber_hip_band_batch_wait(saber_hip_band_batch* b);
+int saber_hip_band_batch_read(saber_hip_band_batch* b, int32_t* score, uint32_t* q_end,
+                              uint32_t* s_end, uint64_t* cells);
+void saber_hip_band_batch_destroy(saber_hip_band_batch* b);
+
+/* Timing of a batch: warmup + iters runs bracketed by HIP events on the
+ * context stream. ms_run = mean wall time of one run on the device;
+ * kernel_ms[v] = mean time of the launches of kernel variant v per run (v
+ * indexes saber_hip_band_variant); nvar is the capacity of kernel_ms on input
+ * and the number of variants on output. cells = band cells per run. */
+int saber_hip_band_batch_time(saber_hip_band_batch* b, int warmup, int iters, double* ms_run,
+                              double* kernel_ms, int* nvar, uint64_t* cells);
+
+/* Kernel variant table of the packed band kernel: variant v uses L lanes per
+ * job pair with C band cells per lane and covers clipped widths
+ * L*C-K+1 .. L*C; bias = 1 for the build that adds a query bias.
+ * Returns SABER_HIP_EINVAL when v is out of range. */
+int saber_hip_band_variant(int v, int* L, int* C, int* K, int* bias);
+
+/* Width -> variant choice used by the planner (the autotuner's output):
+ * widths lo..hi (inclusive) use variant shape (L, C, K); L = 0 routes them to
+ * the exact int32 kernel. */
+int saber_hip_band_choose(saber_hip_ctx* ctx, int w_lo, int w_hi, int L, int C, int K);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SABER_HIP_H */

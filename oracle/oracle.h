@@ -1,0 +1,100 @@
+/*
+ * CPU oracle for the candidate-scoring hot path -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain-C restatement of the reference's scalar algorithms (SABER 1.4.0,
+ * raw-lab/saber). Only tests/, __graft_entry__.smoke() and the cpu_baseline leg
+ * of bench.py may load this library, and only as the checker; the product path
+ * (saber_amd/, libsaber_hip.so) never links or calls it.
+ *
+ * Conventions shared with the reference (src/gpu/mod.rs:9-31):
+ *   residues are alignment codes 0..23 (ARNDCQEGHILKMFPSTWYVBZX*), any code
+ *   >= 24 scores as X (22); a gap of length k costs open + (k-1)*extend; an
+ *   optional bias[i] is added to every aligned column of query position i.
+ *   Matrices are 24x24 int32, row-major (576 entries).
+ *
+ * Pinning (how each function is checked, see tests/test_oracle.py):
+ *   or_sw_reference       brute-force enumeration of all local alignments on
+ *                         tiny inputs; reference known answers (WAGNER=37 at
+ *                         (6,6), Wx4000=44000, Wx3500=38500, nucleotide 20/8);
+ *                         the reference's own CUDA kernel source compiled as
+ *                         C++ (oracle/_ref, when /root/reference is present).
+ *   or_banded_score       full band == or_sw_reference incl. end cell; band
+ *                         score <= full score; explicit (i,j)-coordinate DP
+ *                         with -inf outside the band (independent formulation);
+ *                         Wx300 band +-5 = 3300 known answer.
+ *   or_xdrop              closed form with x -> infinity (best prefix / suffix
+ *                         sums), brute-force bound best_ungapped_on_diagonal,
+ *                         and a direct restatement of the scalar twin
+ *                         (engine.rs:2460-2494) that must agree.
+ *   or_composition_bias   the reference's invariants (non-standard residues
+ *                         get 0; poly-Q <= -3; diverse mean < 1.5) plus an
+ *                         O(n*w) direct evaluation of the formula.
+ */
+#ifndef SABER_ORACLE_H
+#define SABER_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* src/alphabet.rs:39-41 */
+int or_code_index(uint8_t c);
+
+/* src/algorithm/reference.rs:15-52: full Gotoh SW in int64, first row-major
+ * optimum. Writes score and exclusive ends. */
+void or_sw_reference(const uint8_t* q, int64_t m, const uint8_t* s, int64_t n,
+                     const int32_t* mat, const int8_t* bias, int64_t go, int64_t ge,
+                     int64_t* score, int64_t* q_end, int64_t* s_end);
+
+/* src/algorithm/banded.rs:123-224 (scalar kernel) with the dispatcher's result
+ * convention of banded.rs:408-449 (score <= 0 -> zero ends, cells kept). */
+void or_banded_score(const uint8_t* q, int64_t m, const uint8_t* s, int64_t n,
+                     const int8_t* bias, const int32_t* mat, int32_t go, int32_t ge,
+                     int64_t band_lo, int64_t band_hi,
+                     int32_t* score, int64_t* q_end, int64_t* s_end, uint64_t* cells);
+
+/* Independent check of or_banded_score: the same local DP written in (i, j)
+ * matrix coordinates with -inf outside lo <= j-i <= hi (O(m*n) memory). */
+void or_banded_score_ij(const uint8_t* q, int64_t m, const uint8_t* s, int64_t n,
+                        const int8_t* bias, const int32_t* mat, int32_t go, int32_t ge,
+                        int64_t band_lo, int64_t band_hi,
+                        int32_t* score, int64_t* q_end, int64_t* s_end);
+
+/* Batched form used by tests and the CPU baseline: job k scores
+ * q[qoff[k] .. qoff[k]+qlen[k]) against s[soff[k] .. soff[k]+slen[k]) in band
+ * [lo[k], hi[k]] with bias (NULL or indexed like q). */
+void or_banded_many(const uint8_t* qres, const uint8_t* sres, const int8_t* bias,
+                    const uint64_t* qoff, const uint32_t* qlen,
+                    const uint64_t* soff, const uint32_t* slen,
+                    const int32_t* lo, const int32_t* hi, int64_t njobs,
+                    const int32_t* mat, int32_t go, int32_t ge,
+                    int32_t* score, uint32_t* q_end, uint32_t* s_end, uint64_t* cells,
+                    int nthreads);
+
+/* src/search/engine/scan.rs:859-904 (xdrop_cat; identical results to
+ * ungapped_xdrop_prof engine.rs:2425-2455). Right extension includes the seed
+ * pair (qp, sp); left starts at (qp-1, sp-1). Returns score, half-open query
+ * range [q_lo, q_hi) and the step count of xdrop_cat. */
+void or_xdrop(const uint8_t* q, int64_t m, const int8_t* bias, const uint8_t* s, int64_t n,
+              const int32_t* mat, int64_t qp, int64_t sp, int32_t x,
+              int32_t* score, int64_t* q_lo, int64_t* q_hi, uint64_t* steps);
+
+/* Direct restatement of the scalar twin ungapped_xdrop (engine.rs:2460-2494),
+ * used only to cross-check or_xdrop. */
+void or_xdrop_scalar_twin(const uint8_t* q, int64_t m, const int8_t* bias, const uint8_t* s,
+                          int64_t n, const int32_t* mat, int64_t qp, int64_t sp, int32_t x,
+                          int32_t* score, int64_t* q_lo, int64_t* q_hi);
+
+/* src/algorithm/reference.rs:106-119 */
+int64_t or_best_ungapped_on_diagonal(const uint8_t* q, int64_t m, const uint8_t* s, int64_t n,
+                                     int64_t d, const int32_t* mat, const int8_t* bias);
+
+/* src/scoring/bias.rs:56-122 */
+void or_composition_bias(const uint8_t* q, int64_t n, const int32_t* mat, int8_t* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

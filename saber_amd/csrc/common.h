@@ -1,0 +1,78 @@
+// Shared device/host definitions for the SABER MI355X scoring library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace saber {
+
+// Alignment codes 0..23 (ARNDCQEGHILKMFPSTWYVBZX*); any code >= 24 scores as
+// X (src/alphabet.rs:39-41). Device kernels see subject codes pre-doubled
+// (byte offsets into a row of 16-bit table entries) and one extra code:
+constexpr int kCodeX = 22;
+constexpr int kSentinel = 31;        // marks a cell column outside the subject
+constexpr int kTableDim = 32;        // 32 x 32 int16 score table in LDS
+constexpr short kNeg16 = -32768;     // unreachable (saturating arithmetic keeps it)
+constexpr short kMax16 = 32767;      // any job whose best reaches this is rescored in int32
+
+// One banded job after planning (band clipped to its rectangle, rows that
+// hold band cells). Band cell k of row i is subject column j = i + lo + k.
+struct BandJobDev {
+    uint64_t q_off;     // first query residue (and bias entry) in the query blob
+    uint64_t s_off;     // first subject-window residue in the residue blob
+    uint32_t m, n;      // query / window lengths
+    int32_t lo;         // clipped band lo (diagonal j - i)
+    int32_t w;          // clipped band width
+    int32_t i_first;    // first row holding a band cell
+    int32_t rows;       // number of rows holding band cells
+};
+
+struct BandOutputs {
+    int32_t* score;
+    uint32_t* q_end;
+    uint32_t* s_end;
+    uint64_t* cells;
+};
+
+struct BandKernelArgs {
+    const uint32_t* pairs;      // 2 job ids per pair (0xFFFFFFFF = none)
+    uint32_t npairs;
+    const BandJobDev* jobs;
+    const uint8_t* qres;
+    const int8_t* qbias;        // may be null when the kernel is built without bias
+    const uint8_t* sres;
+    const short* table;         // 32 x 32 int16 (global; staged into LDS)
+    int32_t go, ge;
+    BandOutputs out;
+    uint32_t* flagged;          // job ids needing the exact int32 rescore
+    uint32_t* nflagged;
+};
+
+// Generic exact int32 kernel over a list of job ids.
+struct GenericArgs {
+    const uint32_t* ids;
+    uint32_t nids;
+    const BandJobDev* jobs;
+    const uint8_t* qres;
+    const int8_t* qbias;
+    const uint8_t* sres;
+    const int32_t* mat;         // 24 x 24 int32
+    int32_t go, ge;
+    BandOutputs out;
+    int32_t* work;              // per id: 2 * (w + 1) int32 at work_off[id]
+    const uint64_t* work_off;
+};
+
+// One compiled variant of the packed band kernel: L lanes per job pair,
+// C band cells per lane, K capped trailing cells (band width W = L*C covers
+// widths W-K+1 .. W).
+struct BandVariant {
+    int L, C, K;
+    int bias;
+    hipError_t (*launch)(const BandKernelArgs&, hipStream_t);
+};
+
+const BandVariant* band_variants(int* count);
+hipError_t launch_generic(const GenericArgs& a, hipStream_t s);
+
+}  // namespace saber

@@ -1,0 +1,100 @@
+// Exact int32 banded Smith-Waterman kernel: one thread per job.
+//
+// The rescue path of the packed kernel (jobs whose int16 best reached 32767,
+// bands wider than the packed variants, unusual scoring parameters) and the
+// reference for it in tests. It follows the scalar recurrence of
+// banded_score_scalar (src/algorithm/banded.rs:134-224) literally, with the
+// two band rows in a per-job slice of a device work buffer.
+#include "common.h"
+
+namespace saber {
+
+constexpr int32_t kNeg32 = INT32_MIN / 4;
+
+__device__ __forceinline__ int32_t imax(int32_t a, int32_t b) { return a > b ? a : b; }
+
+__global__ __launch_bounds__(128) void generic_band_kernel(GenericArgs a) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.nids) return;
+    const uint32_t id = a.ids[t];
+    const BandJobDev J = a.jobs[id];
+    const int64_t w = J.w;
+    int32_t* hp = a.work + a.work_off[t];
+    int32_t* fp = hp + (w + 1);
+    int32_t* hc = fp + (w + 1);
+    int32_t* fc = hc + (w + 1);
+    for (int64_t k = 0; k <= w; ++k) {
+        hp[k] = kNeg32;
+        fp[k] = kNeg32;
+        hc[k] = kNeg32;
+        fc[k] = kNeg32;
+    }
+    const uint8_t* q = a.qres + J.q_off;
+    const uint8_t* s = a.sres + J.s_off;
+    const int8_t* bias = a.qbias ? a.qbias + J.q_off : nullptr;
+    const int64_t m = J.m, n = J.n, lo = J.lo;
+    const int32_t go = a.go, ge = a.ge;
+    int32_t best = 0;
+    int64_t bq = 0, bs = 0;
+    uint64_t cells = 0;
+    // Rows before i_first hold no band cell: all of their entries are -inf in
+    // the scalar kernel, which is the initial state here.
+    const int64_t i_end = (int64_t)J.i_first + J.rows;
+    for (int64_t i = J.i_first; i < i_end; ++i) {
+        int64_t kmin = -(i + lo);
+        if (kmin < 0) kmin = 0;
+        int64_t kmax = n - 1 - i - lo;
+        if (kmax > w - 1) kmax = w - 1;
+        const uint32_t qc = q[i] < 24 ? q[i] : (uint32_t)kCodeX;
+        const int32_t* row = a.mat + qc * 24;
+        const int32_t b = bias ? (int32_t)bias[i] : 0;
+        int32_t e = kNeg32, hl = kNeg32;
+        for (int64_t k = 0; k < kmin; ++k) {
+            hc[k] = kNeg32;
+            fc[k] = kNeg32;
+        }
+        for (int64_t k = kmin; k <= kmax; ++k) {
+            const int64_t j = i + lo + k;
+            const int32_t ecand = imax(hl - go, e - ge);
+            e = k > kmin ? ecand : kNeg32;
+            const int32_t f = i >= 1 ? imax(hp[k + 1] - go, fp[k + 1] - ge) : kNeg32;
+            const int32_t hd = (i >= 1 && j >= 1) ? hp[k] : 0;
+            const uint32_t sc = s[j] < 24 ? s[j] : (uint32_t)kCodeX;
+            const int32_t diag = hd + row[sc] + b;
+            const int32_t h = imax(imax(diag, e), imax(f, 0));
+            hc[k] = h;
+            fc[k] = f;
+            hl = h;
+            if (h > best) {
+                best = h;
+                bq = i + 1;
+                bs = j + 1;
+            }
+        }
+        for (int64_t k = kmax + 1; k < w; ++k) {
+            hc[k] = kNeg32;
+            fc[k] = kNeg32;
+        }
+        cells += (uint64_t)(kmax - kmin + 1);
+        int32_t* x = hp;
+        hp = hc;
+        hc = x;
+        x = fp;
+        fp = fc;
+        fc = x;
+    }
+    a.out.score[id] = best;
+    a.out.q_end[id] = best > 0 ? (uint32_t)bq : 0u;
+    a.out.s_end[id] = best > 0 ? (uint32_t)bs : 0u;
+    a.out.cells[id] = cells;
+}
+
+hipError_t launch_generic(const GenericArgs& a, hipStream_t s) {
+    if (a.nids == 0) return hipSuccess;
+    const uint32_t block = 128;
+    const uint32_t grid = (a.nids + block - 1) / block;
+    hipLaunchKernelGGL(generic_band_kernel, dim3(grid), dim3(block), 0, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace saber
