@@ -135,10 +135,12 @@ void saber_hip_band_batch_destroy(saber_hip_band_batch* b);
 /* Timing of a batch: warmup + iters runs bracketed by HIP events on the
  * context stream. ms_run = mean wall time of one run on the device;
  * kernel_ms[v] = mean time of the launches of kernel variant v per run (v
- * indexes saber_hip_band_variant); nvar is the capacity of kernel_ms on input
- * and the number of variants on output. cells = band cells per run. */
+ * indexes saber_hip_band_variant) and kernel_cells[v] the band cells those
+ * launches score; nvar is the capacity of both arrays on input and the number
+ * of variants on output. cells = band cells per run. */
 int saber_hip_band_batch_time(saber_hip_band_batch* b, int warmup, int iters, double* ms_run,
-                              double* kernel_ms, int* nvar, uint64_t* cells);
+                              double* kernel_ms, uint64_t* kernel_cells, int* nvar,
+                              uint64_t* cells);
 
 /* Kernel variant table of the packed band kernel: variant v uses L lanes per
  * job pair with C band cells per lane and covers clipped widths

@@ -1,0 +1,147 @@
+"""ctypes loader of the CPU oracle (liboracle.so) -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this
+module, and only as the checker. The functions restate the reference's scalar
+path; see oracle.h for the reference lines and how each one is pinned.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+AVX2_LIB = os.path.join(HERE, "libcpu_avx2.so")
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE, "all"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        u8p, i8p, i32p, i64p, u64p, u32p = (C.POINTER(C.c_uint8), C.POINTER(C.c_int8),
+                                           C.POINTER(C.c_int32), C.POINTER(C.c_int64),
+                                           C.POINTER(C.c_uint64), C.POINTER(C.c_uint32))
+        L.or_sw_reference.argtypes = [u8p, C.c_int64, u8p, C.c_int64, i32p, i8p, C.c_int64, C.c_int64,
+                                      i64p, i64p, i64p]
+        L.or_banded_score.argtypes = [u8p, C.c_int64, u8p, C.c_int64, i8p, i32p, C.c_int32, C.c_int32,
+                                      C.c_int64, C.c_int64, i32p, i64p, i64p, u64p]
+        L.or_banded_score_ij.argtypes = [u8p, C.c_int64, u8p, C.c_int64, i8p, i32p, C.c_int32,
+                                         C.c_int32, C.c_int64, C.c_int64, i32p, i64p, i64p]
+        L.or_banded_many.argtypes = [u8p, u8p, i8p, u64p, u32p, u64p, u32p, i32p, i32p, C.c_int64,
+                                     i32p, C.c_int32, C.c_int32, i32p, u32p, u32p, u64p, C.c_int]
+        L.or_xdrop.argtypes = [u8p, C.c_int64, i8p, u8p, C.c_int64, i32p, C.c_int64, C.c_int64,
+                               C.c_int32, i32p, i64p, i64p, u64p]
+        L.or_xdrop_scalar_twin.argtypes = [u8p, C.c_int64, i8p, u8p, C.c_int64, i32p, C.c_int64,
+                                           C.c_int64, C.c_int32, i32p, i64p, i64p]
+        L.or_best_ungapped_on_diagonal.argtypes = [u8p, C.c_int64, u8p, C.c_int64, C.c_int64, i32p, i8p]
+        L.or_best_ungapped_on_diagonal.restype = C.c_int64
+        L.or_composition_bias.argtypes = [u8p, C.c_int64, i32p, i8p]
+        _lib = L
+    return _lib
+
+
+def _u8(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.uint8))
+
+
+def _p(a, t):
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+def _mat(m):
+    return np.ascontiguousarray(np.asarray(m, dtype=np.int32).reshape(576))
+
+
+def _bias(b):
+    return None if b is None else np.ascontiguousarray(np.asarray(b, dtype=np.int8))
+
+
+def sw_reference(q, s, matrix, bias, go, ge):
+    q, s, mat, b = _u8(q), _u8(s), _mat(matrix), _bias(bias)
+    sc, qe, se = C.c_int64(), C.c_int64(), C.c_int64()
+    lib().or_sw_reference(_p(q, C.c_uint8), len(q), _p(s, C.c_uint8), len(s), _p(mat, C.c_int32),
+                          _p(b, C.c_int8), go, ge, C.byref(sc), C.byref(qe), C.byref(se))
+    return sc.value, qe.value, se.value
+
+
+def banded_score(q, s, bias, matrix, go, ge, lo, hi):
+    q, s, mat, b = _u8(q), _u8(s), _mat(matrix), _bias(bias)
+    sc, qe, se, ce = C.c_int32(), C.c_int64(), C.c_int64(), C.c_uint64()
+    lib().or_banded_score(_p(q, C.c_uint8), len(q), _p(s, C.c_uint8), len(s), _p(b, C.c_int8),
+                          _p(mat, C.c_int32), go, ge, lo, hi, C.byref(sc), C.byref(qe), C.byref(se),
+                          C.byref(ce))
+    return sc.value, qe.value, se.value, ce.value
+
+
+def banded_score_ij(q, s, bias, matrix, go, ge, lo, hi):
+    q, s, mat, b = _u8(q), _u8(s), _mat(matrix), _bias(bias)
+    sc, qe, se = C.c_int32(), C.c_int64(), C.c_int64()
+    lib().or_banded_score_ij(_p(q, C.c_uint8), len(q), _p(s, C.c_uint8), len(s), _p(b, C.c_int8),
+                             _p(mat, C.c_int32), go, ge, lo, hi, C.byref(sc), C.byref(qe), C.byref(se))
+    return sc.value, qe.value, se.value
+
+
+def banded_many(qres, qoff, sres, soff, jobs, matrix, go, ge, qbias=None, nthreads=8):
+    """Oracle for a band batch given as flat arrays (same layout as the GPU
+    batch). Returns (score, q_end, s_end, cells)."""
+    qres, sres = _u8(qres), _u8(sres)
+    qoff = np.asarray(qoff, np.uint64)
+    soff = np.asarray(soff, np.uint64)
+    jq = np.ascontiguousarray(qoff[jobs["query"]])
+    jql = np.ascontiguousarray((qoff[jobs["query"] + 1] - qoff[jobs["query"]]).astype(np.uint32))
+    js = np.ascontiguousarray(soff[jobs["subject"]] + jobs["win_lo"].astype(np.uint64))
+    jsl = np.ascontiguousarray((jobs["win_hi"] - jobs["win_lo"]).astype(np.uint32))
+    lo = np.ascontiguousarray(jobs["band_lo"].astype(np.int32))
+    hi = np.ascontiguousarray(jobs["band_hi"].astype(np.int32))
+    n = len(jobs)
+    score = np.zeros(n, np.int32)
+    qe = np.zeros(n, np.uint32)
+    se = np.zeros(n, np.uint32)
+    ce = np.zeros(n, np.uint64)
+    b = _bias(qbias)
+    mat = _mat(matrix)
+    lib().or_banded_many(_p(qres, C.c_uint8), _p(sres, C.c_uint8), _p(b, C.c_int8), _p(jq, C.c_uint64),
+                         _p(jql, C.c_uint32), _p(js, C.c_uint64), _p(jsl, C.c_uint32), _p(lo, C.c_int32),
+                         _p(hi, C.c_int32), n, _p(mat, C.c_int32), go, ge, _p(score, C.c_int32),
+                         _p(qe, C.c_uint32), _p(se, C.c_uint32), _p(ce, C.c_uint64), nthreads)
+    return score, qe, se, ce
+
+
+def xdrop(q, s, bias, matrix, qp, sp, x):
+    q, s, mat, b = _u8(q), _u8(s), _mat(matrix), _bias(bias)
+    sc, lo_, hi_, st = C.c_int32(), C.c_int64(), C.c_int64(), C.c_uint64()
+    lib().or_xdrop(_p(q, C.c_uint8), len(q), _p(b, C.c_int8), _p(s, C.c_uint8), len(s),
+                   _p(mat, C.c_int32), qp, sp, x, C.byref(sc), C.byref(lo_), C.byref(hi_), C.byref(st))
+    return sc.value, lo_.value, hi_.value, st.value
+
+
+def xdrop_scalar_twin(q, s, bias, matrix, qp, sp, x):
+    q, s, mat, b = _u8(q), _u8(s), _mat(matrix), _bias(bias)
+    sc, lo_, hi_ = C.c_int32(), C.c_int64(), C.c_int64()
+    lib().or_xdrop_scalar_twin(_p(q, C.c_uint8), len(q), _p(b, C.c_int8), _p(s, C.c_uint8), len(s),
+                               _p(mat, C.c_int32), qp, sp, x, C.byref(sc), C.byref(lo_), C.byref(hi_))
+    return sc.value, lo_.value, hi_.value
+
+
+def best_ungapped_on_diagonal(q, s, d, matrix, bias):
+    q, s, mat, b = _u8(q), _u8(s), _mat(matrix), _bias(bias)
+    return lib().or_best_ungapped_on_diagonal(_p(q, C.c_uint8), len(q), _p(s, C.c_uint8), len(s), d,
+                                              _p(mat, C.c_int32), _p(b, C.c_int8))
+
+
+def composition_bias(q, matrix):
+    q, mat = _u8(q), _mat(matrix)
+    out = np.zeros(len(q), np.int8)
+    lib().or_composition_bias(_p(q, C.c_uint8), len(q), _p(mat, C.c_int32), _p(out, C.c_int8))
+    return out

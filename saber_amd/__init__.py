@@ -1,0 +1,351 @@
+"""MI355X (gfx950) candidate-scoring path of SABER, behind SABER's own scoring
+interfaces.
+
+Host mirror of the reference's band-scoring and backend API (names, argument
+meaning and error behaviour follow the Rust originals):
+
+  Band, BandJob, BandScore, Scheme, GapCosts   src/algorithm/banded.rs:21-99,
+                                               band_simd8.rs:29-33, scoring/stats.rs:59-103
+  banded_score(q, s, bias, scheme, band)       src/algorithm/banded.rs:408-449
+  score_banded_many(query, bias, jobs, scheme) src/algorithm/band_simd8.rs:78-185
+  score_many(query, bias, subjects, matrix, gaps)
+                                               src/algorithm/sw_simd.rs:95-153
+  HipBackend.score_windows(...)                trait ScoreBackend, src/gpu/mod.rs:45-59
+
+Every call runs the HIP kernels of libsaber_hip.so on the GPU; there is no
+CPU path here. Inputs are alignment codes 0..23 (codes >= 24 score as X).
+"""
+from dataclasses import dataclass
+import ctypes as C
+
+import numpy as np
+
+from . import _ffi
+from ._ffi import HipError
+from .matrices import ALPHABET, BLOSUM45, BLOSUM62, PAM30
+
+__all__ = [
+    "ALPHABET", "BLOSUM45", "BLOSUM62", "PAM30", "Band", "BandJob", "BandScore", "GapCosts",
+    "Scheme", "Context", "HipBackend", "HipError", "banded_score", "score_banded_many",
+    "score_many", "encode", "default_context",
+]
+
+_ENCODE = np.full(256, 22, dtype=np.uint8)
+for _i, _c in enumerate(ALPHABET):
+    _ENCODE[ord(_c)] = _i
+    if "A" <= _c <= "Z":
+        _ENCODE[ord(_c.lower())] = _i
+
+
+def encode(ascii_seq):
+    """ASCII protein -> alignment codes (src/alphabet.rs:48-78: unknown -> X)."""
+    if isinstance(ascii_seq, str):
+        ascii_seq = ascii_seq.encode()
+    return _ENCODE[np.frombuffer(bytes(ascii_seq), dtype=np.uint8)]
+
+
+@dataclass(frozen=True)
+class GapCosts:
+    """SABER gap convention: a gap of length k costs open + (k-1)*extend."""
+    open: int
+    extend: int
+
+
+@dataclass(frozen=True)
+class Band:
+    """Inclusive diagonal band lo <= j - i <= hi (src/algorithm/banded.rs:30-54)."""
+    lo: int
+    hi: int
+
+    @staticmethod
+    def full(m, n):
+        return Band(-int(m), int(n))
+
+    def width(self):
+        return max(self.hi - self.lo + 1, 0)
+
+    def clip(self, m, n):
+        return Band(max(self.lo, -(int(m) - 1)), min(self.hi, int(n) - 1))
+
+
+@dataclass(frozen=True)
+class BandScore:
+    score: int
+    q_end: int
+    s_end: int
+    cells: int
+
+
+@dataclass(frozen=True)
+class BandJob:
+    subject: object
+    band: Band
+
+
+@dataclass(frozen=True)
+class Scheme:
+    matrix: tuple
+    gap_open: int
+    gap_extend: int
+
+    def ffi(self):
+        s = _ffi.Scheme()
+        flat = np.asarray(self.matrix, dtype=np.int32).reshape(576)
+        s.matrix[:] = [int(v) for v in flat]
+        s.gap_open = int(self.gap_open)
+        s.gap_extend = int(self.gap_extend)
+        return s
+
+
+def _codes(x):
+    return np.ascontiguousarray(np.asarray(x, dtype=np.uint8))
+
+
+def _seqset(seqs, bias=None):
+    """Concatenate sequences (list of code arrays) into an ffi SeqSet."""
+    lens = np.fromiter((len(s) for s in seqs), dtype=np.uint64, count=len(seqs))
+    off = np.zeros(len(seqs) + 1, dtype=np.uint64)
+    np.cumsum(lens, out=off[1:])
+    res = np.concatenate([_codes(s) for s in seqs]) if len(seqs) and off[-1] else np.zeros(1, np.uint8)
+    res = np.ascontiguousarray(res, dtype=np.uint8)
+    b = None
+    if bias is not None:
+        b = np.ascontiguousarray(np.concatenate([np.asarray(x, dtype=np.int8) for x in bias]) if off[-1]
+                                 else np.zeros(1, np.int8))
+    ss = _ffi.SeqSet(_ffi.ptr(res, C.c_uint8), _ffi.ptr(off, C.c_uint64), len(seqs),
+                     _ffi.ptr(b, C.c_int8))
+    return ss, (res, off, b)
+
+
+class BandBatch:
+    """Device-resident band batch (saber_hip_band_batch_*)."""
+
+    def __init__(self, ctx, handle, njobs, keep):
+        self.ctx, self.h, self.njobs, self._keep = ctx, handle, njobs, keep
+
+    def run(self):
+        _ffi.check(_ffi.lib().saber_hip_band_batch_run(self.h))
+
+    def wait(self):
+        _ffi.check(_ffi.lib().saber_hip_band_batch_wait(self.h))
+
+    def read(self):
+        n = self.njobs
+        score = np.zeros(n, np.int32)
+        qe = np.zeros(n, np.uint32)
+        se = np.zeros(n, np.uint32)
+        cells = np.zeros(n, np.uint64)
+        _ffi.check(_ffi.lib().saber_hip_band_batch_read(
+            self.h, _ffi.ptr(score, C.c_int32), _ffi.ptr(qe, C.c_uint32), _ffi.ptr(se, C.c_uint32),
+            _ffi.ptr(cells, C.c_uint64)))
+        return score, qe, se, cells
+
+    def time(self, warmup=1, iters=5):
+        """(ms per run, {variant index: (kernel ms per run, cells per run)},
+        cells per run)."""
+        ms = C.c_double()
+        cap = 256
+        km = (C.c_double * cap)()
+        kc = (C.c_uint64 * cap)()
+        nv = C.c_int(cap)
+        cells = C.c_uint64()
+        _ffi.check(_ffi.lib().saber_hip_band_batch_time(self.h, warmup, iters, C.byref(ms), km, kc,
+                                                          C.byref(nv), C.byref(cells)))
+        per = {v: (km[v], kc[v]) for v in range(min(nv.value, cap)) if km[v] > 0}
+        return ms.value, per, cells.value
+
+    def close(self):
+        if self.h:
+            _ffi.lib().saber_hip_band_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def band_variants():
+    """[(L, C, K, bias)] for every compiled packed-kernel variant."""
+    out = []
+    v = 0
+    L_, C_, K_, B_ = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+    while _ffi.lib().saber_hip_band_variant(v, C.byref(L_), C.byref(C_), C.byref(K_), C.byref(B_)) == 0:
+        out.append((L_.value, C_.value, K_.value, B_.value))
+        v += 1
+    return out
+
+
+class Context:
+    """One device context (saber_hip_create)."""
+
+    def __init__(self, device=0, force_exact32=False):
+        L = _ffi.lib()
+        opts = _ffi.Opts()
+        opts.flags = _ffi.FORCE_EXACT32 if force_exact32 else 0
+        h = C.c_void_p()
+        _ffi.check(L.saber_hip_create(device, C.byref(opts), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            _ffi.lib().saber_hip_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def choose(self, w_lo, w_hi, L, Cc, K):
+        _ffi.check(_ffi.lib().saber_hip_band_choose(self.h, w_lo, w_hi, L, Cc, K))
+
+    def upload_db(self, qres, qoff):
+        off = np.ascontiguousarray(qoff, dtype=np.uint64)
+        res = np.ascontiguousarray(qres, dtype=np.uint8)
+        ss = _ffi.SeqSet(_ffi.ptr(res, C.c_uint8), _ffi.ptr(off, C.c_uint64), len(off) - 1, None)
+        _ffi.check(_ffi.lib().saber_hip_upload_db(self.h, C.byref(ss)))
+
+    @staticmethod
+    def _jobs(query_idx, subject_idx, win_lo, win_hi, band_lo, band_hi):
+        n = len(query_idx)
+        jobs = np.zeros(n, dtype=[("query", "<u4"), ("subject", "<u4"), ("win_lo", "<u4"),
+                                  ("win_hi", "<u4"), ("band_lo", "<i4"), ("band_hi", "<i4")])
+        jobs["query"] = query_idx
+        jobs["subject"] = subject_idx
+        jobs["win_lo"] = win_lo
+        jobs["win_hi"] = win_hi
+        jobs["band_lo"] = band_lo
+        jobs["band_hi"] = band_hi
+        return jobs
+
+    def band_batch(self, qres, qoff, sres, soff, jobs, scheme, qbias=None):
+        """Create a device-resident batch from flat arrays (jobs: structured
+        array with query, subject, win_lo, win_hi, band_lo, band_hi)."""
+        qres = np.ascontiguousarray(qres, np.uint8)
+        qoff = np.ascontiguousarray(qoff, np.uint64)
+        qb = None if qbias is None else np.ascontiguousarray(qbias, np.int8)
+        qs = _ffi.SeqSet(_ffi.ptr(qres, C.c_uint8), _ffi.ptr(qoff, C.c_uint64), len(qoff) - 1,
+                         _ffi.ptr(qb, C.c_int8))
+        keep = [qres, qoff, qb]
+        if sres is not None:
+            sres = np.ascontiguousarray(sres, np.uint8)
+            soff = np.ascontiguousarray(soff, np.uint64)
+            ss = _ffi.SeqSet(_ffi.ptr(sres, C.c_uint8), _ffi.ptr(soff, C.c_uint64), len(soff) - 1, None)
+            sp = C.byref(ss)
+            keep += [sres, soff]
+        else:
+            sp = None
+        jobs = np.ascontiguousarray(jobs)
+        keep.append(jobs)
+        sch = scheme.ffi()
+        h = C.c_void_p()
+        _ffi.check(_ffi.lib().saber_hip_band_batch_create(
+            self.h, C.byref(qs), sp, jobs.ctypes.data_as(C.POINTER(_ffi.BandJob)), len(jobs),
+            C.byref(sch), C.byref(h)))
+        return BandBatch(self, h, len(jobs), keep)
+
+    def score_bands(self, qres, qoff, sres, soff, jobs, scheme, qbias=None):
+        """(score, q_end, s_end, cells) arrays, one entry per job."""
+        b = self.band_batch(qres, qoff, sres, soff, jobs, scheme, qbias)
+        try:
+            b.run()
+            return b.read()
+        finally:
+            b.close()
+
+    def score_windows(self, query, bias, windows, scheme):
+        q = _codes(query)
+        n = len(windows)
+        out = np.zeros(n, np.int32)
+        if n == 0:
+            return out
+        wins = [_codes(w) for w in windows]
+        ptrs = (C.POINTER(C.c_uint8) * n)(*[w.ctypes.data_as(C.POINTER(C.c_uint8)) for w in wins])
+        lens = np.array([len(w) for w in wins], dtype=np.uint32)
+        b = None if bias is None else np.ascontiguousarray(bias, np.int8)
+        sch = scheme.ffi()
+        _ffi.check(_ffi.lib().saber_hip_score_windows(
+            self.h, _ffi.ptr(q, C.c_uint8), len(q), _ffi.ptr(b, C.c_int8), ptrs,
+            _ffi.ptr(lens, C.c_uint32), n, C.byref(sch), _ffi.ptr(out, C.c_int32)))
+        return out
+
+
+_default = None
+
+
+def default_context():
+    global _default
+    if _default is None:
+        _default = Context(0)
+    return _default
+
+
+def score_banded_many(query, bias, jobs, scheme, ctx=None):
+    """Exact banded SW score of `query` against every BandJob, caller order
+    (src/algorithm/band_simd8.rs:78-185). bias must have the query's length."""
+    ctx = ctx or default_context()
+    q = _codes(query)
+    if bias is not None and len(bias) != len(q):
+        raise ValueError("bias length must equal query length")
+    if len(jobs) == 0 or len(q) == 0:
+        return [0] * len(jobs)
+    subs = [_codes(j.subject) for j in jobs]
+    soff = np.zeros(len(subs) + 1, np.uint64)
+    np.cumsum([len(s) for s in subs], out=soff[1:])
+    sres = np.concatenate(subs) if soff[-1] else np.zeros(1, np.uint8)
+    n = len(jobs)
+    jj = Context._jobs(np.zeros(n, np.uint32), np.arange(n, dtype=np.uint32), np.zeros(n, np.uint32),
+                       np.array([len(s) for s in subs], np.uint32),
+                       np.array([j.band.lo for j in jobs], np.int64).clip(-2**31, 2**31 - 1),
+                       np.array([j.band.hi for j in jobs], np.int64).clip(-2**31, 2**31 - 1))
+    score, _, _, _ = ctx.score_bands(q, np.array([0, len(q)], np.uint64), sres, soff, jj, scheme,
+                                     None if bias is None else np.asarray(bias, np.int8))
+    return [int(v) for v in score]
+
+
+def banded_score(q, s, bias, scheme, band, ctx=None):
+    """BandScore of one banded job (src/algorithm/banded.rs:408-449)."""
+    ctx = ctx or default_context()
+    q = _codes(q)
+    s = _codes(s)
+    if bias is not None and len(bias) != len(q):
+        raise ValueError("bias length must equal query length")
+    if len(q) == 0 or len(s) == 0:
+        return BandScore(0, 0, 0, 0)
+    jj = Context._jobs([0], [0], [0], [len(s)], [max(band.lo, -2**31)], [min(band.hi, 2**31 - 1)])
+    sc, qe, se, ce = ctx.score_bands(q, np.array([0, len(q)], np.uint64), s,
+                                     np.array([0, len(s)], np.uint64), jj, scheme,
+                                     None if bias is None else np.asarray(bias, np.int8))
+    return BandScore(int(sc[0]), int(qe[0]), int(se[0]), int(ce[0]))
+
+
+def score_many(query, bias, subjects, matrix, gaps, ctx=None):
+    """Exact full SW scores of `query` against every subject
+    (src/algorithm/sw_simd.rs:95-153)."""
+    ctx = ctx or default_context()
+    q = _codes(query)
+    if bias is not None and len(bias) != len(q):
+        raise ValueError("bias length must equal query length")
+    return [int(v) for v in ctx.score_windows(q, bias, subjects, Scheme(matrix, gaps.open, gaps.extend))]
+
+
+class HipBackend:
+    """ScoreBackend on the MI355X (src/gpu/mod.rs:45-59): exact i32 scores,
+    identical to score_many; errors raise (the engine then falls back)."""
+
+    def __init__(self, device=0):
+        self.ctx = Context(device)
+
+    def name(self):
+        return "hip-gfx950"
+
+    def score_windows(self, query, bias, windows, matrix, gaps):
+        if bias is not None and len(bias) != len(query):
+            raise ValueError("bias length mismatch")
+        if len(windows) == 0:
+            return []
+        return [int(v) for v in self.ctx.score_windows(query, bias, windows,
+                                                        Scheme(matrix, gaps.open, gaps.extend))]

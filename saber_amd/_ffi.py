@@ -1,0 +1,137 @@
+"""ctypes binding of the C ABI in include/saber_hip.h (libsaber_hip.so).
+
+The library is loaded from this package directory. There is no fallback: if
+the library is missing or exports an unexpected ABI, loading raises.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsaber_hip.so")
+WLIB_PATH = os.path.join(HERE, "libsaber_workload.so")
+
+OK, EINVAL, EDEVICE, ENOMEM, ESTATE = 0, 1, 2, 3, 4
+FORCE_EXACT32 = 1
+
+
+class Opts(C.Structure):
+    _fields_ = [("flags", C.c_uint32), ("reserved", C.c_uint32 * 7)]
+
+
+class Scheme(C.Structure):
+    _fields_ = [("matrix", C.c_int32 * 576), ("gap_open", C.c_int32), ("gap_extend", C.c_int32)]
+
+
+class SeqSet(C.Structure):
+    _fields_ = [
+        ("residues", C.POINTER(C.c_uint8)),
+        ("offsets", C.POINTER(C.c_uint64)),
+        ("count", C.c_uint32),
+        ("bias", C.POINTER(C.c_int8)),
+    ]
+
+
+class BandJob(C.Structure):
+    _fields_ = [
+        ("query", C.c_uint32),
+        ("subject", C.c_uint32),
+        ("win_lo", C.c_uint32),
+        ("win_hi", C.c_uint32),
+        ("band_lo", C.c_int32),
+        ("band_hi", C.c_int32),
+    ]
+
+
+P = C.c_void_p
+_SIGS = {
+    "saber_hip_version": (C.c_char_p, []),
+    "saber_hip_last_error": (C.c_char_p, []),
+    "saber_hip_device_count": (C.c_int, [C.POINTER(C.c_int)]),
+    "saber_hip_create": (C.c_int, [C.c_int, C.POINTER(Opts), C.POINTER(P)]),
+    "saber_hip_destroy": (None, [P]),
+    "saber_hip_upload_db": (C.c_int, [P, C.POINTER(SeqSet)]),
+    "saber_hip_score_windows": (C.c_int, [P, C.POINTER(C.c_uint8), C.c_uint32, C.POINTER(C.c_int8),
+                                          C.POINTER(C.POINTER(C.c_uint8)), C.POINTER(C.c_uint32),
+                                          C.c_uint32, C.POINTER(Scheme), C.POINTER(C.c_int32)]),
+    "saber_hip_score_bands": (C.c_int, [P, C.POINTER(SeqSet), C.POINTER(SeqSet), C.POINTER(BandJob),
+                                        C.c_uint64, C.POINTER(Scheme), C.POINTER(C.c_int32),
+                                        C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                        C.POINTER(C.c_uint64)]),
+    "saber_hip_band_batch_create": (C.c_int, [P, C.POINTER(SeqSet), C.POINTER(SeqSet),
+                                              C.POINTER(BandJob), C.c_uint64, C.POINTER(Scheme),
+                                              C.POINTER(P)]),
+    "saber_hip_band_batch_run": (C.c_int, [P]),
+    "saber_hip_band_batch_wait": (C.c_int, [P]),
+    "saber_hip_band_batch_read": (C.c_int, [P, C.POINTER(C.c_int32), C.POINTER(C.c_uint32),
+                                            C.POINTER(C.c_uint32), C.POINTER(C.c_uint64)]),
+    "saber_hip_band_batch_destroy": (None, [P]),
+    "saber_hip_band_batch_time": (C.c_int, [P, C.c_int, C.c_int, C.POINTER(C.c_double),
+                                            C.POINTER(C.c_double), C.POINTER(C.c_uint64),
+                                            C.POINTER(C.c_int), C.POINTER(C.c_uint64)]),
+    "saber_hip_band_variant": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                         C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "saber_hip_band_choose": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load libsaber_hip.so (raises OSError if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OSError(f"{LIB_PATH} is missing: run saber_amd/build.py (or __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+class HipError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"saber_hip error {code}: {msg}")
+        self.code = code
+
+
+def check(rc):
+    if rc != OK:
+        raise HipError(rc, lib().saber_hip_last_error().decode(errors="replace"))
+
+
+def ptr(a, ctype):
+    """ctypes pointer to a contiguous numpy array (None -> NULL)."""
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+_wlib = None
+
+
+def wlib():
+    global _wlib
+    if _wlib is None:
+        if not os.path.exists(WLIB_PATH):
+            raise OSError(f"{WLIB_PATH} is missing: run saber_amd/build.py")
+        W = C.CDLL(WLIB_PATH)
+        W.wl_band_regions.restype = P
+        W.wl_band_regions.argtypes = [C.c_uint64, C.c_uint64, C.c_double, C.c_double, C.c_uint32,
+                                      C.c_uint32, C.c_double, C.POINTER(C.c_int32), C.c_int,
+                                      C.POINTER(C.c_int32)]
+        W.wl_sequences.restype = P
+        W.wl_sequences.argtypes = [C.c_uint64, C.c_uint64, C.c_double, C.c_double, C.c_uint32,
+                                   C.c_uint32, C.POINTER(C.c_int32)]
+        W.wl_sizes.restype = None
+        W.wl_sizes.argtypes = [P] + [C.POINTER(C.c_uint64)] * 4
+        W.wl_copy.restype = None
+        W.wl_copy.argtypes = [P, C.POINTER(C.c_uint8), C.POINTER(C.c_uint64), C.POINTER(C.c_uint8),
+                              C.POINTER(C.c_uint64), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                              C.POINTER(C.c_uint8)]
+        W.wl_free.restype = None
+        W.wl_free.argtypes = [P]
+        _wlib = W
+    return _wlib

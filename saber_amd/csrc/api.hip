@@ -1,0 +1,632 @@
+// C ABI (include/saber_hip.h): context, device-resident database, the band
+// planner and the batch runner.
+//
+// Planner: every job's band is clipped to its rectangle (Band::clip,
+// src/algorithm/banded.rs:45-47); empty jobs score 0. The remaining jobs are
+// bucketed by clipped width into the packed kernel's variants (or the exact
+// int32 kernel), sorted inside a bucket by row count (longest first, so the
+// longest pairs start first and the two jobs of a lane have nearly the same
+// length) and paired two per lane. The kernels write results by job id, so
+// the order of work never changes an output.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <memory>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/saber_hip.h"
+#include "common.h"
+
+using namespace saber;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (expr);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return fail(SABER_HIP_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+
+struct Shape {
+    int L, C, K;
+    int W() const { return L * C; }
+};
+
+__global__ void fill_u32_kernel(uint32_t* p, uint64_t n, uint32_t v) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+hipError_t fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    const uint32_t block = 256;
+    const uint64_t grid = (n + block - 1) / block;
+    hipLaunchKernelGGL(fill_u32_kernel, dim3((uint32_t)grid), dim3(block), 0, s, p, n, v);
+    return hipGetLastError();
+}
+
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    uint64_t n = 0;
+    hipError_t alloc(uint64_t count) {
+        release();
+        n = count;
+        if (count == 0) return hipSuccess;
+        return hipMalloc(reinterpret_cast<void**>(&p), sizeof(T) * count);
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    ~DevBuf() { release(); }
+};
+
+}  // namespace
+
+struct saber_hip_ctx {
+    int device = 0;
+    uint32_t flags = 0;
+    hipStream_t stream = nullptr;
+    std::mutex mu;
+    DevBuf<uint8_t> db;
+    std::vector<uint64_t> db_off;
+    std::vector<int> choice;  // width -> shape index (-1: exact int32 kernel)
+    std::vector<Shape> shapes;
+};
+
+struct VarLaunch {
+    int variant;
+    uint64_t pair_off;
+    uint32_t npairs;
+};
+
+struct saber_hip_band_batch {
+    saber_hip_ctx* ctx = nullptr;
+    uint64_t njobs = 0;
+    bool has_bias = false;
+    int32_t go = 0, ge = 0;
+    DevBuf<uint8_t> q, s;
+    DevBuf<int8_t> bias;
+    DevBuf<short> table;
+    DevBuf<int32_t> mat;
+    DevBuf<BandJobDev> jobs;
+    DevBuf<uint32_t> pairs;
+    std::vector<VarLaunch> launches;
+    // exact int32 kernel: wide / unsuitable jobs known at plan time, plus the
+    // packed kernel's saturated jobs known after it ran
+    std::vector<uint32_t> generic_static;
+    std::vector<BandJobDev> host_jobs;
+    DevBuf<uint32_t> gen_ids;
+    DevBuf<uint64_t> gen_off;
+    DevBuf<int32_t> gen_work;
+    DevBuf<uint32_t> flagged;
+    DevBuf<uint32_t> nflagged;
+    DevBuf<int32_t> score;
+    DevBuf<uint32_t> q_end, s_end;
+    DevBuf<uint64_t> cells;
+    const uint8_t* sres = nullptr;  // own copy or the context database
+    bool ran = false;
+    uint32_t last_flagged = 0;
+};
+
+namespace {
+
+void default_choice(saber_hip_ctx* c) {
+    int nv = 0;
+    const BandVariant* v = band_variants(&nv);
+    c->shapes.clear();
+    for (int i = 0; i < nv; ++i) {
+        if (v[i].bias) continue;
+        c->shapes.push_back(Shape{v[i].L, v[i].C, v[i].K});
+    }
+    int wmax = 0;
+    for (auto& s : c->shapes) wmax = std::max(wmax, s.W());
+    c->choice.assign((size_t)wmax + 1, -1);
+    for (int w = 1; w <= wmax; ++w) {
+        int best = -1;
+        for (int k = 0; k < (int)c->shapes.size(); ++k) {
+            const Shape& s = c->shapes[k];
+            if (s.W() < w || s.W() - s.K >= w) continue;
+            if (best < 0 || s.W() < c->shapes[best].W()) best = k;
+        }
+        c->choice[w] = best;
+    }
+}
+
+int find_variant(int L, int C, int K, int bias) {
+    int nv = 0;
+    const BandVariant* v = band_variants(&nv);
+    for (int i = 0; i < nv; ++i)
+        if (v[i].L == L && v[i].C == C && v[i].K == K && v[i].bias == bias) return i;
+    return -1;
+}
+
+int check_seqset(const saber_hip_seqset* s, const char* what) {
+    if (!s) return fail(SABER_HIP_EINVAL, std::string(what) + " is null");
+    if (s->count > 0 && !s->offsets) return fail(SABER_HIP_EINVAL, std::string(what) + ": offsets null");
+    if (s->count > 0 && s->offsets[s->count] > 0 && !s->residues)
+        return fail(SABER_HIP_EINVAL, std::string(what) + ": residues null");
+    for (uint32_t i = 0; i < s->count; ++i)
+        if (s->offsets[i + 1] < s->offsets[i])
+            return fail(SABER_HIP_EINVAL, std::string(what) + ": offsets not monotone");
+    return SABER_HIP_OK;
+}
+
+template <class T>
+int upload(DevBuf<T>& d, const T* h, uint64_t n, hipStream_t s) {
+    HIP_TRY(d.alloc(n == 0 ? 1 : n));
+    if (n) HIP_TRY(hipMemcpyAsync(d.p, h, sizeof(T) * n, hipMemcpyHostToDevice, s));
+    return SABER_HIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* saber_hip_version(void) { return "saber-mi355x 0.1 (gfx950)"; }
+
+const char* saber_hip_last_error(void) { return g_err.c_str(); }
+
+int saber_hip_device_count(int* count) {
+    if (!count) return fail(SABER_HIP_EINVAL, "count is null");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) n = 0;
+    *count = n;
+    return SABER_HIP_OK;
+}
+
+int saber_hip_create(int device, const saber_hip_opts* opts, saber_hip_ctx** out) {
+    if (!out) return fail(SABER_HIP_EINVAL, "out is null");
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0)
+        return fail(SABER_HIP_EDEVICE, "no HIP device available");
+    if (device < 0 || device >= n) return fail(SABER_HIP_EINVAL, "device ordinal out of range");
+    HIP_TRY(hipSetDevice(device));
+    auto* c = new saber_hip_ctx();
+    c->device = device;
+    c->flags = opts ? opts->flags : 0;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(SABER_HIP_EDEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    default_choice(c);
+    *out = c;
+    return SABER_HIP_OK;
+}
+
+void saber_hip_destroy(saber_hip_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    c->db.release();
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int saber_hip_upload_db(saber_hip_ctx* c, const saber_hip_seqset* subjects) {
+    if (!c) return fail(SABER_HIP_EINVAL, "ctx is null");
+    if (int r = check_seqset(subjects, "subjects")) return r;
+    std::lock_guard<std::mutex> g(c->mu);
+    HIP_TRY(hipSetDevice(c->device));
+    const uint64_t nres = subjects->count ? subjects->offsets[subjects->count] : 0;
+    if (int r = upload(c->db, subjects->residues, nres, c->stream)) return r;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->db_off.assign(subjects->offsets, subjects->offsets + subjects->count + 1);
+    return SABER_HIP_OK;
+}
+
+int saber_hip_band_variant(int v, int* L, int* C, int* K, int* bias) {
+    int nv = 0;
+    const BandVariant* t = band_variants(&nv);
+    if (v < 0 || v >= nv) return fail(SABER_HIP_EINVAL, "variant index out of range");
+    if (L) *L = t[v].L;
+    if (C) *C = t[v].C;
+    if (K) *K = t[v].K;
+    if (bias) *bias = t[v].bias;
+    return SABER_HIP_OK;
+}
+
+int saber_hip_band_choose(saber_hip_ctx* c, int w_lo, int w_hi, int L, int C, int K) {
+    if (!c) return fail(SABER_HIP_EINVAL, "ctx is null");
+    std::lock_guard<std::mutex> g(c->mu);
+    if (w_lo < 1 || w_hi < w_lo) return fail(SABER_HIP_EINVAL, "bad width range");
+    int shape = -1;
+    if (L != 0) {
+        for (int k = 0; k < (int)c->shapes.size(); ++k)
+            if (c->shapes[k].L == L && c->shapes[k].C == C && c->shapes[k].K == K) shape = k;
+        if (shape < 0) return fail(SABER_HIP_EINVAL, "no compiled variant with that shape");
+        const Shape& s = c->shapes[shape];
+        if (w_hi > s.W() || w_lo <= s.W() - s.K)
+            return fail(SABER_HIP_EINVAL, "widths outside the variant's coverage");
+    }
+    if ((size_t)w_hi >= c->choice.size()) {
+        if (L != 0) return fail(SABER_HIP_EINVAL, "width above the widest variant");
+        w_hi = (int)c->choice.size() - 1;
+    }
+    for (int w = w_lo; w <= w_hi; ++w) c->choice[w] = shape;
+    return SABER_HIP_OK;
+}
+
+int saber_hip_band_batch_create(saber_hip_ctx* c, const saber_hip_seqset* queries,
+                                const saber_hip_seqset* subjects, const saber_hip_band_job* jobs,
+                                uint64_t njobs, const saber_hip_scheme* scheme,
+                                saber_hip_band_batch** out) {
+    if (!c || !out || !scheme) return fail(SABER_HIP_EINVAL, "null argument");
+    *out = nullptr;
+    if (njobs > 0 && !jobs) return fail(SABER_HIP_EINVAL, "jobs is null");
+    if (njobs >= (uint64_t)kNone) return fail(SABER_HIP_EINVAL, "too many jobs for one batch");
+    if (int r = check_seqset(queries, "queries")) return r;
+    if (subjects)
+        if (int r = check_seqset(subjects, "subjects")) return r;
+    std::lock_guard<std::mutex> g(c->mu);
+    HIP_TRY(hipSetDevice(c->device));
+    const uint64_t* soffs = subjects ? subjects->offsets : c->db_off.data();
+    const uint32_t nsub = subjects ? subjects->count : (c->db_off.empty() ? 0 : (uint32_t)c->db_off.size() - 1);
+
+    auto b = std::make_unique<saber_hip_band_batch>();
+    b->ctx = c;
+    b->njobs = njobs;
+    b->has_bias = queries->bias != nullptr;
+    b->go = scheme->gap_open;
+    b->ge = scheme->gap_extend;
+
+    // Can the packed int16 kernel be exact for this scheme? (Gap costs within
+    // the range GapCosts::from_user accepts, substitution scores within int8.)
+    bool packed_ok = !(c->flags & SABER_HIP_FORCE_EXACT32);
+    packed_ok = packed_ok && scheme->gap_open >= 0 && scheme->gap_open <= 1000 &&
+                scheme->gap_extend >= 0 && scheme->gap_extend <= 1000;
+    for (int i = 0; i < 576; ++i)
+        if (scheme->matrix[i] < -128 || scheme->matrix[i] > 127) packed_ok = false;
+
+    // Plan.
+    b->host_jobs.assign(njobs, BandJobDev{});
+    std::vector<std::vector<uint32_t>> bucket(c->shapes.size());
+    std::vector<int32_t> rows_of(njobs, 0);
+    for (uint64_t k = 0; k < njobs; ++k) {
+        const saber_hip_band_job& J = jobs[k];
+        if (J.query >= queries->count) return fail(SABER_HIP_EINVAL, "job query index out of range");
+        if (J.subject >= nsub) return fail(SABER_HIP_EINVAL, "job subject index out of range");
+        const uint64_t slen = soffs[J.subject + 1] - soffs[J.subject];
+        if (J.win_hi < J.win_lo || J.win_hi > slen)
+            return fail(SABER_HIP_EINVAL, "job window outside its subject");
+        const int64_t m = (int64_t)(queries->offsets[J.query + 1] - queries->offsets[J.query]);
+        const int64_t n = (int64_t)J.win_hi - J.win_lo;
+        BandJobDev d{};
+        d.q_off = queries->offsets[J.query];
+        d.s_off = soffs[J.subject] + J.win_lo;
+        d.m = (uint32_t)m;
+        d.n = (uint32_t)n;
+        b->host_jobs[k] = d;
+        if (m == 0 || n == 0 || J.band_hi < J.band_lo) continue;
+        const int64_t lo = std::max<int64_t>(J.band_lo, -(m - 1));
+        const int64_t hi = std::min<int64_t>(J.band_hi, n - 1);
+        const int64_t w = hi - lo + 1;
+        if (w <= 0) continue;
+        const int64_t i_first = std::max<int64_t>(0, -hi);
+        const int64_t i_last = std::min<int64_t>(m - 1, n - 1 - lo);
+        if (i_last < i_first) continue;
+        d.lo = (int32_t)lo;
+        d.w = (int32_t)w;
+        d.i_first = (int32_t)i_first;
+        d.rows = (int32_t)(i_last - i_first + 1);
+        b->host_jobs[k] = d;
+        rows_of[k] = d.rows;
+        const int shape = (packed_ok && w < (int64_t)c->choice.size()) ? c->choice[w] : -1;
+        if (shape < 0)
+            b->generic_static.push_back((uint32_t)k);
+        else
+            bucket[shape].push_back((uint32_t)k);
+    }
+    std::vector<uint32_t> pairs;
+    for (size_t sh = 0; sh < bucket.size(); ++sh) {
+        auto& v = bucket[sh];
+        if (v.empty()) continue;
+        std::stable_sort(v.begin(), v.end(), [&](uint32_t x, uint32_t y) { return rows_of[x] > rows_of[y]; });
+        const Shape& s = c->shapes[sh];
+        const int var = find_variant(s.L, s.C, s.K, b->has_bias ? 1 : 0);
+        if (var < 0) return fail(SABER_HIP_EINVAL, "variant table inconsistent");
+        VarLaunch L{var, pairs.size() / 2, (uint32_t)((v.size() + 1) / 2)};
+        for (size_t i = 0; i < v.size(); i += 2) {
+            pairs.push_back(v[i]);
+            pairs.push_back(i + 1 < v.size() ? v[i + 1] : kNone);
+        }
+        b->launches.push_back(L);
+    }
+
+    // Score table for the packed kernel: T[a][c] = S(code_index(a), code_index(c)),
+    // column 31 = sentinel.
+    std::vector<short> table(kTableDim * kTableDim);
+    for (int a = 0; a < kTableDim; ++a)
+        for (int cc = 0; cc < kTableDim; ++cc) {
+            const int aa = a < 24 ? a : kCodeX, ci = cc < 24 ? cc : kCodeX;
+            int v = scheme->matrix[aa * 24 + ci];
+            v = std::max(-32767, std::min(32767, v));
+            table[a * kTableDim + cc] = cc == kSentinel ? kNeg16 : (short)v;
+        }
+
+    hipStream_t st = c->stream;
+    const uint64_t nq = queries->count ? queries->offsets[queries->count] : 0;
+    if (int r = upload(b->q, queries->residues, nq, st)) return r;
+    if (b->has_bias) {
+        if (int r = upload(b->bias, queries->bias, nq, st)) return r;
+    }
+    if (subjects) {
+        const uint64_t ns = subjects->count ? subjects->offsets[subjects->count] : 0;
+        if (int r = upload(b->s, subjects->residues, ns, st)) return r;
+        b->sres = b->s.p;
+    } else {
+        if (!c->db.p && njobs > 0) return fail(SABER_HIP_ESTATE, "no database uploaded");
+        b->sres = c->db.p;
+    }
+    if (int r = upload(b->table, table.data(), table.size(), st)) return r;
+    if (int r = upload(b->mat, scheme->matrix, 576, st)) return r;
+    if (int r = upload(b->jobs, b->host_jobs.data(), njobs, st)) return r;
+    if (int r = upload(b->pairs, pairs.data(), pairs.size(), st)) return r;
+    HIP_TRY(b->flagged.alloc(njobs ? njobs : 1));
+    HIP_TRY(b->nflagged.alloc(1));
+    HIP_TRY(b->score.alloc(njobs ? njobs : 1));
+    HIP_TRY(b->q_end.alloc(njobs ? njobs : 1));
+    HIP_TRY(b->s_end.alloc(njobs ? njobs : 1));
+    HIP_TRY(b->cells.alloc(njobs ? njobs : 1));
+    HIP_TRY(hipStreamSynchronize(st));
+    *out = b.release();
+    return SABER_HIP_OK;
+}
+
+static int run_generic(saber_hip_band_batch* b, const std::vector<uint32_t>& ids) {
+    if (ids.empty()) return SABER_HIP_OK;
+    hipStream_t st = b->ctx->stream;
+    std::vector<uint64_t> off(ids.size());
+    uint64_t tot = 0;
+    for (size_t i = 0; i < ids.size(); ++i) {
+        off[i] = tot;
+        tot += 4 * ((uint64_t)b->host_jobs[ids[i]].w + 1);
+    }
+    if (int r = upload(b->gen_ids, ids.data(), ids.size(), st)) return r;
+    if (int r = upload(b->gen_off, off.data(), off.size(), st)) return r;
+    if (b->gen_work.n < tot) HIP_TRY(b->gen_work.alloc(tot));
+    GenericArgs a{};
+    a.ids = b->gen_ids.p;
+    a.nids = (uint32_t)ids.size();
+    a.jobs = b->jobs.p;
+    a.qres = b->q.p;
+    a.qbias = b->has_bias ? b->bias.p : nullptr;
+    a.sres = b->sres;
+    a.mat = b->mat.p;
+    a.go = b->go;
+    a.ge = b->ge;
+    a.out = BandOutputs{b->score.p, b->q_end.p, b->s_end.p, b->cells.p};
+    a.work = b->gen_work.p;
+    a.work_off = b->gen_off.p;
+    HIP_TRY(launch_generic(a, st));
+    // The id/offset uploads read host vectors owned by this frame.
+    HIP_TRY(hipStreamSynchronize(st));
+    return SABER_HIP_OK;
+}
+
+// Enqueue everything up to (not including) the rescore of saturated jobs.
+static int enqueue_main(saber_hip_band_batch* b, hipEvent_t* ev_variant_begin, hipEvent_t* ev_variant_end) {
+    hipStream_t st = b->ctx->stream;
+    const uint64_t n = b->njobs;
+    HIP_TRY(fill_u32(reinterpret_cast<uint32_t*>(b->score.p), n, 0, st));
+    HIP_TRY(fill_u32(b->q_end.p, n, 0, st));
+    HIP_TRY(fill_u32(b->s_end.p, n, 0, st));
+    HIP_TRY(fill_u32(reinterpret_cast<uint32_t*>(b->cells.p), 2 * n, 0, st));
+    HIP_TRY(fill_u32(b->nflagged.p, 1, 0, st));
+    int nv = 0;
+    const BandVariant* var = band_variants(&nv);
+    for (size_t i = 0; i < b->launches.size(); ++i) {
+        const VarLaunch& L = b->launches[i];
+        BandKernelArgs a{};
+        a.pairs = b->pairs.p + 2 * L.pair_off;
+        a.npairs = L.npairs;
+        a.jobs = b->jobs.p;
+        a.qres = b->q.p;
+        a.qbias = b->has_bias ? b->bias.p : nullptr;
+        a.sres = b->sres;
+        a.table = b->table.p;
+        a.go = b->go;
+        a.ge = b->ge;
+        a.out = BandOutputs{b->score.p, b->q_end.p, b->s_end.p, b->cells.p};
+        a.flagged = b->flagged.p;
+        a.nflagged = b->nflagged.p;
+        if (ev_variant_begin) HIP_TRY(hipEventRecord(ev_variant_begin[i], st));
+        HIP_TRY(var[L.variant].launch(a, st));
+        if (ev_variant_end) HIP_TRY(hipEventRecord(ev_variant_end[i], st));
+    }
+    if (int r = run_generic(b, b->generic_static)) return r;
+    return SABER_HIP_OK;
+}
+
+// Rescore the packed kernel's saturated jobs exactly (needs their count).
+static int finish_flagged(saber_hip_band_batch* b) {
+    hipStream_t st = b->ctx->stream;
+    uint32_t nf = 0;
+    HIP_TRY(hipMemcpyAsync(&nf, b->nflagged.p, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    b->last_flagged = nf;
+    if (nf == 0) return SABER_HIP_OK;
+    std::vector<uint32_t> ids(nf);
+    HIP_TRY(hipMemcpyAsync(ids.data(), b->flagged.p, sizeof(uint32_t) * nf, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    std::sort(ids.begin(), ids.end());
+    if (int r = run_generic(b, ids)) return r;
+    return SABER_HIP_OK;
+}
+
+int saber_hip_band_batch_run(saber_hip_band_batch* b) {
+    if (!b) return fail(SABER_HIP_EINVAL, "batch is null");
+    std::lock_guard<std::mutex> g(b->ctx->mu);
+    HIP_TRY(hipSetDevice(b->ctx->device));
+    if (int r = enqueue_main(b, nullptr, nullptr)) return r;
+    if (int r = finish_flagged(b)) return r;
+    b->ran = true;
+    return SABER_HIP_OK;
+}
+
+int saber_hip_band_batch_wait(saber_hip_band_batch* b) {
+    if (!b) return fail(SABER_HIP_EINVAL, "batch is null");
+    HIP_TRY(hipSetDevice(b->ctx->device));
+    HIP_TRY(hipStreamSynchronize(b->ctx->stream));
+    return SABER_HIP_OK;
+}
+
+int saber_hip_band_batch_read(saber_hip_band_batch* b, int32_t* score, uint32_t* q_end,
+                              uint32_t* s_end, uint64_t* cells) {
+    if (!b) return fail(SABER_HIP_EINVAL, "batch is null");
+    if (!b->ran) return fail(SABER_HIP_ESTATE, "batch has not run");
+    std::lock_guard<std::mutex> g(b->ctx->mu);
+    HIP_TRY(hipSetDevice(b->ctx->device));
+    hipStream_t st = b->ctx->stream;
+    const uint64_t n = b->njobs;
+    if (n == 0) return SABER_HIP_OK;
+    if (score) HIP_TRY(hipMemcpyAsync(score, b->score.p, 4 * n, hipMemcpyDeviceToHost, st));
+    if (q_end) HIP_TRY(hipMemcpyAsync(q_end, b->q_end.p, 4 * n, hipMemcpyDeviceToHost, st));
+    if (s_end) HIP_TRY(hipMemcpyAsync(s_end, b->s_end.p, 4 * n, hipMemcpyDeviceToHost, st));
+    if (cells) HIP_TRY(hipMemcpyAsync(cells, b->cells.p, 8 * n, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return SABER_HIP_OK;
+}
+
+void saber_hip_band_batch_destroy(saber_hip_band_batch* b) {
+    if (!b) return;
+    (void)hipSetDevice(b->ctx->device);
+    (void)hipStreamSynchronize(b->ctx->stream);
+    delete b;
+}
+
+int saber_hip_band_batch_time(saber_hip_band_batch* b, int warmup, int iters, double* ms_run,
+                              double* kernel_ms, uint64_t* kernel_cells, int* nvar,
+                              uint64_t* cells) {
+    if (!b || iters < 1 || warmup < 0) return fail(SABER_HIP_EINVAL, "bad arguments");
+    int nv = 0;
+    band_variants(&nv);
+    const int cap = nvar ? std::min(*nvar, nv) : 0;
+    for (int i = 0; i < warmup; ++i)
+        if (int r = saber_hip_band_batch_run(b)) return r;
+    std::lock_guard<std::mutex> g(b->ctx->mu);
+    HIP_TRY(hipSetDevice(b->ctx->device));
+    hipStream_t st = b->ctx->stream;
+    const size_t nl = b->launches.size();
+    std::vector<hipEvent_t> eb(nl * iters), ee(nl * iters);
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    for (auto& e : eb) HIP_TRY(hipEventCreate(&e));
+    for (auto& e : ee) HIP_TRY(hipEventCreate(&e));
+    HIP_TRY(hipStreamSynchronize(st));
+    HIP_TRY(hipEventRecord(e0, st));
+    for (int it = 0; it < iters; ++it) {
+        if (int r = enqueue_main(b, eb.data() + it * nl, ee.data() + it * nl)) return r;
+        if (int r = finish_flagged(b)) return r;
+    }
+    HIP_TRY(hipEventRecord(e1, st));
+    HIP_TRY(hipEventSynchronize(e1));
+    b->ran = true;
+    float tot = 0.f;
+    HIP_TRY(hipEventElapsedTime(&tot, e0, e1));
+    if (ms_run) *ms_run = tot / iters;
+    if (kernel_ms) {
+        for (int v = 0; v < cap; ++v) kernel_ms[v] = 0.0;
+        for (int it = 0; it < iters; ++it)
+            for (size_t i = 0; i < nl; ++i) {
+                float ms = 0.f;
+                HIP_TRY(hipEventElapsedTime(&ms, eb[it * nl + i], ee[it * nl + i]));
+                const int v = b->launches[i].variant;
+                if (v < cap) kernel_ms[v] += ms / iters;
+            }
+    }
+    if (nvar) *nvar = nv;
+    for (auto& e : eb) (void)hipEventDestroy(e);
+    for (auto& e : ee) (void)hipEventDestroy(e);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (cells || kernel_cells) {
+        std::vector<uint64_t> c(b->njobs);
+        if (b->njobs) {
+            HIP_TRY(hipMemcpy(c.data(), b->cells.p, 8 * b->njobs, hipMemcpyDeviceToHost));
+        }
+        uint64_t t = 0;
+        for (uint64_t x : c) t += x;
+        if (cells) *cells = t;
+        if (kernel_cells) {
+            std::vector<uint32_t> pairs(2 * (b->pairs.n ? b->pairs.n / 2 : 0));
+            if (!pairs.empty())
+                HIP_TRY(hipMemcpy(pairs.data(), b->pairs.p, 4 * pairs.size(), hipMemcpyDeviceToHost));
+            for (int v = 0; v < cap; ++v) kernel_cells[v] = 0;
+            for (const VarLaunch& L : b->launches) {
+                if (L.variant >= cap) continue;
+                for (uint64_t k = 2 * L.pair_off; k < 2 * (L.pair_off + L.npairs); ++k)
+                    if (pairs[k] != kNone) kernel_cells[L.variant] += c[pairs[k]];
+            }
+        }
+    }
+    return SABER_HIP_OK;
+}
+
+int saber_hip_score_bands(saber_hip_ctx* c, const saber_hip_seqset* queries,
+                          const saber_hip_seqset* subjects, const saber_hip_band_job* jobs,
+                          uint64_t njobs, const saber_hip_scheme* scheme, int32_t* score,
+                          uint32_t* q_end, uint32_t* s_end, uint64_t* cells) {
+    if (njobs > 0 && !score) return fail(SABER_HIP_EINVAL, "score is null");
+    saber_hip_band_batch* b = nullptr;
+    if (int r = saber_hip_band_batch_create(c, queries, subjects, jobs, njobs, scheme, &b)) return r;
+    int r = saber_hip_band_batch_run(b);
+    if (r == SABER_HIP_OK) r = saber_hip_band_batch_read(b, score, q_end, s_end, cells);
+    saber_hip_band_batch_destroy(b);
+    return r;
+}
+
+int saber_hip_score_windows(saber_hip_ctx* c, const uint8_t* query, uint32_t qlen,
+                            const int8_t* bias, const uint8_t* const* windows,
+                            const uint32_t* lens, uint32_t n, const saber_hip_scheme* scheme,
+                            int32_t* out) {
+    if (!c || !scheme) return fail(SABER_HIP_EINVAL, "null argument");
+    if (n == 0) return SABER_HIP_OK;
+    if (!out || !windows || !lens) return fail(SABER_HIP_EINVAL, "null argument");
+    if (qlen == 0) {
+        std::fill(out, out + n, 0);
+        return SABER_HIP_OK;
+    }
+    if (!query) return fail(SABER_HIP_EINVAL, "query is null");
+    // Concatenate the windows into one subject set; every job is the full
+    // band of its rectangle (Band::full, src/algorithm/banded.rs:38-40).
+    std::vector<uint64_t> off(n + 1, 0);
+    for (uint32_t k = 0; k < n; ++k) {
+        if (lens[k] && !windows[k]) return fail(SABER_HIP_EINVAL, "window pointer is null");
+        off[k + 1] = off[k] + lens[k];
+    }
+    std::vector<uint8_t> res(off[n] ? off[n] : 1);
+    for (uint32_t k = 0; k < n; ++k)
+        if (lens[k]) std::memcpy(res.data() + off[k], windows[k], lens[k]);
+    uint64_t qoff[2] = {0, qlen};
+    saber_hip_seqset qs{query, qoff, 1, bias};
+    saber_hip_seqset ss{res.data(), off.data(), n, nullptr};
+    std::vector<saber_hip_band_job> jobs(n);
+    for (uint32_t k = 0; k < n; ++k)
+        jobs[k] = saber_hip_band_job{0, k, 0, lens[k], -(int32_t)qlen, (int32_t)lens[k]};
+    return saber_hip_score_bands(c, &qs, &ss, jobs.data(), n, scheme, out, nullptr, nullptr, nullptr);
+}
+
+}  // extern "C"

@@ -36,6 +36,7 @@
 //    cells (to private scratch) whenever that best strictly improves; at the
 //    end the group takes the global best, the earliest row that reached it and
 //    the leftmost cell of that row.
+#pragma once
 #include "common.h"
 
 namespace saber {
@@ -339,25 +340,13 @@ __global__ __launch_bounds__(256) void band_pair_kernel(BandKernelArgs a) {
 }
 
 template <int L, int C, int K, bool BIAS>
-static hipError_t launch_band(const BandKernelArgs& a, hipStream_t s) {
+hipError_t launch_band(const BandKernelArgs& a, hipStream_t s) {
     if (a.npairs == 0) return hipSuccess;
     const uint64_t threads = (uint64_t)a.npairs * L;
     const uint32_t block = 256;
     const uint32_t grid = (uint32_t)((threads + block - 1) / block);
     hipLaunchKernelGGL((band_pair_kernel<L, C, K, BIAS>), dim3(grid), dim3(block), 0, s, a);
     return hipGetLastError();
-}
-
-#define SABER_VARIANT(L, C, K) \
-    {L, C, K, 0, &launch_band<L, C, K, false>}, {L, C, K, 1, &launch_band<L, C, K, true>},
-
-static const BandVariant kVariants[] = {
-#include "band_variants.inc"
-};
-
-const BandVariant* band_variants(int* count) {
-    *count = (int)(sizeof(kVariants) / sizeof(kVariants[0]));
-    return kVariants;
 }
 
 }  // namespace saber

@@ -1,0 +1,71 @@
+"""Synthetic workloads (host side, deterministic from a seed).
+
+band_regions: the batched banded-SW workload -- n candidate regions, each a
+(query window, subject window, centre diagonal) triple; window lengths
+lognormal (median 150, sigma 0.6, clamped 20..2000); band half-widths drawn
+uniformly from {8, 16, 32, 64}; residues i.i.d. from the Robinson background;
+a fraction of regions (0.4) are planted homologues (BLOSUM62-conditional
+substitutions at 25-60 % identity, geometric indels at 0.02/residue with
+mean length 3).
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _ffi
+from .matrices import BLOSUM62
+
+JOB_DTYPE = np.dtype([("query", "<u4"), ("subject", "<u4"), ("win_lo", "<u4"), ("win_hi", "<u4"),
+                      ("band_lo", "<i4"), ("band_hi", "<i4")])
+
+
+def _mat(matrix):
+    return np.ascontiguousarray(np.asarray(matrix, dtype=np.int32).reshape(576))
+
+
+def band_regions(seed, n, median=150.0, sigma=0.6, lmin=20, lmax=2000, homolog_frac=0.4,
+                 half_widths=(8, 16, 32, 64), matrix=BLOSUM62):
+    W = _ffi.wlib()
+    hw = np.asarray(half_widths, dtype=np.int32)
+    mat = _mat(matrix)
+    h = W.wl_band_regions(seed, n, median, sigma, lmin, lmax, homolog_frac,
+                          _ffi.ptr(hw, C.c_int32), len(hw), _ffi.ptr(mat, C.c_int32))
+    try:
+        a, b, c, d = C.c_uint64(), C.c_uint64(), C.c_uint64(), C.c_uint64()
+        W.wl_sizes(h, C.byref(a), C.byref(b), C.byref(c), C.byref(d))
+        qres = np.zeros(max(a.value, 1), np.uint8)
+        qoff = np.zeros(b.value + 1, np.uint64)
+        sres = np.zeros(max(c.value, 1), np.uint8)
+        soff = np.zeros(d.value + 1, np.uint64)
+        lo = np.zeros(n, np.int32)
+        hi = np.zeros(n, np.int32)
+        hom = np.zeros(n, np.uint8)
+        W.wl_copy(h, _ffi.ptr(qres, C.c_uint8), _ffi.ptr(qoff, C.c_uint64), _ffi.ptr(sres, C.c_uint8),
+                  _ffi.ptr(soff, C.c_uint64), _ffi.ptr(lo, C.c_int32), _ffi.ptr(hi, C.c_int32),
+                  _ffi.ptr(hom, C.c_uint8))
+    finally:
+        W.wl_free(h)
+    jobs = np.zeros(n, JOB_DTYPE)
+    jobs["query"] = np.arange(n, dtype=np.uint32)
+    jobs["subject"] = np.arange(n, dtype=np.uint32)
+    jobs["win_lo"] = 0
+    jobs["win_hi"] = (soff[1:] - soff[:-1]).astype(np.uint32)
+    jobs["band_lo"] = lo
+    jobs["band_hi"] = hi
+    return dict(qres=qres, qoff=qoff, sres=sres, soff=soff, jobs=jobs, homolog=hom.astype(bool))
+
+
+def sequences(seed, n, median, sigma, lmin, lmax, matrix=BLOSUM62):
+    """n sequences with lognormal lengths: (residues, offsets)."""
+    W = _ffi.wlib()
+    mat = _mat(matrix)
+    h = W.wl_sequences(seed, n, median, sigma, lmin, lmax, _ffi.ptr(mat, C.c_int32))
+    try:
+        a, b, c, d = C.c_uint64(), C.c_uint64(), C.c_uint64(), C.c_uint64()
+        W.wl_sizes(h, C.byref(a), C.byref(b), C.byref(c), C.byref(d))
+        res = np.zeros(max(a.value, 1), np.uint8)
+        off = np.zeros(b.value + 1, np.uint64)
+        W.wl_copy(h, _ffi.ptr(res, C.c_uint8), _ffi.ptr(off, C.c_uint64), None, None, None, None, None)
+    finally:
+        W.wl_free(h)
+    return res, off

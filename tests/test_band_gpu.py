@@ -1,0 +1,237 @@
+"""Banded / full SW on the GPU vs the oracle: bit-exact scores, end cells and
+cell counts. Cases follow the reference's tests (banded.rs:565-731,
+band_simd8.rs:403-468, sw_simd.rs:488-550) and the batched-region workload."""
+import numpy as np
+import pytest
+
+import saber_amd as S
+from oracle import oracle as O
+from saber_amd import BLOSUM45, BLOSUM62, PAM30, workload
+from tests.gen import Lcg, flat, jobs_array, mutate, rand_seq
+
+pytestmark = pytest.mark.gpu
+
+W = 17
+
+
+def run_both(ctx, qs, ss, lo, hi, mat, go, ge, bias=None):
+    """One job per (q, s) pair; GPU and oracle results as tuples of arrays."""
+    qres, qoff = flat(qs)
+    sres, soff = flat(ss)
+    n = len(qs)
+    jobs = jobs_array(np.arange(n), np.arange(n), np.zeros(n), [len(s) for s in ss], lo, hi)
+    qb = None
+    if bias is not None:
+        qb, _ = flat([np.asarray(b, np.int8).view(np.uint8) for b in bias])
+        qb = qb.view(np.int8)
+    sch = S.Scheme(mat, go, ge)
+    got = ctx.score_bands(qres, qoff, sres, soff, jobs, sch, qb)
+    want = O.banded_many(qres, qoff, sres, soff, jobs, mat, go, ge, qb)
+    return got, want
+
+
+def assert_same(got, want, what=""):
+    names = ("score", "q_end", "s_end", "cells")
+    for name, g, w in zip(names, got, want):
+        bad = np.nonzero(np.asarray(g).astype(np.int64) != np.asarray(w).astype(np.int64))[0]
+        assert len(bad) == 0, f"{what} {name} differs at jobs {bad[:10].tolist()}: got {np.asarray(g)[bad[:5]]} want {np.asarray(w)[bad[:5]]}"
+
+
+def random_cases(seed, count, maxlen, gaps, bias_every=3, wide=False):
+    r = Lcg(seed)
+    qs, ss, lo, hi, bias = [], [], [], [], []
+    for t in range(count):
+        la = 1 + r.next(maxlen)
+        a = rand_seq(r, la)
+        lb = 1 + r.next(maxlen)
+        b = mutate(r, a) if t % 2 == 0 else rand_seq(r, lb)
+        if len(b) == 0:
+            b = rand_seq(r, 3)
+        qs.append(a)
+        ss.append(b)
+        bias.append(np.array([r.next(9) - 4 for _ in range(la)], np.int8))
+        kind = t % 4
+        if kind == 0:
+            lo.append(-la)
+            hi.append(len(b))
+        elif kind == 1:
+            lo.append(-int(r.next(60)))
+            hi.append(int(r.next(60)) + (300 if wide and t % 5 == 0 else 0))
+        elif kind == 2:
+            lo.append(5)
+            hi.append(40)
+        else:
+            c = int(r.next(len(b) + la)) - la
+            h = [8, 16, 32, 64][r.next(4)]
+            lo.append(c - h)
+            hi.append(c + h)
+    return qs, ss, lo, hi, bias
+
+
+@pytest.mark.parametrize("gaps", [(11, 1), (10, 2), (6, 3), (15, 1), (8, 8)])
+def test_random_and_related_pairs(ctx, gaps):
+    # banded.rs:645-671 (simd_and_scalar_kernels_agree): 5 gap pairs, bias,
+    # wide and full bands
+    qs, ss, lo, hi, bias = random_cases(2024 + gaps[0], 300, 220, gaps, wide=True)
+    got, want = run_both(ctx, qs, ss, lo, hi, BLOSUM62, *gaps)
+    assert_same(got, want, f"no bias {gaps}")
+    got, want = run_both(ctx, qs, ss, lo, hi, BLOSUM62, *gaps, bias=bias)
+    assert_same(got, want, f"bias {gaps}")
+
+
+@pytest.mark.parametrize("mat", [BLOSUM62, PAM30, BLOSUM45], ids=["B62", "PAM30", "B45"])
+def test_u8_band_cases_all_matrices(ctx, mat):
+    # band_simd8.rs:403-451: one query, many subjects with related stretches,
+    # bands of width up to 200
+    r = Lcg(7)
+    for trial in range(6):
+        ql = 1 + r.next(220)
+        q = rand_seq(r, ql)
+        bias = np.array([r.next(9) - 4 for _ in range(ql)], np.int8)
+        subs = []
+        for k in range(1 + r.next(70)):
+            l = 1 + r.next(320)
+            s = rand_seq(r, l)
+            if k % 3 == 0:
+                o = r.next(l)
+                for t in range(min(max(l - o, 0), ql)):
+                    if r.next(4) > 0:
+                        s[o + t] = q[t]
+            subs.append(s)
+        lo, hi = [], []
+        for s in subs:
+            c = int(r.next(len(s) + ql)) - ql
+            w = int(r.next(200 if trial % 5 == 0 else 90))
+            lo.append(c - w // 2)
+            hi.append(c + (w + 1) // 2)
+        for go, ge in [(11, 1), (7, 2)]:
+            b = [bias] * len(subs) if trial % 2 == 0 else None
+            got, want = run_both(ctx, [q] * len(subs), subs, lo, hi, mat, go, ge, b)
+            assert_same(got, want, f"trial {trial} gaps {go}/{ge}")
+
+
+def test_exact32_kernel_matches_oracle(ctx32):
+    qs, ss, lo, hi, bias = random_cases(99, 200, 300, (11, 1), wide=True)
+    got, want = run_both(ctx32, qs, ss, lo, hi, BLOSUM62, 11, 1, bias=bias)
+    assert_same(got, want, "exact32")
+
+
+def test_saturation_rescored_exactly(ctx):
+    # Scores above the int16 range: W x 4000 gives 44 000 (banded.rs:719-730),
+    # W x 300 band +-5 gives 3300 next to an unrelated job (band_simd8.rs:454-468)
+    a = np.full(4000, W, np.uint8)
+    other = S.encode("MKVLAAGIVGKLLA")
+    w300 = np.full(300, W, np.uint8)
+    qs = [a, a, w300, w300, a[:3500]]
+    ss = [a, a, w300, other, a[:3500]]
+    lo = [-20, -4000, -5, -3, -3500]
+    hi = [20, 4000, 5, 3, 3500]
+    got, want = run_both(ctx, qs, ss, lo, hi, BLOSUM62, 11, 1)
+    assert_same(got, want, "saturation")
+    score, q_end, s_end, _ = got
+    assert score.tolist() == [44000, 44000, 3300, want[0][3], 38500]
+    assert (q_end[0], s_end[0]) == (4000, 4000)
+
+
+def test_edge_cases(ctx):
+    r = Lcg(11)
+    qs, ss, lo, hi = [], [], [], []
+    for t in range(400):
+        m = 1 + r.next(40)
+        n = 1 + r.next(40)
+        qs.append(rand_seq(r, m))
+        b = rand_seq(r, n) if t % 2 else mutate(r, qs[-1])
+        ss.append(b if len(b) else rand_seq(r, 2))
+        kind = t % 8
+        if kind == 0:       # band entirely right of the rectangle
+            lo.append(len(ss[-1]) + 3); hi.append(len(ss[-1]) + 10)
+        elif kind == 1:     # entirely left
+            lo.append(-m - 10); hi.append(-m - 1)
+        elif kind == 2:     # single diagonal
+            d = int(r.next(n + m)) - m
+            lo.append(d); hi.append(d)
+        elif kind == 3:     # hi < lo
+            lo.append(4); hi.append(2)
+        elif kind == 4:     # clipped on both sides
+            lo.append(-1000); hi.append(1000)
+        elif kind == 5:     # touching the corners
+            lo.append(len(ss[-1]) - 1); hi.append(len(ss[-1]) + 50)
+        elif kind == 6:
+            lo.append(-m - 50); hi.append(-(m - 1))
+        else:
+            c = int(r.next(n + m)) - m
+            h = int(r.next(70))
+            lo.append(c - h); hi.append(c + h)
+    got, want = run_both(ctx, qs, ss, lo, hi, BLOSUM62, 11, 1)
+    assert_same(got, want, "edges")
+    # out-of-alphabet codes score as X
+    q = np.array([0, 1, 2, 200, 4], np.uint8)
+    s1 = np.array([0, 1, 2, 22, 4], np.uint8)
+    s2 = np.array([255, 255], np.uint8)
+    got, want = run_both(ctx, [q, q], [s1, s2], [-5, -5], [5, 2], BLOSUM62, 11, 1)
+    assert_same(got, want, "codes")
+
+
+def test_every_variant_width(ctx):
+    """Each width 1..300 (all packed variants and the int32 route) on related
+    pairs long enough to fill the band."""
+    r = Lcg(123)
+    qs, ss, lo, hi = [], [], [], []
+    for w in range(1, 301):
+        for _ in range(2):
+            m = 60 + r.next(200)
+            a = rand_seq(r, m)
+            b = mutate(r, a)
+            if len(b) < 5:
+                b = rand_seq(r, 80)
+            c = int(r.next(21)) - 10
+            qs.append(a); ss.append(b)
+            lo.append(c - w // 2); hi.append(c - w // 2 + w - 1)
+    got, want = run_both(ctx, qs, ss, lo, hi, BLOSUM62, 11, 1)
+    assert_same(got, want, "widths")
+
+
+def test_region_workload_sample(ctx):
+    """20 000 regions of the batched banded-SW workload (lognormal windows,
+    half-widths 8/16/32/64, 40 % homologues), with and without bias."""
+    d = workload.band_regions(seed=5, n=20000)
+    sch = S.Scheme(BLOSUM62, 11, 1)
+    got = ctx.score_bands(d["qres"], d["qoff"], d["sres"], d["soff"], d["jobs"], sch)
+    want = O.banded_many(d["qres"], d["qoff"], d["sres"], d["soff"], d["jobs"], BLOSUM62, 11, 1)
+    assert_same(got, want, "regions")
+    assert (got[0][d["homolog"]] > 60).mean() > 0.5
+    qb = np.asarray(Lcg(1).next(1) + (np.arange(len(d["qres"])) % 9) - 4, np.int8)
+    got = ctx.score_bands(d["qres"], d["qoff"], d["sres"], d["soff"], d["jobs"], sch, qb)
+    want = O.banded_many(d["qres"], d["qoff"], d["sres"], d["soff"], d["jobs"], BLOSUM62, 11, 1, qb)
+    assert_same(got, want, "regions+bias")
+
+
+def test_score_windows_backend(ctx):
+    # ScoreBackend contract (gpu/mod.rs:9-31) and doctor::check_gpu's shape
+    # (doctor.rs:440-476): qlen 250, bias (i%5)-2, lengths 1+(13i)%700
+    r = Lcg(4)
+    q = rand_seq(r, 250)
+    bias = np.array([(i % 5) - 2 for i in range(250)], np.int8)
+    wins = [rand_seq(r, 1 + (13 * i) % 700) if i % 3 else mutate(r, q)[: 1 + (13 * i) % 700]
+            for i in range(300)]
+    wins = [w if len(w) else rand_seq(r, 1) for w in wins]
+    be = S.HipBackend(0)
+    got = be.score_windows(q, bias, wins, BLOSUM62, S.GapCosts(11, 1))
+    want = [O.sw_reference(q, w, BLOSUM62, bias, 11, 1)[0] for w in wins]
+    assert got == want
+    assert be.score_windows(q, None, [], BLOSUM62, S.GapCosts(11, 1)) == []
+    assert be.score_windows(np.zeros(0, np.uint8), None, wins[:3], BLOSUM62, S.GapCosts(11, 1)) == [0, 0, 0]
+    with pytest.raises(ValueError):
+        be.score_windows(q, bias[:10], wins[:3], BLOSUM62, S.GapCosts(11, 1))
+
+
+def test_reference_named_interfaces(ctx):
+    q = S.encode("WAGNER")
+    sch = S.Scheme(BLOSUM62, 11, 1)
+    assert S.banded_score(q, q, None, sch, S.Band.full(6, 6), ctx) == S.BandScore(37, 6, 6, 36)
+    jobs = [S.BandJob(q, S.Band(-2, 2)), S.BandJob(S.encode("PPPP"), S.Band(-1, 1)),
+            S.BandJob(np.zeros(0, np.uint8), S.Band(0, 0))]
+    assert S.score_banded_many(q, None, jobs, sch, ctx) == [37, 0, 0]
+    wag = S.encode("WAG")
+    assert S.score_many(q, None, [q, wag], BLOSUM62, S.GapCosts(11, 1), ctx) == [
+        37, O.sw_reference(q, wag, BLOSUM62, None, 11, 1)[0]]
