@@ -142,6 +142,41 @@ int saber_hip_band_batch_time(saber_hip_band_batch* b, int warmup, int iters, do
                               double* kernel_ms, uint64_t* kernel_cells, int* nvar,
                               uint64_t* cells);
 
+/* One seed hit for X-drop extension: query position qp, subject position sp
+ * (qp < query length, sp < subject length), as visit_scan passes them to
+ * xdrop_cat (src/search/engine/scan.rs:2414-2420). */
+typedef struct saber_hip_xdrop_hit {
+    uint32_t query;
+    uint32_t subject;
+    uint32_t qp;
+    uint32_t sp;
+} saber_hip_xdrop_hit;
+
+typedef struct saber_hip_xdrop_batch saber_hip_xdrop_batch;
+
+/* Ungapped X-drop of every hit (xdrop_cat, scan.rs:859-904): score, the
+ * half-open query range [q_lo, q_hi) and the steps taken (right steps incl. the
+ * seed pair + left steps). The matrix of `scheme` is used (gap costs are not).
+ * subjects == NULL selects the uploaded database; q_lo, q_hi, steps may be NULL. */
+int saber_hip_xdrop(saber_hip_ctx* ctx, const saber_hip_seqset* queries,
+                    const saber_hip_seqset* subjects, const saber_hip_xdrop_hit* hits,
+                    uint64_t nhits, const saber_hip_scheme* scheme, int32_t x, int32_t* score,
+                    uint32_t* q_lo, uint32_t* q_hi, uint32_t* steps);
+
+/* Device-resident X-drop batch (same pattern as the band batch). time():
+ * mean ms per run of the extension kernel (HIP events around it) and the
+ * total steps per run. */
+int saber_hip_xdrop_batch_create(saber_hip_ctx* ctx, const saber_hip_seqset* queries,
+                                 const saber_hip_seqset* subjects, const saber_hip_xdrop_hit* hits,
+                                 uint64_t nhits, const saber_hip_scheme* scheme, int32_t x,
+                                 saber_hip_xdrop_batch** out);
+int saber_hip_xdrop_batch_run(saber_hip_xdrop_batch* b);
+int saber_hip_xdrop_batch_read(saber_hip_xdrop_batch* b, int32_t* score, uint32_t* q_lo,
+                               uint32_t* q_hi, uint32_t* steps);
+int saber_hip_xdrop_batch_time(saber_hip_xdrop_batch* b, int warmup, int iters, double* ms_run,
+                               uint64_t* steps);
+void saber_hip_xdrop_batch_destroy(saber_hip_xdrop_batch* b);
+
 /* Kernel variant table of the packed band kernel: variant v uses L lanes per
  * job pair with C band cells per lane and covers clipped widths
  * L*C-K+1 .. L*C; bias = 1 for the build that adds a query bias.

@@ -1,0 +1,459 @@
+/*
+ * AVX2 CPU baseline for batched banded SW -- TEST / BENCH INFRASTRUCTURE ONLY.
+ *
+ * A C restatement of the reference's AVX2 band-scoring path, timed on the host
+ * cores beside the GPU (the reference is Rust and cannot be built here):
+ *
+ *  score_banded_many (src/algorithm/band_simd8.rs:78-185): bands clipped to
+ *    their rectangle; the u8 kernel (32 jobs of one query per register, lane
+ *    coordinates c = j - lo, unsigned saturating arithmetic with substitution
+ *    offset, band_simd8.rs:236-388) scores every job of width <= 126 whose
+ *    lane maximum stays below 255 - pmax; other jobs go to
+ *  banded_score (src/algorithm/banded.rs:408-449): the int16 intra-sequence
+ *    kernel (16 band cells per register, E resolved by a max-plus prefix scan
+ *    with a carried chunk value, banded_simd.rs:393-599) which declines when a
+ *    row maximum exceeds 32767 - 16*ge - 192, then the exact int32 kernel.
+ *
+ * Only scores are produced (score_banded_many's output). Every result is
+ * checked against the scalar oracle in tests/test_cpu_baseline.py.
+ */
+#include <immintrin.h>
+#include <pthread.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "oracle.h"
+
+#define U8_MAX_WIDTH 126
+#define INVALID 31
+#define NEG16 (-31000)
+
+static inline int code(uint8_t c) { return c < 24 ? c : 22; }
+
+/* ---------------- u8 inter-sequence kernel (up to 32 jobs of one query) -- */
+
+typedef struct {
+    const uint8_t* s;
+    int64_t n;
+    int64_t lo, w; /* clipped band: lo, hi - lo */
+} u8job;
+
+/* Lane maxima of up to 32 banded jobs of query q. */
+static void batch_u8(const uint8_t* q, int64_t m, const int8_t* bias, const u8job* jobs, int nl,
+                     const int32_t* mat, int32_t go, int32_t ge, int32_t off, uint8_t* maxes,
+                     __m256i* H, __m256i* E, __m256i* B, uint8_t* cols, int64_t cols_cap) {
+    int64_t wmax = 0;
+    for (int l = 0; l < nl; l++)
+        if (jobs[l].w > wmax) wmax = jobs[l].w;
+    const int64_t ncols = m + wmax;
+    (void)cols_cap;
+    /* column-major residues in lane coordinates, INVALID outside a subject */
+    memset(cols, INVALID, (size_t)ncols * 32);
+    for (int l = 0; l < nl; l++) {
+        int64_t c0 = -jobs[l].lo > 0 ? -jobs[l].lo : 0;
+        int64_t c1 = jobs[l].n - jobs[l].lo < ncols ? jobs[l].n - jobs[l].lo : ncols;
+        for (int64_t c = c0; c < c1; c++) cols[c * 32 + l] = (uint8_t)code(jobs[l].s[c + jobs[l].lo]);
+    }
+    int8_t wp1[32];
+    for (int l = 0; l < 32; l++) wp1[l] = l < nl ? (int8_t)(jobs[l].w + 1) : 0;
+    const __m256i wp1v = _mm256_loadu_si256((const __m256i*)wp1);
+    for (int64_t i = 0; i < m; i++) {
+        H[i] = _mm256_setzero_si256();
+        E[i] = _mm256_setzero_si256();
+        B[i] = _mm256_set1_epi8(bias ? bias[i] : 0);
+    }
+    /* shuffle tables: S(a, x) + off for x in 0..15 and 16..23 */
+    __m256i tlo[24], thi[24];
+    for (int a = 0; a < 24; a++) {
+        uint8_t t0[16], t1[16];
+        for (int x = 0; x < 16; x++) t0[x] = (uint8_t)(mat[a * 24 + x] + off);
+        for (int x = 0; x < 16; x++) t1[x] = x < 8 ? (uint8_t)(mat[a * 24 + 16 + x] + off) : 0;
+        tlo[a] = _mm256_broadcastsi128_si256(_mm_loadu_si128((const __m128i*)t0));
+        thi[a] = _mm256_broadcastsi128_si256(_mm_loadu_si128((const __m128i*)t1));
+    }
+    const __m256i gov = _mm256_set1_epi8((char)go), gev = _mm256_set1_epi8((char)ge);
+    const __m256i offv = _mm256_set1_epi8((char)off), zero = _mm256_setzero_si256();
+    const __m256i nib = _mm256_set1_epi8(0x0f), fifteen = _mm256_set1_epi8(15);
+    const __m256i invalid = _mm256_set1_epi8(INVALID), one = _mm256_set1_epi8(1);
+    __m256i max_h = zero, prof[24];
+    uint8_t qc[4096];
+    const uint8_t* qcodes = q;
+    uint8_t* qtmp = NULL;
+    if (m <= 4096) {
+        for (int64_t i = 0; i < m; i++) qc[i] = (uint8_t)code(q[i]);
+        qcodes = qc;
+    } else {
+        qtmp = (uint8_t*)malloc((size_t)m);
+        for (int64_t i = 0; i < m; i++) qtmp[i] = (uint8_t)code(q[i]);
+        qcodes = qtmp;
+    }
+    for (int64_t c = 0; c < ncols; c++) {
+        int64_t r0 = c - wmax > 0 ? c - wmax : 0;
+        int64_t r1 = c < m - 1 ? c : m - 1;
+        if (r0 > r1) continue;
+        const __m256i rv = _mm256_loadu_si256((const __m256i*)(cols + c * 32));
+        const __m256i colv = _mm256_cmpgt_epi8(invalid, rv);
+        const __m256i idx = _mm256_and_si256(rv, nib);
+        const __m256i ishi = _mm256_cmpgt_epi8(rv, fifteen);
+        for (int a = 0; a < 24; a++)
+            prof[a] = _mm256_blendv_epi8(_mm256_shuffle_epi8(tlo[a], idx),
+                                         _mm256_shuffle_epi8(thi[a], idx), ishi);
+        __m256i ul_h = r0 > 0 ? H[r0 - 1] : zero, u_h = zero, u_f = zero;
+        __m256i dv = _mm256_set1_epi8((char)(c - r0));
+        for (int64_t i = r0; i <= r1; i++) {
+            const __m256i valid = _mm256_and_si256(colv, _mm256_cmpgt_epi8(wp1v, dv));
+            const __m256i p = _mm256_add_epi8(prof[qcodes[i]], B[i]);
+            const __m256i ph = H[i];
+            const __m256i e = _mm256_and_si256(
+                _mm256_max_epu8(_mm256_subs_epu8(ph, gov), _mm256_subs_epu8(E[i], gev)), valid);
+            const __m256i f = _mm256_and_si256(
+                _mm256_max_epu8(_mm256_subs_epu8(u_h, gov), _mm256_subs_epu8(u_f, gev)), valid);
+            const __m256i diag = _mm256_subs_epu8(_mm256_adds_epu8(ul_h, p), offv);
+            const __m256i h = _mm256_and_si256(_mm256_max_epu8(_mm256_max_epu8(e, f), diag), valid);
+            max_h = _mm256_max_epu8(max_h, h);
+            ul_h = ph;
+            u_h = h;
+            u_f = f;
+            H[i] = h;
+            E[i] = e;
+            dv = _mm256_sub_epi8(dv, one);
+        }
+    }
+    _mm256_storeu_si256((__m256i*)maxes, max_h);
+    free(qtmp);
+}
+
+/* ---------------- int16 intra-sequence banded kernel (score only) ------- */
+
+static inline __m256i shift_in16(__m256i v, __m256i prev) {
+    /* [prev[15], v0 .. v14] */
+    __m256i t = _mm256_permute2x128_si256(prev, v, 0x21);
+    return _mm256_alignr_epi8(v, t, 14);
+}
+
+static inline __m256i prefix_max16(__m256i v, __m256i negv) {
+    v = _mm256_max_epi16(v, _mm256_alignr_epi8(v, negv, 14));
+    v = _mm256_max_epi16(v, _mm256_alignr_epi8(v, negv, 12));
+    v = _mm256_max_epi16(v, _mm256_alignr_epi8(v, negv, 8));
+    __m256i t = _mm256_permute2x128_si256(v, v, 0x08);
+    __m256i b = _mm256_shuffle_epi32(_mm256_shufflehi_epi16(t, 0xFF), 0xFF);
+    return _mm256_max_epi16(v, _mm256_blend_epi32(negv, b, 0xF0));
+}
+
+static inline int hmax16(__m256i v) {
+    __m128i a = _mm_max_epi16(_mm256_castsi256_si128(v), _mm256_extracti128_si256(v, 1));
+    a = _mm_max_epi16(a, _mm_shuffle_epi32(a, 0x4E));
+    a = _mm_max_epi16(a, _mm_shuffle_epi32(a, 0xB1));
+    a = _mm_max_epi16(a, _mm_shufflelo_epi16(a, 0xB1));
+    return (int16_t)_mm_extract_epi16(a, 0);
+}
+
+/* Returns 1 and the score when every value stayed within the int16 range,
+ * 0 when the caller must rerun exactly (banded_simd.rs:412-415, 580-583). */
+static int banded_i16(const uint8_t* q, int64_t m, const uint8_t* s, int64_t n,
+                      const int8_t* bias, const int32_t* mat, int32_t go, int32_t ge, int64_t lo,
+                      int64_t w, int16_t* buf, int8_t* prof, int32_t* out) {
+    if (go < 0 || go > 1000 || ge < 0 || ge > 1000 || ge > go) return 0;
+    const int32_t limit = 32767 - 16 * ge - 192;
+    const int64_t wp = w + 32;
+    int16_t *hp = buf, *fp = buf + wp, *hc = buf + 2 * wp, *fc = buf + 3 * wp;
+    for (int64_t k = 0; k < wp; k++) {
+        hp[k] = hc[k] = 0;
+        fp[k] = fc[k] = NEG16;
+    }
+    /* subject profile rows for the query letters: prof[a * stride + j] */
+    const int64_t stride = (n + 32 + 15) & ~(int64_t)15;
+    uint32_t present = 0;
+    for (int64_t i = 0; i < m; i++) present |= 1u << code(q[i]);
+    /* row 24: subject codes padded with '*' (23); rows a: 16-byte shuffles */
+    int8_t* codes = prof + 24 * stride;
+    for (int64_t j = 0; j < stride; j++) codes[j] = (int8_t)(j < n ? code(s[j]) : 23);
+    const __m128i nib = _mm_set1_epi8(0x0f), fif = _mm_set1_epi8(15);
+    for (int a = 0; a < 24; a++) {
+        if (!(present >> a & 1)) continue;
+        int8_t lo8[16], hi8[16];
+        for (int x = 0; x < 16; x++) {
+            int v = mat[a * 24 + x];
+            lo8[x] = (int8_t)(v < -128 ? -128 : v > 127 ? 127 : v);
+            v = x < 8 ? mat[a * 24 + 16 + x] : 0;
+            hi8[x] = (int8_t)(v < -128 ? -128 : v > 127 ? 127 : v);
+        }
+        const __m128i tl = _mm_loadu_si128((const __m128i*)lo8), th = _mm_loadu_si128((const __m128i*)hi8);
+        int8_t* row = prof + a * stride;
+        for (int64_t j = 0; j < stride; j += 16) {
+            const __m128i sv = _mm_loadu_si128((const __m128i*)(codes + j));
+            const __m128i idx = _mm_and_si128(sv, nib);
+            const __m128i v = _mm_blendv_epi8(_mm_shuffle_epi8(tl, idx), _mm_shuffle_epi8(th, idx),
+                                              _mm_cmpgt_epi8(sv, fif));
+            _mm_storeu_si128((__m128i*)(row + j), v);
+        }
+    }
+    const __m256i zero = _mm256_setzero_si256(), negv = _mm256_set1_epi16(NEG16);
+    const __m256i gov = _mm256_set1_epi16((short)go), gev = _mm256_set1_epi16((short)ge);
+    const __m256i lane = _mm256_setr_epi16(0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15);
+    const __m256i ramp = _mm256_mullo_epi16(lane, gev);
+    const __m256i ge16 = _mm256_set1_epi16((short)(16 * ge));
+    int32_t best = 0;
+    for (int64_t i = 0; i < m; i++) {
+        int64_t kmin = -(i + lo) > 0 ? -(i + lo) : 0;
+        int64_t kmax = n - 1 - i - lo < w - 1 ? n - 1 - i - lo : w - 1;
+        if (kmin > kmax) {
+            for (int64_t k = 0; k < wp; k++) {
+                hc[k] = k < kmin ? 0 : NEG16;
+                fc[k] = NEG16;
+            }
+        } else {
+            for (int64_t k = 0; k < kmin; k++) {
+                hc[k] = 0;
+                fc[k] = NEG16;
+            }
+            const int8_t* prow = prof + code(q[i]) * stride;
+            const __m256i bv = _mm256_set1_epi16((short)(bias ? bias[i] : 0));
+            __m256i carry = negv, rowmax = zero;
+            const int64_t base = i + lo;
+            for (int64_t k0 = kmin; k0 <= kmax; k0 += 16) {
+                const int64_t nvalid = kmax + 1 - k0 < 16 ? kmax + 1 - k0 : 16;
+                const __m256i sub = _mm256_cvtepi8_epi16(_mm_loadu_si128((const __m128i*)(prow + base + k0)));
+                const __m256i hpk = _mm256_loadu_si256((const __m256i*)(hp + k0));
+                const __m256i hp1 = _mm256_loadu_si256((const __m256i*)(hp + k0 + 1));
+                const __m256i fp1 = _mm256_loadu_si256((const __m256i*)(fp + k0 + 1));
+                const __m256i d = _mm256_adds_epi16(_mm256_adds_epi16(hpk, sub), bv);
+                const __m256i f = _mm256_max_epi16(_mm256_subs_epi16(hp1, gov), _mm256_subs_epi16(fp1, gev));
+                const __m256i hprime = _mm256_max_epi16(_mm256_max_epi16(d, f), zero);
+                const __m256i valid = _mm256_cmpgt_epi16(_mm256_set1_epi16((short)nvalid), lane);
+                const __m256i hprime_v = _mm256_blendv_epi8(negv, hprime, valid);
+                const __m256i open = _mm256_subs_epi16(hprime_v, gov);
+                const __m256i e_loc = _mm256_subs_epi16(
+                    prefix_max16(_mm256_adds_epi16(shift_in16(open, negv), ramp), negv), ramp);
+                const __m256i e = _mm256_max_epi16(e_loc, _mm256_subs_epi16(carry, ramp));
+                const __m256i h = _mm256_max_epi16(hprime, e);
+                const __m256i h_v = _mm256_blendv_epi8(negv, h, valid);
+                const __m256i f_v = _mm256_blendv_epi8(negv, f, valid);
+                _mm256_storeu_si256((__m256i*)(hc + k0), h_v);
+                _mm256_storeu_si256((__m256i*)(fc + k0), f_v);
+                rowmax = _mm256_max_epi16(rowmax, h_v);
+                const __m256i a = _mm256_max_epi16(_mm256_subs_epi16(hprime_v, gov), _mm256_subs_epi16(e_loc, gev));
+                const __m256i a15 = _mm256_shufflehi_epi16(
+                    _mm256_shufflelo_epi16(_mm256_permute4x64_epi64(a, 0xFF), 0xFF), 0xFF);
+                carry = _mm256_max_epi16(a15, _mm256_subs_epi16(carry, ge16));
+            }
+            const int32_t rm = hmax16(rowmax);
+            if (rm > limit) return 0;
+            if (rm > best) best = rm;
+            _mm256_storeu_si256((__m256i*)(hc + kmax + 1), negv);
+            _mm256_storeu_si256((__m256i*)(fc + kmax + 1), negv);
+        }
+        int16_t* t = hp;
+        hp = hc;
+        hc = t;
+        t = fp;
+        fp = fc;
+        fc = t;
+    }
+    *out = best;
+    return 1;
+}
+
+/* ---------------- score_banded_many for one query ----------------------- */
+
+typedef struct {
+    int64_t k, w, n;
+} order_t;
+
+static int order_cmp(const void* a, const void* b) {
+    const order_t *x = (const order_t*)a, *y = (const order_t*)b;
+    if (x->w != y->w) return x->w < y->w ? -1 : 1;
+    if (x->n != y->n) return x->n < y->n ? -1 : 1;
+    return x->k < y->k ? -1 : x->k > y->k;
+}
+
+typedef struct {
+    __m256i *H, *E, *B;
+    int64_t hcap;
+    uint8_t* cols;
+    int64_t ccap;
+    int16_t* buf;
+    int64_t bcap;
+    int8_t* prof;
+    int64_t pcap;
+    order_t* ord;
+    int64_t ocap;
+} scratch_t;
+
+static void* grow(void* p, int64_t* cap, int64_t need, size_t elem, size_t align) {
+    if (need <= *cap) return p;
+    free(p);
+    *cap = need * 2;
+    void* q = NULL;
+    if (posix_memalign(&q, align, (size_t)*cap * elem)) return NULL;
+    return q;
+}
+
+static int32_t exact_score(const uint8_t* q, int64_t m, const uint8_t* s, int64_t n,
+                           const int8_t* bias, const int32_t* mat, int32_t go, int32_t ge,
+                           int64_t lo, int64_t w, scratch_t* sc) {
+    sc->buf = (int16_t*)grow(sc->buf, &sc->bcap, 4 * (w + 32) + 16, 2, 32);
+    sc->prof = (int8_t*)grow(sc->prof, &sc->pcap, 25 * (((n + 32 + 15) & ~15) + 16), 1, 32);
+    int32_t r;
+    if (banded_i16(q, m, s, n, bias, mat, go, ge, lo, w, sc->buf, sc->prof, &r)) return r;
+    int64_t qe, se;
+    uint64_t cells;
+    or_banded_score(q, m, s, n, bias, mat, go, ge, lo, lo + w - 1, &r, &qe, &se, &cells);
+    return r;
+}
+
+static void score_banded_many(const uint8_t* q, int64_t m, const int8_t* bias,
+                              const uint8_t* const* subj, const int64_t* slen, const int64_t* blo,
+                              const int64_t* bhi, int64_t njobs, const int32_t* mat, int32_t go,
+                              int32_t ge, int32_t* out, scratch_t* sc) {
+    for (int64_t k = 0; k < njobs; k++) out[k] = 0;
+    if (njobs == 0 || m == 0) return;
+    int smin = 1 << 30, smax = -(1 << 30), bmin = 0, bmax = 0;
+    for (int t = 0; t < 576; t++) {
+        if (mat[t] < smin) smin = mat[t];
+        if (mat[t] > smax) smax = mat[t];
+    }
+    if (bias) {
+        bmin = 127;
+        bmax = -128;
+        for (int64_t i = 0; i < m; i++) {
+            if (bias[i] < bmin) bmin = bias[i];
+            if (bias[i] > bmax) bmax = bias[i];
+        }
+    }
+    int t0 = smin + (bmin < 0 ? bmin : 0);
+    const int off = -(t0 < 0 ? t0 : 0);
+    const int pmax = smax + (bmax > 0 ? bmax : 0) + off;
+    const int gaps_ok = go >= 0 && go <= 255 && ge >= 0 && ge <= 255;
+    const int u8_ok = pmax < 128 && gaps_ok && off <= 127;
+    sc->ord = (order_t*)grow(sc->ord, &sc->ocap, njobs, sizeof(order_t), 16);
+    int64_t nord = 0;
+    for (int64_t k = 0; k < njobs; k++) {
+        const int64_t n = slen[k];
+        if (n == 0 || bhi[k] < blo[k]) continue;
+        const int64_t lo = blo[k] > -(m - 1) ? blo[k] : -(m - 1);
+        const int64_t hi = bhi[k] < n - 1 ? bhi[k] : n - 1;
+        if (hi < lo) continue;
+        if (u8_ok && hi - lo <= U8_MAX_WIDTH) {
+            order_t o = {k, hi - lo, n};
+            sc->ord[nord++] = o;
+        } else {
+            out[k] = exact_score(q, m, subj[k], n, bias, mat, go, ge, lo, hi - lo + 1, sc);
+        }
+    }
+    if (nord == 0) return;
+    qsort(sc->ord, (size_t)nord, sizeof(order_t), order_cmp);
+    const int limit = 255 - pmax;
+    sc->H = (__m256i*)grow(sc->H, &sc->hcap, 3 * m, sizeof(__m256i), 32);
+    sc->E = sc->H + m;
+    sc->B = sc->H + 2 * m;
+    for (int64_t c = 0; c < nord; c += 32) {
+        const int nl = nord - c < 32 ? (int)(nord - c) : 32;
+        u8job jobs[32];
+        int64_t wmax = 0;
+        for (int l = 0; l < nl; l++) {
+            const int64_t k = sc->ord[c + l].k;
+            const int64_t n = slen[k];
+            const int64_t lo = blo[k] > -(m - 1) ? blo[k] : -(m - 1);
+            const int64_t hi = bhi[k] < n - 1 ? bhi[k] : n - 1;
+            jobs[l].s = subj[k];
+            jobs[l].n = n;
+            jobs[l].lo = lo;
+            jobs[l].w = hi - lo;
+            if (hi - lo > wmax) wmax = hi - lo;
+        }
+        sc->cols = (uint8_t*)grow(sc->cols, &sc->ccap, 32 * (m + wmax + 1), 1, 32);
+        uint8_t maxes[32];
+        batch_u8(q, m, bias, jobs, nl, mat, go, ge, off, maxes, sc->H, sc->E, sc->B, sc->cols, sc->ccap);
+        for (int l = 0; l < nl; l++) {
+            const int64_t k = sc->ord[c + l].k;
+            if (maxes[l] >= limit)
+                out[k] = exact_score(q, m, jobs[l].s, jobs[l].n, bias, mat, go, ge, jobs[l].lo,
+                                     jobs[l].w + 1, sc);
+            else
+                out[k] = maxes[l];
+        }
+    }
+}
+
+/* ---------------- batch driver ------------------------------------------ */
+
+typedef struct {
+    const uint8_t *qres, *sres;
+    const int8_t* bias;
+    const uint64_t* qoff;
+    const uint32_t* qlen;
+    const uint64_t* soff;
+    const uint32_t* slen;
+    const int32_t *lo, *hi;
+    const int64_t* grp; /* group starts (jobs of one query are consecutive) */
+    int64_t g0, g1;
+    const int32_t* mat;
+    int32_t go, ge;
+    int32_t* score;
+} drv_t;
+
+static void* drv_worker(void* p) {
+    drv_t* a = (drv_t*)p;
+    scratch_t sc;
+    memset(&sc, 0, sizeof(sc));
+    int64_t cap = 0;
+    const uint8_t** subj = NULL;
+    int64_t *sl = NULL, *bl = NULL, *bh = NULL;
+    for (int64_t g = a->g0; g < a->g1; g++) {
+        const int64_t k0 = a->grp[g], k1 = a->grp[g + 1], nj = k1 - k0;
+        if (nj > cap) {
+            cap = nj * 2;
+            subj = (const uint8_t**)realloc(subj, sizeof(*subj) * cap);
+            sl = (int64_t*)realloc(sl, 8 * cap);
+            bl = (int64_t*)realloc(bl, 8 * cap);
+            bh = (int64_t*)realloc(bh, 8 * cap);
+        }
+        for (int64_t t = 0; t < nj; t++) {
+            subj[t] = a->sres + a->soff[k0 + t];
+            sl[t] = a->slen[k0 + t];
+            bl[t] = a->lo[k0 + t];
+            bh[t] = a->hi[k0 + t];
+        }
+        score_banded_many(a->qres + a->qoff[k0], a->qlen[k0], a->bias ? a->bias + a->qoff[k0] : NULL,
+                          subj, sl, bl, bh, nj, a->mat, a->go, a->ge, a->score + k0, &sc);
+    }
+    free(subj);
+    free(sl);
+    free(bl);
+    free(bh);
+    free(sc.H);
+    free(sc.cols);
+    free(sc.buf);
+    free(sc.prof);
+    free(sc.ord);
+    return NULL;
+}
+
+/* Scores of a band batch in the flat layout of or_banded_many. Consecutive
+ * jobs with the same query offset form one score_banded_many call; groups are
+ * split across nthreads threads. */
+void cpu_banded_many_avx2(const uint8_t* qres, const uint8_t* sres, const int8_t* bias,
+                          const uint64_t* qoff, const uint32_t* qlen, const uint64_t* soff,
+                          const uint32_t* slen, const int32_t* lo, const int32_t* hi, int64_t njobs,
+                          const int32_t* mat, int32_t go, int32_t ge, int32_t* score, int nthreads) {
+    if (njobs == 0) return;
+    int64_t* grp = (int64_t*)malloc(8 * (size_t)(njobs + 1));
+    int64_t ng = 0;
+    for (int64_t k = 0; k < njobs; k++)
+        if (k == 0 || qoff[k] != qoff[k - 1] || qlen[k] != qlen[k - 1]) grp[ng++] = k;
+    grp[ng] = njobs;
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > 256) nthreads = 256;
+    pthread_t th[256];
+    drv_t args[256];
+    for (int t = 0; t < nthreads; t++) {
+        drv_t a = {qres, sres, bias, qoff, qlen, soff, slen, lo, hi, grp,
+                   ng * t / nthreads, ng * (t + 1) / nthreads, mat, go, ge, score};
+        args[t] = a;
+        pthread_create(&th[t], NULL, drv_worker, &args[t]);
+    }
+    for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
+    free(grp);
+}

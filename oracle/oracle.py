@@ -118,6 +118,47 @@ def banded_many(qres, qoff, sres, soff, jobs, matrix, go, ge, qbias=None, nthrea
     return score, qe, se, ce
 
 
+_avx2 = None
+
+
+def avx2_lib():
+    global _avx2
+    if _avx2 is None:
+        if not os.path.exists(AVX2_LIB):
+            subprocess.run(["make", "-s", "-C", HERE, "avx2"], check=True)
+        L = C.CDLL(AVX2_LIB)
+        u8p, i8p, i32p, u64p, u32p = (C.POINTER(C.c_uint8), C.POINTER(C.c_int8), C.POINTER(C.c_int32),
+                                      C.POINTER(C.c_uint64), C.POINTER(C.c_uint32))
+        L.cpu_banded_many_avx2.argtypes = [u8p, u8p, i8p, u64p, u32p, u64p, u32p, i32p, i32p, C.c_int64,
+                                           i32p, C.c_int32, C.c_int32, i32p, C.c_int]
+        L.cpu_banded_many_avx2.restype = None
+        _avx2 = L
+    return _avx2
+
+
+def banded_many_avx2(qres, qoff, sres, soff, jobs, matrix, go, ge, qbias=None, nthreads=8):
+    """Scores of a band batch by the AVX2 restatement of score_banded_many
+    (jobs of one query must be consecutive to share u8 batches)."""
+    qres, sres = _u8(qres), _u8(sres)
+    qoff = np.asarray(qoff, np.uint64)
+    soff = np.asarray(soff, np.uint64)
+    jq = np.ascontiguousarray(qoff[jobs["query"]])
+    jql = np.ascontiguousarray((qoff[jobs["query"] + 1] - qoff[jobs["query"]]).astype(np.uint32))
+    js = np.ascontiguousarray(soff[jobs["subject"]] + jobs["win_lo"].astype(np.uint64))
+    jsl = np.ascontiguousarray((jobs["win_hi"] - jobs["win_lo"]).astype(np.uint32))
+    lo = np.ascontiguousarray(jobs["band_lo"].astype(np.int32))
+    hi = np.ascontiguousarray(jobs["band_hi"].astype(np.int32))
+    n = len(jobs)
+    score = np.zeros(n, np.int32)
+    b = _bias(qbias)
+    mat = _mat(matrix)
+    avx2_lib().cpu_banded_many_avx2(_p(qres, C.c_uint8), _p(sres, C.c_uint8), _p(b, C.c_int8),
+                                    _p(jq, C.c_uint64), _p(jql, C.c_uint32), _p(js, C.c_uint64),
+                                    _p(jsl, C.c_uint32), _p(lo, C.c_int32), _p(hi, C.c_int32), n,
+                                    _p(mat, C.c_int32), go, ge, _p(score, C.c_int32), nthreads)
+    return score
+
+
 def xdrop(q, s, bias, matrix, qp, sp, x):
     q, s, mat, b = _u8(q), _u8(s), _mat(matrix), _bias(bias)
     sc, lo_, hi_, st = C.c_int32(), C.c_int64(), C.c_int64(), C.c_uint64()

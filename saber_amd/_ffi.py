@@ -119,7 +119,7 @@ def wlib():
             raise OSError(f"{WLIB_PATH} is missing: run saber_amd/build.py")
         W = C.CDLL(WLIB_PATH)
         W.wl_band_regions.restype = P
-        W.wl_band_regions.argtypes = [C.c_uint64, C.c_uint64, C.c_double, C.c_double, C.c_uint32,
+        W.wl_band_regions.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, C.c_double, C.c_double, C.c_uint32,
                                       C.c_uint32, C.c_double, C.POINTER(C.c_int32), C.c_int,
                                       C.POINTER(C.c_int32)]
         W.wl_sequences.restype = P

@@ -630,3 +630,203 @@ int saber_hip_score_windows(saber_hip_ctx* c, const uint8_t* query, uint32_t qle
 }
 
 }  // extern "C"
+
+struct saber_hip_xdrop_batch {
+    saber_hip_ctx* ctx = nullptr;
+    uint64_t nhits = 0;
+    int32_t x = 0;
+    DevBuf<uint8_t> q, s;
+    DevBuf<int8_t> bias;
+    DevBuf<uint64_t> qoff, soff;
+    DevBuf<uint32_t> hits;
+    DevBuf<short> table;
+    DevBuf<uint32_t> counter;
+    DevBuf<int32_t> part_best, score;
+    DevBuf<uint32_t> part_t, part_steps, q_lo, q_hi, steps;
+    const uint8_t* sres = nullptr;
+    const uint64_t* soffp = nullptr;
+    bool has_bias = false, ran = false;
+    int grid = 0;
+};
+
+namespace {
+
+int xdrop_enqueue(saber_hip_xdrop_batch* b) {
+    hipStream_t st = b->ctx->stream;
+    HIP_TRY(fill_u32(b->counter.p, 1, 0, st));
+    XdropArgs a{};
+    a.hits = b->hits.p;
+    a.nhits = (uint32_t)b->nhits;
+    a.qoff = b->qoff.p;
+    a.qres = b->q.p;
+    a.qbias = b->has_bias ? b->bias.p : nullptr;
+    a.soff = b->soffp;
+    a.sres = b->sres;
+    a.table = b->table.p;
+    a.x = b->x;
+    a.next_item = b->counter.p;
+    a.part_best = b->part_best.p;
+    a.part_t = b->part_t.p;
+    a.part_steps = b->part_steps.p;
+    HIP_TRY(launch_xdrop(a, b->score.p, b->q_lo.p, b->q_hi.p, b->steps.p, b->grid, st));
+    return SABER_HIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int saber_hip_xdrop_batch_create(saber_hip_ctx* c, const saber_hip_seqset* queries,
+                                 const saber_hip_seqset* subjects, const saber_hip_xdrop_hit* hits,
+                                 uint64_t nhits, const saber_hip_scheme* scheme, int32_t x,
+                                 saber_hip_xdrop_batch** out) {
+    if (!c || !out || !scheme) return fail(SABER_HIP_EINVAL, "null argument");
+    *out = nullptr;
+    if (nhits > 0 && !hits) return fail(SABER_HIP_EINVAL, "hits is null");
+    if (nhits >= (1ull << 31)) return fail(SABER_HIP_EINVAL, "too many hits for one batch");
+    if (int r = check_seqset(queries, "queries")) return r;
+    if (subjects)
+        if (int r = check_seqset(subjects, "subjects")) return r;
+    for (int i = 0; i < 576; ++i)
+        if (scheme->matrix[i] < -32767 || scheme->matrix[i] > 32767)
+            return fail(SABER_HIP_EINVAL, "substitution score outside the int16 range");
+    std::lock_guard<std::mutex> g(c->mu);
+    HIP_TRY(hipSetDevice(c->device));
+    const uint64_t* soffs = subjects ? subjects->offsets : c->db_off.data();
+    const uint32_t nsub = subjects ? subjects->count : (c->db_off.empty() ? 0 : (uint32_t)c->db_off.size() - 1);
+    for (uint64_t k = 0; k < nhits; ++k) {
+        const saber_hip_xdrop_hit& h = hits[k];
+        if (h.query >= queries->count || h.subject >= nsub)
+            return fail(SABER_HIP_EINVAL, "hit index out of range");
+        if (h.qp >= queries->offsets[h.query + 1] - queries->offsets[h.query] ||
+            h.sp >= soffs[h.subject + 1] - soffs[h.subject])
+            return fail(SABER_HIP_EINVAL, "hit position outside its sequence");
+    }
+    auto b = std::make_unique<saber_hip_xdrop_batch>();
+    b->ctx = c;
+    b->nhits = nhits;
+    b->x = x;
+    b->has_bias = queries->bias != nullptr;
+    std::vector<short> table(kTableDim * kTableDim);
+    for (int a = 0; a < kTableDim; ++a)
+        for (int cc = 0; cc < kTableDim; ++cc)
+            table[a * kTableDim + cc] =
+                (short)scheme->matrix[(a < 24 ? a : kCodeX) * 24 + (cc < 24 ? cc : kCodeX)];
+    hipStream_t st = c->stream;
+    const uint64_t nq = queries->count ? queries->offsets[queries->count] : 0;
+    if (int r = upload(b->q, queries->residues, nq, st)) return r;
+    if (b->has_bias)
+        if (int r = upload(b->bias, queries->bias, nq, st)) return r;
+    if (int r = upload(b->qoff, queries->offsets, (uint64_t)queries->count + 1, st)) return r;
+    if (subjects) {
+        const uint64_t ns = subjects->count ? subjects->offsets[subjects->count] : 0;
+        if (int r = upload(b->s, subjects->residues, ns, st)) return r;
+        if (int r = upload(b->soff, subjects->offsets, (uint64_t)subjects->count + 1, st)) return r;
+        b->sres = b->s.p;
+        b->soffp = b->soff.p;
+    } else {
+        if (!c->db.p && nhits > 0) return fail(SABER_HIP_ESTATE, "no database uploaded");
+        if (int r = upload(b->soff, c->db_off.data(), (uint64_t)c->db_off.size(), st)) return r;
+        b->sres = c->db.p;
+        b->soffp = b->soff.p;
+    }
+    if (int r = upload(b->hits, reinterpret_cast<const uint32_t*>(hits), 4 * nhits, st)) return r;
+    if (int r = upload(b->table, table.data(), table.size(), st)) return r;
+    HIP_TRY(b->counter.alloc(1));
+    const uint64_t n1 = nhits ? nhits : 1;
+    HIP_TRY(b->part_best.alloc(2 * n1));
+    HIP_TRY(b->part_t.alloc(2 * n1));
+    HIP_TRY(b->part_steps.alloc(2 * n1));
+    HIP_TRY(b->score.alloc(n1));
+    HIP_TRY(b->q_lo.alloc(n1));
+    HIP_TRY(b->q_hi.alloc(n1));
+    HIP_TRY(b->steps.alloc(n1));
+    int cus = 0;
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+    // Persistent grid: 8 blocks of 4 waves per CU, never more than the work.
+    const uint64_t need = (2 * nhits + 255) / 256;
+    b->grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)cus * 8, need));
+    HIP_TRY(hipStreamSynchronize(st));
+    *out = b.release();
+    return SABER_HIP_OK;
+}
+
+int saber_hip_xdrop_batch_run(saber_hip_xdrop_batch* b) {
+    if (!b) return fail(SABER_HIP_EINVAL, "batch is null");
+    std::lock_guard<std::mutex> g(b->ctx->mu);
+    HIP_TRY(hipSetDevice(b->ctx->device));
+    if (int r = xdrop_enqueue(b)) return r;
+    b->ran = true;
+    return SABER_HIP_OK;
+}
+
+int saber_hip_xdrop_batch_read(saber_hip_xdrop_batch* b, int32_t* score, uint32_t* q_lo,
+                               uint32_t* q_hi, uint32_t* steps) {
+    if (!b) return fail(SABER_HIP_EINVAL, "batch is null");
+    if (!b->ran) return fail(SABER_HIP_ESTATE, "batch has not run");
+    std::lock_guard<std::mutex> g(b->ctx->mu);
+    HIP_TRY(hipSetDevice(b->ctx->device));
+    hipStream_t st = b->ctx->stream;
+    const uint64_t n = b->nhits;
+    if (n == 0) return SABER_HIP_OK;
+    if (score) HIP_TRY(hipMemcpyAsync(score, b->score.p, 4 * n, hipMemcpyDeviceToHost, st));
+    if (q_lo) HIP_TRY(hipMemcpyAsync(q_lo, b->q_lo.p, 4 * n, hipMemcpyDeviceToHost, st));
+    if (q_hi) HIP_TRY(hipMemcpyAsync(q_hi, b->q_hi.p, 4 * n, hipMemcpyDeviceToHost, st));
+    if (steps) HIP_TRY(hipMemcpyAsync(steps, b->steps.p, 4 * n, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return SABER_HIP_OK;
+}
+
+int saber_hip_xdrop_batch_time(saber_hip_xdrop_batch* b, int warmup, int iters, double* ms_run,
+                               uint64_t* steps) {
+    if (!b || iters < 1 || warmup < 0) return fail(SABER_HIP_EINVAL, "bad arguments");
+    for (int i = 0; i < warmup; ++i)
+        if (int r = saber_hip_xdrop_batch_run(b)) return r;
+    std::lock_guard<std::mutex> g(b->ctx->mu);
+    HIP_TRY(hipSetDevice(b->ctx->device));
+    hipStream_t st = b->ctx->stream;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, st));
+    for (int it = 0; it < iters; ++it)
+        if (int r = xdrop_enqueue(b)) return r;
+    HIP_TRY(hipEventRecord(e1, st));
+    HIP_TRY(hipEventSynchronize(e1));
+    b->ran = true;
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (ms_run) *ms_run = ms / iters;
+    if (steps) {
+        std::vector<uint32_t> s(b->nhits);
+        if (b->nhits) HIP_TRY(hipMemcpy(s.data(), b->steps.p, 4 * b->nhits, hipMemcpyDeviceToHost));
+        uint64_t t = 0;
+        for (uint32_t v : s) t += v;
+        *steps = t;
+    }
+    return SABER_HIP_OK;
+}
+
+void saber_hip_xdrop_batch_destroy(saber_hip_xdrop_batch* b) {
+    if (!b) return;
+    (void)hipSetDevice(b->ctx->device);
+    (void)hipStreamSynchronize(b->ctx->stream);
+    delete b;
+}
+
+int saber_hip_xdrop(saber_hip_ctx* c, const saber_hip_seqset* queries,
+                    const saber_hip_seqset* subjects, const saber_hip_xdrop_hit* hits,
+                    uint64_t nhits, const saber_hip_scheme* scheme, int32_t x, int32_t* score,
+                    uint32_t* q_lo, uint32_t* q_hi, uint32_t* steps) {
+    if (nhits > 0 && !score) return fail(SABER_HIP_EINVAL, "score is null");
+    saber_hip_xdrop_batch* b = nullptr;
+    if (int r = saber_hip_xdrop_batch_create(c, queries, subjects, hits, nhits, scheme, x, &b)) return r;
+    int r = saber_hip_xdrop_batch_run(b);
+    if (r == SABER_HIP_OK) r = saber_hip_xdrop_batch_read(b, score, q_lo, q_hi, steps);
+    saber_hip_xdrop_batch_destroy(b);
+    return r;
+}
+
+}  // extern "C"

@@ -76,3 +76,26 @@ const BandVariant* band_variants(int* count);
 hipError_t launch_generic(const GenericArgs& a, hipStream_t s);
 
 }  // namespace saber
+
+namespace saber {
+
+struct XdropArgs {
+    const uint32_t* hits;      // 4 u32 per hit: query, subject, qp, sp
+    uint32_t nhits;
+    const uint64_t* qoff;      // query offsets (count+1)
+    const uint8_t* qres;
+    const int8_t* qbias;       // may be null
+    const uint64_t* soff;      // subject offsets
+    const uint8_t* sres;
+    const short* table;        // 32 x 32
+    int32_t x;
+    uint32_t* next_item;       // work counter (cleared before launch)
+    int32_t* part_best;        // 2 per hit: right, left
+    uint32_t* part_t;          // 2 per hit: end_t / st_t
+    uint32_t* part_steps;      // 2 per hit
+};
+
+hipError_t launch_xdrop(const XdropArgs& a, int32_t* score, uint32_t* q_lo, uint32_t* q_hi,
+                        uint32_t* steps, int grid, hipStream_t s);
+
+}  // namespace saber

@@ -118,20 +118,28 @@ struct Batch {
 
 extern "C" {
 
-// Banded-region batch: n regions, each its own query window and subject
-// window; job k = (query k, subject k, band [c-h, c+h]).
-void* wl_band_regions(uint64_t seed, uint64_t n, double median, double sigma, uint32_t lmin,
-                      uint32_t lmax, double homolog_frac, const int32_t* half_widths,
-                      int n_half, const int32_t* mat) {
+// Banded-region batch: n regions over ceil(n / per_query) queries (the
+// engine scores all band jobs of one query together). Region k pairs query
+// k / per_query with its own subject window and band [c-h, c+h]: a planted
+// homologue (mutated copy of the query with random flanks, c = left flank)
+// or an unrelated window (c uniform over the middle of the rectangle).
+void* wl_band_regions(uint64_t seed, uint64_t n, uint32_t per_query, double median, double sigma,
+                      uint32_t lmin, uint32_t lmax, double homolog_frac,
+                      const int32_t* half_widths, int n_half, const int32_t* mat) {
     auto* B = new Batch();
     Model M(mat);
     Rng r(seed);
     std::vector<uint8_t> tmp;
+    if (per_query == 0) per_query = 1;
+    size_t q0 = 0;
+    uint32_t m = 0;
     for (uint64_t k = 0; k < n; ++k) {
-        const uint32_t m = lognormal_len(r, median, sigma, lmin, lmax);
-        const size_t q0 = B->qres.size();
-        for (uint32_t i = 0; i < m; ++i) B->qres.push_back(M.residue(r));
-        B->qoff.push_back(B->qres.size());
+        if (k % per_query == 0) {
+            m = lognormal_len(r, median, sigma, lmin, lmax);
+            q0 = B->qres.size();
+            for (uint32_t i = 0; i < m; ++i) B->qres.push_back(M.residue(r));
+            B->qoff.push_back(B->qres.size());
+        }
         const int32_t h = half_widths[r.below((uint64_t)n_half)];
         const bool hom = r.uni() < homolog_frac;
         int64_t c;

@@ -1,7 +1,8 @@
 """Synthetic workloads (host side, deterministic from a seed).
 
 band_regions: the batched banded-SW workload -- n candidate regions, each a
-(query window, subject window, centre diagonal) triple; window lengths
+(query window, subject window, centre diagonal) triple, per_query regions
+sharing one query (as the engine's per-query band batches do); window lengths
 lognormal (median 150, sigma 0.6, clamped 20..2000); band half-widths drawn
 uniformly from {8, 16, 32, 64}; residues i.i.d. from the Robinson background;
 a fraction of regions (0.4) are planted homologues (BLOSUM62-conditional
@@ -23,12 +24,12 @@ def _mat(matrix):
     return np.ascontiguousarray(np.asarray(matrix, dtype=np.int32).reshape(576))
 
 
-def band_regions(seed, n, median=150.0, sigma=0.6, lmin=20, lmax=2000, homolog_frac=0.4,
-                 half_widths=(8, 16, 32, 64), matrix=BLOSUM62):
+def band_regions(seed, n, per_query=32, median=150.0, sigma=0.6, lmin=20, lmax=2000,
+                 homolog_frac=0.4, half_widths=(8, 16, 32, 64), matrix=BLOSUM62):
     W = _ffi.wlib()
     hw = np.asarray(half_widths, dtype=np.int32)
     mat = _mat(matrix)
-    h = W.wl_band_regions(seed, n, median, sigma, lmin, lmax, homolog_frac,
+    h = W.wl_band_regions(seed, n, per_query, median, sigma, lmin, lmax, homolog_frac,
                           _ffi.ptr(hw, C.c_int32), len(hw), _ffi.ptr(mat, C.c_int32))
     try:
         a, b, c, d = C.c_uint64(), C.c_uint64(), C.c_uint64(), C.c_uint64()
@@ -46,7 +47,7 @@ def band_regions(seed, n, median=150.0, sigma=0.6, lmin=20, lmax=2000, homolog_f
     finally:
         W.wl_free(h)
     jobs = np.zeros(n, JOB_DTYPE)
-    jobs["query"] = np.arange(n, dtype=np.uint32)
+    jobs["query"] = (np.arange(n, dtype=np.uint64) // max(per_query, 1)).astype(np.uint32)
     jobs["subject"] = np.arange(n, dtype=np.uint32)
     jobs["win_lo"] = 0
     jobs["win_hi"] = (soff[1:] - soff[:-1]).astype(np.uint32)

@@ -1,0 +1,66 @@
+"""The AVX2 CPU baseline (restatement of score_banded_many + banded_score's
+int16 kernel) must give the oracle's scores; it is what bench.py times on the
+host cores. CPU only."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from saber_amd import BLOSUM45, BLOSUM62, PAM30, workload
+from tests.gen import Lcg, flat, jobs_array, mutate, rand_seq
+
+
+def test_region_workload_scores_match_oracle():
+    d = workload.band_regions(seed=21, n=6000, per_query=32)
+    got = O.banded_many_avx2(d["qres"], d["qoff"], d["sres"], d["soff"], d["jobs"], BLOSUM62, 11, 1)
+    want = O.banded_many(d["qres"], d["qoff"], d["sres"], d["soff"], d["jobs"], BLOSUM62, 11, 1)[0]
+    assert np.array_equal(got, want)
+    qb = ((np.arange(len(d["qres"])) * 7) % 9 - 4).astype(np.int8)
+    got = O.banded_many_avx2(d["qres"], d["qoff"], d["sres"], d["soff"], d["jobs"], BLOSUM62, 11, 1, qb)
+    want = O.banded_many(d["qres"], d["qoff"], d["sres"], d["soff"], d["jobs"], BLOSUM62, 11, 1, qb)[0]
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("mat,gaps", [(BLOSUM62, (11, 1)), (PAM30, (10, 1)), (BLOSUM45, (7, 2)),
+                                      (BLOSUM62, (6, 3)), (BLOSUM62, (8, 8))])
+def test_mixed_bands_one_query_per_group(mat, gaps):
+    r = Lcg(17 + gaps[0])
+    qs, ss, lo, hi = [], [], [], []
+    for g in range(12):
+        q = rand_seq(r, 1 + r.next(300))
+        for k in range(1 + r.next(60)):
+            s = mutate(r, q) if k % 3 == 0 else rand_seq(r, 1 + r.next(300))
+            if len(s) == 0:
+                s = rand_seq(r, 4)
+            qs.append(q)
+            ss.append(s)
+            c = int(r.next(len(s) + len(q))) - len(q)
+            w = int(r.next(300 if k % 7 == 0 else 100))
+            lo.append(c - w // 2)
+            hi.append(c + (w + 1) // 2)
+    # one query entry per group: consecutive jobs share the query offset
+    uq, qidx = [], []
+    for q in qs:
+        if not uq or uq[-1] is not q:
+            uq.append(q)
+        qidx.append(len(uq) - 1)
+    qres, qoff = flat(uq)
+    sres, soff = flat(ss)
+    n = len(ss)
+    jobs = jobs_array(qidx, np.arange(n), np.zeros(n), [len(s) for s in ss], lo, hi)
+    got = O.banded_many_avx2(qres, qoff, sres, soff, jobs, mat, *gaps)
+    want = O.banded_many(qres, qoff, sres, soff, jobs, mat, *gaps)[0]
+    assert np.array_equal(got, want)
+
+
+def test_overflow_and_int16_decline():
+    W = 17
+    a = np.full(4000, W, np.uint8)
+    w300 = np.full(300, W, np.uint8)
+    qres, qoff = flat([w300, a])
+    sres, soff = flat([w300, np.array([12, 8, 17, 10, 0, 0], np.uint8), a, a])
+    jobs = jobs_array([0, 0, 1, 1], [0, 1, 2, 3], [0] * 4, [300, 6, 4000, 4000],
+                      [-5, -3, -20, -4000], [5, 3, 20, 4000])
+    got = O.banded_many_avx2(qres, qoff, sres, soff, jobs, BLOSUM62, 11, 1)
+    want = O.banded_many(qres, qoff, sres, soff, jobs, BLOSUM62, 11, 1)[0]
+    assert got.tolist() == want.tolist()
+    assert got[0] == 3300 and got[2] == 44000 and got[3] == 44000
