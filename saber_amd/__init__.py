@@ -166,6 +166,46 @@ class BandBatch:
             pass
 
 
+class XdropBatch:
+    """Device-resident X-drop batch (saber_hip_xdrop_batch_*)."""
+
+    def __init__(self, ctx, handle, nhits, keep):
+        self.ctx, self.h, self.nhits, self._keep = ctx, handle, nhits, keep
+
+    def run(self):
+        _ffi.check(_ffi.lib().saber_hip_xdrop_batch_run(self.h))
+
+    def read(self):
+        n = self.nhits
+        score = np.zeros(n, np.int32)
+        lo = np.zeros(n, np.uint32)
+        hi = np.zeros(n, np.uint32)
+        steps = np.zeros(n, np.uint32)
+        _ffi.check(_ffi.lib().saber_hip_xdrop_batch_read(
+            self.h, _ffi.ptr(score, C.c_int32), _ffi.ptr(lo, C.c_uint32), _ffi.ptr(hi, C.c_uint32),
+            _ffi.ptr(steps, C.c_uint32)))
+        return score, lo, hi, steps
+
+    def time(self, warmup=1, iters=5):
+        """(ms per run, steps per run)."""
+        ms = C.c_double()
+        st = C.c_uint64()
+        _ffi.check(_ffi.lib().saber_hip_xdrop_batch_time(self.h, warmup, iters, C.byref(ms),
+                                                           C.byref(st)))
+        return ms.value, st.value
+
+    def close(self):
+        if self.h:
+            _ffi.lib().saber_hip_xdrop_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def band_variants():
     """[(L, C, K, bias)] for every compiled packed-kernel variant."""
     out = []
@@ -250,6 +290,44 @@ class Context:
     def score_bands(self, qres, qoff, sres, soff, jobs, scheme, qbias=None):
         """(score, q_end, s_end, cells) arrays, one entry per job."""
         b = self.band_batch(qres, qoff, sres, soff, jobs, scheme, qbias)
+        try:
+            b.run()
+            return b.read()
+        finally:
+            b.close()
+
+    @staticmethod
+    def _seqsets(qres, qoff, qbias, sres, soff):
+        qres = np.ascontiguousarray(qres, np.uint8)
+        qoff = np.ascontiguousarray(qoff, np.uint64)
+        qb = None if qbias is None else np.ascontiguousarray(qbias, np.int8)
+        qs = _ffi.SeqSet(_ffi.ptr(qres, C.c_uint8), _ffi.ptr(qoff, C.c_uint64), len(qoff) - 1,
+                         _ffi.ptr(qb, C.c_int8))
+        keep = [qres, qoff, qb]
+        ss = None
+        if sres is not None:
+            sres = np.ascontiguousarray(sres, np.uint8)
+            soff = np.ascontiguousarray(soff, np.uint64)
+            ss = _ffi.SeqSet(_ffi.ptr(sres, C.c_uint8), _ffi.ptr(soff, C.c_uint64), len(soff) - 1, None)
+            keep += [sres, soff]
+        return qs, ss, keep
+
+    def xdrop_batch(self, qres, qoff, sres, soff, hits, scheme, x, qbias=None):
+        """Device-resident X-drop batch; hits is an (n, 4) uint32 array of
+        (query, subject, qp, sp)."""
+        qs, ss, keep = self._seqsets(qres, qoff, qbias, sres, soff)
+        hits = np.ascontiguousarray(hits, np.uint32).reshape(-1, 4)
+        keep.append(hits)
+        sch = scheme.ffi()
+        h = C.c_void_p()
+        _ffi.check(_ffi.lib().saber_hip_xdrop_batch_create(
+            self.h, C.byref(qs), None if ss is None else C.byref(ss), _ffi.ptr(hits, C.c_uint32),
+            len(hits), C.byref(sch), int(x), C.byref(h)))
+        return XdropBatch(self, h, len(hits), keep)
+
+    def xdrop(self, qres, qoff, sres, soff, hits, scheme, x, qbias=None):
+        """(score, q_lo, q_hi, steps) per hit (xdrop_cat, scan.rs:859-904)."""
+        b = self.xdrop_batch(qres, qoff, sres, soff, hits, scheme, x, qbias)
         try:
             b.run()
             return b.read()

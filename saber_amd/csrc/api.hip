@@ -85,6 +85,9 @@ struct saber_hip_ctx {
     int device = 0;
     uint32_t flags = 0;
     hipStream_t stream = nullptr;
+    hipStream_t work[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t fork = nullptr;
+    hipEvent_t join[4] = {nullptr, nullptr, nullptr, nullptr};
     std::mutex mu;
     DevBuf<uint8_t> db;
     std::vector<uint64_t> db_off;
@@ -92,10 +95,17 @@ struct saber_hip_ctx {
     std::vector<Shape> shapes;
 };
 
+// Band kernel launches of one run are spread over this many streams (all at
+// the default priority) so small width buckets and the tails of large ones
+// overlap.
+constexpr int kStreams = 4;
+
 struct VarLaunch {
     int variant;
     uint64_t pair_off;
     uint32_t npairs;
+    double cost;   // estimated lane-steps
+    int stream;    // index into the context's work streams
 };
 
 struct saber_hip_band_batch {
@@ -122,7 +132,8 @@ struct saber_hip_band_batch {
     DevBuf<int32_t> score;
     DevBuf<uint32_t> q_end, s_end;
     DevBuf<uint64_t> cells;
-    const uint8_t* sres = nullptr;  // own copy or the context database
+    const uint8_t* sres = nullptr;  // own copy or the context database (data start)
+    uint64_t qlen_total = 0, slen_total = 0;
     bool ran = false;
     uint32_t last_flagged = 0;
 };
@@ -177,6 +188,18 @@ int upload(DevBuf<T>& d, const T* h, uint64_t n, hipStream_t s) {
     return SABER_HIP_OK;
 }
 
+// Residue and bias buffers carry guard bytes so kernels may read a few bytes
+// before and after any sequence (clamped into [-16, len + 64)).
+constexpr uint64_t kGuard = 16, kGuardTail = 64;
+
+template <class T>
+int upload_guarded(DevBuf<T>& d, const T* h, uint64_t n, hipStream_t s) {
+    HIP_TRY(d.alloc(n + kGuard + kGuardTail));
+    HIP_TRY(hipMemsetAsync(d.p, 0, sizeof(T) * (n + kGuard + kGuardTail), s));
+    if (n) HIP_TRY(hipMemcpyAsync(d.p + kGuard, h, sizeof(T) * n, hipMemcpyHostToDevice, s));
+    return SABER_HIP_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -206,9 +229,12 @@ int saber_hip_create(int device, const saber_hip_opts* opts, saber_hip_ctx** out
     c->device = device;
     c->flags = opts ? opts->flags : 0;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    for (int k = 0; k < kStreams && e == hipSuccess; ++k) e = hipStreamCreateWithFlags(&c->work[k], hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->fork, hipEventDisableTiming);
+    for (int k = 0; k < kStreams && e == hipSuccess; ++k) e = hipEventCreateWithFlags(&c->join[k], hipEventDisableTiming);
     if (e != hipSuccess) {
-        delete c;
-        return fail(SABER_HIP_EDEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+        saber_hip_destroy(c);
+        return fail(SABER_HIP_EDEVICE, std::string("stream/event creation: ") + hipGetErrorString(e));
     }
     default_choice(c);
     *out = c;
@@ -218,9 +244,16 @@ int saber_hip_create(int device, const saber_hip_opts* opts, saber_hip_ctx** out
 void saber_hip_destroy(saber_hip_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
-    (void)hipStreamSynchronize(c->stream);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (int k = 0; k < kStreams; ++k)
+        if (c->work[k]) (void)hipStreamSynchronize(c->work[k]);
     c->db.release();
-    (void)hipStreamDestroy(c->stream);
+    for (int k = 0; k < kStreams; ++k) {
+        if (c->work[k]) (void)hipStreamDestroy(c->work[k]);
+        if (c->join[k]) (void)hipEventDestroy(c->join[k]);
+    }
+    if (c->fork) (void)hipEventDestroy(c->fork);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
 
@@ -230,7 +263,7 @@ int saber_hip_upload_db(saber_hip_ctx* c, const saber_hip_seqset* subjects) {
     std::lock_guard<std::mutex> g(c->mu);
     HIP_TRY(hipSetDevice(c->device));
     const uint64_t nres = subjects->count ? subjects->offsets[subjects->count] : 0;
-    if (int r = upload(c->db, subjects->residues, nres, c->stream)) return r;
+    if (int r = upload_guarded(c->db, subjects->residues, nres, c->stream)) return r;
     HIP_TRY(hipStreamSynchronize(c->stream));
     c->db_off.assign(subjects->offsets, subjects->offsets + subjects->count + 1);
     return SABER_HIP_OK;
@@ -346,12 +379,27 @@ int saber_hip_band_batch_create(saber_hip_ctx* c, const saber_hip_seqset* querie
         const Shape& s = c->shapes[sh];
         const int var = find_variant(s.L, s.C, s.K, b->has_bias ? 1 : 0);
         if (var < 0) return fail(SABER_HIP_EINVAL, "variant table inconsistent");
-        VarLaunch L{var, pairs.size() / 2, (uint32_t)((v.size() + 1) / 2)};
+        VarLaunch L{var, pairs.size() / 2, (uint32_t)((v.size() + 1) / 2), 0.0, 0};
         for (size_t i = 0; i < v.size(); i += 2) {
             pairs.push_back(v[i]);
             pairs.push_back(i + 1 < v.size() ? v[i + 1] : kNone);
+            // lane-steps of the pair: (longer row count + lag) x band width
+            L.cost += (double)(rows_of[v[i]] + s.L - 1) * s.W();
         }
         b->launches.push_back(L);
+    }
+    // Longest-processing-time assignment of the launches to the streams.
+    std::sort(b->launches.begin(), b->launches.end(),
+              [](const VarLaunch& x, const VarLaunch& y) { return x.cost > y.cost; });
+    {
+        double load[kStreams] = {0};
+        for (VarLaunch& L : b->launches) {
+            int best = 0;
+            for (int k = 1; k < kStreams; ++k)
+                if (load[k] < load[best]) best = k;
+            L.stream = best;
+            load[best] += L.cost;
+        }
     }
 
     // Score table for the packed kernel: T[a][c] = S(code_index(a), code_index(c)),
@@ -367,18 +415,21 @@ int saber_hip_band_batch_create(saber_hip_ctx* c, const saber_hip_seqset* querie
 
     hipStream_t st = c->stream;
     const uint64_t nq = queries->count ? queries->offsets[queries->count] : 0;
-    if (int r = upload(b->q, queries->residues, nq, st)) return r;
+    if (int r = upload_guarded(b->q, queries->residues, nq, st)) return r;
     if (b->has_bias) {
-        if (int r = upload(b->bias, queries->bias, nq, st)) return r;
+        if (int r = upload_guarded(b->bias, queries->bias, nq, st)) return r;
     }
     if (subjects) {
         const uint64_t ns = subjects->count ? subjects->offsets[subjects->count] : 0;
-        if (int r = upload(b->s, subjects->residues, ns, st)) return r;
-        b->sres = b->s.p;
+        if (int r = upload_guarded(b->s, subjects->residues, ns, st)) return r;
+        b->sres = b->s.p + kGuard;
+        b->slen_total = ns;
     } else {
         if (!c->db.p && njobs > 0) return fail(SABER_HIP_ESTATE, "no database uploaded");
-        b->sres = c->db.p;
+        b->sres = c->db.p + kGuard;
+        b->slen_total = c->db_off.empty() ? 0 : c->db_off.back();
     }
+    b->qlen_total = nq;
     if (int r = upload(b->table, table.data(), table.size(), st)) return r;
     if (int r = upload(b->mat, scheme->matrix, 576, st)) return r;
     if (int r = upload(b->jobs, b->host_jobs.data(), njobs, st)) return r;
@@ -410,8 +461,8 @@ static int run_generic(saber_hip_band_batch* b, const std::vector<uint32_t>& ids
     a.ids = b->gen_ids.p;
     a.nids = (uint32_t)ids.size();
     a.jobs = b->jobs.p;
-    a.qres = b->q.p;
-    a.qbias = b->has_bias ? b->bias.p : nullptr;
+    a.qres = b->q.p + kGuard;
+    a.qbias = b->has_bias ? b->bias.p + kGuard : nullptr;
     a.sres = b->sres;
     a.mat = b->mat.p;
     a.go = b->go;
@@ -434,26 +485,38 @@ static int enqueue_main(saber_hip_band_batch* b, hipEvent_t* ev_variant_begin, h
     HIP_TRY(fill_u32(b->s_end.p, n, 0, st));
     HIP_TRY(fill_u32(reinterpret_cast<uint32_t*>(b->cells.p), 2 * n, 0, st));
     HIP_TRY(fill_u32(b->nflagged.p, 1, 0, st));
+    // Fork: the work streams start after the clears on the main stream.
+    saber_hip_ctx* c = b->ctx;
+    HIP_TRY(hipEventRecord(c->fork, st));
+    for (int k = 0; k < kStreams; ++k) HIP_TRY(hipStreamWaitEvent(c->work[k], c->fork, 0));
     int nv = 0;
     const BandVariant* var = band_variants(&nv);
     for (size_t i = 0; i < b->launches.size(); ++i) {
         const VarLaunch& L = b->launches[i];
+        hipStream_t ws = c->work[L.stream];
         BandKernelArgs a{};
         a.pairs = b->pairs.p + 2 * L.pair_off;
         a.npairs = L.npairs;
         a.jobs = b->jobs.p;
-        a.qres = b->q.p;
-        a.qbias = b->has_bias ? b->bias.p : nullptr;
+        a.qres = b->q.p + kGuard;
+        a.qbias = b->has_bias ? b->bias.p + kGuard : nullptr;
         a.sres = b->sres;
+        a.qres_len = b->qlen_total;
+        a.sres_len = b->slen_total;
         a.table = b->table.p;
         a.go = b->go;
         a.ge = b->ge;
         a.out = BandOutputs{b->score.p, b->q_end.p, b->s_end.p, b->cells.p};
         a.flagged = b->flagged.p;
         a.nflagged = b->nflagged.p;
-        if (ev_variant_begin) HIP_TRY(hipEventRecord(ev_variant_begin[i], st));
-        HIP_TRY(var[L.variant].launch(a, st));
-        if (ev_variant_end) HIP_TRY(hipEventRecord(ev_variant_end[i], st));
+        if (ev_variant_begin) HIP_TRY(hipEventRecord(ev_variant_begin[i], ws));
+        HIP_TRY(var[L.variant].launch(a, ws));
+        if (ev_variant_end) HIP_TRY(hipEventRecord(ev_variant_end[i], ws));
+    }
+    // Join: the main stream continues after every work stream.
+    for (int k = 0; k < kStreams; ++k) {
+        HIP_TRY(hipEventRecord(c->join[k], c->work[k]));
+        HIP_TRY(hipStreamWaitEvent(st, c->join[k], 0));
     }
     if (int r = run_generic(b, b->generic_static)) return r;
     return SABER_HIP_OK;
@@ -658,8 +721,8 @@ int xdrop_enqueue(saber_hip_xdrop_batch* b) {
     a.hits = b->hits.p;
     a.nhits = (uint32_t)b->nhits;
     a.qoff = b->qoff.p;
-    a.qres = b->q.p;
-    a.qbias = b->has_bias ? b->bias.p : nullptr;
+    a.qres = b->q.p + kGuard;
+    a.qbias = b->has_bias ? b->bias.p + kGuard : nullptr;
     a.soff = b->soffp;
     a.sres = b->sres;
     a.table = b->table.p;
@@ -714,20 +777,20 @@ int saber_hip_xdrop_batch_create(saber_hip_ctx* c, const saber_hip_seqset* queri
                 (short)scheme->matrix[(a < 24 ? a : kCodeX) * 24 + (cc < 24 ? cc : kCodeX)];
     hipStream_t st = c->stream;
     const uint64_t nq = queries->count ? queries->offsets[queries->count] : 0;
-    if (int r = upload(b->q, queries->residues, nq, st)) return r;
+    if (int r = upload_guarded(b->q, queries->residues, nq, st)) return r;
     if (b->has_bias)
-        if (int r = upload(b->bias, queries->bias, nq, st)) return r;
+        if (int r = upload_guarded(b->bias, queries->bias, nq, st)) return r;
     if (int r = upload(b->qoff, queries->offsets, (uint64_t)queries->count + 1, st)) return r;
     if (subjects) {
         const uint64_t ns = subjects->count ? subjects->offsets[subjects->count] : 0;
-        if (int r = upload(b->s, subjects->residues, ns, st)) return r;
+        if (int r = upload_guarded(b->s, subjects->residues, ns, st)) return r;
         if (int r = upload(b->soff, subjects->offsets, (uint64_t)subjects->count + 1, st)) return r;
-        b->sres = b->s.p;
+        b->sres = b->s.p + kGuard;
         b->soffp = b->soff.p;
     } else {
         if (!c->db.p && nhits > 0) return fail(SABER_HIP_ESTATE, "no database uploaded");
         if (int r = upload(b->soff, c->db_off.data(), (uint64_t)c->db_off.size(), st)) return r;
-        b->sres = c->db.p;
+        b->sres = c->db.p + kGuard;
         b->soffp = b->soff.p;
     }
     if (int r = upload(b->hits, reinterpret_cast<const uint32_t*>(hits), 4 * nhits, st)) return r;

@@ -161,17 +161,40 @@ __global__ __launch_bounds__(256) void band_pair_kernel(BandKernelArgs a) {
         for (int c = 0; c < C; ++c) Wd[h][c >> 2] |= subject_code(raw[c], j0 + c, n[h]) << (8 * (c & 3));
     }
 
-    auto qrow = [&](int h, int r) -> uint32_t {
-        int i = ifirst[h] + r;
-        if (i > (int)m[h] - 1) i = (int)m[h] - 1;
-        uint32_t c = qp[h][i];
-        return (c < 24 ? c : (uint32_t)kCodeX) * 64u;
+    // Per-lane byte streams for the row this lane prepares next (row k = r + 1
+    // at step r): the subject column entering the window, the query code and
+    // the bias. Each is a dword FIFO (cur, n1, n2) refilled every 4 rows with
+    // an unaligned load issued 8 rows ahead, so row k's bytes were requested
+    // two refills earlier. Device buffers carry 16 guard bytes in front and 64
+    // behind, and every load address is clamped into them; bytes from outside
+    // a sequence are replaced (sentinel column / garbage rows never used).
+    auto load4 = [](const uint8_t* buf, int64_t off, uint64_t len) -> uint32_t {
+        const int64_t top = (int64_t)len + 60;
+        off = off < -16 ? -16 : (off > top ? top : off);
+        uint32_t v;
+        __builtin_memcpy(&v, buf + off, 4);
+        return v;
     };
-    auto brow = [&](int h, int r) -> short {
-        int i = ifirst[h] + r;
-        if (i > (int)m[h] - 1) i = (int)m[h] - 1;
-        return (short)bp[h][i];
-    };
+    const int64_t jcol0[2] = {(int64_t)ifirst[0] + lo[0] + l * C + (C - 1),
+                              (int64_t)ifirst[1] + lo[1] + l * C + (C - 1)};
+    uint32_t s_cur[2], s_n1[2], s_n2[2], q_cur[2], q_n1[2], q_n2[2];
+    uint32_t b_cur[2] = {0, 0}, b_n1[2] = {0, 0}, b_n2[2] = {0, 0};
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int64_t sj = soff[h] + jcol0[h], qi = qoff[h] + ifirst[h];
+        s_cur[h] = load4(a.sres, sj + 1, a.sres_len);
+        s_n1[h] = load4(a.sres, sj + 5, a.sres_len);
+        s_n2[h] = load4(a.sres, sj + 9, a.sres_len);
+        q_cur[h] = load4(a.qres, qi + 1, a.qres_len);
+        q_n1[h] = load4(a.qres, qi + 5, a.qres_len);
+        q_n2[h] = load4(a.qres, qi + 9, a.qres_len);
+        if constexpr (BIAS) {
+            const uint8_t* bb = reinterpret_cast<const uint8_t*>(a.qbias);
+            b_cur[h] = load4(bb, qi + 1, a.qres_len);
+            b_n1[h] = load4(bb, qi + 5, a.qres_len);
+            b_n2[h] = load4(bb, qi + 9, a.qres_len);
+        }
+    }
 
     int cbest[2] = {0, 0};
     int csnap_row[2] = {-1, -1};
@@ -179,11 +202,18 @@ __global__ __launch_bounds__(256) void band_pair_kernel(BandKernelArgs a) {
     uint64_t cells[2] = {0, 0};
     uint32_t outHl = bits(NEG), outE = bits(NEG);
 
-    // Row data of the row this lane processes next.
-    int r0 = -l;
-    uint32_t rb[2] = {qrow(0, r0 < 0 ? 0 : r0), qrow(1, r0 < 0 ? 0 : r0)};
+    // Row data of row 0 of this lane.
+    uint32_t rb[2];
     v2s bpair = ZERO;
-    if constexpr (BIAS) bpair = v2s{brow(0, r0 < 0 ? 0 : r0), brow(1, r0 < 0 ? 0 : r0)};
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const uint32_t c = qp[h][ifirst[h] < (int)m[h] ? ifirst[h] : 0];
+        rb[h] = (c < 24 ? c : (uint32_t)kCodeX) * 64u;
+    }
+    if constexpr (BIAS) {
+        bpair = v2s{(short)bp[0][ifirst[0] < (int)m[0] ? ifirst[0] : 0],
+                    (short)bp[1][ifirst[1] < (int)m[1] ? ifirst[1] : 0]};
+    }
 
     const int nsteps = R + L - 1;
     for (int t = 0; t < nsteps; ++t) {
@@ -199,22 +229,6 @@ __global__ __launch_bounds__(256) void band_pair_kernel(BandKernelArgs a) {
             }
         }
         v2s Hl = NEG, E = NEG, rm = ZERO;
-        // Loads for the next row (query codes, bias, the new window column).
-        const int rn = r + 1;
-        uint32_t nraw[2] = {0, 0};
-        int64_t nj[2] = {0, 0};
-        uint32_t rbn[2] = {rb[0], rb[1]};
-        v2s bpn = bpair;
-        if (act) {
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                nj[h] = (int64_t)ifirst[h] + rn + lo[h] + l * C + (C - 1);
-                nraw[h] = subject_raw(sp[h], nj[h], n[h]);
-                rbn[h] = qrow(h, rn);
-            }
-            if constexpr (BIAS) bpn = v2s{brow(0, rn), brow(1, rn)};
-        }
-
         auto cell = [&](int c, v2s hup, v2s fup, v2s el, v2s hl, bool first) {
             const uint32_t ba = (Wd[0][c >> 2] >> (8 * (c & 3))) & 0xFFu;
             const uint32_t bb = (Wd[1][c >> 2] >> (8 * (c & 3))) & 0xFFu;
@@ -277,18 +291,43 @@ __global__ __launch_bounds__(256) void band_pair_kernel(BandKernelArgs a) {
                     }
                 }
             }
-            // Shift the windows one column and append the new column.
+            // Next row's data from the streams; shift the windows one column
+            // and append the new column.
+            const int k = r + 1;
+            short bn[2] = {0, 0};
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
+                const uint32_t sraw = s_cur[h] & 0xFFu, qc = q_cur[h] & 0xFFu;
+                if constexpr (BIAS) bn[h] = (short)(int8_t)(b_cur[h] & 0xFFu);
+                s_cur[h] >>= 8;
+                q_cur[h] >>= 8;
+                if constexpr (BIAS) b_cur[h] >>= 8;
 #pragma unroll
                 for (int x = 0; x < NW - 1; ++x) Wd[h][x] = __builtin_amdgcn_alignbyte(Wd[h][x + 1], Wd[h][x], 1);
                 constexpr int b = (C - 1) & 3;
                 uint32_t last = Wd[h][NW - 1] >> 8;
                 if constexpr (b > 0) last &= (1u << (8 * b)) - 1u;
-                Wd[h][NW - 1] = last | (subject_code(nraw[h], nj[h], n[h]) << (8 * b));
-                rb[h] = rbn[h];
+                Wd[h][NW - 1] = last | (subject_code(sraw, jcol0[h] + k, n[h]) << (8 * b));
+                rb[h] = (qc < 24 ? qc : (uint32_t)kCodeX) * 64u;
             }
-            bpair = bpn;
+            if constexpr (BIAS) bpair = v2s{bn[0], bn[1]};
+            if ((k & 3) == 0) {
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    s_cur[h] = s_n1[h];
+                    s_n1[h] = s_n2[h];
+                    s_n2[h] = load4(a.sres, soff[h] + jcol0[h] + k + 9, a.sres_len);
+                    q_cur[h] = q_n1[h];
+                    q_n1[h] = q_n2[h];
+                    q_n2[h] = load4(a.qres, qoff[h] + ifirst[h] + k + 9, a.qres_len);
+                    if constexpr (BIAS) {
+                        b_cur[h] = b_n1[h];
+                        b_n1[h] = b_n2[h];
+                        b_n2[h] = load4(reinterpret_cast<const uint8_t*>(a.qbias),
+                                        qoff[h] + ifirst[h] + k + 9, a.qres_len);
+                    }
+                }
+            }
         }
     }
 

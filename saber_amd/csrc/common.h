@@ -38,9 +38,10 @@ struct BandKernelArgs {
     const uint32_t* pairs;      // 2 job ids per pair (0xFFFFFFFF = none)
     uint32_t npairs;
     const BandJobDev* jobs;
-    const uint8_t* qres;
-    const int8_t* qbias;        // may be null when the kernel is built without bias
+    const uint8_t* qres;        // device buffers with 16 guard bytes before and
+    const int8_t* qbias;        // 64 after (qbias may be null without bias)
     const uint8_t* sres;
+    uint64_t qres_len, sres_len;
     const short* table;         // 32 x 32 int16 (global; staged into LDS)
     int32_t go, ge;
     BandOutputs out;

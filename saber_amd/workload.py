@@ -70,3 +70,22 @@ def sequences(seed, n, median, sigma, lmin, lmax, matrix=BLOSUM62):
     finally:
         W.wl_free(h)
     return res, off
+
+
+def xdrop_hits(regions, seed, per_region=2):
+    """Seed hits over a band_regions batch: for each region, per_region hits
+    (query, subject, qp, sp); half on the region's centre diagonal (where
+    planted homologues extend far), half uniformly inside the rectangle."""
+    rng = np.random.default_rng(seed)
+    jobs = regions["jobs"]
+    n = len(jobs)
+    q = np.repeat(jobs["query"].astype(np.int64), per_region)
+    s = np.repeat(jobs["subject"].astype(np.int64), per_region)
+    m = (regions["qoff"][q + 1] - regions["qoff"][q]).astype(np.int64)
+    sl = (regions["soff"][s + 1] - regions["soff"][s]).astype(np.int64)
+    c = np.repeat(((jobs["band_lo"].astype(np.int64) + jobs["band_hi"]) // 2), per_region)
+    qp = (rng.random(n * per_region) * m).astype(np.int64)
+    on_diag = (np.arange(n * per_region) % 2) == 0
+    sp = np.where(on_diag, qp + c, (rng.random(n * per_region) * sl).astype(np.int64))
+    sp = np.clip(sp, 0, sl - 1)
+    return np.stack([q, s, qp, sp], axis=1).astype(np.uint32)
