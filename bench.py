@@ -50,7 +50,20 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--xdrop-hits", type=int, default=4_000_000, help="seed hits for the X-drop leg")
     ap.add_argument("--check", type=int, default=3000, help="jobs checked against the oracle")
+    ap.add_argument("--full-queries", type=int, default=20,
+                    help="queries of the exhaustive-mode leg (x 100,000 subjects); 0 skips it")
+    ap.add_argument("--full-subjects", type=int, default=100_000)
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only time a plain device copy of this many bytes and exit")
     args = ap.parse_args()
+
+    if args.copy_probe:
+        import saber_amd as S
+        ctx = S.Context(0)
+        ms = ctx.copy_probe(args.copy_probe, iters=5)
+        print(json.dumps({"copy_bytes": args.copy_probe, "ms": ms,
+                          "GBps": round(2 * args.copy_probe / (ms * 1e-3) / 1e9, 1)}))
+        return 0
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported on this "
@@ -124,6 +137,27 @@ def main():
         print(f"# xdrop {len(hits)} hits: {xms:.3f} ms, {xsteps} steps, "
               f"{xd['gsteps_per_s']} G steps/s, mismatches {xbad}/{len(xi)}", file=sys.stderr, flush=True)
 
+    # Secondary: exhaustive full-matrix SW, queries x subjects drawn like the
+    # exhaustive workload (lognormal median 250, sigma 0.6, clamped 30..3000).
+    fl = None
+    if args.full_queries > 0:
+        fq, fqo = workload.sequences(args.seed + 100, args.full_queries, 250, 0.6, 30, 3000)
+        fs, fso = workload.sequences(args.seed + 200, args.full_subjects, 250, 0.6, 30, 3000)
+        fb = ctx.full_batch(fq, fqo, fs, fso, sch)
+        fms, fcells, fresc = fb.time(warmup=1, iters=2)
+        fsc, fqe, fse = fb.read()
+        fb.close()
+        fbad = 0
+        for k in range(40):
+            a, b = int(rng.integers(args.full_queries)), int(rng.integers(args.full_subjects))
+            w = O.sw_reference(fq[fqo[a]:fqo[a + 1]], fs[fso[b]:fso[b + 1]], BLOSUM62, None, 11, 1)
+            fbad += int((int(fsc[a, b]), int(fqe[a, b]), int(fse[a, b])) != w)
+        fl = {"queries": args.full_queries, "subjects": args.full_subjects, "ms": round(fms, 3),
+              "cells": int(fcells), "gcups": round(fcells / (fms * 1e-3) / 1e9, 1),
+              "int32_rescored_pairs": int(fresc), "parity_sample": 40, "parity_mismatches": fbad}
+        print(f"# full SW {args.full_queries} x {args.full_subjects}: {fms:.2f} ms, "
+              f"{fl['gcups']} GCUPS, mismatches {fbad}/40", file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu:
         nthreads = os.cpu_count() or 1
@@ -190,12 +224,13 @@ def main():
             "traffic": None,
         },
         "cpu_baseline": cpu,
-        "extra": {"xdrop": xd},
+        "extra": {"xdrop": xd, "full_sw": fl},
     }
     print(json.dumps(out), flush=True)
     batch.close()
     ctx.close()
-    return 0 if mismatches == 0 and (xd is None or xd["parity_mismatches"] == 0) else 1
+    ok = mismatches == 0 and (xd is None or xd["parity_mismatches"] == 0) and (fl is None or fl["parity_mismatches"] == 0)
+    return 0 if ok else 1
 
 
 if __name__ == "__main__":

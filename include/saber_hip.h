@@ -142,6 +142,29 @@ int saber_hip_band_batch_time(saber_hip_band_batch* b, int warmup, int iters, do
                               double* kernel_ms, uint64_t* kernel_cells, int* nvar,
                               uint64_t* cells);
 
+/* Exhaustive mode (score_many over whole subjects, src/algorithm/sw_simd.rs:95-153,
+ * with sw_reference's end cell, src/algorithm/reference.rs:15-52; the
+ * exhaustive preset, src/search/presets.rs:318-331): every query against
+ * every subject. Outputs are row-major [query][subject]: score, exclusive
+ * ends of the first row-major optimal cell (0, 0 when the score is 0).
+ * subjects == NULL selects the uploaded database. q_end / s_end may be NULL. */
+typedef struct saber_hip_full_batch saber_hip_full_batch;
+int saber_hip_score_full(saber_hip_ctx* ctx, const saber_hip_seqset* queries,
+                         const saber_hip_seqset* subjects, const saber_hip_scheme* scheme,
+                         int32_t* score, uint32_t* q_end, uint32_t* s_end);
+int saber_hip_full_batch_create(saber_hip_ctx* ctx, const saber_hip_seqset* queries,
+                                const saber_hip_seqset* subjects, const saber_hip_scheme* scheme,
+                                saber_hip_full_batch** out);
+int saber_hip_full_batch_run(saber_hip_full_batch* b);
+int saber_hip_full_batch_read(saber_hip_full_batch* b, int32_t* score, uint32_t* q_end,
+                              uint32_t* s_end);
+/* Mean ms per run (HIP events on the context stream), cells per run
+ * (sum of query lengths x sum of subject lengths) and the pairs the last run
+ * scored with the exact int32 kernel. */
+int saber_hip_full_batch_time(saber_hip_full_batch* b, int warmup, int iters, double* ms_run,
+                              uint64_t* cells, uint64_t* rescued);
+void saber_hip_full_batch_destroy(saber_hip_full_batch* b);
+
 /* One seed hit for X-drop extension: query position qp, subject position sp
  * (qp < query length, sp < subject length), as visit_scan passes them to
  * xdrop_cat (src/search/engine/scan.rs:2414-2420). */
@@ -182,6 +205,11 @@ void saber_hip_xdrop_batch_destroy(saber_hip_xdrop_batch* b);
  * L*C-K+1 .. L*C; bias = 1 for the build that adds a query bias.
  * Returns SABER_HIP_EINVAL when v is out of range. */
 int saber_hip_band_variant(int v, int* L, int* C, int* K, int* bias);
+
+/* Diagnostic (no counterpart in the reference): time a plain device-to-device
+ * copy of `bytes` (16-byte vector loads/stores) iters times; ms = mean per
+ * copy. Used to calibrate traffic counters and report copy bandwidth. */
+int saber_hip_copy_probe(saber_hip_ctx* ctx, uint64_t bytes, int iters, double* ms);
 
 /* Width -> variant choice used by the planner (the autotuner's output):
  * widths lo..hi (inclusive) use variant shape (L, C, K); L = 0 routes them to

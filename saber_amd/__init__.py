@@ -166,6 +166,45 @@ class BandBatch:
             pass
 
 
+class FullBatch:
+    """Device-resident exhaustive batch (saber_hip_full_batch_*)."""
+
+    def __init__(self, ctx, handle, nq, ns, keep):
+        self.ctx, self.h, self.nq, self.ns, self._keep = ctx, handle, nq, ns, keep
+
+    def run(self):
+        _ffi.check(_ffi.lib().saber_hip_full_batch_run(self.h))
+
+    def read(self):
+        shape = (self.nq, self.ns)
+        score = np.zeros(shape, np.int32)
+        qe = np.zeros(shape, np.uint32)
+        se = np.zeros(shape, np.uint32)
+        _ffi.check(_ffi.lib().saber_hip_full_batch_read(
+            self.h, _ffi.ptr(score, C.c_int32), _ffi.ptr(qe, C.c_uint32), _ffi.ptr(se, C.c_uint32)))
+        return score, qe, se
+
+    def time(self, warmup=1, iters=3):
+        """(ms per run, cells per run, pairs rescored in int32)."""
+        ms = C.c_double()
+        cells = C.c_uint64()
+        resc = C.c_uint64()
+        _ffi.check(_ffi.lib().saber_hip_full_batch_time(self.h, warmup, iters, C.byref(ms),
+                                                          C.byref(cells), C.byref(resc)))
+        return ms.value, cells.value, resc.value
+
+    def close(self):
+        if self.h:
+            _ffi.lib().saber_hip_full_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class XdropBatch:
     """Device-resident X-drop batch (saber_hip_xdrop_batch_*)."""
 
@@ -238,6 +277,12 @@ class Context:
             self.close()
         except Exception:
             pass
+
+    def copy_probe(self, nbytes, iters=5):
+        """Mean ms of a plain device copy of nbytes (read nbytes + write nbytes)."""
+        ms = C.c_double()
+        _ffi.check(_ffi.lib().saber_hip_copy_probe(self.h, int(nbytes), int(iters), C.byref(ms)))
+        return ms.value
 
     def choose(self, w_lo, w_hi, L, Cc, K):
         _ffi.check(_ffi.lib().saber_hip_band_choose(self.h, w_lo, w_hi, L, Cc, K))
@@ -324,6 +369,26 @@ class Context:
             self.h, C.byref(qs), None if ss is None else C.byref(ss), _ffi.ptr(hits, C.c_uint32),
             len(hits), C.byref(sch), int(x), C.byref(h)))
         return XdropBatch(self, h, len(hits), keep)
+
+    def full_batch(self, qres, qoff, sres, soff, scheme, qbias=None):
+        """Device-resident exhaustive batch: every query x every subject."""
+        qs, ss, keep = self._seqsets(qres, qoff, qbias, sres, soff)
+        sch = scheme.ffi()
+        h = C.c_void_p()
+        _ffi.check(_ffi.lib().saber_hip_full_batch_create(
+            self.h, C.byref(qs), None if ss is None else C.byref(ss), C.byref(sch), C.byref(h)))
+        nq = len(keep[1]) - 1
+        ns = (len(keep[4]) - 1) if ss is not None else None
+        return FullBatch(self, h, nq, ns, keep)
+
+    def score_full(self, qres, qoff, sres, soff, scheme, qbias=None):
+        """(score, q_end, s_end) arrays of shape [queries, subjects]."""
+        b = self.full_batch(qres, qoff, sres, soff, scheme, qbias)
+        try:
+            b.run()
+            return b.read()
+        finally:
+            b.close()
 
     def xdrop(self, qres, qoff, sres, soff, hits, scheme, x, qbias=None):
         """(score, q_lo, q_hi, steps) per hit (xdrop_cat, scan.rs:859-904)."""

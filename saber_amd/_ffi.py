@@ -68,6 +68,17 @@ _SIGS = {
     "saber_hip_band_batch_time": (C.c_int, [P, C.c_int, C.c_int, C.POINTER(C.c_double),
                                             C.POINTER(C.c_double), C.POINTER(C.c_uint64),
                                             C.POINTER(C.c_int), C.POINTER(C.c_uint64)]),
+    "saber_hip_score_full": (C.c_int, [P, C.POINTER(SeqSet), C.POINTER(SeqSet), C.POINTER(Scheme),
+                                       C.POINTER(C.c_int32), C.POINTER(C.c_uint32),
+                                       C.POINTER(C.c_uint32)]),
+    "saber_hip_full_batch_create": (C.c_int, [P, C.POINTER(SeqSet), C.POINTER(SeqSet),
+                                              C.POINTER(Scheme), C.POINTER(P)]),
+    "saber_hip_full_batch_run": (C.c_int, [P]),
+    "saber_hip_full_batch_read": (C.c_int, [P, C.POINTER(C.c_int32), C.POINTER(C.c_uint32),
+                                            C.POINTER(C.c_uint32)]),
+    "saber_hip_full_batch_time": (C.c_int, [P, C.c_int, C.c_int, C.POINTER(C.c_double),
+                                            C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "saber_hip_full_batch_destroy": (None, [P]),
     "saber_hip_xdrop": (C.c_int, [P, C.POINTER(SeqSet), C.POINTER(SeqSet), C.POINTER(C.c_uint32),
                                   C.c_uint64, C.POINTER(Scheme), C.c_int32, C.POINTER(C.c_int32),
                                   C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
@@ -83,6 +94,7 @@ _SIGS = {
     "saber_hip_band_variant": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                          C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "saber_hip_band_choose": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "saber_hip_copy_probe": (C.c_int, [P, C.c_uint64, C.c_int, C.POINTER(C.c_double)]),
 }
 
 _lib = None

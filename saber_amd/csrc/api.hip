@@ -673,8 +673,8 @@ int saber_hip_score_windows(saber_hip_ctx* c, const uint8_t* query, uint32_t qle
         return SABER_HIP_OK;
     }
     if (!query) return fail(SABER_HIP_EINVAL, "query is null");
-    // Concatenate the windows into one subject set; every job is the full
-    // band of its rectangle (Band::full, src/algorithm/banded.rs:38-40).
+    // Concatenate the windows into one subject set and score the whole
+    // rectangles with the exhaustive kernel (one query x n windows).
     std::vector<uint64_t> off(n + 1, 0);
     for (uint32_t k = 0; k < n; ++k) {
         if (lens[k] && !windows[k]) return fail(SABER_HIP_EINVAL, "window pointer is null");
@@ -686,10 +686,7 @@ int saber_hip_score_windows(saber_hip_ctx* c, const uint8_t* query, uint32_t qle
     uint64_t qoff[2] = {0, qlen};
     saber_hip_seqset qs{query, qoff, 1, bias};
     saber_hip_seqset ss{res.data(), off.data(), n, nullptr};
-    std::vector<saber_hip_band_job> jobs(n);
-    for (uint32_t k = 0; k < n; ++k)
-        jobs[k] = saber_hip_band_job{0, k, 0, lens[k], -(int32_t)qlen, (int32_t)lens[k]};
-    return saber_hip_score_bands(c, &qs, &ss, jobs.data(), n, scheme, out, nullptr, nullptr, nullptr);
+    return saber_hip_score_full(c, &qs, &ss, scheme, out, nullptr, nullptr);
 }
 
 }  // extern "C"
@@ -893,3 +890,432 @@ int saber_hip_xdrop(saber_hip_ctx* c, const saber_hip_seqset* queries,
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Exhaustive full-matrix scoring: every query against every subject.
+// ---------------------------------------------------------------------------
+
+struct FullLaunch {
+    int variant;
+    uint64_t pair_off;
+    uint32_t npairs;
+    double cost;
+    int stream;
+};
+
+struct saber_hip_full_batch {
+    saber_hip_ctx* ctx = nullptr;
+    uint32_t nq = 0, ns = 0;
+    bool has_bias = false;
+    int32_t go = 0, ge = 0;
+    DevBuf<uint8_t> q, s;
+    DevBuf<int8_t> bias;
+    DevBuf<uint64_t> qoff, soff;
+    DevBuf<uint32_t> queries, pairs;
+    DevBuf<short> table;
+    DevBuf<int32_t> mat;
+    std::vector<FullLaunch> launches;
+    std::vector<uint32_t> long_subjects;   // longer than the widest variant
+    std::vector<uint64_t> hq, hs;          // host offsets
+    uint64_t qlen_total = 0, slen_total = 0;
+    const uint8_t* sres = nullptr;
+    DevBuf<int32_t> score;
+    DevBuf<uint32_t> q_end, s_end;
+    DevBuf<uint32_t> flagged, nflagged;
+    // exact int32 rescue (long subjects, saturated pairs)
+    DevBuf<BandJobDev> gjobs;
+    DevBuf<uint32_t> gids;
+    DevBuf<uint64_t> goff, gdst;
+    DevBuf<int32_t> gwork, gscore;
+    DevBuf<uint32_t> gqe, gse;
+    DevBuf<uint64_t> gcells;
+    bool ran = false;
+    uint64_t rescued = 0;
+};
+
+namespace {
+
+__global__ void scatter_full_kernel(const uint64_t* dst, uint32_t n, const int32_t* sc,
+                                    const uint32_t* qe, const uint32_t* se, int32_t* score,
+                                    uint32_t* q_end, uint32_t* s_end) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t o = dst[i];
+    score[o] = sc[i];
+    q_end[o] = qe[i];
+    s_end[o] = se[i];
+}
+
+// Exact int32 scores of (query slot, subject) pairs, written into the outputs.
+int full_rescue(saber_hip_full_batch* b, const std::vector<std::pair<uint32_t, uint32_t>>& prs) {
+    if (prs.empty()) return SABER_HIP_OK;
+    hipStream_t st = b->ctx->stream;
+    const size_t n = prs.size();
+    std::vector<BandJobDev> jobs(n);
+    std::vector<uint32_t> ids(n);
+    std::vector<uint64_t> off(n), dst(n);
+    uint64_t tot = 0;
+    for (size_t k = 0; k < n; ++k) {
+        const uint32_t qi = prs[k].first, si = prs[k].second;
+        const int64_t m = (int64_t)(b->hq[qi + 1] - b->hq[qi]), len = (int64_t)(b->hs[si + 1] - b->hs[si]);
+        BandJobDev d{};
+        d.q_off = b->hq[qi];
+        d.s_off = b->hs[si];
+        d.m = (uint32_t)m;
+        d.n = (uint32_t)len;
+        d.lo = (int32_t)(m > 0 ? -(m - 1) : 0);
+        d.w = (int32_t)(m > 0 && len > 0 ? m + len - 1 : 0);
+        d.i_first = 0;
+        d.rows = (int32_t)(d.w > 0 ? m : 0);
+        jobs[k] = d;
+        ids[k] = (uint32_t)k;
+        off[k] = tot;
+        tot += 4 * ((uint64_t)d.w + 1);
+        dst[k] = (uint64_t)qi * b->ns + si;
+    }
+    if (int r = upload(b->gjobs, jobs.data(), n, st)) return r;
+    if (int r = upload(b->gids, ids.data(), n, st)) return r;
+    if (int r = upload(b->goff, off.data(), n, st)) return r;
+    if (int r = upload(b->gdst, dst.data(), n, st)) return r;
+    if (b->gwork.n < tot) HIP_TRY(b->gwork.alloc(tot));
+    if (b->gscore.n < n) {
+        HIP_TRY(b->gscore.alloc(n));
+        HIP_TRY(b->gqe.alloc(n));
+        HIP_TRY(b->gse.alloc(n));
+        HIP_TRY(b->gcells.alloc(n));
+    }
+    GenericArgs a{};
+    a.ids = b->gids.p;
+    a.nids = (uint32_t)n;
+    a.jobs = b->gjobs.p;
+    a.qres = b->q.p + kGuard;
+    a.qbias = b->has_bias ? b->bias.p + kGuard : nullptr;
+    a.sres = b->sres;
+    a.mat = b->mat.p;
+    a.go = b->go;
+    a.ge = b->ge;
+    a.out = BandOutputs{b->gscore.p, b->gqe.p, b->gse.p, b->gcells.p};
+    a.work = b->gwork.p;
+    a.work_off = b->goff.p;
+    HIP_TRY(launch_generic(a, st));
+    const uint32_t block = 256;
+    hipLaunchKernelGGL(scatter_full_kernel, dim3((uint32_t)((n + block - 1) / block)), dim3(block), 0, st,
+                       b->gdst.p, (uint32_t)n, b->gscore.p, b->gqe.p, b->gse.p, b->score.p, b->q_end.p,
+                       b->s_end.p);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(st));  // host vectors above back the uploads
+    b->rescued += n;
+    return SABER_HIP_OK;
+}
+
+int full_enqueue(saber_hip_full_batch* b) {
+    saber_hip_ctx* c = b->ctx;
+    hipStream_t st = c->stream;
+    HIP_TRY(fill_u32(b->nflagged.p, 1, 0, st));
+    HIP_TRY(hipEventRecord(c->fork, st));
+    for (int k = 0; k < kStreams; ++k) HIP_TRY(hipStreamWaitEvent(c->work[k], c->fork, 0));
+    int nv = 0;
+    const FullVariant* var = full_variants(&nv);
+    const uint32_t chunk = 65535;
+    for (const FullLaunch& L : b->launches) {
+        FullKernelArgs a{};
+        a.pairs = b->pairs.p + 2 * L.pair_off;
+        a.npairs = L.npairs;
+        a.queries = b->queries.p;
+        a.qoff = b->qoff.p;
+        a.qres = b->q.p + kGuard;
+        a.qbias = b->has_bias ? b->bias.p + kGuard : nullptr;
+        a.qres_len = b->qlen_total;
+        a.soff = b->soff.p;
+        a.sres = b->sres;
+        a.table = b->table.p;
+        a.go = b->go;
+        a.ge = b->ge;
+        a.nsub = b->ns;
+        a.score = b->score.p;
+        a.q_end = b->q_end.p;
+        a.s_end = b->s_end.p;
+        a.flagged = b->flagged.p;
+        a.nflagged = b->nflagged.p;
+        for (uint32_t s0 = 0; s0 < b->nq; s0 += chunk) {
+            a.slot0 = s0;
+            HIP_TRY(var[L.variant].launch(a, std::min(chunk, b->nq - s0), c->work[L.stream]));
+        }
+    }
+    for (int k = 0; k < kStreams; ++k) {
+        HIP_TRY(hipEventRecord(c->join[k], c->work[k]));
+        HIP_TRY(hipStreamWaitEvent(st, c->join[k], 0));
+    }
+    // Long subjects (all queries) go to the exact kernel.
+    std::vector<std::pair<uint32_t, uint32_t>> prs;
+    for (uint32_t si : b->long_subjects)
+        for (uint32_t qi = 0; qi < b->nq; ++qi) prs.emplace_back(qi, si);
+    if (int r = full_rescue(b, prs)) return r;
+    // Saturated pairs.
+    uint32_t nf = 0;
+    HIP_TRY(hipMemcpyAsync(&nf, b->nflagged.p, 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    if (nf) {
+        std::vector<uint32_t> f(2 * (size_t)nf);
+        HIP_TRY(hipMemcpy(f.data(), b->flagged.p, 8 * (size_t)nf, hipMemcpyDeviceToHost));
+        prs.clear();
+        for (uint32_t k = 0; k < nf; ++k) prs.emplace_back(f[2 * k], f[2 * k + 1]);
+        std::sort(prs.begin(), prs.end());
+        if (int r = full_rescue(b, prs)) return r;
+    }
+    return SABER_HIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int saber_hip_full_batch_create(saber_hip_ctx* c, const saber_hip_seqset* queries,
+                                const saber_hip_seqset* subjects, const saber_hip_scheme* scheme,
+                                saber_hip_full_batch** out) {
+    if (!c || !out || !scheme) return fail(SABER_HIP_EINVAL, "null argument");
+    *out = nullptr;
+    if (int r = check_seqset(queries, "queries")) return r;
+    if (subjects)
+        if (int r = check_seqset(subjects, "subjects")) return r;
+    std::lock_guard<std::mutex> g(c->mu);
+    HIP_TRY(hipSetDevice(c->device));
+    auto b = std::make_unique<saber_hip_full_batch>();
+    b->ctx = c;
+    b->nq = queries->count;
+    b->has_bias = queries->bias != nullptr;
+    b->go = scheme->gap_open;
+    b->ge = scheme->gap_extend;
+    b->hq.assign(queries->offsets, queries->offsets + queries->count + 1);
+    if (subjects)
+        b->hs.assign(subjects->offsets, subjects->offsets + subjects->count + 1);
+    else
+        b->hs = c->db_off;
+    if (b->hs.empty()) b->hs.push_back(0);
+    b->ns = (uint32_t)b->hs.size() - 1;
+    if ((uint64_t)b->nq * b->ns >= (1ull << 40)) return fail(SABER_HIP_EINVAL, "batch too large");
+    if (b->ns >= kNone) return fail(SABER_HIP_EINVAL, "too many subjects");
+
+    bool packed_ok = !(c->flags & SABER_HIP_FORCE_EXACT32);
+    packed_ok = packed_ok && scheme->gap_open >= 0 && scheme->gap_open <= 1000 &&
+                scheme->gap_extend >= 0 && scheme->gap_extend <= 1000;
+    for (int i = 0; i < 576; ++i)
+        if (scheme->matrix[i] < -128 || scheme->matrix[i] > 127) packed_ok = false;
+
+    // Bucket subjects by length: the narrowest variant with L*C >= length
+    // (ties: fewer lanes per pair).
+    int nv = 0;
+    const FullVariant* var = full_variants(&nv);
+    int widest = 0;
+    for (int v = 0; v < nv; ++v) widest = std::max(widest, var[v].L * var[v].C);
+    std::vector<std::vector<uint32_t>> bucket(nv);
+    std::vector<uint32_t> len(b->ns);
+    for (uint32_t si = 0; si < b->ns; ++si) {
+        len[si] = (uint32_t)(b->hs[si + 1] - b->hs[si]);
+        int best = -1;
+        if (packed_ok) {
+            for (int v = 0; v < nv; ++v) {
+                if (var[v].bias != (b->has_bias ? 1 : 0)) continue;
+                const int W = var[v].L * var[v].C;
+                if ((int64_t)W < (int64_t)len[si]) continue;
+                if (best < 0 || W < var[best].L * var[best].C ||
+                    (W == var[best].L * var[best].C && var[v].L < var[best].L))
+                    best = v;
+            }
+        }
+        if (best < 0)
+            b->long_subjects.push_back(si);
+        else
+            bucket[best].push_back(si);
+    }
+    std::vector<uint32_t> pairs;
+    for (int v = 0; v < nv; ++v) {
+        auto& s = bucket[v];
+        if (s.empty()) continue;
+        std::stable_sort(s.begin(), s.end(), [&](uint32_t x, uint32_t y) { return len[x] > len[y]; });
+        FullLaunch L{v, pairs.size() / 2, (uint32_t)((s.size() + 1) / 2), 0.0, 0};
+        for (size_t i = 0; i < s.size(); i += 2) {
+            pairs.push_back(s[i]);
+            pairs.push_back(i + 1 < s.size() ? s[i + 1] : kNone);
+        }
+        L.cost = (double)L.npairs * var[v].L * var[v].C;
+        b->launches.push_back(L);
+    }
+    std::sort(b->launches.begin(), b->launches.end(),
+              [](const FullLaunch& x, const FullLaunch& y) { return x.cost > y.cost; });
+    double load[kStreams] = {0};
+    for (FullLaunch& L : b->launches) {
+        int best = 0;
+        for (int k = 1; k < kStreams; ++k)
+            if (load[k] < load[best]) best = k;
+        L.stream = best;
+        load[best] += L.cost;
+    }
+
+    std::vector<short> table(kTableDim * kTableDim);
+    for (int a = 0; a < kTableDim; ++a)
+        for (int cc = 0; cc < kTableDim; ++cc) {
+            const int aa = a < 24 ? a : kCodeX, ci = cc < 24 ? cc : kCodeX;
+            const int v = std::max(-32767, std::min(32767, scheme->matrix[aa * 24 + ci]));
+            table[a * kTableDim + cc] = cc == kSentinel ? kNeg16 : (short)v;
+        }
+    std::vector<uint32_t> qids(b->nq);
+    for (uint32_t k = 0; k < b->nq; ++k) qids[k] = k;
+
+    hipStream_t st = c->stream;
+    const uint64_t nqres = b->hq.back();
+    if (int r = upload_guarded(b->q, queries->residues, nqres, st)) return r;
+    if (b->has_bias)
+        if (int r = upload_guarded(b->bias, queries->bias, nqres, st)) return r;
+    if (subjects) {
+        if (int r = upload_guarded(b->s, subjects->residues, b->hs.back(), st)) return r;
+        b->sres = b->s.p + kGuard;
+    } else {
+        if (!c->db.p && b->ns > 0) return fail(SABER_HIP_ESTATE, "no database uploaded");
+        b->sres = c->db.p + kGuard;
+    }
+    b->qlen_total = nqres;
+    b->slen_total = b->hs.back();
+    if (int r = upload(b->qoff, b->hq.data(), b->hq.size(), st)) return r;
+    if (int r = upload(b->soff, b->hs.data(), b->hs.size(), st)) return r;
+    if (int r = upload(b->queries, qids.data(), qids.size(), st)) return r;
+    if (int r = upload(b->pairs, pairs.data(), pairs.size(), st)) return r;
+    if (int r = upload(b->table, table.data(), table.size(), st)) return r;
+    if (int r = upload(b->mat, scheme->matrix, 576, st)) return r;
+    const uint64_t nout = std::max<uint64_t>(1, (uint64_t)b->nq * b->ns);
+    HIP_TRY(b->score.alloc(nout));
+    HIP_TRY(b->q_end.alloc(nout));
+    HIP_TRY(b->s_end.alloc(nout));
+    HIP_TRY(b->flagged.alloc(2 * nout));
+    HIP_TRY(b->nflagged.alloc(1));
+    HIP_TRY(hipStreamSynchronize(st));
+    *out = b.release();
+    return SABER_HIP_OK;
+}
+
+int saber_hip_full_batch_run(saber_hip_full_batch* b) {
+    if (!b) return fail(SABER_HIP_EINVAL, "batch is null");
+    std::lock_guard<std::mutex> g(b->ctx->mu);
+    HIP_TRY(hipSetDevice(b->ctx->device));
+    b->rescued = 0;
+    if (int r = full_enqueue(b)) return r;
+    b->ran = true;
+    return SABER_HIP_OK;
+}
+
+int saber_hip_full_batch_read(saber_hip_full_batch* b, int32_t* score, uint32_t* q_end,
+                              uint32_t* s_end) {
+    if (!b) return fail(SABER_HIP_EINVAL, "batch is null");
+    if (!b->ran) return fail(SABER_HIP_ESTATE, "batch has not run");
+    std::lock_guard<std::mutex> g(b->ctx->mu);
+    HIP_TRY(hipSetDevice(b->ctx->device));
+    hipStream_t st = b->ctx->stream;
+    const uint64_t n = (uint64_t)b->nq * b->ns;
+    if (n == 0) return SABER_HIP_OK;
+    if (score) HIP_TRY(hipMemcpyAsync(score, b->score.p, 4 * n, hipMemcpyDeviceToHost, st));
+    if (q_end) HIP_TRY(hipMemcpyAsync(q_end, b->q_end.p, 4 * n, hipMemcpyDeviceToHost, st));
+    if (s_end) HIP_TRY(hipMemcpyAsync(s_end, b->s_end.p, 4 * n, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return SABER_HIP_OK;
+}
+
+int saber_hip_full_batch_time(saber_hip_full_batch* b, int warmup, int iters, double* ms_run,
+                              uint64_t* cells, uint64_t* rescued) {
+    if (!b || iters < 1 || warmup < 0) return fail(SABER_HIP_EINVAL, "bad arguments");
+    for (int i = 0; i < warmup; ++i)
+        if (int r = saber_hip_full_batch_run(b)) return r;
+    std::lock_guard<std::mutex> g(b->ctx->mu);
+    HIP_TRY(hipSetDevice(b->ctx->device));
+    hipStream_t st = b->ctx->stream;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, st));
+    for (int it = 0; it < iters; ++it) {
+        b->rescued = 0;
+        if (int r = full_enqueue(b)) return r;
+    }
+    HIP_TRY(hipEventRecord(e1, st));
+    HIP_TRY(hipEventSynchronize(e1));
+    b->ran = true;
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (ms_run) *ms_run = ms / iters;
+    if (cells) {
+        uint64_t sq = 0, ss = 0;
+        sq = b->hq.back();
+        ss = b->hs.back();
+        *cells = sq * ss;
+    }
+    if (rescued) *rescued = b->rescued;
+    return SABER_HIP_OK;
+}
+
+void saber_hip_full_batch_destroy(saber_hip_full_batch* b) {
+    if (!b) return;
+    (void)hipSetDevice(b->ctx->device);
+    (void)hipStreamSynchronize(b->ctx->stream);
+    delete b;
+}
+
+int saber_hip_score_full(saber_hip_ctx* c, const saber_hip_seqset* queries,
+                         const saber_hip_seqset* subjects, const saber_hip_scheme* scheme,
+                         int32_t* score, uint32_t* q_end, uint32_t* s_end) {
+    saber_hip_full_batch* b = nullptr;
+    if (int r = saber_hip_full_batch_create(c, queries, subjects, scheme, &b)) return r;
+    int r = saber_hip_full_batch_run(b);
+    if (r == SABER_HIP_OK) r = saber_hip_full_batch_read(b, score, q_end, s_end);
+    saber_hip_full_batch_destroy(b);
+    return r;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Memory probe: a plain device copy of known size (16-byte vector loads and
+// stores), used to calibrate HBM traffic counters and to report achievable
+// copy bandwidth next to the kernels.
+// ---------------------------------------------------------------------------
+
+namespace {
+
+__global__ __launch_bounds__(256) void copy_probe_kernel(const uint4* __restrict__ src,
+                                                         uint4* __restrict__ dst, uint64_t n) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
+}  // namespace
+
+extern "C" int saber_hip_copy_probe(saber_hip_ctx* c, uint64_t bytes, int iters, double* ms) {
+    if (!c || bytes < 16 || iters < 1) return fail(SABER_HIP_EINVAL, "bad arguments");
+    std::lock_guard<std::mutex> g(c->mu);
+    HIP_TRY(hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    const uint64_t n = bytes / 16;
+    DevBuf<uint4> a, b;
+    HIP_TRY(a.alloc(n));
+    HIP_TRY(b.alloc(n));
+    HIP_TRY(hipMemsetAsync(a.p, 1, 16 * n, st));
+    int cus = 0;
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+    const uint32_t grid = (uint32_t)std::min<uint64_t>((uint64_t)cus * 16, (n + 255) / 256);
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    hipLaunchKernelGGL(copy_probe_kernel, dim3(grid), dim3(256), 0, st, a.p, b.p, n);
+    HIP_TRY(hipEventRecord(e0, st));
+    for (int i = 0; i < iters; ++i)
+        hipLaunchKernelGGL(copy_probe_kernel, dim3(grid), dim3(256), 0, st, a.p, b.p, n);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(e1, st));
+    HIP_TRY(hipEventSynchronize(e1));
+    float t = 0.f;
+    HIP_TRY(hipEventElapsedTime(&t, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (ms) *ms = t / iters;
+    return SABER_HIP_OK;
+}

@@ -100,3 +100,38 @@ hipError_t launch_xdrop(const XdropArgs& a, int32_t* score, uint32_t* q_lo, uint
                         uint32_t* steps, int grid, hipStream_t s);
 
 }  // namespace saber
+
+namespace saber {
+
+// Full-matrix (exhaustive) kernel over the cross product of queries and
+// subjects; outputs are [query slot][subject].
+struct FullKernelArgs {
+    const uint32_t* pairs;      // 2 subject ids per pair (0xFFFFFFFF = none)
+    uint32_t npairs;
+    const uint32_t* queries;    // query ids of this launch (blockIdx.y indexes it)
+    const uint64_t* qoff;       // query offsets
+    const uint8_t* qres;        // guard-padded like the band kernel's buffers
+    const int8_t* qbias;
+    uint64_t qres_len;
+    const uint64_t* soff;       // subject offsets
+    const uint8_t* sres;
+    const short* table;
+    int32_t go, ge;
+    uint32_t nsub;              // row stride of the outputs (subjects per query)
+    uint32_t slot0;             // first output row of this launch
+    int32_t* score;
+    uint32_t* q_end;
+    uint32_t* s_end;
+    uint32_t* flagged;          // (slot, subject) pairs needing the int32 rescore
+    uint32_t* nflagged;
+};
+
+struct FullVariant {
+    int L, C;
+    int bias;
+    hipError_t (*launch)(const FullKernelArgs&, uint32_t nslots, hipStream_t);
+};
+
+const FullVariant* full_variants(int* count);
+
+}  // namespace saber

@@ -25,30 +25,21 @@
 
 namespace saber {
 
-struct FullKernelArgs {
-    const uint32_t* pairs;      // 2 subject ids per pair (0xFFFFFFFF = none)
-    uint32_t npairs;
-    const uint32_t* queries;    // query ids of this launch (blockIdx.y indexes it)
-    const uint64_t* qoff;       // query offsets
-    const uint8_t* qres;
-    const int8_t* qbias;
-    uint64_t qres_len;
-    const uint64_t* soff;       // subject offsets
-    const uint8_t* sres;
-    const short* table;
-    int32_t go, ge;
-    uint32_t nsub;              // row stride of the outputs (subjects per query)
-    int32_t* score;             // [query slot][subject]
-    uint32_t* q_end;
-    uint32_t* s_end;
-    uint32_t* flagged;          // (slot, subject) pairs needing the int32 rescore
-    uint32_t* nflagged;
-};
+// Lane i receives lane i-1: within a 16-lane row for groups that tile rows,
+// across the whole wave (wave_shr:1) for 32- and 64-lane groups.
+template <int L>
+__device__ __forceinline__ uint32_t full_from_prev_lane(uint32_t v) {
+    if constexpr (L <= 16)
+        return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xF, 0xF, false);
+    else
+        return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x138, 0xF, 0xF, false);
+}
 
 template <int L, int C, bool BIAS>
 __global__ __launch_bounds__(256) void full_pair_kernel(FullKernelArgs a) {
     static_assert(C >= 2, "variant shape");
-    static_assert(L == 1 || L == 2 || L == 4 || L == 8 || L == 16, "groups must tile a DPP row");
+    static_assert(L == 1 || L == 2 || L == 4 || L == 8 || L == 16 || L == 32 || L == 64,
+                  "groups must tile a wave");
     constexpr int NW = (C + 3) / 4;
     __shared__ short T[kTableDim * kTableDim];
     for (int t = threadIdx.x; t < kTableDim * kTableDim; t += blockDim.x) T[t] = a.table[t];
@@ -59,7 +50,7 @@ __global__ __launch_bounds__(256) void full_pair_kernel(FullKernelArgs a) {
     const uint32_t g = gl / L;
     const int l = (int)(gl % L);
     if (g >= a.npairs) return;
-    const uint32_t slot = blockIdx.y;
+    const uint32_t slot = a.slot0 + blockIdx.y;
     const uint32_t qid = a.queries[slot];
     const int64_t q0 = (int64_t)a.qoff[qid];
     const int m = (int)(a.qoff[qid + 1] - a.qoff[qid]);
@@ -142,8 +133,8 @@ __global__ __launch_bounds__(256) void full_pair_kernel(FullKernelArgs a) {
         const bool act = (r >= 0) && (r < R);
         v2s inHl = ZERO, inE = NEG;
         if constexpr (L > 1) {
-            inHl = pair(from_prev_lane(outHl));
-            inE = pair(from_prev_lane(outE));
+            inHl = pair(full_from_prev_lane<L>(outHl));
+            inE = pair(full_from_prev_lane<L>(outE));
             if (l == 0) {
                 inHl = ZERO;
                 inE = NEG;

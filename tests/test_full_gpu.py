@@ -1,0 +1,94 @@
+"""Exhaustive full-matrix SW on the GPU vs the oracle (sw_reference,
+reference.rs:15-52): every query x every subject, score and first row-major
+end cell bit-exact. Cases follow simd_matches_independent_reference
+(sw_simd.rs:488-519), saturation_is_detected_and_repaired (sw_simd.rs:521-533)
+and the exhaustive-mode consistency test (tests/engine_consistency.rs:225-258)."""
+import numpy as np
+import pytest
+
+import saber_amd as S
+from oracle import oracle as O
+from saber_amd import BLOSUM62, PAM30, workload
+from tests.gen import Lcg, flat, mutate, rand_seq
+
+pytestmark = pytest.mark.gpu
+
+W = 17
+
+
+def oracle_grid(qs, ss, mat, go, ge, qbias=None):
+    sc = np.zeros((len(qs), len(ss)), np.int64)
+    qe = np.zeros_like(sc)
+    se = np.zeros_like(sc)
+    for a, q in enumerate(qs):
+        b = None if qbias is None else qbias[a]
+        for c, s in enumerate(ss):
+            sc[a, c], qe[a, c], se[a, c] = O.sw_reference(q, s, mat, b, go, ge)
+    return sc, qe, se
+
+
+def check(ctx, qs, ss, mat, go, ge, qbias=None):
+    qres, qoff = flat(qs)
+    sres, soff = flat(ss)
+    qb = None if qbias is None else np.concatenate(qbias).astype(np.int8)
+    got = ctx.score_full(qres, qoff, sres, soff, S.Scheme(mat, go, ge), qb)
+    want = oracle_grid(qs, ss, mat, go, ge, qbias)
+    for name, g, w in zip(("score", "q_end", "s_end"), got, want):
+        bad = np.argwhere(g.astype(np.int64) != w)
+        assert len(bad) == 0, f"{name} differs at {bad[:5].tolist()}"
+    return got
+
+
+@pytest.mark.parametrize("gaps", [(11, 1), (10, 2), (13, 2)])
+def test_random_queries_and_subjects(ctx, gaps):
+    r = Lcg(488 + gaps[0])
+    qs = [rand_seq(r, 1 + r.next(300)) for _ in range(6)]
+    ss = []
+    for k in range(60):
+        if k % 4 == 0:
+            s = mutate(r, qs[k % len(qs)])
+            ss.append(s if len(s) else rand_seq(r, 3))
+        else:
+            ss.append(rand_seq(r, 1 + r.next(600)))
+    check(ctx, qs, ss, BLOSUM62, *gaps)
+    bias = [np.array([r.next(9) - 4 for _ in range(len(q))], np.int8) for q in qs]
+    check(ctx, qs, ss, BLOSUM62, *gaps, qbias=bias)
+
+
+def test_subject_lengths_across_all_variants(ctx):
+    """Subjects of every length class up to 3072 and beyond (int32 route)."""
+    r = Lcg(7)
+    q = rand_seq(r, 120)
+    lens = [1, 2, 15, 16, 17, 33, 48, 49, 64, 65, 97, 128, 129, 200, 257, 384, 385, 512, 513, 700,
+            768, 769, 1024, 1025, 1500, 1536, 1537, 2048, 2049, 2560, 3000, 3072, 3073, 3300]
+    ss = []
+    for n in lens:
+        s = rand_seq(r, n)
+        k = min(n, len(q))
+        s[:k] = np.where(np.array([r.next(3) for _ in range(k)]) > 0, q[:k], s[:k])
+        ss.append(s)
+    check(ctx, [q, q[:50]], ss, PAM30, 10, 1)
+
+
+def test_saturation_and_empty(ctx):
+    # W x 3000 scores 33 000 (above int16): flagged and rescored exactly.
+    a = np.full(3000, W, np.uint8)
+    got = check(ctx, [a, np.zeros(0, np.uint8)], [a, a[:100], np.zeros(0, np.uint8)], BLOSUM62, 11, 1)
+    assert got[0][0, 0] == 33000 and (got[1][0, 0], got[2][0, 0]) == (3000, 3000)
+    assert (got[0][1] == 0).all()
+
+
+def test_backend_routes_to_full_kernel(ctx):
+    r = Lcg(3)
+    q = rand_seq(r, 250)
+    wins = [rand_seq(r, 1 + (13 * i) % 700) for i in range(100)]
+    got = S.HipBackend(0).score_windows(q, None, wins, BLOSUM62, S.GapCosts(11, 1))
+    assert got == [O.sw_reference(q, w, BLOSUM62, None, 11, 1)[0] for w in wins]
+
+
+def test_workload_sample(ctx):
+    qres, qoff = workload.sequences(11, 8, 250, 0.6, 30, 3000)
+    sres, soff = workload.sequences(12, 300, 250, 0.6, 30, 3000)
+    qs = [qres[qoff[i]:qoff[i + 1]] for i in range(len(qoff) - 1)]
+    ss = [sres[soff[i]:soff[i + 1]] for i in range(len(soff) - 1)]
+    check(ctx, qs, ss, BLOSUM62, 11, 1)
