@@ -135,3 +135,43 @@ struct FullVariant {
 const FullVariant* full_variants(int* count);
 
 }  // namespace saber
+
+namespace saber {
+
+// Prefilter MLP (mlp.hip). Weights in the operand layouts described there.
+struct MlpWeights {
+    const __bf16* W1T;
+    const __bf16* W2P;
+    const __bf16* W2Q;
+    const float* b1;
+    const float* b2;
+    const float* w3;
+    float b3;
+};
+
+struct MlpGrads {
+    float* dW1;    // [64][256]
+    float* dW2;    // [256][256]
+    float* db1;    // [256]
+    float* db2;    // [256]
+    float* dw3;    // [256]
+    float* db3;    // [1]
+    float* loss;   // [1]
+};
+
+hipError_t mlp_launch_forward(const MlpWeights& w, const __bf16* X, uint64_t n, float* logits,
+                              int grid, hipStream_t s);
+hipError_t mlp_launch_backward_act(const MlpWeights& w, const __bf16* X, const float* labels,
+                                   uint64_t n, float pos_weight, float inv_n, const MlpGrads& gr,
+                                   __bf16* H1, __bf16* H2, __bf16* dH2, __bf16* dH1, float* dy,
+                                   int grid, hipStream_t s);
+hipError_t mlp_launch_colsum(const __bf16* M, const float* wgt, uint64_t n, float* out, hipStream_t s);
+hipError_t mlp_launch_gemm_tn(const __bf16* A, int lda, int a_dim, const __bf16* B, int ldb, int b_dim,
+                              float* C, uint64_t K, int kchunks, hipStream_t s);
+hipError_t mlp_launch_adam(float* p, float* grad, float* m, float* v, uint32_t n, float lr, float b1,
+                           float b2, float eps, float c1, float c2, hipStream_t s);
+hipError_t mlp_launch_pack(const float* W1, const float* W2, __bf16* W1T, __bf16* W2P, __bf16* W2Q,
+                           hipStream_t s);
+hipError_t mlp_launch_to_bf16(const float* src, __bf16* dst, uint64_t n, hipStream_t s);
+
+}  // namespace saber
