@@ -134,12 +134,12 @@ void saber_hip_band_batch_destroy(saber_hip_band_batch* b);
 
 /* Timing of a batch: warmup + iters runs bracketed by HIP events on the
  * context stream. ms_run = mean wall time of one run on the device;
- * kernel_ms[v] = mean time of the launches of kernel variant v per run (v
- * indexes saber_hip_band_variant) and kernel_cells[v] the band cells those
- * launches score; nvar is the capacity of both arrays on input and the number
- * of variants on output. cells = band cells per run. */
+ * band_ms = mean time of the packed kernel's launch (one launch per run, all
+ * shapes); variant_cells[v] = band cells scored by variant v (v indexes
+ * saber_hip_band_variant); nvar is the capacity of variant_cells on input and
+ * the number of variants on output. cells = band cells per run. */
 int saber_hip_band_batch_time(saber_hip_band_batch* b, int warmup, int iters, double* ms_run,
-                              double* kernel_ms, uint64_t* kernel_cells, int* nvar,
+                              double* band_ms, uint64_t* variant_cells, int* nvar,
                               uint64_t* cells);
 
 /* Exhaustive mode (score_many over whole subjects, src/algorithm/sw_simd.rs:95-153,
@@ -215,6 +215,12 @@ int saber_hip_copy_probe(saber_hip_ctx* ctx, uint64_t bytes, int iters, double* 
  * widths lo..hi (inclusive) use variant shape (L, C, K); L = 0 routes them to
  * the exact int32 kernel. */
 int saber_hip_band_choose(saber_hip_ctx* ctx, int w_lo, int w_hi, int L, int C, int K);
+
+/* Planner tuning (no counterpart in the reference): band jobs with more than
+ * latency_rows rows are planned for low per-row latency (short lanes, their
+ * own launches, started first); 0 turns this off. Applies to batches created
+ * afterwards. */
+int saber_hip_band_tune(saber_hip_ctx* ctx, int latency_rows);
 
 #ifdef __cplusplus
 }

@@ -16,13 +16,29 @@ Every call runs the HIP kernels of libsaber_hip.so on the GPU; there is no
 CPU path here. Inputs are alignment codes 0..23 (codes >= 24 score as X).
 """
 from dataclasses import dataclass
+import atexit
 import ctypes as C
+import weakref
 
 import numpy as np
 
 from . import _ffi
 from ._ffi import HipError
 from .matrices import ALPHABET, BLOSUM45, BLOSUM62, PAM30
+
+# Live batches and contexts, released before interpreter exit: the HIP runtime
+# tears itself down after atexit handlers, and a stream destroyed after that
+# aborts the process.
+_live_batches = weakref.WeakSet()
+_live_contexts = weakref.WeakSet()
+
+
+@atexit.register
+def _release_all():
+    for b in list(_live_batches):
+        b.close()
+    for c in list(_live_contexts):
+        c.close()
 
 __all__ = [
     "ALPHABET", "BLOSUM45", "BLOSUM62", "PAM30", "Band", "BandJob", "BandScore", "GapCosts",
@@ -122,6 +138,7 @@ class BandBatch:
 
     def __init__(self, ctx, handle, njobs, keep):
         self.ctx, self.h, self.njobs, self._keep = ctx, handle, njobs, keep
+        _live_batches.add(self)
 
     def run(self):
         _ffi.check(_ffi.lib().saber_hip_band_batch_run(self.h))
@@ -141,18 +158,18 @@ class BandBatch:
         return score, qe, se, cells
 
     def time(self, warmup=1, iters=5):
-        """(ms per run, {variant index: (kernel ms per run, cells per run)},
-        cells per run)."""
+        """(ms per run, ms of the packed kernel's launch per run,
+        {variant index: cells per run}, cells per run)."""
         ms = C.c_double()
+        kms = C.c_double()
         cap = 256
-        km = (C.c_double * cap)()
         kc = (C.c_uint64 * cap)()
         nv = C.c_int(cap)
         cells = C.c_uint64()
-        _ffi.check(_ffi.lib().saber_hip_band_batch_time(self.h, warmup, iters, C.byref(ms), km, kc,
+        _ffi.check(_ffi.lib().saber_hip_band_batch_time(self.h, warmup, iters, C.byref(ms), C.byref(kms), kc,
                                                           C.byref(nv), C.byref(cells)))
-        per = {v: (km[v], kc[v]) for v in range(min(nv.value, cap)) if km[v] > 0}
-        return ms.value, per, cells.value
+        per = {v: kc[v] for v in range(min(nv.value, cap)) if kc[v] > 0}
+        return ms.value, kms.value, per, cells.value
 
     def close(self):
         if self.h:
@@ -171,6 +188,7 @@ class FullBatch:
 
     def __init__(self, ctx, handle, nq, ns, keep):
         self.ctx, self.h, self.nq, self.ns, self._keep = ctx, handle, nq, ns, keep
+        _live_batches.add(self)
 
     def run(self):
         _ffi.check(_ffi.lib().saber_hip_full_batch_run(self.h))
@@ -210,6 +228,7 @@ class XdropBatch:
 
     def __init__(self, ctx, handle, nhits, keep):
         self.ctx, self.h, self.nhits, self._keep = ctx, handle, nhits, keep
+        _live_batches.add(self)
 
     def run(self):
         _ffi.check(_ffi.lib().saber_hip_xdrop_batch_run(self.h))
@@ -266,6 +285,7 @@ class Context:
         h = C.c_void_p()
         _ffi.check(L.saber_hip_create(device, C.byref(opts), C.byref(h)))
         self.h = h
+        _live_contexts.add(self)
 
     def close(self):
         if self.h:
@@ -286,6 +306,11 @@ class Context:
 
     def choose(self, w_lo, w_hi, L, Cc, K):
         _ffi.check(_ffi.lib().saber_hip_band_choose(self.h, w_lo, w_hi, L, Cc, K))
+
+    def tune(self, latency_rows):
+        """Band jobs with more rows than this are planned for per-row latency
+        (0: never)."""
+        _ffi.check(_ffi.lib().saber_hip_band_tune(self.h, int(latency_rows)))
 
     def upload_db(self, qres, qoff):
         off = np.ascontiguousarray(qoff, dtype=np.uint64)

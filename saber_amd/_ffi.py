@@ -94,6 +94,7 @@ _SIGS = {
     "saber_hip_band_variant": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                          C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "saber_hip_band_choose": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "saber_hip_band_tune": (C.c_int, [P, C.c_int]),
     "saber_hip_copy_probe": (C.c_int, [P, C.c_uint64, C.c_int, C.POINTER(C.c_double)]),
 }
 

@@ -46,6 +46,14 @@ constexpr uint32_t kNone = 0xFFFFFFFFu;
 struct Shape {
     int L, C, K;
     int W() const { return L * C; }
+    // Every lane caps its last K cells, so a shape serves widths W-K+1 .. W,
+    // and every width up to W when K = C.
+    bool covers(int64_t w) const { return w >= 1 && w <= W() && (K == C || w > W() - K); }
+    // VALU instructions of one lane per row: about 13 per packed cell, one per
+    // capped cell and a fixed part for the row and stream bookkeeping.
+    double row_cost() const { return 13.0 * C + K + 45.0; }
+    // Work of one job pair per row.
+    double pair_cost() const { return L * row_cost(); }
 };
 
 __global__ void fill_u32_kernel(uint32_t* p, uint64_t n, uint32_t v) {
@@ -59,6 +67,17 @@ hipError_t fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s) {
     const uint64_t grid = (n + block - 1) / block;
     hipLaunchKernelGGL(fill_u32_kernel, dim3((uint32_t)grid), dim3(block), 0, s, p, n, v);
     return hipGetLastError();
+}
+
+// Residue codes >= 24 all score as X (src/alphabet.rs:39-41); every kernel
+// maps them so. Stored residue buffers hold the canonical code (X for any
+// such byte), which lets the band kernel index its 24-row table directly.
+__global__ void canon_codes_kernel(uint8_t* p, uint64_t n) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const uint8_t c = p[i];
+        if (c >= 24) p[i] = (uint8_t)saber::kCodeX;
+    }
 }
 
 template <class T>
@@ -81,6 +100,13 @@ struct DevBuf {
 
 }  // namespace
 
+// One lane steps through every row of its job, so a job's time is at least
+// rows x the lane's per-row time. Jobs whose time at their throughput shape
+// would exceed this fraction of the batch's estimated time are planned for
+// low per-row latency instead (short lanes, launched first). A context can
+// replace the rule by a fixed row count (saber_hip_band_tune).
+constexpr double kLatencyShare = 0.6;
+
 struct saber_hip_ctx {
     int device = 0;
     uint32_t flags = 0;
@@ -91,22 +117,23 @@ struct saber_hip_ctx {
     std::mutex mu;
     DevBuf<uint8_t> db;
     std::vector<uint64_t> db_off;
-    std::vector<int> choice;  // width -> shape index (-1: exact int32 kernel)
+    std::vector<int> choice;      // width -> shape index (-1: exact int32 kernel)
+    std::vector<int> choice_lat;  // the same for jobs with many rows
     std::vector<Shape> shapes;
+    int latency_rows = -1;  // -1: kLatencyShare rule; 0: off; else a row count
 };
 
-// Band kernel launches of one run are spread over this many streams (all at
-// the default priority) so small width buckets and the tails of large ones
+// Full-SW launches of one run are spread over this many streams (all at the
+// default priority) so small subject buckets and the tails of large ones
 // overlap.
 constexpr int kStreams = 4;
 
-struct VarLaunch {
-    int variant;
-    uint64_t pair_off;
-    uint32_t npairs;
-    double cost;   // estimated lane-steps
-    int stream;    // index into the context's work streams
-};
+// Rates for the band planner's time estimates (lane VALU instructions per
+// ms): the whole GPU at about half its issue rate, and one lane of a wave
+// that shares its SIMD with another.
+constexpr double kGpuLaneInstrPerMs = 3.4e10;
+constexpr double kLaneInstrPerMs = 3.0e5;
+
 
 struct saber_hip_band_batch {
     saber_hip_ctx* ctx = nullptr;
@@ -115,11 +142,13 @@ struct saber_hip_band_batch {
     int32_t go = 0, ge = 0;
     DevBuf<uint8_t> q, s;
     DevBuf<int8_t> bias;
-    DevBuf<short> table;
+    DevBuf<uint32_t> table;   // int8 band table as words
     DevBuf<int32_t> mat;
     DevBuf<BandJobDev> jobs;
     DevBuf<uint32_t> pairs;
-    std::vector<VarLaunch> launches;
+    // multi-shape launch: blocks in dispatch order (longest first)
+    std::vector<BandBlock> host_blocks;
+    DevBuf<BandBlock> blocks;
     // exact int32 kernel: wide / unsuitable jobs known at plan time, plus the
     // packed kernel's saturated jobs known after it ran
     std::vector<uint32_t> generic_static;
@@ -140,6 +169,30 @@ struct saber_hip_band_batch {
 
 namespace {
 
+// Query / window lengths the packed band kernel accepts (window columns are
+// handled as 8 * j in 32 bits).
+constexpr int64_t kMaxPackedLen = int64_t(1) << 27;
+
+// sum of x for x in [a, b] (0 when b < a)
+int64_t range_sum(int64_t a, int64_t b) { return b < a ? 0 : (a + b) * (b - a + 1) / 2; }
+
+// Band cells of a planned job: every row i in [i_first, i_first + rows) holds
+// the columns max(0, i + lo) .. min(n - 1, i + lo + w - 1), at least one
+// (the planner keeps exactly those rows), so the count is a sum of clamped
+// linear terms.
+uint64_t band_cells(const BandJobDev& d) {
+    if (d.rows <= 0) return 0;
+    const int64_t i0 = d.i_first, i1 = (int64_t)d.i_first + d.rows - 1;
+    const int64_t lo = d.lo, hi = (int64_t)d.lo + d.w - 1, n = d.n;
+    // sum of min(n - 1, i + hi): i + hi up to i = n - 1 - hi, then n - 1
+    const int64_t ic = n - 1 - hi;
+    int64_t right = range_sum(i0 + hi, std::min(i1, ic) + hi);
+    right += std::max<int64_t>(0, i1 - std::max(i0, ic + 1) + 1) * (n - 1);
+    // sum of max(0, i + lo): 0 below i = -lo
+    const int64_t left = range_sum(std::max(i0, -lo) + lo, i1 + lo);
+    return (uint64_t)(right - left + (i1 - i0 + 1));
+}
+
 void default_choice(saber_hip_ctx* c) {
     int nv = 0;
     const BandVariant* v = band_variants(&nv);
@@ -151,23 +204,23 @@ void default_choice(saber_hip_ctx* c) {
     int wmax = 0;
     for (auto& s : c->shapes) wmax = std::max(wmax, s.W());
     c->choice.assign((size_t)wmax + 1, -1);
+    c->choice_lat.assign((size_t)wmax + 1, -1);
     for (int w = 1; w <= wmax; ++w) {
-        int best = -1;
+        int best = -1, fast = -1;
         for (int k = 0; k < (int)c->shapes.size(); ++k) {
             const Shape& s = c->shapes[k];
-            if (s.W() < w || s.W() - s.K >= w) continue;
-            if (best < 0 || s.W() < c->shapes[best].W()) best = k;
+            if (!s.covers(w)) continue;
+            // throughput: least work per pair-row; latency: least per-row time,
+            // and among equals the widest shape, so that the few many-row
+            // jobs of all widths share one launch
+            if (best < 0 || s.pair_cost() < c->shapes[best].pair_cost()) best = k;
+            if (fast < 0 || s.row_cost() < c->shapes[fast].row_cost() ||
+                (s.row_cost() == c->shapes[fast].row_cost() && s.W() > c->shapes[fast].W()))
+                fast = k;
         }
         c->choice[w] = best;
+        c->choice_lat[w] = fast;
     }
-}
-
-int find_variant(int L, int C, int K, int bias) {
-    int nv = 0;
-    const BandVariant* v = band_variants(&nv);
-    for (int i = 0; i < nv; ++i)
-        if (v[i].L == L && v[i].C == C && v[i].K == K && v[i].bias == bias) return i;
-    return -1;
 }
 
 int check_seqset(const saber_hip_seqset* s, const char* what) {
@@ -197,6 +250,18 @@ int upload_guarded(DevBuf<T>& d, const T* h, uint64_t n, hipStream_t s) {
     HIP_TRY(d.alloc(n + kGuard + kGuardTail));
     HIP_TRY(hipMemsetAsync(d.p, 0, sizeof(T) * (n + kGuard + kGuardTail), s));
     if (n) HIP_TRY(hipMemcpyAsync(d.p + kGuard, h, sizeof(T) * n, hipMemcpyHostToDevice, s));
+    return SABER_HIP_OK;
+}
+
+// A residue buffer: guarded upload, then every code >= 24 stored as X.
+int upload_codes(DevBuf<uint8_t>& d, const uint8_t* h, uint64_t n, hipStream_t s) {
+    if (int r = upload_guarded(d, h, n, s)) return r;
+    if (n) {
+        const uint32_t block = 256;
+        const uint64_t grid = (n + block - 1) / block;
+        hipLaunchKernelGGL(canon_codes_kernel, dim3((uint32_t)grid), dim3(block), 0, s, d.p + kGuard, n);
+        HIP_TRY(hipGetLastError());
+    }
     return SABER_HIP_OK;
 }
 
@@ -263,7 +328,7 @@ int saber_hip_upload_db(saber_hip_ctx* c, const saber_hip_seqset* subjects) {
     std::lock_guard<std::mutex> g(c->mu);
     HIP_TRY(hipSetDevice(c->device));
     const uint64_t nres = subjects->count ? subjects->offsets[subjects->count] : 0;
-    if (int r = upload_guarded(c->db, subjects->residues, nres, c->stream)) return r;
+    if (int r = upload_codes(c->db, subjects->residues, nres, c->stream)) return r;
     HIP_TRY(hipStreamSynchronize(c->stream));
     c->db_off.assign(subjects->offsets, subjects->offsets + subjects->count + 1);
     return SABER_HIP_OK;
@@ -290,14 +355,22 @@ int saber_hip_band_choose(saber_hip_ctx* c, int w_lo, int w_hi, int L, int C, in
             if (c->shapes[k].L == L && c->shapes[k].C == C && c->shapes[k].K == K) shape = k;
         if (shape < 0) return fail(SABER_HIP_EINVAL, "no compiled variant with that shape");
         const Shape& s = c->shapes[shape];
-        if (w_hi > s.W() || w_lo <= s.W() - s.K)
+        if (!s.covers(w_lo) || !s.covers(w_hi))
             return fail(SABER_HIP_EINVAL, "widths outside the variant's coverage");
     }
     if ((size_t)w_hi >= c->choice.size()) {
         if (L != 0) return fail(SABER_HIP_EINVAL, "width above the widest variant");
         w_hi = (int)c->choice.size() - 1;
     }
-    for (int w = w_lo; w <= w_hi; ++w) c->choice[w] = shape;
+    for (int w = w_lo; w <= w_hi; ++w) c->choice[w] = c->choice_lat[w] = shape;
+    return SABER_HIP_OK;
+}
+
+int saber_hip_band_tune(saber_hip_ctx* c, int latency_rows) {
+    if (!c) return fail(SABER_HIP_EINVAL, "ctx is null");
+    if (latency_rows < -1) return fail(SABER_HIP_EINVAL, "latency_rows must be >= -1");
+    std::lock_guard<std::mutex> g(c->mu);
+    c->latency_rows = latency_rows;
     return SABER_HIP_OK;
 }
 
@@ -334,7 +407,11 @@ int saber_hip_band_batch_create(saber_hip_ctx* c, const saber_hip_seqset* querie
 
     // Plan.
     b->host_jobs.assign(njobs, BandJobDev{});
-    std::vector<std::vector<uint32_t>> bucket(c->shapes.size());
+    std::vector<uint64_t> host_cells(njobs, 0);
+    std::vector<int> thr_shape(njobs, -1);  // throughput shape of each packed job
+    double est_work = 0.0;                  // lane instructions, all packed jobs
+    // bucket 2 * shape + 1 holds the shape's many-row jobs
+    std::vector<std::vector<uint32_t>> bucket(2 * c->shapes.size());
     std::vector<int32_t> rows_of(njobs, 0);
     for (uint64_t k = 0; k < njobs; ++k) {
         const saber_hip_band_job& J = jobs[k];
@@ -365,63 +442,94 @@ int saber_hip_band_batch_create(saber_hip_ctx* c, const saber_hip_seqset* querie
         d.rows = (int32_t)(i_last - i_first + 1);
         b->host_jobs[k] = d;
         rows_of[k] = d.rows;
-        const int shape = (packed_ok && w < (int64_t)c->choice.size()) ? c->choice[w] : -1;
-        if (shape < 0)
+        host_cells[k] = band_cells(d);
+        // The packed kernel keeps row indices in 16 bits and window columns
+        // (times 8) in 32 bits.
+        const bool fits = d.rows <= (int32_t)kMaxBandRows && m < kMaxPackedLen && n < kMaxPackedLen;
+        const int shape = (packed_ok && fits && w < (int64_t)c->choice.size()) ? c->choice[w] : -1;
+        if (shape < 0) {
             b->generic_static.push_back((uint32_t)k);
-        else
-            bucket[shape].push_back((uint32_t)k);
+            continue;
+        }
+        thr_shape[k] = shape;
+        est_work += 0.5 * (d.rows + c->shapes[shape].L + 3) * c->shapes[shape].pair_cost();
     }
+    // Many-row jobs: a fixed row count, or the share of the estimated time.
+    const double est_ms = est_work / kGpuLaneInstrPerMs;
+    for (uint64_t k = 0; k < njobs; ++k) {
+        const int shape = thr_shape[k];
+        if (shape < 0) continue;
+        const Shape& s = c->shapes[shape];
+        bool lat;
+        if (c->latency_rows < 0)
+            lat = (rows_of[k] + s.L + 3) * s.row_cost() / kLaneInstrPerMs > kLatencyShare * est_ms;
+        else
+            lat = c->latency_rows > 0 && rows_of[k] > c->latency_rows;
+        const int w = b->host_jobs[k].w;
+        if (lat)
+            bucket[2 * c->choice_lat[w] + 1].push_back((uint32_t)k);
+        else
+            bucket[2 * shape].push_back((uint32_t)k);
+    }
+    // Pairs: each bucket sorted by rows (longest first) and paired with the
+    // next longest, since a pair runs for its longer job's rows. Blocks of
+    // 256 lanes (256 / L pairs) of one shape; all blocks of all shapes are
+    // dispatched in decreasing order of their estimated time (the longest
+    // pair's row steps x the shape's per-row cost), so the hardware
+    // dispatcher packs them longest-processing-time first.
     std::vector<uint32_t> pairs;
-    for (size_t sh = 0; sh < bucket.size(); ++sh) {
-        auto& v = bucket[sh];
+    std::vector<double> block_est;
+    for (size_t bk = 0; bk < bucket.size(); ++bk) {
+        auto& v = bucket[bk];
         if (v.empty()) continue;
         std::stable_sort(v.begin(), v.end(), [&](uint32_t x, uint32_t y) { return rows_of[x] > rows_of[y]; });
-        const Shape& s = c->shapes[sh];
-        const int var = find_variant(s.L, s.C, s.K, b->has_bias ? 1 : 0);
-        if (var < 0) return fail(SABER_HIP_EINVAL, "variant table inconsistent");
-        VarLaunch L{var, pairs.size() / 2, (uint32_t)((v.size() + 1) / 2), 0.0, 0};
+        const int shape = (int)(bk / 2);
+        const Shape& s = c->shapes[shape];
+        const uint32_t per_block = 256u / (uint32_t)s.L;
+        const uint64_t first_pair = pairs.size() / 2;
         for (size_t i = 0; i < v.size(); i += 2) {
             pairs.push_back(v[i]);
             pairs.push_back(i + 1 < v.size() ? v[i + 1] : kNone);
-            // lane-steps of the pair: (longer row count + lag) x band width
-            L.cost += (double)(rows_of[v[i]] + s.L - 1) * s.W();
         }
-        b->launches.push_back(L);
+        const uint64_t np = pairs.size() / 2 - first_pair;
+        for (uint64_t p0 = 0; p0 < np; p0 += per_block) {
+            const uint32_t cnt = (uint32_t)std::min<uint64_t>(per_block, np - p0);
+            const int steps = (rows_of[v[2 * p0]] + s.L - 1 + 3) & ~3;
+            b->host_blocks.push_back(BandBlock{(uint32_t)shape, (uint32_t)(first_pair + p0), cnt, 0});
+            block_est.push_back((double)steps * s.row_cost());
+        }
     }
-    // Longest-processing-time assignment of the launches to the streams.
-    std::sort(b->launches.begin(), b->launches.end(),
-              [](const VarLaunch& x, const VarLaunch& y) { return x.cost > y.cost; });
     {
-        double load[kStreams] = {0};
-        for (VarLaunch& L : b->launches) {
-            int best = 0;
-            for (int k = 1; k < kStreams; ++k)
-                if (load[k] < load[best]) best = k;
-            L.stream = best;
-            load[best] += L.cost;
-        }
+        std::vector<uint32_t> order(b->host_blocks.size());
+        for (uint32_t i = 0; i < order.size(); ++i) order[i] = i;
+        std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return block_est[x] > block_est[y]; });
+        std::vector<BandBlock> sorted(order.size());
+        for (size_t i = 0; i < order.size(); ++i) sorted[i] = b->host_blocks[order[i]];
+        b->host_blocks.swap(sorted);
     }
 
-    // Score table for the packed kernel: T[a][c] = S(code_index(a), code_index(c)),
-    // column 31 = sentinel.
-    std::vector<short> table(kTableDim * kTableDim);
-    for (int a = 0; a < kTableDim; ++a)
-        for (int cc = 0; cc < kTableDim; ++cc) {
-            const int aa = a < 24 ? a : kCodeX, ci = cc < 24 ? cc : kCodeX;
-            int v = scheme->matrix[aa * 24 + ci];
-            v = std::max(-32767, std::min(32767, v));
-            table[a * kTableDim + cc] = cc == kSentinel ? kNeg16 : (short)v;
-        }
+    // Score table for the packed kernel: int8 T[a][c] = S(a, c) for codes
+    // a, c < 24 (stride 25), column 24 = sentinel (-128). Only built when
+    // every score fits int8 (packed_ok); otherwise no job uses it.
+    std::vector<uint32_t> table(kTable8Words, 0);
+    {
+        int8_t* t8 = reinterpret_cast<int8_t*>(table.data());
+        for (int a = 0; a < 24; ++a)
+            for (int cc = 0; cc <= kBandSentinel; ++cc) {
+                const int v = cc == kBandSentinel ? -128 : std::max(-128, std::min(127, scheme->matrix[a * 24 + cc]));
+                t8[a * kTable8Stride + cc] = (int8_t)v;
+            }
+    }
 
     hipStream_t st = c->stream;
     const uint64_t nq = queries->count ? queries->offsets[queries->count] : 0;
-    if (int r = upload_guarded(b->q, queries->residues, nq, st)) return r;
+    if (int r = upload_codes(b->q, queries->residues, nq, st)) return r;
     if (b->has_bias) {
         if (int r = upload_guarded(b->bias, queries->bias, nq, st)) return r;
     }
     if (subjects) {
         const uint64_t ns = subjects->count ? subjects->offsets[subjects->count] : 0;
-        if (int r = upload_guarded(b->s, subjects->residues, ns, st)) return r;
+        if (int r = upload_codes(b->s, subjects->residues, ns, st)) return r;
         b->sres = b->s.p + kGuard;
         b->slen_total = ns;
     } else {
@@ -434,12 +542,13 @@ int saber_hip_band_batch_create(saber_hip_ctx* c, const saber_hip_seqset* querie
     if (int r = upload(b->mat, scheme->matrix, 576, st)) return r;
     if (int r = upload(b->jobs, b->host_jobs.data(), njobs, st)) return r;
     if (int r = upload(b->pairs, pairs.data(), pairs.size(), st)) return r;
+    if (int r = upload(b->blocks, b->host_blocks.data(), b->host_blocks.size(), st)) return r;
     HIP_TRY(b->flagged.alloc(njobs ? njobs : 1));
     HIP_TRY(b->nflagged.alloc(1));
     HIP_TRY(b->score.alloc(njobs ? njobs : 1));
     HIP_TRY(b->q_end.alloc(njobs ? njobs : 1));
     HIP_TRY(b->s_end.alloc(njobs ? njobs : 1));
-    HIP_TRY(b->cells.alloc(njobs ? njobs : 1));
+    if (int r = upload(b->cells, host_cells.data(), njobs, st)) return r;
     HIP_TRY(hipStreamSynchronize(st));
     *out = b.release();
     return SABER_HIP_OK;
@@ -477,47 +586,32 @@ static int run_generic(saber_hip_band_batch* b, const std::vector<uint32_t>& ids
 }
 
 // Enqueue everything up to (not including) the rescore of saturated jobs.
-static int enqueue_main(saber_hip_band_batch* b, hipEvent_t* ev_variant_begin, hipEvent_t* ev_variant_end) {
+static int enqueue_main(saber_hip_band_batch* b, hipEvent_t ev_begin, hipEvent_t ev_end) {
     hipStream_t st = b->ctx->stream;
     const uint64_t n = b->njobs;
     HIP_TRY(fill_u32(reinterpret_cast<uint32_t*>(b->score.p), n, 0, st));
     HIP_TRY(fill_u32(b->q_end.p, n, 0, st));
     HIP_TRY(fill_u32(b->s_end.p, n, 0, st));
-    HIP_TRY(fill_u32(reinterpret_cast<uint32_t*>(b->cells.p), 2 * n, 0, st));
     HIP_TRY(fill_u32(b->nflagged.p, 1, 0, st));
-    // Fork: the work streams start after the clears on the main stream.
-    saber_hip_ctx* c = b->ctx;
-    HIP_TRY(hipEventRecord(c->fork, st));
-    for (int k = 0; k < kStreams; ++k) HIP_TRY(hipStreamWaitEvent(c->work[k], c->fork, 0));
-    int nv = 0;
-    const BandVariant* var = band_variants(&nv);
-    for (size_t i = 0; i < b->launches.size(); ++i) {
-        const VarLaunch& L = b->launches[i];
-        hipStream_t ws = c->work[L.stream];
-        BandKernelArgs a{};
-        a.pairs = b->pairs.p + 2 * L.pair_off;
-        a.npairs = L.npairs;
-        a.jobs = b->jobs.p;
-        a.qres = b->q.p + kGuard;
-        a.qbias = b->has_bias ? b->bias.p + kGuard : nullptr;
-        a.sres = b->sres;
-        a.qres_len = b->qlen_total;
-        a.sres_len = b->slen_total;
-        a.table = b->table.p;
-        a.go = b->go;
-        a.ge = b->ge;
-        a.out = BandOutputs{b->score.p, b->q_end.p, b->s_end.p, b->cells.p};
-        a.flagged = b->flagged.p;
-        a.nflagged = b->nflagged.p;
-        if (ev_variant_begin) HIP_TRY(hipEventRecord(ev_variant_begin[i], ws));
-        HIP_TRY(var[L.variant].launch(a, ws));
-        if (ev_variant_end) HIP_TRY(hipEventRecord(ev_variant_end[i], ws));
-    }
-    // Join: the main stream continues after every work stream.
-    for (int k = 0; k < kStreams; ++k) {
-        HIP_TRY(hipEventRecord(c->join[k], c->work[k]));
-        HIP_TRY(hipStreamWaitEvent(st, c->join[k], 0));
-    }
+    BandKernelArgs a{};
+    a.pairs = b->pairs.p;
+    a.npairs = (uint32_t)(b->pairs.n / 2);
+    a.blocks = b->blocks.p;
+    a.jobs = b->jobs.p;
+    a.qres = b->q.p + kGuard;
+    a.qbias = b->has_bias ? b->bias.p + kGuard : nullptr;
+    a.sres = b->sres;
+    a.qres_len = b->qlen_total;
+    a.sres_len = b->slen_total;
+    a.table = b->table.p;
+    a.go = b->go;
+    a.ge = b->ge;
+    a.out = BandOutputs{b->score.p, b->q_end.p, b->s_end.p, b->cells.p};
+    a.flagged = b->flagged.p;
+    a.nflagged = b->nflagged.p;
+    if (ev_begin) HIP_TRY(hipEventRecord(ev_begin, st));
+    HIP_TRY(launch_band_multi(a, (uint32_t)b->host_blocks.size(), b->has_bias, st));
+    if (ev_end) HIP_TRY(hipEventRecord(ev_end, st));
     if (int r = run_generic(b, b->generic_static)) return r;
     return SABER_HIP_OK;
 }
@@ -580,7 +674,7 @@ void saber_hip_band_batch_destroy(saber_hip_band_batch* b) {
 }
 
 int saber_hip_band_batch_time(saber_hip_band_batch* b, int warmup, int iters, double* ms_run,
-                              double* kernel_ms, uint64_t* kernel_cells, int* nvar,
+                              double* band_ms, uint64_t* variant_cells, int* nvar,
                               uint64_t* cells) {
     if (!b || iters < 1 || warmup < 0) return fail(SABER_HIP_EINVAL, "bad arguments");
     int nv = 0;
@@ -591,8 +685,7 @@ int saber_hip_band_batch_time(saber_hip_band_batch* b, int warmup, int iters, do
     std::lock_guard<std::mutex> g(b->ctx->mu);
     HIP_TRY(hipSetDevice(b->ctx->device));
     hipStream_t st = b->ctx->stream;
-    const size_t nl = b->launches.size();
-    std::vector<hipEvent_t> eb(nl * iters), ee(nl * iters);
+    std::vector<hipEvent_t> eb(iters), ee(iters);
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
@@ -601,7 +694,7 @@ int saber_hip_band_batch_time(saber_hip_band_batch* b, int warmup, int iters, do
     HIP_TRY(hipStreamSynchronize(st));
     HIP_TRY(hipEventRecord(e0, st));
     for (int it = 0; it < iters; ++it) {
-        if (int r = enqueue_main(b, eb.data() + it * nl, ee.data() + it * nl)) return r;
+        if (int r = enqueue_main(b, eb[it], ee[it])) return r;
         if (int r = finish_flagged(b)) return r;
     }
     HIP_TRY(hipEventRecord(e1, st));
@@ -610,38 +703,35 @@ int saber_hip_band_batch_time(saber_hip_band_batch* b, int warmup, int iters, do
     float tot = 0.f;
     HIP_TRY(hipEventElapsedTime(&tot, e0, e1));
     if (ms_run) *ms_run = tot / iters;
-    if (kernel_ms) {
-        for (int v = 0; v < cap; ++v) kernel_ms[v] = 0.0;
-        for (int it = 0; it < iters; ++it)
-            for (size_t i = 0; i < nl; ++i) {
-                float ms = 0.f;
-                HIP_TRY(hipEventElapsedTime(&ms, eb[it * nl + i], ee[it * nl + i]));
-                const int v = b->launches[i].variant;
-                if (v < cap) kernel_ms[v] += ms / iters;
-            }
+    if (band_ms) {
+        double sum = 0.0;
+        for (int it = 0; it < iters; ++it) {
+            float ms = 0.f;
+            HIP_TRY(hipEventElapsedTime(&ms, eb[it], ee[it]));
+            sum += ms;
+        }
+        *band_ms = sum / iters;
     }
     if (nvar) *nvar = nv;
     for (auto& e : eb) (void)hipEventDestroy(e);
     for (auto& e : ee) (void)hipEventDestroy(e);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    if (cells || kernel_cells) {
+    if (cells || variant_cells) {
         std::vector<uint64_t> c(b->njobs);
-        if (b->njobs) {
-            HIP_TRY(hipMemcpy(c.data(), b->cells.p, 8 * b->njobs, hipMemcpyDeviceToHost));
-        }
+        if (b->njobs) HIP_TRY(hipMemcpy(c.data(), b->cells.p, 8 * b->njobs, hipMemcpyDeviceToHost));
         uint64_t t = 0;
         for (uint64_t x : c) t += x;
         if (cells) *cells = t;
-        if (kernel_cells) {
-            std::vector<uint32_t> pairs(2 * (b->pairs.n ? b->pairs.n / 2 : 0));
-            if (!pairs.empty())
-                HIP_TRY(hipMemcpy(pairs.data(), b->pairs.p, 4 * pairs.size(), hipMemcpyDeviceToHost));
-            for (int v = 0; v < cap; ++v) kernel_cells[v] = 0;
-            for (const VarLaunch& L : b->launches) {
-                if (L.variant >= cap) continue;
-                for (uint64_t k = 2 * L.pair_off; k < 2 * (L.pair_off + L.npairs); ++k)
-                    if (pairs[k] != kNone) kernel_cells[L.variant] += c[pairs[k]];
+        if (variant_cells) {
+            std::vector<uint32_t> pairs(b->pairs.n);
+            if (!pairs.empty()) HIP_TRY(hipMemcpy(pairs.data(), b->pairs.p, 4 * pairs.size(), hipMemcpyDeviceToHost));
+            for (int v = 0; v < cap; ++v) variant_cells[v] = 0;
+            for (const BandBlock& k : b->host_blocks) {
+                const int v = 2 * (int)k.shape + (b->has_bias ? 1 : 0);
+                if (v >= cap) continue;
+                for (uint64_t x = 2 * (uint64_t)k.pair_off; x < 2 * ((uint64_t)k.pair_off + k.npairs); ++x)
+                    if (pairs[x] != kNone) variant_cells[v] += c[pairs[x]];
             }
         }
     }
@@ -774,13 +864,13 @@ int saber_hip_xdrop_batch_create(saber_hip_ctx* c, const saber_hip_seqset* queri
                 (short)scheme->matrix[(a < 24 ? a : kCodeX) * 24 + (cc < 24 ? cc : kCodeX)];
     hipStream_t st = c->stream;
     const uint64_t nq = queries->count ? queries->offsets[queries->count] : 0;
-    if (int r = upload_guarded(b->q, queries->residues, nq, st)) return r;
+    if (int r = upload_codes(b->q, queries->residues, nq, st)) return r;
     if (b->has_bias)
         if (int r = upload_guarded(b->bias, queries->bias, nq, st)) return r;
     if (int r = upload(b->qoff, queries->offsets, (uint64_t)queries->count + 1, st)) return r;
     if (subjects) {
         const uint64_t ns = subjects->count ? subjects->offsets[subjects->count] : 0;
-        if (int r = upload_guarded(b->s, subjects->residues, ns, st)) return r;
+        if (int r = upload_codes(b->s, subjects->residues, ns, st)) return r;
         if (int r = upload(b->soff, subjects->offsets, (uint64_t)subjects->count + 1, st)) return r;
         b->sres = b->s.p + kGuard;
         b->soffp = b->soff.p;
@@ -1164,11 +1254,11 @@ int saber_hip_full_batch_create(saber_hip_ctx* c, const saber_hip_seqset* querie
 
     hipStream_t st = c->stream;
     const uint64_t nqres = b->hq.back();
-    if (int r = upload_guarded(b->q, queries->residues, nqres, st)) return r;
+    if (int r = upload_codes(b->q, queries->residues, nqres, st)) return r;
     if (b->has_bias)
         if (int r = upload_guarded(b->bias, queries->bias, nqres, st)) return r;
     if (subjects) {
-        if (int r = upload_guarded(b->s, subjects->residues, b->hs.back(), st)) return r;
+        if (int r = upload_codes(b->s, subjects->residues, b->hs.back(), st)) return r;
         b->sres = b->s.p + kGuard;
     } else {
         if (!c->db.p && b->ns > 0) return fail(SABER_HIP_ESTATE, "no database uploaded");

@@ -2,51 +2,71 @@
 //
 // Semantics: exactly banded_score (src/algorithm/banded.rs:134-224, 408-449):
 // the best score of a local alignment constrained to the diagonal band
-// lo <= j - i <= lo + w - 1, the first row-major cell holding it (exclusive
-// ends), and the number of band cells inside the rectangle.
+// lo <= j - i <= lo + w - 1 and the first row-major cell holding it
+// (exclusive ends). The band-cell count of each job depends only on its
+// geometry and is filled in by the host planner.
 //
 // Layout of the work:
 //  * Two independent jobs share each lane: every DP value is a pair of int16
 //    (job A in the low half, job B in the high half) and every DP operation is
-//    one v_pk_{add,sub,max,min}_i16 with clamp.
+//    one packed VALU op (v_pk_{add,max,min}_i16, v_pk_sub_u16 with clamp).
 //  * The DP runs row by row (query position i outer). The band cells of a row
 //    live in registers in band coordinates (cell k <-> subject column
 //    j = i + lo + k), so the diagonal predecessor is the same register and
 //    the vertical predecessor the next one: no shuffles inside a lane.
 //  * L lanes cooperate on one job pair, each owning C consecutive band cells.
 //    Lane l works on row t - l at step t (a one-row lag per lane), so the
-//    only cross-lane values are the left neighbour's (H, E) from the previous
-//    step and the right neighbour's (H, F) of its first cell of the current
-//    step, both moved with DPP row shifts.
-//  * Substitution scores come from a 32 x 32 int16 table in LDS indexed by the
-//    query code (per row) and the subject code (per cell, kept as pre-doubled
-//    bytes in a register window that shifts by one column per row). Column 31
-//    is a sentinel for subject positions outside the window: its score
-//    (-32768) makes every cell left of the rectangle exactly 0 (the border
-//    value the scalar kernel uses) and keeps cells right of it from ever
-//    influencing a real cell.
-//  * Cells beyond the job's clipped width (k >= w) are capped to -32768 for H
-//    and F; only the last K cells of a band can be beyond, so K cap registers
-//    suffice.
+//    only cross-lane values are the left neighbour's E from the previous step
+//    and the right neighbour's downward F from the current step, each moved
+//    with one DPP row shift.
+//  * Gap states are kept clamped at zero: E' = max(E, 0), F' = max(F, 0).
+//    With H >= 0 this leaves H = max(0, D, E, F) = max(D, E', F') unchanged and
+//    turns each "x - gap" into one unsigned saturating subtract. H - open is
+//    formed once per cell and feeds both the next cell's E and the next row's
+//    F, which each cell stores as the value leaving it downwards.
+//  * Steps run in blocks of four. Each lane keeps its subject columns as a
+//    window of bytes that covers a whole block (C + 3 columns), so the byte of
+//    cell c at step u of a block is at the fixed position c + u; the window
+//    moves by one dword per block. Query codes, bias and new subject columns
+//    arrive as dword streams loaded two blocks ahead.
+//  * Substitution scores come from an int8 table in LDS (24 query rows x 25
+//    columns: the 24 subject codes and a sentinel). Residue buffers are
+//    canonical codes (anything >= 24 stored as X) and subject columns outside
+//    the window read as the sentinel, which scores -128: a cell left of the
+//    rectangle then has a negative diagonal term and zero gap states, so it is
+//    exactly 0 (the border value of the scalar kernel), and a cell right of
+//    it only receives values from cells before it in row-major order, each
+//    lowered (diagonal: by at least 1 even with a +127 bias), so it never
+//    exceeds or comes first among the real cells.
+//  * Cells beyond the job's clipped width (k >= w) are held at 0: their H and
+//    hence their downward F is 0, which is what "outside the band" means for
+//    clamped gap states. Only the last K cells of a band can be beyond.
 //  * Exactness: every H/E/F of a local alignment is bounded by the optimum; the
 //    only upward saturation is H_diag + S + bias > 32767, which forces that
 //    cell's H to 32767. A job whose best reaches 32767 is therefore flagged and
 //    rescored by the exact int32 kernel; every other job is exact.
-//  * End cell: each lane keeps the best of its own cells and snapshots its C
-//    cells (to private scratch) whenever that best strictly improves; at the
-//    end the group takes the global best, the earliest row that reached it and
-//    the leftmost cell of that row.
+//  * End cell: each lane keeps the best of its own cells per job, the row where
+//    that best last rose, and a register snapshot of the row's cells taken
+//    with one bit-select per cell (v_bfi_b32 on the halves that rose). At the
+//    end the group takes the global best, the earliest row that reached it
+//    and the leftmost cell of that row.
 #pragma once
 #include "common.h"
 
 namespace saber {
 
 typedef short v2s __attribute__((ext_vector_type(2)));
+typedef unsigned short v2u __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ v2s vadd(v2s a, v2s b) { return __builtin_elementwise_add_sat(a, b); }
 __device__ __forceinline__ v2s vsub(v2s a, v2s b) { return __builtin_elementwise_sub_sat(a, b); }
 __device__ __forceinline__ v2s vmax(v2s a, v2s b) { return __builtin_elementwise_max(a, b); }
 __device__ __forceinline__ v2s vmin(v2s a, v2s b) { return __builtin_elementwise_min(a, b); }
+// max(a - b, 0) for a, b in [0, 32767].
+__device__ __forceinline__ v2s vsubz(v2s a, v2s b) {
+    return __builtin_bit_cast(v2s, __builtin_elementwise_sub_sat(__builtin_bit_cast(v2u, a),
+                                                                 __builtin_bit_cast(v2u, b)));
+}
 __device__ __forceinline__ uint32_t bits(v2s v) { return __builtin_bit_cast(uint32_t, v); }
 __device__ __forceinline__ v2s pair(uint32_t v) { return __builtin_bit_cast(v2s, v); }
 
@@ -61,38 +81,49 @@ __device__ __forceinline__ uint32_t from_next_lane(uint32_t v) {
 
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 
-// Raw residue at window column j, read from a clamped (always valid) address
-// so the load never sits behind a branch; subject_code() turns it into the
-// pre-doubled code, or the sentinel when j is outside [0, n).
-__device__ __forceinline__ uint32_t subject_raw(const uint8_t* s, int64_t j, uint32_t n) {
-    int64_t jc = j < 0 ? 0 : j;
-    const int64_t top = n > 0 ? (int64_t)n - 1 : 0;
-    jc = jc > top ? top : jc;
-    return s[jc];
-}
-__device__ __forceinline__ uint32_t subject_code(uint32_t raw, int64_t j, uint32_t n) {
-    if (j < 0 || j >= (int64_t)n) return 2u * kSentinel;
-    return 2u * (raw < 24 ? raw : (uint32_t)kCodeX);
+// A byte stream of one sequence: dword loads at base + o for 32-bit offsets o
+// relative to the sequence, clamped into the guard bytes every residue/bias
+// buffer carries (16 before the data, 64 after), so each load is valid; bytes
+// that do not belong to the sequence are replaced or never used by the caller.
+struct ByteStream {
+    const uint8_t* base;
+    int32_t lo, hi;
+    __device__ __forceinline__ void init(const uint8_t* buf, int64_t off, uint64_t len) {
+        base = buf + off;
+        const int64_t l = -16 - off, h = (int64_t)len + 60 - off;
+        lo = (int32_t)(l < INT32_MIN ? INT32_MIN : l);
+        hi = (int32_t)(h > INT32_MAX ? INT32_MAX : h);
+    }
+    __device__ __forceinline__ uint32_t load(int32_t o) const {
+        o = o < lo ? lo : (o > hi ? hi : o);
+        uint32_t v;
+        __builtin_memcpy(&v, base + o, 4);
+        return v;
+    }
+};
+
+// Subject bytes of window columns j0..j0+3 with columns outside [0, n)
+// replaced by the sentinel code. neg8 = -8 * j0 and hi8 = 8 * (n - j0).
+__device__ __forceinline__ uint32_t window_dword(uint32_t raw, int32_t neg8, int32_t hi8) {
+    const int32_t lo_bits = neg8 < 0 ? 0 : (neg8 > 32 ? 32 : neg8);
+    const int32_t hi_bits = hi8 < 0 ? 0 : (hi8 > 32 ? 32 : hi8);
+    const uint64_t ones = 0xFFFFFFFFull;
+    const uint32_t keep = (uint32_t)(ones << lo_bits) & (uint32_t)((ones << hi_bits) >> 32);
+    return (raw & keep) | (kSentinel4 & ~keep);
 }
 
+// One group of L lanes scoring job pair g (lane l of the group); Tb is the
+// score table in LDS.
 template <int L, int C, int K, bool BIAS>
-__global__ __launch_bounds__(256) void band_pair_kernel(BandKernelArgs a) {
+__device__ __forceinline__ void band_pair(const BandKernelArgs& a, const int8_t* Tb, uint32_t g, int l) {
     static_assert(C >= 2 && K >= 1 && K <= C, "variant shape");
     static_assert(L == 1 || L == 2 || L == 4 || L == 8 || L == 16, "groups must tile a DPP row");
-    constexpr int NW = (C + 3) / 4;
-    __shared__ short T[kTableDim * kTableDim];
-    for (int t = threadIdx.x; t < kTableDim * kTableDim; t += blockDim.x) T[t] = a.table[t];
-    __syncthreads();
-    const char* Tb = reinterpret_cast<const char*>(T);
-
-    const uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t g = gl / L;
-    const int l = (int)(gl % L);
-    if (g >= a.npairs) return;
+    constexpr int NW = (C + 6) / 4;  // window dwords: C + 3 columns per block
+    constexpr uint32_t W = (uint32_t)(L * C);
 
     uint32_t jid[2];
     int64_t qoff[2], soff[2];
-    uint32_t m[2], n[2];
+    uint32_t n[2];
     int lo[2], w[2], ifirst[2], rows[2];
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
@@ -101,7 +132,6 @@ __global__ __launch_bounds__(256) void band_pair_kernel(BandKernelArgs a) {
             const BandJobDev J = a.jobs[jid[h]];
             qoff[h] = (int64_t)J.q_off;
             soff[h] = (int64_t)J.s_off;
-            m[h] = J.m;
             n[h] = J.n;
             lo[h] = J.lo;
             w[h] = J.w;
@@ -110,7 +140,6 @@ __global__ __launch_bounds__(256) void band_pair_kernel(BandKernelArgs a) {
         } else {
             qoff[h] = 0;
             soff[h] = 0;
-            m[h] = 1;
             n[h] = 0;
             lo[h] = 0;
             w[h] = 1;
@@ -119,214 +148,153 @@ __global__ __launch_bounds__(256) void band_pair_kernel(BandKernelArgs a) {
         }
     }
     const int R = rows[0] > rows[1] ? rows[0] : rows[1];
-    const uint8_t* qp[2] = {a.qres + qoff[0], a.qres + qoff[1]};
-    const uint8_t* sp[2] = {a.sres + soff[0], a.sres + soff[1]};
-    const int8_t* bp[2] = {nullptr, nullptr};
-    if constexpr (BIAS) {
-        bp[0] = a.qbias + qoff[0];
-        bp[1] = a.qbias + qoff[1];
-    }
 
     const v2s GO = {(short)a.go, (short)a.go};
     const v2s GE = {(short)a.ge, (short)a.ge};
-    const v2s NEG = {kNeg16, kNeg16};
     const v2s ZERO = {0, 0};
 
-    // Caps for the last K cells of this lane: cell k stays if k < w, else -inf.
+    // Caps for the last K cells of this lane: cell k keeps its value if k < w,
+    // else it is held at 0.
     v2s cap[K];
 #pragma unroll
     for (int x = 0; x < K; ++x) {
         const int k = l * C + (C - K) + x;
-        cap[x] = v2s{k < w[0] ? kMax16 : kNeg16, k < w[1] ? kMax16 : kNeg16};
+        cap[x] = v2s{k < w[0] ? kMax16 : (short)0, k < w[1] ? kMax16 : (short)0};
     }
 
     v2s H[C], F[C];
+    uint32_t snap[C];
 #pragma unroll
     for (int c = 0; c < C; ++c) {
         H[c] = ZERO;
-        F[c] = NEG;
+        F[c] = ZERO;
+        snap[c] = 0;
     }
 
-    // Subject window of this lane for row 0 (pre-doubled codes, 4 per word).
+    // At step t lane l computes row r = t - l; its cell c is window column
+    // jb + t + c and its row is query position ifirst + t - l. Block b (steps
+    // 4b .. 4b+3) reads window columns jb + 4b .. jb + 4b + 4 NW - 1.
+    ByteStream ss[2], qs[2], bs[2];
+    int32_t jb[2], qb[2], neg8[2], hi8[2];
     uint32_t Wd[2][NW];
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-#pragma unroll
-        for (int x = 0; x < NW; ++x) Wd[h][x] = 0;
-        const int64_t j0 = (int64_t)ifirst[h] + lo[h] + l * C;
-        uint32_t raw[C];
-#pragma unroll
-        for (int c = 0; c < C; ++c) raw[c] = subject_raw(sp[h], j0 + c, n[h]);
-#pragma unroll
-        for (int c = 0; c < C; ++c) Wd[h][c >> 2] |= subject_code(raw[c], j0 + c, n[h]) << (8 * (c & 3));
-    }
-
-    // Per-lane byte streams for the row this lane prepares next (row k = r + 1
-    // at step r): the subject column entering the window, the query code and
-    // the bias. Each is a dword FIFO (cur, n1, n2) refilled every 4 rows with
-    // an unaligned load issued 8 rows ahead, so row k's bytes were requested
-    // two refills earlier. Device buffers carry 16 guard bytes in front and 64
-    // behind, and every load address is clamped into them; bytes from outside
-    // a sequence are replaced (sentinel column / garbage rows never used).
-    auto load4 = [](const uint8_t* buf, int64_t off, uint64_t len) -> uint32_t {
-        const int64_t top = (int64_t)len + 60;
-        off = off < -16 ? -16 : (off > top ? top : off);
-        uint32_t v;
-        __builtin_memcpy(&v, buf + off, 4);
-        return v;
-    };
-    const int64_t jcol0[2] = {(int64_t)ifirst[0] + lo[0] + l * C + (C - 1),
-                              (int64_t)ifirst[1] + lo[1] + l * C + (C - 1)};
-    uint32_t s_cur[2], s_n1[2], s_n2[2], q_cur[2], q_n1[2], q_n2[2];
+    uint32_t s_n1[2], s_n2[2], q_cur[2], q_n1[2], q_n2[2];
     uint32_t b_cur[2] = {0, 0}, b_n1[2] = {0, 0}, b_n2[2] = {0, 0};
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
-        const int64_t sj = soff[h] + jcol0[h], qi = qoff[h] + ifirst[h];
-        s_cur[h] = load4(a.sres, sj + 1, a.sres_len);
-        s_n1[h] = load4(a.sres, sj + 5, a.sres_len);
-        s_n2[h] = load4(a.sres, sj + 9, a.sres_len);
-        q_cur[h] = load4(a.qres, qi + 1, a.qres_len);
-        q_n1[h] = load4(a.qres, qi + 5, a.qres_len);
-        q_n2[h] = load4(a.qres, qi + 9, a.qres_len);
+        jb[h] = ifirst[h] + lo[h] + l * C - l;
+        qb[h] = ifirst[h] - l;
+        ss[h].init(a.sres, soff[h], a.sres_len);
+        qs[h].init(a.qres, qoff[h], a.qres_len);
+#pragma unroll
+        for (int x = 0; x < NW; ++x) {
+            const int32_t j0 = jb[h] + 4 * x;
+            Wd[h][x] = window_dword(ss[h].load(j0), -8 * j0, 8 * ((int32_t)n[h] - j0));
+        }
+        // Column of the dword appended after block 0, as -8j and 8(n - j).
+        neg8[h] = -8 * (jb[h] + 4 * NW);
+        hi8[h] = 8 * ((int32_t)n[h] - jb[h] - 4 * NW);
+        s_n1[h] = ss[h].load(jb[h] + 4 * NW);
+        s_n2[h] = ss[h].load(jb[h] + 4 * NW + 4);
+        q_cur[h] = qs[h].load(qb[h]);
+        q_n1[h] = qs[h].load(qb[h] + 4);
+        q_n2[h] = qs[h].load(qb[h] + 8);
         if constexpr (BIAS) {
-            const uint8_t* bb = reinterpret_cast<const uint8_t*>(a.qbias);
-            b_cur[h] = load4(bb, qi + 1, a.qres_len);
-            b_n1[h] = load4(bb, qi + 5, a.qres_len);
-            b_n2[h] = load4(bb, qi + 9, a.qres_len);
+            bs[h].init(reinterpret_cast<const uint8_t*>(a.qbias), qoff[h], a.qres_len);
+            b_cur[h] = bs[h].load(qb[h]);
+            b_n1[h] = bs[h].load(qb[h] + 4);
+            b_n2[h] = bs[h].load(qb[h] + 8);
         }
     }
 
-    int cbest[2] = {0, 0};
-    int csnap_row[2] = {-1, -1};
-    uint32_t snap[2][C];
-    uint64_t cells[2] = {0, 0};
-    uint32_t outHl = bits(NEG), outE = bits(NEG);
+    // Best of this lane's cells (both jobs) and the row where it last rose.
+    v2s cbest = ZERO;
+    uint32_t brow = 0;
+    uint32_t outE = 0;
 
-    // Row data of row 0 of this lane.
-    uint32_t rb[2];
-    v2s bpair = ZERO;
+    const int nblocks = (R + L - 1 + 3) / 4;
+    for (int blk = 0; blk < nblocks; ++blk) {
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        const uint32_t c = qp[h][ifirst[h] < (int)m[h] ? ifirst[h] : 0];
-        rb[h] = (c < 24 ? c : (uint32_t)kCodeX) * 64u;
-    }
-    if constexpr (BIAS) {
-        bpair = v2s{(short)bp[0][ifirst[0] < (int)m[0] ? ifirst[0] : 0],
-                    (short)bp[1][ifirst[1] < (int)m[1] ? ifirst[1] : 0]};
-    }
-
-    const int nsteps = R + L - 1;
-    for (int t = 0; t < nsteps; ++t) {
-        const int r = t - l;
-        const bool act = (r >= 0) && (r < R);
-        v2s inHl = NEG, inE = NEG;
-        if constexpr (L > 1) {
-            inHl = pair(from_prev_lane(outHl));
-            inE = pair(from_prev_lane(outE));
-            if (l == 0) {
-                inHl = NEG;
-                inE = NEG;
+        for (int u = 0; u < 4; ++u) {
+            const int r = 4 * blk + u - l;
+            const bool act = (unsigned)r < (unsigned)R;
+            const uint32_t rb0 = ((q_cur[0] >> (8 * u)) & 0xFFu) * kTable8Stride;
+            const uint32_t rb1 = ((q_cur[1] >> (8 * u)) & 0xFFu) * kTable8Stride;
+            v2s bpair = ZERO;
+            if constexpr (BIAS)
+                bpair = v2s{(short)(int8_t)(b_cur[0] >> (8 * u)), (short)(int8_t)(b_cur[1] >> (8 * u))};
+            v2s inE = ZERO;
+            if constexpr (L > 1) {
+                inE = pair(from_prev_lane(outE));
+                if (l == 0) inE = ZERO;
             }
-        }
-        v2s Hl = NEG, E = NEG, rm = ZERO;
-        auto cell = [&](int c, v2s hup, v2s fup, v2s el, v2s hl, bool first) {
-            const uint32_t ba = (Wd[0][c >> 2] >> (8 * (c & 3))) & 0xFFu;
-            const uint32_t bb = (Wd[1][c >> 2] >> (8 * (c & 3))) & 0xFFu;
-            const short sa = *reinterpret_cast<const short*>(Tb + rb[0] + ba);
-            const short sb = *reinterpret_cast<const short*>(Tb + rb[1] + bb);
-            v2s P = v2s{sa, sb};
-            if constexpr (BIAS) P = vadd(P, bpair);
-            const v2s D = vadd(H[c], P);
-            v2s Fn = vmax(vsub(hup, GO), vsub(fup, GE));
-            v2s En = first ? vmax(vsub(hl, GO), vsub(el, GE)) : vmax(vsub(Hl, GO), vsub(E, GE));
-            v2s hv = vmax(vmax(D, En), vmax(Fn, ZERO));
-            if (c >= C - K) {
-                hv = vmin(hv, cap[c - (C - K)]);
-                Fn = vmin(Fn, cap[c - (C - K)]);
-            }
-            H[c] = hv;
-            F[c] = Fn;
-            Hl = hv;
-            E = En;
-            rm = vmax(rm, hv);
-        };
+            v2s E = ZERO, rm = ZERO;
+            // One cell: diagonal H[c], E from the left, F arriving from above.
+            auto cell = [&](int c, v2s fin) {
+                const int p = c + u;
+                const uint32_t ca = (Wd[0][p >> 2] >> (8 * (p & 3))) & 0xFFu;
+                const uint32_t cb = (Wd[1][p >> 2] >> (8 * (p & 3))) & 0xFFu;
+                v2s P = v2s{(short)Tb[rb0 + ca], (short)Tb[rb1 + cb]};
+                if constexpr (BIAS) P = vadd(P, bpair);
+                const v2s D = vadd(H[c], P);
+                v2s hv = vmax(vmax(D, fin), E);
+                if (c >= C - K) hv = vmin(hv, cap[c - (C - K)]);
+                const v2s hg = vsubz(hv, GO);
+                E = vmax(hg, vsubz(E, GE));
+                F[c] = vmax(hg, vsubz(fin, GE));
+                H[c] = hv;
+                rm = vmax(rm, hv);
+            };
 
-        if (act) {
-            // Cell 0: its left neighbour is the previous lane's last cell.
-            if constexpr (C > 1) cell(0, H[1], F[1], inE, inHl, true);
-        }
-        v2s upH = NEG, upF = NEG;
-        if constexpr (L > 1) {
-            upH = pair(from_next_lane(bits(H[0])));
-            upF = pair(from_next_lane(bits(F[0])));
-            if (l == L - 1) {
-                upH = NEG;
-                upF = NEG;
+            if (act) {
+                // Cell 0: its left neighbour is the previous lane's last cell.
+                E = inE;
+                cell(0, F[1]);
             }
-        }
-        if (act) {
+            v2s upF = ZERO;
+            if constexpr (L > 1) {
+                upF = pair(from_next_lane(bits(F[0])));
+                if (l == L - 1) upF = ZERO;
+            }
+            if (act) {
 #pragma unroll
-            for (int c = 1; c < C - 1; ++c) cell(c, H[c + 1], F[c + 1], NEG, NEG, false);
-            cell(C - 1, upH, upF, NEG, NEG, false);
-            outHl = bits(Hl);
-            outE = bits(E);
+                for (int c = 1; c < C - 1; ++c) cell(c, F[c + 1]);
+                cell(C - 1, upF);
+                outE = bits(E);
 
+                // Rows past a job's own rows do not count for it.
+                const v2s live = v2s{r < rows[0] ? kMax16 : (short)0, r < rows[1] ? kMax16 : (short)0};
+                rm = vmin(rm, live);
+                // Halves whose best rose: all-ones (sign of cbest - rm).
+                uint32_t rose = bits(vsub(cbest, rm) >> 15);
+                // Opaque to the optimiser, so the selects below stay one
+                // v_bfi_b32 per cell instead of per-half selects.
+                asm volatile("" : "+v"(rose));
+                if (rose != 0) {
+                    cbest = vmax(cbest, rm);
+                    brow = (rose & ((uint32_t)r * 0x10001u)) | (~rose & brow);
 #pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                if (r < rows[h]) {
-                    const int i = ifirst[h] + r;
-                    const int rmh = h == 0 ? (int)rm.x : (int)rm.y;
-                    if (rmh > cbest[h]) {
-                        cbest[h] = rmh;
-                        csnap_row[h] = i;
-#pragma unroll
-                        for (int c = 0; c < C; ++c) snap[h][c] = bits(H[c]);
-                    }
-                    if (l == 0) {
-                        int kmin = -(i + lo[h]);
-                        if (kmin < 0) kmin = 0;
-                        int kmax = (int)n[h] - 1 - i - lo[h];
-                        if (kmax > w[h] - 1) kmax = w[h] - 1;
-                        cells[h] += (uint64_t)(kmax - kmin + 1);
-                    }
+                    for (int c = 0; c < C; ++c) snap[c] = (rose & bits(H[c])) | (~rose & snap[c]);
                 }
             }
-            // Next row's data from the streams; shift the windows one column
-            // and append the new column.
-            const int k = r + 1;
-            short bn[2] = {0, 0};
+        }
+        // Next block: the window moves one dword, the streams one dword.
+        const int32_t nb = 4 * blk;
 #pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                const uint32_t sraw = s_cur[h] & 0xFFu, qc = q_cur[h] & 0xFFu;
-                if constexpr (BIAS) bn[h] = (short)(int8_t)(b_cur[h] & 0xFFu);
-                s_cur[h] >>= 8;
-                q_cur[h] >>= 8;
-                if constexpr (BIAS) b_cur[h] >>= 8;
+        for (int h = 0; h < 2; ++h) {
 #pragma unroll
-                for (int x = 0; x < NW - 1; ++x) Wd[h][x] = __builtin_amdgcn_alignbyte(Wd[h][x + 1], Wd[h][x], 1);
-                constexpr int b = (C - 1) & 3;
-                uint32_t last = Wd[h][NW - 1] >> 8;
-                if constexpr (b > 0) last &= (1u << (8 * b)) - 1u;
-                Wd[h][NW - 1] = last | (subject_code(sraw, jcol0[h] + k, n[h]) << (8 * b));
-                rb[h] = (qc < 24 ? qc : (uint32_t)kCodeX) * 64u;
-            }
-            if constexpr (BIAS) bpair = v2s{bn[0], bn[1]};
-            if ((k & 3) == 0) {
-#pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    s_cur[h] = s_n1[h];
-                    s_n1[h] = s_n2[h];
-                    s_n2[h] = load4(a.sres, soff[h] + jcol0[h] + k + 9, a.sres_len);
-                    q_cur[h] = q_n1[h];
-                    q_n1[h] = q_n2[h];
-                    q_n2[h] = load4(a.qres, qoff[h] + ifirst[h] + k + 9, a.qres_len);
-                    if constexpr (BIAS) {
-                        b_cur[h] = b_n1[h];
-                        b_n1[h] = b_n2[h];
-                        b_n2[h] = load4(reinterpret_cast<const uint8_t*>(a.qbias),
-                                        qoff[h] + ifirst[h] + k + 9, a.qres_len);
-                    }
-                }
+            for (int x = 0; x < NW - 1; ++x) Wd[h][x] = Wd[h][x + 1];
+            Wd[h][NW - 1] = window_dword(s_n1[h], neg8[h], hi8[h]);
+            neg8[h] -= 32;
+            hi8[h] -= 32;
+            s_n1[h] = s_n2[h];
+            s_n2[h] = ss[h].load(jb[h] + 4 * NW + nb + 8);
+            q_cur[h] = q_n1[h];
+            q_n1[h] = q_n2[h];
+            q_n2[h] = qs[h].load(qb[h] + nb + 12);
+            if constexpr (BIAS) {
+                b_cur[h] = b_n1[h];
+                b_n1[h] = b_n2[h];
+                b_n2[h] = bs[h].load(qb[h] + nb + 12);
             }
         }
     }
@@ -334,22 +302,24 @@ __global__ __launch_bounds__(256) void band_pair_kernel(BandKernelArgs a) {
     // Group reduction: global best, earliest row reaching it, leftmost cell.
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
-        int best = cbest[h];
+        const int mine = h == 0 ? (int)cbest.x : (int)cbest.y;
+        int best = mine;
 #pragma unroll
         for (int d = 1; d < L; d <<= 1) {
             const int o = __shfl_xor(best, d, L);
             best = o > best ? o : best;
         }
         uint32_t key = 0xFFFFFFFFu;
-        if (best > 0 && best < kMax16 && cbest[h] == best) {
-            for (int c = 0; c < C; ++c) {
-                const uint32_t v = snap[h][c];
+        if (best > 0 && best < kMax16 && mine == best) {
+            const uint32_t row = h == 0 ? (brow & 0xFFFFu) : (brow >> 16);
+            int first = C;
+#pragma unroll
+            for (int c = C - 1; c >= 0; --c) {
+                const uint32_t v = snap[c];
                 const int hv = h == 0 ? (int)(short)(v & 0xFFFFu) : (int)(short)(v >> 16);
-                if (hv == best) {
-                    key = (uint32_t)csnap_row[h] * (uint32_t)(L * C) + (uint32_t)(l * C + c);
-                    break;
-                }
+                first = hv == best ? c : first;
             }
+            if (first < C) key = row * W + (uint32_t)(l * C + first);
         }
 #pragma unroll
         for (int d = 1; d < L; d <<= 1) {
@@ -364,28 +334,49 @@ __global__ __launch_bounds__(256) void band_pair_kernel(BandKernelArgs a) {
             } else {
                 uint32_t qe = 0, se = 0;
                 if (best > 0) {
-                    const uint32_t row = key / (uint32_t)(L * C);
-                    const uint32_t k = key % (uint32_t)(L * C);
-                    qe = row + 1;
-                    se = (uint32_t)((int)row + lo[h] + (int)k + 1);
+                    const uint32_t row = key / W;
+                    const uint32_t k = key % W;
+                    const int i = ifirst[h] + (int)row;
+                    qe = (uint32_t)i + 1;
+                    se = (uint32_t)(i + lo[h] + (int)k + 1);
                 }
                 a.out.score[id] = best;
                 a.out.q_end[id] = qe;
                 a.out.s_end[id] = se;
-                a.out.cells[id] = cells[h];
             }
         }
     }
 }
 
-template <int L, int C, int K, bool BIAS>
-hipError_t launch_band(const BandKernelArgs& a, hipStream_t s) {
-    if (a.npairs == 0) return hipSuccess;
-    const uint64_t threads = (uint64_t)a.npairs * L;
-    const uint32_t block = 256;
-    const uint32_t grid = (uint32_t)((threads + block - 1) / block);
-    hipLaunchKernelGGL((band_pair_kernel<L, C, K, BIAS>), dim3(grid), dim3(block), 0, s, a);
-    return hipGetLastError();
+// Every compiled shape (L, C, K), in variant-table order.
+#define SABER_BAND_SHAPES(X)                                                            \
+    X(1, 9, 9) X(1, 13, 13) X(1, 17, 8) X(1, 17, 17)                                    \
+    X(2, 9, 9) X(2, 13, 13) X(2, 17, 8) X(2, 17, 17)                                    \
+    X(4, 9, 9) X(4, 13, 13) X(4, 17, 8) X(4, 17, 17)                                    \
+    X(8, 9, 9) X(8, 13, 13) X(8, 17, 8) X(8, 17, 17)                                    \
+    X(16, 9, 9) X(16, 13, 13) X(16, 17, 16) X(16, 17, 17) X(16, 21, 21)
+
+// All shapes in one launch: block b scores the pairs blocks[b] names with the
+// shape it names, so the planner orders the blocks of every shape together
+// (longest first) and the hardware dispatcher packs them. At least two waves
+// per SIMD: the compiler must fit the largest shape into 256 registers.
+template <bool BIAS>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) void band_multi_kernel(BandKernelArgs a) {
+    __shared__ uint32_t T4[kTable8Words];
+    for (int t = threadIdx.x; t < kTable8Words; t += blockDim.x) T4[t] = a.table[t];
+    __syncthreads();
+    const int8_t* Tb = reinterpret_cast<const int8_t*>(T4);
+    const BandBlock blk = a.blocks[blockIdx.x];
+    int id = 0;
+#define SABER_BAND_CASE(L_, C_, K_)                                                     \
+    if (blk.shape == (uint32_t)id) {                                                    \
+        const uint32_t q = threadIdx.x / (L_);                                          \
+        if (q < blk.npairs) band_pair<L_, C_, K_, BIAS>(a, Tb, blk.pair_off + q, (int)(threadIdx.x % (L_))); \
+        return;                                                                         \
+    }                                                                                   \
+    ++id;
+    SABER_BAND_SHAPES(SABER_BAND_CASE)
+#undef SABER_BAND_CASE
 }
 
 }  // namespace saber

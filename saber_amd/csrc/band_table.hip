@@ -1,28 +1,28 @@
-// Concatenates the compiled band kernel variants into one table.
+// Variant table of the packed band kernel (every compiled shape, without and
+// with bias) and the launcher of the multi-shape kernel.
 #include <vector>
 
-#include "common.h"
+#include "band_kernel.h"
 
 namespace saber {
 
-const BandVariant* band_variants_part_a(int* count);
-const BandVariant* band_variants_part_b(int* count);
-const BandVariant* band_variants_part_c(int* count);
-const BandVariant* band_variants_part_d(int* count);
+hipError_t launch_band_multi_nobias(const BandKernelArgs& a, uint32_t nblocks, hipStream_t s);
+hipError_t launch_band_multi_bias(const BandKernelArgs& a, uint32_t nblocks, hipStream_t s);
 
 const BandVariant* band_variants(int* count) {
     static const std::vector<BandVariant> table = [] {
         std::vector<BandVariant> t;
-        for (auto f : {band_variants_part_a, band_variants_part_b, band_variants_part_c,
-                       band_variants_part_d}) {
-            int n = 0;
-            const BandVariant* p = f(&n);
-            t.insert(t.end(), p, p + n);
-        }
+#define SABER_BAND_ENTRY(L_, C_, K_) t.push_back({L_, C_, K_, 0}); t.push_back({L_, C_, K_, 1});
+        SABER_BAND_SHAPES(SABER_BAND_ENTRY)
+#undef SABER_BAND_ENTRY
         return t;
     }();
     *count = (int)table.size();
     return table.data();
+}
+
+hipError_t launch_band_multi(const BandKernelArgs& a, uint32_t nblocks, bool bias, hipStream_t s) {
+    return bias ? launch_band_multi_bias(a, nblocks, s) : launch_band_multi_nobias(a, nblocks, s);
 }
 
 }  // namespace saber

@@ -15,6 +15,14 @@ constexpr int kTableDim = 32;        // 32 x 32 int16 score table in LDS
 constexpr short kNeg16 = -32768;     // unreachable (saturating arithmetic keeps it)
 constexpr short kMax16 = 32767;      // any job whose best reaches this is rescored in int32
 
+// Band kernel score table: int8, 24 query rows x 25 columns (subject codes
+// 0..23 and the sentinel column 24, scoring -128), staged into LDS as words.
+constexpr int kBandSentinel = 24;
+constexpr uint32_t kSentinel4 = 0x18181818u;  // four sentinel bytes
+constexpr uint32_t kTable8Stride = 25;
+constexpr int kTable8Words = (24 * 25 + 3) / 4;
+constexpr uint32_t kMaxBandRows = 32767;      // rows of one packed job (row index kept in 16 bits)
+
 // One banded job after planning (band clipped to its rectangle, rows that
 // hold band cells). Band cell k of row i is subject column j = i + lo + k.
 struct BandJobDev {
@@ -34,17 +42,24 @@ struct BandOutputs {
     uint64_t* cells;
 };
 
+// One block of the multi-shape band launch: pairs [pair_off, pair_off +
+// npairs) of the pair list, scored with shape `shape`.
+struct BandBlock {
+    uint32_t shape, pair_off, npairs, pad;
+};
+
 struct BandKernelArgs {
     const uint32_t* pairs;      // 2 job ids per pair (0xFFFFFFFF = none)
     uint32_t npairs;
+    const BandBlock* blocks;    // multi-shape launch: one entry per block
     const BandJobDev* jobs;
     const uint8_t* qres;        // device buffers with 16 guard bytes before and
     const int8_t* qbias;        // 64 after (qbias may be null without bias)
     const uint8_t* sres;
     uint64_t qres_len, sres_len;
-    const short* table;         // 32 x 32 int16 (global; staged into LDS)
+    const uint32_t* table;      // int8 band table as words (global; staged into LDS)
     int32_t go, ge;
-    BandOutputs out;
+    BandOutputs out;            // score, q_end, s_end (cells come from the planner)
     uint32_t* flagged;          // job ids needing the exact int32 rescore
     uint32_t* nflagged;
 };
@@ -65,15 +80,16 @@ struct GenericArgs {
 };
 
 // One compiled variant of the packed band kernel: L lanes per job pair,
-// C band cells per lane, K capped trailing cells (band width W = L*C covers
-// widths W-K+1 .. W).
+// C band cells per lane, K capped trailing cells per lane (band width W = L*C
+// covers widths W-K+1 .. W, every width up to W when K = C), with or without
+// the query bias. Variant v is shape v / 2 of the multi-shape kernel.
 struct BandVariant {
     int L, C, K;
     int bias;
-    hipError_t (*launch)(const BandKernelArgs&, hipStream_t);
 };
 
 const BandVariant* band_variants(int* count);
+hipError_t launch_band_multi(const BandKernelArgs& a, uint32_t nblocks, bool bias, hipStream_t s);
 hipError_t launch_generic(const GenericArgs& a, hipStream_t s);
 
 }  // namespace saber

@@ -25,6 +25,20 @@
 
 namespace saber {
 
+// Raw residue at subject column j, read from a clamped (always valid) address
+// so the load never sits behind a branch; subject_code() turns it into the
+// pre-doubled code, or the sentinel when j is outside [0, n).
+__device__ __forceinline__ uint32_t subject_raw(const uint8_t* s, int64_t j, uint32_t n) {
+    int64_t jc = j < 0 ? 0 : j;
+    const int64_t top = n > 0 ? (int64_t)n - 1 : 0;
+    jc = jc > top ? top : jc;
+    return s[jc];
+}
+__device__ __forceinline__ uint32_t subject_code(uint32_t raw, int64_t j, uint32_t n) {
+    if (j < 0 || j >= (int64_t)n) return 2u * kSentinel;
+    return 2u * (raw < 24 ? raw : (uint32_t)kCodeX);
+}
+
 // Lane i receives lane i-1: within a 16-lane row for groups that tile rows,
 // across the whole wave (wave_shr:1) for 32- and 64-lane groups.
 template <int L>

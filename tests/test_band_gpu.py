@@ -170,14 +170,19 @@ def test_edge_cases(ctx):
     s2 = np.array([255, 255], np.uint8)
     got, want = run_both(ctx, [q, q], [s1, s2], [-5, -5], [5, 2], BLOSUM62, 11, 1)
     assert_same(got, want, "codes")
+    # more rows than the packed kernel keeps (32767): the int32 route
+    a = rand_seq(Lcg(5), 33000)
+    b = mutate(Lcg(6), a)
+    got, want = run_both(ctx, [a, a[:300]], [b, b[:300]], [-3, -3], [3, 3], BLOSUM62, 11, 1)
+    assert_same(got, want, "long rows")
 
 
 def test_every_variant_width(ctx):
-    """Each width 1..300 (all packed variants and the int32 route) on related
-    pairs long enough to fill the band."""
+    """Each width 1..340 (all packed variants and the int32 route above 328)
+    on related pairs long enough to fill the band."""
     r = Lcg(123)
     qs, ss, lo, hi = [], [], [], []
-    for w in range(1, 301):
+    for w in range(1, 341):
         for _ in range(2):
             m = 60 + r.next(200)
             a = rand_seq(r, m)
@@ -189,6 +194,38 @@ def test_every_variant_width(ctx):
             lo.append(c - w // 2); hi.append(c - w // 2 + w - 1)
     got, want = run_both(ctx, qs, ss, lo, hi, BLOSUM62, 11, 1)
     assert_same(got, want, "widths")
+
+
+def test_each_compiled_variant():
+    """Every compiled packed variant (with and without bias), forced over the
+    widths it serves, on related pairs: the planner's cost model leaves some
+    of them unused by default."""
+    r = Lcg(77)
+    shapes = sorted({(L, C, K) for L, C, K, _ in S.band_variants()})
+    for L, C, K in shapes:
+        W = L * C
+        lo_w = 1 if K == C else W - K + 1
+        widths = sorted({lo_w, (lo_w + W) // 2, W})
+        ctx = S.Context()
+        ctx.choose(lo_w, W, L, C, K)
+        qs, ss, lo, hi, bias = [], [], [], [], []
+        for w in widths:
+            for t in range(6):
+                # long enough that the band is not clipped to a narrower width
+                m = W // 2 + 40 + r.next(200)
+                a = rand_seq(r, m)
+                b = mutate(r, a) if t % 3 else rand_seq(r, m + r.next(40))
+                if len(b) < W // 2 + 20:
+                    b = rand_seq(r, m)
+                c = int(r.next(31)) - 15
+                qs.append(a); ss.append(b)
+                lo.append(c - w // 2); hi.append(c - w // 2 + w - 1)
+                bias.append(np.array([r.next(9) - 4 for _ in range(m)], np.int8))
+        got, want = run_both(ctx, qs, ss, lo, hi, BLOSUM62, 11, 1)
+        assert_same(got, want, f"variant {(L, C, K)}")
+        got, want = run_both(ctx, qs, ss, lo, hi, BLOSUM62, 11, 1, bias=bias)
+        assert_same(got, want, f"variant {(L, C, K)} bias")
+        ctx.close()
 
 
 def test_region_workload_sample(ctx):
