@@ -38,6 +38,23 @@ VALU_I16X2_TOPS = 2 * VALU_I32_TOPS
 OPS_PER_CELL = 10
 
 
+def measured_traffic(args):
+    """HBM bytes per launch of the band kernel for this workload, from the
+    rocprofv3 FETCH_SIZE / WRITE_SIZE runs summarised under profiles/ (each
+    counter scaled by what it reads for a plain copy of known size), or None
+    when no measurement of this exact workload is recorded."""
+    path = os.path.join(ROOT, "profiles", "band_multi_kernel_1M_regions.traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f)
+    except (OSError, ValueError):
+        return None
+    wl = rec.get("workload", {})
+    if (wl.get("regions"), wl.get("seed"), wl.get("per_query")) != (args.regions, args.seed, args.per_query):
+        return None
+    return int(rec["traffic_bytes"])
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -220,7 +237,7 @@ def main():
             "peak": round(VALU_I16X2_TOPS, 2),
             "unit": "Tops/s (int16 ops, 10 per cell)",
             "frac": round(achieved / VALU_I16X2_TOPS, 4),
-            "traffic": None,
+            "traffic": measured_traffic(args),
         },
         "cpu_baseline": cpu,
         "extra": {"xdrop": xd, "full_sw": fl},
