@@ -218,9 +218,50 @@ int saber_hip_band_choose(saber_hip_ctx* ctx, int w_lo, int w_hi, int L, int C, 
 
 /* Planner tuning (no counterpart in the reference): band jobs with more than
  * latency_rows rows are planned for low per-row latency (short lanes, their
- * own launches, started first); 0 turns this off. Applies to batches created
- * afterwards. */
+ * blocks dispatched first); 0 turns this off, -1 restores the default rule
+ * (jobs whose rows alone would take a large share of the batch's estimated
+ * time). Applies to batches created afterwards. */
 int saber_hip_band_tune(saber_hip_ctx* ctx, int latency_rows);
+
+/* ---- Learned region prefilter (no function in the reference) ----
+ * The model the scope defines for the region prefilter (it plugs in between
+ * the region cap and score_windows, src/search/engine/scan.rs:1476-1485 and
+ * src/search/engine.rs:806-809): a 64 -> 256 -> 256 -> 1 MLP with ReLU,
+ * bf16 weights and activations, fp32 accumulation (bf16 MFMA), trained with
+ * the class-weighted logistic loss
+ *     L = (1/n) sum_r w_r (softplus(y_r) - t_r y_r),  w_r = pos_weight if t_r = 1 else 1,
+ * and Adam. Parameters are fp32 on the host side, in one array of
+ * SABER_HIP_MLP_NPARAMS floats: W1[64][256], b1[256], W2[256][256], b2[256],
+ * w3[256], b3 (h1 = relu(x W1 + b1), h2 = relu(h1 W2 + b2), y = h2 . w3 + b3).
+ * Features are bf16 bit patterns (uint16), row-major [n][64]. */
+#define SABER_HIP_MLP_IN 64
+#define SABER_HIP_MLP_HID 256
+#define SABER_HIP_MLP_NPARAMS (64 * 256 + 256 + 256 * 256 + 256 + 256 + 1)
+typedef struct saber_hip_mlp saber_hip_mlp;
+/* params may be NULL: He-uniform initialisation from `seed`. */
+int saber_hip_mlp_create(saber_hip_ctx* ctx, const float* params, uint64_t seed, saber_hip_mlp** out);
+int saber_hip_mlp_get_params(saber_hip_mlp* m, float* params);
+int saber_hip_mlp_set_params(saber_hip_mlp* m, const float* params);   /* also resets Adam state */
+/* logits[n] of host features x[n][64]. */
+int saber_hip_mlp_forward(saber_hip_mlp* m, const uint16_t* x, uint64_t n, float* logits);
+/* Gradient of L over the batch (same layout as the parameters) and L itself;
+ * parameters unchanged. */
+int saber_hip_mlp_backward(saber_hip_mlp* m, const uint16_t* x, const float* labels, uint64_t n,
+                           float pos_weight, float* grads, float* loss);
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) on the batch; loss may be NULL. */
+int saber_hip_mlp_train_step(saber_hip_mlp* m, const uint16_t* x, const float* labels, uint64_t n,
+                             float pos_weight, float lr, float* loss);
+/* Device-resident data set: upload once, then train / score slices of it
+ * without host traffic. */
+int saber_hip_mlp_upload(saber_hip_mlp* m, const uint16_t* x, const float* labels, uint64_t n);
+int saber_hip_mlp_train_resident(saber_hip_mlp* m, uint64_t row0, uint64_t n, float pos_weight,
+                                 float lr, float* loss);
+int saber_hip_mlp_forward_resident(saber_hip_mlp* m, uint64_t row0, uint64_t n, float* logits);
+/* Timing (HIP events on the context stream) of forward and of one training
+ * step over rows [0, n) of the resident data: mean ms per call. */
+int saber_hip_mlp_time(saber_hip_mlp* m, uint64_t n, int warmup, int iters, double* fwd_ms,
+                       double* step_ms);
+void saber_hip_mlp_destroy(saber_hip_mlp* m);
 
 #ifdef __cplusplus
 }

@@ -139,6 +139,7 @@ class BandBatch:
     def __init__(self, ctx, handle, njobs, keep):
         self.ctx, self.h, self.njobs, self._keep = ctx, handle, njobs, keep
         _live_batches.add(self)
+        ctx._children.add(self)
 
     def run(self):
         _ffi.check(_ffi.lib().saber_hip_band_batch_run(self.h))
@@ -189,6 +190,7 @@ class FullBatch:
     def __init__(self, ctx, handle, nq, ns, keep):
         self.ctx, self.h, self.nq, self.ns, self._keep = ctx, handle, nq, ns, keep
         _live_batches.add(self)
+        ctx._children.add(self)
 
     def run(self):
         _ffi.check(_ffi.lib().saber_hip_full_batch_run(self.h))
@@ -229,6 +231,7 @@ class XdropBatch:
     def __init__(self, ctx, handle, nhits, keep):
         self.ctx, self.h, self.nhits, self._keep = ctx, handle, nhits, keep
         _live_batches.add(self)
+        ctx._children.add(self)
 
     def run(self):
         _ffi.check(_ffi.lib().saber_hip_xdrop_batch_run(self.h))
@@ -283,11 +286,15 @@ class Context:
         opts = _ffi.Opts()
         opts.flags = _ffi.FORCE_EXACT32 if force_exact32 else 0
         h = C.c_void_p()
+        self._children = weakref.WeakSet()   # batches and models on this context
         _ffi.check(L.saber_hip_create(device, C.byref(opts), C.byref(h)))
         self.h = h
         _live_contexts.add(self)
 
     def close(self):
+        """Release the context, after every batch and model created on it."""
+        for ch in list(getattr(self, "_children", ())):
+            ch.close()
         if self.h:
             _ffi.lib().saber_hip_destroy(self.h)
             self.h = None

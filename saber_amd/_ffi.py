@@ -95,6 +95,21 @@ _SIGS = {
                                          C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "saber_hip_band_choose": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
     "saber_hip_band_tune": (C.c_int, [P, C.c_int]),
+    "saber_hip_mlp_create": (C.c_int, [P, C.POINTER(C.c_float), C.c_uint64, C.POINTER(P)]),
+    "saber_hip_mlp_get_params": (C.c_int, [P, C.POINTER(C.c_float)]),
+    "saber_hip_mlp_set_params": (C.c_int, [P, C.POINTER(C.c_float)]),
+    "saber_hip_mlp_forward": (C.c_int, [P, C.POINTER(C.c_uint16), C.c_uint64, C.POINTER(C.c_float)]),
+    "saber_hip_mlp_backward": (C.c_int, [P, C.POINTER(C.c_uint16), C.POINTER(C.c_float), C.c_uint64,
+                                         C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "saber_hip_mlp_train_step": (C.c_int, [P, C.POINTER(C.c_uint16), C.POINTER(C.c_float), C.c_uint64,
+                                           C.c_float, C.c_float, C.POINTER(C.c_float)]),
+    "saber_hip_mlp_upload": (C.c_int, [P, C.POINTER(C.c_uint16), C.POINTER(C.c_float), C.c_uint64]),
+    "saber_hip_mlp_train_resident": (C.c_int, [P, C.c_uint64, C.c_uint64, C.c_float, C.c_float,
+                                               C.POINTER(C.c_float)]),
+    "saber_hip_mlp_forward_resident": (C.c_int, [P, C.c_uint64, C.c_uint64, C.POINTER(C.c_float)]),
+    "saber_hip_mlp_time": (C.c_int, [P, C.c_uint64, C.c_int, C.c_int, C.POINTER(C.c_double),
+                                     C.POINTER(C.c_double)]),
+    "saber_hip_mlp_destroy": (None, [P]),
     "saber_hip_copy_probe": (C.c_int, [P, C.c_uint64, C.c_int, C.POINTER(C.c_double)]),
 }
 

@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <atomic>
 #include <memory>
 #include <cstdio>
@@ -1409,3 +1410,306 @@ extern "C" int saber_hip_copy_probe(saber_hip_ctx* c, uint64_t bytes, int iters,
     if (ms) *ms = t / iters;
     return SABER_HIP_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Learned region prefilter: the MLP of mlp.hip behind the C ABI.
+
+namespace {
+
+// Offsets of the parameter blocks in the flat fp32 array (and of their
+// gradients in the gradient array, whose last entry holds the loss sum).
+constexpr uint64_t kPW1 = 0, kPb1 = 64 * 256, kPW2 = kPb1 + 256, kPb2 = kPW2 + 256 * 256,
+                   kPw3 = kPb2 + 256, kPb3 = kPw3 + 256, kNParams = kPb3 + 1;
+static_assert(kNParams == SABER_HIP_MLP_NPARAMS, "parameter layout");
+
+}  // namespace
+
+struct saber_hip_mlp {
+    saber_hip_ctx* ctx = nullptr;
+    DevBuf<float> p, g, am, av;        // parameters, gradients (+ loss), Adam moments
+    DevBuf<__bf16> W1T, W2P, W2Q;      // bf16 operand layouts of W1, W2
+    uint64_t steps = 0;                // Adam steps taken
+    uint64_t cap = 0;                  // rows the batch buffers hold
+    DevBuf<__bf16> X, H1, H2, dH1, dH2;
+    DevBuf<float> y, lab, dy;
+    DevBuf<__bf16> data_x;             // resident data set
+    DevBuf<float> data_y;
+    uint64_t data_n = 0;
+    int max_grid = 1;
+};
+
+namespace {
+
+MlpWeights mlp_weights(const saber_hip_mlp* m) {
+    return MlpWeights{m->W1T.p, m->W2P.p, m->W2Q.p, m->p.p + kPb1, m->p.p + kPb2, m->p.p + kPw3, m->p.p + kPb3};
+}
+
+MlpGrads mlp_grads(const saber_hip_mlp* m) {
+    return MlpGrads{m->g.p + kPW1, m->g.p + kPW2, m->g.p + kPb1, m->g.p + kPb2,
+                    m->g.p + kPw3, m->g.p + kPb3, m->g.p + kNParams};
+}
+
+int mlp_grid(const saber_hip_mlp* m, uint64_t n) {
+    const uint64_t tiles = (n + 15) / 16, blocks = (tiles + 3) / 4;
+    return (int)std::max<uint64_t>(1, std::min<uint64_t>(blocks, (uint64_t)m->max_grid));
+}
+
+int mlp_reserve(saber_hip_mlp* m, uint64_t n) {
+    if (n <= m->cap) return SABER_HIP_OK;
+    HIP_TRY(m->X.alloc(n * 64));
+    HIP_TRY(m->H1.alloc(n * 256));
+    HIP_TRY(m->H2.alloc(n * 256));
+    HIP_TRY(m->dH1.alloc(n * 256));
+    HIP_TRY(m->dH2.alloc(n * 256));
+    HIP_TRY(m->y.alloc(n));
+    HIP_TRY(m->lab.alloc(n));
+    HIP_TRY(m->dy.alloc(n));
+    m->cap = n;
+    return SABER_HIP_OK;
+}
+
+// Gradient of the batch loss into m->g (zeroed first); X, labels on device.
+int mlp_backward_dev(saber_hip_mlp* m, const __bf16* X, const float* labels, uint64_t n, float pos_weight) {
+    hipStream_t st = m->ctx->stream;
+    if (int r = mlp_reserve(m, n)) return r;
+    HIP_TRY(fill_u32(reinterpret_cast<uint32_t*>(m->g.p), kNParams + 1, 0, st));
+    const MlpGrads gr = mlp_grads(m);
+    HIP_TRY(mlp_launch_backward_act(mlp_weights(m), X, labels, n, pos_weight, 1.0f / (float)n, gr, m->H1.p,
+                                    m->H2.p, m->dH2.p, m->dH1.p, m->dy.p, mlp_grid(m, n), st));
+    // dW2 = H1^T dH2, dW1 = X^T dH1 (split over rows: about 1024 blocks each)
+    HIP_TRY(mlp_launch_gemm_tn(m->H1.p, 256, 256, m->dH2.p, 256, 256, gr.dW2, n, 64, st));
+    HIP_TRY(mlp_launch_gemm_tn(X, 64, 64, m->dH1.p, 256, 256, gr.dW1, n, 256, st));
+    HIP_TRY(mlp_launch_colsum(m->dH2.p, nullptr, n, gr.db2, st));
+    HIP_TRY(mlp_launch_colsum(m->dH1.p, nullptr, n, gr.db1, st));
+    HIP_TRY(mlp_launch_colsum(m->H2.p, m->dy.p, n, gr.dw3, st));
+    return SABER_HIP_OK;
+}
+
+int mlp_adam(saber_hip_mlp* m, float lr) {
+    hipStream_t st = m->ctx->stream;
+    m->steps += 1;
+    const double b1 = 0.9, b2 = 0.999;
+    const float c1 = (float)(1.0 - std::pow(b1, (double)m->steps));
+    const float c2 = (float)(1.0 - std::pow(b2, (double)m->steps));
+    HIP_TRY(mlp_launch_adam(m->p.p, m->g.p, m->am.p, m->av.p, (uint32_t)kNParams, lr, (float)b1, (float)b2,
+                            1e-8f, c1, c2, st));
+    HIP_TRY(mlp_launch_pack(m->p.p + kPW1, m->p.p + kPW2, m->W1T.p, m->W2P.p, m->W2Q.p, st));
+    return SABER_HIP_OK;
+}
+
+int mlp_read_loss(saber_hip_mlp* m, uint64_t n, float* loss) {
+    hipStream_t st = m->ctx->stream;
+    float sum = 0.f;
+    HIP_TRY(hipMemcpyAsync(&sum, m->g.p + kNParams, sizeof(float), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    if (loss) *loss = n ? sum / (float)n : 0.f;
+    return SABER_HIP_OK;
+}
+
+int mlp_set(saber_hip_mlp* m, const float* params) {
+    hipStream_t st = m->ctx->stream;
+    HIP_TRY(hipMemcpyAsync(m->p.p, params, sizeof(float) * kNParams, hipMemcpyHostToDevice, st));
+    HIP_TRY(fill_u32(reinterpret_cast<uint32_t*>(m->am.p), kNParams, 0, st));
+    HIP_TRY(fill_u32(reinterpret_cast<uint32_t*>(m->av.p), kNParams, 0, st));
+    HIP_TRY(mlp_launch_pack(m->p.p + kPW1, m->p.p + kPW2, m->W1T.p, m->W2P.p, m->W2Q.p, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    m->steps = 0;
+    return SABER_HIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int saber_hip_mlp_create(saber_hip_ctx* c, const float* params, uint64_t seed, saber_hip_mlp** out) {
+    if (!c || !out) return fail(SABER_HIP_EINVAL, "null argument");
+    *out = nullptr;
+    std::lock_guard<std::mutex> g(c->mu);
+    HIP_TRY(hipSetDevice(c->device));
+    auto m = std::make_unique<saber_hip_mlp>();
+    m->ctx = c;
+    HIP_TRY(m->p.alloc(kNParams));
+    HIP_TRY(m->g.alloc(kNParams + 1));
+    HIP_TRY(m->am.alloc(kNParams));
+    HIP_TRY(m->av.alloc(kNParams));
+    HIP_TRY(m->W1T.alloc(256 * 64));
+    HIP_TRY(m->W2P.alloc(256 * 256));
+    HIP_TRY(m->W2Q.alloc(256 * 256));
+    int cus = 0;
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+    m->max_grid = std::max(1, cus * 8);
+    std::vector<float> init;
+    if (!params) {
+        // He-uniform weights, zero biases (splitmix64 stream from seed)
+        init.assign(kNParams, 0.f);
+        uint64_t s = seed;
+        auto uni = [&]() {
+            s += 0x9E3779B97F4A7C15ull;
+            uint64_t z = s;
+            z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+            z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+            z ^= z >> 31;
+            return (double)(z >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0;
+        };
+        const double a1 = std::sqrt(6.0 / 64), a2 = std::sqrt(6.0 / 256);
+        for (uint64_t i = 0; i < 64 * 256; ++i) init[kPW1 + i] = (float)(a1 * uni());
+        for (uint64_t i = 0; i < 256 * 256; ++i) init[kPW2 + i] = (float)(a2 * uni());
+        for (uint64_t i = 0; i < 256; ++i) init[kPw3 + i] = (float)(a2 * uni());
+        params = init.data();
+    }
+    if (int r = mlp_set(m.get(), params)) return r;
+    *out = m.release();
+    return SABER_HIP_OK;
+}
+
+int saber_hip_mlp_get_params(saber_hip_mlp* m, float* params) {
+    if (!m || !params) return fail(SABER_HIP_EINVAL, "null argument");
+    std::lock_guard<std::mutex> g(m->ctx->mu);
+    HIP_TRY(hipSetDevice(m->ctx->device));
+    HIP_TRY(hipMemcpyAsync(params, m->p.p, sizeof(float) * kNParams, hipMemcpyDeviceToHost, m->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(m->ctx->stream));
+    return SABER_HIP_OK;
+}
+
+int saber_hip_mlp_set_params(saber_hip_mlp* m, const float* params) {
+    if (!m || !params) return fail(SABER_HIP_EINVAL, "null argument");
+    std::lock_guard<std::mutex> g(m->ctx->mu);
+    HIP_TRY(hipSetDevice(m->ctx->device));
+    return mlp_set(m, params);
+}
+
+int saber_hip_mlp_forward(saber_hip_mlp* m, const uint16_t* x, uint64_t n, float* logits) {
+    if (!m || (n && (!x || !logits))) return fail(SABER_HIP_EINVAL, "null argument");
+    if (n == 0) return SABER_HIP_OK;
+    std::lock_guard<std::mutex> g(m->ctx->mu);
+    HIP_TRY(hipSetDevice(m->ctx->device));
+    hipStream_t st = m->ctx->stream;
+    if (int r = mlp_reserve(m, n)) return r;
+    HIP_TRY(hipMemcpyAsync(m->X.p, x, 2 * 64 * n, hipMemcpyHostToDevice, st));
+    HIP_TRY(mlp_launch_forward(mlp_weights(m), m->X.p, n, m->y.p, mlp_grid(m, n), st));
+    HIP_TRY(hipMemcpyAsync(logits, m->y.p, sizeof(float) * n, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return SABER_HIP_OK;
+}
+
+int saber_hip_mlp_backward(saber_hip_mlp* m, const uint16_t* x, const float* labels, uint64_t n,
+                           float pos_weight, float* grads, float* loss) {
+    if (!m || n == 0 || !x || !labels) return fail(SABER_HIP_EINVAL, "bad arguments");
+    std::lock_guard<std::mutex> g(m->ctx->mu);
+    HIP_TRY(hipSetDevice(m->ctx->device));
+    hipStream_t st = m->ctx->stream;
+    if (int r = mlp_reserve(m, n)) return r;
+    HIP_TRY(hipMemcpyAsync(m->X.p, x, 2 * 64 * n, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(m->lab.p, labels, sizeof(float) * n, hipMemcpyHostToDevice, st));
+    if (int r = mlp_backward_dev(m, m->X.p, m->lab.p, n, pos_weight)) return r;
+    if (grads) HIP_TRY(hipMemcpyAsync(grads, m->g.p, sizeof(float) * kNParams, hipMemcpyDeviceToHost, st));
+    return mlp_read_loss(m, n, loss);
+}
+
+int saber_hip_mlp_train_step(saber_hip_mlp* m, const uint16_t* x, const float* labels, uint64_t n,
+                             float pos_weight, float lr, float* loss) {
+    if (!m || n == 0 || !x || !labels) return fail(SABER_HIP_EINVAL, "bad arguments");
+    std::lock_guard<std::mutex> g(m->ctx->mu);
+    HIP_TRY(hipSetDevice(m->ctx->device));
+    hipStream_t st = m->ctx->stream;
+    if (int r = mlp_reserve(m, n)) return r;
+    HIP_TRY(hipMemcpyAsync(m->X.p, x, 2 * 64 * n, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(m->lab.p, labels, sizeof(float) * n, hipMemcpyHostToDevice, st));
+    if (int r = mlp_backward_dev(m, m->X.p, m->lab.p, n, pos_weight)) return r;
+    if (int r = mlp_adam(m, lr)) return r;
+    return mlp_read_loss(m, n, loss);
+}
+
+int saber_hip_mlp_upload(saber_hip_mlp* m, const uint16_t* x, const float* labels, uint64_t n) {
+    if (!m || (n && !x)) return fail(SABER_HIP_EINVAL, "null argument");
+    std::lock_guard<std::mutex> g(m->ctx->mu);
+    HIP_TRY(hipSetDevice(m->ctx->device));
+    hipStream_t st = m->ctx->stream;
+    HIP_TRY(m->data_x.alloc(n ? n * 64 : 1));
+    HIP_TRY(m->data_y.alloc(n ? n : 1));
+    if (n) HIP_TRY(hipMemcpyAsync(m->data_x.p, x, 2 * 64 * n, hipMemcpyHostToDevice, st));
+    if (n && labels) HIP_TRY(hipMemcpyAsync(m->data_y.p, labels, sizeof(float) * n, hipMemcpyHostToDevice, st));
+    if (n && !labels) HIP_TRY(fill_u32(reinterpret_cast<uint32_t*>(m->data_y.p), n, 0, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    m->data_n = n;
+    return SABER_HIP_OK;
+}
+
+int saber_hip_mlp_train_resident(saber_hip_mlp* m, uint64_t row0, uint64_t n, float pos_weight, float lr,
+                                 float* loss) {
+    if (!m || n == 0 || row0 + n > m->data_n) return fail(SABER_HIP_EINVAL, "rows outside the resident data");
+    std::lock_guard<std::mutex> g(m->ctx->mu);
+    HIP_TRY(hipSetDevice(m->ctx->device));
+    if (int r = mlp_backward_dev(m, m->data_x.p + 64 * row0, m->data_y.p + row0, n, pos_weight)) return r;
+    if (int r = mlp_adam(m, lr)) return r;
+    if (!loss) return SABER_HIP_OK;
+    return mlp_read_loss(m, n, loss);
+}
+
+int saber_hip_mlp_forward_resident(saber_hip_mlp* m, uint64_t row0, uint64_t n, float* logits) {
+    if (!m || row0 + n > m->data_n) return fail(SABER_HIP_EINVAL, "rows outside the resident data");
+    if (n && !logits) return fail(SABER_HIP_EINVAL, "logits is null");
+    if (n == 0) return SABER_HIP_OK;
+    std::lock_guard<std::mutex> g(m->ctx->mu);
+    HIP_TRY(hipSetDevice(m->ctx->device));
+    hipStream_t st = m->ctx->stream;
+    if (int r = mlp_reserve(m, n)) return r;
+    HIP_TRY(mlp_launch_forward(mlp_weights(m), m->data_x.p + 64 * row0, n, m->y.p, mlp_grid(m, n), st));
+    HIP_TRY(hipMemcpyAsync(logits, m->y.p, sizeof(float) * n, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return SABER_HIP_OK;
+}
+
+int saber_hip_mlp_time(saber_hip_mlp* m, uint64_t n, int warmup, int iters, double* fwd_ms, double* step_ms) {
+    if (!m || n == 0 || n > m->data_n || iters < 1 || warmup < 0) return fail(SABER_HIP_EINVAL, "bad arguments");
+    std::lock_guard<std::mutex> g(m->ctx->mu);
+    HIP_TRY(hipSetDevice(m->ctx->device));
+    hipStream_t st = m->ctx->stream;
+    if (int r = mlp_reserve(m, n)) return r;
+    // Training steps change the weights: time them on a copy of the state.
+    std::vector<float> saved(kNParams), sm(kNParams), sv(kNParams);
+    HIP_TRY(hipMemcpyAsync(saved.data(), m->p.p, 4 * kNParams, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(sm.data(), m->am.p, 4 * kNParams, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(sv.data(), m->av.p, 4 * kNParams, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    const uint64_t saved_steps = m->steps;
+    hipEvent_t ev[4];
+    for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
+    const MlpWeights w = mlp_weights(m);
+    for (int i = 0; i < warmup; ++i) HIP_TRY(mlp_launch_forward(w, m->data_x.p, n, m->y.p, mlp_grid(m, n), st));
+    HIP_TRY(hipEventRecord(ev[0], st));
+    for (int i = 0; i < iters; ++i) HIP_TRY(mlp_launch_forward(w, m->data_x.p, n, m->y.p, mlp_grid(m, n), st));
+    HIP_TRY(hipEventRecord(ev[1], st));
+    for (int i = 0; i < iters + warmup; ++i) {
+        if (i == warmup) HIP_TRY(hipEventRecord(ev[2], st));
+        if (int r = mlp_backward_dev(m, m->data_x.p, m->data_y.p, n, 1.0f)) return r;
+        if (int r = mlp_adam(m, 0.0f)) return r;
+    }
+    HIP_TRY(hipEventRecord(ev[3], st));
+    HIP_TRY(hipEventSynchronize(ev[3]));
+    float t1 = 0.f, t2 = 0.f;
+    HIP_TRY(hipEventElapsedTime(&t1, ev[0], ev[1]));
+    HIP_TRY(hipEventElapsedTime(&t2, ev[2], ev[3]));
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    if (fwd_ms) *fwd_ms = t1 / iters;
+    if (step_ms) *step_ms = t2 / iters;
+    HIP_TRY(hipMemcpyAsync(m->p.p, saved.data(), 4 * kNParams, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(m->am.p, sm.data(), 4 * kNParams, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(m->av.p, sv.data(), 4 * kNParams, hipMemcpyHostToDevice, st));
+    HIP_TRY(mlp_launch_pack(m->p.p + kPW1, m->p.p + kPW2, m->W1T.p, m->W2P.p, m->W2Q.p, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    m->steps = saved_steps;
+    return SABER_HIP_OK;
+}
+
+void saber_hip_mlp_destroy(saber_hip_mlp* m) {
+    if (!m) return;
+    {
+        std::lock_guard<std::mutex> g(m->ctx->mu);
+        (void)hipSetDevice(m->ctx->device);
+        (void)hipStreamSynchronize(m->ctx->stream);
+    }
+    delete m;
+}
+
+}  // extern "C"

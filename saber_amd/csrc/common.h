@@ -162,7 +162,7 @@ struct MlpWeights {
     const float* b1;
     const float* b2;
     const float* w3;
-    float b3;
+    const float* b3;   // [1]
 };
 
 struct MlpGrads {

@@ -43,7 +43,7 @@ __device__ __forceinline__ bf16x8 load8(const __bf16* p) {
 //   W1T [256][64]   W1T[h][i] = W1[i][h]           (layer-1 A operand)
 //   W2P [256][256]  W2P[o][s*32 + 8g + t] = W2[perm_unit(s, g, t)][o]   (layer-2 A)
 //   W2Q [256][256]  W2Q[h][s*32 + 8g + t] = W2[h][perm_unit(s, g, t)]   (backward A)
-//   b1, b2, w3 fp32 [256], b3 fp32.
+//   b1, b2, w3 fp32 [256], b3 fp32 [1].
 
 // Forward for one 16-row tile held by one wave. Produces the logit of the
 // lane's row (valid in every lane; rows past n are computed on zeros) and,
@@ -115,7 +115,7 @@ __device__ __forceinline__ float forward_tile(const MlpWeights& w, const __bf16*
     // rows are lane % 16: sum the four lane groups
     y += __shfl_xor(y, 16);
     y += __shfl_xor(y, 32);
-    return y + w.b3;
+    return y + *w.b3;
 }
 
 __global__ __launch_bounds__(256) void mlp_forward_kernel(MlpWeights w, const __bf16* X, uint64_t n,
