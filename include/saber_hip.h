@@ -223,6 +223,50 @@ int saber_hip_band_choose(saber_hip_ctx* ctx, int w_lo, int w_hi, int L, int C, 
  * time). Applies to batches created afterwards. */
 int saber_hip_band_tune(saber_hip_ctx* ctx, int latency_rows);
 
+/* ---- Host pipeline: statistics thresholds (host code, no device) ----
+ * Karlin-Altschul parameters and per-query statistics, restating
+ * src/scoring/stats.rs: the two-stage band cut (floor_lower_bound, used by
+ * score_windows, src/search/engine.rs:1057-1068), the region floor
+ * (min_score, Engine::prepare engine.rs:759-786), E-values and bit scores
+ * (the reporting cut and the prefilter's labels). Optional bit-score floors
+ * are passed as NaN for "none". */
+typedef struct saber_hip_karlin {
+    double lambda, k, log_k, h;
+    double alpha_d_lambda, beta;     /* length adjustment (alpha / lambda, beta) */
+    int32_t calibrated;              /* published gapped parameters */
+    int32_t even_scores_only;
+} saber_hip_karlin;
+typedef struct saber_hip_query_stats {
+    double lambda, k, log_k;
+    int64_t length_adjustment;
+    double eff_query_len, eff_db_len, search_space;
+    int32_t even_scores_only;
+    int32_t per_subject;             /* 0: database model, 1: per-subject model */
+    int64_t query_len;
+    double num_seqs, alpha_d_lambda, beta;
+} saber_hip_query_stats;
+/* gapped_protein (stats.rs:351-396): matrix 62 (BLOSUM62) or 45 (BLOSUM45),
+ * gap costs in SABER convention (open = cost of the first gap residue). */
+int saber_hip_stats_gapped_protein(int matrix, int32_t gap_open, int32_t gap_extend, int allow_uncalibrated,
+                                   saber_hip_karlin* out);
+/* compute_length_adjustment (stats.rs:458-521), NCBI BLAST_ComputeLengthAdjustment. */
+int saber_hip_stats_length_adjustment(double k, double log_k, double alpha_d_lambda, double beta,
+                                      int64_t query_length, int64_t db_length, int64_t db_num_seqs,
+                                      int64_t* adj, int* converged);
+/* SearchStatistics::for_query (stats.rs:599-636). */
+int saber_hip_stats_for_query(const saber_hip_karlin* p, uint64_t db_length, uint64_t db_num_seqs,
+                              int per_subject, uint64_t query_len, saber_hip_query_stats* out);
+/* QueryStatistics methods (stats.rs:646-760). */
+double saber_hip_stats_space_for(const saber_hip_query_stats* q, uint64_t subject_len);
+double saber_hip_stats_space_lower_bound(const saber_hip_query_stats* q, uint64_t subject_len);
+double saber_hip_stats_floor_lower_bound(const saber_hip_query_stats* q, double evalue_cut, double min_bits,
+                                         uint64_t subject_len, int64_t db_floor);
+double saber_hip_stats_evalue_in(const saber_hip_query_stats* q, int64_t raw, double space);
+double saber_hip_stats_evalue_for(const saber_hip_query_stats* q, int64_t raw, uint64_t subject_len);
+double saber_hip_stats_evalue(const saber_hip_query_stats* q, int64_t raw);
+double saber_hip_stats_bitscore(const saber_hip_query_stats* q, int64_t raw);
+int64_t saber_hip_stats_min_score(const saber_hip_query_stats* q, double evalue_cut, double min_bits);
+
 /* ---- Learned region prefilter (no function in the reference) ----
  * The model the scope defines for the region prefilter (it plugs in between
  * the region cap and score_windows, src/search/engine/scan.rs:1476-1485 and

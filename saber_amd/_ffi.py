@@ -31,6 +31,20 @@ class SeqSet(C.Structure):
     ]
 
 
+class Karlin(C.Structure):
+    _fields_ = [("lambda_", C.c_double), ("k", C.c_double), ("log_k", C.c_double), ("h", C.c_double),
+                ("alpha_d_lambda", C.c_double), ("beta", C.c_double), ("calibrated", C.c_int32),
+                ("even_scores_only", C.c_int32)]
+
+
+class QueryStats(C.Structure):
+    _fields_ = [("lambda_", C.c_double), ("k", C.c_double), ("log_k", C.c_double),
+                ("length_adjustment", C.c_int64), ("eff_query_len", C.c_double), ("eff_db_len", C.c_double),
+                ("search_space", C.c_double), ("even_scores_only", C.c_int32), ("per_subject", C.c_int32),
+                ("query_len", C.c_int64), ("num_seqs", C.c_double), ("alpha_d_lambda", C.c_double),
+                ("beta", C.c_double)]
+
+
 class BandJob(C.Structure):
     _fields_ = [
         ("query", C.c_uint32),
@@ -95,6 +109,21 @@ _SIGS = {
                                          C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "saber_hip_band_choose": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
     "saber_hip_band_tune": (C.c_int, [P, C.c_int]),
+    "saber_hip_stats_gapped_protein": (C.c_int, [C.c_int, C.c_int32, C.c_int32, C.c_int, C.POINTER(Karlin)]),
+    "saber_hip_stats_length_adjustment": (C.c_int, [C.c_double, C.c_double, C.c_double, C.c_double, C.c_int64,
+                                                     C.c_int64, C.c_int64, C.POINTER(C.c_int64),
+                                                     C.POINTER(C.c_int)]),
+    "saber_hip_stats_for_query": (C.c_int, [C.POINTER(Karlin), C.c_uint64, C.c_uint64, C.c_int, C.c_uint64,
+                                            C.POINTER(QueryStats)]),
+    "saber_hip_stats_space_for": (C.c_double, [C.POINTER(QueryStats), C.c_uint64]),
+    "saber_hip_stats_space_lower_bound": (C.c_double, [C.POINTER(QueryStats), C.c_uint64]),
+    "saber_hip_stats_floor_lower_bound": (C.c_double, [C.POINTER(QueryStats), C.c_double, C.c_double, C.c_uint64,
+                                                       C.c_int64]),
+    "saber_hip_stats_evalue_in": (C.c_double, [C.POINTER(QueryStats), C.c_int64, C.c_double]),
+    "saber_hip_stats_evalue_for": (C.c_double, [C.POINTER(QueryStats), C.c_int64, C.c_uint64]),
+    "saber_hip_stats_evalue": (C.c_double, [C.POINTER(QueryStats), C.c_int64]),
+    "saber_hip_stats_bitscore": (C.c_double, [C.POINTER(QueryStats), C.c_int64]),
+    "saber_hip_stats_min_score": (C.c_int64, [C.POINTER(QueryStats), C.c_double, C.c_double]),
     "saber_hip_mlp_create": (C.c_int, [P, C.POINTER(C.c_float), C.c_uint64, C.POINTER(P)]),
     "saber_hip_mlp_get_params": (C.c_int, [P, C.POINTER(C.c_float)]),
     "saber_hip_mlp_set_params": (C.c_int, [P, C.POINTER(C.c_float)]),

@@ -35,12 +35,23 @@ def _compile(src):
     return obj
 
 
+def _compile_host(src):
+    obj = os.path.join(OBJ, os.path.basename(src) + ".o")
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", "-c", src, "-o", obj]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"g++ failed for {src}:\n{r.stderr[-4000:]}")
+    return obj
+
+
 def build(jobs=None, verbose=False):
     os.makedirs(OBJ, exist_ok=True)
     srcs = sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
+    # host-only parts of the library (the host pipeline around the kernels)
+    hsrcs = sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.startswith("host_") and f.endswith(".cpp"))
     jobs = jobs or min(8, os.cpu_count() or 1)
     with cf.ThreadPoolExecutor(jobs) as ex:
-        objs = list(ex.map(_compile, srcs))
+        objs = list(ex.map(_compile, srcs)) + list(ex.map(_compile_host, hsrcs))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
