@@ -267,6 +267,71 @@ double saber_hip_stats_evalue(const saber_hip_query_stats* q, int64_t raw);
 double saber_hip_stats_bitscore(const saber_hip_query_stats* q, int64_t raw);
 int64_t saber_hip_stats_min_score(const saber_hip_query_stats* q, double evalue_cut, double min_bits);
 
+/* ---- Host pipeline: hits -> HSPs -> regions -> plans -> scores ---- */
+typedef struct saber_hip_hsp {
+    int64_t diag;                    /* sp - qp */
+    uint32_t q_lo, q_hi;             /* query range of the extension (half-open) */
+    int32_t score;
+    uint32_t reserved;
+} saber_hip_hsp;
+typedef struct saber_hip_region {
+    uint64_t subject;
+    int64_t d_min, d_max;            /* diagonal range */
+    int32_t best;                    /* best HSP score */
+    uint32_t n_hsp;                  /* HSP evidence: count and saturating score sum */
+    int32_t hsp_sum;
+    uint32_t reserved;
+} saber_hip_region;
+typedef struct saber_hip_plan {
+    uint64_t lo, hi;                 /* subject window [lo, hi) */
+    int64_t band_lo, band_hi;        /* band relative to the window (banded plans) */
+    int32_t banded;
+    uint32_t reserved;
+} saber_hip_plan;
+typedef struct saber_hip_plan_params {
+    int64_t window_slack;            /* engine.rs:180 (64) */
+    int64_t score_band;              /* engine.rs:181 */
+    int32_t hsp_threshold;           /* engine.rs:171 (40) */
+    int32_t exhaustive;
+} saber_hip_plan_params;
+/* visit_scan (src/search/engine/scan.rs:2384-2431) for one (query, subject)
+ * group: hits (qp[k], sp[k]) in visit order, with the X-drop result of every
+ * hit already computed (saber_hip_xdrop: extensions are independent, so
+ * scoring all of them and replaying the coverage rule is exact). A hit is
+ * skipped when qp < the q_hi of the last extension on its diagonal; an
+ * extension with score >= hsp_threshold becomes an HSP. hsps must hold n. */
+int saber_hip_visit_hits(uint32_t qlen, uint32_t slen, const uint32_t* qp, const uint32_t* sp, uint64_t n,
+                         const int32_t* score, const uint32_t* q_lo, const uint32_t* q_hi, int32_t hsp_threshold,
+                         saber_hip_hsp* hsps, uint64_t* nhsp, uint64_t* extended, uint64_t* avoided);
+/* finish_subject (src/search/engine.rs:2350-2412): HSPs (sorted in place by
+ * diagonal) joined into regions within 2 * window_slack diagonals, kept when
+ * best >= region_thr or (>= 2 HSPs and sum >= region_thr) (only if
+ * region_thr > hsp_threshold), ordered by (best desc, d_min asc) and cut to
+ * max_regions_per_subject. out must hold n. */
+int saber_hip_finish_subject(uint64_t subject, saber_hip_hsp* hsps, uint64_t n, int64_t window_slack,
+                             int32_t hsp_threshold, int32_t region_thr, uint32_t max_regions_per_subject,
+                             saber_hip_region* out, uint32_t* nout);
+/* Per-query region cap (engine.rs:486-495): regions sorted by (best desc,
+ * subject, d_min, d_max); the first min(n, cap) are kept. */
+int saber_hip_cap_regions(saber_hip_region* regs, uint64_t n, uint64_t cap, uint64_t* nout);
+/* window + plan_windows (src/search/engine.rs:1223-1324): one plan per
+ * region (subject_len[i] = length of region i's subject). */
+int saber_hip_plan_windows(const saber_hip_region* regs, uint64_t n, uint64_t qlen, const uint64_t* subject_len,
+                           const saber_hip_plan_params* p, saber_hip_plan* out);
+/* Banded part of Engine::score_windows (src/search/engine.rs:1021-1099) over
+ * the regions of many queries in two GPU passes: region i of query qidx[i]
+ * with plan plans[i]. When 0 < narrow_band < full_band, banded plans are
+ * first scored in the narrow band [d_min - narrow_band, d_max + narrow_band]
+ * and only those reaching thr[i] (score_prefilter_frac x floor_lower_bound,
+ * from the statistics functions) are rescored in the plan's band
+ * (rescored[i] = 1); otherwise banded plans use the plan's band directly.
+ * Unbanded plans are scored over the whole query x window rectangle.
+ * subjects NULL = the uploaded database. */
+int saber_hip_score_regions(saber_hip_ctx* ctx, const saber_hip_seqset* queries, const saber_hip_seqset* subjects,
+                            const uint32_t* qidx, const saber_hip_region* regs, const saber_hip_plan* plans,
+                            uint64_t n, const saber_hip_scheme* scheme, int64_t narrow_band, int64_t full_band,
+                            const double* thr, int32_t* out, uint8_t* rescored);
+
 /* ---- Learned region prefilter (no function in the reference) ----
  * The model the scope defines for the region prefilter (it plugs in between
  * the region cap and score_windows, src/search/engine/scan.rs:1476-1485 and

@@ -45,6 +45,26 @@ class QueryStats(C.Structure):
                 ("beta", C.c_double)]
 
 
+class Hsp(C.Structure):
+    _fields_ = [("diag", C.c_int64), ("q_lo", C.c_uint32), ("q_hi", C.c_uint32), ("score", C.c_int32),
+                ("reserved", C.c_uint32)]
+
+
+class Region(C.Structure):
+    _fields_ = [("subject", C.c_uint64), ("d_min", C.c_int64), ("d_max", C.c_int64), ("best", C.c_int32),
+                ("n_hsp", C.c_uint32), ("hsp_sum", C.c_int32), ("reserved", C.c_uint32)]
+
+
+class Plan(C.Structure):
+    _fields_ = [("lo", C.c_uint64), ("hi", C.c_uint64), ("band_lo", C.c_int64), ("band_hi", C.c_int64),
+                ("banded", C.c_int32), ("reserved", C.c_uint32)]
+
+
+class PlanParams(C.Structure):
+    _fields_ = [("window_slack", C.c_int64), ("score_band", C.c_int64), ("hsp_threshold", C.c_int32),
+                ("exhaustive", C.c_int32)]
+
+
 class BandJob(C.Structure):
     _fields_ = [
         ("query", C.c_uint32),
@@ -124,6 +144,19 @@ _SIGS = {
     "saber_hip_stats_evalue": (C.c_double, [C.POINTER(QueryStats), C.c_int64]),
     "saber_hip_stats_bitscore": (C.c_double, [C.POINTER(QueryStats), C.c_int64]),
     "saber_hip_stats_min_score": (C.c_int64, [C.POINTER(QueryStats), C.c_double, C.c_double]),
+    "saber_hip_visit_hits": (C.c_int, [C.c_uint32, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                       C.c_uint64, C.POINTER(C.c_int32), C.POINTER(C.c_uint32),
+                                       C.POINTER(C.c_uint32), C.c_int32, C.POINTER(Hsp), C.POINTER(C.c_uint64),
+                                       C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "saber_hip_finish_subject": (C.c_int, [C.c_uint64, C.POINTER(Hsp), C.c_uint64, C.c_int64, C.c_int32, C.c_int32,
+                                           C.c_uint32, C.POINTER(Region), C.POINTER(C.c_uint32)]),
+    "saber_hip_cap_regions": (C.c_int, [C.POINTER(Region), C.c_uint64, C.c_uint64, C.POINTER(C.c_uint64)]),
+    "saber_hip_plan_windows": (C.c_int, [C.POINTER(Region), C.c_uint64, C.c_uint64, C.POINTER(C.c_uint64),
+                                         C.POINTER(PlanParams), C.POINTER(Plan)]),
+    "saber_hip_score_regions": (C.c_int, [P, C.POINTER(SeqSet), C.POINTER(SeqSet), C.POINTER(C.c_uint32),
+                                          C.POINTER(Region), C.POINTER(Plan), C.c_uint64, C.POINTER(Scheme),
+                                          C.c_int64, C.c_int64, C.POINTER(C.c_double), C.POINTER(C.c_int32),
+                                          C.POINTER(C.c_uint8)]),
     "saber_hip_mlp_create": (C.c_int, [P, C.POINTER(C.c_float), C.c_uint64, C.POINTER(P)]),
     "saber_hip_mlp_get_params": (C.c_int, [P, C.POINTER(C.c_float)]),
     "saber_hip_mlp_set_params": (C.c_int, [P, C.POINTER(C.c_float)]),

@@ -1713,3 +1713,77 @@ void saber_hip_mlp_destroy(saber_hip_mlp* m) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Two-stage region scoring: the banded part of Engine::score_windows
+// (src/search/engine.rs:1021-1099) for the regions of many queries at once.
+
+extern "C" int saber_hip_score_regions(saber_hip_ctx* c, const saber_hip_seqset* queries,
+                                       const saber_hip_seqset* subjects, const uint32_t* qidx,
+                                       const saber_hip_region* regs, const saber_hip_plan* plans, uint64_t n,
+                                       const saber_hip_scheme* scheme, int64_t narrow_band, int64_t full_band,
+                                       const double* thr, int32_t* out, uint8_t* rescored) {
+    if (!c || !queries || !scheme || (n && (!qidx || !regs || !plans || !out))) return fail(SABER_HIP_EINVAL, "null argument");
+    if (int r = check_seqset(queries, "queries")) return r;
+    const bool two_stage = narrow_band > 0 && narrow_band < full_band;
+    if (two_stage && n && !thr) return fail(SABER_HIP_EINVAL, "thr is null");
+    std::vector<saber_hip_band_job> narrow, full;
+    std::vector<uint64_t> narrow_of, full_of;
+    for (uint64_t i = 0; i < n; ++i) {
+        if (qidx[i] >= queries->count) return fail(SABER_HIP_EINVAL, "region query index out of range");
+        const saber_hip_region& r = regs[i];
+        const saber_hip_plan& p = plans[i];
+        if (p.hi < p.lo || r.subject > 0xFFFFFFFFull) return fail(SABER_HIP_EINVAL, "bad plan");
+        const int64_t m = (int64_t)(queries->offsets[qidx[i] + 1] - queries->offsets[qidx[i]]);
+        const int64_t nw = (int64_t)(p.hi - p.lo);
+        saber_hip_band_job j{qidx[i], (uint32_t)r.subject, (uint32_t)p.lo, (uint32_t)p.hi, 0, 0};
+        if (p.banded) {
+            if (two_stage) {
+                // narrow band around the region's own diagonals, clipped
+                j.band_lo = (int32_t)std::max<int64_t>(r.d_min - narrow_band - (int64_t)p.lo, -(m - 1));
+                j.band_hi = (int32_t)std::min<int64_t>(r.d_max + narrow_band - (int64_t)p.lo, nw - 1);
+                narrow.push_back(j);
+                narrow_of.push_back(i);
+            } else {
+                j.band_lo = (int32_t)p.band_lo;
+                j.band_hi = (int32_t)p.band_hi;
+                full.push_back(j);
+                full_of.push_back(i);
+            }
+        } else {
+            // the whole window rectangle: the full band of query x window
+            j.band_lo = (int32_t)(-(m - 1));
+            j.band_hi = (int32_t)(nw - 1);
+            full.push_back(j);
+            full_of.push_back(i);
+        }
+        if (rescored) rescored[i] = 0;
+    }
+    if (!narrow.empty()) {
+        std::vector<int32_t> sc(narrow.size());
+        if (int r = saber_hip_score_bands(c, queries, subjects, narrow.data(), narrow.size(), scheme, sc.data(),
+                                          nullptr, nullptr, nullptr))
+            return r;
+        for (size_t k = 0; k < narrow.size(); ++k) {
+            const uint64_t i = narrow_of[k];
+            if ((double)sc[k] >= thr[i]) {
+                saber_hip_band_job j = narrow[k];
+                j.band_lo = (int32_t)plans[i].band_lo;
+                j.band_hi = (int32_t)plans[i].band_hi;
+                full.push_back(j);
+                full_of.push_back(i);
+                if (rescored) rescored[i] = 1;
+            } else {
+                out[i] = sc[k];
+            }
+        }
+    }
+    if (!full.empty()) {
+        std::vector<int32_t> sc(full.size());
+        if (int r = saber_hip_score_bands(c, queries, subjects, full.data(), full.size(), scheme, sc.data(), nullptr,
+                                          nullptr, nullptr))
+            return r;
+        for (size_t k = 0; k < full.size(); ++k) out[full_of[k]] = sc[k];
+    }
+    return SABER_HIP_OK;
+}

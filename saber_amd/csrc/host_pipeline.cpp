@@ -1,0 +1,204 @@
+// Host pipeline around the scoring kernels: hit visiting with diagonal
+// coverage (visit_scan, src/search/engine/scan.rs:2384-2431), HSP -> region
+// joining (finish_subject, src/search/engine.rs:2350-2412 and
+// finish_subject_into, scan.rs:2437-2474), the per-query region cap
+// (region_order, engine.rs:493-495) and window / band planning (window and
+// plan_windows, engine.rs:1223-1324; Band, src/algorithm/banded.rs:30-54).
+#include <algorithm>
+#include <cstdint>
+#include <limits>
+#include <map>
+#include <vector>
+
+#include "../../include/saber_hip.h"
+
+namespace {
+
+constexpr uint64_t kBandScoreMinCells = 1ull << 20;  // engine.rs:401
+constexpr uint64_t kBandCostRatio = 12;              // engine.rs:402
+constexpr int64_t kBandBridge = 1024;                // engine.rs:404
+
+struct Band {
+    int64_t lo, hi;
+    int64_t width() const { return std::max<int64_t>(hi - lo + 1, 0); }
+    Band clip(int64_t m, int64_t n) const { return Band{std::max(lo, -(m - 1)), std::min(hi, n - 1)}; }
+    bool covers(int64_t m, int64_t n) const { return lo <= -(m - 1) && hi >= n - 1; }
+    Band widen(int64_t by) const { return Band{lo - by, hi + by}; }
+};
+
+int32_t sat_add(int32_t a, int32_t b) {
+    const int64_t s = (int64_t)a + b;
+    return (int32_t)std::max<int64_t>(std::min<int64_t>(s, std::numeric_limits<int32_t>::max()),
+                                      std::numeric_limits<int32_t>::min());
+}
+
+// Engine::window (engine.rs:1223-1234).
+void window(int64_t d_min, int64_t d_max, int64_t qlen, int64_t sl, int64_t slack, uint64_t* olo, uint64_t* ohi) {
+    const int64_t lo = std::min(std::max(d_min - slack, (int64_t)0), sl);
+    const int64_t hi = std::max(std::min(qlen + d_max + slack, sl), lo);
+    if ((hi - lo) * 4 >= sl * 3) {
+        *olo = 0;
+        *ohi = (uint64_t)sl;
+    } else {
+        *olo = (uint64_t)lo;
+        *ohi = (uint64_t)hi;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int saber_hip_visit_hits(uint32_t qlen, uint32_t slen, const uint32_t* qp, const uint32_t* sp, uint64_t n,
+                         const int32_t* score, const uint32_t* q_lo, const uint32_t* q_hi, int32_t hsp_threshold,
+                         saber_hip_hsp* hsps, uint64_t* nhsp, uint64_t* extended, uint64_t* avoided) {
+    if (n && (!qp || !sp || !score || !q_lo || !q_hi || !hsps)) return SABER_HIP_EINVAL;
+    // Diagonal d = sp - qp in [-qlen, slen]: cover[d + qlen] = q_hi of the last
+    // extension on it (0 = none yet).
+    std::vector<uint32_t> cover((size_t)qlen + slen + 2, 0);
+    uint64_t nh = 0, ext = 0, av = 0;
+    for (uint64_t k = 0; k < n; ++k) {
+        const int64_t d = (int64_t)sp[k] - (int64_t)qp[k];
+        const int64_t di = d + (int64_t)qlen;
+        if (di < 0 || di >= (int64_t)cover.size() || sp[k] >= slen || qp[k] >= qlen) continue;
+        if (qp[k] < cover[(size_t)di]) {
+            ++av;
+            continue;
+        }
+        ++ext;
+        cover[(size_t)di] = q_hi[k];
+        if (score[k] >= hsp_threshold) hsps[nh++] = saber_hip_hsp{d, q_lo[k], q_hi[k], score[k], 0};
+    }
+    if (nhsp) *nhsp = nh;
+    if (extended) *extended = ext;
+    if (avoided) *avoided = av;
+    return SABER_HIP_OK;
+}
+
+int saber_hip_finish_subject(uint64_t subject, saber_hip_hsp* hsps, uint64_t n, int64_t window_slack,
+                             int32_t hsp_threshold, int32_t region_thr, uint32_t max_regions_per_subject,
+                             saber_hip_region* out, uint32_t* nout) {
+    if (!nout || (n && (!hsps || !out))) return SABER_HIP_EINVAL;
+    *nout = 0;
+    if (n == 0) return SABER_HIP_OK;
+    std::sort(hsps, hsps + n, [](const saber_hip_hsp& a, const saber_hip_hsp& b) { return a.diag < b.diag; });
+    const int64_t join = 2 * window_slack;
+    std::vector<saber_hip_region> regs;
+    for (uint64_t k = 0; k < n; ++k) {
+        const saber_hip_hsp& h = hsps[k];
+        if (!regs.empty() && h.diag - regs.back().d_max <= join) {
+            saber_hip_region& r = regs.back();
+            r.d_max = std::max(r.d_max, h.diag);
+            r.best = std::max(r.best, h.score);
+            r.n_hsp += 1;
+            r.hsp_sum = sat_add(r.hsp_sum, h.score);
+        } else {
+            regs.push_back(saber_hip_region{subject, h.diag, h.diag, h.score, 1, h.score});
+        }
+    }
+    if (region_thr > hsp_threshold) {
+        regs.erase(std::remove_if(regs.begin(), regs.end(),
+                                  [&](const saber_hip_region& r) {
+                                      return !(r.best >= region_thr || (r.n_hsp >= 2 && r.hsp_sum >= region_thr));
+                                  }),
+                   regs.end());
+    }
+    std::sort(regs.begin(), regs.end(), [](const saber_hip_region& a, const saber_hip_region& b) {
+        return a.best != b.best ? a.best > b.best : a.d_min < b.d_min;
+    });
+    const size_t keep = std::min<size_t>(regs.size(), std::max<uint32_t>(max_regions_per_subject, 1));
+    std::copy(regs.begin(), regs.begin() + keep, out);
+    *nout = (uint32_t)keep;
+    return SABER_HIP_OK;
+}
+
+int saber_hip_cap_regions(saber_hip_region* regs, uint64_t n, uint64_t cap, uint64_t* nout) {
+    if (!nout || (n && !regs)) return SABER_HIP_EINVAL;
+    std::sort(regs, regs + n, [](const saber_hip_region& a, const saber_hip_region& b) {
+        if (a.best != b.best) return a.best > b.best;
+        if (a.subject != b.subject) return a.subject < b.subject;
+        if (a.d_min != b.d_min) return a.d_min < b.d_min;
+        return a.d_max < b.d_max;
+    });
+    *nout = std::min(n, cap);
+    return SABER_HIP_OK;
+}
+
+int saber_hip_plan_windows(const saber_hip_region* regs, uint64_t n, uint64_t qlen, const uint64_t* subject_len,
+                           const saber_hip_plan_params* p, saber_hip_plan* out) {
+    if (!p || (n && (!regs || !subject_len || !out))) return SABER_HIP_EINVAL;
+    const int64_t q = (int64_t)qlen;
+    // Chains of strong regions per subject (single linkage within kBandBridge).
+    std::vector<std::pair<int64_t, int64_t>> span(n);
+    std::map<uint64_t, std::vector<uint64_t>> by_subject;
+    for (uint64_t i = 0; i < n; ++i) {
+        span[i] = {regs[i].d_min, regs[i].d_max};
+        by_subject[regs[i].subject].push_back(i);
+    }
+    for (auto& kv : by_subject) {
+        std::vector<uint64_t>& idx = kv.second;
+        int32_t top = 0;
+        for (uint64_t i : idx) top = std::max(top, regs[i].best);
+        const int32_t strong = std::max(2 * p->hsp_threshold, top / 4);
+        idx.erase(std::remove_if(idx.begin(), idx.end(), [&](uint64_t i) { return regs[i].best < strong; }),
+                  idx.end());
+        std::sort(idx.begin(), idx.end(), [&](uint64_t a, uint64_t b) {
+            if (regs[a].d_min != regs[b].d_min) return regs[a].d_min < regs[b].d_min;
+            if (regs[a].d_max != regs[b].d_max) return regs[a].d_max < regs[b].d_max;
+            return a < b;
+        });
+        size_t start = 0;
+        while (start < idx.size()) {
+            int64_t lo = regs[idx[start]].d_min, hi = regs[idx[start]].d_max;
+            size_t end = start + 1;
+            while (end < idx.size() && regs[idx[end]].d_min - hi <= kBandBridge) {
+                hi = std::max(hi, regs[idx[end]].d_max);
+                ++end;
+            }
+            for (size_t k = start; k < end; ++k) span[idx[k]] = {lo, hi};
+            start = end;
+        }
+    }
+    for (uint64_t ri = 0; ri < n; ++ri) {
+        const saber_hip_region& r = regs[ri];
+        const int64_t sl = (int64_t)subject_len[ri];
+        saber_hip_plan own{};
+        window(r.d_min, r.d_max, q, sl, p->window_slack, &own.lo, &own.hi);
+        own.banded = 0;
+        saber_hip_plan& o = out[ri];
+        o = own;
+        if (p->exhaustive || r.best == 0 || p->window_slack >= std::numeric_limits<int64_t>::max() / 8) continue;
+        const int64_t dmin = span[ri].first, dmax = span[ri].second;
+        if (p->score_band > 0) {
+            const int64_t w = p->score_band;
+            const int64_t lo = std::min(std::max(dmin - w, (int64_t)0), sl);
+            const int64_t hi = std::max(std::min(q + dmax + w, sl), lo);
+            const int64_t nn = hi - lo;
+            const Band band = Band{dmin - w - lo, dmax + w - lo}.clip(q, nn);
+            o.lo = (uint64_t)lo;
+            o.hi = (uint64_t)hi;
+            o.banded = (nn == 0 || band.covers(q, nn)) ? 0 : 1;
+            o.band_lo = o.banded ? band.lo : 0;
+            o.band_hi = o.banded ? band.hi : 0;
+            continue;
+        }
+        if ((uint64_t)q * (own.hi - own.lo) < kBandScoreMinCells) continue;
+        uint64_t ulo = 0, uhi = 0;
+        window(dmin, dmax, q, sl, p->window_slack, &ulo, &uhi);
+        const int64_t nn = (int64_t)(uhi - ulo);
+        const Band band = Band{dmin - (int64_t)ulo, dmax - (int64_t)ulo}.widen(p->window_slack).clip(q, nn);
+        if (band.covers(q, nn)) continue;
+        const uint64_t rows = (uint64_t)std::min<int64_t>(q, nn + band.width());
+        const uint64_t rect = (uint64_t)q * (uint64_t)nn;
+        if ((uint64_t)band.width() * rows * kBandCostRatio < rect) {
+            o.lo = ulo;
+            o.hi = uhi;
+            o.banded = 1;
+            o.band_lo = band.lo;
+            o.band_hi = band.hi;
+        }
+    }
+    return SABER_HIP_OK;
+}
+
+}  // extern "C"
