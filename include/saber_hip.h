@@ -266,6 +266,11 @@ double saber_hip_stats_evalue_for(const saber_hip_query_stats* q, int64_t raw, u
 double saber_hip_stats_evalue(const saber_hip_query_stats* q, int64_t raw);
 double saber_hip_stats_bitscore(const saber_hip_query_stats* q, int64_t raw);
 int64_t saber_hip_stats_min_score(const saber_hip_query_stats* q, double evalue_cut, double min_bits);
+/* evalue[i] = evalue_for(raw[i], subject_len[i]) of a query of query_len[i]
+ * residues (queries' statistics are recomputed when query_len changes). */
+int saber_hip_stats_evalue_batch(const saber_hip_karlin* p, uint64_t db_length, uint64_t db_num_seqs,
+                                 int per_subject, uint64_t n, const uint64_t* query_len,
+                                 const uint64_t* subject_len, const int64_t* raw, double* evalue);
 
 /* ---- Host pipeline: hits -> HSPs -> regions -> plans -> scores ---- */
 typedef struct saber_hip_hsp {
@@ -343,6 +348,18 @@ int saber_hip_score_regions(saber_hip_ctx* ctx, const saber_hip_seqset* queries,
  * SABER_HIP_MLP_NPARAMS floats: W1[64][256], b1[256], W2[256][256], b2[256],
  * w3[256], b3 (h1 = relu(x W1 + b1), h2 = relu(h1 W2 + b2), y = h2 . w3 + b3).
  * Features are bf16 bit patterns (uint16), row-major [n][64]. */
+/* Prefilter feature vectors (bf16 bit patterns, [n][64]) of candidate
+ * regions on the device: region i is band job jobs[i] (its query, subject
+ * window and band) with X-drop results xscore/xq_lo/xq_hi of its
+ * hits_per_region (<= 64) hits hits[i * hits_per_region ..]; the feature list
+ * is documented in saber_amd/csrc/features.hip (X-drop evidence, geometry,
+ * query and window composition). mean / inv_std (64 each, or both NULL)
+ * standardise each feature, (f - mean) * inv_std, before the bf16 rounding. */
+int saber_hip_region_features(saber_hip_ctx* ctx, const saber_hip_seqset* queries,
+                              const saber_hip_seqset* subjects, const saber_hip_band_job* jobs, uint64_t n,
+                              const saber_hip_xdrop_hit* hits, const int32_t* xscore, const uint32_t* xq_lo,
+                              const uint32_t* xq_hi, uint32_t hits_per_region, int32_t hsp_threshold,
+                              const float* mean, const float* inv_std, uint16_t* features);
 #define SABER_HIP_MLP_IN 64
 #define SABER_HIP_MLP_HID 256
 #define SABER_HIP_MLP_NPARAMS (64 * 256 + 256 + 256 * 256 + 256 + 256 + 1)

@@ -143,6 +143,9 @@ _SIGS = {
     "saber_hip_stats_evalue_for": (C.c_double, [C.POINTER(QueryStats), C.c_int64, C.c_uint64]),
     "saber_hip_stats_evalue": (C.c_double, [C.POINTER(QueryStats), C.c_int64]),
     "saber_hip_stats_bitscore": (C.c_double, [C.POINTER(QueryStats), C.c_int64]),
+    "saber_hip_stats_evalue_batch": (C.c_int, [C.POINTER(Karlin), C.c_uint64, C.c_uint64, C.c_int, C.c_uint64,
+                                               C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                               C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
     "saber_hip_stats_min_score": (C.c_int64, [C.POINTER(QueryStats), C.c_double, C.c_double]),
     "saber_hip_visit_hits": (C.c_int, [C.c_uint32, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                                        C.c_uint64, C.POINTER(C.c_int32), C.POINTER(C.c_uint32),
@@ -157,6 +160,10 @@ _SIGS = {
                                           C.POINTER(Region), C.POINTER(Plan), C.c_uint64, C.POINTER(Scheme),
                                           C.c_int64, C.c_int64, C.POINTER(C.c_double), C.POINTER(C.c_int32),
                                           C.POINTER(C.c_uint8)]),
+    "saber_hip_region_features": (C.c_int, [P, C.POINTER(SeqSet), C.POINTER(SeqSet), C.POINTER(BandJob), C.c_uint64,
+                                            C.POINTER(C.c_uint32), C.POINTER(C.c_int32), C.POINTER(C.c_uint32),
+                                            C.POINTER(C.c_uint32), C.c_uint32, C.c_int32, C.POINTER(C.c_float),
+                                            C.POINTER(C.c_float), C.POINTER(C.c_uint16)]),
     "saber_hip_mlp_create": (C.c_int, [P, C.POINTER(C.c_float), C.c_uint64, C.POINTER(P)]),
     "saber_hip_mlp_get_params": (C.c_int, [P, C.POINTER(C.c_float)]),
     "saber_hip_mlp_set_params": (C.c_int, [P, C.POINTER(C.c_float)]),

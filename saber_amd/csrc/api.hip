@@ -1787,3 +1787,63 @@ extern "C" int saber_hip_score_regions(saber_hip_ctx* c, const saber_hip_seqset*
     }
     return SABER_HIP_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Prefilter features of candidate regions (features.hip).
+
+
+extern "C" int saber_hip_region_features(saber_hip_ctx* c, const saber_hip_seqset* queries,
+                                         const saber_hip_seqset* subjects, const saber_hip_band_job* jobs,
+                                         uint64_t n, const saber_hip_xdrop_hit* hits, const int32_t* xscore,
+                                         const uint32_t* xq_lo, const uint32_t* xq_hi, uint32_t hits_per_region,
+                                         int32_t hsp_threshold, const float* mean, const float* inv_std,
+                                         uint16_t* features) {
+    if (!c || !queries || !subjects || (n && (!jobs || !features))) return fail(SABER_HIP_EINVAL, "null argument");
+    if (hits_per_region > 64) return fail(SABER_HIP_EINVAL, "at most 64 hits per region");
+    if (n && hits_per_region && (!hits || !xscore || !xq_lo || !xq_hi)) return fail(SABER_HIP_EINVAL, "null hit arrays");
+    if ((mean == nullptr) != (inv_std == nullptr)) return fail(SABER_HIP_EINVAL, "mean and inv_std go together");
+    if (int r = check_seqset(queries, "queries")) return r;
+    if (int r = check_seqset(subjects, "subjects")) return r;
+    for (uint64_t k = 0; k < n; ++k) {
+        const saber_hip_band_job& J = jobs[k];
+        if (J.query >= queries->count || J.subject >= subjects->count) return fail(SABER_HIP_EINVAL, "job index out of range");
+        if (J.win_hi < J.win_lo || J.win_hi > subjects->offsets[J.subject + 1] - subjects->offsets[J.subject])
+            return fail(SABER_HIP_EINVAL, "job window outside its subject");
+    }
+    if (n == 0) return SABER_HIP_OK;
+    std::lock_guard<std::mutex> g(c->mu);
+    HIP_TRY(hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    const uint64_t nq = queries->offsets[queries->count], ns = subjects->offsets[subjects->count];
+    const uint64_t nh = n * hits_per_region;
+    DevBuf<saber_hip_band_job> dj;
+    DevBuf<uint64_t> dqo, dso;
+    DevBuf<uint8_t> dq, ds;
+    DevBuf<uint32_t> dh, dlo, dhi;
+    DevBuf<int32_t> dsc;
+    DevBuf<__bf16> dout;
+    DevBuf<float> dmean, dinv;
+    if (mean) {
+        if (int r = upload(dmean, mean, 64, st)) return r;
+        if (int r = upload(dinv, inv_std, 64, st)) return r;
+    }
+    if (int r = upload(dj, jobs, n, st)) return r;
+    if (int r = upload(dqo, queries->offsets, (uint64_t)queries->count + 1, st)) return r;
+    if (int r = upload(dso, subjects->offsets, (uint64_t)subjects->count + 1, st)) return r;
+    if (int r = upload_codes(dq, queries->residues, nq, st)) return r;
+    if (int r = upload_codes(ds, subjects->residues, ns, st)) return r;
+    if (int r = upload(dh, reinterpret_cast<const uint32_t*>(hits), 4 * nh, st)) return r;
+    if (int r = upload(dsc, xscore, nh, st)) return r;
+    if (int r = upload(dlo, xq_lo, nh, st)) return r;
+    if (int r = upload(dhi, xq_hi, nh, st)) return r;
+    HIP_TRY(dout.alloc(64 * n));
+    FeatureArgs a{dj.p, n, dqo.p, dq.p + kGuard, dso.p, ds.p + kGuard, dh.p, dsc.p, dlo.p, dhi.p,
+                  hits_per_region, hsp_threshold, mean ? dmean.p : nullptr, mean ? dinv.p : nullptr, dout.p};
+    int cus = 0;
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+    const int grid = (int)std::min<uint64_t>((n + 3) / 4, (uint64_t)cus * 16);
+    HIP_TRY(launch_region_features(a, grid, st));
+    HIP_TRY(hipMemcpyAsync(features, dout.p, 2 * 64 * n, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return SABER_HIP_OK;
+}

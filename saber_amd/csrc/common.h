@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "../../include/saber_hip.h"
+
 namespace saber {
 
 // Alignment codes 0..23 (ARNDCQEGHILKMFPSTWYVBZX*); any code >= 24 scores as
@@ -189,5 +191,27 @@ hipError_t mlp_launch_adam(float* p, float* grad, float* m, float* v, uint32_t n
 hipError_t mlp_launch_pack(const float* W1, const float* W2, __bf16* W1T, __bf16* W2P, __bf16* W2Q,
                            hipStream_t s);
 hipError_t mlp_launch_to_bf16(const float* src, __bf16* dst, uint64_t n, hipStream_t s);
+
+// Prefilter region features (features.hip): region r = jobs[r], hits
+// r * per_region .. of hits (4 u32: query, subject, qp, sp) with their X-drop
+// results; out [n][64] bf16.
+struct FeatureArgs {
+    const saber_hip_band_job* jobs;
+    uint64_t n;
+    const uint64_t* qoff;
+    const uint8_t* qres;
+    const uint64_t* soff;
+    const uint8_t* sres;
+    const uint32_t* hits;     
+    const int32_t* xscore;
+    const uint32_t* xlo;
+    const uint32_t* xhi;
+    uint32_t per_region;
+    int32_t hsp_threshold;
+    const float* mean;         // optional standardisation: (f - mean) * inv_std
+    const float* inv_std;
+    __bf16* out;
+};
+hipError_t launch_region_features(const FeatureArgs& a, int grid, hipStream_t s);
 
 }  // namespace saber

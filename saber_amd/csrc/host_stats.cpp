@@ -228,3 +228,19 @@ int64_t saber_hip_stats_min_score(const saber_hip_query_stats* q, double evalue_
 }
 
 }  // extern "C"
+
+extern "C" int saber_hip_stats_evalue_batch(const saber_hip_karlin* p, uint64_t db_length, uint64_t db_num_seqs,
+                                            int per_subject, uint64_t n, const uint64_t* query_len,
+                                            const uint64_t* subject_len, const int64_t* raw, double* evalue) {
+    if (!p || (n && (!query_len || !subject_len || !raw || !evalue))) return SABER_HIP_EINVAL;
+    saber_hip_query_stats q{};
+    uint64_t last_m = ~0ull;
+    for (uint64_t i = 0; i < n; ++i) {
+        if (query_len[i] != last_m) {
+            saber_hip_stats_for_query(p, db_length, db_num_seqs, per_subject, query_len[i], &q);
+            last_m = query_len[i];
+        }
+        evalue[i] = saber_hip_stats_evalue_for(&q, raw[i], subject_len[i]);
+    }
+    return SABER_HIP_OK;
+}

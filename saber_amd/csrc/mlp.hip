@@ -16,6 +16,8 @@
 // k permutation: for a 32-wide k step s and lane group g (= lane / 16), slot
 // t' in 0..7 of the operand holds unit 32*s + 4*g + t' (t' < 4) or
 // 32*s + 16 + 4*g + (t' - 4) (t' >= 4).
+#include <algorithm>
+
 #include "common.h"
 
 namespace saber {
@@ -118,16 +120,92 @@ __device__ __forceinline__ float forward_tile(const MlpWeights& w, const __bf16*
     return y + *w.b3;
 }
 
+// Inference: weights stay in registers for the whole launch, split across
+// the four waves of a block: wave v owns hidden units [64v, 64v + 64) of both
+// layers (W1T rows: 8 fragments, W2P rows: 32 fragments). Per 16-row tile,
+// every wave computes its quarter of layer 1 from the tile's features and
+// writes it (bf16, after ReLU) to LDS; after a barrier each wave reads the
+// whole layer-1 tile back as layer-2 operands (in the permuted k order W2P
+// expects), computes its quarter of layer 2 and its part of the output dot
+// product, and wave 0 adds the four parts. LDS is double-buffered by tile.
+constexpr int kHs = kHid + 8;  // LDS row stride (bf16) of the layer-1 tile
+
 __global__ __launch_bounds__(256) void mlp_forward_kernel(MlpWeights w, const __bf16* X, uint64_t n,
                                                           float* logits) {
-    const int lane = threadIdx.x & 63;
-    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-    for (uint64_t tile = wave; tile * 16 < n; tile += nwaves) {
-        const uint64_t row0 = tile * 16;
-        const float y = forward_tile<false>(w, X, row0, n, lane, nullptr);
-        const uint64_t row = row0 + (lane & 15);
-        if (lane < 16 && row < n) logits[row] = y;
+    __shared__ __bf16 Hs[2][16 * kHs];
+    __shared__ float part[2][4][16];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int g = lane >> 4, r = lane & 15;
+    bf16x8 a1[4][2], a2[4][8];
+#pragma unroll
+    for (int ht = 0; ht < 4; ++ht)
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks)
+            a1[ht][ks] = load8(w.W1T + (uint64_t)(64 * wv + 16 * ht + r) * kIn + 32 * ks + 8 * g);
+#pragma unroll
+    for (int ot = 0; ot < 4; ++ot)
+#pragma unroll
+        for (int s = 0; s < 8; ++s)
+            a2[ot][s] = load8(w.W2P + (uint64_t)(64 * wv + 16 * ot + r) * kHid + 32 * s + 8 * g);
+    float b1v[4][4], b2v[4][4], w3v[4][4];
+#pragma unroll
+    for (int t4 = 0; t4 < 4; ++t4)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int u = 64 * wv + 16 * t4 + 4 * g + t;
+            b1v[t4][t] = w.b1[u];
+            b2v[t4][t] = w.b2[u];
+            w3v[t4][t] = w.w3[u];
+        }
+    const float b3 = *w.b3;
+    const uint64_t ntiles = (n + 15) / 16;
+    int buf = 0;
+    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x, buf ^= 1) {
+        const uint64_t row = tile * 16 + r;
+        bf16x8 xb[2];
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks) xb[ks] = row < n ? load8(X + row * kIn + 32 * ks + 8 * g) : bf16x8{};
+        // layer 1, this wave's 64 units -> LDS
+#pragma unroll
+        for (int ht = 0; ht < 4; ++ht) {
+            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+            acc = mfma(a1[ht][0], xb[0], acc);
+            acc = mfma(a1[ht][1], xb[1], acc);
+            __bf16 hv[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const float v = acc[t] + b1v[ht][t];
+                hv[t] = (__bf16)(v > 0.f ? v : 0.f);
+            }
+            __builtin_memcpy(&Hs[buf][r * kHs + 64 * wv + 16 * ht + 4 * g], hv, 8);
+        }
+        __syncthreads();
+        // layer 2, this wave's 64 units, operands from the whole layer-1 tile
+        f32x4 acc2[4];
+#pragma unroll
+        for (int ot = 0; ot < 4; ++ot) acc2[ot] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            bf16x8 hb;
+            __builtin_memcpy(&hb, &Hs[buf][r * kHs + 32 * s + 4 * g], 8);
+            __builtin_memcpy(reinterpret_cast<char*>(&hb) + 8, &Hs[buf][r * kHs + 32 * s + 16 + 4 * g], 8);
+#pragma unroll
+            for (int ot = 0; ot < 4; ++ot) acc2[ot] = mfma(a2[ot][s], hb, acc2[ot]);
+        }
+        float y = 0.f;
+#pragma unroll
+        for (int ot = 0; ot < 4; ++ot)
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const float v = acc2[ot][t] + b2v[ot][t];
+                y += (v > 0.f ? v : 0.f) * w3v[ot][t];
+            }
+        y += __shfl_xor(y, 16);
+        y += __shfl_xor(y, 32);
+        if (lane < 16) part[buf][wv][lane] = y;
+        __syncthreads();
+        if (wv == 0 && lane < 16 && row < n)
+            logits[row] = part[buf][0][lane] + part[buf][1][lane] + part[buf][2][lane] + part[buf][3][lane] + b3;
     }
 }
 
@@ -326,7 +404,10 @@ __global__ void f32_to_bf16_kernel(const float* src, __bf16* dst, uint64_t n) {
 hipError_t mlp_launch_forward(const MlpWeights& w, const __bf16* X, uint64_t n, float* logits,
                               int grid, hipStream_t s) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(mlp_forward_kernel, dim3(grid), dim3(256), 0, s, w, X, n, logits);
+    // one 16-row tile per block per step; grid = blocks resident at once
+    const uint64_t tiles = (n + 15) / 16;
+    const int g = (int)std::min<uint64_t>(tiles, (uint64_t)grid);
+    hipLaunchKernelGGL(mlp_forward_kernel, dim3(g), dim3(256), 0, s, w, X, n, logits);
     return hipGetLastError();
 }
 

@@ -130,3 +130,47 @@ class Mlp:
             self.close()
         except Exception:
             pass
+
+
+def region_features(ctx, qres, qoff, sres, soff, jobs, hits, xscore, xq_lo, xq_hi, hits_per_region,
+                    hsp_threshold=40, mean=None, std=None):
+    """bf16 feature vectors [n, 64] (uint16 bit patterns) of the regions
+    `jobs` (band-job structured array) with the X-drop results of their hits
+    (hits: [n * hits_per_region, 4] u32 query, subject, qp, sp); with mean and
+    std, each feature is standardised on the device before rounding."""
+    mu = inv = None
+    if mean is not None:
+        mu = np.ascontiguousarray(mean, np.float32)
+        s = np.asarray(std, np.float32)
+        inv = np.ascontiguousarray(np.where(s > 0, 1.0 / np.where(s > 0, s, 1), 0.0), np.float32)
+    qres = np.ascontiguousarray(qres, np.uint8)
+    qoff = np.ascontiguousarray(qoff, np.uint64)
+    sres = np.ascontiguousarray(sres, np.uint8)
+    soff = np.ascontiguousarray(soff, np.uint64)
+    qs = _ffi.SeqSet(_ffi.ptr(qres, C.c_uint8), _ffi.ptr(qoff, C.c_uint64), len(qoff) - 1, None)
+    ss = _ffi.SeqSet(_ffi.ptr(sres, C.c_uint8), _ffi.ptr(soff, C.c_uint64), len(soff) - 1, None)
+    jobs = np.ascontiguousarray(jobs)
+    h = np.ascontiguousarray(hits, np.uint32).reshape(-1, 4)
+    xs = np.ascontiguousarray(xscore, np.int32)
+    lo = np.ascontiguousarray(xq_lo, np.uint32)
+    hi = np.ascontiguousarray(xq_hi, np.uint32)
+    n = len(jobs)
+    out = np.zeros((max(n, 1), N_IN), np.uint16)
+    _ffi.check(_ffi.lib().saber_hip_region_features(
+        ctx.h, C.byref(qs), C.byref(ss), jobs.ctypes.data_as(C.POINTER(_ffi.BandJob)), n,
+        _ffi.ptr(h, C.c_uint32), _ffi.ptr(xs, C.c_int32), _ffi.ptr(lo, C.c_uint32), _ffi.ptr(hi, C.c_uint32),
+        int(hits_per_region), int(hsp_threshold), _ffi.ptr(mu, C.c_float), _ffi.ptr(inv, C.c_float),
+        _ffi.ptr(out, C.c_uint16)))
+    return out[:n]
+
+
+def fold_normalization(params, mean, std):
+    """Parameters that give, on raw features x, the outputs the model
+    `params` gives on standardised features (x - mean) / std."""
+    p = np.array(params, dtype=np.float64)
+    w1 = p[W1:B1].reshape(N_IN, N_HID)
+    s = np.where(np.asarray(std, np.float64) > 0, np.asarray(std, np.float64), 1.0)
+    mu = np.asarray(mean, np.float64)
+    p[B1:W2] = p[B1:W2] - (mu / s) @ w1
+    p[W1:B1] = (w1 / s[:, None]).ravel()
+    return p.astype(np.float32)

@@ -37,6 +37,20 @@ def compute_length_adjustment(k, log_k, alpha_d_lambda, beta, query_length, db_l
     return adj.value, bool(conv.value)
 
 
+def evalues(params, db_length, db_num_seqs, model, query_len, subject_len, raw):
+    """E-values of many hits (numpy arrays; fastest with query_len sorted)."""
+    import numpy as np
+    m = np.ascontiguousarray(query_len, np.uint64)
+    n = np.ascontiguousarray(subject_len, np.uint64)
+    r = np.ascontiguousarray(raw, np.int64)
+    out = np.zeros(len(r), np.float64)
+    _ffi.check(_ffi.lib().saber_hip_stats_evalue_batch(
+        C.byref(params._k), int(db_length), int(db_num_seqs), int(model), len(r),
+        m.ctypes.data_as(C.POINTER(C.c_uint64)), n.ctypes.data_as(C.POINTER(C.c_uint64)),
+        r.ctypes.data_as(C.POINTER(C.c_int64)), out.ctypes.data_as(C.POINTER(C.c_double))))
+    return out
+
+
 class QueryStatistics:
     def __init__(self, q):
         self._q = q

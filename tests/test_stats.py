@@ -90,6 +90,18 @@ def test_min_score_is_exact_inverse():
     assert (q.bitscore(s) >= 50.0 and q.bitscore(s - 1) < 50.0) or q.evalue(s - 1) > 10.0
 
 
+def test_evalue_batch_equals_single_calls():
+    p = b62()
+    st = ST.SearchStatistics(p, 147_000_000, 331_000, ST.PER_SUBJECT)
+    rng = np.random.default_rng(9)
+    m = np.sort(rng.integers(20, 2000, 300))
+    n = rng.integers(20, 5000, 300)
+    raw = rng.integers(-5, 300, 300)
+    got = ST.evalues(p, 147_000_000, 331_000, ST.PER_SUBJECT, m, n, raw)
+    want = [st.for_query(int(a)).evalue_for(int(r), int(b)) for a, b, r in zip(m, n, raw)]
+    assert np.array_equal(got, np.array(want))
+
+
 def test_floor_lower_bound_below_exact_floor():
     # the per-subject floor bound never exceeds the smallest passing score
     q = ST.SearchStatistics(b62(), 147_000_000, 331_000, ST.PER_SUBJECT).for_query(170)
