@@ -19,7 +19,11 @@ The printed JSON line also carries
   cpu_baseline  the AVX2 restatement of the reference's score_banded_many
                 (oracle/cpu_avx2.c) on all host threads over a bounded sample
                 of whole queries of the same batch;
-  extra.xdrop   X-drop extension of 4 seed hits per region (steps/s).
+  extra.xdrop   X-drop extension of 4 seed hits per region (steps/s);
+  extra.full_sw exhaustive full-matrix SW (queries x subjects, GCUPS);
+  extra.prefilter  the learned region prefilter: hit loss, kept fraction,
+                scoring-stage time with and without it, MLP forward
+                throughput against the bf16 MFMA peak.
 """
 import argparse
 import json
@@ -55,6 +59,89 @@ def measured_traffic(args):
     return int(rec["traffic_bytes"])
 
 
+def prefilter_leg(ctx, args, sch):
+    """Learned region prefilter (64 -> 256 -> 256 -> 1 bf16 MFMA MLP) on
+    synthetic candidate regions: 10 % planted homologues, 8 seed hits per
+    region (half on the region's centre diagonal). Labels: the exact band
+    score reaches E <= 10 (per-subject model, BLOSUM62 11/1, a 147 M-residue
+    / 331 k-sequence database). Features are computed on the device from the
+    X-drop results, standardised with training-set statistics. The model is
+    trained on the train set; its cut is set for 99.5 % recall on a
+    validation set; the test set gives the hit loss, the kept fraction and the
+    time of the scoring stage with and without the filter."""
+    from saber_amd import prefilter as P, stats as ST, workload
+    per = 8
+    kp = ST.gapped_protein(62, 11, 1)
+
+    def make(seed, n):
+        d = workload.band_regions(seed=seed, n=n, per_query=1000, homolog_frac=0.1)
+        hits = workload.xdrop_hits(d, seed=seed + 1, per_region=per)
+        xs, xlo, xhi, _ = ctx.xdrop(d["qres"], d["qoff"], d["sres"], d["soff"], hits, sch, 20)
+        b = ctx.band_batch(d["qres"], d["qoff"], d["sres"], d["soff"], d["jobs"], sch)
+        b.run()
+        score = b.read()[0]
+        b.close()
+        J = d["jobs"]
+        m = (d["qoff"][1:] - d["qoff"][:-1])[J["query"]]
+        e = ST.evalues(kp, 147_000_000, 331_000, ST.PER_SUBJECT, m, (J["win_hi"] - J["win_lo"]).astype(np.uint64),
+                       score)
+        d["raw"] = (hits, xs, xlo, xhi)
+        return d, (e <= 10.0).astype(np.float32)
+
+    t0 = time.time()
+    dtr, ytr = make(args.seed + 1001, args.pf_train)
+    dva, yva = make(args.seed + 1002, args.pf_train // 4)
+    dte, yte = make(args.seed + 1003, args.pf_test)
+    feats = lambda d, mu=None, sd=None: P.region_features(ctx, d["qres"], d["qoff"], d["sres"], d["soff"], d["jobs"],
+                                                          *d["raw"], per, 40, mean=mu, std=sd)
+    raw = P.bf16_float(feats(dtr))
+    mu, sd = raw.mean(0), raw.std(0)
+    sd = np.where(sd > 1e-6, sd, 0.0)
+    Xtr, Xva, Xte = feats(dtr, mu, sd), feats(dva, mu, sd), feats(dte, mu, sd)
+    prep_s = time.time() - t0
+    m = P.Mlp(ctx, seed=args.seed)
+    m.upload(Xtr, ytr)
+    bs, epochs, lr = 65536, 12, 2e-3
+    t0 = time.time()
+    for ep in range(epochs):
+        for r0 in range(0, len(ytr), bs):
+            m.train_resident(r0, min(bs, len(ytr) - r0), lr=lr, pos_weight=4.0, want_loss=False)
+    loss = m.train_resident(0, min(bs, len(ytr)), lr=0.0, pos_weight=4.0)
+    train_s = time.time() - t0
+    lv = m.forward(Xva)
+    tau = float(np.quantile(lv[yva > 0.5], 1 - 0.995))
+    lt = m.forward(Xte)
+    keep = lt >= tau
+    hit_loss = float(1 - keep[yte > 0.5].mean())
+
+    def band_ms(sel):
+        b = ctx.band_batch(dte["qres"], dte["qoff"], dte["sres"], dte["soff"], dte["jobs"][sel], sch)
+        ms = b.time(warmup=2, iters=max(3, args.steps // 2))[0]
+        b.close()
+        return ms
+    t_all = band_ms(np.arange(len(yte)))
+    t_keep = band_ms(np.nonzero(keep)[0])
+    m.upload(Xte)
+    f_ms, s_ms = m.time(len(yte), warmup=2, iters=max(3, args.steps // 2))
+    m.upload(Xtr, ytr)
+    _, s64 = m.time(bs, warmup=1, iters=5)
+    m.close()
+    flop = 164352 * len(yte)
+    out = {"train_regions": int(len(ytr)), "test_regions": int(len(yte)), "hits_per_region": per,
+           "positives": round(float(yte.mean()), 4), "recall_target": 0.995, "hit_loss": round(hit_loss, 5),
+           "kept_fraction": round(float(keep.mean()), 4), "band_ms_all": round(t_all, 4),
+           "band_ms_kept": round(t_keep, 4), "mlp_forward_ms": round(f_ms, 4),
+           "mlp_forward_tflops": round(flop / (f_ms * 1e-3) / 1e12, 1),
+           "mlp_forward_mfma_frac": round(flop / (f_ms * 1e-3) / 2.5e15, 4),
+           "train_step_ms_65536": round(s64, 3), "epochs": epochs, "train_s": round(train_s, 2),
+           "final_loss": round(float(loss), 4), "prep_s": round(prep_s, 1),
+           "scoring_speedup": round(t_all / (f_ms + t_keep), 3)}
+    print(f"# prefilter: kept {out['kept_fraction']}, hit loss {out['hit_loss']}, band {t_all:.3f} -> "
+          f"{t_keep:.3f} ms + mlp {f_ms:.3f} ms ({out['mlp_forward_tflops']} TFLOP/s): "
+          f"{out['scoring_speedup']}x", file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -71,6 +158,9 @@ def main():
     ap.add_argument("--full-queries", type=int, default=20,
                     help="queries of the exhaustive-mode leg (x 100,000 subjects); 0 skips it")
     ap.add_argument("--full-subjects", type=int, default=100_000)
+    ap.add_argument("--pf-train", type=int, default=4_000_000,
+                    help="training regions of the prefilter leg (0: skip); the test set has --pf-test")
+    ap.add_argument("--pf-test", type=int, default=4_000_000)
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only time a plain device copy of this many bytes and exit")
     args = ap.parse_args()
@@ -174,6 +264,8 @@ def main():
         print(f"# full SW {args.full_queries} x {args.full_subjects}: {fms:.2f} ms, "
               f"{fl['gcups']} GCUPS, mismatches {fbad}/40", file=sys.stderr, flush=True)
 
+    pf = prefilter_leg(ctx, args, sch) if args.pf_train > 0 else None
+
     cpu = None
     if not args.no_cpu:
         nthreads = os.cpu_count() or 1
@@ -240,7 +332,7 @@ def main():
             "traffic": measured_traffic(args),
         },
         "cpu_baseline": cpu,
-        "extra": {"xdrop": xd, "full_sw": fl},
+        "extra": {"xdrop": xd, "full_sw": fl, "prefilter": pf},
     }
     print(json.dumps(out), flush=True)
     batch.close()
