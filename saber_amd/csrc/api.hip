@@ -118,6 +118,7 @@ struct saber_hip_ctx {
     std::mutex mu;
     DevBuf<uint8_t> db;
     std::vector<uint64_t> db_off;
+    int cus = 0;                  // compute units of the device
     std::vector<int> choice;      // width -> shape index (-1: exact int32 kernel)
     std::vector<int> choice_lat;  // the same for jobs with many rows
     std::vector<Shape> shapes;
@@ -144,6 +145,8 @@ struct saber_hip_band_batch {
     DevBuf<uint8_t> q, s;
     DevBuf<int8_t> bias;
     DevBuf<uint32_t> table;   // int8 band table as words
+    DevBuf<uint32_t> pair_table;
+    DevBuf<uint32_t> next_chunk;
     DevBuf<int32_t> mat;
     DevBuf<BandJobDev> jobs;
     DevBuf<uint32_t> pairs;
@@ -294,6 +297,8 @@ int saber_hip_create(int device, const saber_hip_opts* opts, saber_hip_ctx** out
     auto* c = new saber_hip_ctx();
     c->device = device;
     c->flags = opts ? opts->flags : 0;
+    if (hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || c->cus < 1)
+        c->cus = 1;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     for (int k = 0; k < kStreams && e == hipSuccess; ++k) e = hipStreamCreateWithFlags(&c->work[k], hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->fork, hipEventDisableTiming);
@@ -472,38 +477,79 @@ int saber_hip_band_batch_create(saber_hip_ctx* c, const saber_hip_seqset* querie
         else
             bucket[2 * shape].push_back((uint32_t)k);
     }
-    // Pairs: each bucket sorted by rows (longest first) and paired with the
-    // next longest, since a pair runs for its longer job's rows. Blocks of
-    // 256 lanes (256 / L pairs) of one shape; all blocks of all shapes are
-    // dispatched in decreasing order of their estimated time (the longest
-    // pair's row steps x the shape's per-row cost), so the hardware
-    // dispatcher packs them longest-processing-time first.
+    // Pairs. A pair runs for its longer job's rows, so jobs are paired with
+    // the next longest of their bucket. Jobs of one query that start on the
+    // same query row are paired among themselves first: such a pair reads
+    // the same query code on every row and uses the pair-table flavour. The
+    // rest (different queries or first rows, odd ones out) pair by rows.
+    // Work chunks of 64 lanes (64 / L pairs) of one shape and flavour are
+    // claimed by the persistent kernel in decreasing order of their estimated
+    // time (the longest pair's row steps x the shape's per-row cost), longest
+    // processing time first.
     std::vector<uint32_t> pairs;
-    std::vector<double> block_est;
-    for (size_t bk = 0; bk < bucket.size(); ++bk) {
-        auto& v = bucket[bk];
-        if (v.empty()) continue;
-        std::stable_sort(v.begin(), v.end(), [&](uint32_t x, uint32_t y) { return rows_of[x] > rows_of[y]; });
-        const int shape = (int)(bk / 2);
+    std::vector<double> chunk_est;
+    auto add_chunks = [&](const std::vector<uint32_t>& v, int shape, uint32_t flags) {
+        if (v.empty()) return;
         const Shape& s = c->shapes[shape];
-        const uint32_t per_block = 256u / (uint32_t)s.L;
+        const uint32_t per_chunk = 64u / (uint32_t)s.L;
         const uint64_t first_pair = pairs.size() / 2;
         for (size_t i = 0; i < v.size(); i += 2) {
             pairs.push_back(v[i]);
             pairs.push_back(i + 1 < v.size() ? v[i + 1] : kNone);
         }
         const uint64_t np = pairs.size() / 2 - first_pair;
-        for (uint64_t p0 = 0; p0 < np; p0 += per_block) {
-            const uint32_t cnt = (uint32_t)std::min<uint64_t>(per_block, np - p0);
+        for (uint64_t p0 = 0; p0 < np; p0 += per_chunk) {
+            const uint32_t cnt = (uint32_t)std::min<uint64_t>(per_chunk, np - p0);
             const int steps = (rows_of[v[2 * p0]] + s.L - 1 + 3) & ~3;
-            b->host_blocks.push_back(BandBlock{(uint32_t)shape, (uint32_t)(first_pair + p0), cnt, 0});
-            block_est.push_back((double)steps * s.row_cost());
+            b->host_blocks.push_back(BandBlock{(uint32_t)shape, (uint32_t)(first_pair + p0), cnt, flags});
+            chunk_est.push_back((double)steps * s.row_cost());
         }
+    };
+    for (size_t bk = 0; bk < bucket.size(); ++bk) {
+        auto& v = bucket[bk];
+        if (v.empty()) continue;
+        const int shape = (int)(bk / 2);
+        // group by (query start, first row), longest first within a group
+        std::stable_sort(v.begin(), v.end(), [&](uint32_t x, uint32_t y) {
+            const BandJobDev &ja = b->host_jobs[x], &jb = b->host_jobs[y];
+            if (ja.q_off != jb.q_off) return ja.q_off < jb.q_off;
+            if (ja.i_first != jb.i_first) return ja.i_first < jb.i_first;
+            return rows_of[x] > rows_of[y];
+        });
+        std::vector<uint32_t> shared, rest;
+        for (size_t i = 0; i < v.size();) {
+            size_t j = i;
+            while (j < v.size() && b->host_jobs[v[j]].q_off == b->host_jobs[v[i]].q_off &&
+                   b->host_jobs[v[j]].i_first == b->host_jobs[v[i]].i_first)
+                ++j;
+            const size_t even = i + ((j - i) & ~(size_t)1);
+            shared.insert(shared.end(), v.begin() + i, v.begin() + even);
+            if (even < j) rest.push_back(v[even]);
+            i = j;
+        }
+        // shared pairs in order of their longer job's rows (pairs stay intact)
+        {
+            std::vector<size_t> order(shared.size() / 2);
+            for (size_t k = 0; k < order.size(); ++k) order[k] = k;
+            std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) {
+                return rows_of[shared[2 * x]] > rows_of[shared[2 * y]];
+            });
+            std::vector<uint32_t> sorted(shared.size());
+            for (size_t k = 0; k < order.size(); ++k) {
+                sorted[2 * k] = shared[2 * order[k]];
+                sorted[2 * k + 1] = shared[2 * order[k] + 1];
+            }
+            shared.swap(sorted);
+        }
+        std::stable_sort(rest.begin(), rest.end(), [&](uint32_t x, uint32_t y) { return rows_of[x] > rows_of[y]; });
+        add_chunks(shared, shape, kChunkShared);
+        add_chunks(rest, shape, 0);
     }
+    if (pairs.size() / 2 >= 0xFFFFFFFFull) return fail(SABER_HIP_EINVAL, "too many job pairs for one batch");
     {
         std::vector<uint32_t> order(b->host_blocks.size());
         for (uint32_t i = 0; i < order.size(); ++i) order[i] = i;
-        std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return block_est[x] > block_est[y]; });
+        std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return chunk_est[x] > chunk_est[y]; });
         std::vector<BandBlock> sorted(order.size());
         for (size_t i = 0; i < order.size(); ++i) sorted[i] = b->host_blocks[order[i]];
         b->host_blocks.swap(sorted);
@@ -520,6 +566,20 @@ int saber_hip_band_batch_create(saber_hip_ctx* c, const saber_hip_seqset* querie
                 const int v = cc == kBandSentinel ? -128 : std::max(-128, std::min(127, scheme->matrix[a * 24 + cc]));
                 t8[a * kTable8Stride + cc] = (int8_t)v;
             }
+    }
+
+    // Pair table of the shared-query flavour: (S(q, a), S(q, b)) per query code
+    // q < 24 and subject codes a, b <= 24, from the int8 table.
+    std::vector<uint32_t> ptable(kPairWords, 0);
+    {
+        const int8_t* t8 = reinterpret_cast<const int8_t*>(table.data());
+        for (int q = 0; q < 24; ++q)
+            for (int x = 0; x <= kBandSentinel; ++x)
+                for (int y = 0; y <= kBandSentinel; ++y) {
+                    const uint16_t lo = (uint16_t)(int16_t)t8[q * kTable8Stride + x];
+                    const uint16_t hi = (uint16_t)(int16_t)t8[q * kTable8Stride + y];
+                    ptable[(size_t)q * 625 + x * 25 + y] = (uint32_t)lo | ((uint32_t)hi << 16);
+                }
     }
 
     hipStream_t st = c->stream;
@@ -540,6 +600,8 @@ int saber_hip_band_batch_create(saber_hip_ctx* c, const saber_hip_seqset* querie
     }
     b->qlen_total = nq;
     if (int r = upload(b->table, table.data(), table.size(), st)) return r;
+    if (int r = upload(b->pair_table, ptable.data(), ptable.size(), st)) return r;
+    HIP_TRY(b->next_chunk.alloc(1));
     if (int r = upload(b->mat, scheme->matrix, 576, st)) return r;
     if (int r = upload(b->jobs, b->host_jobs.data(), njobs, st)) return r;
     if (int r = upload(b->pairs, pairs.data(), pairs.size(), st)) return r;
@@ -594,10 +656,14 @@ static int enqueue_main(saber_hip_band_batch* b, hipEvent_t ev_begin, hipEvent_t
     HIP_TRY(fill_u32(b->q_end.p, n, 0, st));
     HIP_TRY(fill_u32(b->s_end.p, n, 0, st));
     HIP_TRY(fill_u32(b->nflagged.p, 1, 0, st));
+    HIP_TRY(fill_u32(b->next_chunk.p, 1, 0, st));
     BandKernelArgs a{};
     a.pairs = b->pairs.p;
     a.npairs = (uint32_t)(b->pairs.n / 2);
     a.blocks = b->blocks.p;
+    a.nchunks = (uint32_t)b->host_blocks.size();
+    a.next_chunk = b->next_chunk.p;
+    a.pair_table = b->pair_table.p;
     a.jobs = b->jobs.p;
     a.qres = b->q.p + kGuard;
     a.qbias = b->has_bias ? b->bias.p + kGuard : nullptr;
@@ -611,7 +677,9 @@ static int enqueue_main(saber_hip_band_batch* b, hipEvent_t ev_begin, hipEvent_t
     a.flagged = b->flagged.p;
     a.nflagged = b->nflagged.p;
     if (ev_begin) HIP_TRY(hipEventRecord(ev_begin, st));
-    HIP_TRY(launch_band_multi(a, (uint32_t)b->host_blocks.size(), b->has_bias, st));
+    // one persistent workgroup per CU (fewer when there is little work)
+    const uint32_t wg = (uint32_t)std::min<uint64_t>((uint64_t)b->ctx->cus, (b->host_blocks.size() + 11) / 12);
+    if (a.nchunks) HIP_TRY(launch_band_multi(a, wg, b->has_bias, st));
     if (ev_end) HIP_TRY(hipEventRecord(ev_end, st));
     if (int r = run_generic(b, b->generic_static)) return r;
     return SABER_HIP_OK;

@@ -112,14 +112,32 @@ __device__ __forceinline__ uint32_t window_dword(uint32_t raw, int32_t neg8, int
     return (raw & keep) | (kSentinel4 & ~keep);
 }
 
-// One group of L lanes scoring job pair g (lane l of the group); Tb is the
-// score table in LDS.
-template <int L, int C, int K, bool BIAS>
-__device__ __forceinline__ void band_pair(const BandKernelArgs& a, const int8_t* Tb, uint32_t g, int l) {
+// Pair-table entry offset (bytes) of subject codes a (job A) and b (job B):
+// entry a * 25 + b of a query row.
+__device__ __forceinline__ uint32_t pair_entries(uint32_t wa, uint32_t wb, int half) {
+    // bytes 2*half, 2*half+1 of both windows -> two 16-bit entry offsets
+    const uint32_t sel = half ? 0x0C030C02u : 0x0C010C00u;
+    const v2u a = __builtin_bit_cast(v2u, __builtin_amdgcn_perm(0u, wa, sel));
+    const v2u b = __builtin_bit_cast(v2u, __builtin_amdgcn_perm(0u, wb, sel));
+    const v2u e = a * (v2u){100, 100} + (b << (v2u){2, 2});
+    return __builtin_bit_cast(uint32_t, e);
+}
+
+// One group of L lanes scoring job pair g (lane l of the group). Tb is the
+// int8 score table in LDS; Tp the pair table (SHARED only: both jobs of the
+// pair have the same query and first row, so a row's query code is common
+// and one LDS read of Tp[q][sA][sB] returns both scores already packed).
+template <int L, int C, int K, bool BIAS, bool SHARED>
+__device__ __attribute__((noinline)) void band_pair(const BandKernelArgs& a, const int8_t* Tb, const uint32_t* Tp, uint32_t g,
+                                          int l) {
     static_assert(C >= 2 && K >= 1 && K <= C, "variant shape");
     static_assert(L == 1 || L == 2 || L == 4 || L == 8 || L == 16, "groups must tile a DPP row");
-    constexpr int NW = (C + 6) / 4;  // window dwords: C + 3 columns per block
+    constexpr int NW = (C + 6) / 4;               // byte window: dwords of 4 columns
+    constexpr int NP = ((C + 3 + 3) / 4) * 2;     // pair window: dwords of 2 entries
+    constexpr int SPAN = SHARED ? 2 * NP : 4 * NW;  // window columns
+    constexpr int NH = SHARED ? 1 : 2;            // query / bias streams
     constexpr uint32_t W = (uint32_t)(L * C);
+    const char* Tpb = reinterpret_cast<const char*>(Tp);
 
     uint32_t jid[2];
     int64_t qoff[2], soff[2];
@@ -173,31 +191,52 @@ __device__ __forceinline__ void band_pair(const BandKernelArgs& a, const int8_t*
 
     // At step t lane l computes row r = t - l; its cell c is window column
     // jb + t + c and its row is query position ifirst + t - l. Block b (steps
-    // 4b .. 4b+3) reads window columns jb + 4b .. jb + 4b + 4 NW - 1.
-    ByteStream ss[2], qs[2], bs[2];
-    int32_t jb[2], qb[2], neg8[2], hi8[2];
-    uint32_t Wd[2][NW];
-    uint32_t s_n1[2], s_n2[2], q_cur[2], q_n1[2], q_n2[2];
-    uint32_t b_cur[2] = {0, 0}, b_n1[2] = {0, 0}, b_n2[2] = {0, 0};
+    // 4b .. 4b+3) holds window columns jb + 4b .. jb + 4b + SPAN - 1: as bytes
+    // (Wd, per job) or as pair-table entry offsets (Wp, both jobs).
+    ByteStream ss[2], qs[NH], bs[NH];
+    int32_t jb[2], qb[NH], neg8[2], hi8[2];
+    uint32_t Wd[SHARED ? 1 : 2][SHARED ? 1 : NW];
+    uint32_t Wp[SHARED ? NP : 1];
+    uint32_t s_n1[2], s_n2[2], q_cur[NH], q_n1[NH], q_n2[NH];
+    uint32_t b_cur[NH], b_n1[NH], b_n2[NH];
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
         jb[h] = ifirst[h] + lo[h] + l * C - l;
-        qb[h] = ifirst[h] - l;
         ss[h].init(a.sres, soff[h], a.sres_len);
-        qs[h].init(a.qres, qoff[h], a.qres_len);
+        if constexpr (!SHARED) {
 #pragma unroll
-        for (int x = 0; x < NW; ++x) {
-            const int32_t j0 = jb[h] + 4 * x;
-            Wd[h][x] = window_dword(ss[h].load(j0), -8 * j0, 8 * ((int32_t)n[h] - j0));
+            for (int x = 0; x < NW; ++x) {
+                const int32_t j0 = jb[h] + 4 * x;
+                Wd[h][x] = window_dword(ss[h].load(j0), -8 * j0, 8 * ((int32_t)n[h] - j0));
+            }
         }
         // Column of the dword appended after block 0, as -8j and 8(n - j).
-        neg8[h] = -8 * (jb[h] + 4 * NW);
-        hi8[h] = 8 * ((int32_t)n[h] - jb[h] - 4 * NW);
-        s_n1[h] = ss[h].load(jb[h] + 4 * NW);
-        s_n2[h] = ss[h].load(jb[h] + 4 * NW + 4);
+        neg8[h] = -8 * (jb[h] + SPAN);
+        hi8[h] = 8 * ((int32_t)n[h] - jb[h] - SPAN);
+        s_n1[h] = ss[h].load(jb[h] + SPAN);
+        s_n2[h] = ss[h].load(jb[h] + SPAN + 4);
+    }
+    if constexpr (SHARED) {
+#pragma unroll
+        for (int x = 0; x < NP / 2; ++x) {
+            uint32_t wv[2];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int32_t j0 = jb[h] + 4 * x;
+                wv[h] = window_dword(ss[h].load(j0), -8 * j0, 8 * ((int32_t)n[h] - j0));
+            }
+            Wp[2 * x] = pair_entries(wv[0], wv[1], 0);
+            Wp[2 * x + 1] = pair_entries(wv[0], wv[1], 1);
+        }
+    }
+#pragma unroll
+    for (int h = 0; h < NH; ++h) {
+        qb[h] = ifirst[h] - l;
+        qs[h].init(a.qres, qoff[h], a.qres_len);
         q_cur[h] = qs[h].load(qb[h]);
         q_n1[h] = qs[h].load(qb[h] + 4);
         q_n2[h] = qs[h].load(qb[h] + 8);
+        b_cur[h] = b_n1[h] = b_n2[h] = 0;
         if constexpr (BIAS) {
             bs[h].init(reinterpret_cast<const uint8_t*>(a.qbias), qoff[h], a.qres_len);
             b_cur[h] = bs[h].load(qb[h]);
@@ -217,11 +256,20 @@ __device__ __forceinline__ void band_pair(const BandKernelArgs& a, const int8_t*
         for (int u = 0; u < 4; ++u) {
             const int r = 4 * blk + u - l;
             const bool act = (unsigned)r < (unsigned)R;
-            const uint32_t rb0 = ((q_cur[0] >> (8 * u)) & 0xFFu) * kTable8Stride;
-            const uint32_t rb1 = ((q_cur[1] >> (8 * u)) & 0xFFu) * kTable8Stride;
+            uint32_t rb0, rb1 = 0;
             v2s bpair = ZERO;
-            if constexpr (BIAS)
-                bpair = v2s{(short)(int8_t)(b_cur[0] >> (8 * u)), (short)(int8_t)(b_cur[1] >> (8 * u))};
+            if constexpr (SHARED) {
+                rb0 = ((q_cur[0] >> (8 * u)) & 0xFFu) * kPairRowBytes;
+                if constexpr (BIAS) {
+                    const short bv = (short)(int8_t)(b_cur[0] >> (8 * u));
+                    bpair = v2s{bv, bv};
+                }
+            } else {
+                rb0 = ((q_cur[0] >> (8 * u)) & 0xFFu) * kTable8Stride;
+                rb1 = ((q_cur[NH - 1] >> (8 * u)) & 0xFFu) * kTable8Stride;
+                if constexpr (BIAS)
+                    bpair = v2s{(short)(int8_t)(b_cur[0] >> (8 * u)), (short)(int8_t)(b_cur[NH - 1] >> (8 * u))};
+            }
             v2s inE = ZERO;
             if constexpr (L > 1) {
                 inE = pair(from_prev_lane(outE));
@@ -231,9 +279,17 @@ __device__ __forceinline__ void band_pair(const BandKernelArgs& a, const int8_t*
             // One cell: diagonal H[c], E from the left, F arriving from above.
             auto cell = [&](int c, v2s fin) {
                 const int p = c + u;
-                const uint32_t ca = (Wd[0][p >> 2] >> (8 * (p & 3))) & 0xFFu;
-                const uint32_t cb = (Wd[1][p >> 2] >> (8 * (p & 3))) & 0xFFu;
-                v2s P = v2s{(short)Tb[rb0 + ca], (short)Tb[rb1 + cb]};
+                v2s P;
+                if constexpr (SHARED) {
+                    const uint32_t e = (Wp[p >> 1] >> (16 * (p & 1))) & 0xFFFFu;
+                    uint32_t pv;
+                    __builtin_memcpy(&pv, Tpb + rb0 + e, 4);
+                    P = pair(pv);
+                } else {
+                    const uint32_t ca = (Wd[0][p >> 2] >> (8 * (p & 3))) & 0xFFu;
+                    const uint32_t cb = (Wd[NH - 1][p >> 2] >> (8 * (p & 3))) & 0xFFu;
+                    P = v2s{(short)Tb[rb0 + ca], (short)Tb[rb1 + cb]};
+                }
                 if constexpr (BIAS) P = vadd(P, bpair);
                 const v2s D = vadd(H[c], P);
                 v2s hv = vmax(vmax(D, fin), E);
@@ -277,17 +333,30 @@ __device__ __forceinline__ void band_pair(const BandKernelArgs& a, const int8_t*
                 }
             }
         }
-        // Next block: the window moves one dword, the streams one dword.
+        // Next block: the window moves four columns, the streams one dword.
         const int32_t nb = 4 * blk;
+        uint32_t wnew[2];
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-#pragma unroll
-            for (int x = 0; x < NW - 1; ++x) Wd[h][x] = Wd[h][x + 1];
-            Wd[h][NW - 1] = window_dword(s_n1[h], neg8[h], hi8[h]);
+            wnew[h] = window_dword(s_n1[h], neg8[h], hi8[h]);
             neg8[h] -= 32;
             hi8[h] -= 32;
             s_n1[h] = s_n2[h];
-            s_n2[h] = ss[h].load(jb[h] + 4 * NW + nb + 8);
+            s_n2[h] = ss[h].load(jb[h] + SPAN + nb + 8);
+            if constexpr (!SHARED) {
+#pragma unroll
+                for (int x = 0; x < NW - 1; ++x) Wd[h][x] = Wd[h][x + 1];
+                Wd[h][NW - 1] = wnew[h];
+            }
+        }
+        if constexpr (SHARED) {
+#pragma unroll
+            for (int x = 0; x < NP - 2; ++x) Wp[x] = Wp[x + 2];
+            Wp[NP - 2] = pair_entries(wnew[0], wnew[1], 0);
+            Wp[NP - 1] = pair_entries(wnew[0], wnew[1], 1);
+        }
+#pragma unroll
+        for (int h = 0; h < NH; ++h) {
             q_cur[h] = q_n1[h];
             q_n1[h] = q_n2[h];
             q_n2[h] = qs[h].load(qb[h] + nb + 12);
@@ -356,27 +425,45 @@ __device__ __forceinline__ void band_pair(const BandKernelArgs& a, const int8_t*
     X(8, 9, 9) X(8, 13, 13) X(8, 17, 8) X(8, 17, 17)                                    \
     X(16, 9, 9) X(16, 13, 13) X(16, 17, 16) X(16, 17, 17) X(16, 21, 21)
 
-// All shapes in one launch: block b scores the pairs blocks[b] names with the
-// shape it names, so the planner orders the blocks of every shape together
-// (longest first) and the hardware dispatcher packs them. At least two waves
-// per SIMD: the compiler must fit the largest shape into 256 registers.
+// All shapes in one persistent launch: one workgroup of 12 waves per CU
+// stages both score tables in LDS once, then each wave claims work chunks in
+// the planner's order (longest first) with one atomic add per chunk until the
+// list is exhausted. A chunk is up to 64 lanes of one shape and one flavour:
+// the pair-table flavour for pairs that share their query rows, the per-job
+// table otherwise. 12 waves per CU = three per SIMD, so every shape must fit
+// 168 registers.
 template <bool BIAS>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) void band_multi_kernel(BandKernelArgs a) {
+__global__ __launch_bounds__(768) void band_multi_kernel(BandKernelArgs a) {
     __shared__ uint32_t T4[kTable8Words];
+    __shared__ uint32_t TP[kPairWords];
     for (int t = threadIdx.x; t < kTable8Words; t += blockDim.x) T4[t] = a.table[t];
+    for (int t = threadIdx.x; t < kPairWords; t += blockDim.x) TP[t] = a.pair_table[t];
     __syncthreads();
     const int8_t* Tb = reinterpret_cast<const int8_t*>(T4);
-    const BandBlock blk = a.blocks[blockIdx.x];
-    int id = 0;
-#define SABER_BAND_CASE(L_, C_, K_)                                                     \
-    if (blk.shape == (uint32_t)id) {                                                    \
-        const uint32_t q = threadIdx.x / (L_);                                          \
-        if (q < blk.npairs) band_pair<L_, C_, K_, BIAS>(a, Tb, blk.pair_off + q, (int)(threadIdx.x % (L_))); \
-        return;                                                                         \
-    }                                                                                   \
-    ++id;
-    SABER_BAND_SHAPES(SABER_BAND_CASE)
+    const int lane = threadIdx.x & 63;
+    for (;;) {
+        uint32_t ci = 0;
+        if (lane == 0) ci = atomicAdd(a.next_chunk, 1u);
+        ci = __builtin_amdgcn_readfirstlane(ci);
+        if (ci >= a.nchunks) break;
+        const BandBlock blk = a.blocks[ci];
+        const bool shared = (blk.flags & kChunkShared) != 0;
+        int id = 0;
+#define SABER_BAND_CASE(L_, C_, K_)                                                              \
+        if (blk.shape == (uint32_t)id) {                                                         \
+            const uint32_t q = (uint32_t)lane / (L_);                                            \
+            if (q < blk.npairs) {                                                                \
+                if (shared)                                                                      \
+                    band_pair<L_, C_, K_, BIAS, true>(a, Tb, TP, blk.pair_off + q, lane % (L_));  \
+                else                                                                             \
+                    band_pair<L_, C_, K_, BIAS, false>(a, Tb, TP, blk.pair_off + q, lane % (L_)); \
+            }                                                                                    \
+            continue;                                                                            \
+        }                                                                                        \
+        ++id;
+        SABER_BAND_SHAPES(SABER_BAND_CASE)
 #undef SABER_BAND_CASE
+    }
 }
 
 }  // namespace saber

@@ -44,16 +44,27 @@ struct BandOutputs {
     uint64_t* cells;
 };
 
-// One block of the multi-shape band launch: pairs [pair_off, pair_off +
-// npairs) of the pair list, scored with shape `shape`.
+// One work chunk of the multi-shape band launch (one wave): pairs
+// [pair_off, pair_off + npairs) of the pair list, scored with shape `shape`;
+// flags: kChunkShared when both jobs of every pair share their query rows.
 struct BandBlock {
-    uint32_t shape, pair_off, npairs, pad;
+    uint32_t shape, pair_off, npairs, flags;
 };
+constexpr uint32_t kChunkShared = 1u;
+
+// Pair table of the shared-query flavour: for each query code q < 24 and
+// subject codes a, b <= 24 (24 = sentinel), the dword (S(q, a), S(q, b)) as
+// two int16, at byte (q * 625 + a * 25 + b) * 4.
+constexpr uint32_t kPairRowBytes = 625 * 4;
+constexpr int kPairWords = 24 * 625;
 
 struct BandKernelArgs {
     const uint32_t* pairs;      // 2 job ids per pair (0xFFFFFFFF = none)
     uint32_t npairs;
-    const BandBlock* blocks;    // multi-shape launch: one entry per block
+    const BandBlock* blocks;    // work chunks in dispatch order
+    uint32_t nchunks;
+    uint32_t* next_chunk;       // claim counter (zero before the launch)
+    const uint32_t* pair_table; // kPairWords words (global; staged into LDS)
     const BandJobDev* jobs;
     const uint8_t* qres;        // device buffers with 16 guard bytes before and
     const int8_t* qbias;        // 64 after (qbias may be null without bias)
@@ -91,7 +102,7 @@ struct BandVariant {
 };
 
 const BandVariant* band_variants(int* count);
-hipError_t launch_band_multi(const BandKernelArgs& a, uint32_t nblocks, bool bias, hipStream_t s);
+hipError_t launch_band_multi(const BandKernelArgs& a, uint32_t nworkgroups, bool bias, hipStream_t s);
 hipError_t launch_generic(const GenericArgs& a, hipStream_t s);
 
 }  // namespace saber

@@ -196,10 +196,28 @@ def test_every_variant_width(ctx):
     assert_same(got, want, "widths")
 
 
+def run_shared(ctx, qs, ss, jq, lo, hi, mat, go, ge, bias=None):
+    """Jobs (query jq[k], subject k, band [lo, hi]) where several jobs share a
+    query: pairs of jobs with the same query and first row use the kernel's
+    pair-table flavour, the rest the per-job flavour."""
+    qres, qoff = flat(qs)
+    sres, soff = flat(ss)
+    n = len(ss)
+    jobs = jobs_array(np.asarray(jq), np.arange(n), np.zeros(n), [len(s) for s in ss], lo, hi)
+    qb = None
+    if bias is not None:
+        qb, _ = flat([np.asarray(b, np.int8).view(np.uint8) for b in bias])
+        qb = qb.view(np.int8)
+    got = ctx.score_bands(qres, qoff, sres, soff, jobs, S.Scheme(mat, go, ge), qb)
+    want = O.banded_many(qres, qoff, sres, soff, jobs, mat, go, ge, qb)
+    return got, want
+
+
 def test_each_compiled_variant():
-    """Every compiled packed variant (with and without bias), forced over the
-    widths it serves, on related pairs: the planner's cost model leaves some
-    of them unused by default."""
+    """Every compiled packed variant, both kernel flavours (jobs sharing a
+    query and first row, and not), with and without bias, forced over the
+    widths it serves: the planner's cost model leaves some shapes unused by
+    default."""
     r = Lcg(77)
     shapes = sorted({(L, C, K) for L, C, K, _ in S.band_variants()})
     for L, C, K in shapes:
@@ -208,22 +226,28 @@ def test_each_compiled_variant():
         widths = sorted({lo_w, (lo_w + W) // 2, W})
         ctx = S.Context()
         ctx.choose(lo_w, W, L, C, K)
-        qs, ss, lo, hi, bias = [], [], [], [], []
+        qs, ss, jq, lo, hi, bias = [], [], [], [], [], []
         for w in widths:
-            for t in range(6):
-                # long enough that the band is not clipped to a narrower width
-                m = W // 2 + 40 + r.next(200)
-                a = rand_seq(r, m)
+            # long enough that the band is not clipped to a narrower width
+            m = W // 2 + 40 + r.next(200)
+            a = rand_seq(r, m)
+            qs.append(a)
+            bias.append(np.array([r.next(9) - 4 for _ in range(m)], np.int8))
+            for t in range(7):
+                if t == 6:          # an odd one out, alone on its own query
+                    a = rand_seq(r, m)
+                    qs.append(a)
+                    bias.append(np.array([r.next(9) - 4 for _ in range(m)], np.int8))
                 b = mutate(r, a) if t % 3 else rand_seq(r, m + r.next(40))
                 if len(b) < W // 2 + 20:
                     b = rand_seq(r, m)
                 c = int(r.next(31)) - 15
-                qs.append(a); ss.append(b)
+                ss.append(b)
+                jq.append(len(qs) - 1)
                 lo.append(c - w // 2); hi.append(c - w // 2 + w - 1)
-                bias.append(np.array([r.next(9) - 4 for _ in range(m)], np.int8))
-        got, want = run_both(ctx, qs, ss, lo, hi, BLOSUM62, 11, 1)
+        got, want = run_shared(ctx, qs, ss, jq, lo, hi, BLOSUM62, 11, 1)
         assert_same(got, want, f"variant {(L, C, K)}")
-        got, want = run_both(ctx, qs, ss, lo, hi, BLOSUM62, 11, 1, bias=bias)
+        got, want = run_shared(ctx, qs, ss, jq, lo, hi, BLOSUM62, 11, 1, bias=bias)
         assert_same(got, want, f"variant {(L, C, K)} bias")
         ctx.close()
 
