@@ -860,22 +860,24 @@ struct saber_hip_xdrop_batch {
     DevBuf<uint32_t> hits;
     DevBuf<short> table;
     DevBuf<uint32_t> counter;
-    DevBuf<int32_t> part_best, score;
-    DevBuf<uint32_t> part_t, part_steps, q_lo, q_hi, steps;
+    DevBuf<uint4> desc;
+    DevBuf<uint4> res;               // per hit: score, q_lo, q_hi, steps
     const uint8_t* sres = nullptr;
     const uint64_t* soffp = nullptr;
-    bool has_bias = false, ran = false;
+    bool has_bias = false, wide = false, ran = false;
     int grid = 0;
+    uint32_t chunk = 0;
 };
 
 namespace {
 
 int xdrop_enqueue(saber_hip_xdrop_batch* b) {
     hipStream_t st = b->ctx->stream;
-    HIP_TRY(fill_u32(b->counter.p, 1, 0, st));
+    HIP_TRY(fill_u32(b->counter.p, kXdropGroups * kXdropCounterStride, 0, st));
     XdropArgs a{};
     a.hits = b->hits.p;
     a.nhits = (uint32_t)b->nhits;
+    a.chunk = b->chunk;
     a.qoff = b->qoff.p;
     a.qres = b->q.p + kGuard;
     a.qbias = b->has_bias ? b->bias.p + kGuard : nullptr;
@@ -883,11 +885,10 @@ int xdrop_enqueue(saber_hip_xdrop_batch* b) {
     a.sres = b->sres;
     a.table = b->table.p;
     a.x = b->x;
-    a.next_item = b->counter.p;
-    a.part_best = b->part_best.p;
-    a.part_t = b->part_t.p;
-    a.part_steps = b->part_steps.p;
-    HIP_TRY(launch_xdrop(a, b->score.p, b->q_lo.p, b->q_hi.p, b->steps.p, b->grid, st));
+    a.next_chunk = b->counter.p;
+    a.desc = b->desc.p;
+    a.res = b->res.p;
+    HIP_TRY(launch_xdrop(a, b->has_bias, b->wide, b->grid, st));
     return SABER_HIP_OK;
 }
 
@@ -926,11 +927,13 @@ int saber_hip_xdrop_batch_create(saber_hip_ctx* c, const saber_hip_seqset* queri
     b->nhits = nhits;
     b->x = x;
     b->has_bias = queries->bias != nullptr;
-    std::vector<short> table(kTableDim * kTableDim);
-    for (int a = 0; a < kTableDim; ++a)
-        for (int cc = 0; cc < kTableDim; ++cc)
-            table[a * kTableDim + cc] =
-                (short)scheme->matrix[(a < 24 ? a : kCodeX) * 24 + (cc < 24 ? cc : kCodeX)];
+    std::vector<short> table(kXdropTable);
+    for (int a = 0; a < 32; ++a)
+        for (int cc = 0; cc < 256; ++cc) {
+            const int32_t v = scheme->matrix[(a < 24 ? a : kCodeX) * 24 + (cc < 24 ? cc : kCodeX)];
+            table[a * 256 + cc] = (short)v;
+            if (v < -128 || v > 127) b->wide = true;
+        }
     hipStream_t st = c->stream;
     const uint64_t nq = queries->count ? queries->offsets[queries->count] : 0;
     if (int r = upload_codes(b->q, queries->residues, nq, st)) return r;
@@ -951,20 +954,20 @@ int saber_hip_xdrop_batch_create(saber_hip_ctx* c, const saber_hip_seqset* queri
     }
     if (int r = upload(b->hits, reinterpret_cast<const uint32_t*>(hits), 4 * nhits, st)) return r;
     if (int r = upload(b->table, table.data(), table.size(), st)) return r;
-    HIP_TRY(b->counter.alloc(1));
+    HIP_TRY(b->counter.alloc(kXdropGroups * kXdropCounterStride));
     const uint64_t n1 = nhits ? nhits : 1;
-    HIP_TRY(b->part_best.alloc(2 * n1));
-    HIP_TRY(b->part_t.alloc(2 * n1));
-    HIP_TRY(b->part_steps.alloc(2 * n1));
-    HIP_TRY(b->score.alloc(n1));
-    HIP_TRY(b->q_lo.alloc(n1));
-    HIP_TRY(b->q_hi.alloc(n1));
-    HIP_TRY(b->steps.alloc(n1));
+    HIP_TRY(b->desc.alloc(2 * n1));
+    HIP_TRY(b->res.alloc(n1));
     int cus = 0;
     HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
-    // Persistent grid: 8 blocks of 4 waves per CU, never more than the work.
-    const uint64_t need = (2 * nhits + 255) / 256;
-    b->grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)cus * 8, need));
+    // Persistent grid: as many blocks of 4 waves as are resident at once, never
+    // more than the work; a group's hits are cut into chunks of about a
+    // quarter of a wave's share (at least 64: one descriptor load).
+    const uint64_t need = (nhits + 255) / 256;
+    b->grid = (int)std::max<uint64_t>(
+        1, std::min<uint64_t>((uint64_t)cus * xdrop_blocks_per_cu(b->has_bias, b->wide), need));
+    const uint64_t waves = 4ull * (uint64_t)b->grid;
+    b->chunk = (uint32_t)std::max<uint64_t>(64, nhits / (4 * waves));
     HIP_TRY(hipStreamSynchronize(st));
     *out = b.release();
     return SABER_HIP_OK;
@@ -988,11 +991,15 @@ int saber_hip_xdrop_batch_read(saber_hip_xdrop_batch* b, int32_t* score, uint32_
     hipStream_t st = b->ctx->stream;
     const uint64_t n = b->nhits;
     if (n == 0) return SABER_HIP_OK;
-    if (score) HIP_TRY(hipMemcpyAsync(score, b->score.p, 4 * n, hipMemcpyDeviceToHost, st));
-    if (q_lo) HIP_TRY(hipMemcpyAsync(q_lo, b->q_lo.p, 4 * n, hipMemcpyDeviceToHost, st));
-    if (q_hi) HIP_TRY(hipMemcpyAsync(q_hi, b->q_hi.p, 4 * n, hipMemcpyDeviceToHost, st));
-    if (steps) HIP_TRY(hipMemcpyAsync(steps, b->steps.p, 4 * n, hipMemcpyDeviceToHost, st));
+    std::vector<uint4> r(n);
+    HIP_TRY(hipMemcpyAsync(r.data(), b->res.p, sizeof(uint4) * n, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
+    for (uint64_t i = 0; i < n; ++i) {
+        if (score) score[i] = (int32_t)r[i].x;
+        if (q_lo) q_lo[i] = r[i].y;
+        if (q_hi) q_hi[i] = r[i].z;
+        if (steps) steps[i] = r[i].w;
+    }
     return SABER_HIP_OK;
 }
 
@@ -1019,10 +1026,10 @@ int saber_hip_xdrop_batch_time(saber_hip_xdrop_batch* b, int warmup, int iters, 
     (void)hipEventDestroy(e1);
     if (ms_run) *ms_run = ms / iters;
     if (steps) {
-        std::vector<uint32_t> s(b->nhits);
-        if (b->nhits) HIP_TRY(hipMemcpy(s.data(), b->steps.p, 4 * b->nhits, hipMemcpyDeviceToHost));
+        std::vector<uint4> r(b->nhits);
+        if (b->nhits) HIP_TRY(hipMemcpy(r.data(), b->res.p, sizeof(uint4) * b->nhits, hipMemcpyDeviceToHost));
         uint64_t t = 0;
-        for (uint32_t v : s) t += v;
+        for (const uint4& v : r) t += v.w;
         *steps = t;
     }
     return SABER_HIP_OK;

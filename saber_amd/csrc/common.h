@@ -109,24 +109,33 @@ hipError_t launch_generic(const GenericArgs& a, hipStream_t s);
 
 namespace saber {
 
+// X-drop score table: 32 query rows x 256 subject columns, indexed by
+// (q << 8 | s); rows and columns past the 24 codes hold X scores.
+constexpr int kXdropTable = 32 * 256;
+// hit ranges / claim counters of the X-drop kernel (counters 256 B apart)
+constexpr uint32_t kXdropGroups = 8;
+constexpr uint32_t kXdropCounterStride = 64;
+
 struct XdropArgs {
     const uint32_t* hits;      // 4 u32 per hit: query, subject, qp, sp
     uint32_t nhits;
+    uint32_t chunk;            // hits per claim
     const uint64_t* qoff;      // query offsets (count+1)
     const uint8_t* qres;
     const int8_t* qbias;       // may be null
     const uint64_t* soff;      // subject offsets
     const uint8_t* sres;
-    const short* table;        // 32 x 32
+    const short* table;        // kXdropTable entries
     int32_t x;
-    uint32_t* next_item;       // work counter (cleared before launch)
-    int32_t* part_best;        // 2 per hit: right, left
-    uint32_t* part_t;          // 2 per hit: end_t / st_t
-    uint32_t* part_steps;      // 2 per hit
+    uint32_t* next_chunk;      // kXdropGroups claim counters, kXdropCounterStride apart (cleared before launch)
+    void* desc;                // 32 B per hit, written by the descriptor pass
+    void* res;                 // 16 B per hit: score, q_lo, q_hi, steps
 };
 
-hipError_t launch_xdrop(const XdropArgs& a, int32_t* score, uint32_t* q_lo, uint32_t* q_hi,
-                        uint32_t* steps, int grid, hipStream_t s);
+// Workgroups of the X-drop kernel resident per CU (its grid is CUs x this).
+int xdrop_blocks_per_cu(bool bias, bool wide);
+// wide: some score lies outside int8 (int16 table in LDS).
+hipError_t launch_xdrop(const XdropArgs& a, bool bias, bool wide, int grid, hipStream_t s);
 
 }  // namespace saber
 

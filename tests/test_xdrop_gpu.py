@@ -67,3 +67,19 @@ def test_bad_hit_is_rejected(ctx):
     sres, soff = flat([rand_seq(Lcg(2), 10)])
     with pytest.raises(S.HipError):
         ctx.xdrop(qres, qoff, sres, soff, np.array([[0, 0, 10, 0]], np.uint32), S.Scheme(BLOSUM62, 11, 1), 20)
+
+
+def test_wide_scores_and_uploaded_database(ctx):
+    """Scores outside int8 (BLOSUM62 x 20) take the int16 table; subjects come
+    from the uploaded database; the oracle is the same restatement."""
+    d = workload.band_regions(seed=11, n=2000, per_query=50)
+    hits = workload.xdrop_hits(d, seed=5, per_region=3)
+    mat = (np.asarray(BLOSUM62, np.int64) * 20).tolist()
+    sch = S.Scheme(mat, 220, 20)
+    bias = ((np.arange(len(d["qres"])) * 7) % 11 - 5).astype(np.int8)
+    ctx.upload_db(d["sres"], d["soff"])
+    for qb in (None, bias):
+        got = np.stack(ctx.xdrop(d["qres"], d["qoff"], None, None, hits, sch, 400, qb), axis=1).astype(np.int64)
+        want = oracle_all(d["qres"], d["qoff"], d["sres"], d["soff"], hits, mat, 400, qb)
+        bad = np.nonzero((got != want).any(axis=1))[0]
+        assert len(bad) == 0, f"bias={qb is not None} hits {bad[:5]}: {got[bad[:3]]} vs {want[bad[:3]]}"
