@@ -20,7 +20,8 @@ The printed JSON line also carries
                 (oracle/cpu_avx2.c) on all host threads over a bounded sample
                 of whole queries of the same batch;
   extra.xdrop   X-drop extension of 4 seed hits per region (steps/s);
-  extra.full_sw exhaustive full-matrix SW (queries x subjects, GCUPS);
+  extra.full_sw exhaustive full-matrix SW (queries x subjects, GCUPS; scores
+                only, as score_many, and with the end cell);
   extra.prefilter  the learned region prefilter: hit loss, kept fraction,
                 scoring-stage time with and without it, MLP forward
                 throughput against the bf16 MFMA peak.
@@ -249,20 +250,29 @@ def main():
     if args.full_queries > 0:
         fq, fqo = workload.sequences(args.seed + 100, args.full_queries, 250, 0.6, 30, 3000)
         fs, fso = workload.sequences(args.seed + 200, args.full_subjects, 250, 0.6, 30, 3000)
-        fb = ctx.full_batch(fq, fqo, fs, fso, sch)
+        # score_many returns scores only: the headline figure is the scores-only
+        # kernel; the end-cell kernel (sw_reference's first optimal cell) is
+        # timed beside it.
+        fb = ctx.full_batch(fq, fqo, fs, fso, sch, ends=False)
         fms, fcells, fresc = fb.time(warmup=1, iters=2)
-        fsc, fqe, fse = fb.read()
+        fsc = fb.read()[0]
+        fb.close()
+        fb = ctx.full_batch(fq, fqo, fs, fso, sch, ends=True)
+        ems, _, _ = fb.time(warmup=1, iters=2)
+        esc, fqe, fse = fb.read()
         fb.close()
         fbad = 0
         for k in range(40):
             a, b = int(rng.integers(args.full_queries)), int(rng.integers(args.full_subjects))
             w = O.sw_reference(fq[fqo[a]:fqo[a + 1]], fs[fso[b]:fso[b + 1]], BLOSUM62, None, 11, 1)
-            fbad += int((int(fsc[a, b]), int(fqe[a, b]), int(fse[a, b])) != w)
+            fbad += int(int(fsc[a, b]) != w[0]) + int((int(esc[a, b]), int(fqe[a, b]), int(fse[a, b])) != w)
         fl = {"queries": args.full_queries, "subjects": args.full_subjects, "ms": round(fms, 3),
               "cells": int(fcells), "gcups": round(fcells / (fms * 1e-3) / 1e9, 1),
+              "ms_with_ends": round(ems, 3), "gcups_with_ends": round(fcells / (ems * 1e-3) / 1e9, 1),
               "int32_rescored_pairs": int(fresc), "parity_sample": 40, "parity_mismatches": fbad}
         print(f"# full SW {args.full_queries} x {args.full_subjects}: {fms:.2f} ms, "
-              f"{fl['gcups']} GCUPS, mismatches {fbad}/40", file=sys.stderr, flush=True)
+              f"{fl['gcups']} GCUPS (with end cells {ems:.2f} ms, {fl['gcups_with_ends']} GCUPS), "
+              f"mismatches {fbad}/40", file=sys.stderr, flush=True)
 
     pf = prefilter_leg(ctx, args, sch) if args.pf_train > 0 else None
 

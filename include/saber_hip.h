@@ -147,13 +147,16 @@ int saber_hip_band_batch_time(saber_hip_band_batch* b, int warmup, int iters, do
  * exhaustive preset, src/search/presets.rs:318-331): every query against
  * every subject. Outputs are row-major [query][subject]: score, exclusive
  * ends of the first row-major optimal cell (0, 0 when the score is 0).
- * subjects == NULL selects the uploaded database. q_end / s_end may be NULL. */
+ * subjects == NULL selects the uploaded database. q_end / s_end may be NULL:
+ * when both are, only scores are computed (score_many's output, and faster).
+ * A batch computes end cells when created with ends != 0; reading ends from
+ * a scores-only batch is SABER_HIP_ESTATE. */
 typedef struct saber_hip_full_batch saber_hip_full_batch;
 int saber_hip_score_full(saber_hip_ctx* ctx, const saber_hip_seqset* queries,
                          const saber_hip_seqset* subjects, const saber_hip_scheme* scheme,
                          int32_t* score, uint32_t* q_end, uint32_t* s_end);
 int saber_hip_full_batch_create(saber_hip_ctx* ctx, const saber_hip_seqset* queries,
-                                const saber_hip_seqset* subjects, const saber_hip_scheme* scheme,
+                                const saber_hip_seqset* subjects, const saber_hip_scheme* scheme, int ends,
                                 saber_hip_full_batch** out);
 int saber_hip_full_batch_run(saber_hip_full_batch* b);
 int saber_hip_full_batch_read(saber_hip_full_batch* b, int32_t* score, uint32_t* q_end,

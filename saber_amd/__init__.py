@@ -187,8 +187,8 @@ class BandBatch:
 class FullBatch:
     """Device-resident exhaustive batch (saber_hip_full_batch_*)."""
 
-    def __init__(self, ctx, handle, nq, ns, keep):
-        self.ctx, self.h, self.nq, self.ns, self._keep = ctx, handle, nq, ns, keep
+    def __init__(self, ctx, handle, nq, ns, keep, ends=True):
+        self.ctx, self.h, self.nq, self.ns, self._keep, self.ends = ctx, handle, nq, ns, keep, ends
         _live_batches.add(self)
         ctx._children.add(self)
 
@@ -196,10 +196,12 @@ class FullBatch:
         _ffi.check(_ffi.lib().saber_hip_full_batch_run(self.h))
 
     def read(self):
+        """(score, q_end, s_end) of shape [queries, subjects]; a scores-only
+        batch returns (score, None, None)."""
         shape = (self.nq, self.ns)
         score = np.zeros(shape, np.int32)
-        qe = np.zeros(shape, np.uint32)
-        se = np.zeros(shape, np.uint32)
+        qe = np.zeros(shape, np.uint32) if self.ends else None
+        se = np.zeros(shape, np.uint32) if self.ends else None
         _ffi.check(_ffi.lib().saber_hip_full_batch_read(
             self.h, _ffi.ptr(score, C.c_int32), _ffi.ptr(qe, C.c_uint32), _ffi.ptr(se, C.c_uint32)))
         return score, qe, se
@@ -402,20 +404,22 @@ class Context:
             len(hits), C.byref(sch), int(x), C.byref(h)))
         return XdropBatch(self, h, len(hits), keep)
 
-    def full_batch(self, qres, qoff, sres, soff, scheme, qbias=None):
-        """Device-resident exhaustive batch: every query x every subject."""
+    def full_batch(self, qres, qoff, sres, soff, scheme, qbias=None, ends=True):
+        """Device-resident exhaustive batch: every query x every subject
+        (ends=False: scores only, as score_many)."""
         qs, ss, keep = self._seqsets(qres, qoff, qbias, sres, soff)
         sch = scheme.ffi()
         h = C.c_void_p()
         _ffi.check(_ffi.lib().saber_hip_full_batch_create(
-            self.h, C.byref(qs), None if ss is None else C.byref(ss), C.byref(sch), C.byref(h)))
+            self.h, C.byref(qs), None if ss is None else C.byref(ss), C.byref(sch), int(bool(ends)), C.byref(h)))
         nq = len(keep[1]) - 1
         ns = (len(keep[4]) - 1) if ss is not None else None
-        return FullBatch(self, h, nq, ns, keep)
+        return FullBatch(self, h, nq, ns, keep, bool(ends))
 
-    def score_full(self, qres, qoff, sres, soff, scheme, qbias=None):
-        """(score, q_end, s_end) arrays of shape [queries, subjects]."""
-        b = self.full_batch(qres, qoff, sres, soff, scheme, qbias)
+    def score_full(self, qres, qoff, sres, soff, scheme, qbias=None, ends=True):
+        """(score, q_end, s_end) arrays of shape [queries, subjects] (q_end and
+        s_end None when ends=False)."""
+        b = self.full_batch(qres, qoff, sres, soff, scheme, qbias, ends)
         try:
             b.run()
             return b.read()

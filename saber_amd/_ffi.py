@@ -106,7 +106,7 @@ _SIGS = {
                                        C.POINTER(C.c_int32), C.POINTER(C.c_uint32),
                                        C.POINTER(C.c_uint32)]),
     "saber_hip_full_batch_create": (C.c_int, [P, C.POINTER(SeqSet), C.POINTER(SeqSet),
-                                              C.POINTER(Scheme), C.POINTER(P)]),
+                                              C.POINTER(Scheme), C.c_int, C.POINTER(P)]),
     "saber_hip_full_batch_run": (C.c_int, [P]),
     "saber_hip_full_batch_read": (C.c_int, [P, C.POINTER(C.c_int32), C.POINTER(C.c_uint32),
                                             C.POINTER(C.c_uint32)]),

@@ -227,6 +227,22 @@ void default_choice(saber_hip_ctx* c) {
     }
 }
 
+// Pair table of the packed kernels: TP[q][a][b] = S(q,a) | S(q,b) << 16 (int16
+// halves) for query codes q < 24 and subject codes a, b <= 24, where code 24
+// (the sentinel column) scores -128 and the scores are clamped to int8 (the
+// packed kernels only run when every score fits).
+std::vector<uint32_t> build_pair_table(const int32_t* matrix) {
+    std::vector<uint32_t> t(kPairWords, 0);
+    auto s8 = [&](int q, int x) -> int16_t {
+        return (int16_t)(x == kBandSentinel ? -128 : std::max(-128, std::min(127, matrix[q * 24 + x])));
+    };
+    for (int q = 0; q < 24; ++q)
+        for (int x = 0; x <= kBandSentinel; ++x)
+            for (int y = 0; y <= kBandSentinel; ++y)
+                t[(size_t)q * 625 + x * 25 + y] = (uint32_t)(uint16_t)s8(q, x) | ((uint32_t)(uint16_t)s8(q, y) << 16);
+    return t;
+}
+
 int check_seqset(const saber_hip_seqset* s, const char* what) {
     if (!s) return fail(SABER_HIP_EINVAL, std::string(what) + " is null");
     if (s->count > 0 && !s->offsets) return fail(SABER_HIP_EINVAL, std::string(what) + ": offsets null");
@@ -568,19 +584,7 @@ int saber_hip_band_batch_create(saber_hip_ctx* c, const saber_hip_seqset* querie
             }
     }
 
-    // Pair table of the shared-query flavour: (S(q, a), S(q, b)) per query code
-    // q < 24 and subject codes a, b <= 24, from the int8 table.
-    std::vector<uint32_t> ptable(kPairWords, 0);
-    {
-        const int8_t* t8 = reinterpret_cast<const int8_t*>(table.data());
-        for (int q = 0; q < 24; ++q)
-            for (int x = 0; x <= kBandSentinel; ++x)
-                for (int y = 0; y <= kBandSentinel; ++y) {
-                    const uint16_t lo = (uint16_t)(int16_t)t8[q * kTable8Stride + x];
-                    const uint16_t hi = (uint16_t)(int16_t)t8[q * kTable8Stride + y];
-                    ptable[(size_t)q * 625 + x * 25 + y] = (uint32_t)lo | ((uint32_t)hi << 16);
-                }
-    }
+    const std::vector<uint32_t> ptable = build_pair_table(scheme->matrix);
 
     hipStream_t st = c->stream;
     const uint64_t nq = queries->count ? queries->offsets[queries->count] : 0;
@@ -1078,8 +1082,9 @@ struct saber_hip_full_batch {
     DevBuf<int8_t> bias;
     DevBuf<uint64_t> qoff, soff;
     DevBuf<uint32_t> queries, pairs;
-    DevBuf<short> table;
+    DevBuf<uint32_t> pair_table;
     DevBuf<int32_t> mat;
+    bool ends = true;                      // end cells computed (else scores only)
     std::vector<FullLaunch> launches;
     std::vector<uint32_t> long_subjects;   // longer than the widest variant
     std::vector<uint64_t> hq, hs;          // host offsets
@@ -1194,7 +1199,7 @@ int full_enqueue(saber_hip_full_batch* b) {
         a.qres_len = b->qlen_total;
         a.soff = b->soff.p;
         a.sres = b->sres;
-        a.table = b->table.p;
+        a.pair_table = b->pair_table.p;
         a.go = b->go;
         a.ge = b->ge;
         a.nsub = b->ns;
@@ -1237,7 +1242,7 @@ int full_enqueue(saber_hip_full_batch* b) {
 extern "C" {
 
 int saber_hip_full_batch_create(saber_hip_ctx* c, const saber_hip_seqset* queries,
-                                const saber_hip_seqset* subjects, const saber_hip_scheme* scheme,
+                                const saber_hip_seqset* subjects, const saber_hip_scheme* scheme, int ends,
                                 saber_hip_full_batch** out) {
     if (!c || !out || !scheme) return fail(SABER_HIP_EINVAL, "null argument");
     *out = nullptr;
@@ -1252,6 +1257,7 @@ int saber_hip_full_batch_create(saber_hip_ctx* c, const saber_hip_seqset* querie
     b->has_bias = queries->bias != nullptr;
     b->go = scheme->gap_open;
     b->ge = scheme->gap_extend;
+    b->ends = ends != 0;
     b->hq.assign(queries->offsets, queries->offsets + queries->count + 1);
     if (subjects)
         b->hs.assign(subjects->offsets, subjects->offsets + subjects->count + 1);
@@ -1281,7 +1287,7 @@ int saber_hip_full_batch_create(saber_hip_ctx* c, const saber_hip_seqset* querie
         int best = -1;
         if (packed_ok) {
             for (int v = 0; v < nv; ++v) {
-                if (var[v].bias != (b->has_bias ? 1 : 0)) continue;
+                if (var[v].bias != (b->has_bias ? 1 : 0) || var[v].ends != (b->ends ? 1 : 0)) continue;
                 const int W = var[v].L * var[v].C;
                 if ((int64_t)W < (int64_t)len[si]) continue;
                 if (best < 0 || W < var[best].L * var[best].C ||
@@ -1318,13 +1324,7 @@ int saber_hip_full_batch_create(saber_hip_ctx* c, const saber_hip_seqset* querie
         load[best] += L.cost;
     }
 
-    std::vector<short> table(kTableDim * kTableDim);
-    for (int a = 0; a < kTableDim; ++a)
-        for (int cc = 0; cc < kTableDim; ++cc) {
-            const int aa = a < 24 ? a : kCodeX, ci = cc < 24 ? cc : kCodeX;
-            const int v = std::max(-32767, std::min(32767, scheme->matrix[aa * 24 + ci]));
-            table[a * kTableDim + cc] = cc == kSentinel ? kNeg16 : (short)v;
-        }
+    const std::vector<uint32_t> ptable = build_pair_table(scheme->matrix);
     std::vector<uint32_t> qids(b->nq);
     for (uint32_t k = 0; k < b->nq; ++k) qids[k] = k;
 
@@ -1346,7 +1346,7 @@ int saber_hip_full_batch_create(saber_hip_ctx* c, const saber_hip_seqset* querie
     if (int r = upload(b->soff, b->hs.data(), b->hs.size(), st)) return r;
     if (int r = upload(b->queries, qids.data(), qids.size(), st)) return r;
     if (int r = upload(b->pairs, pairs.data(), pairs.size(), st)) return r;
-    if (int r = upload(b->table, table.data(), table.size(), st)) return r;
+    if (int r = upload(b->pair_table, ptable.data(), ptable.size(), st)) return r;
     if (int r = upload(b->mat, scheme->matrix, 576, st)) return r;
     const uint64_t nout = std::max<uint64_t>(1, (uint64_t)b->nq * b->ns);
     HIP_TRY(b->score.alloc(nout));
@@ -1377,6 +1377,7 @@ int saber_hip_full_batch_read(saber_hip_full_batch* b, int32_t* score, uint32_t*
     HIP_TRY(hipSetDevice(b->ctx->device));
     hipStream_t st = b->ctx->stream;
     const uint64_t n = (uint64_t)b->nq * b->ns;
+    if (!b->ends && (q_end || s_end)) return fail(SABER_HIP_ESTATE, "batch computes scores only");
     if (n == 0) return SABER_HIP_OK;
     if (score) HIP_TRY(hipMemcpyAsync(score, b->score.p, 4 * n, hipMemcpyDeviceToHost, st));
     if (q_end) HIP_TRY(hipMemcpyAsync(q_end, b->q_end.p, 4 * n, hipMemcpyDeviceToHost, st));
@@ -1430,7 +1431,7 @@ int saber_hip_score_full(saber_hip_ctx* c, const saber_hip_seqset* queries,
                          const saber_hip_seqset* subjects, const saber_hip_scheme* scheme,
                          int32_t* score, uint32_t* q_end, uint32_t* s_end) {
     saber_hip_full_batch* b = nullptr;
-    if (int r = saber_hip_full_batch_create(c, queries, subjects, scheme, &b)) return r;
+    if (int r = saber_hip_full_batch_create(c, queries, subjects, scheme, q_end || s_end, &b)) return r;
     int r = saber_hip_full_batch_run(b);
     if (r == SABER_HIP_OK) r = saber_hip_full_batch_read(b, score, q_end, s_end);
     saber_hip_full_batch_destroy(b);

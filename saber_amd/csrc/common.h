@@ -9,12 +9,8 @@
 namespace saber {
 
 // Alignment codes 0..23 (ARNDCQEGHILKMFPSTWYVBZX*); any code >= 24 scores as
-// X (src/alphabet.rs:39-41). Device kernels see subject codes pre-doubled
-// (byte offsets into a row of 16-bit table entries) and one extra code:
+// X (src/alphabet.rs:39-41), so residues are stored canonical on upload.
 constexpr int kCodeX = 22;
-constexpr int kSentinel = 31;        // marks a cell column outside the subject
-constexpr int kTableDim = 32;        // 32 x 32 int16 score table in LDS
-constexpr short kNeg16 = -32768;     // unreachable (saturating arithmetic keeps it)
 constexpr short kMax16 = 32767;      // any job whose best reaches this is rescored in int32
 
 // Band kernel score table: int8, 24 query rows x 25 columns (subject codes
@@ -153,7 +149,7 @@ struct FullKernelArgs {
     uint64_t qres_len;
     const uint64_t* soff;       // subject offsets
     const uint8_t* sres;
-    const short* table;
+    const uint32_t* pair_table; // kPairWords words (global; staged into LDS)
     int32_t go, ge;
     uint32_t nsub;              // row stride of the outputs (subjects per query)
     uint32_t slot0;             // first output row of this launch
@@ -167,6 +163,7 @@ struct FullKernelArgs {
 struct FullVariant {
     int L, C;
     int bias;
+    int ends;                   // 1: score and end cell, 0: score only
     hipError_t (*launch)(const FullKernelArgs&, uint32_t nslots, hipStream_t);
 };
 

@@ -1,8 +1,8 @@
 // Packed-int16 full-matrix Smith-Waterman kernel for gfx950 (exhaustive mode).
 //
-// Semantics: score_many's score (src/algorithm/sw_simd.rs:95-153) with the end
-// cell of sw_reference (src/algorithm/reference.rs:15-52): the first row-major
-// optimal cell (exclusive ends), plus the m*n cell count.
+// Semantics: score_many's score (src/algorithm/sw_simd.rs:95-153), and with
+// ENDS the end cell of sw_reference (src/algorithm/reference.rs:15-52): the
+// first row-major optimal cell (exclusive ends).
 //
 // Work: the cross product of a query set and a subject set. Subjects are
 // bucketed by length into variants; two subjects of similar length share each
@@ -15,29 +15,26 @@
 // a lane, the previous lane's last cell from the previous step), its upper
 // neighbour (own register from the previous row) and its diagonal (the upper
 // neighbour of the left cell: the previous lane's last cell from two steps
-// back), so only the left lane's (H, E) cross lanes, once per step, by DPP.
-// Subject codes are fixed per lane and held in registers for the whole job;
-// columns past the subject's end read the sentinel column (-32768), which keeps
-// them from influencing any real cell. Saturation and the end cell are handled
-// as in the band kernel.
+// back), so only the left lane's H and E cross lanes, once per step, by DPP.
+//
+// Scores: both halves of a lane score against the same query residue q, so
+// one LDS read of the pair table TP[q][a][b] = S(q,a) | S(q,b) << 16 gives
+// both. The subject codes of a lane's columns are fixed for the whole job, so
+// each column's entry offset (a*100 + b*4, 16 bits) is computed once; a cell's
+// lookup is one add to the row's base and one LDS read. Columns past a
+// subject's end use code 24, whose score is -128 (the sentinel): such cells
+// lie right of every real cell of their row and only receive values from real
+// cells, lowered, so they can neither exceed a real cell nor come first.
+//
+// Recurrence with clamped gap states E' = max(E, 0), F' = max(F, 0): with
+// H >= 0, H = max(0, D, E, F) = max(D, E', F'), and every "x - gap" of a
+// clamped state is one unsigned saturating subtract; H - open feeds both the
+// next column's E and the next row's F. Additions saturate at 32767: a job
+// whose best reaches it is flagged for the exact int32 kernel.
 #pragma once
 #include "band_kernel.h"
 
 namespace saber {
-
-// Raw residue at subject column j, read from a clamped (always valid) address
-// so the load never sits behind a branch; subject_code() turns it into the
-// pre-doubled code, or the sentinel when j is outside [0, n).
-__device__ __forceinline__ uint32_t subject_raw(const uint8_t* s, int64_t j, uint32_t n) {
-    int64_t jc = j < 0 ? 0 : j;
-    const int64_t top = n > 0 ? (int64_t)n - 1 : 0;
-    jc = jc > top ? top : jc;
-    return s[jc];
-}
-__device__ __forceinline__ uint32_t subject_code(uint32_t raw, int64_t j, uint32_t n) {
-    if (j < 0 || j >= (int64_t)n) return 2u * kSentinel;
-    return 2u * (raw < 24 ? raw : (uint32_t)kCodeX);
-}
 
 // Lane i receives lane i-1: within a 16-lane row for groups that tile rows,
 // across the whole wave (wave_shr:1) for 32- and 64-lane groups.
@@ -49,18 +46,26 @@ __device__ __forceinline__ uint32_t full_from_prev_lane(uint32_t v) {
         return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x138, 0xF, 0xF, false);
 }
 
-template <int L, int C, bool BIAS>
-__global__ __launch_bounds__(256) void full_pair_kernel(FullKernelArgs a) {
-    static_assert(C >= 2, "variant shape");
+// Workgroup size: 12 waves (3 per SIMD: 168 registers) unless the shape needs
+// more registers (wide lanes; end-cell snapshots): 8 waves (256 registers),
+// or 4 waves for end-cell snapshots of the widest lanes (which still spill a
+// little; they only serve subjects longer than 2048).
+template <int C, bool ENDS>
+constexpr int kFullThreads = ENDS ? (C <= 24 ? 768 : C <= 32 ? 512 : 256) : (C <= 32 ? 768 : 512);
+
+template <int L, int C, bool BIAS, bool ENDS>
+__global__ __launch_bounds__((kFullThreads<C, ENDS>)) void full_pair_kernel(FullKernelArgs a) {
+    static_assert(C >= 2 && C % 2 == 0, "variant shape");
     static_assert(L == 1 || L == 2 || L == 4 || L == 8 || L == 16 || L == 32 || L == 64,
                   "groups must tile a wave");
-    constexpr int NW = (C + 3) / 4;
-    __shared__ short T[kTableDim * kTableDim];
-    for (int t = threadIdx.x; t < kTableDim * kTableDim; t += blockDim.x) T[t] = a.table[t];
+    constexpr int NT = kFullThreads<C, ENDS>;
+    constexpr int NE = C / 2;
+    __shared__ uint32_t TP[kPairWords];
+    for (int t = threadIdx.x; t < kPairWords; t += NT) TP[t] = a.pair_table[t];
     __syncthreads();
-    const char* Tb = reinterpret_cast<const char*>(T);
+    const char* Tpb = reinterpret_cast<const char*>(TP);
 
-    const uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t gl = blockIdx.x * NT + threadIdx.x;
     const uint32_t g = gl / L;
     const int l = (int)(gl % L);
     if (g >= a.npairs) return;
@@ -69,8 +74,7 @@ __global__ __launch_bounds__(256) void full_pair_kernel(FullKernelArgs a) {
     const int64_t q0 = (int64_t)a.qoff[qid];
     const int m = (int)(a.qoff[qid + 1] - a.qoff[qid]);
 
-    uint32_t sid[2];
-    uint32_t n[2];
+    uint32_t sid[2], n[2];
     const uint8_t* sp[2];
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
@@ -84,30 +88,37 @@ __global__ __launch_bounds__(256) void full_pair_kernel(FullKernelArgs a) {
             sp[h] = a.sres;
         }
     }
-    const int R = m;  // every row of the query holds cells of both jobs
+
+    // Pair-table entry offsets of the lane's columns, two per dword.
+    uint32_t E2[NE];
+    {
+        const int64_t j0 = (int64_t)l * C;
+        uint32_t code[2][C];
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                const int64_t j = j0 + c;
+                // clamped address: the load never sits behind a branch
+                const int64_t jc = j < (int64_t)n[h] ? j : (n[h] > 0 ? (int64_t)n[h] - 1 : 0);
+                const uint32_t raw = sp[h][jc];
+                code[h][c] = j < (int64_t)n[h] ? raw : (uint32_t)kBandSentinel;
+            }
+#pragma unroll
+        for (int x = 0; x < NE; ++x)
+            E2[x] = (code[0][2 * x] * 100u + code[1][2 * x] * 4u) |
+                    ((code[0][2 * x + 1] * 100u + code[1][2 * x + 1] * 4u) << 16);
+    }
 
     const v2s GO = {(short)a.go, (short)a.go};
     const v2s GE = {(short)a.ge, (short)a.ge};
-    const v2s NEG = {kNeg16, kNeg16};
     const v2s ZERO = {0, 0};
 
     v2s H[C], F[C];
 #pragma unroll
     for (int c = 0; c < C; ++c) {
         H[c] = ZERO;
-        F[c] = NEG;
-    }
-    uint32_t Wd[2][NW];
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-#pragma unroll
-        for (int x = 0; x < NW; ++x) Wd[h][x] = 0;
-        const int64_t j0 = (int64_t)l * C;
-        uint32_t raw[C];
-#pragma unroll
-        for (int c = 0; c < C; ++c) raw[c] = subject_raw(sp[h], j0 + c, n[h]);
-#pragma unroll
-        for (int c = 0; c < C; ++c) Wd[h][c >> 2] |= subject_code(raw[c], j0 + c, n[h]) << (8 * (c & 3));
+        F[c] = ZERO;
     }
 
     auto load4 = [](const uint8_t* buf, int64_t off, uint64_t len) -> uint32_t {
@@ -127,74 +138,68 @@ __global__ __launch_bounds__(256) void full_pair_kernel(FullKernelArgs a) {
         b_n1 = load4(bb, q0 + 5, a.qres_len);
         b_n2 = load4(bb, q0 + 9, a.qres_len);
     }
-    uint32_t qc0 = m > 0 ? a.qres[q0] : 0;
-    uint32_t rb = (qc0 < 24 ? qc0 : (uint32_t)kCodeX) * 64u;
+    uint32_t rb = (m > 0 ? (uint32_t)a.qres[q0] : 0u) * kPairRowBytes;
     v2s bpair = ZERO;
     if constexpr (BIAS) {
         const short b0 = m > 0 ? (short)a.qbias[q0] : (short)0;
         bpair = v2s{b0, b0};
     }
 
-    int cbest[2] = {0, 0};
-    int csnap_row[2] = {-1, -1};
-    uint32_t snap[2][C];
-    uint32_t outHl = 0u, outE = bits(NEG);   // lane 0's left border: H = 0, E = -inf
-    uint32_t diagIn = 0u;                    // H(i-1, l*C-1) for the current row
+    v2s best = ZERO;
+    int brow[2] = {-1, -1};   // ENDS: row where the half's best last rose
+    uint32_t snap[ENDS ? C : 1];
+#pragma unroll
+    for (int c = 0; c < (ENDS ? C : 1); ++c) snap[c] = 0;
+    uint32_t outH = 0u, outE = 0u;   // lane 0's left border: H = 0, E' = 0
+    uint32_t diagIn = 0u;            // H(i-1, l*C-1) for the current row
 
-    const int nsteps = R + L - 1;
+    const int nsteps = m + L - 1;
     for (int t = 0; t < nsteps; ++t) {
         const int r = t - l;
-        const bool act = (r >= 0) && (r < R);
-        v2s inHl = ZERO, inE = NEG;
+        v2s inH = ZERO, inE = ZERO;
         if constexpr (L > 1) {
-            inHl = pair(full_from_prev_lane<L>(outHl));
+            inH = pair(full_from_prev_lane<L>(outH));
             inE = pair(full_from_prev_lane<L>(outE));
             if (l == 0) {
-                inHl = ZERO;
-                inE = NEG;
+                inH = ZERO;
+                inE = ZERO;
             }
         }
-        if (act) {
-            v2s Hl = inHl, E = inE, rm = ZERO;
-            v2s hd = pair(diagIn);
+        if (r >= 0 && r < m) {
+            v2s E = inE, rm = ZERO, hd = pair(diagIn);
+            const char* row = Tpb + rb;
 #pragma unroll
             for (int c = 0; c < C; ++c) {
-                const uint32_t ba = (Wd[0][c >> 2] >> (8 * (c & 3))) & 0xFFu;
-                const uint32_t bq = (Wd[1][c >> 2] >> (8 * (c & 3))) & 0xFFu;
-                const short sa = *reinterpret_cast<const short*>(Tb + rb + ba);
-                const short sb = *reinterpret_cast<const short*>(Tb + rb + bq);
-                v2s P = v2s{sa, sb};
+                const uint32_t e = (E2[c >> 1] >> (16 * (c & 1))) & 0xFFFFu;
+                v2s P;
+                __builtin_memcpy(&P, row + e, 4);
                 if constexpr (BIAS) P = vadd(P, bpair);
-                const v2s up = H[c];
                 const v2s D = vadd(hd, P);
-                const v2s Fn = vmax(vsub(up, GO), vsub(F[c], GE));
-                const v2s En = vmax(vsub(Hl, GO), vsub(E, GE));
-                const v2s hv = vmax(vmax(D, En), vmax(Fn, ZERO));
-                hd = up;
+                hd = H[c];
+                const v2s hv = vmax(vmax(D, F[c]), E);
+                const v2s hg = vsubz(hv, GO);
+                E = vmax(hg, vsubz(E, GE));
+                F[c] = vmax(hg, vsubz(F[c], GE));
                 H[c] = hv;
-                F[c] = Fn;
-                Hl = hv;
-                E = En;
                 rm = vmax(rm, hv);
             }
-            outHl = bits(Hl);
+            outH = bits(H[C - 1]);
             outE = bits(E);
-            diagIn = bits(inHl);
+            diagIn = bits(inH);
+            if constexpr (ENDS) {
+                // halves whose best rose: remember the row and its cells
+                const bool r0 = rm.x > best.x, r1 = rm.y > best.y;
+                uint32_t rose = (r0 ? 0x0000FFFFu : 0u) | (r1 ? 0xFFFF0000u : 0u);
+                asm volatile("" : "+v"(rose));
+                brow[0] = r0 ? r : brow[0];
+                brow[1] = r1 ? r : brow[1];
 #pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                if (r < m && sid[h] != kNone) {
-                    const int rmh = h == 0 ? (int)rm.x : (int)rm.y;
-                    if (rmh > cbest[h]) {
-                        cbest[h] = rmh;
-                        csnap_row[h] = r;
-#pragma unroll
-                        for (int c = 0; c < C; ++c) snap[h][c] = bits(H[c]);
-                    }
-                }
+                for (int c = 0; c < C; ++c) snap[c] = (bits(H[c]) & rose) | (snap[c] & ~rose);
             }
+            best = vmax(best, rm);
+            // next row's query code and bias
             const int k = r + 1;
-            const uint32_t qc = q_cur & 0xFFu;
-            rb = (qc < 24 ? qc : (uint32_t)kCodeX) * 64u;
+            rb = (q_cur & 0xFFu) * kPairRowBytes;
             q_cur >>= 8;
             if constexpr (BIAS) {
                 const short bn = (short)(int8_t)(b_cur & 0xFFu);
@@ -214,62 +219,68 @@ __global__ __launch_bounds__(256) void full_pair_kernel(FullKernelArgs a) {
         } else if (r < 0) {
             // Keep the left lane's previous-step H: it is the diagonal of
             // this lane's first row when that row starts.
-            diagIn = l == 0 ? 0u : bits(inHl);
+            diagIn = l == 0 ? 0u : bits(inH);
         }
     }
 
-    // Group reduction: global best, earliest row reaching it, leftmost column.
+    // Group reduction: global best; with ENDS the earliest row reaching it and
+    // the leftmost column of that row.
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
-        int best = cbest[h];
+        const int mine = h == 0 ? (int)best.x : (int)best.y;
+        int top = mine;
 #pragma unroll
         for (int d = 1; d < L; d <<= 1) {
-            const int o = __shfl_xor(best, d, L);
-            best = o > best ? o : best;
+            const int o = __shfl_xor(top, d, L);
+            top = o > top ? o : top;
         }
         uint32_t key = 0xFFFFFFFFu;
-        if (best > 0 && best < kMax16 && cbest[h] == best) {
-            for (int c = 0; c < C; ++c) {
-                const uint32_t v = snap[h][c];
-                const int hv = h == 0 ? (int)(short)(v & 0xFFFFu) : (int)(short)(v >> 16);
-                if (hv == best) {
-                    key = (uint32_t)csnap_row[h] * (uint32_t)(L * C) + (uint32_t)(l * C + c);
-                    break;
+        if constexpr (ENDS) {
+            if (top > 0 && top < kMax16 && mine == top) {
+                for (int c = 0; c < C; ++c) {
+                    const uint32_t v = snap[c];
+                    const int hv = h == 0 ? (int)(short)(v & 0xFFFFu) : (int)(short)(v >> 16);
+                    if (hv == top) {
+                        key = (uint32_t)brow[h] * (uint32_t)(L * C) + (uint32_t)(l * C + c);
+                        break;
+                    }
                 }
             }
-        }
 #pragma unroll
-        for (int d = 1; d < L; d <<= 1) {
-            const uint32_t o = (uint32_t)__shfl_xor((int)key, d, L);
-            key = o < key ? o : key;
+            for (int d = 1; d < L; d <<= 1) {
+                const uint32_t o = (uint32_t)__shfl_xor((int)key, d, L);
+                key = o < key ? o : key;
+            }
         }
         if (l == 0 && sid[h] != kNone) {
             const uint64_t o = (uint64_t)slot * a.nsub + sid[h];
-            if (best >= kMax16) {
+            if (top >= kMax16) {
                 const uint32_t f = atomicAdd(a.nflagged, 1u);
                 a.flagged[2 * f] = slot;
                 a.flagged[2 * f + 1] = sid[h];
             } else {
-                uint32_t qe = 0, se = 0;
-                if (best > 0) {
-                    qe = key / (uint32_t)(L * C) + 1;
-                    se = key % (uint32_t)(L * C) + 1;
+                a.score[o] = top;
+                if constexpr (ENDS) {
+                    uint32_t qe = 0, se = 0;
+                    if (top > 0) {
+                        qe = key / (uint32_t)(L * C) + 1;
+                        se = key % (uint32_t)(L * C) + 1;
+                    }
+                    a.q_end[o] = qe;
+                    a.s_end[o] = se;
                 }
-                a.score[o] = best;
-                if (a.q_end) a.q_end[o] = qe;
-                if (a.s_end) a.s_end[o] = se;
             }
         }
     }
 }
 
-template <int L, int C, bool BIAS>
+template <int L, int C, bool BIAS, bool ENDS>
 hipError_t launch_full(const FullKernelArgs& a, uint32_t nslots, hipStream_t s) {
     if (a.npairs == 0 || nslots == 0) return hipSuccess;
+    constexpr int NT = kFullThreads<C, ENDS>;
     const uint64_t threads = (uint64_t)a.npairs * L;
-    const uint32_t block = 256;
-    const uint32_t grid = (uint32_t)((threads + block - 1) / block);
-    hipLaunchKernelGGL((full_pair_kernel<L, C, BIAS>), dim3(grid, nslots), dim3(block), 0, s, a);
+    const uint32_t grid = (uint32_t)((threads + NT - 1) / NT);
+    hipLaunchKernelGGL((full_pair_kernel<L, C, BIAS, ENDS>), dim3(grid, nslots), dim3(NT), 0, s, a);
     return hipGetLastError();
 }
 

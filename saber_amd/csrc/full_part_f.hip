@@ -1,12 +1,24 @@
-// Compiled variants of the full-matrix kernel, part f (L lanes per subject
-// pair, C subject columns per lane; subjects of length up to L*C).
+// Compiled variants of the full-matrix kernel, part f: with a query bias, score only
+// (L lanes per subject pair, C subject columns per lane; subjects of length
+// up to L*C).
 #include "full_kernel.h"
 
 namespace saber {
 
-#define SABER_FULL(L, C) {L, C, 0, &launch_full<L, C, false>}, {L, C, 1, &launch_full<L, C, true>},
+#define SABER_FULL(L, C) {L, C, 1, 0, &launch_full<L, C, true, false>},
 
 static const FullVariant kPart[] = {
+SABER_FULL(1, 16)
+SABER_FULL(1, 24)
+SABER_FULL(1, 32)
+SABER_FULL(1, 40)
+SABER_FULL(1, 48)
+SABER_FULL(2, 32)
+SABER_FULL(2, 40)
+SABER_FULL(2, 48)
+SABER_FULL(4, 32)
+SABER_FULL(4, 40)
+SABER_FULL(4, 48)
 SABER_FULL(8, 32)
 SABER_FULL(8, 40)
 SABER_FULL(8, 48)

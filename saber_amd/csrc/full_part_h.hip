@@ -1,0 +1,35 @@
+// Compiled variants of the full-matrix kernel, part h: with a query bias, score and end cell
+// (L lanes per subject pair, C subject columns per lane; subjects of length
+// up to L*C).
+#include "full_kernel.h"
+
+namespace saber {
+
+#define SABER_FULL(L, C) {L, C, 1, 1, &launch_full<L, C, true, true>},
+
+static const FullVariant kPart[] = {
+SABER_FULL(1, 16)
+SABER_FULL(1, 24)
+SABER_FULL(1, 32)
+SABER_FULL(2, 24)
+SABER_FULL(2, 32)
+SABER_FULL(4, 24)
+SABER_FULL(4, 32)
+SABER_FULL(8, 24)
+SABER_FULL(8, 32)
+SABER_FULL(16, 24)
+SABER_FULL(16, 32)
+SABER_FULL(32, 24)
+SABER_FULL(32, 32)
+SABER_FULL(64, 24)
+SABER_FULL(64, 32)
+SABER_FULL(64, 40)
+SABER_FULL(64, 48)
+};
+
+const FullVariant* full_variants_part_h(int* count) {
+    *count = (int)(sizeof(kPart) / sizeof(kPart[0]));
+    return kPart;
+}
+
+}  // namespace saber

@@ -92,3 +92,41 @@ def test_workload_sample(ctx):
     qs = [qres[qoff[i]:qoff[i + 1]] for i in range(len(qoff) - 1)]
     ss = [sres[soff[i]:soff[i + 1]] for i in range(len(soff) - 1)]
     check(ctx, qs, ss, BLOSUM62, 11, 1)
+
+
+@pytest.mark.parametrize("with_bias", [False, True])
+def test_scores_only_kernel(ctx, with_bias):
+    """The scores-only kernels (score_many's output) agree with the oracle on
+    every length class, including subjects past the widest end-cell shape
+    (2048) that still fit the widest scores-only shape (3072); reading end
+    cells from a scores-only batch is an error."""
+    r = Lcg(95 + with_bias)
+    qs = [rand_seq(r, 1 + r.next(400)) for _ in range(4)]
+    lens = [1, 16, 17, 40, 48, 49, 96, 97, 257, 640, 641, 1025, 2048, 2049, 2600, 3072, 3073]
+    ss = []
+    for k, n in enumerate(lens):
+        s = rand_seq(r, n)
+        if k % 2 == 0:
+            q = qs[k % len(qs)]
+            kk = min(n, len(q))
+            s[:kk] = np.where(np.array([r.next(3) for _ in range(kk)]) > 0, q[:kk], s[:kk])
+        ss.append(s)
+    qres, qoff = flat(qs)
+    sres, soff = flat(ss)
+    bias = [np.array([r.next(9) - 4 for _ in range(len(q))], np.int8) for q in qs] if with_bias else None
+    qb = None if bias is None else np.concatenate(bias).astype(np.int8)
+    sch = S.Scheme(BLOSUM62, 11, 1)
+    b = ctx.full_batch(qres, qoff, sres, soff, sch, qb, ends=False)
+    try:
+        b.run()
+        got = b.read()[0]
+        with pytest.raises(S.HipError):
+            import ctypes as C
+            from saber_amd import _ffi
+            qe = np.zeros(got.shape, np.uint32)
+            _ffi.check(_ffi.lib().saber_hip_full_batch_read(b.h, None, _ffi.ptr(qe, C.c_uint32), None))
+    finally:
+        b.close()
+    want = oracle_grid(qs, ss, BLOSUM62, 11, 1, bias)[0]
+    bad = np.argwhere(got.astype(np.int64) != want)
+    assert len(bad) == 0, f"score differs at {bad[:5].tolist()}"
