@@ -113,7 +113,8 @@ int saber_hip_score_windows(saber_hip_ctx* ctx, const uint8_t* query, uint32_t q
  * score, exclusive ends (q_end, s_end) of the first row-major optimal cell in
  * window coordinates (0, 0 when the score is 0) and the band cells inside the
  * rectangle. subjects == NULL selects the uploaded database. Any of q_end,
- * s_end, cells may be NULL. */
+ * s_end, cells may be NULL; with both ends NULL only scores are computed
+ * (score_banded_many's output, and faster). */
 int saber_hip_score_bands(saber_hip_ctx* ctx, const saber_hip_seqset* queries,
                           const saber_hip_seqset* subjects, const saber_hip_band_job* jobs,
                           uint64_t njobs, const saber_hip_scheme* scheme, int32_t* score,
@@ -121,11 +122,13 @@ int saber_hip_score_bands(saber_hip_ctx* ctx, const saber_hip_seqset* queries,
 
 /* Device-resident batch: create uploads the inputs and plans the work once;
  * run enqueues the whole scoring pass (asynchronous, on the context stream;
- * may be repeated); wait blocks until it is done; read copies the results. */
+ * may be repeated); wait blocks until it is done; read copies the results.
+ * ends = 0 computes scores only; reading end cells from such a batch is
+ * SABER_HIP_ESTATE. */
 int saber_hip_band_batch_create(saber_hip_ctx* ctx, const saber_hip_seqset* queries,
                                 const saber_hip_seqset* subjects,
                                 const saber_hip_band_job* jobs, uint64_t njobs,
-                                const saber_hip_scheme* scheme, saber_hip_band_batch** out);
+                                const saber_hip_scheme* scheme, int ends, saber_hip_band_batch** out);
 int saber_hip_band_batch_run(saber_hip_band_batch* b);
 int saber_hip_band_batch_wait(saber_hip_band_batch* b);
 int saber_hip_band_batch_read(saber_hip_band_batch* b, int32_t* score, uint32_t* q_end,

@@ -136,8 +136,8 @@ def _seqset(seqs, bias=None):
 class BandBatch:
     """Device-resident band batch (saber_hip_band_batch_*)."""
 
-    def __init__(self, ctx, handle, njobs, keep):
-        self.ctx, self.h, self.njobs, self._keep = ctx, handle, njobs, keep
+    def __init__(self, ctx, handle, njobs, keep, ends=True):
+        self.ctx, self.h, self.njobs, self._keep, self.ends = ctx, handle, njobs, keep, ends
         _live_batches.add(self)
         ctx._children.add(self)
 
@@ -148,10 +148,12 @@ class BandBatch:
         _ffi.check(_ffi.lib().saber_hip_band_batch_wait(self.h))
 
     def read(self):
+        """(score, q_end, s_end, cells); a scores-only batch returns
+        (score, None, None, cells)."""
         n = self.njobs
         score = np.zeros(n, np.int32)
-        qe = np.zeros(n, np.uint32)
-        se = np.zeros(n, np.uint32)
+        qe = np.zeros(n, np.uint32) if self.ends else None
+        se = np.zeros(n, np.uint32) if self.ends else None
         cells = np.zeros(n, np.uint64)
         _ffi.check(_ffi.lib().saber_hip_band_batch_read(
             self.h, _ffi.ptr(score, C.c_int32), _ffi.ptr(qe, C.c_uint32), _ffi.ptr(se, C.c_uint32),
@@ -340,9 +342,10 @@ class Context:
         jobs["band_hi"] = band_hi
         return jobs
 
-    def band_batch(self, qres, qoff, sres, soff, jobs, scheme, qbias=None):
+    def band_batch(self, qres, qoff, sres, soff, jobs, scheme, qbias=None, ends=True):
         """Create a device-resident batch from flat arrays (jobs: structured
-        array with query, subject, win_lo, win_hi, band_lo, band_hi)."""
+        array with query, subject, win_lo, win_hi, band_lo, band_hi).
+        ends=False: scores only (score_banded_many's output; faster)."""
         qres = np.ascontiguousarray(qres, np.uint8)
         qoff = np.ascontiguousarray(qoff, np.uint64)
         qb = None if qbias is None else np.ascontiguousarray(qbias, np.int8)
@@ -363,12 +366,13 @@ class Context:
         h = C.c_void_p()
         _ffi.check(_ffi.lib().saber_hip_band_batch_create(
             self.h, C.byref(qs), sp, jobs.ctypes.data_as(C.POINTER(_ffi.BandJob)), len(jobs),
-            C.byref(sch), C.byref(h)))
-        return BandBatch(self, h, len(jobs), keep)
+            C.byref(sch), int(bool(ends)), C.byref(h)))
+        return BandBatch(self, h, len(jobs), keep, bool(ends))
 
-    def score_bands(self, qres, qoff, sres, soff, jobs, scheme, qbias=None):
-        """(score, q_end, s_end, cells) arrays, one entry per job."""
-        b = self.band_batch(qres, qoff, sres, soff, jobs, scheme, qbias)
+    def score_bands(self, qres, qoff, sres, soff, jobs, scheme, qbias=None, ends=True):
+        """(score, q_end, s_end, cells) arrays, one entry per job (q_end and
+        s_end None when ends=False: scores only)."""
+        b = self.band_batch(qres, qoff, sres, soff, jobs, scheme, qbias, ends)
         try:
             b.run()
             return b.read()

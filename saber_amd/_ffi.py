@@ -93,7 +93,7 @@ _SIGS = {
                                         C.POINTER(C.c_uint64)]),
     "saber_hip_band_batch_create": (C.c_int, [P, C.POINTER(SeqSet), C.POINTER(SeqSet),
                                               C.POINTER(BandJob), C.c_uint64, C.POINTER(Scheme),
-                                              C.POINTER(P)]),
+                                              C.c_int, C.POINTER(P)]),
     "saber_hip_band_batch_run": (C.c_int, [P]),
     "saber_hip_band_batch_wait": (C.c_int, [P]),
     "saber_hip_band_batch_read": (C.c_int, [P, C.POINTER(C.c_int32), C.POINTER(C.c_uint32),

@@ -141,6 +141,7 @@ struct saber_hip_band_batch {
     saber_hip_ctx* ctx = nullptr;
     uint64_t njobs = 0;
     bool has_bias = false;
+    bool ends = true;          // end cells computed (else scores only)
     int32_t go = 0, ge = 0;
     DevBuf<uint8_t> q, s;
     DevBuf<int8_t> bias;
@@ -398,7 +399,7 @@ int saber_hip_band_tune(saber_hip_ctx* c, int latency_rows) {
 
 int saber_hip_band_batch_create(saber_hip_ctx* c, const saber_hip_seqset* queries,
                                 const saber_hip_seqset* subjects, const saber_hip_band_job* jobs,
-                                uint64_t njobs, const saber_hip_scheme* scheme,
+                                uint64_t njobs, const saber_hip_scheme* scheme, int ends,
                                 saber_hip_band_batch** out) {
     if (!c || !out || !scheme) return fail(SABER_HIP_EINVAL, "null argument");
     *out = nullptr;
@@ -416,6 +417,7 @@ int saber_hip_band_batch_create(saber_hip_ctx* c, const saber_hip_seqset* querie
     b->ctx = c;
     b->njobs = njobs;
     b->has_bias = queries->bias != nullptr;
+    b->ends = ends != 0;
     b->go = scheme->gap_open;
     b->ge = scheme->gap_extend;
 
@@ -683,7 +685,7 @@ static int enqueue_main(saber_hip_band_batch* b, hipEvent_t ev_begin, hipEvent_t
     if (ev_begin) HIP_TRY(hipEventRecord(ev_begin, st));
     // one persistent workgroup per CU (fewer when there is little work)
     const uint32_t wg = (uint32_t)std::min<uint64_t>((uint64_t)b->ctx->cus, (b->host_blocks.size() + 11) / 12);
-    if (a.nchunks) HIP_TRY(launch_band_multi(a, wg, b->has_bias, st));
+    if (a.nchunks) HIP_TRY(launch_band_multi(a, wg, b->has_bias, b->ends, st));
     if (ev_end) HIP_TRY(hipEventRecord(ev_end, st));
     if (int r = run_generic(b, b->generic_static)) return r;
     return SABER_HIP_OK;
@@ -730,6 +732,7 @@ int saber_hip_band_batch_read(saber_hip_band_batch* b, int32_t* score, uint32_t*
     HIP_TRY(hipSetDevice(b->ctx->device));
     hipStream_t st = b->ctx->stream;
     const uint64_t n = b->njobs;
+    if (!b->ends && (q_end || s_end)) return fail(SABER_HIP_ESTATE, "batch computes scores only");
     if (n == 0) return SABER_HIP_OK;
     if (score) HIP_TRY(hipMemcpyAsync(score, b->score.p, 4 * n, hipMemcpyDeviceToHost, st));
     if (q_end) HIP_TRY(hipMemcpyAsync(q_end, b->q_end.p, 4 * n, hipMemcpyDeviceToHost, st));
@@ -817,7 +820,7 @@ int saber_hip_score_bands(saber_hip_ctx* c, const saber_hip_seqset* queries,
                           uint32_t* q_end, uint32_t* s_end, uint64_t* cells) {
     if (njobs > 0 && !score) return fail(SABER_HIP_EINVAL, "score is null");
     saber_hip_band_batch* b = nullptr;
-    if (int r = saber_hip_band_batch_create(c, queries, subjects, jobs, njobs, scheme, &b)) return r;
+    if (int r = saber_hip_band_batch_create(c, queries, subjects, jobs, njobs, scheme, q_end || s_end, &b)) return r;
     int r = saber_hip_band_batch_run(b);
     if (r == SABER_HIP_OK) r = saber_hip_band_batch_read(b, score, q_end, s_end, cells);
     saber_hip_band_batch_destroy(b);

@@ -127,7 +127,7 @@ __device__ __forceinline__ uint32_t pair_entries(uint32_t wa, uint32_t wb, int h
 // int8 score table in LDS; Tp the pair table (SHARED only: both jobs of the
 // pair have the same query and first row, so a row's query code is common
 // and one LDS read of Tp[q][sA][sB] returns both scores already packed).
-template <int L, int C, int K, bool BIAS, bool SHARED>
+template <int L, int C, int K, bool BIAS, bool SHARED, bool ENDS>
 __device__ __attribute__((noinline)) void band_pair(const BandKernelArgs& a, const int8_t* Tb, const uint32_t* Tp, uint32_t g,
                                           int l) {
     static_assert(C >= 2 && K >= 1 && K <= C, "variant shape");
@@ -181,13 +181,14 @@ __device__ __attribute__((noinline)) void band_pair(const BandKernelArgs& a, con
     }
 
     v2s H[C], F[C];
-    uint32_t snap[C];
+    uint32_t snap[ENDS ? C : 1];   // ENDS: the lane's cells in the row where its best last rose
 #pragma unroll
     for (int c = 0; c < C; ++c) {
         H[c] = ZERO;
         F[c] = ZERO;
-        snap[c] = 0;
     }
+#pragma unroll
+    for (int c = 0; c < (ENDS ? C : 1); ++c) snap[c] = 0;
 
     // At step t lane l computes row r = t - l; its cell c is window column
     // jb + t + c and its row is query position ifirst + t - l. Block b (steps
@@ -320,16 +321,20 @@ __device__ __attribute__((noinline)) void band_pair(const BandKernelArgs& a, con
                 // Rows past a job's own rows do not count for it.
                 const v2s live = v2s{r < rows[0] ? kMax16 : (short)0, r < rows[1] ? kMax16 : (short)0};
                 rm = vmin(rm, live);
-                // Halves whose best rose: all-ones (sign of cbest - rm).
-                uint32_t rose = bits(vsub(cbest, rm) >> 15);
-                // Opaque to the optimiser, so the selects below stay one
-                // v_bfi_b32 per cell instead of per-half selects.
-                asm volatile("" : "+v"(rose));
-                if (rose != 0) {
-                    cbest = vmax(cbest, rm);
-                    brow = (rose & ((uint32_t)r * 0x10001u)) | (~rose & brow);
+                if constexpr (ENDS) {
+                    // Halves whose best rose: all-ones (sign of cbest - rm).
+                    uint32_t rose = bits(vsub(cbest, rm) >> 15);
+                    // Opaque to the optimiser, so the selects below stay one
+                    // v_bfi_b32 per cell instead of per-half selects.
+                    asm volatile("" : "+v"(rose));
+                    if (rose != 0) {
+                        cbest = vmax(cbest, rm);
+                        brow = (rose & ((uint32_t)r * 0x10001u)) | (~rose & brow);
 #pragma unroll
-                    for (int c = 0; c < C; ++c) snap[c] = (rose & bits(H[c])) | (~rose & snap[c]);
+                        for (int c = 0; c < C; ++c) snap[c] = (rose & bits(H[c])) | (~rose & snap[c]);
+                    }
+                } else {
+                    cbest = vmax(cbest, rm);
                 }
             }
         }
@@ -379,7 +384,7 @@ __device__ __attribute__((noinline)) void band_pair(const BandKernelArgs& a, con
             best = o > best ? o : best;
         }
         uint32_t key = 0xFFFFFFFFu;
-        if (best > 0 && best < kMax16 && mine == best) {
+        if (ENDS && best > 0 && best < kMax16 && mine == best) {
             const uint32_t row = h == 0 ? (brow & 0xFFFFu) : (brow >> 16);
             int first = C;
 #pragma unroll
@@ -390,10 +395,12 @@ __device__ __attribute__((noinline)) void band_pair(const BandKernelArgs& a, con
             }
             if (first < C) key = row * W + (uint32_t)(l * C + first);
         }
+        if constexpr (ENDS) {
 #pragma unroll
-        for (int d = 1; d < L; d <<= 1) {
-            const uint32_t o = (uint32_t)__shfl_xor((int)key, d, L);
-            key = o < key ? o : key;
+            for (int d = 1; d < L; d <<= 1) {
+                const uint32_t o = (uint32_t)__shfl_xor((int)key, d, L);
+                key = o < key ? o : key;
+            }
         }
         if (l == 0 && jid[h] != kNone) {
             const uint32_t id = jid[h];
@@ -401,17 +408,19 @@ __device__ __attribute__((noinline)) void band_pair(const BandKernelArgs& a, con
                 const uint32_t slot = atomicAdd(a.nflagged, 1u);
                 a.flagged[slot] = id;
             } else {
-                uint32_t qe = 0, se = 0;
-                if (best > 0) {
-                    const uint32_t row = key / W;
-                    const uint32_t k = key % W;
-                    const int i = ifirst[h] + (int)row;
-                    qe = (uint32_t)i + 1;
-                    se = (uint32_t)(i + lo[h] + (int)k + 1);
-                }
                 a.out.score[id] = best;
-                a.out.q_end[id] = qe;
-                a.out.s_end[id] = se;
+                if constexpr (ENDS) {
+                    uint32_t qe = 0, se = 0;
+                    if (best > 0) {
+                        const uint32_t row = key / W;
+                        const uint32_t k = key % W;
+                        const int i = ifirst[h] + (int)row;
+                        qe = (uint32_t)i + 1;
+                        se = (uint32_t)(i + lo[h] + (int)k + 1);
+                    }
+                    a.out.q_end[id] = qe;
+                    a.out.s_end[id] = se;
+                }
             }
         }
     }
@@ -432,7 +441,7 @@ __device__ __attribute__((noinline)) void band_pair(const BandKernelArgs& a, con
 // the pair-table flavour for pairs that share their query rows, the per-job
 // table otherwise. 12 waves per CU = three per SIMD, so every shape must fit
 // 168 registers.
-template <bool BIAS>
+template <bool BIAS, bool ENDS>
 __global__ __launch_bounds__(768) void band_multi_kernel(BandKernelArgs a) {
     __shared__ uint32_t T4[kTable8Words];
     __shared__ uint32_t TP[kPairWords];
@@ -454,9 +463,9 @@ __global__ __launch_bounds__(768) void band_multi_kernel(BandKernelArgs a) {
             const uint32_t q = (uint32_t)lane / (L_);                                            \
             if (q < blk.npairs) {                                                                \
                 if (shared)                                                                      \
-                    band_pair<L_, C_, K_, BIAS, true>(a, Tb, TP, blk.pair_off + q, lane % (L_));  \
+                    band_pair<L_, C_, K_, BIAS, true, ENDS>(a, Tb, TP, blk.pair_off + q, lane % (L_)); \
                 else                                                                             \
-                    band_pair<L_, C_, K_, BIAS, false>(a, Tb, TP, blk.pair_off + q, lane % (L_)); \
+                    band_pair<L_, C_, K_, BIAS, false, ENDS>(a, Tb, TP, blk.pair_off + q, lane % (L_)); \
             }                                                                                    \
             continue;                                                                            \
         }                                                                                        \

@@ -1,11 +1,11 @@
-// The multi-shape packed band kernel with a per-query-position bias.
+// The multi-shape packed band kernel with query bias, scores and end cells.
 #include "band_kernel.h"
 
 namespace saber {
 
-hipError_t launch_band_multi_bias(const BandKernelArgs& a, uint32_t nblocks, hipStream_t s) {
+hipError_t launch_band_multi_bias_ends(const BandKernelArgs& a, uint32_t nblocks, hipStream_t s) {
     if (nblocks == 0) return hipSuccess;
-    hipLaunchKernelGGL(band_multi_kernel<true>, dim3(nblocks), dim3(768), 0, s, a);
+    hipLaunchKernelGGL((band_multi_kernel<true, true>), dim3(nblocks), dim3(768), 0, s, a);
     return hipGetLastError();
 }
 

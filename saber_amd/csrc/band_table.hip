@@ -6,8 +6,10 @@
 
 namespace saber {
 
-hipError_t launch_band_multi_nobias(const BandKernelArgs& a, uint32_t nblocks, hipStream_t s);
-hipError_t launch_band_multi_bias(const BandKernelArgs& a, uint32_t nblocks, hipStream_t s);
+hipError_t launch_band_multi_nobias_ends(const BandKernelArgs& a, uint32_t nblocks, hipStream_t s);
+hipError_t launch_band_multi_bias_ends(const BandKernelArgs& a, uint32_t nblocks, hipStream_t s);
+hipError_t launch_band_multi_nobias_scores(const BandKernelArgs& a, uint32_t nblocks, hipStream_t s);
+hipError_t launch_band_multi_bias_scores(const BandKernelArgs& a, uint32_t nblocks, hipStream_t s);
 
 const BandVariant* band_variants(int* count) {
     static const std::vector<BandVariant> table = [] {
@@ -21,8 +23,9 @@ const BandVariant* band_variants(int* count) {
     return table.data();
 }
 
-hipError_t launch_band_multi(const BandKernelArgs& a, uint32_t nblocks, bool bias, hipStream_t s) {
-    return bias ? launch_band_multi_bias(a, nblocks, s) : launch_band_multi_nobias(a, nblocks, s);
+hipError_t launch_band_multi(const BandKernelArgs& a, uint32_t nblocks, bool bias, bool ends, hipStream_t s) {
+    if (ends) return bias ? launch_band_multi_bias_ends(a, nblocks, s) : launch_band_multi_nobias_ends(a, nblocks, s);
+    return bias ? launch_band_multi_bias_scores(a, nblocks, s) : launch_band_multi_nobias_scores(a, nblocks, s);
 }
 
 }  // namespace saber

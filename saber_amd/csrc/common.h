@@ -98,7 +98,7 @@ struct BandVariant {
 };
 
 const BandVariant* band_variants(int* count);
-hipError_t launch_band_multi(const BandKernelArgs& a, uint32_t nworkgroups, bool bias, hipStream_t s);
+hipError_t launch_band_multi(const BandKernelArgs& a, uint32_t nworkgroups, bool bias, bool ends, hipStream_t s);
 hipError_t launch_generic(const GenericArgs& a, hipStream_t s);
 
 }  // namespace saber

@@ -27,7 +27,16 @@ def run_both(ctx, qs, ss, lo, hi, mat, go, ge, bias=None):
     sch = S.Scheme(mat, go, ge)
     got = ctx.score_bands(qres, qoff, sres, soff, jobs, sch, qb)
     want = O.banded_many(qres, qoff, sres, soff, jobs, mat, go, ge, qb)
+    check_scores_only(ctx, qres, qoff, sres, soff, jobs, sch, qb, want)
     return got, want
+
+
+def check_scores_only(ctx, qres, qoff, sres, soff, jobs, sch, qb, want):
+    """The scores-only build (score_banded_many's output) gives the oracle's
+    scores on the same jobs."""
+    sc = ctx.score_bands(qres, qoff, sres, soff, jobs, sch, qb, ends=False)[0]
+    bad = np.nonzero(sc.astype(np.int64) != np.asarray(want[0]).astype(np.int64))[0]
+    assert len(bad) == 0, f"scores-only build differs at jobs {bad[:10].tolist()}"
 
 
 def assert_same(got, want, what=""):
@@ -208,8 +217,10 @@ def run_shared(ctx, qs, ss, jq, lo, hi, mat, go, ge, bias=None):
     if bias is not None:
         qb, _ = flat([np.asarray(b, np.int8).view(np.uint8) for b in bias])
         qb = qb.view(np.int8)
-    got = ctx.score_bands(qres, qoff, sres, soff, jobs, S.Scheme(mat, go, ge), qb)
+    sch = S.Scheme(mat, go, ge)
+    got = ctx.score_bands(qres, qoff, sres, soff, jobs, sch, qb)
     want = O.banded_many(qres, qoff, sres, soff, jobs, mat, go, ge, qb)
+    check_scores_only(ctx, qres, qoff, sres, soff, jobs, sch, qb, want)
     return got, want
 
 
@@ -260,11 +271,13 @@ def test_region_workload_sample(ctx):
     got = ctx.score_bands(d["qres"], d["qoff"], d["sres"], d["soff"], d["jobs"], sch)
     want = O.banded_many(d["qres"], d["qoff"], d["sres"], d["soff"], d["jobs"], BLOSUM62, 11, 1)
     assert_same(got, want, "regions")
+    check_scores_only(ctx, d["qres"], d["qoff"], d["sres"], d["soff"], d["jobs"], sch, None, want)
     assert (got[0][d["homolog"]] > 60).mean() > 0.5
     qb = np.asarray(Lcg(1).next(1) + (np.arange(len(d["qres"])) % 9) - 4, np.int8)
     got = ctx.score_bands(d["qres"], d["qoff"], d["sres"], d["soff"], d["jobs"], sch, qb)
     want = O.banded_many(d["qres"], d["qoff"], d["sres"], d["soff"], d["jobs"], BLOSUM62, 11, 1, qb)
     assert_same(got, want, "regions+bias")
+    check_scores_only(ctx, d["qres"], d["qoff"], d["sres"], d["soff"], d["jobs"], sch, qb, want)
 
 
 def test_score_windows_backend(ctx):
