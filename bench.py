@@ -159,6 +159,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="target duration of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the planner's modelled shape choice instead of measuring it first")
     ap.add_argument("--xdrop-hits", type=int, default=4_000_000, help="seed hits for the X-drop leg")
     ap.add_argument("--check", type=int, default=3000, help="jobs checked against the oracle")
     ap.add_argument("--full-queries", type=int, default=20,
@@ -192,6 +194,14 @@ def main():
     gen_s = time.time() - t0
     sch = S.Scheme(BLOSUM62, 11, 1)
     ctx = S.Context(0)
+    tune = None
+    if not args.no_autotune:
+        # measured shape per class of band widths, on a sample of this batch
+        t0 = time.time()
+        tune = ctx.autotune(d["qres"], d["qoff"], d["sres"], d["soff"], d["jobs"], sch, target_cells=1 << 62)
+        print(f"# autotune {time.time() - t0:.1f}s: " + ", ".join(
+            f"w {e['w_lo']}-{e['w_hi']} -> {e['shape']}{' (changed)' if e['changed'] else ''}" for e in tune)
+            + (f"; sample {tune[-1]['gcups']} GCUPS" if tune else ""), file=sys.stderr, flush=True)
     t0 = time.time()
     # score_banded_many returns scores only: the step is the scores-only build;
     # the end-cell build (banded_score's outputs) is timed beside it.
@@ -356,7 +366,8 @@ def main():
             "traffic": measured_traffic(args),
         },
         "cpu_baseline": cpu,
-        "extra": {"band_with_end_cells": {"ms_per_step": round(ems, 4), "launch_ms": round(eband_ms, 4),
+        "extra": {"autotune": tune,
+                  "band_with_end_cells": {"ms_per_step": round(ems, 4), "launch_ms": round(eband_ms, 4),
                                           "gcups": round(cells / (ems * 1e-3) / 1e9, 2)},
                   "xdrop": xd, "full_sw": fl, "prefilter": pf},
     }

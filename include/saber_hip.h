@@ -222,6 +222,29 @@ int saber_hip_copy_probe(saber_hip_ctx* ctx, uint64_t bytes, int iters, double* 
  * the exact int32 kernel. */
 int saber_hip_band_choose(saber_hip_ctx* ctx, int w_lo, int w_hi, int L, int C, int K);
 
+/* Measured shape choice (no counterpart in the reference; the autotuner of
+ * the band planner). The sample is the batch's first jobs in order, up to
+ * target_cells band cells (0: 2e9), so every trial times the batch's own mix
+ * of widths and pairings. The band widths the packed kernel serves are split
+ * into classes of widths covered by the same shapes; starting from the
+ * modelled choice, classes holding at least 1 % of the sample's cells are
+ * visited in decreasing share, each covering shape is tried for the class,
+ * and it is kept when the sample's packed launch (scores only, HIP events,
+ * 5 runs) gets faster by more than 1 %. The result becomes the context's
+ * width -> shape table for batches created afterwards. report (may be NULL)
+ * receives up to nreport visited classes (reserved = 1 where the shape
+ * changed; gcups = the sample's rate after tuning); nclasses = their count. */
+typedef struct saber_hip_tune_entry {
+    int32_t w_lo, w_hi;              /* class widths */
+    int32_t L, C, K, reserved;       /* chosen shape */
+    uint64_t cells;                  /* the class's band cells in the sample */
+    double gcups;                    /* the sample's rate after tuning */
+} saber_hip_tune_entry;
+int saber_hip_band_autotune(saber_hip_ctx* ctx, const saber_hip_seqset* queries,
+                            const saber_hip_seqset* subjects, const saber_hip_band_job* jobs,
+                            uint64_t njobs, const saber_hip_scheme* scheme, uint64_t target_cells,
+                            saber_hip_tune_entry* report, int nreport, int* nclasses);
+
 /* Planner tuning (no counterpart in the reference): band jobs with more than
  * latency_rows rows are planned for low per-row latency (short lanes, their
  * blocks dispatched first); 0 turns this off, -1 restores the default rule

@@ -323,6 +323,21 @@ class Context:
         (0: never)."""
         _ffi.check(_ffi.lib().saber_hip_band_tune(self.h, int(latency_rows)))
 
+    def autotune(self, qres, qoff, sres, soff, jobs, scheme, target_cells=2_000_000_000):
+        """Measured shape per class of band widths on the first jobs of this
+        batch (saber_hip_band_autotune); returns one dict per visited class."""
+        qs, ss, keep = self._seqsets(qres, qoff, None, sres, soff)
+        jobs = np.ascontiguousarray(jobs)
+        sch = scheme.ffi()
+        rep = (_ffi.TuneEntry * 64)()
+        n = C.c_int()
+        _ffi.check(_ffi.lib().saber_hip_band_autotune(
+            self.h, C.byref(qs), None if ss is None else C.byref(ss),
+            jobs.ctypes.data_as(C.POINTER(_ffi.BandJob)), len(jobs), C.byref(sch), int(target_cells), rep, 64,
+            C.byref(n)))
+        return [{"w_lo": e.w_lo, "w_hi": e.w_hi, "shape": (e.L, e.C, e.K), "changed": bool(e.reserved),
+                 "cells": int(e.cells), "gcups": round(e.gcups, 1)} for e in rep[:min(n.value, 64)]]
+
     def upload_db(self, qres, qoff):
         off = np.ascontiguousarray(qoff, dtype=np.uint64)
         res = np.ascontiguousarray(qres, dtype=np.uint8)

@@ -60,6 +60,11 @@ class Plan(C.Structure):
                 ("banded", C.c_int32), ("reserved", C.c_uint32)]
 
 
+class TuneEntry(C.Structure):
+    _fields_ = [("w_lo", C.c_int32), ("w_hi", C.c_int32), ("L", C.c_int32), ("C", C.c_int32),
+                ("K", C.c_int32), ("reserved", C.c_int32), ("cells", C.c_uint64), ("gcups", C.c_double)]
+
+
 class PlanParams(C.Structure):
     _fields_ = [("window_slack", C.c_int64), ("score_band", C.c_int64), ("hsp_threshold", C.c_int32),
                 ("exhaustive", C.c_int32)]
@@ -129,6 +134,9 @@ _SIGS = {
                                          C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "saber_hip_band_choose": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
     "saber_hip_band_tune": (C.c_int, [P, C.c_int]),
+    "saber_hip_band_autotune": (C.c_int, [P, C.POINTER(SeqSet), C.POINTER(SeqSet), C.POINTER(BandJob), C.c_uint64,
+                                          C.POINTER(Scheme), C.c_uint64, C.POINTER(TuneEntry), C.c_int,
+                                          C.POINTER(C.c_int)]),
     "saber_hip_stats_gapped_protein": (C.c_int, [C.c_int, C.c_int32, C.c_int32, C.c_int, C.POINTER(Karlin)]),
     "saber_hip_stats_length_adjustment": (C.c_int, [C.c_double, C.c_double, C.c_double, C.c_double, C.c_int64,
                                                      C.c_int64, C.c_int64, C.POINTER(C.c_int64),

@@ -397,6 +397,149 @@ int saber_hip_band_tune(saber_hip_ctx* c, int latency_rows) {
     return SABER_HIP_OK;
 }
 
+int saber_hip_band_autotune(saber_hip_ctx* c, const saber_hip_seqset* queries,
+                            const saber_hip_seqset* subjects, const saber_hip_band_job* jobs,
+                            uint64_t njobs, const saber_hip_scheme* scheme, uint64_t target_cells,
+                            saber_hip_tune_entry* report, int nreport, int* nclasses) {
+    if (!c || !scheme) return fail(SABER_HIP_EINVAL, "null argument");
+    if (njobs > 0 && !jobs) return fail(SABER_HIP_EINVAL, "jobs is null");
+    if (int r = check_seqset(queries, "queries")) return r;
+    if (subjects)
+        if (int r = check_seqset(subjects, "subjects")) return r;
+    if (nclasses) *nclasses = 0;
+    if (target_cells == 0) target_cells = 2000000000ull;
+    std::vector<Shape> shapes;
+    std::vector<int> choice0;
+    {
+        std::lock_guard<std::mutex> g(c->mu);
+        shapes = c->shapes;
+        choice0 = c->choice;
+    }
+    int wmax = 0;
+    for (const Shape& sh : shapes) wmax = std::max(wmax, sh.W());
+    // Classes: maximal runs of widths covered by the same set of shapes.
+    std::vector<std::pair<int, int>> classes;
+    std::vector<int> wclass((size_t)wmax + 1, -1);
+    std::vector<bool> prev;
+    for (int w = 1; w <= wmax; ++w) {
+        std::vector<bool> cov(shapes.size());
+        bool any = false;
+        for (size_t k = 0; k < shapes.size(); ++k) any |= (cov[k] = shapes[k].covers(w));
+        if (!any) {
+            prev.clear();
+            continue;
+        }
+        if (cov != prev) classes.push_back({w, w});
+        classes.back().second = w;
+        wclass[w] = (int)classes.size() - 1;
+        prev = cov;
+    }
+    // The sample: the batch's first jobs in order, up to target_cells band
+    // cells, so that every trial times the same mix of widths, pairings and
+    // queries as the batch itself.
+    const uint64_t* soffs = subjects ? subjects->offsets : c->db_off.data();
+    const uint32_t nsub = subjects ? subjects->count : (c->db_off.empty() ? 0 : (uint32_t)c->db_off.size() - 1);
+    std::vector<uint64_t> ccells(classes.size(), 0);
+    uint64_t nsample = 0, tot = 0;
+    for (; nsample < njobs && tot < target_cells; ++nsample) {
+        const saber_hip_band_job& J = jobs[nsample];
+        if (J.query >= queries->count || J.subject >= nsub) return fail(SABER_HIP_EINVAL, "job index out of range");
+        const uint64_t slen = soffs[J.subject + 1] - soffs[J.subject];
+        if (J.win_hi < J.win_lo || J.win_hi > slen) return fail(SABER_HIP_EINVAL, "job window outside its subject");
+        const int64_t m = (int64_t)(queries->offsets[J.query + 1] - queries->offsets[J.query]);
+        const int64_t n = (int64_t)J.win_hi - J.win_lo;
+        if (m == 0 || n == 0 || J.band_hi < J.band_lo) continue;
+        const int64_t lo = std::max<int64_t>(J.band_lo, -(m - 1));
+        const int64_t hi = std::min<int64_t>(J.band_hi, n - 1);
+        const int64_t w = hi - lo + 1;
+        const int64_t i_first = std::max<int64_t>(0, -hi), i_last = std::min<int64_t>(m - 1, n - 1 - lo);
+        if (w <= 0 || i_last < i_first) continue;
+        BandJobDev d{};
+        d.m = (uint32_t)m;
+        d.n = (uint32_t)n;
+        d.lo = (int32_t)lo;
+        d.w = (int32_t)w;
+        d.i_first = (int32_t)i_first;
+        d.rows = (int32_t)(i_last - i_first + 1);
+        const uint64_t cells = band_cells(d);
+        tot += cells;
+        if (w <= wmax && wclass[w] >= 0) ccells[wclass[w]] += cells;
+    }
+    // Coordinate descent from the modelled choice: the classes in decreasing
+    // share of the sample's cells; per class every covering shape, kept when
+    // the sample's packed launch gets faster by more than 1 %.
+    std::vector<int> order;
+    for (size_t cl = 0; cl < classes.size(); ++cl)
+        if (ccells[cl] * 100 >= tot && ccells[cl] > 0) order.push_back((int)cl);
+    std::sort(order.begin(), order.end(), [&](int x, int y) { return ccells[x] > ccells[y]; });
+    int saved_rows = -1;
+    std::vector<int> saved_lat;
+    {
+        std::lock_guard<std::mutex> g(c->mu);
+        saved_rows = c->latency_rows;
+        saved_lat = c->choice_lat;
+    }
+    auto trial = [&](double* ms_out) -> int {
+        saber_hip_band_batch* b = nullptr;
+        if (int r = saber_hip_band_batch_create(c, queries, subjects, jobs, nsample, scheme, 0, &b)) return r;
+        double ms = 0.0, band_ms = 0.0;
+        const int r = saber_hip_band_batch_time(b, 1, 5, &ms, &band_ms, nullptr, nullptr, nullptr);
+        saber_hip_band_batch_destroy(b);
+        *ms_out = band_ms;
+        return r;
+    };
+    double best_ms = 0.0;
+    int status = trial(&best_ms);
+    std::vector<int> chosen(classes.size(), -1);
+    for (int cl : order) {
+        if (status != SABER_HIP_OK) break;
+        const auto [w_lo, w_hi] = classes[cl];
+        const int start = choice0[w_lo];
+        chosen[cl] = start;
+        for (size_t k = 0; k < shapes.size(); ++k) {
+            if ((int)k == start || !shapes[k].covers(w_lo) || !shapes[k].covers(w_hi)) continue;
+            {
+                std::lock_guard<std::mutex> g(c->mu);
+                for (int w = w_lo; w <= w_hi; ++w) c->choice[w] = (int)k;
+            }
+            double ms = 0.0;
+            status = trial(&ms);
+            if (status != SABER_HIP_OK) break;
+            if (ms < best_ms * 0.99) {
+                best_ms = ms;
+                chosen[cl] = (int)k;
+            }
+        }
+        std::lock_guard<std::mutex> g(c->mu);
+        for (int w = w_lo; w <= w_hi; ++w) c->choice[w] = chosen[cl] >= 0 ? chosen[cl] : choice0[w];
+    }
+    {
+        std::lock_guard<std::mutex> g(c->mu);
+        c->latency_rows = saved_rows;
+        c->choice_lat = saved_lat;
+        if (status != SABER_HIP_OK) c->choice = choice0;
+    }
+    if (status != SABER_HIP_OK) return status;
+    int nout = 0;
+    for (int cl : order) {
+        if (chosen[cl] < 0) continue;
+        if (report && nout < nreport) {
+            saber_hip_tune_entry& e = report[nout];
+            e.w_lo = classes[cl].first;
+            e.w_hi = classes[cl].second;
+            e.L = shapes[chosen[cl]].L;
+            e.C = shapes[chosen[cl]].C;
+            e.K = shapes[chosen[cl]].K;
+            e.reserved = chosen[cl] != choice0[classes[cl].first];
+            e.cells = ccells[cl];
+            e.gcups = best_ms > 0.0 ? (double)tot / best_ms / 1e6 : 0.0;
+        }
+        ++nout;
+    }
+    if (nclasses) *nclasses = nout;
+    return SABER_HIP_OK;
+}
+
 int saber_hip_band_batch_create(saber_hip_ctx* c, const saber_hip_seqset* queries,
                                 const saber_hip_seqset* subjects, const saber_hip_band_job* jobs,
                                 uint64_t njobs, const saber_hip_scheme* scheme, int ends,

@@ -309,3 +309,23 @@ def test_reference_named_interfaces(ctx):
     wag = S.encode("WAG")
     assert S.score_many(q, None, [q, wag], BLOSUM62, S.GapCosts(11, 1), ctx) == [
         37, O.sw_reference(q, wag, BLOSUM62, None, 11, 1)[0]]
+
+
+def test_autotune_choice_is_exact():
+    """The measured shape choice (saber_hip_band_autotune) installs covering
+    shapes only, and batches planned with it stay bit-exact."""
+    ctx = S.Context()
+    d = workload.band_regions(seed=8, n=20000)
+    sch = S.Scheme(BLOSUM62, 11, 1)
+    rep = ctx.autotune(d["qres"], d["qoff"], d["sres"], d["soff"], d["jobs"], sch, target_cells=20_000_000)
+    assert rep, "no width class was tuned"
+    for e in rep:
+        L, C, K = e["shape"]
+        W = L * C
+        lo_w = 1 if K == C else W - K + 1
+        assert lo_w <= e["w_lo"] <= e["w_hi"] <= W, e
+        assert e["gcups"] > 0
+    got = ctx.score_bands(d["qres"], d["qoff"], d["sres"], d["soff"], d["jobs"], sch)
+    want = O.banded_many(d["qres"], d["qoff"], d["sres"], d["soff"], d["jobs"], BLOSUM62, 11, 1)
+    assert_same(got, want, "autotuned")
+    ctx.close()
