@@ -52,6 +52,7 @@ extern "C" {
 
 /* Context options. */
 #define SABER_HIP_FORCE_EXACT32 1u /* score every band job with the int32 kernel (testing) */
+#define SABER_HIP_NO_GRAPH 2u      /* enqueue band passes directly instead of replaying a captured graph */
 typedef struct saber_hip_opts {
     uint32_t flags;
     uint32_t reserved[7];
@@ -123,6 +124,11 @@ int saber_hip_score_bands(saber_hip_ctx* ctx, const saber_hip_seqset* queries,
 /* Device-resident batch: create uploads the inputs and plans the work once;
  * run enqueues the whole scoring pass (asynchronous, on the context stream;
  * may be repeated); wait blocks until it is done; read copies the results.
+ * A pass runs entirely on the device (output clear, the persistent packed
+ * kernel, the int32 kernel over wide jobs and over the packed kernel's
+ * saturated jobs, whose count it reads on the device); the first run captures
+ * it as a HIP graph, later runs replay the graph (SABER_HIP_NO_GRAPH in the
+ * context flags enqueues it directly instead).
  * ends = 0 computes scores only; reading end cells from such a batch is
  * SABER_HIP_ESTATE. */
 int saber_hip_band_batch_create(saber_hip_ctx* ctx, const saber_hip_seqset* queries,
@@ -136,11 +142,13 @@ int saber_hip_band_batch_read(saber_hip_band_batch* b, int32_t* score, uint32_t*
 void saber_hip_band_batch_destroy(saber_hip_band_batch* b);
 
 /* Timing of a batch: warmup + iters runs bracketed by HIP events on the
- * context stream. ms_run = mean wall time of one run on the device;
- * band_ms = mean time of the packed kernel's launch (one launch per run, all
- * shapes); variant_cells[v] = band cells scored by variant v (v indexes
- * saber_hip_band_variant); nvar is the capacity of variant_cells on input and
- * the number of variants on output. cells = band cells per run. */
+ * context stream. ms_run = mean wall time of one run on the device (as run
+ * enqueues it: the graph replay); band_ms = mean time of the packed kernel's
+ * launch (one launch per run, all shapes; measured over iters further passes
+ * enqueued directly with events around the launch); variant_cells[v] = band
+ * cells scored by variant v (v indexes saber_hip_band_variant); nvar is the
+ * capacity of variant_cells on input and the number of variants on output.
+ * cells = band cells per run. */
 int saber_hip_band_batch_time(saber_hip_band_batch* b, int warmup, int iters, double* ms_run,
                               double* band_ms, uint64_t* variant_cells, int* nvar,
                               uint64_t* cells);

@@ -285,10 +285,10 @@ def band_variants():
 class Context:
     """One device context (saber_hip_create)."""
 
-    def __init__(self, device=0, force_exact32=False):
+    def __init__(self, device=0, force_exact32=False, graphs=True):
         L = _ffi.lib()
         opts = _ffi.Opts()
-        opts.flags = _ffi.FORCE_EXACT32 if force_exact32 else 0
+        opts.flags = (_ffi.FORCE_EXACT32 if force_exact32 else 0) | (0 if graphs else _ffi.NO_GRAPH)
         h = C.c_void_p()
         self._children = weakref.WeakSet()   # batches and models on this context
         _ffi.check(L.saber_hip_create(device, C.byref(opts), C.byref(h)))

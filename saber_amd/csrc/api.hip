@@ -158,9 +158,16 @@ struct saber_hip_band_batch {
     // packed kernel's saturated jobs known after it ran
     std::vector<uint32_t> generic_static;
     std::vector<BandJobDev> host_jobs;
-    DevBuf<uint32_t> gen_ids;
+    DevBuf<uint32_t> gen_ids;      // static int32 jobs (known at plan time), uploaded once
     DevBuf<uint64_t> gen_off;
     DevBuf<int32_t> gen_work;
+    // saturated jobs are rescored by a fixed grid reading their count on the
+    // device: flag_threads threads with flag_slot int32 of work each
+    DevBuf<int32_t> flag_work;
+    uint32_t flag_threads = 0, flag_slot = 0;
+    // the whole step captured once and replayed
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
     DevBuf<uint32_t> flagged;
     DevBuf<uint32_t> nflagged;
     DevBuf<int32_t> score;
@@ -757,6 +764,33 @@ int saber_hip_band_batch_create(saber_hip_ctx* c, const saber_hip_seqset* querie
     if (int r = upload(b->blocks, b->host_blocks.data(), b->host_blocks.size(), st)) return r;
     HIP_TRY(b->flagged.alloc(njobs ? njobs : 1));
     HIP_TRY(b->nflagged.alloc(1));
+    {
+        // static int32 jobs: ids and work offsets, uploaded once
+        const std::vector<uint32_t>& ids = b->generic_static;
+        if (!ids.empty()) {
+            std::vector<uint64_t> off(ids.size());
+            uint64_t tot = 0;
+            for (size_t i = 0; i < ids.size(); ++i) {
+                off[i] = tot;
+                tot += 4 * ((uint64_t)b->host_jobs[ids[i]].w + 1);
+            }
+            if (int r = upload(b->gen_ids, ids.data(), ids.size(), st)) return r;
+            if (int r = upload(b->gen_off, off.data(), off.size(), st)) return r;
+            HIP_TRY(b->gen_work.alloc(tot));
+        }
+        // saturated-job slots: any packed job may saturate, so a slot holds
+        // the widest packed job's two band rows
+        int32_t wmax = 0;
+        std::vector<bool> is_static(njobs, false);
+        for (uint32_t k : ids) is_static[k] = true;
+        for (uint64_t k = 0; k < njobs; ++k)
+            if (!is_static[k] && b->host_jobs[k].rows > 0) wmax = std::max(wmax, b->host_jobs[k].w);
+        if (njobs > 0) {
+            b->flag_slot = 4u * ((uint32_t)wmax + 1u);
+            b->flag_threads = (uint32_t)std::min<uint64_t>(8192, std::max<uint64_t>(128, (njobs / 64 + 127) / 128 * 128));
+            HIP_TRY(b->flag_work.alloc((uint64_t)b->flag_threads * b->flag_slot));
+        }
+    }
     HIP_TRY(b->score.alloc(njobs ? njobs : 1));
     HIP_TRY(b->q_end.alloc(njobs ? njobs : 1));
     HIP_TRY(b->s_end.alloc(njobs ? njobs : 1));
@@ -766,21 +800,8 @@ int saber_hip_band_batch_create(saber_hip_ctx* c, const saber_hip_seqset* querie
     return SABER_HIP_OK;
 }
 
-static int run_generic(saber_hip_band_batch* b, const std::vector<uint32_t>& ids) {
-    if (ids.empty()) return SABER_HIP_OK;
-    hipStream_t st = b->ctx->stream;
-    std::vector<uint64_t> off(ids.size());
-    uint64_t tot = 0;
-    for (size_t i = 0; i < ids.size(); ++i) {
-        off[i] = tot;
-        tot += 4 * ((uint64_t)b->host_jobs[ids[i]].w + 1);
-    }
-    if (int r = upload(b->gen_ids, ids.data(), ids.size(), st)) return r;
-    if (int r = upload(b->gen_off, off.data(), off.size(), st)) return r;
-    if (b->gen_work.n < tot) HIP_TRY(b->gen_work.alloc(tot));
+static GenericArgs generic_args(saber_hip_band_batch* b) {
     GenericArgs a{};
-    a.ids = b->gen_ids.p;
-    a.nids = (uint32_t)ids.size();
     a.jobs = b->jobs.p;
     a.qres = b->q.p + kGuard;
     a.qbias = b->has_bias ? b->bias.p + kGuard : nullptr;
@@ -789,16 +810,14 @@ static int run_generic(saber_hip_band_batch* b, const std::vector<uint32_t>& ids
     a.go = b->go;
     a.ge = b->ge;
     a.out = BandOutputs{b->score.p, b->q_end.p, b->s_end.p, b->cells.p};
-    a.work = b->gen_work.p;
-    a.work_off = b->gen_off.p;
-    HIP_TRY(launch_generic(a, st));
-    // The id/offset uploads read host vectors owned by this frame.
-    HIP_TRY(hipStreamSynchronize(st));
-    return SABER_HIP_OK;
+    return a;
 }
 
-// Enqueue everything up to (not including) the rescore of saturated jobs.
-static int enqueue_main(saber_hip_band_batch* b, hipEvent_t ev_begin, hipEvent_t ev_end) {
+// One scoring pass, entirely on the device (so it can be captured as a
+// graph): output clear, the packed kernel, the int32 kernel over the jobs
+// routed to it at plan time and over the packed kernel's saturated jobs (their
+// count is read on the device). ev_begin / ev_end bracket the packed kernel.
+static int enqueue_step(saber_hip_band_batch* b, hipEvent_t ev_begin, hipEvent_t ev_end) {
     hipStream_t st = b->ctx->stream;
     const uint64_t n = b->njobs;
     HIP_TRY(fill_u32(reinterpret_cast<uint32_t*>(b->score.p), n, 0, st));
@@ -830,23 +849,42 @@ static int enqueue_main(saber_hip_band_batch* b, hipEvent_t ev_begin, hipEvent_t
     const uint32_t wg = (uint32_t)std::min<uint64_t>((uint64_t)b->ctx->cus, (b->host_blocks.size() + 11) / 12);
     if (a.nchunks) HIP_TRY(launch_band_multi(a, wg, b->has_bias, b->ends, st));
     if (ev_end) HIP_TRY(hipEventRecord(ev_end, st));
-    if (int r = run_generic(b, b->generic_static)) return r;
+    if (!b->generic_static.empty()) {
+        GenericArgs g = generic_args(b);
+        g.ids = b->gen_ids.p;
+        g.nids = (uint32_t)b->generic_static.size();
+        g.work = b->gen_work.p;
+        g.work_off = b->gen_off.p;
+        HIP_TRY(launch_generic(g, st));
+    }
+    if (b->flag_threads) {
+        GenericArgs g = generic_args(b);
+        g.ids = b->flagged.p;
+        g.work = b->flag_work.p;
+        HIP_TRY(launch_generic_slots(g, b->nflagged.p, b->flag_threads, b->flag_slot, st));
+    }
     return SABER_HIP_OK;
 }
 
-// Rescore the packed kernel's saturated jobs exactly (needs their count).
-static int finish_flagged(saber_hip_band_batch* b) {
+// Enqueue one pass: the captured graph (built on first use) unless graphs
+// are switched off for the context.
+static int enqueue_run(saber_hip_band_batch* b) {
     hipStream_t st = b->ctx->stream;
-    uint32_t nf = 0;
-    HIP_TRY(hipMemcpyAsync(&nf, b->nflagged.p, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
-    b->last_flagged = nf;
-    if (nf == 0) return SABER_HIP_OK;
-    std::vector<uint32_t> ids(nf);
-    HIP_TRY(hipMemcpyAsync(ids.data(), b->flagged.p, sizeof(uint32_t) * nf, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
-    std::sort(ids.begin(), ids.end());
-    if (int r = run_generic(b, ids)) return r;
+    if (b->ctx->flags & SABER_HIP_NO_GRAPH) return enqueue_step(b, nullptr, nullptr);
+    if (!b->graph_exec) {
+        HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        const int r = enqueue_step(b, nullptr, nullptr);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(st, &g);
+        if (r != SABER_HIP_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return r;
+        }
+        HIP_TRY(e);
+        b->graph = g;
+        HIP_TRY(hipGraphInstantiate(&b->graph_exec, b->graph, nullptr, nullptr, 0));
+    }
+    HIP_TRY(hipGraphLaunch(b->graph_exec, st));
     return SABER_HIP_OK;
 }
 
@@ -854,8 +892,7 @@ int saber_hip_band_batch_run(saber_hip_band_batch* b) {
     if (!b) return fail(SABER_HIP_EINVAL, "batch is null");
     std::lock_guard<std::mutex> g(b->ctx->mu);
     HIP_TRY(hipSetDevice(b->ctx->device));
-    if (int r = enqueue_main(b, nullptr, nullptr)) return r;
-    if (int r = finish_flagged(b)) return r;
+    if (int r = enqueue_run(b)) return r;
     b->ran = true;
     return SABER_HIP_OK;
 }
@@ -889,6 +926,8 @@ void saber_hip_band_batch_destroy(saber_hip_band_batch* b) {
     if (!b) return;
     (void)hipSetDevice(b->ctx->device);
     (void)hipStreamSynchronize(b->ctx->stream);
+    if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
     delete b;
 }
 
@@ -910,19 +949,24 @@ int saber_hip_band_batch_time(saber_hip_band_batch* b, int warmup, int iters, do
     HIP_TRY(hipEventCreate(&e1));
     for (auto& e : eb) HIP_TRY(hipEventCreate(&e));
     for (auto& e : ee) HIP_TRY(hipEventCreate(&e));
+    // whole passes as the caller runs them (the captured graph)
     HIP_TRY(hipStreamSynchronize(st));
     HIP_TRY(hipEventRecord(e0, st));
-    for (int it = 0; it < iters; ++it) {
-        if (int r = enqueue_main(b, eb[it], ee[it])) return r;
-        if (int r = finish_flagged(b)) return r;
-    }
+    for (int it = 0; it < iters; ++it)
+        if (int r = enqueue_run(b)) return r;
     HIP_TRY(hipEventRecord(e1, st));
     HIP_TRY(hipEventSynchronize(e1));
     b->ran = true;
     float tot = 0.f;
     HIP_TRY(hipEventElapsedTime(&tot, e0, e1));
     if (ms_run) *ms_run = tot / iters;
+    HIP_TRY(hipMemcpy(&b->last_flagged, b->nflagged.p, sizeof(uint32_t), hipMemcpyDeviceToHost));
     if (band_ms) {
+        // the packed kernel alone: the same passes enqueued directly, with
+        // events around its launch
+        for (int it = 0; it < iters; ++it)
+            if (int r = enqueue_step(b, eb[it], ee[it])) return r;
+        HIP_TRY(hipStreamSynchronize(st));
         double sum = 0.0;
         for (int it = 0; it < iters; ++it) {
             float ms = 0.f;

@@ -100,6 +100,10 @@ struct BandVariant {
 const BandVariant* band_variants(int* count);
 hipError_t launch_band_multi(const BandKernelArgs& a, uint32_t nworkgroups, bool bias, bool ends, hipStream_t s);
 hipError_t launch_generic(const GenericArgs& a, hipStream_t s);
+// a.ids[0 .. *count) with the count read on the device; nthreads threads (a
+// multiple of 128), each with slot_ints int32 of a.work (a.work_off unused).
+hipError_t launch_generic_slots(const GenericArgs& a, const uint32_t* count, uint32_t nthreads, uint32_t slot_ints,
+                                hipStream_t s);
 
 }  // namespace saber
 

@@ -13,13 +13,11 @@ constexpr int32_t kNeg32 = INT32_MIN / 4;
 
 __device__ __forceinline__ int32_t imax(int32_t a, int32_t b) { return a > b ? a : b; }
 
-__global__ __launch_bounds__(128) void generic_band_kernel(GenericArgs a) {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= a.nids) return;
-    const uint32_t id = a.ids[t];
+// One job, its two band rows in work (4 * (w + 1) int32).
+__device__ void generic_job(const GenericArgs& a, uint32_t id, int32_t* work) {
     const BandJobDev J = a.jobs[id];
     const int64_t w = J.w;
-    int32_t* hp = a.work + a.work_off[t];
+    int32_t* hp = work;
     int32_t* fp = hp + (w + 1);
     int32_t* hc = fp + (w + 1);
     int32_t* fc = hc + (w + 1);
@@ -87,6 +85,33 @@ __global__ __launch_bounds__(128) void generic_band_kernel(GenericArgs a) {
     a.out.q_end[id] = best > 0 ? (uint32_t)bq : 0u;
     a.out.s_end[id] = best > 0 ? (uint32_t)bs : 0u;
     a.out.cells[id] = cells;
+}
+
+// One thread per listed job, work at work_off[thread].
+__global__ __launch_bounds__(128) void generic_band_kernel(GenericArgs a) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.nids) return;
+    generic_job(a, a.ids[t], a.work + a.work_off[t]);
+}
+
+// The jobs a.ids[0 .. *count) for a count known only on the device (the
+// packed kernel's saturated jobs): a fixed grid walks the list, each thread
+// with its own work slot of slot_ints int32.
+__global__ __launch_bounds__(128) void generic_slots_kernel(GenericArgs a, const uint32_t* count, uint32_t slot_ints) {
+    const uint32_t n = *count;
+    const uint32_t nt = gridDim.x * blockDim.x;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    int32_t* work = a.work + (uint64_t)t * slot_ints;
+    for (uint32_t i = t; i < n; i += nt) generic_job(a, a.ids[i], work);
+}
+
+hipError_t launch_generic_slots(const GenericArgs& a, const uint32_t* count, uint32_t nthreads, uint32_t slot_ints,
+                                hipStream_t s) {
+    if (nthreads == 0) return hipSuccess;
+    const uint32_t block = 128;
+    hipLaunchKernelGGL(generic_slots_kernel, dim3((nthreads + block - 1) / block), dim3(block), 0, s, a, count,
+                       slot_ints);
+    return hipGetLastError();
 }
 
 hipError_t launch_generic(const GenericArgs& a, hipStream_t s) {

@@ -329,3 +329,47 @@ def test_autotune_choice_is_exact():
     want = O.banded_many(d["qres"], d["qoff"], d["sres"], d["soff"], d["jobs"], BLOSUM62, 11, 1)
     assert_same(got, want, "autotuned")
     ctx.close()
+
+
+def test_graph_replays_match_direct_passes():
+    """Band passes replayed from the captured graph equal passes enqueued
+    directly, on every replay, with other work and synchronisations between
+    the replays; saturated jobs (rescored by the int32 kernel, their count read
+    on the device) and jobs too wide for the packed kernel included."""
+    d = workload.band_regions(seed=13, n=6000)
+    jobs = d["jobs"].copy()
+    jobs["band_lo"][:20] = -600          # wider than every packed shape
+    jobs["band_hi"][:20] = 600
+    a = np.full(3000, W, np.uint8)        # W x 3000 saturates int16
+    qres = np.concatenate([d["qres"], a])
+    qoff = np.concatenate([d["qoff"], [d["qoff"][-1] + 3000]]).astype(np.uint64)
+    sres = np.concatenate([d["sres"], a])
+    soff = np.concatenate([d["soff"], [d["soff"][-1] + 3000]]).astype(np.uint64)
+    extra = np.zeros(3, dtype=jobs.dtype)
+    extra["query"] = len(qoff) - 2
+    extra["subject"] = len(soff) - 2
+    extra["win_lo"] = 0
+    extra["win_hi"] = 3000
+    extra["band_lo"] = [-8, -40, 0]
+    extra["band_hi"] = [8, 40, 0]
+    jobs = np.concatenate([jobs, extra])
+    sch = S.Scheme(BLOSUM62, 11, 1)
+    direct = S.Context(graphs=False)
+    want = direct.score_bands(qres, qoff, sres, soff, jobs, sch)
+    direct.close()
+    assert (want[0][-3:] == 33000).sum() >= 2
+    ctx = S.Context()
+    for ends in (True, False):
+        b = ctx.band_batch(qres, qoff, sres, soff, jobs, sch, ends=ends)
+        other = ctx.band_batch(d["qres"], d["qoff"], d["sres"], d["soff"], d["jobs"][:500], sch)
+        for rep in range(4):
+            b.run()
+            got = b.read()
+            assert np.array_equal(got[0], want[0]), f"replay {rep} scores"
+            if ends:
+                assert all(np.array_equal(g, w) for g, w in zip(got[1:], want[1:])), f"replay {rep} ends/cells"
+            other.run()          # other work between the replays
+            other.wait()
+        b.close()
+        other.close()
+    ctx.close()
