@@ -397,6 +397,15 @@ int saber_hip_region_features(saber_hip_ctx* ctx, const saber_hip_seqset* querie
                               const saber_hip_xdrop_hit* hits, const int32_t* xscore, const uint32_t* xq_lo,
                               const uint32_t* xq_hi, uint32_t hits_per_region, int32_t hsp_threshold,
                               const float* mean, const float* inv_std, uint16_t* features);
+/* The same, also timing the feature kernel (a diagnostic for the prefilter's
+ * cost): kernel_ms = mean of iters further launches on the uploaded inputs,
+ * HIP events on the context stream. */
+int saber_hip_region_features_time(saber_hip_ctx* ctx, const saber_hip_seqset* queries,
+                                   const saber_hip_seqset* subjects, const saber_hip_band_job* jobs, uint64_t n,
+                                   const saber_hip_xdrop_hit* hits, const int32_t* xscore, const uint32_t* xq_lo,
+                                   const uint32_t* xq_hi, uint32_t hits_per_region, int32_t hsp_threshold,
+                                   const float* mean, const float* inv_std, uint16_t* features, int iters,
+                                   double* kernel_ms);
 #define SABER_HIP_MLP_IN 64
 #define SABER_HIP_MLP_HID 256
 #define SABER_HIP_MLP_NPARAMS (64 * 256 + 256 + 256 * 256 + 256 + 256 + 1)

@@ -173,6 +173,11 @@ _SIGS = {
                                             C.POINTER(C.c_uint32), C.POINTER(C.c_int32), C.POINTER(C.c_uint32),
                                             C.POINTER(C.c_uint32), C.c_uint32, C.c_int32, C.POINTER(C.c_float),
                                             C.POINTER(C.c_float), C.POINTER(C.c_uint16)]),
+    "saber_hip_region_features_time": (C.c_int, [P, C.POINTER(SeqSet), C.POINTER(SeqSet), C.POINTER(BandJob),
+                                                 C.c_uint64, C.POINTER(C.c_uint32), C.POINTER(C.c_int32),
+                                                 C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.c_uint32, C.c_int32,
+                                                 C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_uint16),
+                                                 C.c_int, C.POINTER(C.c_double)]),
     "saber_hip_mlp_create": (C.c_int, [P, C.POINTER(C.c_float), C.c_uint64, C.POINTER(P)]),
     "saber_hip_mlp_get_params": (C.c_int, [P, C.POINTER(C.c_float)]),
     "saber_hip_mlp_set_params": (C.c_int, [P, C.POINTER(C.c_float)]),

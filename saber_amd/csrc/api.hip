@@ -2058,12 +2058,12 @@ extern "C" int saber_hip_score_regions(saber_hip_ctx* c, const saber_hip_seqset*
 // Prefilter features of candidate regions (features.hip).
 
 
-extern "C" int saber_hip_region_features(saber_hip_ctx* c, const saber_hip_seqset* queries,
-                                         const saber_hip_seqset* subjects, const saber_hip_band_job* jobs,
-                                         uint64_t n, const saber_hip_xdrop_hit* hits, const int32_t* xscore,
-                                         const uint32_t* xq_lo, const uint32_t* xq_hi, uint32_t hits_per_region,
-                                         int32_t hsp_threshold, const float* mean, const float* inv_std,
-                                         uint16_t* features) {
+static int region_features_impl(saber_hip_ctx* c, const saber_hip_seqset* queries,
+                                const saber_hip_seqset* subjects, const saber_hip_band_job* jobs, uint64_t n,
+                                const saber_hip_xdrop_hit* hits, const int32_t* xscore, const uint32_t* xq_lo,
+                                const uint32_t* xq_hi, uint32_t hits_per_region, int32_t hsp_threshold,
+                                const float* mean, const float* inv_std, uint16_t* features, int iters,
+                                double* kernel_ms) {
     if (!c || !queries || !subjects || (n && (!jobs || !features))) return fail(SABER_HIP_EINVAL, "null argument");
     if (hits_per_region > 64) return fail(SABER_HIP_EINVAL, "at most 64 hits per region");
     if (n && hits_per_region && (!hits || !xscore || !xq_lo || !xq_hi)) return fail(SABER_HIP_EINVAL, "null hit arrays");
@@ -2103,13 +2103,49 @@ extern "C" int saber_hip_region_features(saber_hip_ctx* c, const saber_hip_seqse
     if (int r = upload(dlo, xq_lo, nh, st)) return r;
     if (int r = upload(dhi, xq_hi, nh, st)) return r;
     HIP_TRY(dout.alloc(64 * n));
+    DevBuf<float> dcomp;
+    HIP_TRY(dcomp.alloc(20 * ((uint64_t)queries->count + 1)));
     FeatureArgs a{dj.p, n, dqo.p, dq.p + kGuard, dso.p, ds.p + kGuard, dh.p, dsc.p, dlo.p, dhi.p,
-                  hits_per_region, hsp_threshold, mean ? dmean.p : nullptr, mean ? dinv.p : nullptr, dout.p};
-    int cus = 0;
-    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
-    const int grid = (int)std::min<uint64_t>((n + 3) / 4, (uint64_t)cus * 16);
-    HIP_TRY(launch_region_features(a, grid, st));
+                  hits_per_region, hsp_threshold, mean ? dmean.p : nullptr, mean ? dinv.p : nullptr, dout.p,
+                  dcomp.p};
+    HIP_TRY(launch_region_features(a, queries->count, st));
     HIP_TRY(hipMemcpyAsync(features, dout.p, 2 * 64 * n, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
+    if (iters > 0 && kernel_ms) {
+        hipEvent_t e0, e1;
+        HIP_TRY(hipEventCreate(&e0));
+        HIP_TRY(hipEventCreate(&e1));
+        HIP_TRY(hipEventRecord(e0, st));
+        for (int it = 0; it < iters; ++it) HIP_TRY(launch_region_features(a, queries->count, st));
+        HIP_TRY(hipEventRecord(e1, st));
+        HIP_TRY(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        *kernel_ms = ms / iters;
+    }
     return SABER_HIP_OK;
+}
+
+extern "C" int saber_hip_region_features(saber_hip_ctx* c, const saber_hip_seqset* queries,
+                                         const saber_hip_seqset* subjects, const saber_hip_band_job* jobs,
+                                         uint64_t n, const saber_hip_xdrop_hit* hits, const int32_t* xscore,
+                                         const uint32_t* xq_lo, const uint32_t* xq_hi, uint32_t hits_per_region,
+                                         int32_t hsp_threshold, const float* mean, const float* inv_std,
+                                         uint16_t* features) {
+    return region_features_impl(c, queries, subjects, jobs, n, hits, xscore, xq_lo, xq_hi, hits_per_region,
+                                hsp_threshold, mean, inv_std, features, 0, nullptr);
+}
+
+extern "C" int saber_hip_region_features_time(saber_hip_ctx* c, const saber_hip_seqset* queries,
+                                              const saber_hip_seqset* subjects, const saber_hip_band_job* jobs,
+                                              uint64_t n, const saber_hip_xdrop_hit* hits, const int32_t* xscore,
+                                              const uint32_t* xq_lo, const uint32_t* xq_hi,
+                                              uint32_t hits_per_region, int32_t hsp_threshold, const float* mean,
+                                              const float* inv_std, uint16_t* features, int iters,
+                                              double* kernel_ms) {
+    if (iters < 1 || !kernel_ms) return fail(SABER_HIP_EINVAL, "iters must be >= 1 and kernel_ms set");
+    return region_features_impl(c, queries, subjects, jobs, n, hits, xscore, xq_lo, xq_hi, hits_per_region,
+                                hsp_threshold, mean, inv_std, features, iters, kernel_ms);
 }

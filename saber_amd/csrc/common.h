@@ -232,7 +232,9 @@ struct FeatureArgs {
     const float* mean;         // optional standardisation: (f - mean) * inv_std
     const float* inv_std;
     __bf16* out;
+    float* qcomp;              // work: composition of each query (20 per query)
 };
-hipError_t launch_region_features(const FeatureArgs& a, int grid, hipStream_t s);
+// Query compositions (nq queries), then one lane per region.
+hipError_t launch_region_features(const FeatureArgs& a, uint32_t nq, hipStream_t s);
 
 }  // namespace saber

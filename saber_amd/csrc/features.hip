@@ -2,7 +2,8 @@
 // the device from what the pipeline holds when the prefilter runs (after the
 // X-drop stage and the window/band plan, before banded scoring): the region's
 // X-drop evidence, its geometry and the residue composition of the query and
-// the subject window. One wave per region.
+// the subject window. Query compositions are computed once per query; then
+// one lane per region.
 //
 // Features (bf16, 64 per region):
 //   0  best X-drop score of the region's hits / 64
@@ -26,91 +27,175 @@
 
 namespace saber {
 
-__global__ __launch_bounds__(256) void region_features_kernel(FeatureArgs a) {
-    __shared__ uint32_t hist[4][40];
+// Query composition, once per query (regions of one query share it): counts
+// of codes 0..19 over the query / its length. One wave per query.
+__global__ __launch_bounds__(256) void query_composition_kernel(const uint64_t* qoff, const uint8_t* qres,
+                                                                uint32_t nq, float* comp) {
+    __shared__ uint32_t hist[4][24];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-    for (uint64_t r = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; r < a.n; r += nw) {
-        const saber_hip_band_job J = a.jobs[r];
-        if (lane < 40) hist[wv][lane] = 0;
+    const uint32_t nw = (gridDim.x * blockDim.x) >> 6;
+    for (uint32_t qi = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; qi < nq; qi += nw) {
+        if (lane < 24) hist[wv][lane] = 0;
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
-        const uint8_t* q = a.qres + a.qoff[J.query];
-        const uint64_t m = a.qoff[J.query + 1] - a.qoff[J.query];
-        const uint8_t* s = a.sres + a.soff[J.subject] + J.win_lo;
-        const uint64_t nwin = J.win_hi - J.win_lo;
-        for (uint64_t i = lane; i < m; i += 64) {
-            const uint32_t c = q[i];
-            if (c < 20) atomicAdd(&hist[wv][c], 1u);
-        }
-        for (uint64_t i = lane; i < nwin; i += 64) {
-            const uint32_t c = s[i];
-            if (c < 20) atomicAdd(&hist[wv][20 + c], 1u);
-        }
-        // X-drop evidence of the region's hits (lane k: hit k)
-        int32_t sc = -2147483647, hsp = 0, hsum = 0, ext = 0;
-        int64_t dmin = INT64_MAX, dmax = INT64_MIN;
-        float ssum = 0.f;
-        if ((uint32_t)lane < a.per_region) {
-            const uint64_t h = r * a.per_region + lane;
-            sc = a.xscore[h];
-            ssum = (float)sc;
-            ext = (int32_t)(a.xhi[h] - a.xlo[h]);
-            if (sc >= a.hsp_threshold) {
-                hsp = 1;
-                hsum = sc;
-                const int64_t d = (int64_t)a.hits[4 * h + 3] - (int64_t)a.hits[4 * h + 2];
-                dmin = d;
-                dmax = d;
-            }
-        }
-        for (int o = 32; o >= 1; o >>= 1) {
-            sc = max(sc, __shfl_xor(sc, o));
-            hsp += __shfl_xor(hsp, o);
-            hsum += __shfl_xor(hsum, o);
-            ext = max(ext, __shfl_xor(ext, o));
-            ssum += __shfl_xor(ssum, o);
-            dmin = min(dmin, (int64_t)__shfl_xor((long long)dmin, o));
-            dmax = max(dmax, (int64_t)__shfl_xor((long long)dmax, o));
-        }
+        const uint8_t* q = qres + qoff[qi];
+        const uint64_t m = qoff[qi + 1] - qoff[qi];
+        for (uint64_t i = lane; i < m; i += 64) atomicAdd(&hist[wv][min((uint32_t)q[i], 23u)], 1u);
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
-        const int64_t mm = (int64_t)m, nn = (int64_t)nwin;
-        const int64_t blo = J.band_lo > -(mm - 1) ? J.band_lo : -(mm - 1);
-        const int64_t bhi = J.band_hi < nn - 1 ? J.band_hi : nn - 1;
-        const float width = bhi >= blo ? (float)(bhi - blo + 1) : 0.f;
-        const float best = a.per_region ? (float)sc : 0.f;
-        const float pr = a.per_region ? (float)a.per_region : 1.f;
-        float f = 0.f;
-        switch (lane) {
-            case 0: f = best / 64.f; break;
-            case 1: f = logf(1.f + fmaxf(best, 0.f)); break;
-            case 2: f = (float)hsp / pr; break;
-            case 3: f = (float)hsum / 128.f; break;
-            case 4: f = hsp >= 2 ? (float)(dmax - dmin) / 64.f : 0.f; break;
-            case 5: f = width / 128.f; break;
-            case 6: f = logf(1.f + (float)nwin); break;
-            case 7: f = logf(1.f + (float)m); break;
-            case 8: f = best / logf(2.f + (float)m * (float)nwin); break;
-            case 9: f = a.per_region ? ssum / pr / 64.f : 0.f; break;
-            case 10: f = (float)ext / 64.f; break;
-            case 11: f = best / (1.f + (float)ext); break;
-            default:
-                if (lane >= 12 && lane < 32)
-                    f = m ? (float)hist[wv][lane - 12] / (float)m : 0.f;
-                else if (lane >= 32 && lane < 52)
-                    f = nwin ? (float)hist[wv][lane - 12] / (float)nwin : 0.f;
-        }
-        if (a.mean) f = (f - a.mean[lane]) * a.inv_std[lane];
-        a.out[r * 64 + lane] = (__bf16)f;
+        if (lane < 20) comp[(uint64_t)qi * 20 + lane] = m ? (float)hist[wv][lane] / (float)m : 0.f;
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
     }
 }
 
-hipError_t launch_region_features(const FeatureArgs& a, int grid, hipStream_t s) {
+// One lane per region: the region's X-drop evidence, the composition of its
+// subject window (a per-lane histogram in LDS), its geometry and its query's
+// composition; the 64 features are stored as eight 16-byte rows. The kernel
+// is bound by memory latency, so the window is read in rounds of eight
+// independent 16-byte loads (128 residues per round trip).
+constexpr int kHistStride = 25;   // 24 codes, odd stride against LDS bank conflicts
+
+__global__ __launch_bounds__(256) void region_features_kernel(FeatureArgs a) {
+    __shared__ uint32_t hist[256 * kHistStride];
+    uint32_t* H = hist + threadIdx.x * kHistStride;
+    const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= a.n) return;
+    const saber_hip_band_job J = a.jobs[r];
+    const uint64_t m = a.qoff[J.query + 1] - a.qoff[J.query];
+    const uint8_t* s = a.sres + a.soff[J.subject] + J.win_lo;
+    const uint64_t nwin = J.win_hi - J.win_lo;
+
+    // subject-window composition: aligned 16-byte blocks from the one holding
+    // the window's first residue; loads past the window's last block re-read
+    // that block, and bytes outside the window are not counted (every block
+    // read lies inside the buffer, guard bytes included)
+#pragma unroll
+    for (int c = 0; c < 24; ++c) H[c] = 0;
+    if (nwin) {
+        const uintptr_t p = reinterpret_cast<uintptr_t>(s), pe = p + nwin;
+        const uintptr_t first = p & ~uintptr_t(15), last = (pe - 1) & ~uintptr_t(15);
+        const uint8_t* s0 = s - (p & 15);   // the first block, as a pointer
+        for (uintptr_t base = first; base <= last; base += 128) {
+            uint4 v[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const uintptr_t at = base + 16 * j;
+                const uintptr_t la = at <= last ? at : last;
+                v[j] = *reinterpret_cast<const uint4*>(s0 + (la - first));
+            }
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const uintptr_t at = base + 16 * j;
+                const int64_t b_lo = (int64_t)p - (int64_t)at;   // count bytes b with b_lo <= b < b_hi
+                const int64_t b_hi = (int64_t)pe - (int64_t)at;
+                const uint32_t d[4] = {v[j].x, v[j].y, v[j].z, v[j].w};
+#pragma unroll
+                for (int b = 0; b < 16; ++b)
+                    if (b >= b_lo && b < b_hi) atomicAdd(&H[min((d[b >> 2] >> (8 * (b & 3))) & 0xFFu, 23u)], 1u);
+            }
+        }
+    }
+
+    // X-drop evidence of the region's hits (four hits per 16-byte load when
+    // the region's hits start 16-byte aligned)
+    int32_t best = a.per_region ? INT32_MIN : 0, hsp = 0, hsum = 0, ext = 0;
+    int64_t dmin = INT64_MAX, dmax = INT64_MIN;
+    float ssum = 0.f;
+    auto take = [&](int32_t sc, uint32_t lo, uint32_t hi, uint32_t qp, uint32_t sp) {
+        best = max(best, sc);
+        ssum += (float)sc;
+        ext = max(ext, (int32_t)(hi - lo));
+        if (sc >= a.hsp_threshold) {
+            ++hsp;
+            hsum += sc;
+            const int64_t d = (int64_t)sp - (int64_t)qp;
+            dmin = min(dmin, d);
+            dmax = max(dmax, d);
+        }
+    };
+    const uint64_t h0 = r * a.per_region;
+    if ((a.per_region & 3) == 0) {
+        for (uint32_t k = 0; k < a.per_region; k += 4) {
+            const uint64_t h = h0 + k;
+            const int4 sc = *reinterpret_cast<const int4*>(a.xscore + h);
+            const uint4 lo = *reinterpret_cast<const uint4*>(a.xlo + h);
+            const uint4 hi = *reinterpret_cast<const uint4*>(a.xhi + h);
+            const uint4 t0 = *reinterpret_cast<const uint4*>(a.hits + 4 * h);
+            const uint4 t1 = *reinterpret_cast<const uint4*>(a.hits + 4 * h + 4);
+            const uint4 t2 = *reinterpret_cast<const uint4*>(a.hits + 4 * h + 8);
+            const uint4 t3 = *reinterpret_cast<const uint4*>(a.hits + 4 * h + 12);
+            take(sc.x, lo.x, hi.x, t0.z, t0.w);
+            take(sc.y, lo.y, hi.y, t1.z, t1.w);
+            take(sc.z, lo.z, hi.z, t2.z, t2.w);
+            take(sc.w, lo.w, hi.w, t3.z, t3.w);
+        }
+    } else {
+        for (uint32_t k = 0; k < a.per_region; ++k) {
+            const uint64_t h = h0 + k;
+            take(a.xscore[h], a.xlo[h], a.xhi[h], a.hits[4 * h + 2], a.hits[4 * h + 3]);
+        }
+    }
+
+    const int64_t mm = (int64_t)m, nn = (int64_t)nwin;
+    const int64_t blo = J.band_lo > -(mm - 1) ? J.band_lo : -(mm - 1);
+    const int64_t bhi = J.band_hi < nn - 1 ? J.band_hi : nn - 1;
+    const float width = bhi >= blo ? (float)(bhi - blo + 1) : 0.f;
+    const float fb = (float)best;
+    const float pr = a.per_region ? (float)a.per_region : 1.f;
+    float f[64];
+    f[0] = fb / 64.f;
+    f[1] = logf(1.f + fmaxf(fb, 0.f));
+    f[2] = (float)hsp / pr;
+    f[3] = (float)hsum / 128.f;
+    f[4] = hsp >= 2 ? (float)(dmax - dmin) / 64.f : 0.f;
+    f[5] = width / 128.f;
+    f[6] = logf(1.f + (float)nwin);
+    f[7] = logf(1.f + (float)m);
+    f[8] = fb / logf(2.f + (float)m * (float)nwin);
+    f[9] = a.per_region ? ssum / pr / 64.f : 0.f;
+    f[10] = (float)ext / 64.f;
+    f[11] = fb / (1.f + (float)ext);
+    const float4* qc = reinterpret_cast<const float4*>(a.qcomp + (uint64_t)J.query * 20);   // 80 B rows
+#pragma unroll
+    for (int x = 0; x < 5; ++x) {
+        const float4 v = qc[x];
+        f[12 + 4 * x] = v.x;
+        f[13 + 4 * x] = v.y;
+        f[14 + 4 * x] = v.z;
+        f[15 + 4 * x] = v.w;
+    }
+#pragma unroll
+    for (int c = 0; c < 20; ++c) f[32 + c] = nwin ? (float)H[c] / (float)nwin : 0.f;
+#pragma unroll
+    for (int c = 52; c < 64; ++c) f[c] = 0.f;
+    if (a.mean) {
+#pragma unroll
+        for (int c = 0; c < 64; ++c) f[c] = (f[c] - a.mean[c]) * a.inv_std[c];
+    }
+    uint4* out = reinterpret_cast<uint4*>(a.out + r * 64);
+#pragma unroll
+    for (int x = 0; x < 8; ++x) {
+        uint32_t w4[4];
+#pragma unroll
+        for (int y = 0; y < 4; ++y) {
+            const __bf16 lo = (__bf16)f[8 * x + 2 * y], hi = (__bf16)f[8 * x + 2 * y + 1];
+            uint16_t bl, bh;
+            __builtin_memcpy(&bl, &lo, 2);
+            __builtin_memcpy(&bh, &hi, 2);
+            w4[y] = (uint32_t)bl | ((uint32_t)bh << 16);
+        }
+        out[x] = make_uint4(w4[0], w4[1], w4[2], w4[3]);
+    }
+}
+
+hipError_t launch_region_features(const FeatureArgs& a, uint32_t nq, hipStream_t s) {
     if (a.n == 0) return hipSuccess;
-    hipLaunchKernelGGL(region_features_kernel, dim3(grid), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(query_composition_kernel, dim3((uint32_t)std::min<uint64_t>((nq + 3) / 4, 4096)), dim3(256),
+                       0, s, a.qoff, a.qres, nq, a.qcomp);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(region_features_kernel, dim3((uint32_t)((a.n + 255) / 256)), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

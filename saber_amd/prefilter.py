@@ -133,11 +133,12 @@ class Mlp:
 
 
 def region_features(ctx, qres, qoff, sres, soff, jobs, hits, xscore, xq_lo, xq_hi, hits_per_region,
-                    hsp_threshold=40, mean=None, std=None):
+                    hsp_threshold=40, mean=None, std=None, time_iters=0):
     """bf16 feature vectors [n, 64] (uint16 bit patterns) of the regions
     `jobs` (band-job structured array) with the X-drop results of their hits
     (hits: [n * hits_per_region, 4] u32 query, subject, qp, sp); with mean and
-    std, each feature is standardised on the device before rounding."""
+    std, each feature is standardised on the device before rounding. With
+    time_iters > 0, returns (features, mean ms of the feature kernel)."""
     mu = inv = None
     if mean is not None:
         mu = np.ascontiguousarray(mean, np.float32)
@@ -156,11 +157,15 @@ def region_features(ctx, qres, qoff, sres, soff, jobs, hits, xscore, xq_lo, xq_h
     hi = np.ascontiguousarray(xq_hi, np.uint32)
     n = len(jobs)
     out = np.zeros((max(n, 1), N_IN), np.uint16)
-    _ffi.check(_ffi.lib().saber_hip_region_features(
-        ctx.h, C.byref(qs), C.byref(ss), jobs.ctypes.data_as(C.POINTER(_ffi.BandJob)), n,
-        _ffi.ptr(h, C.c_uint32), _ffi.ptr(xs, C.c_int32), _ffi.ptr(lo, C.c_uint32), _ffi.ptr(hi, C.c_uint32),
-        int(hits_per_region), int(hsp_threshold), _ffi.ptr(mu, C.c_float), _ffi.ptr(inv, C.c_float),
-        _ffi.ptr(out, C.c_uint16)))
+    args = (ctx.h, C.byref(qs), C.byref(ss), jobs.ctypes.data_as(C.POINTER(_ffi.BandJob)), n,
+            _ffi.ptr(h, C.c_uint32), _ffi.ptr(xs, C.c_int32), _ffi.ptr(lo, C.c_uint32), _ffi.ptr(hi, C.c_uint32),
+            int(hits_per_region), int(hsp_threshold), _ffi.ptr(mu, C.c_float), _ffi.ptr(inv, C.c_float),
+            _ffi.ptr(out, C.c_uint16))
+    if time_iters > 0:
+        ms = C.c_double()
+        _ffi.check(_ffi.lib().saber_hip_region_features_time(*args, int(time_iters), C.byref(ms)))
+        return out[:n], ms.value
+    _ffi.check(_ffi.lib().saber_hip_region_features(*args))
     return out[:n]
 
 
