@@ -72,7 +72,7 @@ def prefilter_leg(ctx, args, sch):
     score reaches E <= 10 (per-subject model, BLOSUM62 11/1, a 147 M-residue
     / 331 k-sequence database). Features are computed on the device from the
     X-drop results, standardised with training-set statistics. The model is
-    trained on the train set; its cut is set for 99.5 % recall on a
+    trained on the train set; its cut is set for --pf-recall (99.5 %) recall on a
     validation set; the test set gives the hit loss, the kept fraction and the
     time of the scoring stage with and without the filter (with it: feature
     kernel + MLP forward + band scoring of the kept regions, device times;
@@ -111,7 +111,7 @@ def prefilter_leg(ctx, args, sch):
     prep_s = time.time() - t0
     m = P.Mlp(ctx, seed=args.seed)
     m.upload(Xtr, ytr)
-    bs, epochs, lr = 65536, 12, 2e-3
+    bs, epochs, lr = 65536, args.pf_epochs, 2e-3
     t0 = time.time()
     for ep in range(epochs):
         for r0 in range(0, len(ytr), bs):
@@ -119,7 +119,7 @@ def prefilter_leg(ctx, args, sch):
     loss = m.train_resident(0, min(bs, len(ytr)), lr=0.0, pos_weight=4.0)
     train_s = time.time() - t0
     lv = m.forward(Xva)
-    tau = float(np.quantile(lv[yva > 0.5], 1 - 0.995))
+    tau = float(np.quantile(lv[yva > 0.5], 1 - args.pf_recall))
     lt = m.forward(Xte)
     keep = lt >= tau
     hit_loss = float(1 - keep[yte > 0.5].mean())
@@ -138,7 +138,7 @@ def prefilter_leg(ctx, args, sch):
     m.close()
     flop = 164352 * len(yte)
     out = {"train_regions": int(len(ytr)), "test_regions": int(len(yte)), "hits_per_region": per,
-           "positives": round(float(yte.mean()), 4), "recall_target": 0.995, "hit_loss": round(hit_loss, 5),
+           "positives": round(float(yte.mean()), 4), "recall_target": args.pf_recall, "hit_loss": round(hit_loss, 5),
            "kept_fraction": round(float(keep.mean()), 4), "band_ms_all": round(t_all, 4),
            "band_ms_kept": round(t_keep, 4), "mlp_forward_ms": round(f_ms, 4),
            "mlp_forward_tflops": round(flop / (f_ms * 1e-3) / 1e12, 1),
@@ -174,6 +174,9 @@ def main():
     ap.add_argument("--pf-train", type=int, default=4_000_000,
                     help="training regions of the prefilter leg (0: skip); the test set has --pf-test")
     ap.add_argument("--pf-test", type=int, default=4_000_000)
+    ap.add_argument("--pf-recall", type=float, default=0.995,
+                    help="recall of the prefilter's cut on the validation set")
+    ap.add_argument("--pf-epochs", type=int, default=12)
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only time a plain device copy of this many bytes and exit")
     args = ap.parse_args()
