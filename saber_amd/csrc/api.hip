@@ -1702,6 +1702,7 @@ struct saber_hip_mlp {
     DevBuf<float> data_y;
     uint64_t data_n = 0;
     int max_grid = 1;
+    int cus = 1;
 };
 
 namespace {
@@ -1804,6 +1805,7 @@ int saber_hip_mlp_create(saber_hip_ctx* c, const float* params, uint64_t seed, s
     int cus = 0;
     HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
     m->max_grid = std::max(1, cus * 8);
+    m->cus = std::max(1, cus);
     std::vector<float> init;
     if (!params) {
         // He-uniform weights, zero biases (splitmix64 stream from seed)
@@ -1852,7 +1854,7 @@ int saber_hip_mlp_forward(saber_hip_mlp* m, const uint16_t* x, uint64_t n, float
     hipStream_t st = m->ctx->stream;
     if (int r = mlp_reserve(m, n)) return r;
     HIP_TRY(hipMemcpyAsync(m->X.p, x, 2 * 64 * n, hipMemcpyHostToDevice, st));
-    HIP_TRY(mlp_launch_forward(mlp_weights(m), m->X.p, n, m->y.p, mlp_grid(m, n), st));
+    HIP_TRY(mlp_launch_forward(mlp_weights(m), m->X.p, n, m->y.p, m->cus, st));
     HIP_TRY(hipMemcpyAsync(logits, m->y.p, sizeof(float) * n, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     return SABER_HIP_OK;
@@ -1920,7 +1922,7 @@ int saber_hip_mlp_forward_resident(saber_hip_mlp* m, uint64_t row0, uint64_t n, 
     HIP_TRY(hipSetDevice(m->ctx->device));
     hipStream_t st = m->ctx->stream;
     if (int r = mlp_reserve(m, n)) return r;
-    HIP_TRY(mlp_launch_forward(mlp_weights(m), m->data_x.p + 64 * row0, n, m->y.p, mlp_grid(m, n), st));
+    HIP_TRY(mlp_launch_forward(mlp_weights(m), m->data_x.p + 64 * row0, n, m->y.p, m->cus, st));
     HIP_TRY(hipMemcpyAsync(logits, m->y.p, sizeof(float) * n, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     return SABER_HIP_OK;
@@ -1942,9 +1944,9 @@ int saber_hip_mlp_time(saber_hip_mlp* m, uint64_t n, int warmup, int iters, doub
     hipEvent_t ev[4];
     for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
     const MlpWeights w = mlp_weights(m);
-    for (int i = 0; i < warmup; ++i) HIP_TRY(mlp_launch_forward(w, m->data_x.p, n, m->y.p, mlp_grid(m, n), st));
+    for (int i = 0; i < warmup; ++i) HIP_TRY(mlp_launch_forward(w, m->data_x.p, n, m->y.p, m->cus, st));
     HIP_TRY(hipEventRecord(ev[0], st));
-    for (int i = 0; i < iters; ++i) HIP_TRY(mlp_launch_forward(w, m->data_x.p, n, m->y.p, mlp_grid(m, n), st));
+    for (int i = 0; i < iters; ++i) HIP_TRY(mlp_launch_forward(w, m->data_x.p, n, m->y.p, m->cus, st));
     HIP_TRY(hipEventRecord(ev[1], st));
     for (int i = 0; i < iters + warmup; ++i) {
         if (i == warmup) HIP_TRY(hipEventRecord(ev[2], st));

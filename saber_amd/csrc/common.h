@@ -199,7 +199,7 @@ struct MlpGrads {
 };
 
 hipError_t mlp_launch_forward(const MlpWeights& w, const __bf16* X, uint64_t n, float* logits,
-                              int grid, hipStream_t s);
+                              int cus, hipStream_t s);
 hipError_t mlp_launch_backward_act(const MlpWeights& w, const __bf16* X, const float* labels,
                                    uint64_t n, float pos_weight, float inv_n, const MlpGrads& gr,
                                    __bf16* H1, __bf16* H2, __bf16* dH2, __bf16* dH1, float* dy,

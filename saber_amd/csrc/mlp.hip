@@ -120,92 +120,206 @@ __device__ __forceinline__ float forward_tile(const MlpWeights& w, const __bf16*
     return y + *w.b3;
 }
 
-// Inference: weights stay in registers for the whole launch, split across
-// the four waves of a block: wave v owns hidden units [64v, 64v + 64) of both
-// layers (W1T rows: 8 fragments, W2P rows: 32 fragments). Per 16-row tile,
-// every wave computes its quarter of layer 1 from the tile's features and
-// writes it (bf16, after ReLU) to LDS; after a barrier each wave reads the
-// whole layer-1 tile back as layer-2 operands (in the permuted k order W2P
-// expects), computes its quarter of layer 2 and its part of the output dot
-// product, and wave 0 adds the four parts. LDS is double-buffered by tile.
-constexpr int kHs = kHid + 8;  // LDS row stride (bf16) of the layer-1 tile
+// Inference: both weight matrices live in LDS for the whole launch (W1T 32 KB
+// + W2P 128 KB, the 160 KB a CU holds), staged once per block, one block per
+// CU. Every wave then runs whole row groups on its own -- no barrier after the
+// staging: R 16-row tiles per group, layer 1 from W1T fragments, its ReLU'd
+// bf16 results kept in registers as layer 2's B operands (the permuted k
+// order of W2P), layer 2 in chunks of OC output tiles with the ReLU . w3
+// epilogue. Each A fragment read from LDS feeds R MFMAs. Biases initialise
+// the accumulators (MFMA C operand).
+//
+// LDS layout, chunk-column major: 16-byte chunk c (k values 8c .. 8c+7) of
+// weight row `row` sits at (c * 256 + row) * 16, W2P after W1T. The 16 lanes
+// of a lane group read rows 16*tile .. +15 of one chunk column -- 256
+// contiguous bytes, free of bank conflicts -- and every fragment address is
+// the lane's base plus a compile-time offset.
+constexpr uint32_t kW1Bytes = kHid * kIn * 2, kW2Bytes = kHid * kHid * 2;
+constexpr uint32_t kChunkCol = 256 * 16;   // bytes of one chunk column
 
-__global__ __launch_bounds__(256) void mlp_forward_kernel(MlpWeights w, const __bf16* X, uint64_t n,
-                                                          float* logits) {
-    __shared__ __bf16 Hs[2][16 * kHs];
-    __shared__ float part[2][4][16];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+__device__ __forceinline__ bf16x8 lds8(const char* base, uint32_t off) {
+    return *reinterpret_cast<const bf16x8*>(base + off);
+}
+
+// Two fp32 values -> ReLU'd bf16 pair in one dword (low half = a). ReLU is
+// taken on the bf16 bit patterns as signed 16-bit integers: negative values
+// (sign bit set) become +0, the rest are kept -- the same as rounding
+// max(x, 0).
+__device__ __forceinline__ uint32_t relu_pack(float a, float b) {
+    typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+    typedef short s16x2 __attribute__((ext_vector_type(2)));
+    typedef float f32x2 __attribute__((ext_vector_type(2)));
+    const bf16x2 h = __builtin_convertvector((f32x2{a, b}), bf16x2);
+    s16x2 v;
+    __builtin_memcpy(&v, &h, 4);
+    const s16x2 z = {0, 0};
+    v = v > z ? v : z;
+    uint32_t out;
+    __builtin_memcpy(&out, &v, 4);
+    return out;
+}
+
+// max(x, 0) as a signed integer max on the bit pattern (exact for every
+// non-NaN x; one instruction, no canonicalisation)
+__device__ __forceinline__ float relu(float x) {
+    return __int_as_float(max(__float_as_int(x), 0));
+}
+
+template <int R, int OC, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void mlp_forward_kernel(MlpWeights w, const __bf16* X, uint64_t n,
+                                                                 float* logits) {
+    __shared__ __attribute__((aligned(16))) char lds[kW1Bytes + kW2Bytes];
+    // staging: consecutive threads fill consecutive LDS chunks (one chunk
+    // column, consecutive rows)
+    for (uint32_t i = threadIdx.x; i < (kW1Bytes + kW2Bytes) / 16; i += blockDim.x) {
+        const uint32_t c = i >> 8, row = i & 255;
+        const uint4 v = c < 8 ? reinterpret_cast<const uint4*>(w.W1T)[row * 8 + c]
+                              : reinterpret_cast<const uint4*>(w.W2P)[row * 32 + (c - 8)];
+        *reinterpret_cast<uint4*>(lds + 16 * i) = v;
+    }
+    __syncthreads();
+
+    const int lane = threadIdx.x & 63;
     const int g = lane >> 4, r = lane & 15;
-    bf16x8 a1[4][2], a2[4][8];
-#pragma unroll
-    for (int ht = 0; ht < 4; ++ht)
-#pragma unroll
-        for (int ks = 0; ks < 2; ++ks)
-            a1[ht][ks] = load8(w.W1T + (uint64_t)(64 * wv + 16 * ht + r) * kIn + 32 * ks + 8 * g);
-#pragma unroll
-    for (int ot = 0; ot < 4; ++ot)
-#pragma unroll
-        for (int s = 0; s < 8; ++s)
-            a2[ot][s] = load8(w.W2P + (uint64_t)(64 * wv + 16 * ot + r) * kHid + 32 * s + 8 * g);
-    float b1v[4][4], b2v[4][4], w3v[4][4];
-#pragma unroll
-    for (int t4 = 0; t4 < 4; ++t4)
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            const int u = 64 * wv + 16 * t4 + 4 * g + t;
-            b1v[t4][t] = w.b1[u];
-            b2v[t4][t] = w.b2[u];
-            w3v[t4][t] = w.w3[u];
-        }
+    const uint64_t nwaves = (uint64_t)gridDim.x * WAVES;
+    const uint64_t groups = (n + 16 * R - 1) / (16 * R);
     const float b3 = *w.b3;
-    const uint64_t ntiles = (n + 15) / 16;
-    int buf = 0;
-    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x, buf ^= 1) {
-        const uint64_t row = tile * 16 + r;
-        bf16x8 xb[2];
+    // Biases held across the wave: lane (g, r) holds units 16r + 4g .. +3 of
+    // b1, b2 and w3; the accumulators of a tile take them from lane (g, tile)
+    // by ds_bpermute (an LDS-pipe shuffle, no memory access), which keeps the
+    // vector-memory counter for the input rows alone.
+    const f32x4 vb1 = *reinterpret_cast<const f32x4*>(w.b1 + 16 * r + 4 * g);
+    const f32x4 vb2 = *reinterpret_cast<const f32x4*>(w.b2 + 16 * r + 4 * g);
+    const f32x4 vw3 = *reinterpret_cast<const f32x4*>(w.w3 + 16 * r + 4 * g);
+    // input rows of a group (rows past n read row n - 1; their logits are not
+    // stored), loaded one group ahead
+    auto load_x = [&](uint64_t grp, bf16x8 (&xb)[R][2]) {
 #pragma unroll
-        for (int ks = 0; ks < 2; ++ks) xb[ks] = row < n ? load8(X + row * kIn + 32 * ks + 8 * g) : bf16x8{};
-        // layer 1, this wave's 64 units -> LDS
+        for (int t = 0; t < R; ++t) {
+            const uint64_t row = min(grp * 16 * R + 16 * t + r, n - 1);
 #pragma unroll
-        for (int ht = 0; ht < 4; ++ht) {
-            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-            acc = mfma(a1[ht][0], xb[0], acc);
-            acc = mfma(a1[ht][1], xb[1], acc);
-            __bf16 hv[4];
-#pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                const float v = acc[t] + b1v[ht][t];
-                hv[t] = (__bf16)(v > 0.f ? v : 0.f);
-            }
-            __builtin_memcpy(&Hs[buf][r * kHs + 64 * wv + 16 * ht + 4 * g], hv, 8);
+            for (int ks = 0; ks < 2; ++ks) xb[t][ks] = load8(X + row * kIn + 32 * ks + 8 * g);
         }
-        __syncthreads();
-        // layer 2, this wave's 64 units, operands from the whole layer-1 tile
-        f32x4 acc2[4];
+    };
+    uint64_t grp = (uint64_t)blockIdx.x * WAVES + (threadIdx.x >> 6);
+    bf16x8 xn[R][2];
+    load_x(grp, xn);
+    for (; grp < groups; grp += nwaves) {
+        const uint64_t row0 = grp * 16 * R;
+        bf16x8 xb[R][2];
 #pragma unroll
-        for (int ot = 0; ot < 4; ++ot) acc2[ot] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int s = 0; s < 8; ++s) {
-            bf16x8 hb;
-            __builtin_memcpy(&hb, &Hs[buf][r * kHs + 32 * s + 4 * g], 8);
-            __builtin_memcpy(reinterpret_cast<char*>(&hb) + 8, &Hs[buf][r * kHs + 32 * s + 16 + 4 * g], 8);
-#pragma unroll
-            for (int ot = 0; ot < 4; ++ot) acc2[ot] = mfma(a2[ot][s], hb, acc2[ot]);
+        for (int t = 0; t < R; ++t) {
+            xb[t][0] = xn[t][0];
+            xb[t][1] = xn[t][1];
         }
-        float y = 0.f;
+        load_x(grp + nwaves, xn);
+        // Loop-invariant addresses are made opaque inside the loop, so that
+        // each fragment or shuffle address is a base plus an offset that fits
+        // the instruction's immediate field (rather than one register per
+        // address, hoisted out of the loop): the lane's LDS base (chunk
+        // column g, row r) and the shuffle base (lane 16g).
+        uint32_t lb = (uint32_t)g * kChunkCol + (uint32_t)r * 16u, sh = 64u * g;
+        asm volatile("" : "+v"(lb), "+v"(sh));
+        auto bcast = [&](const f32x4& v, int tile) {
+            f32x4 o;
 #pragma unroll
-        for (int ot = 0; ot < 4; ++ot)
+            for (int e = 0; e < 4; ++e)
+                o[e] = __int_as_float(__builtin_amdgcn_ds_bpermute((int)(sh + 4 * tile), __float_as_int(v[e])));
+            return o;
+        };
+        const char* L1 = lds + lb;
+        // layer 1: tiles 2s, 2s+1 form layer 2's operand for k step s; the
+        // next tile's fragments are read while this tile's MFMAs run
+        uint32_t hw[R][8][4];   // bf16 pairs: k step s, dwords 0-1 tile 2s, 2-3 tile 2s+1
+        bf16x8 a0 = lds8(L1, 0), a1 = lds8(L1, 4 * kChunkCol);
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                const float v = acc2[ot][t] + b2v[ot][t];
-                y += (v > 0.f ? v : 0.f) * w3v[ot][t];
+        for (int ht = 0; ht < 16; ++ht) {
+            bf16x8 n0, n1;
+            if (ht < 15) {
+                n0 = lds8(L1, 256 * (ht + 1));
+                n1 = lds8(L1, 4 * kChunkCol + 256 * (ht + 1));
             }
-        y += __shfl_xor(y, 16);
-        y += __shfl_xor(y, 32);
-        if (lane < 16) part[buf][wv][lane] = y;
-        __syncthreads();
-        if (wv == 0 && lane < 16 && row < n)
-            logits[row] = part[buf][0][lane] + part[buf][1][lane] + part[buf][2][lane] + part[buf][3][lane] + b3;
+            const f32x4 bias = bcast(vb1, ht);
+#pragma unroll
+            for (int t = 0; t < R; ++t) {
+                const f32x4 acc = mfma(a1, xb[t][1], mfma(a0, xb[t][0], bias));
+                hw[t][ht >> 1][2 * (ht & 1)] = relu_pack(acc[0], acc[1]);
+                hw[t][ht >> 1][2 * (ht & 1) + 1] = relu_pack(acc[2], acc[3]);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+            if (ht < 15) {
+                a0 = n0;
+                a1 = n1;
+            }
+        }
+        // layer 2 in chunks of OC output tiles, then ReLU . w3: step j covers
+        // output tiles o0 .. o0 + OC - 1 (o0 = OC * (j / 8)) at k step s = j % 8,
+        // which reads chunk column 4s + g of W2P; the next step's fragments
+        // are read while this step's MFMAs run
+        constexpr int kSteps = 8 * (16 / OC);
+        auto frag2 = [&](int j, int oc) {
+            const int o0 = OC * (j / 8), s = j % 8;
+            return lds8(L1, kW1Bytes + (uint32_t)s * 4 * kChunkCol + 256 * (o0 + oc));
+        };
+        float y[R];
+#pragma unroll
+        for (int t = 0; t < R; ++t) y[t] = 0.f;
+        f32x4 acc[R][OC];
+        bf16x8 a[OC];
+#pragma unroll
+        for (int oc = 0; oc < OC; ++oc) a[oc] = frag2(0, oc);
+#pragma unroll
+        for (int j = 0; j < kSteps; ++j) {
+            const int o0 = OC * (j / 8), s = j % 8;
+            bf16x8 nx[OC];
+            if (j + 1 < kSteps) {
+#pragma unroll
+                for (int oc = 0; oc < OC; ++oc) nx[oc] = frag2(j + 1, oc);
+            }
+            if (s == 0) {
+#pragma unroll
+                for (int oc = 0; oc < OC; ++oc) {
+                    const f32x4 bias = bcast(vb2, o0 + oc);
+#pragma unroll
+                    for (int t = 0; t < R; ++t) acc[t][oc] = bias;
+                }
+            }
+#pragma unroll
+            for (int t = 0; t < R; ++t) {
+                bf16x8 hb;
+                __builtin_memcpy(&hb, hw[t][s], 16);
+#pragma unroll
+                for (int oc = 0; oc < OC; ++oc) acc[t][oc] = mfma(a[oc], hb, acc[t][oc]);
+            }
+            if (s == 7) {
+#pragma unroll
+                for (int oc = 0; oc < OC; ++oc) {
+                    const f32x4 w3 = bcast(vw3, o0 + oc);
+#pragma unroll
+                    for (int t = 0; t < R; ++t)
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) y[t] += relu(acc[t][oc][e]) * w3[e];
+                }
+                // keep the sums here (not sunk to their use after the loop,
+                // which would hold every accumulator until then)
+#pragma unroll
+                for (int t = 0; t < R; ++t) asm volatile("" : "+v"(y[t]));
+            }
+            __builtin_amdgcn_sched_barrier(0);
+            if (j + 1 < kSteps) {
+#pragma unroll
+                for (int oc = 0; oc < OC; ++oc) a[oc] = nx[oc];
+            }
+        }
+        // rows are lane % 16: sum the four lane groups
+#pragma unroll
+        for (int t = 0; t < R; ++t) {
+            float v = y[t];
+            v += __shfl_xor(v, 16);
+            v += __shfl_xor(v, 32);
+            const uint64_t row = row0 + 16 * t + r;
+            if (g == 0 && row < n) logits[row] = v + b3;
+        }
     }
 }
 
@@ -401,13 +515,19 @@ __global__ void f32_to_bf16_kernel(const float* src, __bf16* dst, uint64_t n) {
 
 // ---- launchers ----
 
+// Forward shape: 4 row tiles per weight read (64 rows per group), output
+// tiles in pairs, 8 waves per block (2 per SIMD at 244 VGPRs).
+constexpr int kFwdR = 4, kFwdOC = 2, kFwdWaves = 8;
+
 hipError_t mlp_launch_forward(const MlpWeights& w, const __bf16* X, uint64_t n, float* logits,
-                              int grid, hipStream_t s) {
+                              int cus, hipStream_t s) {
     if (n == 0) return hipSuccess;
-    // one 16-row tile per block per step; grid = blocks resident at once
-    const uint64_t tiles = (n + 15) / 16;
-    const int g = (int)std::min<uint64_t>(tiles, (uint64_t)grid);
-    hipLaunchKernelGGL(mlp_forward_kernel, dim3(g), dim3(256), 0, s, w, X, n, logits);
+    // one block per CU (the weights fill its LDS), no more blocks than groups
+    const uint64_t rows_per_block = 16 * kFwdR * kFwdWaves;
+    const uint64_t blocks = (n + rows_per_block - 1) / rows_per_block;
+    const int g = (int)std::max<uint64_t>(1, std::min<uint64_t>(blocks, (uint64_t)cus));
+    hipLaunchKernelGGL((mlp_forward_kernel<kFwdR, kFwdOC, kFwdWaves>), dim3(g), dim3(64 * kFwdWaves), 0, s, w, X,
+                       n, logits);
     return hipGetLastError();
 }
 

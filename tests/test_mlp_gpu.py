@@ -34,7 +34,10 @@ def test_forward_matches_reference(ctx):
     p = rand_params(1)
     m = P.Mlp(ctx, p)
     rng = np.random.default_rng(2)
-    for n in (1, 15, 16, 17, 1000, 4099):
+    # 64-row groups per wave, 8 waves per block, one block per CU: the sizes
+    # cover partial tiles and groups, and (300007) waves that loop over
+    # several groups with the next group's rows loaded ahead
+    for n in (1, 15, 16, 17, 63, 64, 65, 1000, 4099, 300007):
         x = rng.normal(0, 1, (n, 64)).astype(np.float32)
         got = m.forward(x)
         want, _ = mlp_ref.forward(p, x)
