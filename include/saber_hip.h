@@ -52,7 +52,7 @@ extern "C" {
 
 /* Context options. */
 #define SABER_HIP_FORCE_EXACT32 1u /* score every band job with the int32 kernel (testing) */
-#define SABER_HIP_NO_GRAPH 2u      /* enqueue band passes directly instead of replaying a captured graph */
+#define SABER_HIP_NO_GRAPH 2u      /* enqueue band and full passes directly instead of replaying a captured graph */
 typedef struct saber_hip_opts {
     uint32_t flags;
     uint32_t reserved[7];
@@ -126,9 +126,10 @@ int saber_hip_score_bands(saber_hip_ctx* ctx, const saber_hip_seqset* queries,
  * may be repeated); wait blocks until it is done; read copies the results.
  * A pass runs entirely on the device (output clear, the persistent packed
  * kernel, the int32 kernel over wide jobs and over the packed kernel's
- * saturated jobs, whose count it reads on the device); the first run captures
- * it as a HIP graph, later runs replay the graph (SABER_HIP_NO_GRAPH in the
- * context flags enqueues it directly instead).
+ * saturated jobs, whose count it reads on the device); the first run is
+ * enqueued directly, the second captures the pass as a HIP graph and later
+ * runs replay it (SABER_HIP_NO_GRAPH in the context flags enqueues every run
+ * directly).
  * ends = 0 computes scores only; reading end cells from such a batch is
  * SABER_HIP_ESTATE. */
 int saber_hip_band_batch_create(saber_hip_ctx* ctx, const saber_hip_seqset* queries,
@@ -161,7 +162,13 @@ int saber_hip_band_batch_time(saber_hip_band_batch* b, int warmup, int iters, do
  * subjects == NULL selects the uploaded database. q_end / s_end may be NULL:
  * when both are, only scores are computed (score_many's output, and faster).
  * A batch computes end cells when created with ends != 0; reading ends from
- * a scores-only batch is SABER_HIP_ESTATE. */
+ * a scores-only batch is SABER_HIP_ESTATE.
+ * A full batch's pass runs on the device like a band batch's: one persistent
+ * kernel per register class of the subject-length shapes drains a chunk list
+ * of (shape, query) segments, longest queries first; pairs with subjects
+ * longer than the widest shape, and pairs whose packed score saturates, are
+ * rescored by the int32 kernel, their count read on the device. Runs after
+ * the first replay a captured HIP graph (see the band batch). */
 typedef struct saber_hip_full_batch saber_hip_full_batch;
 int saber_hip_score_full(saber_hip_ctx* ctx, const saber_hip_seqset* queries,
                          const saber_hip_seqset* subjects, const saber_hip_scheme* scheme,

@@ -16,6 +16,8 @@
 #include <memory>
 #include <cstdio>
 #include <cstring>
+#include <functional>
+#include <map>
 #include <mutex>
 #include <numeric>
 #include <string>
@@ -119,6 +121,9 @@ struct saber_hip_ctx {
     DevBuf<uint8_t> db;
     std::vector<uint64_t> db_off;
     int cus = 0;                  // compute units of the device
+    // tuned full-matrix shapes: default shape id -> chosen shape id, by
+    // flavour (0 scores only, 1 end cells)
+    std::map<int, int> full_tuned[2];
     std::vector<int> choice;      // width -> shape index (-1: exact int32 kernel)
     std::vector<int> choice_lat;  // the same for jobs with many rows
     std::vector<Shape> shapes;
@@ -168,6 +173,7 @@ struct saber_hip_band_batch {
     // the whole step captured once and replayed
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
+    uint64_t passes = 0;                   // passes enqueued so far
     DevBuf<uint32_t> flagged;
     DevBuf<uint32_t> nflagged;
     DevBuf<int32_t> score;
@@ -866,11 +872,13 @@ static int enqueue_step(saber_hip_band_batch* b, hipEvent_t ev_begin, hipEvent_t
     return SABER_HIP_OK;
 }
 
-// Enqueue one pass: the captured graph (built on first use) unless graphs
-// are switched off for the context.
+// Enqueue one pass: the first pass directly (a batch run once, as by
+// saber_hip_score_bands, never pays for a capture), later passes from the
+// graph captured on the second, unless graphs are switched off for the
+// context.
 static int enqueue_run(saber_hip_band_batch* b) {
     hipStream_t st = b->ctx->stream;
-    if (b->ctx->flags & SABER_HIP_NO_GRAPH) return enqueue_step(b, nullptr, nullptr);
+    if ((b->ctx->flags & SABER_HIP_NO_GRAPH) || b->passes++ == 0) return enqueue_step(b, nullptr, nullptr);
     if (!b->graph_exec) {
         HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         const int r = enqueue_step(b, nullptr, nullptr);
@@ -1255,12 +1263,13 @@ int saber_hip_xdrop(saber_hip_ctx* c, const saber_hip_seqset* queries,
 // Exhaustive full-matrix scoring: every query against every subject.
 // ---------------------------------------------------------------------------
 
-struct FullLaunch {
-    int variant;
-    uint64_t pair_off;
-    uint32_t npairs;
-    double cost;
-    int stream;
+// One register class of the full-matrix kernels: its persistent kernel's
+// segments (in the batch's segment buffer) and chunk count.
+struct FullClass {
+    int nt = 0;
+    uint32_t seg_off = 0, nsegs = 0;
+    uint64_t nchunks = 0;
+    uint32_t grid = 0;
 };
 
 struct saber_hip_full_batch {
@@ -1271,160 +1280,147 @@ struct saber_hip_full_batch {
     DevBuf<uint8_t> q, s;
     DevBuf<int8_t> bias;
     DevBuf<uint64_t> qoff, soff;
-    DevBuf<uint32_t> queries, pairs;
+    DevBuf<uint32_t> qorder, pairs;
     DevBuf<uint32_t> pair_table;
     DevBuf<int32_t> mat;
     bool ends = true;                      // end cells computed (else scores only)
-    std::vector<FullLaunch> launches;
-    std::vector<uint32_t> long_subjects;   // longer than the widest variant
+    std::vector<FullClass> classes;
+    std::vector<FullSeg> host_segs;
+    DevBuf<FullSeg> segs;
+    DevBuf<uint32_t> next;                 // claim counters, one per class (64 B apart)
     std::vector<uint64_t> hq, hs;          // host offsets
     uint64_t qlen_total = 0, slen_total = 0;
     const uint8_t* sres = nullptr;
     DevBuf<int32_t> score;
     DevBuf<uint32_t> q_end, s_end;
     DevBuf<uint32_t> flagged, nflagged;
-    // exact int32 rescue (long subjects, saturated pairs)
-    DevBuf<BandJobDev> gjobs;
-    DevBuf<uint32_t> gids;
-    DevBuf<uint64_t> goff, gdst;
-    DevBuf<int32_t> gwork, gscore;
-    DevBuf<uint32_t> gqe, gse;
-    DevBuf<uint64_t> gcells;
+    // exact int32 path: pairs with subjects longer than the widest shape (a
+    // static list) and pairs the packed kernel flagged, one work slot per thread
+    DevBuf<uint32_t> static_pairs, static_count;
+    uint64_t nstatic = 0;
+    DevBuf<int32_t> gwork;
+    uint32_t gthreads = 0, gslot = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    uint64_t passes = 0;                   // passes enqueued so far
     bool ran = false;
-    uint64_t rescued = 0;
+    uint64_t last_flagged = 0;
 };
 
 namespace {
 
-__global__ void scatter_full_kernel(const uint64_t* dst, uint32_t n, const int32_t* sc,
-                                    const uint32_t* qe, const uint32_t* se, int32_t* score,
-                                    uint32_t* q_end, uint32_t* s_end) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint64_t o = dst[i];
-    score[o] = sc[i];
-    q_end[o] = qe[i];
-    s_end[o] = se[i];
-}
+constexpr uint32_t kCounterStride = 16;   // u32 between class counters
 
-// Exact int32 scores of (query slot, subject) pairs, written into the outputs.
-int full_rescue(saber_hip_full_batch* b, const std::vector<std::pair<uint32_t, uint32_t>>& prs) {
-    if (prs.empty()) return SABER_HIP_OK;
-    hipStream_t st = b->ctx->stream;
-    const size_t n = prs.size();
-    std::vector<BandJobDev> jobs(n);
-    std::vector<uint32_t> ids(n);
-    std::vector<uint64_t> off(n), dst(n);
-    uint64_t tot = 0;
-    for (size_t k = 0; k < n; ++k) {
-        const uint32_t qi = prs[k].first, si = prs[k].second;
-        const int64_t m = (int64_t)(b->hq[qi + 1] - b->hq[qi]), len = (int64_t)(b->hs[si + 1] - b->hs[si]);
-        BandJobDev d{};
-        d.q_off = b->hq[qi];
-        d.s_off = b->hs[si];
-        d.m = (uint32_t)m;
-        d.n = (uint32_t)len;
-        d.lo = (int32_t)(m > 0 ? -(m - 1) : 0);
-        d.w = (int32_t)(m > 0 && len > 0 ? m + len - 1 : 0);
-        d.i_first = 0;
-        d.rows = (int32_t)(d.w > 0 ? m : 0);
-        jobs[k] = d;
-        ids[k] = (uint32_t)k;
-        off[k] = tot;
-        tot += 4 * ((uint64_t)d.w + 1);
-        dst[k] = (uint64_t)qi * b->ns + si;
-    }
-    if (int r = upload(b->gjobs, jobs.data(), n, st)) return r;
-    if (int r = upload(b->gids, ids.data(), n, st)) return r;
-    if (int r = upload(b->goff, off.data(), n, st)) return r;
-    if (int r = upload(b->gdst, dst.data(), n, st)) return r;
-    if (b->gwork.n < tot) HIP_TRY(b->gwork.alloc(tot));
-    if (b->gscore.n < n) {
-        HIP_TRY(b->gscore.alloc(n));
-        HIP_TRY(b->gqe.alloc(n));
-        HIP_TRY(b->gse.alloc(n));
-        HIP_TRY(b->gcells.alloc(n));
-    }
+GenericArgs full_generic_args(saber_hip_full_batch* b) {
     GenericArgs a{};
-    a.ids = b->gids.p;
-    a.nids = (uint32_t)n;
-    a.jobs = b->gjobs.p;
     a.qres = b->q.p + kGuard;
     a.qbias = b->has_bias ? b->bias.p + kGuard : nullptr;
     a.sres = b->sres;
     a.mat = b->mat.p;
     a.go = b->go;
     a.ge = b->ge;
-    a.out = BandOutputs{b->gscore.p, b->gqe.p, b->gse.p, b->gcells.p};
     a.work = b->gwork.p;
-    a.work_off = b->goff.p;
-    HIP_TRY(launch_generic(a, st));
-    const uint32_t block = 256;
-    hipLaunchKernelGGL(scatter_full_kernel, dim3((uint32_t)((n + block - 1) / block)), dim3(block), 0, st,
-                       b->gdst.p, (uint32_t)n, b->gscore.p, b->gqe.p, b->gse.p, b->score.p, b->q_end.p,
-                       b->s_end.p);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipStreamSynchronize(st));  // host vectors above back the uploads
-    b->rescued += n;
-    return SABER_HIP_OK;
+    return a;
 }
 
-int full_enqueue(saber_hip_full_batch* b) {
+// One pass, entirely on the device (so it can be captured as a graph): claim
+// counters cleared, the class kernels (concurrent, on the context's work
+// streams), then the int32 kernel over the static long-subject pairs and
+// over the pairs the packed kernels flagged (count read on the device).
+// ev_begin / ev_end bracket the packed kernels.
+int full_enqueue(saber_hip_full_batch* b, hipEvent_t ev_begin, hipEvent_t ev_end) {
     saber_hip_ctx* c = b->ctx;
     hipStream_t st = c->stream;
     HIP_TRY(fill_u32(b->nflagged.p, 1, 0, st));
-    HIP_TRY(hipEventRecord(c->fork, st));
-    for (int k = 0; k < kStreams; ++k) HIP_TRY(hipStreamWaitEvent(c->work[k], c->fork, 0));
-    int nv = 0;
-    const FullVariant* var = full_variants(&nv);
-    const uint32_t chunk = 65535;
-    for (const FullLaunch& L : b->launches) {
-        FullKernelArgs a{};
-        a.pairs = b->pairs.p + 2 * L.pair_off;
-        a.npairs = L.npairs;
-        a.queries = b->queries.p;
-        a.qoff = b->qoff.p;
-        a.qres = b->q.p + kGuard;
-        a.qbias = b->has_bias ? b->bias.p + kGuard : nullptr;
-        a.qres_len = b->qlen_total;
-        a.soff = b->soff.p;
-        a.sres = b->sres;
-        a.pair_table = b->pair_table.p;
-        a.go = b->go;
-        a.ge = b->ge;
-        a.nsub = b->ns;
-        a.score = b->score.p;
-        a.q_end = b->q_end.p;
-        a.s_end = b->s_end.p;
-        a.flagged = b->flagged.p;
-        a.nflagged = b->nflagged.p;
-        for (uint32_t s0 = 0; s0 < b->nq; s0 += chunk) {
-            a.slot0 = s0;
-            HIP_TRY(var[L.variant].launch(a, std::min(chunk, b->nq - s0), c->work[L.stream]));
+    HIP_TRY(fill_u32(b->next.p, kCounterStride * std::max<size_t>(1, b->classes.size()), 0, st));
+    FullKernelArgs a{};
+    a.pairs = b->pairs.p;
+    a.qorder = b->qorder.p;
+    a.qoff = b->qoff.p;
+    a.qres = b->q.p + kGuard;
+    a.qbias = b->has_bias ? b->bias.p + kGuard : nullptr;
+    a.qres_len = b->qlen_total;
+    a.soff = b->soff.p;
+    a.sres = b->sres;
+    a.pair_table = b->pair_table.p;
+    a.go = b->go;
+    a.ge = b->ge;
+    a.nsub = b->ns;
+    a.score = b->score.p;
+    a.q_end = b->q_end.p;
+    a.s_end = b->s_end.p;
+    a.flagged = b->flagged.p;
+    a.nflagged = b->nflagged.p;
+    if (ev_begin) HIP_TRY(hipEventRecord(ev_begin, st));
+    const size_t nc = b->classes.size();
+    if (nc > 1) {
+        HIP_TRY(hipEventRecord(c->fork, st));
+        for (size_t k = 1; k < nc; ++k) HIP_TRY(hipStreamWaitEvent(c->work[k - 1], c->fork, 0));
+    }
+    for (size_t k = 0; k < nc; ++k) {
+        const FullClass& fc = b->classes[k];
+        a.segs = b->segs.p + fc.seg_off;
+        a.nsegs = fc.nsegs;
+        a.nchunks = (uint32_t)fc.nchunks;
+        a.next_chunk = b->next.p + kCounterStride * k;
+        HIP_TRY(launch_full_multi(b->has_bias, b->ends, fc.nt, a, fc.grid, k == 0 ? st : c->work[k - 1]));
+    }
+    if (nc > 1)
+        for (size_t k = 1; k < nc; ++k) {
+            HIP_TRY(hipEventRecord(c->join[k - 1], c->work[k - 1]));
+            HIP_TRY(hipStreamWaitEvent(st, c->join[k - 1], 0));
         }
-    }
-    for (int k = 0; k < kStreams; ++k) {
-        HIP_TRY(hipEventRecord(c->join[k], c->work[k]));
-        HIP_TRY(hipStreamWaitEvent(st, c->join[k], 0));
-    }
-    // Long subjects (all queries) go to the exact kernel.
-    std::vector<std::pair<uint32_t, uint32_t>> prs;
-    for (uint32_t si : b->long_subjects)
-        for (uint32_t qi = 0; qi < b->nq; ++qi) prs.emplace_back(qi, si);
-    if (int r = full_rescue(b, prs)) return r;
-    // Saturated pairs.
-    uint32_t nf = 0;
-    HIP_TRY(hipMemcpyAsync(&nf, b->nflagged.p, 4, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
-    if (nf) {
-        std::vector<uint32_t> f(2 * (size_t)nf);
-        HIP_TRY(hipMemcpy(f.data(), b->flagged.p, 8 * (size_t)nf, hipMemcpyDeviceToHost));
-        prs.clear();
-        for (uint32_t k = 0; k < nf; ++k) prs.emplace_back(f[2 * k], f[2 * k + 1]);
-        std::sort(prs.begin(), prs.end());
-        if (int r = full_rescue(b, prs)) return r;
-    }
+    if (ev_end) HIP_TRY(hipEventRecord(ev_end, st));
+    const GenericArgs g = full_generic_args(b);
+    if (b->nstatic)
+        HIP_TRY(launch_full_int32(g, b->static_pairs.p, b->static_count.p, b->qoff.p, b->soff.p, b->ns, b->score.p,
+                                  b->q_end.p, b->s_end.p, b->gthreads, b->gslot, st));
+    HIP_TRY(launch_full_int32(g, b->flagged.p, b->nflagged.p, b->qoff.p, b->soff.p, b->ns, b->score.p, b->q_end.p,
+                              b->s_end.p, b->gthreads, b->gslot, st));
     return SABER_HIP_OK;
+}
+
+// Enqueue one pass: the first directly, later ones from the graph captured
+// on the second (as for band batches), unless graphs are switched off.
+int full_enqueue_run(saber_hip_full_batch* b) {
+    hipStream_t st = b->ctx->stream;
+    if ((b->ctx->flags & SABER_HIP_NO_GRAPH) || b->passes++ == 0) return full_enqueue(b, nullptr, nullptr);
+    if (!b->graph_exec) {
+        HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        const int r = full_enqueue(b, nullptr, nullptr);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(st, &g);
+        if (r != SABER_HIP_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return r;
+        }
+        HIP_TRY(e);
+        b->graph = g;
+        HIP_TRY(hipGraphInstantiate(&b->graph_exec, b->graph, nullptr, nullptr, 0));
+    }
+    HIP_TRY(hipGraphLaunch(b->graph_exec, st));
+    return SABER_HIP_OK;
+}
+
+// Shape of a subject length: the narrowest shape with L*C >= length (ties:
+// fewer lanes), or the context's tuned choice for that length's class; -1
+// when no shape covers it.
+int full_shape_for(const saber_hip_ctx* c, bool ends, uint32_t len) {
+    int ns = 0;
+    const FullShape* sh = full_shapes(ends, &ns);
+    int best = -1;
+    for (int v = 0; v < ns; ++v) {
+        const int W = sh[v].L * sh[v].C;
+        if ((int64_t)W < (int64_t)len) continue;
+        if (best < 0 || W < sh[best].L * sh[best].C || (W == sh[best].L * sh[best].C && sh[v].L < sh[best].L))
+            best = v;
+    }
+    if (best >= 0) {
+        const auto& tuned = c->full_tuned[ends ? 1 : 0];
+        const auto it = tuned.find(best);
+        if (it != tuned.end()) best = it->second;
+    }
+    return best;
 }
 
 }  // namespace
@@ -1464,60 +1460,90 @@ int saber_hip_full_batch_create(saber_hip_ctx* c, const saber_hip_seqset* querie
     for (int i = 0; i < 576; ++i)
         if (scheme->matrix[i] < -128 || scheme->matrix[i] > 127) packed_ok = false;
 
-    // Bucket subjects by length: the narrowest variant with L*C >= length
-    // (ties: fewer lanes per pair).
-    int nv = 0;
-    const FullVariant* var = full_variants(&nv);
-    int widest = 0;
-    for (int v = 0; v < nv; ++v) widest = std::max(widest, var[v].L * var[v].C);
-    std::vector<std::vector<uint32_t>> bucket(nv);
-    std::vector<uint32_t> len(b->ns);
+    // Subjects by shape (longest first within a shape), paired two per lane;
+    // subjects no shape covers go to the int32 kernel with every query.
+    int nshape = 0;
+    const FullShape* sh = full_shapes(b->ends, &nshape);
+    std::vector<std::vector<uint32_t>> bucket(nshape);
+    std::vector<uint32_t> len(b->ns), long_subjects;
+    uint32_t max_s = 0;
     for (uint32_t si = 0; si < b->ns; ++si) {
         len[si] = (uint32_t)(b->hs[si + 1] - b->hs[si]);
-        int best = -1;
-        if (packed_ok) {
-            for (int v = 0; v < nv; ++v) {
-                if (var[v].bias != (b->has_bias ? 1 : 0) || var[v].ends != (b->ends ? 1 : 0)) continue;
-                const int W = var[v].L * var[v].C;
-                if ((int64_t)W < (int64_t)len[si]) continue;
-                if (best < 0 || W < var[best].L * var[best].C ||
-                    (W == var[best].L * var[best].C && var[v].L < var[best].L))
-                    best = v;
-            }
-        }
-        if (best < 0)
-            b->long_subjects.push_back(si);
+        max_s = std::max(max_s, len[si]);
+        const int v = packed_ok ? full_shape_for(c, b->ends, len[si]) : -1;
+        if (v < 0)
+            long_subjects.push_back(si);
         else
-            bucket[best].push_back(si);
+            bucket[v].push_back(si);
     }
     std::vector<uint32_t> pairs;
-    for (int v = 0; v < nv; ++v) {
+    std::vector<uint32_t> pair_off(nshape, 0), npairs(nshape, 0);
+    for (int v = 0; v < nshape; ++v) {
         auto& s = bucket[v];
-        if (s.empty()) continue;
         std::stable_sort(s.begin(), s.end(), [&](uint32_t x, uint32_t y) { return len[x] > len[y]; });
-        FullLaunch L{v, pairs.size() / 2, (uint32_t)((s.size() + 1) / 2), 0.0, 0};
+        pair_off[v] = (uint32_t)(pairs.size() / 2);
+        npairs[v] = (uint32_t)((s.size() + 1) / 2);
         for (size_t i = 0; i < s.size(); i += 2) {
             pairs.push_back(s[i]);
             pairs.push_back(i + 1 < s.size() ? s[i + 1] : kNone);
         }
-        L.cost = (double)L.npairs * var[v].L * var[v].C;
-        b->launches.push_back(L);
     }
-    std::sort(b->launches.begin(), b->launches.end(),
-              [](const FullLaunch& x, const FullLaunch& y) { return x.cost > y.cost; });
-    double load[kStreams] = {0};
-    for (FullLaunch& L : b->launches) {
-        int best = 0;
-        for (int k = 1; k < kStreams; ++k)
-            if (load[k] < load[best]) best = k;
-        L.stream = best;
-        load[best] += L.cost;
-    }
+    // Queries longest first; segments per register class, costliest chunk
+    // first (a chunk runs m + L - 1 steps of C columns).
+    std::vector<uint32_t> qorder(b->nq);
+    for (uint32_t k = 0; k < b->nq; ++k) qorder[k] = k;
+    auto qlen = [&](uint32_t k) { return b->hq[k + 1] - b->hq[k]; };
+    std::stable_sort(qorder.begin(), qorder.end(), [&](uint32_t x, uint32_t y) { return qlen(x) > qlen(y); });
+    const uint64_t max_m = b->nq ? qlen(qorder[0]) : 0;
+    std::vector<int> nts;
+    for (int v = 0; v < nshape; ++v)
+        if (npairs[v] && std::find(nts.begin(), nts.end(), sh[v].nt) == nts.end()) nts.push_back(sh[v].nt);
+    std::sort(nts.begin(), nts.end(), std::greater<int>());
+    if (b->nq > 0)
+        for (int nt : nts) {
+            std::vector<int> vs;
+            for (int v = 0; v < nshape; ++v)
+                if (npairs[v] && sh[v].nt == nt) vs.push_back(v);
+            std::stable_sort(vs.begin(), vs.end(), [&](int x, int y) {
+                return (max_m + sh[x].L - 1) * sh[x].C > (max_m + sh[y].L - 1) * sh[y].C;
+            });
+            if ((int)vs.size() > kMaxFullSegs) return fail(SABER_HIP_EINVAL, "too many shapes in a class");
+            FullClass fc;
+            fc.nt = nt;
+            fc.seg_off = (uint32_t)b->host_segs.size();
+            uint64_t start = 0;
+            for (int v : vs) {
+                FullSeg sg{};
+                sg.start = (uint32_t)start;
+                sg.shape = (uint32_t)v;
+                sg.nch = (npairs[v] + 64 / sh[v].L - 1) / (64 / sh[v].L);
+                sg.pair_off = pair_off[v];
+                sg.npairs = npairs[v];
+                b->host_segs.push_back(sg);
+                start += (uint64_t)b->nq * sg.nch;
+            }
+            if (start >= (1ull << 32)) return fail(SABER_HIP_EINVAL, "batch too large for one pass");
+            fc.nsegs = (uint32_t)vs.size();
+            fc.nchunks = start;
+            // persistent blocks: as many as fit at once, fewer when the
+            // chunks would not give every wave one
+            const int per_cu = full_multi_blocks_per_cu(b->has_bias, b->ends, nt);
+            fc.grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)c->cus * per_cu,
+                                                                         (start + nt / 64 - 1) / (nt / 64)));
+            b->classes.push_back(fc);
+        }
+    if (b->classes.size() > (size_t)kStreams + 1) return fail(SABER_HIP_EINVAL, "too many register classes");
+    // int32 pairs: long subjects with every query
+    std::vector<uint32_t> spairs;
+    for (uint32_t si : long_subjects)
+        for (uint32_t qi = 0; qi < b->nq; ++qi) {
+            spairs.push_back(qi);
+            spairs.push_back(si);
+        }
+    b->nstatic = spairs.size() / 2;
+    if (b->nstatic >= (1ull << 32)) return fail(SABER_HIP_EINVAL, "batch too large");
 
     const std::vector<uint32_t> ptable = build_pair_table(scheme->matrix);
-    std::vector<uint32_t> qids(b->nq);
-    for (uint32_t k = 0; k < b->nq; ++k) qids[k] = k;
-
     hipStream_t st = c->stream;
     const uint64_t nqres = b->hq.back();
     if (int r = upload_codes(b->q, queries->residues, nqres, st)) return r;
@@ -1534,16 +1560,27 @@ int saber_hip_full_batch_create(saber_hip_ctx* c, const saber_hip_seqset* querie
     b->slen_total = b->hs.back();
     if (int r = upload(b->qoff, b->hq.data(), b->hq.size(), st)) return r;
     if (int r = upload(b->soff, b->hs.data(), b->hs.size(), st)) return r;
-    if (int r = upload(b->queries, qids.data(), qids.size(), st)) return r;
+    if (int r = upload(b->qorder, qorder.data(), qorder.size(), st)) return r;
     if (int r = upload(b->pairs, pairs.data(), pairs.size(), st)) return r;
+    if (int r = upload(b->segs, b->host_segs.data(), b->host_segs.size(), st)) return r;
     if (int r = upload(b->pair_table, ptable.data(), ptable.size(), st)) return r;
     if (int r = upload(b->mat, scheme->matrix, 576, st)) return r;
+    if (b->nstatic) {
+        if (int r = upload(b->static_pairs, spairs.data(), spairs.size(), st)) return r;
+        const uint32_t cnt = (uint32_t)b->nstatic;
+        if (int r = upload(b->static_count, &cnt, 1, st)) return r;
+    }
+    HIP_TRY(b->next.alloc(kCounterStride * std::max<size_t>(1, b->classes.size())));
     const uint64_t nout = std::max<uint64_t>(1, (uint64_t)b->nq * b->ns);
     HIP_TRY(b->score.alloc(nout));
     HIP_TRY(b->q_end.alloc(nout));
     HIP_TRY(b->s_end.alloc(nout));
     HIP_TRY(b->flagged.alloc(2 * nout));
     HIP_TRY(b->nflagged.alloc(1));
+    // int32 work slots: two band rows (4 arrays) of the widest full band
+    b->gslot = (uint32_t)(4 * (max_m + max_s + 1));
+    b->gthreads = (uint32_t)std::min<uint64_t>(1024, std::max<uint64_t>(256, b->nstatic));
+    HIP_TRY(b->gwork.alloc((uint64_t)b->gthreads * b->gslot));
     HIP_TRY(hipStreamSynchronize(st));
     *out = b.release();
     return SABER_HIP_OK;
@@ -1553,8 +1590,7 @@ int saber_hip_full_batch_run(saber_hip_full_batch* b) {
     if (!b) return fail(SABER_HIP_EINVAL, "batch is null");
     std::lock_guard<std::mutex> g(b->ctx->mu);
     HIP_TRY(hipSetDevice(b->ctx->device));
-    b->rescued = 0;
-    if (int r = full_enqueue(b)) return r;
+    if (int r = full_enqueue_run(b)) return r;
     b->ran = true;
     return SABER_HIP_OK;
 }
@@ -1587,11 +1623,10 @@ int saber_hip_full_batch_time(saber_hip_full_batch* b, int warmup, int iters, do
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipStreamSynchronize(st));
     HIP_TRY(hipEventRecord(e0, st));
-    for (int it = 0; it < iters; ++it) {
-        b->rescued = 0;
-        if (int r = full_enqueue(b)) return r;
-    }
+    for (int it = 0; it < iters; ++it)
+        if (int r = full_enqueue_run(b)) return r;
     HIP_TRY(hipEventRecord(e1, st));
     HIP_TRY(hipEventSynchronize(e1));
     b->ran = true;
@@ -1599,14 +1634,12 @@ int saber_hip_full_batch_time(saber_hip_full_batch* b, int warmup, int iters, do
     HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    uint32_t nf = 0;
+    HIP_TRY(hipMemcpy(&nf, b->nflagged.p, 4, hipMemcpyDeviceToHost));
+    b->last_flagged = nf;
     if (ms_run) *ms_run = ms / iters;
-    if (cells) {
-        uint64_t sq = 0, ss = 0;
-        sq = b->hq.back();
-        ss = b->hs.back();
-        *cells = sq * ss;
-    }
-    if (rescued) *rescued = b->rescued;
+    if (cells) *cells = b->hq.back() * b->hs.back();
+    if (rescued) *rescued = b->nstatic + b->last_flagged;
     return SABER_HIP_OK;
 }
 
@@ -1614,6 +1647,8 @@ void saber_hip_full_batch_destroy(saber_hip_full_batch* b) {
     if (!b) return;
     (void)hipSetDevice(b->ctx->device);
     (void)hipStreamSynchronize(b->ctx->stream);
+    if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
     delete b;
 }
 

@@ -104,6 +104,10 @@ hipError_t launch_generic(const GenericArgs& a, hipStream_t s);
 // multiple of 128), each with slot_ints int32 of a.work (a.work_off unused).
 hipError_t launch_generic_slots(const GenericArgs& a, const uint32_t* count, uint32_t nthreads, uint32_t slot_ints,
                                 hipStream_t s);
+// Full-matrix (query, subject) pairs scored exactly, count read on the device.
+hipError_t launch_full_int32(const GenericArgs& a, const uint32_t* pairs, const uint32_t* count, const uint64_t* qoff,
+                             const uint64_t* soff, uint32_t nsub, int32_t* score, uint32_t* q_end, uint32_t* s_end,
+                             uint32_t nthreads, uint32_t slot_ints, hipStream_t s);
 
 }  // namespace saber
 
@@ -143,10 +147,26 @@ namespace saber {
 
 // Full-matrix (exhaustive) kernel over the cross product of queries and
 // subjects; outputs are [query slot][subject].
+constexpr int kMaxFullSegs = 32;   // segments per register class (shapes)
+
+// A segment of the full-matrix chunk list: one shape's subject pairs against
+// every query; chunk ch of query rank qr is chunk start + qr * nch + ch.
+struct FullSeg {
+    uint32_t start;             // first chunk
+    uint32_t shape;             // shape id in the flavour's list
+    uint32_t nch;               // chunks per query (64 / L pairs each)
+    uint32_t pair_off;          // first pair of the shape in the pair list
+    uint32_t npairs;
+    uint32_t pad[3];
+};
+
 struct FullKernelArgs {
     const uint32_t* pairs;      // 2 subject ids per pair (0xFFFFFFFF = none)
-    uint32_t npairs;
-    const uint32_t* queries;    // query ids of this launch (blockIdx.y indexes it)
+    const FullSeg* segs;        // this register class's segments
+    uint32_t nsegs;
+    uint32_t nchunks;
+    uint32_t* next_chunk;       // claim counter (zero before the launch)
+    const uint32_t* qorder;     // query ids by rank (longest first)
     const uint64_t* qoff;       // query offsets
     const uint8_t* qres;        // guard-padded like the band kernel's buffers
     const int8_t* qbias;
@@ -156,22 +176,25 @@ struct FullKernelArgs {
     const uint32_t* pair_table; // kPairWords words (global; staged into LDS)
     int32_t go, ge;
     uint32_t nsub;              // row stride of the outputs (subjects per query)
-    uint32_t slot0;             // first output row of this launch
     int32_t* score;
     uint32_t* q_end;
     uint32_t* s_end;
-    uint32_t* flagged;          // (slot, subject) pairs needing the int32 rescore
+    uint32_t* flagged;          // (query, subject) pairs needing the int32 rescore
     uint32_t* nflagged;
 };
 
-struct FullVariant {
-    int L, C;
-    int bias;
-    int ends;                   // 1: score and end cell, 0: score only
-    hipError_t (*launch)(const FullKernelArgs&, uint32_t nslots, hipStream_t);
+// A compiled full-matrix shape: L lanes per subject pair, C columns per lane;
+// nt = threads per block of its register class.
+struct FullShape {
+    int L, C, nt;
 };
 
-const FullVariant* full_variants(int* count);
+// The shapes of a flavour (ends: with end cells), in shape-id order.
+const FullShape* full_shapes(bool ends, int* count);
+// One register class's persistent kernel (nt threads per block).
+hipError_t launch_full_multi(bool bias, bool ends, int nt, const FullKernelArgs& a, uint32_t grid, hipStream_t s);
+// Blocks of that kernel resident per CU.
+int full_multi_blocks_per_cu(bool bias, bool ends, int nt);
 
 }  // namespace saber
 

@@ -5,9 +5,10 @@
 // first row-major optimal cell (exclusive ends).
 //
 // Work: the cross product of a query set and a subject set. Subjects are
-// bucketed by length into variants; two subjects of similar length share each
-// lane (int16 halves), both against the same query (blockIdx.y), so the query
-// residue of a row is common to the pair.
+// bucketed by length into shapes; two subjects of similar length share each
+// lane (int16 halves), both against the same query, so the query residue of a
+// row is common to the pair. One persistent kernel per register class drains
+// a chunk list covering every (shape, query) segment.
 //
 // Layout: rows are query positions, columns subject positions. L lanes serve a
 // subject pair, lane l owning columns [l*C, l*C + C) and working on row t - l
@@ -46,34 +47,38 @@ __device__ __forceinline__ uint32_t full_from_prev_lane(uint32_t v) {
         return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x138, 0xF, 0xF, false);
 }
 
-// Workgroup size: 12 waves (3 per SIMD: 168 registers) unless the shape needs
-// more registers (wide lanes; end-cell snapshots): 8 waves (256 registers),
-// or 4 waves for end-cell snapshots of the widest lanes (which still spill a
-// little; they only serve subjects longer than 2048).
+// Compiled shapes (L lanes per subject pair, C subject columns per lane:
+// subjects up to L*C long), by flavour; a shape's id is its position in the
+// list. The scores-only flavour adds wider lanes (C = 40, 48) for every L;
+// the end-cell flavour, whose snapshots cost registers, keeps C <= 32 except
+// for the longest subjects.
+#define SABER_FULL_SCORE_SHAPES(X)                                                              \
+    X(1, 16) X(1, 24) X(1, 32) X(1, 40) X(1, 48) X(2, 32) X(2, 40) X(2, 48) X(4, 32) X(4, 40)   \
+    X(4, 48) X(8, 32) X(8, 40) X(8, 48) X(16, 32) X(16, 40) X(16, 48) X(32, 32) X(32, 40)       \
+    X(32, 48) X(64, 32) X(64, 40) X(64, 48)
+#define SABER_FULL_END_SHAPES(X)                                                                \
+    X(1, 16) X(1, 24) X(1, 32) X(2, 24) X(2, 32) X(4, 24) X(4, 32) X(8, 24) X(8, 32) X(16, 24) \
+    X(16, 32) X(32, 24) X(32, 32) X(64, 24) X(64, 32) X(64, 40) X(64, 48)
+
+// Workgroup size of a shape's register class: 12 waves (3 per SIMD: 168
+// registers) unless the shape needs more registers (wide lanes; end-cell
+// snapshots): 8 waves (256 registers), or 4 waves for end-cell snapshots of
+// the widest lanes (which still spill a little; they only serve subjects
+// longer than 2048).
 template <int C, bool ENDS>
 constexpr int kFullThreads = ENDS ? (C <= 24 ? 768 : C <= 32 ? 512 : 256) : (C <= 32 ? 768 : 512);
 
+// One subject pair (pair index g of a.pairs, lane l of its L) against query
+// qid: the whole matrix, outputs at row qid of the [query][subject] arrays.
 template <int L, int C, bool BIAS, bool ENDS>
-__global__ __launch_bounds__((kFullThreads<C, ENDS>)) void full_pair_kernel(FullKernelArgs a) {
+__device__ __attribute__((noinline)) void full_group(const FullKernelArgs& a, const char* Tpb, uint32_t qid,
+                                                     uint32_t g, int l) {
     static_assert(C >= 2 && C % 2 == 0, "variant shape");
     static_assert(L == 1 || L == 2 || L == 4 || L == 8 || L == 16 || L == 32 || L == 64,
                   "groups must tile a wave");
-    constexpr int NT = kFullThreads<C, ENDS>;
     constexpr int NE = C / 2;
-    __shared__ uint32_t TP[kPairWords];
-    for (int t = threadIdx.x; t < kPairWords; t += NT) TP[t] = a.pair_table[t];
-    __syncthreads();
-    const char* Tpb = reinterpret_cast<const char*>(TP);
-
-    const uint32_t gl = blockIdx.x * NT + threadIdx.x;
-    const uint32_t g = gl / L;
-    const int l = (int)(gl % L);
-    if (g >= a.npairs) return;
-    const uint32_t slot = a.slot0 + blockIdx.y;
-    const uint32_t qid = a.queries[slot];
     const int64_t q0 = (int64_t)a.qoff[qid];
     const int m = (int)(a.qoff[qid + 1] - a.qoff[qid]);
-
     uint32_t sid[2], n[2];
     const uint8_t* sp[2];
 #pragma unroll
@@ -253,10 +258,10 @@ __global__ __launch_bounds__((kFullThreads<C, ENDS>)) void full_pair_kernel(Full
             }
         }
         if (l == 0 && sid[h] != kNone) {
-            const uint64_t o = (uint64_t)slot * a.nsub + sid[h];
+            const uint64_t o = (uint64_t)qid * a.nsub + sid[h];
             if (top >= kMax16) {
                 const uint32_t f = atomicAdd(a.nflagged, 1u);
-                a.flagged[2 * f] = slot;
+                a.flagged[2 * f] = qid;
                 a.flagged[2 * f + 1] = sid[h];
             } else {
                 a.score[o] = top;
@@ -274,14 +279,66 @@ __global__ __launch_bounds__((kFullThreads<C, ENDS>)) void full_pair_kernel(Full
     }
 }
 
-template <int L, int C, bool BIAS, bool ENDS>
-hipError_t launch_full(const FullKernelArgs& a, uint32_t nslots, hipStream_t s) {
-    if (a.npairs == 0 || nslots == 0) return hipSuccess;
-    constexpr int NT = kFullThreads<C, ENDS>;
-    const uint64_t threads = (uint64_t)a.npairs * L;
-    const uint32_t grid = (uint32_t)((threads + NT - 1) / NT);
-    hipLaunchKernelGGL((full_pair_kernel<L, C, BIAS, ENDS>), dim3(grid, nslots), dim3(NT), 0, s, a);
+// Persistent kernel over the shapes of one register class (kFullThreads ==
+// NT): each wave claims chunks -- 64 / L subject pairs of one shape against
+// one query -- from a single counter until the list is exhausted. The chunk
+// list is a sequence of segments, one per shape, each query-rank major
+// (queries longest first); a segment's chunk ch of query rank qr is chunk
+// start + qr * nch + ch. Every lane of a chunk runs the same query, so the
+// same number of steps.
+template <bool BIAS, bool ENDS, int NT>
+__global__ __launch_bounds__(NT) void full_multi_kernel(FullKernelArgs a) {
+    __shared__ uint32_t TP[kPairWords];
+    __shared__ FullSeg S[kMaxFullSegs];
+    for (int t = threadIdx.x; t < kPairWords; t += NT) TP[t] = a.pair_table[t];
+    for (int t = threadIdx.x; t < (int)a.nsegs; t += NT) S[t] = a.segs[t];
+    __syncthreads();
+    const char* Tpb = reinterpret_cast<const char*>(TP);
+    const int lane = threadIdx.x & 63;
+    for (;;) {
+        uint32_t ci = 0;
+        if (lane == 0) ci = atomicAdd(a.next_chunk, 1u);
+        ci = __builtin_amdgcn_readfirstlane(ci);
+        if (ci >= a.nchunks) break;
+        int k = 0;
+        while (k + 1 < (int)a.nsegs && S[k + 1].start <= ci) ++k;
+        const uint32_t shape = S[k].shape, nch = S[k].nch, pair_off = S[k].pair_off, npairs = S[k].npairs;
+        const uint32_t rel = ci - S[k].start;
+        const uint32_t qr = rel / nch, ch = rel - qr * nch;
+        const uint32_t qid = a.qorder[qr];
+        int id = 0;
+#define SABER_FULL_CASE(L_, C_)                                                                 \
+        if constexpr (kFullThreads<C_, ENDS> == NT) {                                           \
+            if (shape == (uint32_t)id) {                                                        \
+                const uint32_t gi = ch * (64 / (L_)) + (uint32_t)lane / (L_);                   \
+                if (gi < npairs) full_group<L_, C_, BIAS, ENDS>(a, Tpb, qid, pair_off + gi, lane % (L_)); \
+                continue;                                                                       \
+            }                                                                                   \
+        }                                                                                       \
+        ++id;
+        if constexpr (ENDS) {
+            SABER_FULL_END_SHAPES(SABER_FULL_CASE)
+        } else {
+            SABER_FULL_SCORE_SHAPES(SABER_FULL_CASE)
+        }
+#undef SABER_FULL_CASE
+    }
+}
+
+// Launch and residency of one class kernel (instantiated per flavour in
+// full_multi_*.hip).
+template <bool BIAS, bool ENDS, int NT>
+hipError_t launch_full_class(const FullKernelArgs& a, uint32_t grid, hipStream_t s) {
+    hipLaunchKernelGGL((full_multi_kernel<BIAS, ENDS, NT>), dim3(grid), dim3(NT), 0, s, a);
     return hipGetLastError();
+}
+
+template <bool BIAS, bool ENDS, int NT>
+int full_class_blocks_per_cu() {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, full_multi_kernel<BIAS, ENDS, NT>, NT, 0) != hipSuccess)
+        return 1;
+    return nb > 0 ? nb : 1;
 }
 
 }  // namespace saber

@@ -13,9 +13,14 @@ constexpr int32_t kNeg32 = INT32_MIN / 4;
 
 __device__ __forceinline__ int32_t imax(int32_t a, int32_t b) { return a > b ? a : b; }
 
+struct JobResult {
+    int32_t score;
+    uint32_t q_end, s_end;
+    uint64_t cells;
+};
+
 // One job, its two band rows in work (4 * (w + 1) int32).
-__device__ void generic_job(const GenericArgs& a, uint32_t id, int32_t* work) {
-    const BandJobDev J = a.jobs[id];
+__device__ JobResult generic_run(const GenericArgs& a, const BandJobDev& J, int32_t* work) {
     const int64_t w = J.w;
     int32_t* hp = work;
     int32_t* fp = hp + (w + 1);
@@ -81,10 +86,15 @@ __device__ void generic_job(const GenericArgs& a, uint32_t id, int32_t* work) {
         fp = fc;
         fc = x;
     }
-    a.out.score[id] = best;
-    a.out.q_end[id] = best > 0 ? (uint32_t)bq : 0u;
-    a.out.s_end[id] = best > 0 ? (uint32_t)bs : 0u;
-    a.out.cells[id] = cells;
+    return JobResult{best, best > 0 ? (uint32_t)bq : 0u, best > 0 ? (uint32_t)bs : 0u, cells};
+}
+
+__device__ void generic_job(const GenericArgs& a, uint32_t id, int32_t* work) {
+    const JobResult r = generic_run(a, a.jobs[id], work);
+    a.out.score[id] = r.score;
+    a.out.q_end[id] = r.q_end;
+    a.out.s_end[id] = r.s_end;
+    a.out.cells[id] = r.cells;
 }
 
 // One thread per listed job, work at work_off[thread].
@@ -111,6 +121,49 @@ hipError_t launch_generic_slots(const GenericArgs& a, const uint32_t* count, uin
     const uint32_t block = 128;
     hipLaunchKernelGGL(generic_slots_kernel, dim3((nthreads + block - 1) / block), dim3(block), 0, s, a, count,
                        slot_ints);
+    return hipGetLastError();
+}
+
+// Full-matrix pairs (query, subject) = (pairs[2i], pairs[2i+1]) for
+// i < *count, scored exactly: the whole matrix as one band (lo = -(m-1),
+// width m + n - 1), results written to row query, column subject of the
+// [query][subject] outputs (stride nsub). A fixed grid walks the list, each
+// thread with its own work slot of slot_ints int32.
+__global__ __launch_bounds__(128) void full_int32_kernel(GenericArgs a, const uint32_t* pairs, const uint32_t* count,
+                                                         const uint64_t* qoff, const uint64_t* soff, uint32_t nsub,
+                                                         int32_t* score, uint32_t* q_end, uint32_t* s_end,
+                                                         uint32_t slot_ints) {
+    const uint32_t n = *count;
+    const uint32_t nt = gridDim.x * blockDim.x;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    int32_t* work = a.work + (uint64_t)t * slot_ints;
+    for (uint32_t i = t; i < n; i += nt) {
+        const uint32_t qi = pairs[2 * i], si = pairs[2 * i + 1];
+        const int64_t m = (int64_t)(qoff[qi + 1] - qoff[qi]), len = (int64_t)(soff[si + 1] - soff[si]);
+        BandJobDev J{};
+        J.q_off = qoff[qi];
+        J.s_off = soff[si];
+        J.m = (uint32_t)m;
+        J.n = (uint32_t)len;
+        J.lo = (int32_t)(m > 0 ? -(m - 1) : 0);
+        J.w = (int32_t)(m > 0 && len > 0 ? m + len - 1 : 0);
+        J.i_first = 0;
+        J.rows = J.w > 0 ? (int32_t)m : 0;
+        const JobResult r = generic_run(a, J, work);
+        const uint64_t o = (uint64_t)qi * nsub + si;
+        score[o] = r.score;
+        q_end[o] = r.q_end;
+        s_end[o] = r.s_end;
+    }
+}
+
+hipError_t launch_full_int32(const GenericArgs& a, const uint32_t* pairs, const uint32_t* count, const uint64_t* qoff,
+                             const uint64_t* soff, uint32_t nsub, int32_t* score, uint32_t* q_end, uint32_t* s_end,
+                             uint32_t nthreads, uint32_t slot_ints, hipStream_t s) {
+    if (nthreads == 0) return hipSuccess;
+    const uint32_t block = 128;
+    hipLaunchKernelGGL(full_int32_kernel, dim3((nthreads + block - 1) / block), dim3(block), 0, s, a, pairs, count,
+                       qoff, soff, nsub, score, q_end, s_end, slot_ints);
     return hipGetLastError();
 }
 

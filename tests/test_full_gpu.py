@@ -130,3 +130,60 @@ def test_scores_only_kernel(ctx, with_bias):
     want = oracle_grid(qs, ss, BLOSUM62, 11, 1, bias)[0]
     bad = np.argwhere(got.astype(np.int64) != want)
     assert len(bad) == 0, f"score differs at {bad[:5].tolist()}"
+
+
+@pytest.mark.parametrize("ends", [True, False])
+def test_graph_replays_and_int32_paths(ctx, ends):
+    """Full passes replayed from the captured graph equal a pass enqueued
+    directly and the oracle, on every replay, with other work between the
+    replays: subjects longer than the widest shape (the static int32 list) and
+    saturating pairs (flagged by the packed kernels, rescored by the int32
+    kernel from the count on the device) included; the forced-int32 context
+    agrees. A matrix scoring 100 per identity saturates int16 within 330
+    residues, which keeps the int32 work small."""
+    r = Lcg(61 + ends)
+    mat = np.full((24, 24), -30, np.int32)
+    np.fill_diagonal(mat, 100)
+    hot = rand_seq(r, 400)                     # self score 40 000: saturates
+    qs = [rand_seq(r, 1 + r.next(350)) for _ in range(5)] + [hot]
+    ss = [rand_seq(r, 1 + r.next(900)) for _ in range(40)] + [hot, rand_seq(r, 3300), hot[:300]]
+    for k in range(0, 40, 5):
+        q = qs[k % 5]
+        kk = min(len(ss[k]), len(q))
+        ss[k][:kk] = np.where(np.array([r.next(3) for _ in range(kk)]) > 0, q[:kk], ss[k][:kk])
+    qres, qoff = flat(qs)
+    sres, soff = flat(ss)
+    sch = S.Scheme(mat, 11, 1)
+    want = oracle_grid(qs, ss, mat, 11, 1)
+    assert want[0][5, 40] == 40000 and want[0][5, 42] == 30000
+    direct = S.Context(graphs=False)
+    got = direct.score_full(qres, qoff, sres, soff, sch, ends=ends)
+    direct.close()
+    exact = S.Context(force_exact32=True)
+    got32 = exact.score_full(qres, qoff, sres, soff, sch, ends=ends)
+    exact.close()
+    for g in (got, got32):
+        assert np.array_equal(g[0], want[0])
+        if ends:
+            assert np.array_equal(g[1], want[1]) and np.array_equal(g[2], want[2])
+    b = ctx.full_batch(qres, qoff, sres, soff, sch, ends=ends)
+    other = ctx.full_batch(qres[:qoff[2]], qoff[:3], sres, soff, sch, ends=False)
+    try:
+        for rep in range(4):
+            b.run()
+            g = b.read()
+            assert np.array_equal(g[0], want[0]), f"replay {rep} scores"
+            if ends:
+                assert np.array_equal(g[1], want[1]) and np.array_equal(g[2], want[2]), f"replay {rep} ends"
+            other.run()
+            other.read()
+        ms, cells, rescued = b.time(warmup=0, iters=2)
+        # every query with the 3300-long subject (column 41), plus the pairs
+        # of the packed kernels reaching 32 767
+        packed = np.delete(want[0], 41, axis=1)
+        flagged = int((packed >= 32767).sum())
+        assert flagged >= 1 and rescued == len(qs) + flagged, (rescued, flagged)
+        assert cells == int(qoff[-1]) * int(soff[-1]) and ms > 0
+    finally:
+        b.close()
+        other.close()
