@@ -23,7 +23,8 @@ The printed JSON line also carries
                 of whole queries of the same batch;
   extra.xdrop   X-drop extension of 4 seed hits per region (steps/s);
   extra.full_sw exhaustive full-matrix SW (queries x subjects, GCUPS; scores
-                only, as score_many, and with the end cell);
+                only, as score_many, and with the end cell; shapes changed by
+                the autotuner);
   extra.prefilter  the learned region prefilter: hit loss, kept fraction,
                 scoring-stage time with and without it, MLP forward
                 throughput against the bf16 MFMA peak.
@@ -284,7 +285,16 @@ def main():
         fs, fso = workload.sequences(args.seed + 200, args.full_subjects, 250, 0.6, 30, 3000)
         # score_many returns scores only: the headline figure is the scores-only
         # kernel; the end-cell kernel (sw_reference's first optimal cell) is
-        # timed beside it.
+        # timed beside it. The shape per subject-length group is measured
+        # first (both flavours) unless --no-autotune.
+        ftune = None
+        if not args.no_autotune:
+            t0 = time.time()
+            ftune = {"scores": ctx.full_autotune(fq, fqo, fs, fso, sch, ends=False),
+                     "ends": ctx.full_autotune(fq, fqo, fs, fso, sch, ends=True)}
+            print(f"# full-SW autotune {time.time() - t0:.1f}s: " + ", ".join(
+                f"{k} {e['len_lo']}-{e['len_hi']}->{e['shape']}" for k, v in ftune.items() for e in v
+                if e["changed"]), file=sys.stderr, flush=True)
         fb = ctx.full_batch(fq, fqo, fs, fso, sch, ends=False)
         fms, fcells, fresc = fb.time(warmup=1, iters=2)
         fsc = fb.read()[0]
@@ -301,7 +311,10 @@ def main():
         fl = {"queries": args.full_queries, "subjects": args.full_subjects, "ms": round(fms, 3),
               "cells": int(fcells), "gcups": round(fcells / (fms * 1e-3) / 1e9, 1),
               "ms_with_ends": round(fems, 3), "gcups_with_ends": round(fcells / (fems * 1e-3) / 1e9, 1),
-              "int32_rescored_pairs": int(fresc), "parity_sample": 40, "parity_mismatches": fbad}
+              "int32_rescored_pairs": int(fresc), "parity_sample": 40, "parity_mismatches": fbad,
+              "autotune": None if ftune is None else {
+                  k: [{"len": [e["len_lo"], e["len_hi"]], "shape": list(e["shape"])} for e in v if e["changed"]]
+                  for k, v in ftune.items()}}
         print(f"# full SW {args.full_queries} x {args.full_subjects}: {fms:.2f} ms, "
               f"{fl['gcups']} GCUPS (with end cells {fems:.2f} ms, {fl['gcups_with_ends']} GCUPS), "
               f"mismatches {fbad}/40", file=sys.stderr, flush=True)

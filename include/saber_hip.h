@@ -260,6 +260,21 @@ int saber_hip_band_autotune(saber_hip_ctx* ctx, const saber_hip_seqset* queries,
                             uint64_t njobs, const saber_hip_scheme* scheme, uint64_t target_cells,
                             saber_hip_tune_entry* report, int nreport, int* nclasses);
 
+/* Full-matrix shape autotuner (no counterpart in the reference): subjects
+ * are grouped by their modelled shape (the narrowest with L*C >= length);
+ * per group, in decreasing share of the residues, every shape covering the
+ * group's longest subject up to twice its width is timed on a sample (whole
+ * queries spread over the length order, up to target_cells cells against all
+ * subjects; 0 = 5e10) and kept when the pass gets faster by more than 1 %.
+ * The choice applies to full batches of this flavour (ends) created
+ * afterwards on the context. report entries: w_lo / w_hi = the group's
+ * subject lengths, L, C = the chosen shape (K = 0; reserved = 1 when it is
+ * not the modelled one), cells = the group's cells in the sample, gcups = the
+ * sample's rate after tuning. */
+int saber_hip_full_autotune(saber_hip_ctx* ctx, const saber_hip_seqset* queries, const saber_hip_seqset* subjects,
+                            const saber_hip_scheme* scheme, int ends, uint64_t target_cells,
+                            saber_hip_tune_entry* report, int nreport, int* nclasses);
+
 /* Planner tuning (no counterpart in the reference): band jobs with more than
  * latency_rows rows are planned for low per-row latency (short lanes, their
  * blocks dispatched first); 0 turns this off, -1 restores the default rule

@@ -338,6 +338,20 @@ class Context:
         return [{"w_lo": e.w_lo, "w_hi": e.w_hi, "shape": (e.L, e.C, e.K), "changed": bool(e.reserved),
                  "cells": int(e.cells), "gcups": round(e.gcups, 1)} for e in rep[:min(n.value, 64)]]
 
+    def full_autotune(self, qres, qoff, sres, soff, scheme, qbias=None, ends=False, target_cells=50_000_000_000):
+        """Measured full-matrix shape per subject-length group on a sample of
+        whole queries (saber_hip_full_autotune); applies to full batches of
+        this flavour created afterwards. One dict per visited group."""
+        qs, ss, keep = self._seqsets(qres, qoff, qbias, sres, soff)
+        sch = scheme.ffi()
+        rep = (_ffi.TuneEntry * 64)()
+        n = C.c_int()
+        _ffi.check(_ffi.lib().saber_hip_full_autotune(
+            self.h, C.byref(qs), None if ss is None else C.byref(ss), C.byref(sch), int(bool(ends)),
+            int(target_cells), rep, 64, C.byref(n)))
+        return [{"len_lo": e.w_lo, "len_hi": e.w_hi, "shape": (e.L, e.C), "changed": bool(e.reserved),
+                 "cells": int(e.cells), "gcups": round(e.gcups, 1)} for e in rep[:min(n.value, 64)]]
+
     def upload_db(self, qres, qoff):
         off = np.ascontiguousarray(qoff, dtype=np.uint64)
         res = np.ascontiguousarray(qres, dtype=np.uint8)

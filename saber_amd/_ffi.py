@@ -138,6 +138,8 @@ _SIGS = {
     "saber_hip_band_autotune": (C.c_int, [P, C.POINTER(SeqSet), C.POINTER(SeqSet), C.POINTER(BandJob), C.c_uint64,
                                           C.POINTER(Scheme), C.c_uint64, C.POINTER(TuneEntry), C.c_int,
                                           C.POINTER(C.c_int)]),
+    "saber_hip_full_autotune": (C.c_int, [P, C.POINTER(SeqSet), C.POINTER(SeqSet), C.POINTER(Scheme), C.c_int,
+                                          C.c_uint64, C.POINTER(TuneEntry), C.c_int, C.POINTER(C.c_int)]),
     "saber_hip_stats_gapped_protein": (C.c_int, [C.c_int, C.c_int32, C.c_int32, C.c_int, C.POINTER(Karlin)]),
     "saber_hip_stats_length_adjustment": (C.c_int, [C.c_double, C.c_double, C.c_double, C.c_double, C.c_int64,
                                                      C.c_int64, C.c_int64, C.POINTER(C.c_int64),

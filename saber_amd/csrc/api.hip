@@ -1652,6 +1652,133 @@ void saber_hip_full_batch_destroy(saber_hip_full_batch* b) {
     delete b;
 }
 
+int saber_hip_full_autotune(saber_hip_ctx* c, const saber_hip_seqset* queries, const saber_hip_seqset* subjects,
+                            const saber_hip_scheme* scheme, int ends, uint64_t target_cells,
+                            saber_hip_tune_entry* report, int nreport, int* nclasses) {
+    if (!c || !scheme) return fail(SABER_HIP_EINVAL, "null argument");
+    if (int r = check_seqset(queries, "queries")) return r;
+    if (subjects)
+        if (int r = check_seqset(subjects, "subjects")) return r;
+    if (nclasses) *nclasses = 0;
+    if (target_cells == 0) target_cells = 50000000000ull;
+    const int fl = ends ? 1 : 0;
+    std::vector<uint64_t> soffs;
+    std::map<int, int> saved;
+    {
+        std::lock_guard<std::mutex> g(c->mu);
+        soffs = subjects ? std::vector<uint64_t>(subjects->offsets, subjects->offsets + subjects->count + 1) : c->db_off;
+        saved = c->full_tuned[fl];
+        c->full_tuned[fl].clear();   // descend from the modelled choice
+    }
+    if (soffs.size() < 2 || queries->count == 0) return SABER_HIP_OK;
+    const uint32_t ns = (uint32_t)soffs.size() - 1;
+    int nshape = 0;
+    const FullShape* sh = full_shapes(ends != 0, &nshape);
+    // Length classes: subjects by modelled shape.
+    std::vector<uint32_t> lo(nshape, UINT32_MAX), hi(nshape, 0);
+    std::vector<uint64_t> res(nshape, 0);
+    for (uint32_t si = 0; si < ns; ++si) {
+        const uint32_t len = (uint32_t)(soffs[si + 1] - soffs[si]);
+        const int v = full_shape_for(c, ends != 0, len);
+        if (v < 0) continue;
+        lo[v] = std::min(lo[v], len);
+        hi[v] = std::max(hi[v], len);
+        res[v] += len;
+    }
+    // The sample: whole queries spread over the length order, up to
+    // target_cells cells against every subject.
+    const uint64_t stot = soffs.back() - soffs.front();
+    std::vector<uint32_t> qord(queries->count);
+    for (uint32_t k = 0; k < queries->count; ++k) qord[k] = k;
+    auto qlen = [&](uint32_t k) { return queries->offsets[k + 1] - queries->offsets[k]; };
+    std::stable_sort(qord.begin(), qord.end(), [&](uint32_t x, uint32_t y) { return qlen(x) > qlen(y); });
+    uint64_t qtot = 0;
+    for (uint32_t k = 0; k < queries->count; ++k) qtot += qlen(k);
+    const uint64_t per_q = std::max<uint64_t>(1, qtot / queries->count) * std::max<uint64_t>(1, stot);
+    const uint32_t nsq = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(queries->count, target_cells / per_q));
+    std::vector<uint8_t> sq_res;
+    std::vector<int8_t> sq_bias;
+    std::vector<uint64_t> sq_off{0};
+    for (uint32_t k = 0; k < nsq; ++k) {
+        const uint32_t qi = qord[(uint64_t)k * queries->count / nsq];
+        const uint64_t a = queries->offsets[qi], b = queries->offsets[qi + 1];
+        sq_res.insert(sq_res.end(), queries->residues + a, queries->residues + b);
+        if (queries->bias) sq_bias.insert(sq_bias.end(), queries->bias + a, queries->bias + b);
+        sq_off.push_back(sq_res.size());
+    }
+    saber_hip_seqset sq{};
+    sq.residues = sq_res.data();
+    sq.offsets = sq_off.data();
+    sq.count = nsq;
+    sq.bias = queries->bias ? sq_bias.data() : nullptr;
+    const uint64_t sample_cells = sq_res.size() * stot;
+    auto trial = [&](double* ms_out) -> int {
+        saber_hip_full_batch* b = nullptr;
+        if (int r = saber_hip_full_batch_create(c, &sq, subjects, scheme, ends, &b)) return r;
+        const int r = saber_hip_full_batch_time(b, 1, 5, ms_out, nullptr, nullptr);
+        saber_hip_full_batch_destroy(b);
+        return r;
+    };
+    // Coordinate descent: classes in decreasing share of the residues; per
+    // class every shape covering its longest subject up to twice its width,
+    // kept when the sample's pass gets faster by more than 1 %.
+    std::vector<int> order;
+    for (int v = 0; v < nshape; ++v)
+        if (res[v] * 100 >= stot && res[v] > 0) order.push_back(v);
+    std::sort(order.begin(), order.end(), [&](int x, int y) { return res[x] > res[y]; });
+    double best_ms = 0.0;
+    int status = trial(&best_ms);
+    std::vector<int> chosen(nshape, -1);
+    for (int v : order) {
+        if (status != SABER_HIP_OK) break;
+        chosen[v] = v;
+        const int Wv = sh[v].L * sh[v].C;
+        for (int u = 0; u < nshape; ++u) {
+            const int Wu = sh[u].L * sh[u].C;
+            if (u == v || (uint32_t)Wu < hi[v] || Wu > 2 * Wv) continue;
+            {
+                std::lock_guard<std::mutex> g(c->mu);
+                c->full_tuned[fl][v] = u;
+            }
+            double ms = 0.0;
+            status = trial(&ms);
+            if (status != SABER_HIP_OK) break;
+            if (ms < best_ms * 0.99) {
+                best_ms = ms;
+                chosen[v] = u;
+            }
+        }
+        std::lock_guard<std::mutex> g(c->mu);
+        if (chosen[v] != v)
+            c->full_tuned[fl][v] = chosen[v];
+        else
+            c->full_tuned[fl].erase(v);
+    }
+    if (status != SABER_HIP_OK) {
+        std::lock_guard<std::mutex> g(c->mu);
+        c->full_tuned[fl] = saved;
+        return status;
+    }
+    int n = 0;
+    for (int v : order) {
+        if (report && n < nreport) {
+            saber_hip_tune_entry& e = report[n];
+            e = saber_hip_tune_entry{};
+            e.w_lo = (int32_t)lo[v];
+            e.w_hi = (int32_t)hi[v];
+            e.L = sh[chosen[v]].L;
+            e.C = sh[chosen[v]].C;
+            e.K = 0;
+            e.reserved = chosen[v] != v;
+            e.cells = (uint64_t)sq_res.size() * res[v];
+            e.gcups = best_ms > 0 ? (double)sample_cells / (best_ms * 1e-3) / 1e9 : 0.0;
+        }
+        ++n;
+    }
+    if (nclasses) *nclasses = n;
+    return SABER_HIP_OK;
+}
+
 int saber_hip_score_full(saber_hip_ctx* c, const saber_hip_seqset* queries,
                          const saber_hip_seqset* subjects, const saber_hip_scheme* scheme,
                          int32_t* score, uint32_t* q_end, uint32_t* s_end) {

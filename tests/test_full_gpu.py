@@ -187,3 +187,28 @@ def test_graph_replays_and_int32_paths(ctx, ends):
     finally:
         b.close()
         other.close()
+
+
+def test_autotune_choice_is_exact():
+    """The measured shape choice (saber_hip_full_autotune) picks, per group of
+    subject lengths, shapes that cover the group's longest subject, and full
+    batches planned with it stay bit-exact."""
+    c = S.Context()
+    qres, qoff = workload.sequences(21, 4, 250, 0.6, 30, 3000)
+    sres, soff = workload.sequences(22, 800, 250, 0.6, 30, 3000)
+    sch = S.Scheme(BLOSUM62, 11, 1)
+    try:
+        rep = c.full_autotune(qres, qoff, sres, soff, sch, ends=False, target_cells=100_000_000)
+        assert rep, "no length group was tuned"
+        for e in rep:
+            L, Cw = e["shape"]
+            assert e["len_lo"] <= e["len_hi"] <= L * Cw, e
+            assert e["gcups"] > 0 and e["cells"] > 0
+        got = c.score_full(qres, qoff, sres, soff, sch, ends=False)[0]
+    finally:
+        c.close()
+    qs = [qres[qoff[i]:qoff[i + 1]] for i in range(len(qoff) - 1)]
+    ss = [sres[soff[i]:soff[i + 1]] for i in range(len(soff) - 1)]
+    want = oracle_grid(qs, ss, BLOSUM62, 11, 1)[0]
+    bad = np.argwhere(got.astype(np.int64) != want)
+    assert len(bad) == 0, f"score differs at {bad[:5].tolist()}"
