@@ -171,16 +171,24 @@ __device__ __attribute__((noinline)) void full_group(const FullKernelArgs& a, co
             }
         }
         if (r >= 0 && r < m) {
-            v2s E = inE, rm = ZERO, hd = pair(diagIn);
+            v2s E = inE, rm = ZERO;
             const char* row = Tpb + rb;
-#pragma unroll
-            for (int c = 0; c < C; ++c) {
+            auto score = [&](int c) {
                 const uint32_t e = (E2[c >> 1] >> (16 * (c & 1))) & 0xFFFFu;
                 v2s P;
                 __builtin_memcpy(&P, row + e, 4);
                 if constexpr (BIAS) P = vadd(P, bpair);
-                const v2s D = vadd(hd, P);
-                hd = H[c];
+                return P;
+            };
+            // The diagonal term of column c + 1 is formed from the old H[c]
+            // before column c's new H is: the old value's last use comes
+            // first, so the new one can take its register (no moves to
+            // restore the registers at the end of the row).
+            v2s Dn = vadd(pair(diagIn), score(0));
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                const v2s D = Dn;
+                if (c + 1 < C) Dn = vadd(H[c], score(c + 1));
                 const v2s hv = vmax(vmax(D, F[c]), E);
                 const v2s hg = vsubz(hv, GO);
                 E = vmax(hg, vsubz(E, GE));
