@@ -154,6 +154,36 @@ int saber_hip_band_batch_time(saber_hip_band_batch* b, int warmup, int iters, do
                               double* band_ms, uint64_t* variant_cells, int* nvar,
                               uint64_t* cells);
 
+/* Banded traceback: banded_align_until (src/algorithm/banded.rs:472-521)
+ * for each job -- the alignment of the job's best band cell, found by the
+ * scalar recurrence of banded_align_scalar (banded.rs:227-333), the row stop of
+ * the AVX2 kernels the reference dispatches to (banded_simd.rs:324, 594: rows
+ * after the first holding band cells whose running best reaches stop_at[j]
+ * are not computed; stop_at == NULL: none) and walk (banded.rs:336-403);
+ * Engine::traceback (src/search/engine.rs:1328-1393) passes the window's exact
+ * score as stop_at. Status per job: 0 alignment (LocalAlignment), 1 none
+ * (Ok(None): empty input or band, best <= 0), 2 the direction matrix (query
+ * length x clipped band width bytes) exceeds max_dir_bytes (Err). Query
+ * coordinates and window coordinates (subject position - win_lo) are
+ * half-open; the run-length operations ('M', 'I', 'D' with lengths, in
+ * alignment order) of job j are written at out[j].op_off, which is the sum of
+ * query + window lengths of the jobs before it: op_codes / op_lens need that
+ * sum over all jobs (op_capacity). Runs on the context stream; blocking. */
+typedef struct saber_hip_alignment {
+    int32_t status;
+    int32_t score;
+    uint32_t q_start, q_end;
+    uint32_t s_start, s_end;
+    uint32_t nops;
+    uint32_t reserved;
+    uint64_t op_off;
+    uint64_t cells;
+} saber_hip_alignment;
+int saber_hip_band_align(saber_hip_ctx* ctx, const saber_hip_seqset* queries, const saber_hip_seqset* subjects,
+                         const saber_hip_band_job* jobs, const int32_t* stop_at, uint64_t njobs,
+                         const saber_hip_scheme* scheme, uint64_t max_dir_bytes, saber_hip_alignment* out,
+                         uint8_t* op_codes, uint32_t* op_lens, uint64_t op_capacity);
+
 /* Exhaustive mode (score_many over whole subjects, src/algorithm/sw_simd.rs:95-153,
  * with sw_reference's end cell, src/algorithm/reference.rs:15-52; the
  * exhaustive preset, src/search/presets.rs:318-331): every query against

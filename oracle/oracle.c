@@ -153,6 +153,152 @@ void or_banded_score(const uint8_t* q, int64_t m, const uint8_t* s, int64_t n,
     }
 }
 
+/* banded.rs:227-333 with the AVX2 row stop (banded_simd.rs:324, 594), walk
+ * banded.rs:336-403. */
+int or_band_align(const uint8_t* q, int64_t m, const uint8_t* s, int64_t n, const int8_t* bias,
+                  const int32_t* mat, int32_t go, int32_t ge, int64_t band_lo, int64_t band_hi,
+                  int32_t stop_at, uint64_t max_dir_bytes, int32_t* score, int64_t* q_start,
+                  int64_t* q_end, int64_t* s_start, int64_t* s_end, uint64_t* cells,
+                  uint8_t* op_codes, uint32_t* op_lens, int64_t* nops) {
+    *score = 0;
+    *q_start = *q_end = *s_start = *s_end = 0;
+    *cells = 0;
+    *nops = 0;
+    if (m == 0 || n == 0 || band_hi < band_lo) return 1;
+    int64_t lo = band_lo > -(m - 1) ? band_lo : -(m - 1);
+    int64_t hi = band_hi < n - 1 ? band_hi : n - 1;
+    int64_t w = hi - lo + 1;
+    if (w <= 0) return 1;
+    if ((uint64_t)m * (uint64_t)w > max_dir_bytes) return 2;
+    uint8_t* dir = (uint8_t*)calloc((size_t)(m * w), 1);
+    int32_t* hp = (int32_t*)malloc(sizeof(int32_t) * (size_t)(w + 1));
+    int32_t* fp = (int32_t*)malloc(sizeof(int32_t) * (size_t)(w + 1));
+    int32_t* hc = (int32_t*)malloc(sizeof(int32_t) * (size_t)(w + 1));
+    int32_t* fc = (int32_t*)malloc(sizeof(int32_t) * (size_t)(w + 1));
+    for (int64_t k = 0; k <= w; k++) hp[k] = fp[k] = hc[k] = fc[k] = NEG32;
+    int32_t best = 0;
+    int64_t bi = 0, bk = 0;
+    uint64_t nc = 0;
+    for (int64_t i = 0; i < m; i++) {
+        int64_t kmin, kmax;
+        const int has_cells = row_range(i, lo, w, n, &kmin, &kmax);
+        if (!has_cells) {
+            for (int64_t k = 0; k < w; k++) hc[k] = fc[k] = NEG32;
+        } else {
+            const int32_t b = bias ? bias[i] : 0;
+            int32_t e = NEG32, hl = NEG32;
+            uint8_t* drow = dir + i * w;
+            for (int64_t k = kmin; k <= kmax; k++) {
+                const int64_t j = i + lo + k;
+                const int32_t eo = hl - go, ee = e - ge;
+                const int e_ext = ee > eo;
+                e = k > kmin ? imax(ee, eo) : NEG32;
+                const int32_t fo = hp[k + 1] - go, fe = fp[k + 1] - ge;
+                const int f_ext = fe > fo;
+                const int32_t f = i >= 1 ? imax(fe, fo) : NEG32;
+                const int32_t hd = (i >= 1 && j >= 1) ? hp[k] : 0;
+                const int32_t diag = hd + S(mat, q[i], s[j]) + b;
+                const int c1 = diag > 0;
+                const int32_t h1 = imax(diag, 0);
+                const int c2 = e > h1;
+                const int32_t h2 = imax(h1, e);
+                const int c3 = f > h2;
+                const int32_t h = imax(h2, f);
+                const int org = c3 ? 3 : (c2 ? 2 : c1);
+                hc[k] = h;
+                fc[k] = f;
+                drow[k] = (uint8_t)(org | (e_ext << 2) | (f_ext << 3));
+                hl = h;
+                if (h > best) {
+                    best = h;
+                    bi = i;
+                    bk = k;
+                }
+            }
+            nc += (uint64_t)(kmax - kmin + 1);
+            for (int64_t k = 0; k < kmin; k++) hc[k] = fc[k] = NEG32;
+            for (int64_t k = kmax + 1; k < w; k++) hc[k] = fc[k] = NEG32;
+        }
+        int32_t* t = hp;
+        hp = hc;
+        hc = t;
+        t = fp;
+        fp = fc;
+        fc = t;
+        /* the AVX2 kernels test the stop after rows holding band cells only */
+        if (has_cells && best >= stop_at) break;
+    }
+    free(hp);
+    free(fp);
+    free(hc);
+    free(fc);
+    *cells = nc;
+    if (best <= 0) {
+        free(dir);
+        return 1;
+    }
+    /* walk: ops collected from the end cell backwards */
+    uint8_t* rev = (uint8_t*)malloc((size_t)(m + n + 1));
+    int64_t nrev = 0;
+    int64_t i = bi, k = bk;
+    int state = 0; /* 0 = H, 1 = E (query gap, 'D'), 2 = F (subject gap, 'I') */
+    for (;;) {
+        const uint8_t d = dir[i * w + k];
+        if (state == 0) {
+            const int org = d & 3;
+            if (org == 0) break;
+            if (org == 1) {
+                rev[nrev++] = 'M';
+                const int64_t j = i + lo + k;
+                if (i == 0 || j == 0) break;
+                i -= 1;
+            } else if (org == 2) {
+                state = 1;
+            } else {
+                state = 2;
+            }
+        } else if (state == 1) {
+            rev[nrev++] = 'D';
+            k -= 1;
+            if (!(d & 4)) state = 0;
+        } else {
+            rev[nrev++] = 'I';
+            i -= 1;
+            k += 1;
+            if (!(d & 8)) state = 0;
+        }
+    }
+    const int64_t qe = bi + 1, se = bi + lo + bk + 1;
+    int64_t qs = qe, ss = se, nr = 0;
+    for (int64_t x = nrev - 1; x >= 0; x--) {
+        const uint8_t op = rev[x];
+        if (op == 'M') {
+            qs--;
+            ss--;
+        } else if (op == 'I') {
+            qs--;
+        } else {
+            ss--;
+        }
+        if (nr > 0 && op_codes[nr - 1] == op) {
+            op_lens[nr - 1]++;
+        } else {
+            op_codes[nr] = op;
+            op_lens[nr] = 1;
+            nr++;
+        }
+    }
+    free(rev);
+    free(dir);
+    *score = best;
+    *q_start = qs;
+    *q_end = qe;
+    *s_start = ss;
+    *s_end = se;
+    *nops = nr;
+    return 0;
+}
+
 /* Independent (i, j)-coordinate formulation: H/E/F over the whole (m+1)x(n+1)
  * matrix, cells outside the band (or its clip) are unreachable. */
 void or_banded_score_ij(const uint8_t* q, int64_t m, const uint8_t* s, int64_t n,

@@ -26,6 +26,15 @@
  *                         sums), brute-force bound best_ungapped_on_diagonal,
  *                         and a direct restatement of the scalar twin
  *                         (engine.rs:2460-2494) that must agree.
+ *   or_band_align         its score, end cell and cells equal or_banded_score
+ *                         (full band: or_sw_reference); the CIGAR rescores
+ *                         (an independent column-by-column evaluation in the
+ *                         tests, reference.rs:58-105) to the score and ends
+ *                         at the end cell; every column stays inside the band;
+ *                         stopping at the known score gives the same
+ *                         alignment with no more cells (banded.rs:734-758);
+ *                         the memory cap is reported (banded.rs:760-767);
+ *                         Wx4000 in band +-20 = 44 000 as 4000M.
  *   or_composition_bias   the reference's invariants (non-standard residues
  *                         get 0; poly-Q <= -3; diverse mean < 1.5) plus an
  *                         O(n*w) direct evaluation of the formula.
@@ -54,6 +63,21 @@ void or_banded_score(const uint8_t* q, int64_t m, const uint8_t* s, int64_t n,
                      const int8_t* bias, const int32_t* mat, int32_t go, int32_t ge,
                      int64_t band_lo, int64_t band_hi,
                      int32_t* score, int64_t* q_end, int64_t* s_end, uint64_t* cells);
+
+/* Banded SW with traceback: src/algorithm/banded.rs:227-333
+ * (banded_align_scalar: direction bits org | e_ext << 2 | f_ext << 3, ties
+ * diagonal > E > F, extension only when strictly better) with the row stop of
+ * the AVX2 kernels the dispatcher runs (banded_simd.rs:324, 594: rows after
+ * the first one whose running best reaches stop_at are not computed), then
+ * walk (banded.rs:336-403). Returns 0 with an alignment, 1 when there is none
+ * (empty input, empty band, best score <= 0), 2 when m * w direction bytes
+ * exceed max_dir_bytes. ops: run-length codes 'M' / 'I' / 'D' in alignment
+ * order; capacity m + n runs. */
+int or_band_align(const uint8_t* q, int64_t m, const uint8_t* s, int64_t n, const int8_t* bias,
+                  const int32_t* mat, int32_t go, int32_t ge, int64_t band_lo, int64_t band_hi,
+                  int32_t stop_at, uint64_t max_dir_bytes, int32_t* score, int64_t* q_start,
+                  int64_t* q_end, int64_t* s_start, int64_t* s_end, uint64_t* cells,
+                  uint8_t* op_codes, uint32_t* op_lens, int64_t* nops);
 
 /* Independent check of or_banded_score: the same local DP written in (i, j)
  * matrix coordinates with -inf outside lo <= j-i <= hi (O(m*n) memory). */

@@ -45,6 +45,10 @@ def lib():
         L.or_best_ungapped_on_diagonal.argtypes = [u8p, C.c_int64, u8p, C.c_int64, C.c_int64, i32p, i8p]
         L.or_best_ungapped_on_diagonal.restype = C.c_int64
         L.or_composition_bias.argtypes = [u8p, C.c_int64, i32p, i8p]
+        L.or_band_align.argtypes = [u8p, C.c_int64, u8p, C.c_int64, i8p, i32p, C.c_int32, C.c_int32,
+                                    C.c_int64, C.c_int64, C.c_int32, C.c_uint64, i32p, i64p, i64p, i64p,
+                                    i64p, u64p, u8p, u32p, i64p]
+        L.or_band_align.restype = C.c_int
         _lib = L
     return _lib
 
@@ -186,3 +190,27 @@ def composition_bias(q, matrix):
     out = np.zeros(len(q), np.int8)
     lib().or_composition_bias(_p(q, C.c_uint8), len(q), _p(mat, C.c_int32), _p(out, C.c_int8))
     return out
+
+
+def band_align(q, s, matrix, bias, go, ge, band_lo, band_hi, stop_at=2**31 - 1, max_dir_bytes=2**63):
+    """banded_align_until (banded.rs:472-521): None when there is no
+    alignment, "cap" when the direction matrix exceeds max_dir_bytes, else
+    (score, q_start, q_end, s_start, s_end, ops, cells) with ops a list of
+    (op, length), op in b"MID"."""
+    q, s, mat, b = _u8(q), _u8(s), _mat(matrix), _bias(bias)
+    cap = len(q) + len(s) + 1
+    codes = np.zeros(cap, np.uint8)
+    lens = np.zeros(cap, np.uint32)
+    sc = C.c_int32()
+    qs, qe, ss, se, nops = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64()
+    cells = C.c_uint64()
+    r = lib().or_band_align(_p(q, C.c_uint8), len(q), _p(s, C.c_uint8), len(s), _p(b, C.c_int8),
+                            _p(mat, C.c_int32), go, ge, band_lo, band_hi, int(stop_at), int(max_dir_bytes),
+                            C.byref(sc), C.byref(qs), C.byref(qe), C.byref(ss), C.byref(se), C.byref(cells),
+                            _p(codes, C.c_uint8), _p(lens, C.c_uint32), C.byref(nops))
+    if r == 1:
+        return None
+    if r == 2:
+        return "cap"
+    ops = [(bytes([int(codes[k])]), int(lens[k])) for k in range(nops.value)]
+    return (sc.value, qs.value, qe.value, ss.value, se.value, ops, cells.value)

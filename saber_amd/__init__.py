@@ -338,6 +338,43 @@ class Context:
         return [{"w_lo": e.w_lo, "w_hi": e.w_hi, "shape": (e.L, e.C, e.K), "changed": bool(e.reserved),
                  "cells": int(e.cells), "gcups": round(e.gcups, 1)} for e in rep[:min(n.value, 64)]]
 
+    def band_align(self, qres, qoff, sres, soff, jobs, scheme, stop_at=None, qbias=None,
+                   max_dir_bytes=32 << 20):
+        """Banded traceback per job (saber_hip_band_align: banded_align_until,
+        banded.rs:472-521; the default cap is the engine's
+        traceback_max_bytes, engine.rs:191). One entry per job: None (no
+        alignment), "cap" (direction matrix over max_dir_bytes) or
+        (score, q_start, q_end, s_start, s_end, ops, cells) with window
+        coordinates and ops a list of (op, length), op in b"MID"."""
+        qs, ss, keep = self._seqsets(qres, qoff, qbias, sres, soff)
+        jobs = np.ascontiguousarray(jobs)
+        n = len(jobs)
+        qo = np.asarray(qoff, np.int64)
+        so = np.asarray(soff if soff is not None else [0], np.int64)
+        m = (qo[jobs["query"].astype(np.int64) + 1] - qo[jobs["query"].astype(np.int64)]) if n else np.zeros(0)
+        w = jobs["win_hi"].astype(np.int64) - jobs["win_lo"].astype(np.int64) if n else np.zeros(0)
+        cap = int((m + w).sum()) if n else 0
+        out = (_ffi.Alignment * max(n, 1))()
+        codes = np.zeros(max(cap, 1), np.uint8)
+        lens = np.zeros(max(cap, 1), np.uint32)
+        st = None if stop_at is None else np.ascontiguousarray(stop_at, dtype=np.int32)
+        sch = scheme.ffi()
+        _ffi.check(_ffi.lib().saber_hip_band_align(
+            self.h, C.byref(qs), None if ss is None else C.byref(ss), jobs.ctypes.data_as(C.POINTER(_ffi.BandJob)),
+            _ffi.ptr(st, C.c_int32), n, C.byref(sch), int(max_dir_bytes), out, _ffi.ptr(codes, C.c_uint8),
+            _ffi.ptr(lens, C.c_uint32), cap))
+        res = []
+        for k in range(n):
+            o = out[k]
+            if o.status == 1:
+                res.append(None)
+            elif o.status == 2:
+                res.append("cap")
+            else:
+                ops = [(bytes([int(codes[o.op_off + x])]), int(lens[o.op_off + x])) for x in range(o.nops)]
+                res.append((o.score, o.q_start, o.q_end, o.s_start, o.s_end, ops, o.cells))
+        return res
+
     def full_autotune(self, qres, qoff, sres, soff, scheme, qbias=None, ends=False, target_cells=50_000_000_000):
         """Measured full-matrix shape per subject-length group on a sample of
         whole queries (saber_hip_full_autotune); applies to full batches of

@@ -66,6 +66,12 @@ class TuneEntry(C.Structure):
                 ("K", C.c_int32), ("reserved", C.c_int32), ("cells", C.c_uint64), ("gcups", C.c_double)]
 
 
+class Alignment(C.Structure):
+    _fields_ = [("status", C.c_int32), ("score", C.c_int32), ("q_start", C.c_uint32), ("q_end", C.c_uint32),
+                ("s_start", C.c_uint32), ("s_end", C.c_uint32), ("nops", C.c_uint32), ("reserved", C.c_uint32),
+                ("op_off", C.c_uint64), ("cells", C.c_uint64)]
+
+
 class PlanParams(C.Structure):
     _fields_ = [("window_slack", C.c_int64), ("score_band", C.c_int64), ("hsp_threshold", C.c_int32),
                 ("exhaustive", C.c_int32)]
@@ -97,6 +103,10 @@ _SIGS = {
                                         C.c_uint64, C.POINTER(Scheme), C.POINTER(C.c_int32),
                                         C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                                         C.POINTER(C.c_uint64)]),
+    "saber_hip_band_align": (C.c_int, [P, C.POINTER(SeqSet), C.POINTER(SeqSet), C.POINTER(BandJob),
+                                       C.POINTER(C.c_int32), C.c_uint64, C.POINTER(Scheme), C.c_uint64,
+                                       C.POINTER(Alignment), C.POINTER(C.c_uint8), C.POINTER(C.c_uint32),
+                                       C.c_uint64]),
     "saber_hip_band_batch_create": (C.c_int, [P, C.POINTER(SeqSet), C.POINTER(SeqSet),
                                               C.POINTER(BandJob), C.c_uint64, C.POINTER(Scheme),
                                               C.c_int, C.POINTER(P)]),

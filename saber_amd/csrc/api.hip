@@ -1022,6 +1022,191 @@ int saber_hip_score_bands(saber_hip_ctx* c, const saber_hip_seqset* queries,
     return r;
 }
 
+int saber_hip_band_align(saber_hip_ctx* c, const saber_hip_seqset* queries, const saber_hip_seqset* subjects,
+                         const saber_hip_band_job* jobs, const int32_t* stop_at, uint64_t njobs,
+                         const saber_hip_scheme* scheme, uint64_t max_dir_bytes, saber_hip_alignment* out,
+                         uint8_t* op_codes, uint32_t* op_lens, uint64_t op_capacity) {
+    if (!c || !scheme) return fail(SABER_HIP_EINVAL, "null argument");
+    if (njobs == 0) return SABER_HIP_OK;
+    if (!jobs || !out) return fail(SABER_HIP_EINVAL, "null argument");
+    if (njobs >= (1ull << 32)) return fail(SABER_HIP_EINVAL, "too many jobs");
+    if (int r = check_seqset(queries, "queries")) return r;
+    if (subjects)
+        if (int r = check_seqset(subjects, "subjects")) return r;
+    std::lock_guard<std::mutex> g(c->mu);
+    HIP_TRY(hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    const uint64_t* soffs = subjects ? subjects->offsets : c->db_off.data();
+    const uint32_t nsub = subjects ? subjects->count : (c->db_off.empty() ? 0 : (uint32_t)c->db_off.size() - 1);
+    if (!subjects && !c->db.p) return fail(SABER_HIP_ESTATE, "no database uploaded");
+    // Jobs: clipped bands (Band::clip, banded.rs:45-47), the reference's
+    // memory check on m * w direction bytes, output operation offsets.
+    std::vector<BandJobDev> dj(njobs);
+    std::vector<uint64_t> op_off(njobs);
+    std::vector<uint8_t> capped(njobs, 0);
+    uint64_t ops_total = 0;
+    for (uint64_t k = 0; k < njobs; ++k) {
+        const saber_hip_band_job& J = jobs[k];
+        if (J.query >= queries->count) return fail(SABER_HIP_EINVAL, "job query index out of range");
+        if (J.subject >= nsub) return fail(SABER_HIP_EINVAL, "job subject index out of range");
+        const uint64_t slen = soffs[J.subject + 1] - soffs[J.subject];
+        if (J.win_hi < J.win_lo || J.win_hi > slen) return fail(SABER_HIP_EINVAL, "job window outside its subject");
+        const int64_t m = (int64_t)(queries->offsets[J.query + 1] - queries->offsets[J.query]);
+        const int64_t n = (int64_t)J.win_hi - J.win_lo;
+        BandJobDev d{};
+        d.q_off = queries->offsets[J.query];
+        d.s_off = soffs[J.subject] + J.win_lo;
+        d.m = (uint32_t)m;
+        d.n = (uint32_t)n;
+        op_off[k] = ops_total;
+        ops_total += (uint64_t)(m + n);
+        if (m > 0 && n > 0 && J.band_hi >= J.band_lo) {
+            const int64_t lo = std::max<int64_t>(J.band_lo, -(m - 1));
+            const int64_t hi = std::min<int64_t>(J.band_hi, n - 1);
+            const int64_t w = hi - lo + 1;
+            if (w > 0) {
+                if ((uint64_t)m * (uint64_t)w > max_dir_bytes) {
+                    capped[k] = 1;
+                } else {
+                    const int64_t i_first = std::max<int64_t>(0, -hi);
+                    const int64_t i_last = std::min<int64_t>(m - 1, n - 1 - lo);
+                    d.lo = (int32_t)lo;
+                    d.w = (int32_t)w;
+                    d.i_first = (int32_t)i_first;
+                    d.rows = (int32_t)(i_last >= i_first ? i_last - i_first + 1 : 0);
+                }
+            }
+        }
+        dj[k] = d;
+    }
+    if (ops_total > op_capacity || (ops_total > 0 && (!op_codes || !op_lens)))
+        return fail(SABER_HIP_EINVAL, "operation buffers need the sum over jobs of query + window length entries");
+    DevBuf<uint8_t> q, sbuf;
+    DevBuf<int8_t> bias;
+    DevBuf<BandJobDev> djobs;
+    DevBuf<int32_t> mat, stops;
+    DevBuf<AlignRecord> recs;
+    const uint64_t nqres = queries->offsets[queries->count];
+    if (int r = upload_codes(q, queries->residues, nqres, st)) return r;
+    if (queries->bias)
+        if (int r = upload_guarded(bias, queries->bias, nqres, st)) return r;
+    const uint8_t* sres = nullptr;
+    if (subjects) {
+        if (int r = upload_codes(sbuf, subjects->residues, subjects->offsets[subjects->count], st)) return r;
+        sres = sbuf.p + kGuard;
+    } else {
+        sres = c->db.p + kGuard;
+    }
+    if (int r = upload(djobs, dj.data(), njobs, st)) return r;
+    if (int r = upload(mat, scheme->matrix, 576, st)) return r;
+    if (stop_at)
+        if (int r = upload(stops, stop_at, njobs, st)) return r;
+    HIP_TRY(recs.alloc(njobs));
+    // Launches of jobs whose scratch (work rows, direction bytes, operation
+    // bytes) fits a budget.
+    constexpr uint64_t kBudget = 512ull << 20;
+    std::vector<AlignRecord> hrec(njobs);
+    std::vector<uint32_t> ids;
+    std::vector<uint64_t> woff, doff, ooff;
+    DevBuf<uint32_t> dids;
+    DevBuf<uint64_t> dwoff, ddoff, dooff;
+    DevBuf<int32_t> work;
+    DevBuf<uint8_t> dir, raw, codes;
+    DevBuf<uint32_t> lens;
+    std::vector<uint8_t> hcodes;
+    std::vector<uint32_t> hlens;
+    uint64_t k = 0;
+    while (k < njobs) {
+        ids.clear();
+        woff.clear();
+        doff.clear();
+        ooff.clear();
+        uint64_t wt = 0, dt = 0, ot = 0;
+        for (; k < njobs; ++k) {
+            const BandJobDev& d = dj[k];
+            if (capped[k] || d.w <= 0 || d.rows <= 0) continue;
+            const uint64_t wb = 16ull * ((uint64_t)d.w + 1), db = (uint64_t)d.rows * (uint64_t)d.w,
+                           ob = (uint64_t)d.m + d.n;
+            if (!ids.empty() && (wt + dt + 6 * ot + wb + db + 6 * ob > kBudget || ids.size() >= (1u << 20))) break;
+            ids.push_back((uint32_t)k);
+            woff.push_back(wt / 4);
+            doff.push_back(dt);
+            ooff.push_back(ot);
+            wt += wb;
+            dt += db;
+            ot += ob;
+        }
+        if (ids.empty()) break;
+        if (int r = upload(dids, ids.data(), ids.size(), st)) return r;
+        if (int r = upload(dwoff, woff.data(), woff.size(), st)) return r;
+        if (int r = upload(ddoff, doff.data(), doff.size(), st)) return r;
+        if (int r = upload(dooff, ooff.data(), ooff.size(), st)) return r;
+        if (work.n < wt / 4) HIP_TRY(work.alloc(wt / 4));
+        if (dir.n < dt) HIP_TRY(dir.alloc(dt));
+        if (raw.n < ot) {
+            HIP_TRY(raw.alloc(ot));
+            HIP_TRY(codes.alloc(ot));
+            HIP_TRY(lens.alloc(ot));
+        }
+        AlignArgs a{};
+        a.ids = dids.p;
+        a.n = (uint32_t)ids.size();
+        a.jobs = djobs.p;
+        a.qres = q.p + kGuard;
+        a.qbias = queries->bias ? bias.p + kGuard : nullptr;
+        a.sres = sres;
+        a.mat = mat.p;
+        a.go = scheme->gap_open;
+        a.ge = scheme->gap_extend;
+        a.stop_at = stop_at ? stops.p : nullptr;
+        a.work_off = dwoff.p;
+        a.work = work.p;
+        a.dir_off = ddoff.p;
+        a.dir = dir.p;
+        a.op_off = dooff.p;
+        a.op_raw = raw.p;
+        a.op_codes = codes.p;
+        a.op_lens = lens.p;
+        a.out = recs.p;
+        HIP_TRY(launch_band_align(a, st));
+        hcodes.resize(ot);
+        hlens.resize(ot);
+        HIP_TRY(hipMemcpyAsync(hcodes.data(), codes.p, ot, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(hlens.data(), lens.p, 4 * ot, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(hrec.data(), recs.p, sizeof(AlignRecord) * njobs, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        for (size_t x = 0; x < ids.size(); ++x) {
+            const uint32_t id = ids[x];
+            const AlignRecord& r = hrec[id];
+            if (r.status == 0 && r.nops) {
+                std::memcpy(op_codes + op_off[id], hcodes.data() + ooff[x], r.nops);
+                std::memcpy(op_lens + op_off[id], hlens.data() + ooff[x], 4ull * r.nops);
+            }
+            saber_hip_alignment& o = out[id];
+            o = saber_hip_alignment{};
+            o.status = r.status;
+            o.score = r.score;
+            o.q_start = r.q_start;
+            o.q_end = r.q_end;
+            o.s_start = r.s_start;
+            o.s_end = r.s_end;
+            o.nops = r.status == 0 ? r.nops : 0;
+            o.op_off = op_off[id];
+            o.cells = r.cells;
+        }
+    }
+    // jobs that did not run: capped (status 2) or without band cells (status 1)
+    for (uint64_t j = 0; j < njobs; ++j) {
+        const BandJobDev& d = dj[j];
+        if (capped[j] || d.w <= 0 || d.rows <= 0) {
+            out[j] = saber_hip_alignment{};
+            out[j].status = capped[j] ? 2 : 1;
+            out[j].op_off = op_off[j];
+        }
+    }
+    return SABER_HIP_OK;
+}
+
 int saber_hip_score_windows(saber_hip_ctx* c, const uint8_t* query, uint32_t qlen,
                             const int8_t* bias, const uint8_t* const* windows,
                             const uint32_t* lens, uint32_t n, const saber_hip_scheme* scheme,

@@ -147,6 +147,39 @@ namespace saber {
 
 // Full-matrix (exhaustive) kernel over the cross product of queries and
 // subjects; outputs are [query slot][subject].
+// Banded traceback (align.hip): one record per job.
+struct AlignRecord {
+    int32_t status;             // 0 alignment, 1 none (empty band or best <= 0)
+    int32_t score;
+    uint32_t q_start, q_end, s_start, s_end;
+    uint32_t nops;              // run-length operations at the job's op offset
+    uint32_t pad;
+    uint64_t cells;
+};
+
+struct AlignArgs {
+    const uint32_t* ids;        // jobs of this launch
+    uint32_t n;
+    const BandJobDev* jobs;
+    const uint8_t* qres;
+    const int8_t* qbias;
+    const uint8_t* sres;
+    const int32_t* mat;         // 24 x 24 int32
+    int32_t go, ge;
+    const int32_t* stop_at;     // per job, or null
+    const uint64_t* work_off;   // per launch thread: 4 * (w + 1) int32
+    int32_t* work;
+    const uint64_t* dir_off;    // per launch thread: rows * w bytes
+    uint8_t* dir;
+    const uint64_t* op_off;     // per launch thread: m + n entries
+    uint8_t* op_raw;
+    uint8_t* op_codes;
+    uint32_t* op_lens;
+    AlignRecord* out;           // indexed by job id
+};
+
+hipError_t launch_band_align(const AlignArgs& a, hipStream_t s);
+
 constexpr int kMaxFullSegs = 32;   // segments per register class (shapes)
 
 // A segment of the full-matrix chunk list: one shape's subject pairs against
