@@ -22,6 +22,8 @@ The printed JSON line also carries
                 (oracle/cpu_avx2.c) on all host threads over a bounded sample
                 of whole queries of the same batch;
   extra.xdrop   X-drop extension of 4 seed hits per region (steps/s);
+  extra.traceback  banded traceback of the best-scoring regions, stopping at
+                their exact score (alignments/s, the oracle on one core);
   extra.full_sw exhaustive full-matrix SW (queries x subjects, GCUPS; scores
                 only, as score_many, and with the end cell; shapes changed by
                 the autotuner);
@@ -168,6 +170,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the planner's modelled shape choice instead of measuring it first")
     ap.add_argument("--xdrop-hits", type=int, default=4_000_000, help="seed hits for the X-drop leg")
+    ap.add_argument("--traceback", type=int, default=20_000,
+                    help="best-scoring regions aligned with traceback (0: skip the leg)")
     ap.add_argument("--check", type=int, default=3000, help="jobs checked against the oracle")
     ap.add_argument("--full-queries", type=int, default=20,
                     help="queries of the exhaustive-mode leg (x 100,000 subjects); 0 skips it")
@@ -253,6 +257,39 @@ def main():
     if mismatches:
         print(f"# PARITY FAILURE: {mismatches} mismatching fields in {len(idx)} sampled jobs",
               file=sys.stderr)
+
+    # Secondary: banded traceback (banded_align_until, the consumer of the band
+    # scores) of the best-scoring regions, as the engine runs it for reported
+    # hits: stop at the region's exact score (engine.rs:1360). Kernel time by
+    # HIP events; the oracle's scalar restatement timed on one core beside it.
+    tb = None
+    if args.traceback > 0:
+        ntb = min(args.traceback, args.regions)
+        top = np.sort(np.argsort(-score.astype(np.int64), kind="stable")[:ntb])
+        tjobs = d["jobs"][top]
+        tstop = score[top].astype(np.int32)
+        res, tms = ctx.band_align(d["qres"], d["qoff"], d["sres"], d["soff"], tjobs, sch, stop_at=tstop,
+                                  time_iters=3)
+        tcells = sum(r[6] for r in res if isinstance(r, tuple))
+        ti = rng.choice(ntb, size=min(300, ntb), replace=False)
+        tbad = 0
+        t0 = time.time()
+        for k in ti.tolist():
+            J = tjobs[k]
+            q = d["qres"][d["qoff"][J["query"]]:d["qoff"][J["query"] + 1]]
+            s0 = d["soff"][J["subject"]]
+            w = O.band_align(q, d["sres"][s0 + J["win_lo"]:s0 + J["win_hi"]], BLOSUM62, None, 11, 1,
+                             int(J["band_lo"]), int(J["band_hi"]), int(tstop[k]), 32 << 20)
+            tbad += int(w != res[k])
+        cpu_per = (time.time() - t0) / len(ti)
+        tb = {"alignments": int(ntb), "kernel_ms": round(tms, 3),
+              "alignments_per_s": round(ntb / (tms * 1e-3), 1), "cells": int(tcells),
+              "gcups": round(tcells / (tms * 1e-3) / 1e9, 2),
+              "cpu_oracle_alignments_per_s_1core": round(1.0 / cpu_per, 1),
+              "parity_sample": int(len(ti)), "parity_mismatches": tbad}
+        print(f"# traceback {ntb} alignments: kernel {tms:.3f} ms ({tb['alignments_per_s']:.0f}/s, "
+              f"{tb['gcups']} GCUPS), oracle 1 core {tb['cpu_oracle_alignments_per_s_1core']}/s, "
+              f"mismatches {tbad}/{len(ti)}", file=sys.stderr, flush=True)
 
     # Secondary: X-drop ungapped extension over seed hits of the same batch
     # (work unit: extension steps, xdrop_steps of scan.rs:902).
@@ -390,7 +427,7 @@ def main():
         "extra": {"autotune": tune,
                   "band_with_end_cells": {"ms_per_step": round(ems, 4), "launch_ms": round(eband_ms, 4),
                                           "gcups": round(cells / (ems * 1e-3) / 1e9, 2)},
-                  "xdrop": xd, "full_sw": fl, "prefilter": pf},
+                  "xdrop": xd, "traceback": tb, "full_sw": fl, "prefilter": pf},
     }
     print(json.dumps(out), flush=True)
     batch.close()

@@ -183,6 +183,13 @@ int saber_hip_band_align(saber_hip_ctx* ctx, const saber_hip_seqset* queries, co
                          const saber_hip_band_job* jobs, const int32_t* stop_at, uint64_t njobs,
                          const saber_hip_scheme* scheme, uint64_t max_dir_bytes, saber_hip_alignment* out,
                          uint8_t* op_codes, uint32_t* op_lens, uint64_t op_capacity);
+/* saber_hip_band_align, also timing iters repetitions of each kernel launch
+ * with HIP events: *kernel_ms = their mean summed over the launches. */
+int saber_hip_band_align_time(saber_hip_ctx* ctx, const saber_hip_seqset* queries, const saber_hip_seqset* subjects,
+                              const saber_hip_band_job* jobs, const int32_t* stop_at, uint64_t njobs,
+                              const saber_hip_scheme* scheme, uint64_t max_dir_bytes, saber_hip_alignment* out,
+                              uint8_t* op_codes, uint32_t* op_lens, uint64_t op_capacity, int iters,
+                              double* kernel_ms);
 
 /* Exhaustive mode (score_many over whole subjects, src/algorithm/sw_simd.rs:95-153,
  * with sw_reference's end cell, src/algorithm/reference.rs:15-52; the

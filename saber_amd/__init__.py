@@ -339,13 +339,14 @@ class Context:
                  "cells": int(e.cells), "gcups": round(e.gcups, 1)} for e in rep[:min(n.value, 64)]]
 
     def band_align(self, qres, qoff, sres, soff, jobs, scheme, stop_at=None, qbias=None,
-                   max_dir_bytes=32 << 20):
+                   max_dir_bytes=32 << 20, time_iters=0):
         """Banded traceback per job (saber_hip_band_align: banded_align_until,
         banded.rs:472-521; the default cap is the engine's
         traceback_max_bytes, engine.rs:191). One entry per job: None (no
         alignment), "cap" (direction matrix over max_dir_bytes) or
         (score, q_start, q_end, s_start, s_end, ops, cells) with window
-        coordinates and ops a list of (op, length), op in b"MID"."""
+        coordinates and ops a list of (op, length), op in b"MID". With
+        time_iters > 0, returns (list, mean kernel ms)."""
         qs, ss, keep = self._seqsets(qres, qoff, qbias, sres, soff)
         jobs = np.ascontiguousarray(jobs)
         n = len(jobs)
@@ -359,10 +360,14 @@ class Context:
         lens = np.zeros(max(cap, 1), np.uint32)
         st = None if stop_at is None else np.ascontiguousarray(stop_at, dtype=np.int32)
         sch = scheme.ffi()
-        _ffi.check(_ffi.lib().saber_hip_band_align(
-            self.h, C.byref(qs), None if ss is None else C.byref(ss), jobs.ctypes.data_as(C.POINTER(_ffi.BandJob)),
-            _ffi.ptr(st, C.c_int32), n, C.byref(sch), int(max_dir_bytes), out, _ffi.ptr(codes, C.c_uint8),
-            _ffi.ptr(lens, C.c_uint32), cap))
+        args = (self.h, C.byref(qs), None if ss is None else C.byref(ss),
+                jobs.ctypes.data_as(C.POINTER(_ffi.BandJob)), _ffi.ptr(st, C.c_int32), n, C.byref(sch),
+                int(max_dir_bytes), out, _ffi.ptr(codes, C.c_uint8), _ffi.ptr(lens, C.c_uint32), cap)
+        kms = C.c_double()
+        if time_iters > 0:
+            _ffi.check(_ffi.lib().saber_hip_band_align_time(*args, int(time_iters), C.byref(kms)))
+        else:
+            _ffi.check(_ffi.lib().saber_hip_band_align(*args))
         res = []
         for k in range(n):
             o = out[k]
@@ -373,7 +378,7 @@ class Context:
             else:
                 ops = [(bytes([int(codes[o.op_off + x])]), int(lens[o.op_off + x])) for x in range(o.nops)]
                 res.append((o.score, o.q_start, o.q_end, o.s_start, o.s_end, ops, o.cells))
-        return res
+        return (res, kms.value) if time_iters > 0 else res
 
     def full_autotune(self, qres, qoff, sres, soff, scheme, qbias=None, ends=False, target_cells=50_000_000_000):
         """Measured full-matrix shape per subject-length group on a sample of

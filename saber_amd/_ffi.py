@@ -107,6 +107,10 @@ _SIGS = {
                                        C.POINTER(C.c_int32), C.c_uint64, C.POINTER(Scheme), C.c_uint64,
                                        C.POINTER(Alignment), C.POINTER(C.c_uint8), C.POINTER(C.c_uint32),
                                        C.c_uint64]),
+    "saber_hip_band_align_time": (C.c_int, [P, C.POINTER(SeqSet), C.POINTER(SeqSet), C.POINTER(BandJob),
+                                            C.POINTER(C.c_int32), C.c_uint64, C.POINTER(Scheme), C.c_uint64,
+                                            C.POINTER(Alignment), C.POINTER(C.c_uint8), C.POINTER(C.c_uint32),
+                                            C.c_uint64, C.c_int, C.POINTER(C.c_double)]),
     "saber_hip_band_batch_create": (C.c_int, [P, C.POINTER(SeqSet), C.POINTER(SeqSet),
                                               C.POINTER(BandJob), C.c_uint64, C.POINTER(Scheme),
                                               C.c_int, C.POINTER(P)]),

@@ -7,9 +7,10 @@
 // row holding band cells whose running best reaches stop_at), and walk
 // (banded.rs:336-403).
 //
-// One thread per job, exact int32: the two band rows in a per-job slice of a
-// work buffer, the direction bytes of the job's band rows (rows i_first ..
-// i_first + rows - 1, w bytes each) in a per-job slice of a direction buffer.
+// One thread per job, exact int32: the two band rows in LDS or a per-job
+// slice of a work buffer (below), the direction bytes of the job's band rows
+// (rows i_first .. i_first + rows - 1, w bytes each) in a per-job slice of a
+// direction buffer.
 // The walk writes the reversed operations into a per-job byte slice of m + n
 // bytes, then the run-length operations in alignment order at the same
 // offset of the output arrays.
@@ -21,7 +22,20 @@ constexpr int32_t kNegAlign = INT32_MIN / 4;
 
 __device__ __forceinline__ int32_t amax(int32_t a, int32_t b) { return a > b ? a : b; }
 
-__global__ __launch_bounds__(128) void band_align_kernel(AlignArgs a) {
+// Band rows updated in place: cell k of the current row reads the previous
+// row's H[k] (diagonal) and H[k+1], F[k+1] (from above) before it writes H[k],
+// F[k]; cells below the row's first band cell are never valid (the first
+// cell only moves left from row to row), cells past its last are reset after
+// the row. Jobs of band width <= kLdsWidth keep the two rows in LDS (2 KB a
+// thread), wider ones in a per-job slice of a work buffer. A group's subject
+// codes are one dword load (the residue buffers carry guard bytes past their
+// end), issued one group ahead.
+constexpr int kLdsWidth = kAlignLdsWidth;
+constexpr int kLdsThreads = 64;
+
+template <bool LDS>
+__global__ __launch_bounds__(kLdsThreads) void band_align_kernel(AlignArgs a) {
+    __shared__ int32_t lrows[LDS ? kLdsThreads * 2 * (kLdsWidth + 1) : 1];
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= a.n) return;
     const uint32_t id = a.ids[t];
@@ -32,16 +46,12 @@ __global__ __launch_bounds__(128) void band_align_kernel(AlignArgs a) {
         a.out[id] = rec;
         return;
     }
-    const int64_t w = J.w, lo = J.lo, n = J.n, m = J.m;
-    int32_t* hp = a.work + a.work_off[t];
-    int32_t* fp = hp + (w + 1);
-    int32_t* hc = fp + (w + 1);
-    int32_t* fc = hc + (w + 1);
+    const int64_t w = J.w, lo = J.lo, n = J.n;
+    int32_t* H = LDS ? lrows + threadIdx.x * 2 * (kLdsWidth + 1) : a.work + a.work_off[t];
+    int32_t* F = H + (LDS ? (kLdsWidth + 1) : (w + 1));
     for (int64_t k = 0; k <= w; ++k) {
-        hp[k] = kNegAlign;
-        fp[k] = kNegAlign;
-        hc[k] = kNegAlign;
-        fc[k] = kNegAlign;
+        H[k] = kNegAlign;
+        F[k] = kNegAlign;
     }
     uint8_t* dir = a.dir + a.dir_off[t];
     const uint8_t* q = a.qres + J.q_off;
@@ -52,6 +62,7 @@ __global__ __launch_bounds__(128) void band_align_kernel(AlignArgs a) {
     int32_t best = 0;
     int64_t bi = 0, bk = 0;
     uint64_t cells = 0;
+    int64_t kmax_prev = w - 1;
     const int64_t i_end = (int64_t)J.i_first + J.rows;
     for (int64_t i = J.i_first; i < i_end; ++i) {
         int64_t kmin = -(i + lo);
@@ -63,27 +74,26 @@ __global__ __launch_bounds__(128) void band_align_kernel(AlignArgs a) {
         const int32_t b = bias ? (int32_t)bias[i] : 0;
         uint8_t* drow = dir + (i - J.i_first) * w;
         int32_t e = kNegAlign, hl = kNegAlign;
-        for (int64_t k = 0; k < kmin; ++k) {
-            hc[k] = kNegAlign;
-            fc[k] = kNegAlign;
-        }
-        // cells in groups of four: the previous row's values and the subject
-        // codes of a group are loaded together before its cells are formed
+        int32_t hnext = H[kmin];   // previous row's H at the next cell
+        uint32_t snext;            // subject codes of the next group
+        __builtin_memcpy(&snext, s + (i + lo + kmin), 4);
         for (int64_t k0 = kmin; k0 <= kmax; k0 += 4) {
             const int nk = (int)(kmax - k0 + 1 < 4 ? kmax - k0 + 1 : 4);
             int32_t hpv[5], fpv[4];
             uint32_t scv[4];
-            // (indices clamped to the group: hp up to k0 + nk <= w, the rest
-            // up to its last cell)
+            const uint32_t sw = snext;
+            if (k0 + 4 <= kmax) __builtin_memcpy(&snext, s + (i + lo + k0 + 4), 4);
+            hpv[0] = hnext;
 #pragma unroll
-            for (int u = 0; u < 5; ++u) hpv[u] = hp[k0 + (u < nk ? u : nk)];
+            for (int u = 1; u < 5; ++u) hpv[u] = H[k0 + (u < nk ? u : nk)];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const int64_t k = k0 + (u < nk ? u : nk - 1);
-                fpv[u] = fp[k + 1];
-                const uint8_t sv = s[i + lo + k];
+                fpv[u] = F[k + 1];
+                const uint32_t sv = (sw >> (8 * u)) & 0xFFu;
                 scv[u] = sv < 24 ? sv : (uint32_t)kCodeX;
             }
+            hnext = hpv[nk];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 if (u >= nk) break;
@@ -104,8 +114,8 @@ __global__ __launch_bounds__(128) void band_align_kernel(AlignArgs a) {
                 const uint32_t c3 = f > h2;
                 const int32_t h = amax(h2, f);
                 const uint32_t org = c3 ? 3u : (c2 ? 2u : c1);
-                hc[k] = h;
-                fc[k] = f;
+                H[k] = h;
+                F[k] = f;
                 drow[k] = (uint8_t)(org | (e_ext << 2) | (f_ext << 3));
                 hl = h;
                 if (h > best) {
@@ -115,17 +125,13 @@ __global__ __launch_bounds__(128) void band_align_kernel(AlignArgs a) {
                 }
             }
         }
-        for (int64_t k = kmax + 1; k < w; ++k) {
-            hc[k] = kNegAlign;
-            fc[k] = kNegAlign;
+        // the previous row's cells past this row's last band cell
+        for (int64_t k = kmax + 1; k <= kmax_prev; ++k) {
+            H[k] = kNegAlign;
+            F[k] = kNegAlign;
         }
+        kmax_prev = kmax;
         cells += (uint64_t)(kmax - kmin + 1);
-        int32_t* x = hp;
-        hp = hc;
-        hc = x;
-        x = fp;
-        fp = fc;
-        fc = x;
         if (best >= stop) break;
     }
     rec.cells = cells;
@@ -194,10 +200,13 @@ __global__ __launch_bounds__(128) void band_align_kernel(AlignArgs a) {
     a.out[id] = rec;
 }
 
-hipError_t launch_band_align(const AlignArgs& a, hipStream_t s) {
+hipError_t launch_band_align(const AlignArgs& a, bool lds, hipStream_t s) {
     if (a.n == 0) return hipSuccess;
-    const uint32_t block = 128;
-    hipLaunchKernelGGL(band_align_kernel, dim3((a.n + block - 1) / block), dim3(block), 0, s, a);
+    const uint32_t grid = (a.n + kLdsThreads - 1) / kLdsThreads;
+    if (lds)
+        hipLaunchKernelGGL(band_align_kernel<true>, dim3(grid), dim3(kLdsThreads), 0, s, a);
+    else
+        hipLaunchKernelGGL(band_align_kernel<false>, dim3(grid), dim3(kLdsThreads), 0, s, a);
     return hipGetLastError();
 }
 

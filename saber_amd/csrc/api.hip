@@ -1022,10 +1022,18 @@ int saber_hip_score_bands(saber_hip_ctx* c, const saber_hip_seqset* queries,
     return r;
 }
 
-int saber_hip_band_align(saber_hip_ctx* c, const saber_hip_seqset* queries, const saber_hip_seqset* subjects,
-                         const saber_hip_band_job* jobs, const int32_t* stop_at, uint64_t njobs,
-                         const saber_hip_scheme* scheme, uint64_t max_dir_bytes, saber_hip_alignment* out,
-                         uint8_t* op_codes, uint32_t* op_lens, uint64_t op_capacity) {
+}  // extern "C"
+
+namespace {
+
+// saber_hip_band_align; iters > 0 also times iters repetitions of every
+// kernel launch with HIP events (the kernel is idempotent) and adds their mean
+// to *kernel_ms.
+int band_align_impl(saber_hip_ctx* c, const saber_hip_seqset* queries, const saber_hip_seqset* subjects,
+                    const saber_hip_band_job* jobs, const int32_t* stop_at, uint64_t njobs,
+                    const saber_hip_scheme* scheme, uint64_t max_dir_bytes, saber_hip_alignment* out,
+                    uint8_t* op_codes, uint32_t* op_lens, uint64_t op_capacity, int iters, double* kernel_ms) {
+    if (kernel_ms) *kernel_ms = 0.0;
     if (!c || !scheme) return fail(SABER_HIP_EINVAL, "null argument");
     if (njobs == 0) return SABER_HIP_OK;
     if (!jobs || !out) return fail(SABER_HIP_EINVAL, "null argument");
@@ -1103,7 +1111,8 @@ int saber_hip_band_align(saber_hip_ctx* c, const saber_hip_seqset* queries, cons
         if (int r = upload(stops, stop_at, njobs, st)) return r;
     HIP_TRY(recs.alloc(njobs));
     // Launches of jobs whose scratch (work rows, direction bytes, operation
-    // bytes) fits a budget.
+    // bytes) fits a budget; jobs of band width <= kAlignLdsWidth keep their
+    // rows in LDS, the others in the work buffer (separate launches).
     constexpr uint64_t kBudget = 512ull << 20;
     std::vector<AlignRecord> hrec(njobs);
     std::vector<uint32_t> ids;
@@ -1115,84 +1124,101 @@ int saber_hip_band_align(saber_hip_ctx* c, const saber_hip_seqset* queries, cons
     DevBuf<uint32_t> lens;
     std::vector<uint8_t> hcodes;
     std::vector<uint32_t> hlens;
-    uint64_t k = 0;
-    while (k < njobs) {
-        ids.clear();
-        woff.clear();
-        doff.clear();
-        ooff.clear();
-        uint64_t wt = 0, dt = 0, ot = 0;
-        for (; k < njobs; ++k) {
-            const BandJobDev& d = dj[k];
-            if (capped[k] || d.w <= 0 || d.rows <= 0) continue;
-            const uint64_t wb = 16ull * ((uint64_t)d.w + 1), db = (uint64_t)d.rows * (uint64_t)d.w,
-                           ob = (uint64_t)d.m + d.n;
-            if (!ids.empty() && (wt + dt + 6 * ot + wb + db + 6 * ob > kBudget || ids.size() >= (1u << 20))) break;
-            ids.push_back((uint32_t)k);
-            woff.push_back(wt / 4);
-            doff.push_back(dt);
-            ooff.push_back(ot);
-            wt += wb;
-            dt += db;
-            ot += ob;
-        }
-        if (ids.empty()) break;
-        if (int r = upload(dids, ids.data(), ids.size(), st)) return r;
-        if (int r = upload(dwoff, woff.data(), woff.size(), st)) return r;
-        if (int r = upload(ddoff, doff.data(), doff.size(), st)) return r;
-        if (int r = upload(dooff, ooff.data(), ooff.size(), st)) return r;
-        if (work.n < wt / 4) HIP_TRY(work.alloc(wt / 4));
-        if (dir.n < dt) HIP_TRY(dir.alloc(dt));
-        if (raw.n < ot) {
-            HIP_TRY(raw.alloc(ot));
-            HIP_TRY(codes.alloc(ot));
-            HIP_TRY(lens.alloc(ot));
-        }
-        AlignArgs a{};
-        a.ids = dids.p;
-        a.n = (uint32_t)ids.size();
-        a.jobs = djobs.p;
-        a.qres = q.p + kGuard;
-        a.qbias = queries->bias ? bias.p + kGuard : nullptr;
-        a.sres = sres;
-        a.mat = mat.p;
-        a.go = scheme->gap_open;
-        a.ge = scheme->gap_extend;
-        a.stop_at = stop_at ? stops.p : nullptr;
-        a.work_off = dwoff.p;
-        a.work = work.p;
-        a.dir_off = ddoff.p;
-        a.dir = dir.p;
-        a.op_off = dooff.p;
-        a.op_raw = raw.p;
-        a.op_codes = codes.p;
-        a.op_lens = lens.p;
-        a.out = recs.p;
-        HIP_TRY(launch_band_align(a, st));
-        hcodes.resize(ot);
-        hlens.resize(ot);
-        HIP_TRY(hipMemcpyAsync(hcodes.data(), codes.p, ot, hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipMemcpyAsync(hlens.data(), lens.p, 4 * ot, hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipMemcpyAsync(hrec.data(), recs.p, sizeof(AlignRecord) * njobs, hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipStreamSynchronize(st));
-        for (size_t x = 0; x < ids.size(); ++x) {
-            const uint32_t id = ids[x];
-            const AlignRecord& r = hrec[id];
-            if (r.status == 0 && r.nops) {
-                std::memcpy(op_codes + op_off[id], hcodes.data() + ooff[x], r.nops);
-                std::memcpy(op_lens + op_off[id], hlens.data() + ooff[x], 4ull * r.nops);
+    for (int pass = 0; pass < 2; ++pass) {
+        const bool lds = pass == 0;
+        uint64_t k = 0;
+        while (k < njobs) {
+            ids.clear();
+            woff.clear();
+            doff.clear();
+            ooff.clear();
+            uint64_t wt = 0, dt = 0, ot = 0;
+            for (; k < njobs; ++k) {
+                const BandJobDev& d = dj[k];
+                if (capped[k] || d.w <= 0 || d.rows <= 0 || (d.w <= kAlignLdsWidth) != lds) continue;
+                const uint64_t wb = lds ? 0 : 8ull * ((uint64_t)d.w + 1), db = (uint64_t)d.rows * (uint64_t)d.w,
+                               ob = (uint64_t)d.m + d.n;
+                if (!ids.empty() && (wt + dt + 6 * ot + wb + db + 6 * ob > kBudget || ids.size() >= (1u << 20))) break;
+                ids.push_back((uint32_t)k);
+                woff.push_back(wt / 4);
+                doff.push_back(dt);
+                ooff.push_back(ot);
+                wt += wb;
+                dt += db;
+                ot += ob;
             }
-            saber_hip_alignment& o = out[id];
-            o = saber_hip_alignment{};
-            o.status = r.status;
-            o.score = r.score;
-            o.q_start = r.q_start;
-            o.q_end = r.q_end;
-            o.s_start = r.s_start;
-            o.s_end = r.s_end;
-            o.nops = r.status == 0 ? r.nops : 0;
-            o.op_off = op_off[id];
-            o.cells = r.cells;
+            if (ids.empty()) break;
+            if (int r = upload(dids, ids.data(), ids.size(), st)) return r;
+            if (int r = upload(dwoff, woff.data(), woff.size(), st)) return r;
+            if (int r = upload(ddoff, doff.data(), doff.size(), st)) return r;
+            if (int r = upload(dooff, ooff.data(), ooff.size(), st)) return r;
+            if (wt && work.n < wt / 4) HIP_TRY(work.alloc(wt / 4));
+            if (dir.n < dt) HIP_TRY(dir.alloc(dt));
+            if (raw.n < ot) {
+                HIP_TRY(raw.alloc(ot));
+                HIP_TRY(codes.alloc(ot));
+                HIP_TRY(lens.alloc(ot));
+            }
+            AlignArgs a{};
+            a.ids = dids.p;
+            a.n = (uint32_t)ids.size();
+            a.jobs = djobs.p;
+            a.qres = q.p + kGuard;
+            a.qbias = queries->bias ? bias.p + kGuard : nullptr;
+            a.sres = sres;
+            a.mat = mat.p;
+            a.go = scheme->gap_open;
+            a.ge = scheme->gap_extend;
+            a.stop_at = stop_at ? stops.p : nullptr;
+            a.work_off = dwoff.p;
+            a.work = work.p;
+            a.dir_off = ddoff.p;
+            a.dir = dir.p;
+            a.op_off = dooff.p;
+            a.op_raw = raw.p;
+            a.op_codes = codes.p;
+            a.op_lens = lens.p;
+            a.out = recs.p;
+            HIP_TRY(launch_band_align(a, lds, st));
+            if (iters > 0 && kernel_ms) {
+                hipEvent_t e0, e1;
+                HIP_TRY(hipEventCreate(&e0));
+                HIP_TRY(hipEventCreate(&e1));
+                HIP_TRY(hipEventRecord(e0, st));
+                for (int it = 0; it < iters; ++it) HIP_TRY(launch_band_align(a, lds, st));
+                HIP_TRY(hipEventRecord(e1, st));
+                HIP_TRY(hipEventSynchronize(e1));
+                float ms = 0.f;
+                HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+                (void)hipEventDestroy(e0);
+                (void)hipEventDestroy(e1);
+                *kernel_ms += ms / iters;
+            }
+            hcodes.resize(ot);
+            hlens.resize(ot);
+            HIP_TRY(hipMemcpyAsync(hcodes.data(), codes.p, ot, hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipMemcpyAsync(hlens.data(), lens.p, 4 * ot, hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipMemcpyAsync(hrec.data(), recs.p, sizeof(AlignRecord) * njobs, hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipStreamSynchronize(st));
+            for (size_t x = 0; x < ids.size(); ++x) {
+                const uint32_t id = ids[x];
+                const AlignRecord& r = hrec[id];
+                if (r.status == 0 && r.nops) {
+                    std::memcpy(op_codes + op_off[id], hcodes.data() + ooff[x], r.nops);
+                    std::memcpy(op_lens + op_off[id], hlens.data() + ooff[x], 4ull * r.nops);
+                }
+                saber_hip_alignment& o = out[id];
+                o = saber_hip_alignment{};
+                o.status = r.status;
+                o.score = r.score;
+                o.q_start = r.q_start;
+                o.q_end = r.q_end;
+                o.s_start = r.s_start;
+                o.s_end = r.s_end;
+                o.nops = r.status == 0 ? r.nops : 0;
+                o.op_off = op_off[id];
+                o.cells = r.cells;
+            }
         }
     }
     // jobs that did not run: capped (status 2) or without band cells (status 1)
@@ -1205,6 +1231,28 @@ int saber_hip_band_align(saber_hip_ctx* c, const saber_hip_seqset* queries, cons
         }
     }
     return SABER_HIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int saber_hip_band_align(saber_hip_ctx* c, const saber_hip_seqset* queries, const saber_hip_seqset* subjects,
+                         const saber_hip_band_job* jobs, const int32_t* stop_at, uint64_t njobs,
+                         const saber_hip_scheme* scheme, uint64_t max_dir_bytes, saber_hip_alignment* out,
+                         uint8_t* op_codes, uint32_t* op_lens, uint64_t op_capacity) {
+    return band_align_impl(c, queries, subjects, jobs, stop_at, njobs, scheme, max_dir_bytes, out, op_codes,
+                           op_lens, op_capacity, 0, nullptr);
+}
+
+int saber_hip_band_align_time(saber_hip_ctx* c, const saber_hip_seqset* queries, const saber_hip_seqset* subjects,
+                              const saber_hip_band_job* jobs, const int32_t* stop_at, uint64_t njobs,
+                              const saber_hip_scheme* scheme, uint64_t max_dir_bytes, saber_hip_alignment* out,
+                              uint8_t* op_codes, uint32_t* op_lens, uint64_t op_capacity, int iters,
+                              double* kernel_ms) {
+    if (iters < 1 || !kernel_ms) return fail(SABER_HIP_EINVAL, "bad arguments");
+    return band_align_impl(c, queries, subjects, jobs, stop_at, njobs, scheme, max_dir_bytes, out, op_codes,
+                           op_lens, op_capacity, iters, kernel_ms);
 }
 
 int saber_hip_score_windows(saber_hip_ctx* c, const uint8_t* query, uint32_t qlen,

@@ -167,7 +167,7 @@ struct AlignArgs {
     const int32_t* mat;         // 24 x 24 int32
     int32_t go, ge;
     const int32_t* stop_at;     // per job, or null
-    const uint64_t* work_off;   // per launch thread: 4 * (w + 1) int32
+    const uint64_t* work_off;   // per launch thread: 2 * (w + 1) int32 (wide jobs)
     int32_t* work;
     const uint64_t* dir_off;    // per launch thread: rows * w bytes
     uint8_t* dir;
@@ -178,7 +178,10 @@ struct AlignArgs {
     AlignRecord* out;           // indexed by job id
 };
 
-hipError_t launch_band_align(const AlignArgs& a, hipStream_t s);
+// lds: every job of the launch has band width <= kAlignLdsWidth (rows kept
+// in LDS)
+hipError_t launch_band_align(const AlignArgs& a, bool lds, hipStream_t s);
+constexpr int kAlignLdsWidth = 255;
 
 constexpr int kMaxFullSegs = 32;   // segments per register class (shapes)
 
