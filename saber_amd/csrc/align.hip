@@ -7,10 +7,10 @@
 // row holding band cells whose running best reaches stop_at), and walk
 // (banded.rs:336-403).
 //
-// One thread per job, exact int32: the two band rows in LDS or a per-job
-// slice of a work buffer (below), the direction bytes of the job's band rows
-// (rows i_first .. i_first + rows - 1, w bytes each) in a per-job slice of a
-// direction buffer.
+// Band widths up to 1024 run L lanes per job (the lane-parallel kernel
+// below, direction rows of 16 L bytes), wider ones one thread per job (rows
+// of w bytes); the direction bytes of a job's band rows (rows i_first ..
+// i_first + rows - 1) go to a per-job slice of a direction buffer.
 // The walk writes the reversed operations into a per-job byte slice of m + n
 // bytes, then the run-length operations in alignment order at the same
 // offset of the output arrays.
@@ -22,20 +22,85 @@ constexpr int32_t kNegAlign = INT32_MIN / 4;
 
 __device__ __forceinline__ int32_t amax(int32_t a, int32_t b) { return a > b ? a : b; }
 
-// Band rows updated in place: cell k of the current row reads the previous
-// row's H[k] (diagonal) and H[k+1], F[k+1] (from above) before it writes H[k],
-// F[k]; cells below the row's first band cell are never valid (the first
-// cell only moves left from row to row), cells past its last are reset after
-// the row. Jobs of band width <= kLdsWidth keep the two rows in LDS (2 KB a
-// thread), wider ones in a per-job slice of a work buffer. A group's subject
-// codes are one dword load (the residue buffers carry guard bytes past their
-// end), issued one group ahead.
-constexpr int kLdsWidth = kAlignLdsWidth;
-constexpr int kLdsThreads = 64;
+// Jobs wider than the lane-parallel form (below): one thread per job, the two
+// band rows updated in place in a per-job slice of a work buffer (a cell
+// reads the previous row's H[k], H[k+1], F[k+1] before it writes H[k], F[k];
+// cells below the row's first band cell are never valid -- the first cell
+// only moves left from row to row -- and cells past its last are reset after
+// the row); cells four at a time, the previous row's values and one dword of
+// subject codes loaded together (the codes one group ahead; the residue
+// buffers carry guard bytes past their end).
+constexpr int kWideThreads = 64;
 
-template <bool LDS>
-__global__ __launch_bounds__(kLdsThreads) void band_align_kernel(AlignArgs a) {
-    __shared__ int32_t lrows[LDS ? kLdsThreads * 2 * (kLdsWidth + 1) : 1];
+__device__ void write_walk(const AlignArgs& a, uint32_t slot, const BandJobDev& J, const uint8_t* dir, int64_t stride,
+                           int32_t best, int64_t bi, int64_t bk, AlignRecord rec) {
+    const uint32_t J_id = a.ids[slot];
+    const int64_t lo = J.lo;
+    // walk: operations from the end cell backwards
+    uint8_t* rev = a.op_raw + a.op_off[slot];
+    int64_t nrev = 0;
+    int64_t i = bi, k = bk;
+    int state = 0;   // 0 = H, 1 = E (query gap, 'D'), 2 = F (subject gap, 'I')
+    for (;;) {
+        const uint8_t d = dir[(i - J.i_first) * stride + k];
+        if (state == 0) {
+            const uint32_t org = d & 3u;
+            if (org == 0) break;
+            if (org == 1) {
+                rev[nrev++] = 'M';
+                if (i == 0 || i + lo + k == 0) break;
+                i -= 1;
+            } else {
+                state = org == 2 ? 1 : 2;
+            }
+        } else if (state == 1) {
+            rev[nrev++] = 'D';
+            k -= 1;
+            if (!(d & 4u)) state = 0;
+        } else {
+            rev[nrev++] = 'I';
+            i -= 1;
+            k += 1;
+            if (!(d & 8u)) state = 0;
+        }
+    }
+    const int64_t qe = bi + 1, se = bi + lo + bk + 1;
+    int64_t qs = qe, ss = se;
+    uint8_t* oc = a.op_codes + a.op_off[slot];
+    uint32_t* ol = a.op_lens + a.op_off[slot];
+    uint32_t nr = 0;
+    for (int64_t x = nrev - 1; x >= 0; --x) {
+        const uint8_t op = rev[x];
+        if (op == 'M') {
+            --qs;
+            --ss;
+        } else if (op == 'I') {
+            --qs;
+        } else {
+            --ss;
+        }
+        if (nr > 0 && oc[nr - 1] == op) {
+            ++ol[nr - 1];
+        } else {
+            oc[nr] = op;
+            ol[nr] = 1;
+            ++nr;
+        }
+    }
+    rec.status = 0;
+    rec.score = best;
+    rec.q_start = (uint32_t)qs;
+    rec.q_end = (uint32_t)qe;
+    rec.s_start = (uint32_t)ss;
+    rec.s_end = (uint32_t)se;
+    rec.nops = nr;
+    a.out[J_id] = rec;
+}
+
+__global__ __launch_bounds__(kWideThreads) void band_align_kernel(AlignArgs a) {
+    __shared__ int32_t M[576];
+    for (int x = threadIdx.x; x < 576; x += blockDim.x) M[x] = a.mat[x];
+    __syncthreads();
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= a.n) return;
     const uint32_t id = a.ids[t];
@@ -47,8 +112,8 @@ __global__ __launch_bounds__(kLdsThreads) void band_align_kernel(AlignArgs a) {
         return;
     }
     const int64_t w = J.w, lo = J.lo, n = J.n;
-    int32_t* H = LDS ? lrows + threadIdx.x * 2 * (kLdsWidth + 1) : a.work + a.work_off[t];
-    int32_t* F = H + (LDS ? (kLdsWidth + 1) : (w + 1));
+    int32_t* H = a.work + a.work_off[t];
+    int32_t* F = H + (w + 1);
     for (int64_t k = 0; k <= w; ++k) {
         H[k] = kNegAlign;
         F[k] = kNegAlign;
@@ -70,7 +135,7 @@ __global__ __launch_bounds__(kLdsThreads) void band_align_kernel(AlignArgs a) {
         int64_t kmax = n - 1 - i - lo;
         if (kmax > w - 1) kmax = w - 1;
         const uint32_t qc = q[i] < 24 ? q[i] : (uint32_t)kCodeX;
-        const int32_t* row = a.mat + qc * 24;
+        const int32_t* row = M + qc * 24;
         const int32_t b = bias ? (int32_t)bias[i] : 0;
         uint8_t* drow = dir + (i - J.i_first) * w;
         int32_t e = kNegAlign, hl = kNegAlign;
@@ -139,74 +204,273 @@ __global__ __launch_bounds__(kLdsThreads) void band_align_kernel(AlignArgs a) {
         a.out[id] = rec;
         return;
     }
-    // walk: operations from the end cell backwards
-    uint8_t* rev = a.op_raw + a.op_off[t];
-    int64_t nrev = 0;
-    int64_t i = bi, k = bk;
-    int state = 0;   // 0 = H, 1 = E (query gap, 'D'), 2 = F (subject gap, 'I')
-    for (;;) {
-        const uint8_t d = dir[(i - J.i_first) * w + k];
-        if (state == 0) {
-            const uint32_t org = d & 3u;
-            if (org == 0) break;
-            if (org == 1) {
-                rev[nrev++] = 'M';
-                if (i == 0 || i + lo + k == 0) break;
-                i -= 1;
-            } else {
-                state = org == 2 ? 1 : 2;
-            }
-        } else if (state == 1) {
-            rev[nrev++] = 'D';
-            k -= 1;
-            if (!(d & 4u)) state = 0;
-        } else {
-            rev[nrev++] = 'I';
-            i -= 1;
-            k += 1;
-            if (!(d & 8u)) state = 0;
-        }
-    }
-    const int64_t qe = bi + 1, se = bi + lo + bk + 1;
-    int64_t qs = qe, ss = se;
-    uint8_t* oc = a.op_codes + a.op_off[t];
-    uint32_t* ol = a.op_lens + a.op_off[t];
-    uint32_t nr = 0;
-    for (int64_t x = nrev - 1; x >= 0; --x) {
-        const uint8_t op = rev[x];
-        if (op == 'M') {
-            --qs;
-            --ss;
-        } else if (op == 'I') {
-            --qs;
-        } else {
-            --ss;
-        }
-        if (nr > 0 && oc[nr - 1] == op) {
-            ++ol[nr - 1];
-        } else {
-            oc[nr] = op;
-            ol[nr] = 1;
-            ++nr;
-        }
-    }
-    rec.status = 0;
-    rec.score = best;
-    rec.q_start = (uint32_t)qs;
-    rec.q_end = (uint32_t)qe;
-    rec.s_start = (uint32_t)ss;
-    rec.s_end = (uint32_t)se;
-    rec.nops = nr;
-    a.out[id] = rec;
+    write_walk(a, t, J, dir, w, best, bi, bk, rec);
 }
 
-hipError_t launch_band_align(const AlignArgs& a, bool lds, hipStream_t s) {
-    if (a.n == 0) return hipSuccess;
-    const uint32_t grid = (a.n + kLdsThreads - 1) / kLdsThreads;
-    if (lds)
-        hipLaunchKernelGGL(band_align_kernel<true>, dim3(grid), dim3(kLdsThreads), 0, s, a);
+// Lane-parallel form for band widths up to 64 * 16: L lanes per job, lane l
+// owning band cells l*C .. l*C + C - 1 and working on row r = t - l at step t
+// (the scoring kernel's layout, in exact int32 with NEG outside the band). A
+// cell's diagonal and upper neighbours are the lane's own previous-row values
+// except the last cell's upper-right one, which is the right lane's first
+// cell of the row above -- computed at the same step, so every lane forms its
+// first cell first and the values cross lanes once, by DPP -- and the first
+// cell's left neighbour is the left lane's last cell of the same row (from the
+// previous step). Each lane keeps its first row-major best (value, row,
+// cell) and, for the row stop, a snapshot of it taken at the end of the first
+// row in which it reached stop_at: the row stop (first row holding a cell >=
+// stop_at) is the least such row over the lanes, and the best over the rows
+// up to it is among the snapshots of the lanes that reached it there. After a
+// device-scope fence, the group's first lane walks the direction bytes.
+constexpr int kLaneCells = 16;
+
+template <int L>
+__device__ __forceinline__ int32_t lane_from_prev(int32_t v) {
+    if constexpr (L <= 16)
+        return __builtin_amdgcn_update_dpp(0, v, 0x111, 0xF, 0xF, false);   // row_shr:1
     else
-        hipLaunchKernelGGL(band_align_kernel<false>, dim3(grid), dim3(kLdsThreads), 0, s, a);
+        return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xF, 0xF, false);   // wave_shr:1
+}
+template <int L>
+__device__ __forceinline__ int32_t lane_from_next(int32_t v) {
+    if constexpr (L <= 16)
+        return __builtin_amdgcn_update_dpp(0, v, 0x101, 0xF, 0xF, false);   // row_shl:1
+    else
+        return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xF, 0xF, false);   // wave_shl:1
+}
+
+// One job on the L lanes of a group: launch slot g, lane l (M: the
+// substitution matrix in LDS).
+template <int L>
+__device__ __forceinline__ void align_lanes_job(const AlignArgs& a, const int32_t* M, uint32_t g, int l) {
+    constexpr int C = kLaneCells;
+    const uint32_t id = a.ids[g];
+    const BandJobDev J = a.jobs[id];
+    const int32_t w = J.w, lo = J.lo, n = (int32_t)J.n, rows = J.rows, i_first = J.i_first;
+    constexpr int32_t W = L * C;   // direction-row stride: a lane's 16 bytes are one aligned store
+    const int32_t go = a.go, ge = a.ge;
+    const int32_t stop = a.stop_at ? a.stop_at[id] : INT32_MAX;
+    const uint8_t* q = a.qres + J.q_off;
+    const uint8_t* s = a.sres + J.s_off;
+    const int8_t* bias = a.qbias ? a.qbias + J.q_off : nullptr;
+    uint8_t* dir = a.dir + a.dir_off[g];
+    int32_t H[C], F[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        H[c] = kNegAlign;
+        F[c] = kNegAlign;
+    }
+    int32_t outE = kNegAlign, outH = kNegAlign;   // this lane's last cell, for the right lane
+    int32_t best = 0, bi = 0, bk = 0;             // first row-major best of this lane
+    int32_t sbest = 0, si = 0, sk = 0, srow = INT32_MAX;   // snapshot at the first row reaching stop
+    // subject bytes of a row: columns i + lo + l*C .. + C - 1, as C/4 + 1
+    // unaligned dwords clamped into the buffer's guard bytes (bytes outside
+    // the window belong to cells outside the band rectangle, never used)
+    auto load_cols = [&](int32_t j0, uint32_t (&v)[C / 4]) {
+#pragma unroll
+        for (int x = 0; x < C / 4; ++x) {
+            int32_t o = j0 + 4 * x;
+            o = o < -16 ? -16 : (o > n + 60 ? n + 60 : o);
+            __builtin_memcpy(&v[x], s + o, 4);
+        }
+    };
+    uint32_t sv[C / 4], svn[C / 4];
+    load_cols(i_first + lo + l * C - l, svn);
+    // query code and bias of the next row, loaded a step ahead (index
+    // clamped into the query)
+    const int32_t m = (int32_t)J.m;
+    auto qrow = [&](int32_t i, uint32_t& qc, int32_t& b) {
+        const int32_t ic = i < 0 ? 0 : (i >= m ? m - 1 : i);
+        const uint8_t qv = q[ic];
+        qc = qv < 24 ? qv : (uint32_t)kCodeX;
+        b = bias ? (int32_t)bias[ic] : 0;
+    };
+    uint32_t qc_next;
+    int32_t b_next;
+    qrow(i_first - l, qc_next, b_next);
+    const int32_t nsteps = rows + L - 1;
+    for (int32_t t = 0; t < nsteps; ++t) {
+        const int32_t r = t - l;
+        const bool act = r >= 0 && r < rows;
+        const int32_t i = i_first + r;
+#pragma unroll
+        for (int x = 0; x < C / 4; ++x) sv[x] = svn[x];
+        load_cols(i + 1 + lo + l * C, svn);   // next row's columns
+        // left lane's last cell of this row (previous step)
+        int32_t inE = lane_from_prev<L>(outE), inH = lane_from_prev<L>(outH);
+        if (l == 0) {
+            inE = kNegAlign;
+            inH = kNegAlign;
+        }
+        const uint32_t qc = qc_next;
+        const int32_t b = b_next;
+        qrow(i + 1, qc_next, b_next);
+        int32_t kmin = 0, kmax = -1;
+        if (act) {
+            kmin = -(i + lo);
+            if (kmin < 0) kmin = 0;
+            kmax = n - 1 - i - lo;
+            if (kmax > w - 1) kmax = w - 1;
+        }
+        const int32_t* row = M + qc * 24;
+        uint32_t d4[C / 4] = {};   // the lane's direction bytes of this row
+        int32_t e = inE, hl = inH;
+        int32_t rowbest = INT32_MIN;
+        // one cell: old H/F of the cell (diagonal) and of the next one (up)
+        auto cell = [&](int c, int32_t hup, int32_t fup) {
+            const int32_t k = l * C + c;
+            const bool valid = act && k >= kmin && k <= kmax;
+            const int32_t j = i + lo + k;
+            const int32_t eo = hl - go, ee = e - ge;
+            const uint32_t e_ext = ee > eo;
+            const int32_t ev = k > kmin ? amax(ee, eo) : kNegAlign;
+            const int32_t fo = hup - go, fe = fup - ge;
+            const uint32_t f_ext = fe > fo;
+            const int32_t f = i >= 1 ? amax(fe, fo) : kNegAlign;
+            const int32_t hd = (i >= 1 && j >= 1) ? H[c] : 0;
+            const uint32_t code = (sv[c >> 2] >> (8 * (c & 3))) & 0xFFu;
+            const int32_t diag = hd + row[code < 24 ? code : (uint32_t)kCodeX] + b;
+            const uint32_t c1 = diag > 0;
+            const int32_t h1 = amax(diag, 0);
+            const uint32_t c2 = ev > h1;
+            const int32_t h2 = amax(h1, ev);
+            const uint32_t c3 = f > h2;
+            const int32_t h = amax(h2, f);
+            const uint32_t org = c3 ? 3u : (c2 ? 2u : c1);
+            if (valid) {
+                H[c] = h;
+                F[c] = f;
+                e = ev;
+                hl = h;
+                d4[c >> 2] |= (org | (e_ext << 2) | (f_ext << 3)) << (8 * (c & 3));
+                if (h > best) {
+                    best = h;
+                    bi = i;
+                    bk = k;
+                }
+                rowbest = h > rowbest ? h : rowbest;
+            } else if (act) {
+                H[c] = kNegAlign;
+                F[c] = kNegAlign;
+                e = kNegAlign;
+                hl = kNegAlign;
+            }
+        };
+        // first cell, then its new values to the left lane (the cell above-
+        // right of that lane's last cell)
+        cell(0, H[1], F[1]);
+        int32_t upH = lane_from_next<L>(H[0]), upF = lane_from_next<L>(F[0]);
+        if (l == L - 1) {
+            upH = kNegAlign;
+            upF = kNegAlign;
+        }
+#pragma unroll
+        for (int c = 1; c < C - 1; ++c) cell(c, H[c + 1], F[c + 1]);
+        cell(C - 1, upH, upF);
+        if (act) {
+            *reinterpret_cast<uint4*>(dir + (int64_t)r * W + l * C) = make_uint4(d4[0], d4[1], d4[2], d4[3]);
+            outE = e;
+            outH = hl;
+            if (srow == INT32_MAX && best >= stop && rowbest != INT32_MIN) {
+                srow = i;
+                sbest = best;
+                si = bi;
+                sk = bk;
+            }
+        }
+    }
+    // the row stop and the group's best
+    int32_t stop_row = srow;
+#pragma unroll
+    for (int d = 1; d < L; d <<= 1) {
+        const int32_t o = __shfl_xor(stop_row, d, L);
+        stop_row = o < stop_row ? o : stop_row;
+    }
+    int32_t cv, ci, ck;
+    if (stop_row != INT32_MAX) {
+        const bool mine = srow == stop_row;
+        cv = mine ? sbest : INT32_MIN;
+        ci = si;
+        ck = sk;
+    } else {
+        cv = best;
+        ci = bi;
+        ck = bk;
+    }
+    int32_t top = cv;
+#pragma unroll
+    for (int d = 1; d < L; d <<= 1) {
+        const int32_t o = __shfl_xor(top, d, L);
+        top = o > top ? o : top;
+    }
+    uint32_t key = (cv == top && top > 0) ? (uint32_t)(ci - i_first) * (uint32_t)W + (uint32_t)ck : 0xFFFFFFFFu;
+#pragma unroll
+    for (int d = 1; d < L; d <<= 1) {
+        const uint32_t o = (uint32_t)__shfl_xor((int)key, d, L);
+        key = o < key ? o : key;
+    }
+    __threadfence();
+    if (l != 0) return;
+    // cells of the rows computed by the reference (up to the row stop)
+    const int64_t last_row = stop_row != INT32_MAX ? (int64_t)stop_row : i_first + rows - 1;
+    uint64_t cells = 0;
+    for (int64_t i = i_first; i <= last_row; ++i) {
+        int64_t kmin = -(i + lo);
+        if (kmin < 0) kmin = 0;
+        int64_t kmax = n - 1 - i - lo;
+        if (kmax > w - 1) kmax = w - 1;
+        cells += (uint64_t)(kmax - kmin + 1);
+    }
+    AlignRecord rec{};
+    rec.status = 1;
+    rec.cells = cells;
+    if (top <= 0) {
+        a.out[id] = rec;
+        return;
+    }
+    const int64_t bi0 = i_first + key / (uint32_t)W, bk0 = key % (uint32_t)W;
+    write_walk(a, g, J, dir, W, top, bi0, bk0, rec);
+}
+
+// Every lane class in one launch: blocks [block_start[x], block_start[x+1])
+// run class x (L = 1 << x lanes per job, 256 / L jobs a block), their jobs at
+// launch slots slot_start[x] .. slot_start[x + 1] - 1.
+__global__ __launch_bounds__(256) void band_align_lanes_kernel(AlignArgs a, AlignLaneClasses cls) {
+    __shared__ int32_t M[576];
+    for (int x = threadIdx.x; x < 576; x += blockDim.x) M[x] = a.mat[x];
+    __syncthreads();
+    int x = 0;
+    while (x + 1 < kAlignLaneClasses && blockIdx.x >= cls.block_start[x + 1]) ++x;
+    const uint32_t lb = blockIdx.x - cls.block_start[x];
+    const uint32_t end = cls.slot_start[x + 1];
+    switch (x) {
+#define SABER_ALIGN_CLASS(X, L)                                                              \
+    case X: {                                                                             \
+        const uint32_t g = cls.slot_start[X] + (lb * 256 + threadIdx.x) / (L);            \
+        if (g < end) align_lanes_job<L>(a, M, g, (int)(threadIdx.x % (L)));               \
+        break;                                                                            \
+    }
+        SABER_ALIGN_CLASS(0, 1)
+        SABER_ALIGN_CLASS(1, 2)
+        SABER_ALIGN_CLASS(2, 4)
+        SABER_ALIGN_CLASS(3, 8)
+        SABER_ALIGN_CLASS(4, 16)
+        SABER_ALIGN_CLASS(5, 32)
+        SABER_ALIGN_CLASS(6, 64)
+#undef SABER_ALIGN_CLASS
+    default: break;
+    }
+}
+
+hipError_t launch_band_align(const AlignArgs& a, hipStream_t s) {
+    if (a.n == 0) return hipSuccess;
+    hipLaunchKernelGGL(band_align_kernel, dim3((a.n + kWideThreads - 1) / kWideThreads), dim3(kWideThreads), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_band_align_lanes(const AlignArgs& a, const AlignLaneClasses& cls, hipStream_t s) {
+    const uint32_t blocks = cls.block_start[kAlignLaneClasses];
+    if (blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(band_align_lanes_kernel, dim3(blocks), dim3(256), 0, s, a, cls);
     return hipGetLastError();
 }
 

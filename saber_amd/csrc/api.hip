@@ -1111,8 +1111,8 @@ int band_align_impl(saber_hip_ctx* c, const saber_hip_seqset* queries, const sab
         if (int r = upload(stops, stop_at, njobs, st)) return r;
     HIP_TRY(recs.alloc(njobs));
     // Launches of jobs whose scratch (work rows, direction bytes, operation
-    // bytes) fits a budget; jobs of band width <= kAlignLdsWidth keep their
-    // rows in LDS, the others in the work buffer (separate launches).
+    // bytes) fits a budget, per lane class (jobs of band width <= 16 L on L
+    // lanes, wider ones on one thread with their rows in the work buffer).
     constexpr uint64_t kBudget = 512ull << 20;
     std::vector<AlignRecord> hrec(njobs);
     std::vector<uint32_t> ids;
@@ -1124,22 +1124,45 @@ int band_align_impl(saber_hip_ctx* c, const saber_hip_seqset* queries, const sab
     DevBuf<uint32_t> lens;
     std::vector<uint8_t> hcodes;
     std::vector<uint32_t> hlens;
-    for (int pass = 0; pass < 2; ++pass) {
-        const bool lds = pass == 0;
-        uint64_t k = 0;
-        while (k < njobs) {
+    // Lane class of a job: x for 1 << x lanes (band width <= 16 << x), or
+    // kAlignLaneClasses for one thread per job. Runnable jobs in class order,
+    // longest first within a class (long jobs start first).
+    auto class_of = [&](int32_t w) {
+        for (int x = 0; x < kAlignLaneClasses; ++x)
+            if (w <= kAlignLaneCells << x) return x;
+        return kAlignLaneClasses;
+    };
+    std::vector<uint32_t> order;
+    std::vector<uint8_t> cls(njobs, 0);
+    for (uint64_t j = 0; j < njobs; ++j) {
+        const BandJobDev& d = dj[j];
+        if (capped[j] || d.w <= 0 || d.rows <= 0) continue;
+        cls[j] = (uint8_t)class_of(d.w);
+        order.push_back((uint32_t)j);
+    }
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
+        return cls[x] != cls[y] ? cls[x] < cls[y] : dj[x].rows > dj[y].rows;
+    });
+    size_t pos = 0;
+    {
+        while (pos < order.size()) {
             ids.clear();
             woff.clear();
             doff.clear();
             ooff.clear();
             uint64_t wt = 0, dt = 0, ot = 0;
-            for (; k < njobs; ++k) {
+            const bool lanes = cls[order[pos]] < kAlignLaneClasses;
+            for (; pos < order.size(); ++pos) {
+                const uint32_t k = order[pos];
                 const BandJobDev& d = dj[k];
-                if (capped[k] || d.w <= 0 || d.rows <= 0 || (d.w <= kAlignLdsWidth) != lds) continue;
-                const uint64_t wb = lds ? 0 : 8ull * ((uint64_t)d.w + 1), db = (uint64_t)d.rows * (uint64_t)d.w,
+                if ((cls[k] < kAlignLaneClasses) != lanes) break;
+                // direction rows of 16 L bytes on L lanes (16-byte stores), w bytes on one thread
+                const uint64_t wb = lanes ? 0 : 8ull * ((uint64_t)d.w + 1),
+                               db = (uint64_t)d.rows * (lanes ? (uint64_t)kAlignLaneCells << cls[k] : (uint64_t)d.w),
                                ob = (uint64_t)d.m + d.n;
-                if (!ids.empty() && (wt + dt + 6 * ot + wb + db + 6 * ob > kBudget || ids.size() >= (1u << 20))) break;
-                ids.push_back((uint32_t)k);
+                if (!ids.empty() && (wt + dt + 6 * ot + wb + db + 6 * ob > kBudget || ids.size() >= (1u << 20)))
+                    break;
+                ids.push_back(k);
                 woff.push_back(wt / 4);
                 doff.push_back(dt);
                 ooff.push_back(ot);
@@ -1147,7 +1170,20 @@ int band_align_impl(saber_hip_ctx* c, const saber_hip_seqset* queries, const sab
                 dt += db;
                 ot += ob;
             }
-            if (ids.empty()) break;
+            AlignLaneClasses lc{};
+            if (lanes) {
+                uint32_t slot = 0, blk = 0;
+                for (int x = 0; x < kAlignLaneClasses; ++x) {
+                    lc.slot_start[x] = slot;
+                    lc.block_start[x] = blk;
+                    uint32_t cnt = 0;
+                    while (slot + cnt < ids.size() && cls[ids[slot + cnt]] == x) ++cnt;
+                    slot += cnt;
+                    blk += (uint32_t)(((uint64_t)cnt * (1u << x) + 255) / 256);
+                }
+                lc.slot_start[kAlignLaneClasses] = slot;
+                lc.block_start[kAlignLaneClasses] = blk;
+            }
             if (int r = upload(dids, ids.data(), ids.size(), st)) return r;
             if (int r = upload(dwoff, woff.data(), woff.size(), st)) return r;
             if (int r = upload(ddoff, doff.data(), doff.size(), st)) return r;
@@ -1179,13 +1215,14 @@ int band_align_impl(saber_hip_ctx* c, const saber_hip_seqset* queries, const sab
             a.op_codes = codes.p;
             a.op_lens = lens.p;
             a.out = recs.p;
-            HIP_TRY(launch_band_align(a, lds, st));
+            auto launch = [&]() { return lanes ? launch_band_align_lanes(a, lc, st) : launch_band_align(a, st); };
+            HIP_TRY(launch());
             if (iters > 0 && kernel_ms) {
                 hipEvent_t e0, e1;
                 HIP_TRY(hipEventCreate(&e0));
                 HIP_TRY(hipEventCreate(&e1));
                 HIP_TRY(hipEventRecord(e0, st));
-                for (int it = 0; it < iters; ++it) HIP_TRY(launch_band_align(a, lds, st));
+                for (int it = 0; it < iters; ++it) HIP_TRY(launch());
                 HIP_TRY(hipEventRecord(e1, st));
                 HIP_TRY(hipEventSynchronize(e1));
                 float ms = 0.f;

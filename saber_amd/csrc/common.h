@@ -167,7 +167,7 @@ struct AlignArgs {
     const int32_t* mat;         // 24 x 24 int32
     int32_t go, ge;
     const int32_t* stop_at;     // per job, or null
-    const uint64_t* work_off;   // per launch thread: 2 * (w + 1) int32 (wide jobs)
+    const uint64_t* work_off;   // per launch slot: 2 * (w + 1) int32 (one-thread jobs)
     int32_t* work;
     const uint64_t* dir_off;    // per launch thread: rows * w bytes
     uint8_t* dir;
@@ -178,10 +178,16 @@ struct AlignArgs {
     AlignRecord* out;           // indexed by job id
 };
 
-// lds: every job of the launch has band width <= kAlignLdsWidth (rows kept
-// in LDS)
-hipError_t launch_band_align(const AlignArgs& a, bool lds, hipStream_t s);
-constexpr int kAlignLdsWidth = 255;
+// One thread per job, rows in the work buffer (band widths over 16 * 64).
+hipError_t launch_band_align(const AlignArgs& a, hipStream_t s);
+constexpr int kAlignLaneCells = 16, kAlignMaxLanes = 64, kAlignLaneClasses = 7;
+// Lane classes of one launch (class x: 1 << x lanes per job): the launch's
+// slots and blocks of each class, as prefix offsets.
+struct AlignLaneClasses {
+    uint32_t slot_start[kAlignLaneClasses + 1];
+    uint32_t block_start[kAlignLaneClasses + 1];
+};
+hipError_t launch_band_align_lanes(const AlignArgs& a, const AlignLaneClasses& cls, hipStream_t s);
 
 constexpr int kMaxFullSegs = 32;   // segments per register class (shapes)
 
