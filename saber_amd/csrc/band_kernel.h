@@ -96,9 +96,7 @@ struct ByteStream {
     }
     __device__ __forceinline__ uint32_t load(int32_t o) const {
         o = o < lo ? lo : (o > hi ? hi : o);
-        uint32_t v;
-        __builtin_memcpy(&v, base + o, 4);
-        return v;
+        return gload_u32(base + o);
     }
 };
 

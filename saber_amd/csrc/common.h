@@ -11,6 +11,23 @@ namespace saber {
 // Alignment codes 0..23 (ARNDCQEGHILKMFPSTWYVBZX*); any code >= 24 scores as
 // X (src/alphabet.rs:39-41), so residues are stored canonical on upload.
 constexpr int kCodeX = 22;
+
+// An unaligned dword load from global memory issued as a global (not flat)
+// access. A generic pointer that reaches a non-inlined function is loaded
+// with flat instructions, which count against the LDS wait counter as well:
+// every wait for an LDS read would then also wait for this load.
+__device__ __forceinline__ uint32_t gload_u32(const void* p) {
+    struct __attribute__((packed)) U32 { uint32_t v; };
+    return ((const __attribute__((address_space(1))) U32*)p)->v;
+}
+
+// A byte pointer into LDS that keeps its address space across non-inlined
+// calls (a generic one is read there with flat loads), and an aligned dword
+// read through it.
+typedef const __attribute__((address_space(3))) char* lds_bytes;
+__device__ __forceinline__ uint32_t lds_u32(lds_bytes p) {
+    return *(const __attribute__((address_space(3))) uint32_t*)p;
+}
 constexpr short kMax16 = 32767;      // any job whose best reaches this is rescored in int32
 
 // Band kernel score table: int8, 24 query rows x 25 columns (subject codes

@@ -71,7 +71,7 @@ constexpr int kFullThreads = ENDS ? (C <= 24 ? 768 : C <= 32 ? 512 : 256) : (C <
 // One subject pair (pair index g of a.pairs, lane l of its L) against query
 // qid: the whole matrix, outputs at row qid of the [query][subject] arrays.
 template <int L, int C, bool BIAS, bool ENDS>
-__device__ __attribute__((noinline)) void full_group(const FullKernelArgs& a, const char* Tpb, uint32_t qid,
+__device__ __attribute__((noinline)) void full_group(const FullKernelArgs& a, lds_bytes Tpb, uint32_t qid,
                                                      uint32_t g, int l) {
     static_assert(C >= 2 && C % 2 == 0, "variant shape");
     static_assert(L == 1 || L == 2 || L == 4 || L == 8 || L == 16 || L == 32 || L == 64,
@@ -129,9 +129,7 @@ __device__ __attribute__((noinline)) void full_group(const FullKernelArgs& a, co
     auto load4 = [](const uint8_t* buf, int64_t off, uint64_t len) -> uint32_t {
         const int64_t top = (int64_t)len + 60;
         off = off < -16 ? -16 : (off > top ? top : off);
-        uint32_t v;
-        __builtin_memcpy(&v, buf + off, 4);
-        return v;
+        return gload_u32(buf + off);
     };
     // Query code / bias streams for rows 1, 2, ... (row 0 read directly).
     uint32_t q_cur = load4(a.qres, q0 + 1, a.qres_len), q_n1 = load4(a.qres, q0 + 5, a.qres_len),
@@ -172,11 +170,10 @@ __device__ __attribute__((noinline)) void full_group(const FullKernelArgs& a, co
         }
         if (r >= 0 && r < m) {
             v2s E = inE, rm = ZERO;
-            const char* row = Tpb + rb;
+            const lds_bytes row = Tpb + rb;
             auto score = [&](int c) {
                 const uint32_t e = (E2[c >> 1] >> (16 * (c & 1))) & 0xFFFFu;
-                v2s P;
-                __builtin_memcpy(&P, row + e, 4);
+                v2s P = __builtin_bit_cast(v2s, lds_u32(row + e));
                 if constexpr (BIAS) P = vadd(P, bpair);
                 return P;
             };
@@ -301,7 +298,7 @@ __global__ __launch_bounds__(NT) void full_multi_kernel(FullKernelArgs a) {
     for (int t = threadIdx.x; t < kPairWords; t += NT) TP[t] = a.pair_table[t];
     for (int t = threadIdx.x; t < (int)a.nsegs; t += NT) S[t] = a.segs[t];
     __syncthreads();
-    const char* Tpb = reinterpret_cast<const char*>(TP);
+    const lds_bytes Tpb = (lds_bytes)TP;
     const int lane = threadIdx.x & 63;
     for (;;) {
         uint32_t ci = 0;
