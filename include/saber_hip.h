@@ -168,7 +168,10 @@ int saber_hip_band_batch_time(saber_hip_band_batch* b, int warmup, int iters, do
  * half-open; the run-length operations ('M', 'I', 'D' with lengths, in
  * alignment order) of job j are written at out[j].op_off, which is the sum of
  * query + window lengths of the jobs before it: op_codes / op_lens need that
- * sum over all jobs (op_capacity). Runs on the context stream; blocking. */
+ * sum over all jobs (op_capacity). Runs on the context stream; blocking.
+ * Jobs run in launches whose device scratch (direction bytes, work rows,
+ * operation bytes) fits a budget: 2048 MiB, or the environment variable
+ * SABER_HIP_ALIGN_SCRATCH_MB (read at each call). */
 typedef struct saber_hip_alignment {
     int32_t status;
     int32_t score;

@@ -11,9 +11,10 @@
 // below, direction rows of 16 L bytes), wider ones one thread per job (rows
 // of w bytes); the direction bytes of a job's band rows (rows i_first ..
 // i_first + rows - 1) go to a per-job slice of a direction buffer.
-// The walk writes the reversed operations into a per-job byte slice of m + n
-// bytes, then the run-length operations in alignment order at the same
-// offset of the output arrays.
+// The walk merges equal operations into runs as it goes (in registers),
+// writes the runs in reverse order at the job's offset of the output arrays
+// and then reverses them in place: no value it stores is read back while
+// walking.
 #include "common.h"
 
 namespace saber {
@@ -36,9 +37,25 @@ __device__ void write_walk(const AlignArgs& a, uint32_t slot, const BandJobDev& 
                            int32_t best, int64_t bi, int64_t bk, AlignRecord rec) {
     const uint32_t J_id = a.ids[slot];
     const int64_t lo = J.lo;
-    // walk: operations from the end cell backwards
-    uint8_t* rev = a.op_raw + a.op_off[slot];
-    int64_t nrev = 0;
+    uint8_t* oc = a.op_codes + a.op_off[slot];
+    uint32_t* ol = a.op_lens + a.op_off[slot];
+    // walk from the end cell backwards; runs of equal operations in reverse
+    uint32_t nr = 0, run = 0;
+    uint8_t cur = 0;
+    int64_t nq = 0, ns = 0;   // query / window residues the operations consume
+    auto emit = [&](uint8_t op) {
+        if (op == cur) {
+            ++run;
+            return;
+        }
+        if (run) {
+            oc[nr] = cur;
+            ol[nr] = run;
+            ++nr;
+        }
+        cur = op;
+        run = 1;
+    };
     int64_t i = bi, k = bk;
     int state = 0;   // 0 = H, 1 = E (query gap, 'D'), 2 = F (subject gap, 'I')
     for (;;) {
@@ -47,51 +64,47 @@ __device__ void write_walk(const AlignArgs& a, uint32_t slot, const BandJobDev& 
             const uint32_t org = d & 3u;
             if (org == 0) break;
             if (org == 1) {
-                rev[nrev++] = 'M';
+                emit('M');
+                ++nq;
+                ++ns;
                 if (i == 0 || i + lo + k == 0) break;
                 i -= 1;
             } else {
                 state = org == 2 ? 1 : 2;
             }
         } else if (state == 1) {
-            rev[nrev++] = 'D';
+            emit('D');
+            ++ns;
             k -= 1;
             if (!(d & 4u)) state = 0;
         } else {
-            rev[nrev++] = 'I';
+            emit('I');
+            ++nq;
             i -= 1;
             k += 1;
             if (!(d & 8u)) state = 0;
         }
     }
-    const int64_t qe = bi + 1, se = bi + lo + bk + 1;
-    int64_t qs = qe, ss = se;
-    uint8_t* oc = a.op_codes + a.op_off[slot];
-    uint32_t* ol = a.op_lens + a.op_off[slot];
-    uint32_t nr = 0;
-    for (int64_t x = nrev - 1; x >= 0; --x) {
-        const uint8_t op = rev[x];
-        if (op == 'M') {
-            --qs;
-            --ss;
-        } else if (op == 'I') {
-            --qs;
-        } else {
-            --ss;
-        }
-        if (nr > 0 && oc[nr - 1] == op) {
-            ++ol[nr - 1];
-        } else {
-            oc[nr] = op;
-            ol[nr] = 1;
-            ++nr;
-        }
+    if (run) {
+        oc[nr] = cur;
+        ol[nr] = run;
+        ++nr;
     }
+    // alignment order
+    for (int64_t x = 0, y = (int64_t)nr - 1; x < y; ++x, --y) {
+        const uint8_t cx = oc[x], cy = oc[y];
+        const uint32_t lx = ol[x], ly = ol[y];
+        oc[x] = cy;
+        oc[y] = cx;
+        ol[x] = ly;
+        ol[y] = lx;
+    }
+    const int64_t qe = bi + 1, se = bi + lo + bk + 1;
     rec.status = 0;
     rec.score = best;
-    rec.q_start = (uint32_t)qs;
+    rec.q_start = (uint32_t)(qe - nq);
     rec.q_end = (uint32_t)qe;
-    rec.s_start = (uint32_t)ss;
+    rec.s_start = (uint32_t)(se - ns);
     rec.s_end = (uint32_t)se;
     rec.nops = nr;
     a.out[J_id] = rec;
@@ -251,7 +264,13 @@ __device__ __forceinline__ void align_lanes_job(const AlignArgs& a, const int32_
     const int32_t stop = a.stop_at ? a.stop_at[id] : INT32_MAX;
     const uint8_t* q = a.qres + J.q_off;
     const uint8_t* s = a.sres + J.s_off;
-    const int8_t* bias = a.qbias ? a.qbias + J.q_off : nullptr;
+    // The bias load is unconditional (from the query codes when there is no
+    // bias, the value then masked), and so is the direction store (steps
+    // outside the job's rows write a spare row after them): the number of
+    // memory operations per step is fixed, so waiting for the next row's codes
+    // does not wait for this row's store.
+    const bool has_bias = a.qbias != nullptr;
+    const int8_t* bias = has_bias ? a.qbias + J.q_off : reinterpret_cast<const int8_t*>(q);
     uint8_t* dir = a.dir + a.dir_off[g];
     int32_t H[C], F[C];
 #pragma unroll
@@ -281,8 +300,9 @@ __device__ __forceinline__ void align_lanes_job(const AlignArgs& a, const int32_
     auto qrow = [&](int32_t i, uint32_t& qc, int32_t& b) {
         const int32_t ic = i < 0 ? 0 : (i >= m ? m - 1 : i);
         const uint8_t qv = q[ic];
+        const int32_t bv = bias[ic];
         qc = qv < 24 ? qv : (uint32_t)kCodeX;
-        b = bias ? (int32_t)bias[ic] : 0;
+        b = has_bias ? bv : 0;
     };
     uint32_t qc_next;
     int32_t b_next;
@@ -311,11 +331,21 @@ __device__ __forceinline__ void align_lanes_job(const AlignArgs& a, const int32_
             kmax = n - 1 - i - lo;
             if (kmax > w - 1) kmax = w - 1;
         }
+        // the row's scores, read from LDS together before the cell chain
         const int32_t* row = M + qc * 24;
+        int32_t sc[C];
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            const uint32_t code = (sv[c >> 2] >> (8 * (c & 3))) & 0xFFu;
+            sc[c] = row[code < 24 ? code : (uint32_t)kCodeX] + b;
+        }
         uint32_t d4[C / 4] = {};   // the lane's direction bytes of this row
         int32_t e = inE, hl = inH;
-        int32_t rowbest = INT32_MIN;
-        // one cell: old H/F of the cell (diagonal) and of the next one (up)
+        bool has_cells = false;   // a band cell of this lane in this row
+        const bool up_ok = i >= 1;
+        // one cell: old H/F of the cell (diagonal) and of the next one (up);
+        // branch-free: a cell outside the band rectangle takes NEG on an
+        // active row and keeps its values on a lag step
         auto cell = [&](int c, int32_t hup, int32_t fup) {
             const int32_t k = l * C + c;
             const bool valid = act && k >= kmin && k <= kmax;
@@ -325,10 +355,9 @@ __device__ __forceinline__ void align_lanes_job(const AlignArgs& a, const int32_
             const int32_t ev = k > kmin ? amax(ee, eo) : kNegAlign;
             const int32_t fo = hup - go, fe = fup - ge;
             const uint32_t f_ext = fe > fo;
-            const int32_t f = i >= 1 ? amax(fe, fo) : kNegAlign;
-            const int32_t hd = (i >= 1 && j >= 1) ? H[c] : 0;
-            const uint32_t code = (sv[c >> 2] >> (8 * (c & 3))) & 0xFFu;
-            const int32_t diag = hd + row[code < 24 ? code : (uint32_t)kCodeX] + b;
+            const int32_t f = up_ok ? amax(fe, fo) : kNegAlign;
+            const int32_t hd = (up_ok && j >= 1) ? H[c] : 0;
+            const int32_t diag = hd + sc[c];
             const uint32_t c1 = diag > 0;
             const int32_t h1 = amax(diag, 0);
             const uint32_t c2 = ev > h1;
@@ -336,24 +365,17 @@ __device__ __forceinline__ void align_lanes_job(const AlignArgs& a, const int32_
             const uint32_t c3 = f > h2;
             const int32_t h = amax(h2, f);
             const uint32_t org = c3 ? 3u : (c2 ? 2u : c1);
-            if (valid) {
-                H[c] = h;
-                F[c] = f;
-                e = ev;
-                hl = h;
-                d4[c >> 2] |= (org | (e_ext << 2) | (f_ext << 3)) << (8 * (c & 3));
-                if (h > best) {
-                    best = h;
-                    bi = i;
-                    bk = k;
-                }
-                rowbest = h > rowbest ? h : rowbest;
-            } else if (act) {
-                H[c] = kNegAlign;
-                F[c] = kNegAlign;
-                e = kNegAlign;
-                hl = kNegAlign;
-            }
+            const int32_t hv = valid ? h : kNegAlign;
+            H[c] = act ? hv : H[c];
+            F[c] = act ? (valid ? f : kNegAlign) : F[c];
+            e = act ? (valid ? ev : kNegAlign) : e;
+            hl = act ? hv : hl;
+            d4[c >> 2] |= (valid ? (org | (e_ext << 2) | (f_ext << 3)) : 0u) << (8 * (c & 3));
+            const bool better = hv > best;   // hv is NEG unless valid
+            best = better ? hv : best;
+            bi = better ? i : bi;
+            bk = better ? k : bk;
+            has_cells = has_cells || valid;
         };
         // first cell, then its new values to the left lane (the cell above-
         // right of that lane's last cell)
@@ -366,11 +388,11 @@ __device__ __forceinline__ void align_lanes_job(const AlignArgs& a, const int32_
 #pragma unroll
         for (int c = 1; c < C - 1; ++c) cell(c, H[c + 1], F[c + 1]);
         cell(C - 1, upH, upF);
+        *reinterpret_cast<uint4*>(dir + (int64_t)(act ? r : rows) * W + l * C) = make_uint4(d4[0], d4[1], d4[2], d4[3]);
         if (act) {
-            *reinterpret_cast<uint4*>(dir + (int64_t)r * W + l * C) = make_uint4(d4[0], d4[1], d4[2], d4[3]);
             outE = e;
             outH = hl;
-            if (srow == INT32_MAX && best >= stop && rowbest != INT32_MIN) {
+            if (srow == INT32_MAX && best >= stop && has_cells) {
                 srow = i;
                 sbest = best;
                 si = bi;
@@ -431,16 +453,17 @@ __device__ __forceinline__ void align_lanes_job(const AlignArgs& a, const int32_
     write_walk(a, g, J, dir, W, top, bi0, bk0, rec);
 }
 
-// Every lane class in one launch: blocks [block_start[x], block_start[x+1])
-// run class x (L = 1 << x lanes per job, 256 / L jobs a block), their jobs at
-// launch slots slot_start[x] .. slot_start[x + 1] - 1.
+// Every lane class in one launch: class x (L = 1 << x lanes per job, 256 / L
+// jobs a block) has block_start[x + 1] - block_start[x] blocks, its jobs at
+// launch slots slot_start[x] .. slot_start[x + 1] - 1; the block order table
+// maps launch blocks to (class, block of the class), longest jobs first.
 __global__ __launch_bounds__(256) void band_align_lanes_kernel(AlignArgs a, AlignLaneClasses cls) {
     __shared__ int32_t M[576];
     for (int x = threadIdx.x; x < 576; x += blockDim.x) M[x] = a.mat[x];
     __syncthreads();
-    int x = 0;
-    while (x + 1 < kAlignLaneClasses && blockIdx.x >= cls.block_start[x + 1]) ++x;
-    const uint32_t lb = blockIdx.x - cls.block_start[x];
+    const uint32_t e = cls.order[blockIdx.x];
+    const int x = (int)(e >> 24);
+    const uint32_t lb = e & 0xFFFFFFu;
     const uint32_t end = cls.slot_start[x + 1];
     switch (x) {
 #define SABER_ALIGN_CLASS(X, L)                                                              \

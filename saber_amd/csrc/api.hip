@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <atomic>
 #include <memory>
 #include <cstdio>
@@ -1113,14 +1114,20 @@ int band_align_impl(saber_hip_ctx* c, const saber_hip_seqset* queries, const sab
     // Launches of jobs whose scratch (work rows, direction bytes, operation
     // bytes) fits a budget, per lane class (jobs of band width <= 16 L on L
     // lanes, wider ones on one thread with their rows in the work buffer).
-    constexpr uint64_t kBudget = 512ull << 20;
+    uint64_t kBudget = 2048ull << 20;
+    if (const char* e = std::getenv("SABER_HIP_ALIGN_SCRATCH_MB")) {
+        const long long v = std::strtoll(e, nullptr, 10);
+        if (v > 0 && v < (1ll << 24)) kBudget = (uint64_t)v << 20;
+    }
     std::vector<AlignRecord> hrec(njobs);
     std::vector<uint32_t> ids;
     std::vector<uint64_t> woff, doff, ooff;
-    DevBuf<uint32_t> dids;
+    DevBuf<uint32_t> dids, dorder;
+    std::vector<uint32_t> border;
+    std::vector<uint64_t> bkey;
     DevBuf<uint64_t> dwoff, ddoff, dooff;
     DevBuf<int32_t> work;
-    DevBuf<uint8_t> dir, raw, codes;
+    DevBuf<uint8_t> dir, codes;
     DevBuf<uint32_t> lens;
     std::vector<uint8_t> hcodes;
     std::vector<uint32_t> hlens;
@@ -1156,11 +1163,13 @@ int band_align_impl(saber_hip_ctx* c, const saber_hip_seqset* queries, const sab
                 const uint32_t k = order[pos];
                 const BandJobDev& d = dj[k];
                 if ((cls[k] < kAlignLaneClasses) != lanes) break;
-                // direction rows of 16 L bytes on L lanes (16-byte stores), w bytes on one thread
+                // direction rows of 16 L bytes on L lanes (16-byte stores, one
+                // spare row for the lag steps), w bytes on one thread
                 const uint64_t wb = lanes ? 0 : 8ull * ((uint64_t)d.w + 1),
-                               db = (uint64_t)d.rows * (lanes ? (uint64_t)kAlignLaneCells << cls[k] : (uint64_t)d.w),
+                               db = lanes ? ((uint64_t)d.rows + 1) * ((uint64_t)kAlignLaneCells << cls[k])
+                                          : (uint64_t)d.rows * (uint64_t)d.w,
                                ob = (uint64_t)d.m + d.n;
-                if (!ids.empty() && (wt + dt + 6 * ot + wb + db + 6 * ob > kBudget || ids.size() >= (1u << 20)))
+                if (!ids.empty() && (wt + dt + 5 * ot + wb + db + 5 * ob > kBudget || ids.size() >= (1u << 20)))
                     break;
                 ids.push_back(k);
                 woff.push_back(wt / 4);
@@ -1183,6 +1192,26 @@ int band_align_impl(saber_hip_ctx* c, const saber_hip_seqset* queries, const sab
                 }
                 lc.slot_start[kAlignLaneClasses] = slot;
                 lc.block_start[kAlignLaneClasses] = blk;
+                // Blocks by the steps of their first (longest) job, rows +
+                // L - 1, longest first across the classes.
+                border.clear();
+                bkey.clear();
+                for (int x = 0; x < kAlignLaneClasses; ++x) {
+                    const uint32_t per = 256u >> x;
+                    for (uint32_t b = 0; b < lc.block_start[x + 1] - lc.block_start[x]; ++b) {
+                        const uint32_t first = ids[lc.slot_start[x] + b * per];
+                        border.push_back((uint32_t)x << 24 | b);
+                        bkey.push_back((uint64_t)dj[first].rows + (1u << x) - 1);
+                    }
+                }
+                std::vector<uint32_t> bidx(border.size());
+                for (uint32_t b = 0; b < bidx.size(); ++b) bidx[b] = b;
+                std::stable_sort(bidx.begin(), bidx.end(), [&](uint32_t u, uint32_t v) { return bkey[u] > bkey[v]; });
+                std::vector<uint32_t> sorted(bidx.size());
+                for (size_t b = 0; b < bidx.size(); ++b) sorted[b] = border[bidx[b]];
+                border.swap(sorted);
+                if (int r = upload(dorder, border.data(), border.size(), st)) return r;
+                lc.order = dorder.p;
             }
             if (int r = upload(dids, ids.data(), ids.size(), st)) return r;
             if (int r = upload(dwoff, woff.data(), woff.size(), st)) return r;
@@ -1190,8 +1219,7 @@ int band_align_impl(saber_hip_ctx* c, const saber_hip_seqset* queries, const sab
             if (int r = upload(dooff, ooff.data(), ooff.size(), st)) return r;
             if (wt && work.n < wt / 4) HIP_TRY(work.alloc(wt / 4));
             if (dir.n < dt) HIP_TRY(dir.alloc(dt));
-            if (raw.n < ot) {
-                HIP_TRY(raw.alloc(ot));
+            if (codes.n < ot) {
                 HIP_TRY(codes.alloc(ot));
                 HIP_TRY(lens.alloc(ot));
             }
@@ -1211,7 +1239,6 @@ int band_align_impl(saber_hip_ctx* c, const saber_hip_seqset* queries, const sab
             a.dir_off = ddoff.p;
             a.dir = dir.p;
             a.op_off = dooff.p;
-            a.op_raw = raw.p;
             a.op_codes = codes.p;
             a.op_lens = lens.p;
             a.out = recs.p;

@@ -189,7 +189,6 @@ struct AlignArgs {
     const uint64_t* dir_off;    // per launch thread: rows * w bytes
     uint8_t* dir;
     const uint64_t* op_off;     // per launch thread: m + n entries
-    uint8_t* op_raw;
     uint8_t* op_codes;
     uint32_t* op_lens;
     AlignRecord* out;           // indexed by job id
@@ -199,10 +198,12 @@ struct AlignArgs {
 hipError_t launch_band_align(const AlignArgs& a, hipStream_t s);
 constexpr int kAlignLaneCells = 16, kAlignMaxLanes = 64, kAlignLaneClasses = 7;
 // Lane classes of one launch (class x: 1 << x lanes per job): the launch's
-// slots and blocks of each class, as prefix offsets.
+// slots and blocks of each class, as prefix offsets, and the block order:
+// launch block b runs block order[b] & 0xFFFFFF of class order[b] >> 24.
 struct AlignLaneClasses {
     uint32_t slot_start[kAlignLaneClasses + 1];
     uint32_t block_start[kAlignLaneClasses + 1];
+    const uint32_t* order;
 };
 hipError_t launch_band_align_lanes(const AlignArgs& a, const AlignLaneClasses& cls, hipStream_t s);
 

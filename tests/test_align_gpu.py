@@ -115,9 +115,12 @@ def test_workload_regions(ctx):
     assert sum(isinstance(g, tuple) for g in got) > 1000
 
 
-def test_jobs_spanning_several_launches(ctx):
-    """Direction matrices of ~10 MB each: the jobs run in more than one launch
-    of the library's scratch budget, results placed by job."""
+@pytest.mark.parametrize("half,budget_mb", [(1800, 200), (400, 16)])
+def test_jobs_spanning_several_launches(ctx, monkeypatch, half, budget_mb):
+    """Direction matrices of ~10 MB (one thread per job, band width 3601) or
+    ~3 MB (64 lanes per job, width 801) each, under a scratch budget of a few
+    of them: the jobs run in several launches, results placed by job."""
+    monkeypatch.setenv("SABER_HIP_ALIGN_SCRATCH_MB", str(budget_mb))
     r = Lcg(77)
     qs, ss, rows = [], [], []
     for t in range(52):
@@ -125,6 +128,6 @@ def test_jobs_spanning_several_launches(ctx):
         b = mutate(r, a)
         qs.append(a)
         ss.append(b if len(b) else a)
-        rows.append((t, t, 0, len(ss[-1]), -1800, 1800))
+        rows.append((t, t, 0, len(ss[-1]), -half, half))
     jobs = jobs_array(*zip(*rows))
     check(run(ctx, qs, ss, jobs, PAM30, 9, 1), oracle_align(qs, ss, jobs, PAM30, 9, 1))
