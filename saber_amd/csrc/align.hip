@@ -7,10 +7,11 @@
 // row holding band cells whose running best reaches stop_at), and walk
 // (banded.rs:336-403).
 //
-// Band widths up to 1024 run L lanes per job (the lane-parallel kernel
-// below, direction rows of 16 L bytes), wider ones one thread per job (rows
-// of w bytes); the direction bytes of a job's band rows (rows i_first ..
-// i_first + rows - 1) go to a per-job slice of a direction buffer.
+// Band widths up to 64 * 17 run L lanes of C cells per job (the
+// lane-parallel kernel below, direction rows of L C bytes), wider ones one
+// thread per job (rows of w bytes); the direction bytes of a job's band rows
+// (rows i_first .. i_first + rows - 1) go to a per-job slice of a direction
+// buffer.
 // The walk merges equal operations into runs as it goes (in registers),
 // writes the runs in reverse order at the job's offset of the output arrays
 // and then reverses them in place: no value it stores is read back while
@@ -220,8 +221,9 @@ __global__ __launch_bounds__(kWideThreads) void band_align_kernel(AlignArgs a) {
     write_walk(a, t, J, dir, w, best, bi, bk, rec);
 }
 
-// Lane-parallel form for band widths up to 64 * 16: L lanes per job, lane l
-// owning band cells l*C .. l*C + C - 1 and working on row r = t - l at step t
+// Lane-parallel form for band widths up to 64 * 17: L lanes per job of C
+// cells each (17, or 5 for long jobs), lane l owning band cells
+// l*C .. l*C + C - 1 and working on row r = t - l at step t
 // (the scoring kernel's layout, in exact int32 with NEG outside the band). A
 // cell's diagonal and upper neighbours are the lane's own previous-row values
 // except the last cell's upper-right one, which is the right lane's first
@@ -234,8 +236,6 @@ __global__ __launch_bounds__(kWideThreads) void band_align_kernel(AlignArgs a) {
 // stop_at) is the least such row over the lanes, and the best over the rows
 // up to it is among the snapshots of the lanes that reached it there. After a
 // device-scope fence, the group's first lane walks the direction bytes.
-constexpr int kLaneCells = 16;
-
 template <int L>
 __device__ __forceinline__ int32_t lane_from_prev(int32_t v) {
     if constexpr (L <= 16)
@@ -253,13 +253,14 @@ __device__ __forceinline__ int32_t lane_from_next(int32_t v) {
 
 // One job on the L lanes of a group: launch slot g, lane l (M: the
 // substitution matrix in LDS).
-template <int L>
+template <int L, int C>
 __device__ __forceinline__ void align_lanes_job(const AlignArgs& a, const int32_t* M, uint32_t g, int l) {
-    constexpr int C = kLaneCells;
+    static_assert(C == 5 || C == 17, "lane cells");
+    constexpr int NW = (C + 3) / 4;   // dwords of a lane's subject codes / direction bytes
     const uint32_t id = a.ids[g];
     const BandJobDev J = a.jobs[id];
     const int32_t w = J.w, lo = J.lo, n = (int32_t)J.n, rows = J.rows, i_first = J.i_first;
-    constexpr int32_t W = L * C;   // direction-row stride: a lane's 16 bytes are one aligned store
+    constexpr int32_t W = L * C;   // direction-row stride: a lane's C bytes are contiguous
     const int32_t go = a.go, ge = a.ge;
     const int32_t stop = a.stop_at ? a.stop_at[id] : INT32_MAX;
     const uint8_t* q = a.qres + J.q_off;
@@ -281,18 +282,18 @@ __device__ __forceinline__ void align_lanes_job(const AlignArgs& a, const int32_
     int32_t outE = kNegAlign, outH = kNegAlign;   // this lane's last cell, for the right lane
     int32_t best = 0, bi = 0, bk = 0;             // first row-major best of this lane
     int32_t sbest = 0, si = 0, sk = 0, srow = INT32_MAX;   // snapshot at the first row reaching stop
-    // subject bytes of a row: columns i + lo + l*C .. + C - 1, as C/4 + 1
+    // subject bytes of a row: columns i + lo + l*C .. + C - 1, as NW
     // unaligned dwords clamped into the buffer's guard bytes (bytes outside
     // the window belong to cells outside the band rectangle, never used)
-    auto load_cols = [&](int32_t j0, uint32_t (&v)[C / 4]) {
+    auto load_cols = [&](int32_t j0, uint32_t (&v)[NW]) {
 #pragma unroll
-        for (int x = 0; x < C / 4; ++x) {
+        for (int x = 0; x < NW; ++x) {
             int32_t o = j0 + 4 * x;
             o = o < -16 ? -16 : (o > n + 60 ? n + 60 : o);
             __builtin_memcpy(&v[x], s + o, 4);
         }
     };
-    uint32_t sv[C / 4], svn[C / 4];
+    uint32_t sv[NW], svn[NW];
     load_cols(i_first + lo + l * C - l, svn);
     // query code and bias of the next row, loaded a step ahead (index
     // clamped into the query)
@@ -313,7 +314,7 @@ __device__ __forceinline__ void align_lanes_job(const AlignArgs& a, const int32_
         const bool act = r >= 0 && r < rows;
         const int32_t i = i_first + r;
 #pragma unroll
-        for (int x = 0; x < C / 4; ++x) sv[x] = svn[x];
+        for (int x = 0; x < NW; ++x) sv[x] = svn[x];
         load_cols(i + 1 + lo + l * C, svn);   // next row's columns
         // left lane's last cell of this row (previous step)
         int32_t inE = lane_from_prev<L>(outE), inH = lane_from_prev<L>(outH);
@@ -339,7 +340,7 @@ __device__ __forceinline__ void align_lanes_job(const AlignArgs& a, const int32_
             const uint32_t code = (sv[c >> 2] >> (8 * (c & 3))) & 0xFFu;
             sc[c] = row[code < 24 ? code : (uint32_t)kCodeX] + b;
         }
-        uint32_t d4[C / 4] = {};   // the lane's direction bytes of this row
+        uint32_t d4[NW] = {};   // the lane's direction bytes of this row
         int32_t e = inE, hl = inH;
         bool has_cells = false;   // a band cell of this lane in this row
         const bool up_ok = i >= 1;
@@ -388,7 +389,11 @@ __device__ __forceinline__ void align_lanes_job(const AlignArgs& a, const int32_
 #pragma unroll
         for (int c = 1; c < C - 1; ++c) cell(c, H[c + 1], F[c + 1]);
         cell(C - 1, upH, upF);
-        *reinterpret_cast<uint4*>(dir + (int64_t)(act ? r : rows) * W + l * C) = make_uint4(d4[0], d4[1], d4[2], d4[3]);
+        // C bytes at their cells' offsets: whole dwords (unaligned), then
+        // the last byte
+        uint8_t* drow = dir + (int64_t)(act ? r : rows) * W + l * C;
+        __builtin_memcpy(drow, d4, 4 * (NW - 1));
+        drow[C - 1] = (uint8_t)d4[NW - 1];
         if (act) {
             outE = e;
             outH = hl;
@@ -466,19 +471,25 @@ __global__ __launch_bounds__(256) void band_align_lanes_kernel(AlignArgs a, Alig
     const uint32_t lb = e & 0xFFFFFFu;
     const uint32_t end = cls.slot_start[x + 1];
     switch (x) {
-#define SABER_ALIGN_CLASS(X, L)                                                              \
-    case X: {                                                                             \
-        const uint32_t g = cls.slot_start[X] + (lb * 256 + threadIdx.x) / (L);            \
-        if (g < end) align_lanes_job<L>(a, M, g, (int)(threadIdx.x % (L)));               \
-        break;                                                                            \
+#define SABER_ALIGN_CLASS(X)                                                                         \
+    case X: {                                                                                        \
+        constexpr int L = align_class_lanes(X), C = align_class_cells(X);                            \
+        const uint32_t g = cls.slot_start[X] + (lb * 256 + threadIdx.x) / L;                        \
+        if (g < end) align_lanes_job<L, C>(a, M, g, (int)(threadIdx.x % L));                        \
+        break;                                                                                       \
     }
-        SABER_ALIGN_CLASS(0, 1)
-        SABER_ALIGN_CLASS(1, 2)
-        SABER_ALIGN_CLASS(2, 4)
-        SABER_ALIGN_CLASS(3, 8)
-        SABER_ALIGN_CLASS(4, 16)
-        SABER_ALIGN_CLASS(5, 32)
-        SABER_ALIGN_CLASS(6, 64)
+        SABER_ALIGN_CLASS(0)
+        SABER_ALIGN_CLASS(1)
+        SABER_ALIGN_CLASS(2)
+        SABER_ALIGN_CLASS(3)
+        SABER_ALIGN_CLASS(4)
+        SABER_ALIGN_CLASS(5)
+        SABER_ALIGN_CLASS(6)
+        SABER_ALIGN_CLASS(7)
+        SABER_ALIGN_CLASS(8)
+        SABER_ALIGN_CLASS(9)
+        SABER_ALIGN_CLASS(10)
+        SABER_ALIGN_CLASS(11)
 #undef SABER_ALIGN_CLASS
     default: break;
     }

@@ -1112,8 +1112,9 @@ int band_align_impl(saber_hip_ctx* c, const saber_hip_seqset* queries, const sab
         if (int r = upload(stops, stop_at, njobs, st)) return r;
     HIP_TRY(recs.alloc(njobs));
     // Launches of jobs whose scratch (work rows, direction bytes, operation
-    // bytes) fits a budget, per lane class (jobs of band width <= 16 L on L
-    // lanes, wider ones on one thread with their rows in the work buffer).
+    // bytes) fits a budget: the lane classes (band widths up to 64 x 17) in
+    // one kind of launch, wider jobs on one thread each with their rows in
+    // the work buffer in the other.
     uint64_t kBudget = 2048ull << 20;
     if (const char* e = std::getenv("SABER_HIP_ALIGN_SCRATCH_MB")) {
         const long long v = std::strtoll(e, nullptr, 10);
@@ -1131,21 +1132,39 @@ int band_align_impl(saber_hip_ctx* c, const saber_hip_seqset* queries, const sab
     DevBuf<uint32_t> lens;
     std::vector<uint8_t> hcodes;
     std::vector<uint32_t> hlens;
-    // Lane class of a job: x for 1 << x lanes (band width <= 16 << x), or
-    // kAlignLaneClasses for one thread per job. Runnable jobs in class order,
-    // longest first within a class (long jobs start first).
-    auto class_of = [&](int32_t w) {
-        for (int x = 0; x < kAlignLaneClasses; ++x)
-            if (w <= kAlignLaneCells << x) return x;
+    // Lane class of a job: the least lane count of 16 cells that covers the
+    // band width (kAlignLaneClasses: one thread per job). A job whose row
+    // steps on those lanes (rows + L - 1) exceed both kAlignMinLongSteps and
+    // the batch's mean load of a SIMD lane -- the sum over jobs of steps x
+    // lanes over the device's 1024 SIMDs x 64 lanes -- would finish after the
+    // rest: it takes the least 4-cell class that covers its width, if any.
+    // Runnable jobs in class order, longest first within a class.
+    auto class_in = [&](int32_t w, int first, int last) {
+        for (int x = first; x < last; ++x)
+            if (w <= align_class_lanes(x) * align_class_cells(x)) return x;
         return kAlignLaneClasses;
     };
     std::vector<uint32_t> order;
     std::vector<uint8_t> cls(njobs, 0);
+    uint64_t lane_steps = 0;
     for (uint64_t j = 0; j < njobs; ++j) {
         const BandJobDev& d = dj[j];
         if (capped[j] || d.w <= 0 || d.rows <= 0) continue;
-        cls[j] = (uint8_t)class_of(d.w);
+        cls[j] = (uint8_t)class_in(d.w, 0, kAlignWideClasses);
+        if (cls[j] < kAlignLaneClasses) {
+            const uint64_t L = (uint64_t)align_class_lanes(cls[j]);
+            lane_steps += ((uint64_t)d.rows + L - 1) * L;
+        }
         order.push_back((uint32_t)j);
+    }
+    const uint64_t long_steps = std::max<uint64_t>(kAlignMinLongSteps, lane_steps / (1024 * 64));
+    for (const uint32_t j : order) {
+        if (cls[j] >= kAlignLaneClasses) continue;
+        const BandJobDev& d = dj[j];
+        if ((uint64_t)d.rows + (uint64_t)align_class_lanes(cls[j]) - 1 > long_steps) {
+            const int x = class_in(d.w, kAlignWideClasses, kAlignLaneClasses);
+            if (x < kAlignLaneClasses) cls[j] = (uint8_t)x;
+        }
     }
     std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
         return cls[x] != cls[y] ? cls[x] < cls[y] : dj[x].rows > dj[y].rows;
@@ -1163,10 +1182,12 @@ int band_align_impl(saber_hip_ctx* c, const saber_hip_seqset* queries, const sab
                 const uint32_t k = order[pos];
                 const BandJobDev& d = dj[k];
                 if ((cls[k] < kAlignLaneClasses) != lanes) break;
-                // direction rows of 16 L bytes on L lanes (16-byte stores, one
-                // spare row for the lag steps), w bytes on one thread
+                // direction rows of L C bytes on L lanes of C cells (one
+                // store per lane and row, one spare row for the lag steps), w
+                // bytes on one thread
                 const uint64_t wb = lanes ? 0 : 8ull * ((uint64_t)d.w + 1),
-                               db = lanes ? ((uint64_t)d.rows + 1) * ((uint64_t)kAlignLaneCells << cls[k])
+                               db = lanes ? ((uint64_t)d.rows + 1) *
+                                                (uint64_t)(align_class_lanes(cls[k]) * align_class_cells(cls[k]))
                                           : (uint64_t)d.rows * (uint64_t)d.w,
                                ob = (uint64_t)d.m + d.n;
                 if (!ids.empty() && (wt + dt + 5 * ot + wb + db + 5 * ob > kBudget || ids.size() >= (1u << 20)))
@@ -1188,20 +1209,21 @@ int band_align_impl(saber_hip_ctx* c, const saber_hip_seqset* queries, const sab
                     uint32_t cnt = 0;
                     while (slot + cnt < ids.size() && cls[ids[slot + cnt]] == x) ++cnt;
                     slot += cnt;
-                    blk += (uint32_t)(((uint64_t)cnt * (1u << x) + 255) / 256);
+                    blk += (uint32_t)(((uint64_t)cnt * (uint64_t)align_class_lanes(x) + 255) / 256);
                 }
                 lc.slot_start[kAlignLaneClasses] = slot;
                 lc.block_start[kAlignLaneClasses] = blk;
-                // Blocks by the steps of their first (longest) job, rows +
-                // L - 1, longest first across the classes.
+                // Blocks by the time of their first (longest) job, rows +
+                // L - 1 steps of C cells a lane, longest first across the
+                // classes.
                 border.clear();
                 bkey.clear();
                 for (int x = 0; x < kAlignLaneClasses; ++x) {
-                    const uint32_t per = 256u >> x;
+                    const uint32_t lanes_x = (uint32_t)align_class_lanes(x), per = 256u / lanes_x;
                     for (uint32_t b = 0; b < lc.block_start[x + 1] - lc.block_start[x]; ++b) {
                         const uint32_t first = ids[lc.slot_start[x] + b * per];
                         border.push_back((uint32_t)x << 24 | b);
-                        bkey.push_back((uint64_t)dj[first].rows + (1u << x) - 1);
+                        bkey.push_back(((uint64_t)dj[first].rows + lanes_x - 1) * (uint64_t)align_class_cells(x));
                     }
                 }
                 std::vector<uint32_t> bidx(border.size());

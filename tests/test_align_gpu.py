@@ -131,3 +131,34 @@ def test_jobs_spanning_several_launches(ctx, monkeypatch, half, budget_mb):
         rows.append((t, t, 0, len(ss[-1]), -half, half))
     jobs = jobs_array(*zip(*rows))
     check(run(ctx, qs, ss, jobs, PAM30, 9, 1), oracle_align(qs, ss, jobs, PAM30, 9, 1))
+
+
+@pytest.mark.parametrize("stop_known", [False, True])
+def test_long_jobs_four_cell_lanes(ctx, stop_known):
+    """Jobs longer than the batch's mean lane load (and 64 row steps) with
+    band widths up to 256 run 4 cells per lane on 4 to 64 lanes, the others
+    16 cells per lane: widths 1 .. 257, jobs of 20 .. 1200 rows (both forms
+    in one launch), homologous and random pairs, with query bias and the row
+    stop."""
+    r = Lcg(91)
+    qs, ss, bias, rows = [], [], [], []
+    widths = [1, 3, 16, 17, 32, 33, 64, 65, 128, 129, 200, 256, 257]
+    t = 0
+    for w in widths:
+        for m in (20 + r.next(30), 383, 384, 700 + r.next(500)):
+            a = rand_seq(r, m)
+            b = mutate(r, a) if t % 3 else rand_seq(r, m + r.next(40))
+            qs.append(a)
+            ss.append(b if len(b) else a)
+            bias.append(np.array([r.next(7) - 3 for _ in range(m)], np.int8))
+            lo = -(w // 2) - r.next(3)
+            rows.append((t, t, 0, len(ss[-1]), lo, lo + w - 1))
+            t += 1
+    jobs = jobs_array(*zip(*rows))
+    want = oracle_align(qs, ss, jobs, BLOSUM62, 11, 1, qbias=bias)
+    stop = None
+    if stop_known:
+        stop = np.array([f[0] if isinstance(f, tuple) else 2**31 - 1 for f in want], np.int32)
+        want = oracle_align(qs, ss, jobs, BLOSUM62, 11, 1, qbias=bias, stop=stop)
+    check(run(ctx, qs, ss, jobs, BLOSUM62, 11, 1, qbias=bias, stop=stop), want)
+    assert sum(isinstance(g, tuple) for g in want) > len(widths) * 2
