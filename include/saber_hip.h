@@ -186,6 +186,42 @@ int saber_hip_band_align(saber_hip_ctx* ctx, const saber_hip_seqset* queries, co
                          const saber_hip_band_job* jobs, const int32_t* stop_at, uint64_t njobs,
                          const saber_hip_scheme* scheme, uint64_t max_dir_bytes, saber_hip_alignment* out,
                          uint8_t* op_codes, uint32_t* op_lens, uint64_t op_capacity);
+/* Engine::traceback (src/search/engine.rs:1328-1393) and its fallback
+ * traceback_from_end (:1403-1446) for a batch of reported hits: per job the
+ * banded traceback of the window, over the region's diagonals widened by a
+ * pad (max(band_pad, 4), doubled after each attempt that neither reproduces
+ * want nor covers the window; the full window when exhaustive or best == 0),
+ * bounded by the scoring band when has_limit; a job whose band exceeds
+ * max_dir_bytes locates the window's first optimal end cell with a score pass
+ * over the scoring band (banded_score) and widens around that cell's
+ * diagonal up to max(end score, want). Every attempt of every job in a round
+ * runs in one saber_hip_band_align launch. out[j] is the alignment (status 0)
+ * or none (1); exact[j] = 1 when its score reaches want. Operations as for
+ * saber_hip_band_align (job j's at the sum of query + window lengths of the
+ * jobs before it). counters may be NULL; else it receives the engine's
+ * counters summed over the batch. Runs on the context stream; blocking. */
+typedef struct saber_hip_traceback_job {
+    uint32_t query, subject;
+    uint32_t win_lo, win_hi;     /* the window: subject residues [win_lo, win_hi) */
+    int64_t d_min, d_max;        /* the region's diagonals (subject - query position) */
+    int32_t best;                /* the region's best X-drop score (0: full window) */
+    int32_t want;                /* the window's exact score */
+    int32_t has_limit;           /* bound by the scoring band [limit_lo, limit_hi] */
+    int32_t reserved;
+    int64_t limit_lo, limit_hi;  /* window diagonals */
+} saber_hip_traceback_job;
+typedef struct saber_hip_traceback_counters {
+    uint64_t tracebacks;              /* banded_align_until calls */
+    uint64_t band_widenings;
+    uint64_t band_capped;
+    uint64_t end_anchored_tracebacks;
+    uint64_t banded_cells;
+} saber_hip_traceback_counters;
+int saber_hip_traceback(saber_hip_ctx* ctx, const saber_hip_seqset* queries, const saber_hip_seqset* subjects,
+                        const saber_hip_traceback_job* jobs, uint64_t njobs, const saber_hip_scheme* scheme,
+                        int64_t band_pad, int exhaustive, uint64_t max_dir_bytes, saber_hip_alignment* out,
+                        uint8_t* exact, uint8_t* op_codes, uint32_t* op_lens, uint64_t op_capacity,
+                        saber_hip_traceback_counters* counters);
 /* saber_hip_band_align, also timing iters repetitions of each kernel launch
  * with HIP events: *kernel_ms = their mean summed over the launches. */
 int saber_hip_band_align_time(saber_hip_ctx* ctx, const saber_hip_seqset* queries, const saber_hip_seqset* subjects,

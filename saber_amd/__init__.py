@@ -26,6 +26,11 @@ from . import _ffi
 from ._ffi import HipError
 from .matrices import ALPHABET, BLOSUM45, BLOSUM62, PAM30
 
+# One hit for Context.traceback (saber_hip_traceback_job).
+TRACEBACK_JOB = np.dtype([("query", "<u4"), ("subject", "<u4"), ("win_lo", "<u4"), ("win_hi", "<u4"),
+                          ("d_min", "<i8"), ("d_max", "<i8"), ("best", "<i4"), ("want", "<i4"),
+                          ("has_limit", "<i4"), ("reserved", "<i4"), ("limit_lo", "<i8"), ("limit_hi", "<i8")])
+
 # Live batches and contexts, released before interpreter exit: the HIP runtime
 # tears itself down after atexit handlers, and a stream destroyed after that
 # aborts the process.
@@ -379,6 +384,45 @@ class Context:
                 ops = [(bytes([int(codes[o.op_off + x])]), int(lens[o.op_off + x])) for x in range(o.nops)]
                 res.append((o.score, o.q_start, o.q_end, o.s_start, o.s_end, ops, o.cells))
         return (res, kms.value) if time_iters > 0 else res
+
+    def traceback(self, qres, qoff, sres, soff, jobs, scheme, band_pad=8, exhaustive=False, qbias=None,
+                  max_dir_bytes=32 << 20):
+        """Engine::traceback (engine.rs:1328-1393, with traceback_from_end
+        :1403-1446) for a batch of reported hits (saber_hip_traceback). jobs
+        is a TRACEBACK_JOB array (window, the region's diagonals d_min/d_max
+        in subject coordinates, its best X-drop score, want = the window's
+        exact score, optional scoring band limit_lo/limit_hi in window
+        diagonals). Returns (results, counters): per job None or
+        ((score, q_start, q_end, s_start, s_end, ops, cells), exact), and the
+        engine's counters summed over the batch."""
+        qs, ss, keep = self._seqsets(qres, qoff, qbias, sres, soff)
+        jobs = np.ascontiguousarray(jobs, dtype=TRACEBACK_JOB)
+        n = len(jobs)
+        qo = np.asarray(qoff, np.int64)
+        m = (qo[jobs["query"].astype(np.int64) + 1] - qo[jobs["query"].astype(np.int64)]) if n else np.zeros(0)
+        w = jobs["win_hi"].astype(np.int64) - jobs["win_lo"].astype(np.int64) if n else np.zeros(0)
+        cap = int((m + w).sum()) if n else 0
+        out = (_ffi.Alignment * max(n, 1))()
+        ex = np.zeros(max(n, 1), np.uint8)
+        codes = np.zeros(max(cap, 1), np.uint8)
+        lens = np.zeros(max(cap, 1), np.uint32)
+        cnt = _ffi.TracebackCounters()
+        sch = scheme.ffi()
+        _ffi.check(_ffi.lib().saber_hip_traceback(
+            self.h, C.byref(qs), None if ss is None else C.byref(ss),
+            jobs.ctypes.data_as(C.POINTER(_ffi.TracebackJob)), n, C.byref(sch), int(band_pad), int(bool(exhaustive)),
+            int(max_dir_bytes), out, _ffi.ptr(ex, C.c_uint8), _ffi.ptr(codes, C.c_uint8),
+            _ffi.ptr(lens, C.c_uint32), cap, C.byref(cnt)))
+        res = []
+        for k in range(n):
+            o = out[k]
+            if o.status != 0:
+                res.append(None)
+                continue
+            ops = [(bytes([int(codes[o.op_off + x])]), int(lens[o.op_off + x])) for x in range(o.nops)]
+            res.append(((o.score, o.q_start, o.q_end, o.s_start, o.s_end, ops, o.cells), bool(ex[k])))
+        counters = {f: int(getattr(cnt, f)) for f, _ in _ffi.TracebackCounters._fields_}
+        return res, counters
 
     def full_autotune(self, qres, qoff, sres, soff, scheme, qbias=None, ends=False, target_cells=50_000_000_000):
         """Measured full-matrix shape per subject-length group on a sample of

@@ -72,6 +72,18 @@ class Alignment(C.Structure):
                 ("op_off", C.c_uint64), ("cells", C.c_uint64)]
 
 
+class TracebackJob(C.Structure):
+    _fields_ = [("query", C.c_uint32), ("subject", C.c_uint32), ("win_lo", C.c_uint32), ("win_hi", C.c_uint32),
+                ("d_min", C.c_int64), ("d_max", C.c_int64), ("best", C.c_int32), ("want", C.c_int32),
+                ("has_limit", C.c_int32), ("reserved", C.c_int32), ("limit_lo", C.c_int64),
+                ("limit_hi", C.c_int64)]
+
+
+class TracebackCounters(C.Structure):
+    _fields_ = [("tracebacks", C.c_uint64), ("band_widenings", C.c_uint64), ("band_capped", C.c_uint64),
+                ("end_anchored_tracebacks", C.c_uint64), ("banded_cells", C.c_uint64)]
+
+
 class PlanParams(C.Structure):
     _fields_ = [("window_slack", C.c_int64), ("score_band", C.c_int64), ("hsp_threshold", C.c_int32),
                 ("exhaustive", C.c_int32)]
@@ -107,6 +119,10 @@ _SIGS = {
                                        C.POINTER(C.c_int32), C.c_uint64, C.POINTER(Scheme), C.c_uint64,
                                        C.POINTER(Alignment), C.POINTER(C.c_uint8), C.POINTER(C.c_uint32),
                                        C.c_uint64]),
+    "saber_hip_traceback": (C.c_int, [P, C.POINTER(SeqSet), C.POINTER(SeqSet), C.POINTER(TracebackJob),
+                                      C.c_uint64, C.POINTER(Scheme), C.c_int64, C.c_int, C.c_uint64,
+                                      C.POINTER(Alignment), C.POINTER(C.c_uint8), C.POINTER(C.c_uint8),
+                                      C.POINTER(C.c_uint32), C.c_uint64, C.POINTER(TracebackCounters)]),
     "saber_hip_band_align_time": (C.c_int, [P, C.POINTER(SeqSet), C.POINTER(SeqSet), C.POINTER(BandJob),
                                             C.POINTER(C.c_int32), C.c_uint64, C.POINTER(Scheme), C.c_uint64,
                                             C.POINTER(Alignment), C.POINTER(C.c_uint8), C.POINTER(C.c_uint32),

@@ -1341,6 +1341,253 @@ int saber_hip_band_align_time(saber_hip_ctx* c, const saber_hip_seqset* queries,
                            op_lens, op_capacity, iters, kernel_ms);
 }
 
+// Engine::traceback (engine.rs:1328-1393) and traceback_from_end
+// (:1403-1446), batched: every job's next banded_align_until attempt of a
+// round runs in one band_align launch, the score passes of the jobs falling
+// back to their end cell in one score_bands call. Per job the decisions are
+// the reference's, in the reference's order.
+int saber_hip_traceback(saber_hip_ctx* c, const saber_hip_seqset* queries, const saber_hip_seqset* subjects,
+                        const saber_hip_traceback_job* jobs, uint64_t njobs, const saber_hip_scheme* scheme,
+                        int64_t band_pad, int exhaustive, uint64_t max_dir_bytes, saber_hip_alignment* out,
+                        uint8_t* exact, uint8_t* op_codes, uint32_t* op_lens, uint64_t op_capacity,
+                        saber_hip_traceback_counters* counters) {
+    if (counters) *counters = saber_hip_traceback_counters{};
+    if (!c || !scheme) return fail(SABER_HIP_EINVAL, "null argument");
+    if (njobs == 0) return SABER_HIP_OK;
+    if (!jobs || !out || !exact) return fail(SABER_HIP_EINVAL, "null argument");
+    if (njobs >= (1ull << 32)) return fail(SABER_HIP_EINVAL, "too many jobs");
+    if (int r = check_seqset(queries, "queries")) return r;
+    if (subjects)
+        if (int r = check_seqset(subjects, "subjects")) return r;
+    std::vector<uint64_t> slen;   // subject lengths
+    {
+        std::lock_guard<std::mutex> g(c->mu);
+        if (!subjects && !c->db.p) return fail(SABER_HIP_ESTATE, "no database uploaded");
+        const uint64_t* so = subjects ? subjects->offsets : c->db_off.data();
+        const uint32_t ns = subjects ? subjects->count : (uint32_t)c->db_off.size() - 1;
+        slen.resize(ns);
+        for (uint32_t k = 0; k < ns; ++k) slen[k] = so[k + 1] - so[k];
+    }
+    struct Tb {
+        int64_t m, n, pad;
+        int64_t base_lo, base_hi;
+        bool base_full, done;
+        int phase;          // 0 region band, 1 around the end cell
+        bool need_score;    // the end cell is not known yet
+        int64_t scope_lo, scope_hi, d_end;
+        int32_t target;
+        int64_t cur_lo, cur_hi;
+        bool covers;
+        bool have_best;
+        saber_hip_alignment best;
+        std::vector<uint8_t> bc;
+        std::vector<uint32_t> bl;
+    };
+    std::vector<Tb> T(njobs);
+    std::vector<uint64_t> op_off(njobs);
+    uint64_t ops_total = 0;
+    const int64_t pad0 = std::max<int64_t>(band_pad, 4);
+    for (uint64_t k = 0; k < njobs; ++k) {
+        const saber_hip_traceback_job& J = jobs[k];
+        if (J.query >= queries->count) return fail(SABER_HIP_EINVAL, "job query out of range");
+        if (J.subject >= slen.size()) return fail(SABER_HIP_EINVAL, "job subject out of range");
+        if (J.win_hi < J.win_lo || J.win_hi > slen[J.subject]) return fail(SABER_HIP_EINVAL, "job window outside its subject");
+        Tb& t = T[k];
+        t.m = (int64_t)(queries->offsets[J.query + 1] - queries->offsets[J.query]);
+        t.n = (int64_t)J.win_hi - J.win_lo;
+        if (t.m >= (1ll << 31) || t.n >= (1ll << 31)) return fail(SABER_HIP_EINVAL, "sequence too long");
+        t.pad = pad0;
+        // Band::full(m, n) when exhaustive or best == 0, else the region's
+        // diagonals in window coordinates; a base equal to the full band is
+        // not widened
+        const bool full = exhaustive || J.best == 0;
+        t.base_lo = full ? -t.m : J.d_min - (int64_t)J.win_lo;
+        t.base_hi = full ? t.n : J.d_max - (int64_t)J.win_lo;
+        t.base_full = t.base_lo == -t.m && t.base_hi == t.n;
+        t.done = false;
+        t.phase = 0;
+        t.need_score = false;
+        t.have_best = false;
+        t.scope_lo = J.has_limit ? J.limit_lo : -t.m;   // the scoring band, or Band::full
+        t.scope_hi = J.has_limit ? J.limit_hi : t.n;
+        op_off[k] = ops_total;
+        ops_total += (uint64_t)(t.m + t.n);
+        out[k] = saber_hip_alignment{};
+        out[k].status = 1;
+        out[k].op_off = op_off[k];
+        exact[k] = 0;
+    }
+    if (ops_total > op_capacity || (ops_total > 0 && (!op_codes || !op_lens)))
+        return fail(SABER_HIP_EINVAL, "operation buffers need the sum over jobs of query + window length entries");
+    saber_hip_traceback_counters cnt{};
+    // a band in window diagonals as an int32 job band: values beyond the
+    // rectangle move to its edge (one past it for an empty band), which
+    // Band::clip maps to the same band
+    auto job_band = [](int64_t lo, int64_t hi, int64_t m, int64_t n, saber_hip_band_job& o) {
+        o.band_lo = (int32_t)std::min<int64_t>(std::max<int64_t>(lo, -m), n + 1);
+        o.band_hi = (int32_t)std::min<int64_t>(std::max<int64_t>(hi, -m - 1), n);
+    };
+    auto finish = [&](uint64_t k, const saber_hip_alignment* a, const uint8_t* oc, const uint32_t* ol, bool ex) {
+        Tb& t = T[k];
+        t.done = true;
+        if (!a) return;
+        out[k] = *a;
+        out[k].op_off = op_off[k];
+        if (a->nops) {
+            std::memcpy(op_codes + op_off[k], oc, a->nops);
+            std::memcpy(op_lens + op_off[k], ol, 4ull * a->nops);
+        }
+        exact[k] = ex ? 1 : 0;
+    };
+    // the cap branch: traceback_from_end found nothing, so the best
+    // alignment so far, not exact
+    auto give_up = [&](uint64_t k) {
+        Tb& t = T[k];
+        cnt.band_capped += 1;
+        if (t.have_best) finish(k, &t.best, t.bc.data(), t.bl.data(), false);
+        else finish(k, nullptr, nullptr, nullptr, false);
+    };
+    std::vector<uint64_t> ids;
+    std::vector<saber_hip_band_job> bj;
+    std::vector<int32_t> stops;
+    std::vector<saber_hip_alignment> res;
+    std::vector<uint8_t> rc;
+    std::vector<uint32_t> rl;
+    std::vector<int32_t> esc;
+    std::vector<uint32_t> eqe, ese;
+    std::vector<uint64_t> ecells;
+    for (;;) {
+        // score passes of the jobs that fell back to their end cell
+        ids.clear();
+        bj.clear();
+        for (uint64_t k = 0; k < njobs; ++k)
+            if (!T[k].done && T[k].need_score) {
+                const saber_hip_traceback_job& J = jobs[k];
+                saber_hip_band_job b{J.query, J.subject, J.win_lo, J.win_hi, 0, 0};
+                job_band(T[k].scope_lo, T[k].scope_hi, T[k].m, T[k].n, b);
+                ids.push_back(k);
+                bj.push_back(b);
+            }
+        if (!ids.empty()) {
+            esc.assign(ids.size(), 0);
+            eqe.assign(ids.size(), 0);
+            ese.assign(ids.size(), 0);
+            ecells.assign(ids.size(), 0);
+            if (int r = saber_hip_score_bands(c, queries, subjects, bj.data(), bj.size(), scheme, esc.data(),
+                                              eqe.data(), ese.data(), ecells.data()))
+                return r;
+            for (size_t x = 0; x < ids.size(); ++x) {
+                Tb& t = T[ids[x]];
+                t.need_score = false;
+                cnt.banded_cells += ecells[x];
+                if (esc[x] <= 0) {
+                    give_up(ids[x]);
+                    continue;
+                }
+                cnt.end_anchored_tracebacks += 1;
+                t.phase = 1;
+                t.target = std::max(esc[x], jobs[ids[x]].want);
+                t.d_end = (int64_t)ese[x] - (int64_t)eqe[x];
+                t.pad = pad0;
+            }
+        }
+        // this round's attempts
+        ids.clear();
+        bj.clear();
+        stops.clear();
+        uint64_t round_ops = 0;
+        for (uint64_t k = 0; k < njobs; ++k) {
+            Tb& t = T[k];
+            if (t.done) continue;
+            const saber_hip_traceback_job& J = jobs[k];
+            int64_t lo, hi;
+            bool covers;
+            int32_t stop;
+            if (t.phase == 0) {
+                lo = t.base_full ? t.base_lo : t.base_lo - t.pad;
+                hi = t.base_full ? t.base_hi : t.base_hi + t.pad;
+                covers = lo <= -(t.m - 1) && hi >= t.n - 1;
+                if (J.has_limit) {
+                    lo = std::max<int64_t>(lo, J.limit_lo);
+                    hi = std::min<int64_t>(hi, J.limit_hi);
+                    covers = covers || (lo <= J.limit_lo && hi >= J.limit_hi);
+                }
+                stop = J.want;
+            } else {
+                lo = std::max<int64_t>(t.d_end - t.pad, t.scope_lo);
+                hi = std::min<int64_t>(t.d_end + t.pad, t.scope_hi);
+                covers = lo <= t.scope_lo && hi >= t.scope_hi;
+                stop = t.target;
+            }
+            t.cur_lo = lo;
+            t.cur_hi = hi;
+            t.covers = covers;
+            saber_hip_band_job b{J.query, J.subject, J.win_lo, J.win_hi, 0, 0};
+            job_band(lo, hi, t.m, t.n, b);
+            ids.push_back(k);
+            bj.push_back(b);
+            stops.push_back(stop);
+            round_ops += (uint64_t)(t.m + t.n);
+        }
+        if (ids.empty()) break;
+        cnt.tracebacks += ids.size();
+        res.assign(ids.size(), saber_hip_alignment{});
+        rc.resize(round_ops);
+        rl.resize(round_ops);
+        if (int r = band_align_impl(c, queries, subjects, bj.data(), stops.data(), bj.size(), scheme, max_dir_bytes,
+                                    res.data(), rc.data(), rl.data(), round_ops, 0, nullptr))
+            return r;
+        for (size_t x = 0; x < ids.size(); ++x) {
+            const uint64_t k = ids[x];
+            Tb& t = T[k];
+            const saber_hip_alignment& a = res[x];
+            const uint8_t* oc = rc.data() + a.op_off;
+            const uint32_t* ol = rl.data() + a.op_off;
+            if (t.phase == 0) {
+                if (a.status == 0) {
+                    cnt.banded_cells += a.cells;
+                    t.have_best = true;
+                    t.best = a;
+                    t.bc.assign(oc, oc + a.nops);
+                    t.bl.assign(ol, ol + a.nops);
+                    if (a.score >= jobs[k].want || t.covers) {
+                        finish(k, &t.best, t.bc.data(), t.bl.data(), a.score >= jobs[k].want);
+                        continue;
+                    }
+                } else if (a.status == 1) {
+                    if (t.covers) {
+                        finish(k, nullptr, nullptr, nullptr, false);
+                        continue;
+                    }
+                } else {
+                    t.need_score = true;   // over the cap: traceback_from_end
+                    continue;
+                }
+            } else {
+                if (a.status == 0) {
+                    cnt.banded_cells += a.cells;
+                    if (a.score >= t.target || t.covers) {
+                        finish(k, &a, oc, ol, a.score >= jobs[k].want);
+                        continue;
+                    }
+                } else if (a.status == 1) {
+                    if (t.covers) {
+                        give_up(k);
+                        continue;
+                    }
+                } else {
+                    give_up(k);
+                    continue;
+                }
+            }
+            cnt.band_widenings += 1;
+            t.pad *= 2;
+        }
+    }
+    if (counters) *counters = cnt;
+    return SABER_HIP_OK;
+}
+
 int saber_hip_score_windows(saber_hip_ctx* c, const uint8_t* query, uint32_t qlen,
                             const int8_t* bias, const uint8_t* const* windows,
                             const uint32_t* lens, uint32_t n, const saber_hip_scheme* scheme,
