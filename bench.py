@@ -23,7 +23,8 @@ The printed JSON line also carries
                 of whole queries of the same batch;
   extra.xdrop   X-drop extension of 4 seed hits per region (steps/s);
   extra.traceback  banded traceback of the best-scoring regions, stopping at
-                their exact score (alignments/s, the oracle on one core);
+                their exact score (alignments/s, the oracle on one core), and
+                the engine's widening loop over them as hits (wall time);
   extra.full_sw exhaustive full-matrix SW (queries x subjects, GCUPS; scores
                 only, as score_many, and with the end cell; shapes changed by
                 the autotuner);
@@ -290,6 +291,33 @@ def main():
         print(f"# traceback {ntb} alignments: kernel {tms:.3f} ms ({tb['alignments_per_s']:.0f}/s, "
               f"{tb['gcups']} GCUPS), oracle 1 core {tb['cpu_oracle_alignments_per_s_1core']}/s, "
               f"mismatches {tbad}/{len(ti)}", file=sys.stderr, flush=True)
+        # The consumer: Engine::traceback's widening loop over the same hits
+        # (saber_hip_traceback), region diagonals at the band centre, the
+        # scoring band as the limit, want = the exact band score, the
+        # engine's default band_pad 16 (engine.rs:179). Every hit must come
+        # back exact (the limit band reproduces its score). Wall time of the
+        # library call (host rounds, uploads and copies included; not the
+        # Python conversion of its results).
+        hj = np.zeros(ntb, S.TRACEBACK_JOB)
+        for f in ("query", "subject", "win_lo", "win_hi"):
+            hj[f] = tjobs[f]
+        centre = (tjobs["band_lo"].astype(np.int64) + tjobs["band_hi"].astype(np.int64)) // 2
+        hj["d_min"] = centre + tjobs["win_lo"].astype(np.int64)
+        hj["d_max"] = hj["d_min"]
+        hj["best"] = 1
+        hj["want"] = tstop
+        hj["has_limit"] = 1
+        hj["limit_lo"] = tjobs["band_lo"]
+        hj["limit_hi"] = tjobs["band_hi"]
+        ctx.traceback(d["qres"], d["qoff"], d["sres"], d["soff"], hj[:64], sch, band_pad=16)   # warm-up
+        hres, hcnt, hms = ctx.traceback(d["qres"], d["qoff"], d["sres"], d["soff"], hj, sch, band_pad=16,
+                                        timed=True)
+        hbad = sum(1 for r, w in zip(hres, tstop.tolist()) if r is None or not r[1] or r[0][0] != w)
+        tb["consumer"] = {"hits": int(ntb), "wall_ms": round(hms, 2), "hits_per_s": round(ntb / (hms * 1e-3), 1),
+                          "counters": hcnt, "not_exact": hbad}
+        print(f"# traceback consumer {ntb} hits: {hms:.1f} ms wall ({tb['consumer']['hits_per_s']:.0f}/s), "
+              f"{hcnt['tracebacks']} attempts, {hcnt['band_widenings']} widenings, not exact {hbad}",
+              file=sys.stderr, flush=True)
 
     # Secondary: X-drop ungapped extension over seed hits of the same batch
     # (work unit: extension steps, xdrop_steps of scan.rs:902).

@@ -17,6 +17,7 @@ CPU path here. Inputs are alignment codes 0..23 (codes >= 24 score as X).
 """
 from dataclasses import dataclass
 import atexit
+import time
 import ctypes as C
 import weakref
 
@@ -386,7 +387,7 @@ class Context:
         return (res, kms.value) if time_iters > 0 else res
 
     def traceback(self, qres, qoff, sres, soff, jobs, scheme, band_pad=8, exhaustive=False, qbias=None,
-                  max_dir_bytes=32 << 20):
+                  max_dir_bytes=32 << 20, timed=False):
         """Engine::traceback (engine.rs:1328-1393, with traceback_from_end
         :1403-1446) for a batch of reported hits (saber_hip_traceback). jobs
         is a TRACEBACK_JOB array (window, the region's diagonals d_min/d_max
@@ -394,7 +395,8 @@ class Context:
         exact score, optional scoring band limit_lo/limit_hi in window
         diagonals). Returns (results, counters): per job None or
         ((score, q_start, q_end, s_start, s_end, ops, cells), exact), and the
-        engine's counters summed over the batch."""
+        engine's counters summed over the batch; with timed, also the wall
+        milliseconds of the library call alone."""
         qs, ss, keep = self._seqsets(qres, qoff, qbias, sres, soff)
         jobs = np.ascontiguousarray(jobs, dtype=TRACEBACK_JOB)
         n = len(jobs)
@@ -408,11 +410,13 @@ class Context:
         lens = np.zeros(max(cap, 1), np.uint32)
         cnt = _ffi.TracebackCounters()
         sch = scheme.ffi()
+        t0 = time.perf_counter()
         _ffi.check(_ffi.lib().saber_hip_traceback(
             self.h, C.byref(qs), None if ss is None else C.byref(ss),
             jobs.ctypes.data_as(C.POINTER(_ffi.TracebackJob)), n, C.byref(sch), int(band_pad), int(bool(exhaustive)),
             int(max_dir_bytes), out, _ffi.ptr(ex, C.c_uint8), _ffi.ptr(codes, C.c_uint8),
             _ffi.ptr(lens, C.c_uint32), cap, C.byref(cnt)))
+        call_ms = (time.perf_counter() - t0) * 1e3
         res = []
         for k in range(n):
             o = out[k]
@@ -422,7 +426,7 @@ class Context:
             ops = [(bytes([int(codes[o.op_off + x])]), int(lens[o.op_off + x])) for x in range(o.nops)]
             res.append(((o.score, o.q_start, o.q_end, o.s_start, o.s_end, ops, o.cells), bool(ex[k])))
         counters = {f: int(getattr(cnt, f)) for f, _ in _ffi.TracebackCounters._fields_}
-        return res, counters
+        return (res, counters, call_ms) if timed else (res, counters)
 
     def full_autotune(self, qres, qoff, sres, soff, scheme, qbias=None, ends=False, target_cells=50_000_000_000):
         """Measured full-matrix shape per subject-length group on a sample of

@@ -1027,13 +1027,28 @@ int saber_hip_score_bands(saber_hip_ctx* c, const saber_hip_seqset* queries,
 
 namespace {
 
+// The residues (and bias) of one band_align call on the device and its
+// scratch (work rows, direction bytes, operations), kept by a caller that
+// makes several calls over the same sequence sets: the residues are uploaded
+// once, the scratch grows to the largest call.
+struct AlignSeqs {
+    DevBuf<uint8_t> q, sbuf;
+    DevBuf<int8_t> bias;
+    bool ready = false;
+    DevBuf<int32_t> work;
+    DevBuf<uint8_t> dir, codes;
+    DevBuf<uint32_t> lens;
+};
+
 // saber_hip_band_align; iters > 0 also times iters repetitions of every
 // kernel launch with HIP events (the kernel is idempotent) and adds their mean
 // to *kernel_ms.
+
 int band_align_impl(saber_hip_ctx* c, const saber_hip_seqset* queries, const saber_hip_seqset* subjects,
                     const saber_hip_band_job* jobs, const int32_t* stop_at, uint64_t njobs,
                     const saber_hip_scheme* scheme, uint64_t max_dir_bytes, saber_hip_alignment* out,
-                    uint8_t* op_codes, uint32_t* op_lens, uint64_t op_capacity, int iters, double* kernel_ms) {
+                    uint8_t* op_codes, uint32_t* op_lens, uint64_t op_capacity, int iters, double* kernel_ms,
+                    AlignSeqs* seqs = nullptr) {
     if (kernel_ms) *kernel_ms = 0.0;
     if (!c || !scheme) return fail(SABER_HIP_EINVAL, "null argument");
     if (njobs == 0) return SABER_HIP_OK;
@@ -1090,22 +1105,23 @@ int band_align_impl(saber_hip_ctx* c, const saber_hip_seqset* queries, const sab
     }
     if (ops_total > op_capacity || (ops_total > 0 && (!op_codes || !op_lens)))
         return fail(SABER_HIP_EINVAL, "operation buffers need the sum over jobs of query + window length entries");
-    DevBuf<uint8_t> q, sbuf;
-    DevBuf<int8_t> bias;
+    AlignSeqs local;
+    AlignSeqs& sq = seqs ? *seqs : local;
+    DevBuf<uint8_t>& q = sq.q;
+    DevBuf<int8_t>& bias = sq.bias;
     DevBuf<BandJobDev> djobs;
     DevBuf<int32_t> mat, stops;
     DevBuf<AlignRecord> recs;
-    const uint64_t nqres = queries->offsets[queries->count];
-    if (int r = upload_codes(q, queries->residues, nqres, st)) return r;
-    if (queries->bias)
-        if (int r = upload_guarded(bias, queries->bias, nqres, st)) return r;
-    const uint8_t* sres = nullptr;
-    if (subjects) {
-        if (int r = upload_codes(sbuf, subjects->residues, subjects->offsets[subjects->count], st)) return r;
-        sres = sbuf.p + kGuard;
-    } else {
-        sres = c->db.p + kGuard;
+    if (!sq.ready) {
+        const uint64_t nqres = queries->offsets[queries->count];
+        if (int r = upload_codes(q, queries->residues, nqres, st)) return r;
+        if (queries->bias)
+            if (int r = upload_guarded(bias, queries->bias, nqres, st)) return r;
+        if (subjects)
+            if (int r = upload_codes(sq.sbuf, subjects->residues, subjects->offsets[subjects->count], st)) return r;
+        sq.ready = true;
     }
+    const uint8_t* sres = subjects ? sq.sbuf.p + kGuard : c->db.p + kGuard;
     if (int r = upload(djobs, dj.data(), njobs, st)) return r;
     if (int r = upload(mat, scheme->matrix, 576, st)) return r;
     if (stop_at)
@@ -1127,9 +1143,10 @@ int band_align_impl(saber_hip_ctx* c, const saber_hip_seqset* queries, const sab
     std::vector<uint32_t> border;
     std::vector<uint64_t> bkey;
     DevBuf<uint64_t> dwoff, ddoff, dooff;
-    DevBuf<int32_t> work;
-    DevBuf<uint8_t> dir, codes;
-    DevBuf<uint32_t> lens;
+    DevBuf<int32_t>& work = sq.work;
+    DevBuf<uint8_t>& dir = sq.dir;
+    DevBuf<uint8_t>& codes = sq.codes;
+    DevBuf<uint32_t>& lens = sq.lens;
     std::vector<uint8_t> hcodes;
     std::vector<uint32_t> hlens;
     // Lane class of a job: the least lane count of 16 cells that covers the
@@ -1456,6 +1473,7 @@ int saber_hip_traceback(saber_hip_ctx* c, const saber_hip_seqset* queries, const
     std::vector<int32_t> esc;
     std::vector<uint32_t> eqe, ese;
     std::vector<uint64_t> ecells;
+    AlignSeqs seqs;   // the residues, uploaded by the first round for all
     for (;;) {
         // score passes of the jobs that fell back to their end cell
         ids.clear();
@@ -1535,7 +1553,7 @@ int saber_hip_traceback(saber_hip_ctx* c, const saber_hip_seqset* queries, const
         rc.resize(round_ops);
         rl.resize(round_ops);
         if (int r = band_align_impl(c, queries, subjects, bj.data(), stops.data(), bj.size(), scheme, max_dir_bytes,
-                                    res.data(), rc.data(), rl.data(), round_ops, 0, nullptr))
+                                    res.data(), rc.data(), rl.data(), round_ops, 0, nullptr, &seqs))
             return r;
         for (size_t x = 0; x < ids.size(); ++x) {
             const uint64_t k = ids[x];
