@@ -122,7 +122,7 @@ def make_case(seed, nhits, limit_every=3, zero_best_every=7):
     return qs, ss, rows, biases
 
 
-def run_case(ctx, qs, ss, rows, biases, band_pad, exhaustive, cap, use_bias, go=11, ge=1):
+def run_case(ctx, qs, ss, rows, biases, band_pad, exhaustive, cap, use_bias, go=11, ge=1, use_db=False):
     jobs = np.zeros(len(rows), S.TRACEBACK_JOB)
     cnt = dict.fromkeys(KEYS, 0)
     want_res = []
@@ -141,8 +141,11 @@ def run_case(ctx, qs, ss, rows, biases, band_pad, exhaustive, cap, use_bias, go=
     qres, qoff = flat(qs)
     sres, soff = flat(ss)
     qb = np.concatenate(biases).astype(np.int8) if use_bias else None
-    got, gcnt = ctx.traceback(qres, qoff, sres, soff, jobs, S.Scheme(BLOSUM62, go, ge), band_pad=band_pad,
-                              exhaustive=exhaustive, qbias=qb, max_dir_bytes=cap)
+    if use_db:
+        ctx.upload_db(sres, soff)
+    got, gcnt = ctx.traceback(qres, qoff, None if use_db else sres, None if use_db else soff, jobs,
+                              S.Scheme(BLOSUM62, go, ge), band_pad=band_pad, exhaustive=exhaustive, qbias=qb,
+                              max_dir_bytes=cap)
     for k, (g, w) in enumerate(zip(got, want_res)):
         assert g == w, (k, rows[k], g, w)
     assert gcnt == cnt, (gcnt, cnt)
@@ -174,3 +177,12 @@ def test_cap_sends_hits_to_the_end_cell(ctx, cap):
     assert cnt["end_anchored_tracebacks"] > 0
     if cap == 6000:
         assert cnt["band_capped"] > 0
+
+
+def test_uploaded_database(ctx):
+    """The engine's call: subjects are the uploaded database (subjects ==
+    NULL), including the end-cell fallback's score passes."""
+    qs, ss, rows, biases = make_case(61, 40, limit_every=2)
+    cnt = run_case(ctx, qs, ss, rows, biases, band_pad=16, exhaustive=False, cap=20000, use_bias=False,
+                   use_db=True)
+    assert cnt["tracebacks"] >= 40 and cnt["end_anchored_tracebacks"] > 0
