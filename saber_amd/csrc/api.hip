@@ -1031,6 +1031,21 @@ namespace {
 // scratch (work rows, direction bytes, operations), kept by a caller that
 // makes several calls over the same sequence sets: the residues are uploaded
 // once, the scratch grows to the largest call.
+// Host staging that grows to the largest request and is not initialised
+// (every byte read back was first written by a copy or the caller).
+template <class T>
+struct HostBuf {
+    std::unique_ptr<T[]> p;
+    uint64_t n = 0;
+    T* get(uint64_t count) {
+        if (count > n) {
+            p.reset(new T[count]);
+            n = count;
+        }
+        return p.get();
+    }
+};
+
 struct AlignSeqs {
     DevBuf<uint8_t> q, sbuf;
     DevBuf<int8_t> bias;
@@ -1038,6 +1053,8 @@ struct AlignSeqs {
     DevBuf<int32_t> work;
     DevBuf<uint8_t> dir, codes;
     DevBuf<uint32_t> lens;
+    HostBuf<uint8_t> hcodes;
+    HostBuf<uint32_t> hlens;
 };
 
 // saber_hip_band_align; iters > 0 also times iters repetitions of every
@@ -1147,8 +1164,8 @@ int band_align_impl(saber_hip_ctx* c, const saber_hip_seqset* queries, const sab
     DevBuf<uint8_t>& dir = sq.dir;
     DevBuf<uint8_t>& codes = sq.codes;
     DevBuf<uint32_t>& lens = sq.lens;
-    std::vector<uint8_t> hcodes;
-    std::vector<uint32_t> hlens;
+    HostBuf<uint8_t>& hcodes = sq.hcodes;
+    HostBuf<uint32_t>& hlens = sq.hlens;
     // Lane class of a job: the least lane count of 16 cells that covers the
     // band width (kAlignLaneClasses: one thread per job). A job whose row
     // steps on those lanes (rows + L - 1) exceed both kAlignMinLongSteps and
@@ -1297,18 +1314,18 @@ int band_align_impl(saber_hip_ctx* c, const saber_hip_seqset* queries, const sab
                 (void)hipEventDestroy(e1);
                 *kernel_ms += ms / iters;
             }
-            hcodes.resize(ot);
-            hlens.resize(ot);
-            HIP_TRY(hipMemcpyAsync(hcodes.data(), codes.p, ot, hipMemcpyDeviceToHost, st));
-            HIP_TRY(hipMemcpyAsync(hlens.data(), lens.p, 4 * ot, hipMemcpyDeviceToHost, st));
+            uint8_t* hc = hcodes.get(ot);
+            uint32_t* hl = hlens.get(ot);
+            HIP_TRY(hipMemcpyAsync(hc, codes.p, ot, hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipMemcpyAsync(hl, lens.p, 4 * ot, hipMemcpyDeviceToHost, st));
             HIP_TRY(hipMemcpyAsync(hrec.data(), recs.p, sizeof(AlignRecord) * njobs, hipMemcpyDeviceToHost, st));
             HIP_TRY(hipStreamSynchronize(st));
             for (size_t x = 0; x < ids.size(); ++x) {
                 const uint32_t id = ids[x];
                 const AlignRecord& r = hrec[id];
                 if (r.status == 0 && r.nops) {
-                    std::memcpy(op_codes + op_off[id], hcodes.data() + ooff[x], r.nops);
-                    std::memcpy(op_lens + op_off[id], hlens.data() + ooff[x], 4ull * r.nops);
+                    std::memcpy(op_codes + op_off[id], hc + ooff[x], r.nops);
+                    std::memcpy(op_lens + op_off[id], hl + ooff[x], 4ull * r.nops);
                 }
                 saber_hip_alignment& o = out[id];
                 o = saber_hip_alignment{};
@@ -1468,8 +1485,8 @@ int saber_hip_traceback(saber_hip_ctx* c, const saber_hip_seqset* queries, const
     std::vector<saber_hip_band_job> bj;
     std::vector<int32_t> stops;
     std::vector<saber_hip_alignment> res;
-    std::vector<uint8_t> rc;
-    std::vector<uint32_t> rl;
+    HostBuf<uint8_t> rc;
+    HostBuf<uint32_t> rl;
     std::vector<int32_t> esc;
     std::vector<uint32_t> eqe, ese;
     std::vector<uint64_t> ecells;
@@ -1550,17 +1567,17 @@ int saber_hip_traceback(saber_hip_ctx* c, const saber_hip_seqset* queries, const
         if (ids.empty()) break;
         cnt.tracebacks += ids.size();
         res.assign(ids.size(), saber_hip_alignment{});
-        rc.resize(round_ops);
-        rl.resize(round_ops);
+        uint8_t* rcp = rc.get(std::max<uint64_t>(round_ops, 1));
+        uint32_t* rlp = rl.get(std::max<uint64_t>(round_ops, 1));
         if (int r = band_align_impl(c, queries, subjects, bj.data(), stops.data(), bj.size(), scheme, max_dir_bytes,
-                                    res.data(), rc.data(), rl.data(), round_ops, 0, nullptr, &seqs))
+                                    res.data(), rcp, rlp, round_ops, 0, nullptr, &seqs))
             return r;
         for (size_t x = 0; x < ids.size(); ++x) {
             const uint64_t k = ids[x];
             Tb& t = T[k];
             const saber_hip_alignment& a = res[x];
-            const uint8_t* oc = rc.data() + a.op_off;
-            const uint32_t* ol = rl.data() + a.op_off;
+            const uint8_t* oc = rcp + a.op_off;
+            const uint32_t* ol = rlp + a.op_off;
             if (t.phase == 0) {
                 if (a.status == 0) {
                     cnt.banded_cells += a.cells;
