@@ -194,8 +194,8 @@ int saber_hip_band_align(saber_hip_ctx* ctx, const saber_hip_seqset* queries, co
  * bounded by the scoring band when has_limit; a job whose band exceeds
  * max_dir_bytes locates the window's first optimal end cell with a score pass
  * over the scoring band (banded_score) and widens around that cell's
- * diagonal up to max(end score, want). Every attempt of every job in a round
- * runs in one saber_hip_band_align launch. out[j] is the alignment (status 0)
+ * diagonal up to max(end score, want). The attempts of all jobs in a round
+ * go to one saber_hip_band_align call. out[j] is the alignment (status 0)
  * or none (1); exact[j] = 1 when its score reaches want. Operations as for
  * saber_hip_band_align (job j's at the sum of query + window lengths of the
  * jobs before it). counters may be NULL; else it receives the engine's

@@ -1377,7 +1377,7 @@ int saber_hip_band_align_time(saber_hip_ctx* c, const saber_hip_seqset* queries,
 
 // Engine::traceback (engine.rs:1328-1393) and traceback_from_end
 // (:1403-1446), batched: every job's next banded_align_until attempt of a
-// round runs in one band_align launch, the score passes of the jobs falling
+// round goes to one band_align call, the score passes of the jobs falling
 // back to their end cell in one score_bands call. Per job the decisions are
 // the reference's, in the reference's order.
 int saber_hip_traceback(saber_hip_ctx* c, const saber_hip_seqset* queries, const saber_hip_seqset* subjects,
