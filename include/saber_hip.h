@@ -222,6 +222,17 @@ int saber_hip_traceback(saber_hip_ctx* ctx, const saber_hip_seqset* queries, con
                         int64_t band_pad, int exhaustive, uint64_t max_dir_bytes, saber_hip_alignment* out,
                         uint8_t* exact, uint8_t* op_codes, uint32_t* op_lens, uint64_t op_capacity,
                         saber_hip_traceback_counters* counters);
+/* saber_hip_traceback with the counters also per job (job_counters[j], may be
+ * NULL): a caller that replays the engine's reporting loop
+ * (engine.rs:1173-1208: max_target_seqs, max_hsps, overlapping hits) over
+ * speculatively traced hits sums the counters of the hits it used only.
+ * Both entry points fail with SABER_HIP_ESTATE when subjects == NULL and the
+ * database is replaced by saber_hip_upload_db during the call. */
+int saber_hip_traceback_ex(saber_hip_ctx* ctx, const saber_hip_seqset* queries, const saber_hip_seqset* subjects,
+                           const saber_hip_traceback_job* jobs, uint64_t njobs, const saber_hip_scheme* scheme,
+                           int64_t band_pad, int exhaustive, uint64_t max_dir_bytes, saber_hip_alignment* out,
+                           uint8_t* exact, uint8_t* op_codes, uint32_t* op_lens, uint64_t op_capacity,
+                           saber_hip_traceback_counters* counters, saber_hip_traceback_counters* job_counters);
 /* saber_hip_band_align, also timing iters repetitions of each kernel launch
  * with HIP events: *kernel_ms = their mean summed over the launches. */
 int saber_hip_band_align_time(saber_hip_ctx* ctx, const saber_hip_seqset* queries, const saber_hip_seqset* subjects,
@@ -472,6 +483,148 @@ int saber_hip_score_regions(saber_hip_ctx* ctx, const saber_hip_seqset* queries,
                             uint64_t n, const saber_hip_scheme* scheme, int64_t narrow_band, int64_t full_band,
                             const double* thr, int32_t* out, uint8_t* rescored);
 
+
+/* ---- Search: the scan schedule end to end (Engine::search_scan,
+ * src/search/engine/scan.rs:1261-1497) over the uploaded database ----
+ *
+ * Per query batch: preparation (composition bias, statistics, floors,
+ * Engine::prepare engine.rs:759-786, masking off), seed tables of every shape
+ * (build_batch scan.rs:1500-1596), the database scan with the 32-residue
+ * window identity filter (scan_hits / filter_avx2 scan.rs:1603-1804,
+ * 2242-2323), visits in (query position, shape, subject position) order per
+ * (query, subject) with diagonal coverage and X-drop (visit_chunk_inner /
+ * visit_scan scan.rs:2084-2225, 2384-2431), regions (finish_subject_into
+ * scan.rs:2437-2474), the per-query cap, plan_windows, the two-stage banded
+ * scores (score_windows engine.rs:1021-1099) and finish (E-values, the
+ * progressive traceback, make_hit, engine.rs:1127-1221, 1447-1530).
+ * Seed lookup, window filter, survivor ordering, X-drop, coverage and region
+ * building run on the device; the per-query cap, planning, thresholds,
+ * E-values and the reporting loop run on the host, the scoring and traceback
+ * passes on the device. */
+#define SABER_HIP_MAX_SHAPES 32
+#define SABER_HIP_MAX_SHAPE_SPAN 64
+typedef struct saber_hip_seed_scheme {
+    uint8_t alphabet_map[24];          /* SeedAlphabet::map: reduced letter per code, 255 = not seedable */
+    uint32_t alphabet_size;            /* SeedAlphabet::size */
+    uint32_t nshapes;
+    char shapes[SABER_HIP_MAX_SHAPES][SABER_HIP_MAX_SHAPE_SPAN + 4];  /* SeedShape patterns ("11110111"), NUL-terminated */
+} saber_hip_seed_scheme;
+/* SearchParams (engine.rs:62-197) fields on the path; saber_hip_search_defaults
+ * fills the default preset (presets.rs:129-140) with the CLI defaults. */
+typedef struct saber_hip_search_params {
+    double evalue;                     /* -E: 10 */
+    uint32_t max_target_seqs;          /* -k: 10 */
+    uint32_t max_hsps;                 /* 1 */
+    int32_t hsp_threshold;             /* 40 */
+    int32_t x_drop_ungapped;           /* 20 */
+    double hsp_floor_fraction;         /* 0.65 */
+    uint32_t hit_filter_min_ids;       /* 27 (0 = no window filter) */
+    uint32_t hit_filter_exact_weight;  /* 2 */
+    int64_t band_pad;                  /* 16 */
+    int64_t window_slack;              /* 64 */
+    int64_t score_band;                /* 48 (0 = window scoring) */
+    int64_t score_prefilter_band;      /* 16 (0 = one pass) */
+    double score_prefilter_frac;       /* 0.75 */
+    uint64_t max_candidates;           /* 5000 */
+    uint32_t max_regions_per_subject;  /* 4 (at most 8 here) */
+    int32_t comp_bias;                 /* 1 */
+    uint64_t seed_max_occ;             /* 0 = auto_occ_cap */
+    uint64_t traceback_max_bytes;      /* 32 MiB */
+    uint64_t ref_db_residues;          /* 40 000 000 */
+    double min_identity, min_query_cover, min_subject_cover;   /* 0 */
+    int32_t per_subject_evalues;       /* 1: EvalueModel::PerSubject (default), 0: Database */
+    uint32_t reserved[7];
+} saber_hip_search_params;
+void saber_hip_search_defaults(saber_hip_search_params* p);
+/* The seed scheme of the default preset: Murphy-10 and DIAMOND's 16 weight-7
+ * spaced seeds (presets.rs:41-58, alphabet.rs:325-327). */
+void saber_hip_seed_scheme_default(saber_hip_seed_scheme* s);
+
+/* A reported hit (Hit, engine.rs:312-337): 1-based starts, inclusive ends;
+ * its run-length operations ('M', 'I', 'D') are runs [op_off, op_off + nops)
+ * of its query's operation list. */
+typedef struct saber_hip_hit {
+    uint64_t subject;
+    int32_t score;
+    uint32_t exact_in_window;
+    double bitscore, evalue;
+    uint64_t q_start, q_end, s_start, s_end;
+    uint64_t columns, matches, mismatches, positives, gap_opens, gap_columns;
+    uint64_t q_len, s_len;
+    uint64_t op_off;
+    uint32_t nops, reserved;
+} saber_hip_hit;
+/* Counters of the run (Counters, engine.rs:203-297; the names are the
+ * reference's --stats keys). */
+typedef struct saber_hip_search_counters {
+    uint64_t queries, query_residues, seed_positions, seeds_frequent_skipped;
+    uint64_t scan_candidates, hits, hits_filtered, scan_groups;
+    uint64_t ungapped_extensions, extensions_avoided, xdrop_steps, hsps;
+    uint64_t regions, regions_dropped_by_cap, regions_weak_dropped, regions_below_evalue;
+    uint64_t band_scored_regions, band_rescored, banded_cells;
+    uint64_t tracebacks, band_widenings, band_capped, end_anchored_tracebacks;
+    uint64_t filtered_identity_coverage, alignments;
+    uint64_t regions_pruned;           /* removed by the learned prefilter (no reference counter) */
+    uint64_t reserved[6];
+} saber_hip_search_counters;
+/* Wall time of each stage (ms, host clock around device-synchronised stages). */
+typedef struct saber_hip_search_timing {
+    double prepare, seed_tables, scan, sort, xdrop, regions, plan, prefilter, score, finish, total;
+    double reserved[5];
+} saber_hip_search_timing;
+/* Learned prefilter between plan_windows and the scoring passes (the MLP of
+ * saber_hip_mlp_*): regions whose logit is below threshold are not scored.
+ * mean / inv_std standardise the features (may be NULL). */
+typedef struct saber_hip_mlp saber_hip_mlp;
+typedef struct saber_hip_prefilter {
+    saber_hip_mlp* model;
+    const float* mean;
+    const float* inv_std;
+    float threshold;
+    uint32_t reserved;
+} saber_hip_prefilter;
+/* One candidate region of a query after the cap, with its plan and score
+ * (score 0 when pruned); flags: 1 rescored in the full band, 2 its score
+ * passes the E-value cut, 4 reported as a hit, 8 pruned by the prefilter. */
+typedef struct saber_hip_search_region {
+    uint64_t subject;
+    int64_t d_min, d_max;
+    int32_t best;
+    uint32_t n_hsp;
+    int32_t hsp_sum;
+    int32_t score;
+    uint64_t win_lo, win_hi;
+    int64_t band_lo, band_hi;
+    int32_t banded;
+    uint32_t flags;
+    double evalue;
+} saber_hip_search_region;
+typedef struct saber_hip_search saber_hip_search;
+/* Runs the search of `queries` (alignment codes) against the uploaded
+ * database. scoring: matrix and gap costs; karlin: gapped parameters
+ * (saber_hip_stats_gapped_protein); prefilter may be NULL. The result holds
+ * the hits, regions, counters and timings until saber_hip_search_destroy. */
+int saber_hip_search_run(saber_hip_ctx* ctx, const saber_hip_seqset* queries, const saber_hip_seed_scheme* seeds,
+                         const saber_hip_scheme* scoring, const saber_hip_karlin* karlin,
+                         const saber_hip_search_params* params, const saber_hip_prefilter* prefilter,
+                         saber_hip_search** out);
+uint64_t saber_hip_search_hit_count(const saber_hip_search* s, uint32_t query);
+int saber_hip_search_hits(const saber_hip_search* s, uint32_t query, saber_hip_hit* out);
+/* The query's operation list (all its hits), op_count entries. */
+uint64_t saber_hip_search_op_count(const saber_hip_search* s, uint32_t query);
+int saber_hip_search_ops(const saber_hip_search* s, uint32_t query, uint8_t* codes, uint32_t* lens);
+uint64_t saber_hip_search_region_count(const saber_hip_search* s, uint32_t query);
+int saber_hip_search_regions(const saber_hip_search* s, uint32_t query, saber_hip_search_region* out);
+/* Prefilter features (bf16 [n][64], the feature list of features.hip) of
+ * every capped region of every query, in query order then region order;
+ * mean / inv_std as for the prefilter. n = sum of region counts. */
+int saber_hip_search_features(saber_hip_search* s, const float* mean, const float* inv_std, uint16_t* out);
+int saber_hip_search_get_counters(const saber_hip_search* s, saber_hip_search_counters* out);
+int saber_hip_search_get_timing(const saber_hip_search* s, saber_hip_search_timing* out);
+void saber_hip_search_destroy(saber_hip_search* s);
+/* composition_bias (src/scoring/bias.rs:80-122), host code. */
+int saber_hip_composition_bias(const uint8_t* query, uint64_t n, const int32_t* matrix, int8_t* out);
+
 /* ---- Learned region prefilter (no function in the reference) ----
  * The model the scope defines for the region prefilter (it plugs in between
  * the region cap and score_windows, src/search/engine/scan.rs:1476-1485 and
@@ -507,7 +660,6 @@ int saber_hip_region_features_time(saber_hip_ctx* ctx, const saber_hip_seqset* q
 #define SABER_HIP_MLP_IN 64
 #define SABER_HIP_MLP_HID 256
 #define SABER_HIP_MLP_NPARAMS (64 * 256 + 256 + 256 * 256 + 256 + 256 + 1)
-typedef struct saber_hip_mlp saber_hip_mlp;
 /* params may be NULL: He-uniform initialisation from `seed`. */
 int saber_hip_mlp_create(saber_hip_ctx* ctx, const float* params, uint64_t seed, saber_hip_mlp** out);
 int saber_hip_mlp_get_params(saber_hip_mlp* m, float* params);

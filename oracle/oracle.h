@@ -118,6 +118,58 @@ int64_t or_best_ungapped_on_diagonal(const uint8_t* q, int64_t m, const uint8_t*
 /* src/scoring/bias.rs:56-122 */
 void or_composition_bias(const uint8_t* q, int64_t n, const int32_t* mat, int8_t* out);
 
+
+/* ---- Scan schedule (oracle/scan.c): query batch -> candidate regions ---- */
+typedef struct or_scan_params {
+    const uint8_t* alpha_map;   /* [24]: reduced letter per alignment code, 255 = not seedable */
+    uint32_t asize;             /* letters of the seed alphabet */
+    uint32_t nshapes;
+    const uint16_t* offsets;    /* [nshapes][16]: care positions of each shape */
+    const uint32_t* weight;     /* [nshapes] */
+    const uint32_t* span;       /* [nshapes] */
+    uint32_t min_ids, kx;       /* window filter (0 = off), exact-identity weight */
+    int32_t x, hsp_threshold;   /* X-drop, HSP threshold */
+    int64_t window_slack;
+    uint32_t max_regions_per_subject;
+    uint32_t pad;
+    uint64_t cap;               /* max_candidates */
+    uint64_t seed_max_occ;      /* 0 = auto_occ_cap */
+    uint64_t ref_db_residues;
+} or_scan_params;
+
+typedef struct or_scan_counters {
+    uint64_t seed_positions, seeds_frequent_skipped, scan_candidates, hits, hits_filtered;
+    uint64_t ungapped_extensions, extensions_avoided, xdrop_steps, hsps, regions_weak_dropped;
+    uint64_t regions_dropped_by_cap, regions, scan_groups;
+} or_scan_counters;
+
+/* A candidate region with its HSP evidence (count, saturating sum). */
+typedef struct or_region {
+    uint64_t subject;
+    int64_t d_min, d_max;
+    int32_t best;
+    uint32_t n_hsp;
+    int32_t hsp_sum;
+    uint32_t pad;
+} or_region;
+
+/* auto_occ_cap, src/search/index.rs:329-336 */
+uint64_t or_auto_occ_cap(uint64_t total, uint64_t ncodes, uint64_t db_residues, uint64_t ref_residues);
+/* Engine::search_scan up to the per-query capped region lists. qbias: NULL
+ * or one int8 per query residue; region_thr[q]: the query's region
+ * threshold (hsp_threshold_for). log_survivors: keep every window-filter
+ * survivor as (query, subject, qp, sp, shape) in visit order. Returns a
+ * handle for the accessors below. */
+void* or_scan(const uint8_t* qres, const uint64_t* qoff, uint32_t nq, const int8_t* qbias,
+              const uint8_t* sres, const uint64_t* soff, uint32_t ns, const int32_t* mat,
+              const int32_t* region_thr, const or_scan_params* p, int log_survivors,
+              or_scan_counters* ctr);
+uint64_t or_scan_nregions(void* h, uint32_t q);
+void or_scan_regions(void* h, uint32_t q, or_region* out);
+uint64_t or_scan_nsurvivors(void* h);
+void or_scan_survivors(void* h, uint32_t* out);
+void or_scan_free(void* h);
+
 #ifdef __cplusplus
 }
 #endif

@@ -386,7 +386,7 @@ class Context:
                 res.append((o.score, o.q_start, o.q_end, o.s_start, o.s_end, ops, o.cells))
         return (res, kms.value) if time_iters > 0 else res
 
-    def traceback(self, qres, qoff, sres, soff, jobs, scheme, band_pad=8, exhaustive=False, qbias=None,
+    def traceback(self, qres, qoff, sres, soff, jobs, scheme, band_pad=16, exhaustive=False, qbias=None,
                   max_dir_bytes=32 << 20, timed=False):
         """Engine::traceback (engine.rs:1328-1393, with traceback_from_end
         :1403-1446) for a batch of reported hits (saber_hip_traceback). jobs

@@ -100,8 +100,88 @@ class BandJob(C.Structure):
     ]
 
 
+class SeedScheme(C.Structure):
+    _fields_ = [("alphabet_map", C.c_uint8 * 24), ("alphabet_size", C.c_uint32), ("nshapes", C.c_uint32),
+                ("shapes", (C.c_char * 68) * 32)]
+
+
+class SearchParams(C.Structure):
+    _fields_ = [("evalue", C.c_double), ("max_target_seqs", C.c_uint32), ("max_hsps", C.c_uint32),
+                ("hsp_threshold", C.c_int32), ("x_drop_ungapped", C.c_int32), ("hsp_floor_fraction", C.c_double),
+                ("hit_filter_min_ids", C.c_uint32), ("hit_filter_exact_weight", C.c_uint32),
+                ("band_pad", C.c_int64), ("window_slack", C.c_int64), ("score_band", C.c_int64),
+                ("score_prefilter_band", C.c_int64), ("score_prefilter_frac", C.c_double),
+                ("max_candidates", C.c_uint64), ("max_regions_per_subject", C.c_uint32), ("comp_bias", C.c_int32),
+                ("seed_max_occ", C.c_uint64), ("traceback_max_bytes", C.c_uint64), ("ref_db_residues", C.c_uint64),
+                ("min_identity", C.c_double), ("min_query_cover", C.c_double), ("min_subject_cover", C.c_double),
+                ("per_subject_evalues", C.c_int32), ("reserved", C.c_uint32 * 7)]
+
+
+class Hit(C.Structure):
+    _fields_ = [("subject", C.c_uint64), ("score", C.c_int32), ("exact_in_window", C.c_uint32),
+                ("bitscore", C.c_double), ("evalue", C.c_double), ("q_start", C.c_uint64), ("q_end", C.c_uint64),
+                ("s_start", C.c_uint64), ("s_end", C.c_uint64), ("columns", C.c_uint64), ("matches", C.c_uint64),
+                ("mismatches", C.c_uint64), ("positives", C.c_uint64), ("gap_opens", C.c_uint64),
+                ("gap_columns", C.c_uint64), ("q_len", C.c_uint64), ("s_len", C.c_uint64), ("op_off", C.c_uint64),
+                ("nops", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+SEARCH_COUNTERS = ["queries", "query_residues", "seed_positions", "seeds_frequent_skipped", "scan_candidates", "hits",
+                   "hits_filtered", "scan_groups", "ungapped_extensions", "extensions_avoided", "xdrop_steps", "hsps",
+                   "regions", "regions_dropped_by_cap", "regions_weak_dropped", "regions_below_evalue",
+                   "band_scored_regions", "band_rescored", "banded_cells", "tracebacks", "band_widenings",
+                   "band_capped", "end_anchored_tracebacks", "filtered_identity_coverage", "alignments",
+                   "regions_pruned"]
+
+
+class SearchCounters(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in SEARCH_COUNTERS] + [("reserved", C.c_uint64 * 6)]
+
+
+SEARCH_STAGES = ["prepare", "seed_tables", "scan", "sort", "xdrop", "regions", "plan", "prefilter", "score", "finish",
+                 "total"]
+
+
+class SearchTiming(C.Structure):
+    _fields_ = [(n, C.c_double) for n in SEARCH_STAGES] + [("reserved", C.c_double * 5)]
+
+
+class Prefilter(C.Structure):
+    _fields_ = [("model", C.c_void_p), ("mean", C.POINTER(C.c_float)), ("inv_std", C.POINTER(C.c_float)),
+                ("threshold", C.c_float), ("reserved", C.c_uint32)]
+
+
+class SearchRegion(C.Structure):
+    _fields_ = [("subject", C.c_uint64), ("d_min", C.c_int64), ("d_max", C.c_int64), ("best", C.c_int32),
+                ("n_hsp", C.c_uint32), ("hsp_sum", C.c_int32), ("score", C.c_int32), ("win_lo", C.c_uint64),
+                ("win_hi", C.c_uint64), ("band_lo", C.c_int64), ("band_hi", C.c_int64), ("banded", C.c_int32),
+                ("flags", C.c_uint32), ("evalue", C.c_double)]
+
+
 P = C.c_void_p
 _SIGS = {
+    "saber_hip_search_defaults": (None, [C.POINTER(SearchParams)]),
+    "saber_hip_seed_scheme_default": (None, [C.POINTER(SeedScheme)]),
+    "saber_hip_search_run": (C.c_int, [P, C.POINTER(SeqSet), C.POINTER(SeedScheme), C.POINTER(Scheme),
+                                       C.POINTER(Karlin), C.POINTER(SearchParams), C.POINTER(Prefilter),
+                                       C.POINTER(P)]),
+    "saber_hip_search_hit_count": (C.c_uint64, [P, C.c_uint32]),
+    "saber_hip_search_hits": (C.c_int, [P, C.c_uint32, C.POINTER(Hit)]),
+    "saber_hip_search_op_count": (C.c_uint64, [P, C.c_uint32]),
+    "saber_hip_search_ops": (C.c_int, [P, C.c_uint32, C.POINTER(C.c_uint8), C.POINTER(C.c_uint32)]),
+    "saber_hip_search_region_count": (C.c_uint64, [P, C.c_uint32]),
+    "saber_hip_search_regions": (C.c_int, [P, C.c_uint32, C.POINTER(SearchRegion)]),
+    "saber_hip_search_features": (C.c_int, [P, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_uint16)]),
+    "saber_hip_search_get_counters": (C.c_int, [P, C.POINTER(SearchCounters)]),
+    "saber_hip_search_get_timing": (C.c_int, [P, C.POINTER(SearchTiming)]),
+    "saber_hip_search_destroy": (None, [P]),
+    "saber_hip_composition_bias": (C.c_int, [C.POINTER(C.c_uint8), C.c_uint64, C.POINTER(C.c_int32),
+                                             C.POINTER(C.c_int8)]),
+    "saber_hip_traceback_ex": (C.c_int, [P, C.POINTER(SeqSet), C.POINTER(SeqSet), C.POINTER(TracebackJob),
+                                         C.c_uint64, C.POINTER(Scheme), C.c_int64, C.c_int, C.c_uint64,
+                                         C.POINTER(Alignment), C.POINTER(C.c_uint8), C.POINTER(C.c_uint8),
+                                         C.POINTER(C.c_uint32), C.c_uint64, C.POINTER(TracebackCounters),
+                                         C.POINTER(TracebackCounters)]),
     "saber_hip_version": (C.c_char_p, []),
     "saber_hip_last_error": (C.c_char_p, []),
     "saber_hip_device_count": (C.c_int, [C.POINTER(C.c_int)]),
@@ -280,6 +360,15 @@ def wlib():
         W.wl_sequences.restype = P
         W.wl_sequences.argtypes = [C.c_uint64, C.c_uint64, C.c_double, C.c_double, C.c_uint32,
                                    C.c_uint32, C.POINTER(C.c_int32)]
+        W.wl_search_set.restype = P
+        W.wl_search_set.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64, C.c_double, C.c_double, C.c_uint32,
+                                    C.c_uint32, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_int32)]
+        W.wl_family_set.restype = P
+        W.wl_family_set.argtypes = [C.c_uint64, C.c_uint64, C.c_double, C.c_double, C.c_uint32, C.c_uint32,
+                                    C.c_double, C.c_uint32, C.c_double, C.c_double, C.c_uint64, C.c_double,
+                                    C.c_double, C.c_uint32, C.POINTER(C.c_int32)]
+        W.wl_source.restype = None
+        W.wl_source.argtypes = [P, C.POINTER(C.c_int32)]
         W.wl_sizes.restype = None
         W.wl_sizes.argtypes = [P] + [C.POINTER(C.c_uint64)] * 4
         W.wl_copy.restype = None

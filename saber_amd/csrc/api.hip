@@ -26,51 +26,28 @@
 
 #include "../../include/saber_hip.h"
 #include "common.h"
+#include "ctx.h"
 
 using namespace saber;
 
-namespace {
+namespace saber {
 
+namespace {
 thread_local std::string g_err;
+}  // namespace
 
 int fail(int code, const std::string& msg) {
     g_err = msg;
     return code;
 }
 
-#define HIP_TRY(expr)                                                                   \
-    do {                                                                                \
-        hipError_t e_ = (expr);                                                         \
-        if (e_ != hipSuccess)                                                           \
-            return fail(SABER_HIP_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
-    } while (0)
+const char* g_err_cstr() { return g_err.c_str(); }
 
-constexpr uint32_t kNone = 0xFFFFFFFFu;
-
-struct Shape {
-    int L, C, K;
-    int W() const { return L * C; }
-    // Every lane caps its last K cells, so a shape serves widths W-K+1 .. W,
-    // and every width up to W when K = C.
-    bool covers(int64_t w) const { return w >= 1 && w <= W() && (K == C || w > W() - K); }
-    // VALU instructions of one lane per row: about 13 per packed cell, one per
-    // capped cell and a fixed part for the row and stream bookkeeping.
-    double row_cost() const { return 13.0 * C + K + 45.0; }
-    // Work of one job pair per row.
-    double pair_cost() const { return L * row_cost(); }
-};
+namespace {
 
 __global__ void fill_u32_kernel(uint32_t* p, uint64_t n, uint32_t v) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
-}
-
-hipError_t fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s) {
-    if (n == 0) return hipSuccess;
-    const uint32_t block = 256;
-    const uint64_t grid = (n + block - 1) / block;
-    hipLaunchKernelGGL(fill_u32_kernel, dim3((uint32_t)grid), dim3(block), 0, s, p, n, v);
-    return hipGetLastError();
 }
 
 // Residue codes >= 24 all score as X (src/alphabet.rs:39-41); every kernel
@@ -84,25 +61,39 @@ __global__ void canon_codes_kernel(uint8_t* p, uint64_t n) {
     }
 }
 
-template <class T>
-struct DevBuf {
-    T* p = nullptr;
-    uint64_t n = 0;
-    hipError_t alloc(uint64_t count) {
-        release();
-        n = count;
-        if (count == 0) return hipSuccess;
-        return hipMalloc(reinterpret_cast<void**>(&p), sizeof(T) * count);
-    }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-    }
-    ~DevBuf() { release(); }
-};
-
 }  // namespace
+
+hipError_t fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    const uint32_t block = 256;
+    const uint64_t grid = (n + block - 1) / block;
+    hipLaunchKernelGGL(fill_u32_kernel, dim3((uint32_t)grid), dim3(block), 0, s, p, n, v);
+    return hipGetLastError();
+}
+
+int check_seqset(const saber_hip_seqset* s, const char* what) {
+    if (!s) return fail(SABER_HIP_EINVAL, std::string(what) + " is null");
+    if (s->count > 0 && !s->offsets) return fail(SABER_HIP_EINVAL, std::string(what) + ": offsets null");
+    if (s->count > 0 && s->offsets[s->count] > 0 && !s->residues)
+        return fail(SABER_HIP_EINVAL, std::string(what) + ": residues null");
+    for (uint32_t i = 0; i < s->count; ++i)
+        if (s->offsets[i + 1] < s->offsets[i])
+            return fail(SABER_HIP_EINVAL, std::string(what) + ": offsets not monotone");
+    return SABER_HIP_OK;
+}
+
+int upload_codes(DevBuf<uint8_t>& d, const uint8_t* h, uint64_t n, hipStream_t s) {
+    if (int r = upload_guarded(d, h, n, s)) return r;
+    if (n) {
+        const uint32_t block = 256;
+        const uint64_t grid = (n + block - 1) / block;
+        hipLaunchKernelGGL(canon_codes_kernel, dim3((uint32_t)grid), dim3(block), 0, s, d.p + kGuard, n);
+        HIP_TRY(hipGetLastError());
+    }
+    return SABER_HIP_OK;
+}
+
+}  // namespace saber
 
 // One lane steps through every row of its job, so a job's time is at least
 // rows x the lane's per-row time. Jobs whose time at their throughput shape
@@ -110,31 +101,6 @@ struct DevBuf {
 // low per-row latency instead (short lanes, launched first). A context can
 // replace the rule by a fixed row count (saber_hip_band_tune).
 constexpr double kLatencyShare = 0.6;
-
-struct saber_hip_ctx {
-    int device = 0;
-    uint32_t flags = 0;
-    hipStream_t stream = nullptr;
-    hipStream_t work[4] = {nullptr, nullptr, nullptr, nullptr};
-    hipEvent_t fork = nullptr;
-    hipEvent_t join[4] = {nullptr, nullptr, nullptr, nullptr};
-    std::mutex mu;
-    DevBuf<uint8_t> db;
-    std::vector<uint64_t> db_off;
-    int cus = 0;                  // compute units of the device
-    // tuned full-matrix shapes: default shape id -> chosen shape id, by
-    // flavour (0 scores only, 1 end cells)
-    std::map<int, int> full_tuned[2];
-    std::vector<int> choice;      // width -> shape index (-1: exact int32 kernel)
-    std::vector<int> choice_lat;  // the same for jobs with many rows
-    std::vector<Shape> shapes;
-    int latency_rows = -1;  // -1: kLatencyShare rule; 0: off; else a row count
-};
-
-// Full-SW launches of one run are spread over this many streams (all at the
-// default priority) so small subject buckets and the tails of large ones
-// overlap.
-constexpr int kStreams = 4;
 
 // Rates for the band planner's time estimates (lane VALU instructions per
 // ms): the whole GPU at about half its issue rate, and one lane of a wave
@@ -258,55 +224,13 @@ std::vector<uint32_t> build_pair_table(const int32_t* matrix) {
     return t;
 }
 
-int check_seqset(const saber_hip_seqset* s, const char* what) {
-    if (!s) return fail(SABER_HIP_EINVAL, std::string(what) + " is null");
-    if (s->count > 0 && !s->offsets) return fail(SABER_HIP_EINVAL, std::string(what) + ": offsets null");
-    if (s->count > 0 && s->offsets[s->count] > 0 && !s->residues)
-        return fail(SABER_HIP_EINVAL, std::string(what) + ": residues null");
-    for (uint32_t i = 0; i < s->count; ++i)
-        if (s->offsets[i + 1] < s->offsets[i])
-            return fail(SABER_HIP_EINVAL, std::string(what) + ": offsets not monotone");
-    return SABER_HIP_OK;
-}
-
-template <class T>
-int upload(DevBuf<T>& d, const T* h, uint64_t n, hipStream_t s) {
-    HIP_TRY(d.alloc(n == 0 ? 1 : n));
-    if (n) HIP_TRY(hipMemcpyAsync(d.p, h, sizeof(T) * n, hipMemcpyHostToDevice, s));
-    return SABER_HIP_OK;
-}
-
-// Residue and bias buffers carry guard bytes so kernels may read a few bytes
-// before and after any sequence (clamped into [-16, len + 64)).
-constexpr uint64_t kGuard = 16, kGuardTail = 64;
-
-template <class T>
-int upload_guarded(DevBuf<T>& d, const T* h, uint64_t n, hipStream_t s) {
-    HIP_TRY(d.alloc(n + kGuard + kGuardTail));
-    HIP_TRY(hipMemsetAsync(d.p, 0, sizeof(T) * (n + kGuard + kGuardTail), s));
-    if (n) HIP_TRY(hipMemcpyAsync(d.p + kGuard, h, sizeof(T) * n, hipMemcpyHostToDevice, s));
-    return SABER_HIP_OK;
-}
-
-// A residue buffer: guarded upload, then every code >= 24 stored as X.
-int upload_codes(DevBuf<uint8_t>& d, const uint8_t* h, uint64_t n, hipStream_t s) {
-    if (int r = upload_guarded(d, h, n, s)) return r;
-    if (n) {
-        const uint32_t block = 256;
-        const uint64_t grid = (n + block - 1) / block;
-        hipLaunchKernelGGL(canon_codes_kernel, dim3((uint32_t)grid), dim3(block), 0, s, d.p + kGuard, n);
-        HIP_TRY(hipGetLastError());
-    }
-    return SABER_HIP_OK;
-}
-
 }  // namespace
 
 extern "C" {
 
 const char* saber_hip_version(void) { return "saber-mi355x 0.1 (gfx950)"; }
 
-const char* saber_hip_last_error(void) { return g_err.c_str(); }
+const char* saber_hip_last_error(void) { return saber::g_err_cstr(); }
 
 int saber_hip_device_count(int* count) {
     if (!count) return fail(SABER_HIP_EINVAL, "count is null");
@@ -349,7 +273,9 @@ void saber_hip_destroy(saber_hip_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (int k = 0; k < kStreams; ++k)
         if (c->work[k]) (void)hipStreamSynchronize(c->work[k]);
+    c->scan.reset();
     c->db.release();
+    c->db_off_dev.release();
     for (int k = 0; k < kStreams; ++k) {
         if (c->work[k]) (void)hipStreamDestroy(c->work[k]);
         if (c->join[k]) (void)hipEventDestroy(c->join[k]);
@@ -365,9 +291,17 @@ int saber_hip_upload_db(saber_hip_ctx* c, const saber_hip_seqset* subjects) {
     std::lock_guard<std::mutex> g(c->mu);
     HIP_TRY(hipSetDevice(c->device));
     const uint64_t nres = subjects->count ? subjects->offsets[subjects->count] : 0;
+    c->scan.reset();
+    ++c->db_gen;
     if (int r = upload_codes(c->db, subjects->residues, nres, c->stream)) return r;
+    if (int r = upload(c->db_off_dev, subjects->offsets, (uint64_t)subjects->count + 1, c->stream)) return r;
     HIP_TRY(hipStreamSynchronize(c->stream));
     c->db_off.assign(subjects->offsets, subjects->offsets + subjects->count + 1);
+    c->db_host.resize(nres);
+    for (uint64_t i = 0; i < nres; ++i) {
+        const uint8_t x = subjects->residues[i];
+        c->db_host[i] = x < 24 ? x : (uint8_t)kCodeX;
+    }
     return SABER_HIP_OK;
 }
 
@@ -1380,11 +1314,11 @@ int saber_hip_band_align_time(saber_hip_ctx* c, const saber_hip_seqset* queries,
 // round goes to one band_align call, the score passes of the jobs falling
 // back to their end cell in one score_bands call. Per job the decisions are
 // the reference's, in the reference's order.
-int saber_hip_traceback(saber_hip_ctx* c, const saber_hip_seqset* queries, const saber_hip_seqset* subjects,
-                        const saber_hip_traceback_job* jobs, uint64_t njobs, const saber_hip_scheme* scheme,
-                        int64_t band_pad, int exhaustive, uint64_t max_dir_bytes, saber_hip_alignment* out,
-                        uint8_t* exact, uint8_t* op_codes, uint32_t* op_lens, uint64_t op_capacity,
-                        saber_hip_traceback_counters* counters) {
+int saber_hip_traceback_ex(saber_hip_ctx* c, const saber_hip_seqset* queries, const saber_hip_seqset* subjects,
+                           const saber_hip_traceback_job* jobs, uint64_t njobs, const saber_hip_scheme* scheme,
+                           int64_t band_pad, int exhaustive, uint64_t max_dir_bytes, saber_hip_alignment* out,
+                           uint8_t* exact, uint8_t* op_codes, uint32_t* op_lens, uint64_t op_capacity,
+                           saber_hip_traceback_counters* counters, saber_hip_traceback_counters* job_counters) {
     if (counters) *counters = saber_hip_traceback_counters{};
     if (!c || !scheme) return fail(SABER_HIP_EINVAL, "null argument");
     if (njobs == 0) return SABER_HIP_OK;
@@ -1393,7 +1327,10 @@ int saber_hip_traceback(saber_hip_ctx* c, const saber_hip_seqset* queries, const
     if (int r = check_seqset(queries, "queries")) return r;
     if (subjects)
         if (int r = check_seqset(subjects, "subjects")) return r;
+    if (job_counters)
+        for (uint64_t k = 0; k < njobs; ++k) job_counters[k] = saber_hip_traceback_counters{};
     std::vector<uint64_t> slen;   // subject lengths
+    uint64_t gen = 0;             // the database this call reads (subjects == NULL)
     {
         std::lock_guard<std::mutex> g(c->mu);
         if (!subjects && !c->db.p) return fail(SABER_HIP_ESTATE, "no database uploaded");
@@ -1401,7 +1338,18 @@ int saber_hip_traceback(saber_hip_ctx* c, const saber_hip_seqset* queries, const
         const uint32_t ns = subjects ? subjects->count : (uint32_t)c->db_off.size() - 1;
         slen.resize(ns);
         for (uint32_t k = 0; k < ns; ++k) slen[k] = so[k + 1] - so[k];
+        gen = c->db_gen;
     }
+    // A database upload from another thread between two rounds would mix
+    // two databases in one result: every round checks the generation.
+    auto same_db = [&]() -> int {
+        if (subjects) return SABER_HIP_OK;
+        std::lock_guard<std::mutex> g(c->mu);
+        if (c->db_gen != gen) return fail(SABER_HIP_ESTATE, "the database was replaced during the traceback");
+        return SABER_HIP_OK;
+    };
+    // per-job counters beside the batch sums
+    auto jc = [&](uint64_t k) -> saber_hip_traceback_counters* { return job_counters ? &job_counters[k] : nullptr; };
     struct Tb {
         int64_t m, n, pad;
         int64_t base_lo, base_hi;
@@ -1478,6 +1426,7 @@ int saber_hip_traceback(saber_hip_ctx* c, const saber_hip_seqset* queries, const
     auto give_up = [&](uint64_t k) {
         Tb& t = T[k];
         cnt.band_capped += 1;
+        if (auto* p = jc(k)) p->band_capped += 1;
         if (t.have_best) finish(k, &t.best, t.bc.data(), t.bl.data(), false);
         else finish(k, nullptr, nullptr, nullptr, false);
     };
@@ -1515,11 +1464,13 @@ int saber_hip_traceback(saber_hip_ctx* c, const saber_hip_seqset* queries, const
                 Tb& t = T[ids[x]];
                 t.need_score = false;
                 cnt.banded_cells += ecells[x];
+                if (auto* p = jc(ids[x])) p->banded_cells += ecells[x];
                 if (esc[x] <= 0) {
                     give_up(ids[x]);
                     continue;
                 }
                 cnt.end_anchored_tracebacks += 1;
+                if (auto* p = jc(ids[x])) p->end_anchored_tracebacks += 1;
                 t.phase = 1;
                 t.target = std::max(esc[x], jobs[ids[x]].want);
                 t.d_end = (int64_t)ese[x] - (int64_t)eqe[x];
@@ -1566,6 +1517,9 @@ int saber_hip_traceback(saber_hip_ctx* c, const saber_hip_seqset* queries, const
         }
         if (ids.empty()) break;
         cnt.tracebacks += ids.size();
+        if (job_counters)
+            for (uint64_t k : ids) job_counters[k].tracebacks += 1;
+        if (int r = same_db()) return r;
         res.assign(ids.size(), saber_hip_alignment{});
         uint8_t* rcp = rc.get(std::max<uint64_t>(round_ops, 1));
         uint32_t* rlp = rl.get(std::max<uint64_t>(round_ops, 1));
@@ -1581,6 +1535,7 @@ int saber_hip_traceback(saber_hip_ctx* c, const saber_hip_seqset* queries, const
             if (t.phase == 0) {
                 if (a.status == 0) {
                     cnt.banded_cells += a.cells;
+                    if (auto* p = jc(k)) p->banded_cells += a.cells;
                     t.have_best = true;
                     t.best = a;
                     t.bc.assign(oc, oc + a.nops);
@@ -1601,6 +1556,7 @@ int saber_hip_traceback(saber_hip_ctx* c, const saber_hip_seqset* queries, const
             } else {
                 if (a.status == 0) {
                     cnt.banded_cells += a.cells;
+                    if (auto* p = jc(k)) p->banded_cells += a.cells;
                     if (a.score >= t.target || t.covers) {
                         finish(k, &a, oc, ol, a.score >= jobs[k].want);
                         continue;
@@ -1616,11 +1572,34 @@ int saber_hip_traceback(saber_hip_ctx* c, const saber_hip_seqset* queries, const
                 }
             }
             cnt.band_widenings += 1;
+            if (auto* p = jc(k)) p->band_widenings += 1;
             t.pad *= 2;
         }
     }
+    if (int r = same_db()) return r;
+    {
+        // the call's device buffers, released on the context's device
+        std::lock_guard<std::mutex> g(c->mu);
+        (void)hipSetDevice(c->device);
+        seqs.q.release();
+        seqs.sbuf.release();
+        seqs.bias.release();
+        seqs.work.release();
+        seqs.dir.release();
+        seqs.codes.release();
+        seqs.lens.release();
+    }
     if (counters) *counters = cnt;
     return SABER_HIP_OK;
+}
+
+int saber_hip_traceback(saber_hip_ctx* c, const saber_hip_seqset* queries, const saber_hip_seqset* subjects,
+                        const saber_hip_traceback_job* jobs, uint64_t njobs, const saber_hip_scheme* scheme,
+                        int64_t band_pad, int exhaustive, uint64_t max_dir_bytes, saber_hip_alignment* out,
+                        uint8_t* exact, uint8_t* op_codes, uint32_t* op_lens, uint64_t op_capacity,
+                        saber_hip_traceback_counters* counters) {
+    return saber_hip_traceback_ex(c, queries, subjects, jobs, njobs, scheme, band_pad, exhaustive, max_dir_bytes,
+                                  out, exact, op_codes, op_lens, op_capacity, counters, nullptr);
 }
 
 int saber_hip_score_windows(saber_hip_ctx* c, const uint8_t* query, uint32_t qlen,
@@ -2754,14 +2733,30 @@ extern "C" int saber_hip_score_regions(saber_hip_ctx* c, const saber_hip_seqset*
                                        const saber_hip_region* regs, const saber_hip_plan* plans, uint64_t n,
                                        const saber_hip_scheme* scheme, int64_t narrow_band, int64_t full_band,
                                        const double* thr, int32_t* out, uint8_t* rescored) {
+    return saber::score_regions_impl(c, queries, subjects, qidx, regs, plans, n, scheme, narrow_band, full_band,
+                                     thr, out, rescored, nullptr, nullptr);
+}
+
+int saber::score_regions_impl(saber_hip_ctx* c, const saber_hip_seqset* queries, const saber_hip_seqset* subjects,
+                              const uint32_t* qidx, const saber_hip_region* regs, const saber_hip_plan* plans,
+                              uint64_t n, const saber_hip_scheme* scheme, int64_t narrow_band, int64_t full_band,
+                              const double* thr, int32_t* out, uint8_t* rescored, uint64_t* banded_cells,
+                              const uint8_t* skip) {
+    if (banded_cells) *banded_cells = 0;
     if (!c || !queries || !scheme || (n && (!qidx || !regs || !plans || !out))) return fail(SABER_HIP_EINVAL, "null argument");
     if (int r = check_seqset(queries, "queries")) return r;
     const bool two_stage = narrow_band > 0 && narrow_band < full_band;
     if (two_stage && n && !thr) return fail(SABER_HIP_EINVAL, "thr is null");
     std::vector<saber_hip_band_job> narrow, full;
     std::vector<uint64_t> narrow_of, full_of;
+    std::vector<uint8_t> banded_job;   // full-pass job k comes from a banded plan
     for (uint64_t i = 0; i < n; ++i) {
         if (qidx[i] >= queries->count) return fail(SABER_HIP_EINVAL, "region query index out of range");
+        if (rescored) rescored[i] = 0;
+        if (skip && skip[i]) {
+            out[i] = 0;
+            continue;
+        }
         const saber_hip_region& r = regs[i];
         const saber_hip_plan& p = plans[i];
         if (p.hi < p.lo || r.subject > 0xFFFFFFFFull) return fail(SABER_HIP_EINVAL, "bad plan");
@@ -2780,6 +2775,7 @@ extern "C" int saber_hip_score_regions(saber_hip_ctx* c, const saber_hip_seqset*
                 j.band_hi = (int32_t)p.band_hi;
                 full.push_back(j);
                 full_of.push_back(i);
+                banded_job.push_back(1);
             }
         } else {
             // the whole window rectangle: the full band of query x window
@@ -2787,14 +2783,22 @@ extern "C" int saber_hip_score_regions(saber_hip_ctx* c, const saber_hip_seqset*
             j.band_hi = (int32_t)(nw - 1);
             full.push_back(j);
             full_of.push_back(i);
+            banded_job.push_back(0);
         }
-        if (rescored) rescored[i] = 0;
     }
+    std::vector<uint64_t> cells;
+    uint64_t* cells_p = nullptr;
     if (!narrow.empty()) {
         std::vector<int32_t> sc(narrow.size());
+        if (banded_cells) {
+            cells.assign(narrow.size(), 0);
+            cells_p = cells.data();
+        }
         if (int r = saber_hip_score_bands(c, queries, subjects, narrow.data(), narrow.size(), scheme, sc.data(),
-                                          nullptr, nullptr, nullptr))
+                                          nullptr, nullptr, cells_p))
             return r;
+        if (banded_cells)
+            for (uint64_t x : cells) *banded_cells += x;
         for (size_t k = 0; k < narrow.size(); ++k) {
             const uint64_t i = narrow_of[k];
             if ((double)sc[k] >= thr[i]) {
@@ -2803,6 +2807,7 @@ extern "C" int saber_hip_score_regions(saber_hip_ctx* c, const saber_hip_seqset*
                 j.band_hi = (int32_t)plans[i].band_hi;
                 full.push_back(j);
                 full_of.push_back(i);
+                banded_job.push_back(1);
                 if (rescored) rescored[i] = 1;
             } else {
                 out[i] = sc[k];
@@ -2811,9 +2816,17 @@ extern "C" int saber_hip_score_regions(saber_hip_ctx* c, const saber_hip_seqset*
     }
     if (!full.empty()) {
         std::vector<int32_t> sc(full.size());
+        cells_p = nullptr;
+        if (banded_cells) {
+            cells.assign(full.size(), 0);
+            cells_p = cells.data();
+        }
         if (int r = saber_hip_score_bands(c, queries, subjects, full.data(), full.size(), scheme, sc.data(), nullptr,
-                                          nullptr, nullptr))
+                                          nullptr, cells_p))
             return r;
+        if (banded_cells)
+            for (size_t k = 0; k < full.size(); ++k)
+                if (banded_job[k]) *banded_cells += cells[k];
         for (size_t k = 0; k < full.size(); ++k) out[full_of[k]] = sc[k];
     }
     return SABER_HIP_OK;

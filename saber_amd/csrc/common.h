@@ -157,6 +157,14 @@ struct XdropArgs {
 int xdrop_blocks_per_cu(bool bias, bool wide);
 // wide: some score lies outside int8 (int16 table in LDS).
 hipError_t launch_xdrop(const XdropArgs& a, bool bias, bool wide, int grid, hipStream_t s);
+// The X-drop kernel alone, over descriptors already in a.desc (32 B per hit:
+// absolute query / subject positions of the seed, its query position, the
+// steps available to the right including the seed and to the left).
+hipError_t launch_xdrop_described(const XdropArgs& a, bool bias, bool wide, int grid, hipStream_t s);
+struct alignas(16) XdropDescDev {
+    uint64_t q_abs, s_abs;
+    uint32_t qp, n_right, n_left, pad;
+};
 
 }  // namespace saber
 

@@ -201,4 +201,61 @@ int saber_hip_plan_windows(const saber_hip_region* regs, uint64_t n, uint64_t ql
     return SABER_HIP_OK;
 }
 
+
+// composition_bias (src/scoring/bias.rs:56-122): the per-position correction
+// c_i = sum_a p_a S(q_i, a) - (sum over the +-20 window, j != i, of
+// S(q_i, q_j)) / (|W| - 1), rounded half away from zero, clamped to +-127;
+// 0 for non-standard residues and for queries shorter than 2. Floating-point
+// operations in the reference's order (Robinson frequencies normalised by
+// their left-to-right sum, background scores summed a = 0..19), so every
+// rounding is the reference's.
+int saber_hip_composition_bias(const uint8_t* query, uint64_t n, const int32_t* matrix, int8_t* out) {
+    if (n > 0 && (!query || !out)) return 1;
+    if (!matrix) return 1;
+    for (uint64_t i = 0; i < n; ++i) out[i] = 0;
+    if (n < 2) return 0;
+    static const double kRobinson[20] = {78.05, 51.29, 44.87, 53.64, 19.25, 42.64, 62.95,
+                                         73.77, 21.99, 51.42, 90.19, 57.44, 22.43, 38.56,
+                                         52.03, 71.20, 58.41, 13.30, 32.16, 64.41};
+    double tot = 0.0;
+    for (double v : kRobinson) tot += v;
+    double p[20];
+    for (int a = 0; a < 20; ++a) p[a] = kRobinson[a] / tot;
+    double bg[20];
+    for (int r = 0; r < 20; ++r) {
+        double acc = 0.0;
+        for (int a = 0; a < 20; ++a) acc += p[a] * (double)matrix[r * 24 + a];
+        bg[r] = acc;
+    }
+    int32_t acc[20] = {0};
+    const uint64_t half = 20;
+    uint64_t lo = 0, hi = 0;   // window [lo, hi)
+    for (uint64_t i = 0; i < n; ++i) {
+        const uint64_t want_hi = std::min(i + half + 1, n);
+        const uint64_t want_lo = i > half ? i - half : 0;
+        while (hi < want_hi) {
+            const int c = std::min<int>(query[hi], 23);
+            for (int r = 0; r < 20; ++r) acc[r] += matrix[r * 24 + c];
+            ++hi;
+        }
+        while (lo < want_lo) {
+            const int c = std::min<int>(query[lo], 23);
+            for (int r = 0; r < 20; ++r) acc[r] -= matrix[r * 24 + c];
+            ++lo;
+        }
+        const int r = query[i];
+        if (r < 20) {
+            const int32_t w = (int32_t)(hi - lo);
+            if (w >= 2) {
+                const int32_t own = matrix[r * 24 + r];
+                const double neigh = (double)(acc[r] - own) / (double)(w - 1);
+                const double f = bg[r] - neigh;
+                const int32_t v = (int32_t)(f < 0.0 ? f - 0.5 : f + 0.5);
+                out[i] = (int8_t)std::max(-127, std::min(127, v));
+            }
+        }
+    }
+    return 0;
+}
+
 }  // extern "C"

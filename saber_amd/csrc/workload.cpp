@@ -112,6 +112,7 @@ struct Batch {
     std::vector<uint64_t> qoff{0}, soff{0};
     std::vector<int32_t> band_lo, band_hi;
     std::vector<uint8_t> homolog;
+    std::vector<int32_t> source;
 };
 
 }  // namespace
@@ -179,6 +180,105 @@ void* wl_sequences(uint64_t seed, uint64_t n, double median, double sigma, uint3
         B->qoff.push_back(B->qres.size());
     }
     return B;
+}
+
+// Search set (blastp workloads): nq queries with lognormal lengths; nsub
+// subjects, each with probability homolog_frac a planted homologue of a
+// uniformly chosen query (the whole query mutated at an identity uniform in
+// [id_lo, id_hi] with geometric indels 0.02/residue, mean 3, between random
+// flanks of 0..30 residues), else an unrelated sequence of lognormal length.
+// source[k] = query index + 1 of subject k's template (0 = unrelated).
+void* wl_search_set(uint64_t seed, uint64_t nq, uint64_t nsub, double median, double sigma, uint32_t lmin,
+                    uint32_t lmax, double homolog_frac, double id_lo, double id_hi, const int32_t* mat) {
+    auto* B = new Batch();
+    Model M(mat);
+    Rng r(seed);
+    for (uint64_t k = 0; k < nq; ++k) {
+        const uint32_t m = lognormal_len(r, median, sigma, lmin, lmax);
+        for (uint32_t i = 0; i < m; ++i) B->qres.push_back(M.residue(r));
+        B->qoff.push_back(B->qres.size());
+    }
+    std::vector<uint8_t> tmp;
+    for (uint64_t k = 0; k < nsub; ++k) {
+        if (nq > 0 && r.uni() < homolog_frac) {
+            const uint64_t q = r.below(nq);
+            const double ident = id_lo + (id_hi - id_lo) * r.uni();
+            const uint32_t lf = (uint32_t)r.below(31), rf = (uint32_t)r.below(31);
+            for (uint32_t t = 0; t < lf; ++t) B->sres.push_back(M.residue(r));
+            tmp.clear();
+            mutate(M, r, B->qres.data() + B->qoff[q], (uint32_t)(B->qoff[q + 1] - B->qoff[q]), ident, 0.02, 3.0,
+                   tmp);
+            B->sres.insert(B->sres.end(), tmp.begin(), tmp.end());
+            for (uint32_t t = 0; t < rf; ++t) B->sres.push_back(M.residue(r));
+            B->source.push_back((int32_t)(q + 1));
+        } else {
+            const uint32_t n = lognormal_len(r, median, sigma, lmin, lmax);
+            for (uint32_t t = 0; t < n; ++t) B->sres.push_back(M.residue(r));
+            B->source.push_back(0);
+        }
+        B->soff.push_back(B->sres.size());
+    }
+    return B;
+}
+
+// Family set (the self-search at scale): nq queries in planted families
+// (a family's root has lognormal length; its members are mutated copies of
+// the root at an identity uniform in [id_lo, id_hi] with geometric indels;
+// sizes 2 + geometric(mean fam_extra), at most fam_max, the last family cut
+// to fill nq), in shuffled order; the database is the queries followed by
+// ndecoy reversed unrelated sequences of lognormal length (decoy_median,
+// decoy_sigma, clamped to [lmin, decoy_max]). source[k] = family id of
+// query k.
+void* wl_family_set(uint64_t seed, uint64_t nq, double median, double sigma, uint32_t lmin, uint32_t lmax,
+                    double fam_extra, uint32_t fam_max, double id_lo, double id_hi, uint64_t ndecoy,
+                    double decoy_median, double decoy_sigma, uint32_t decoy_max, const int32_t* mat) {
+    auto* B = new Batch();
+    Model M(mat);
+    Rng r(seed);
+    std::vector<std::vector<uint8_t>> members;
+    std::vector<int32_t> fam;
+    std::vector<uint8_t> root, tmp;
+    int32_t f = 0;
+    while (members.size() < nq) {
+        uint32_t size = 2;
+        const double p = 1.0 / (1.0 + fam_extra);
+        while (size < fam_max && r.uni() >= p) ++size;
+        const uint32_t m = lognormal_len(r, median, sigma, lmin, lmax);
+        root.clear();
+        for (uint32_t i = 0; i < m; ++i) root.push_back(M.residue(r));
+        for (uint32_t k = 0; k < size && members.size() < nq; ++k) {
+            tmp.clear();
+            mutate(M, r, root.data(), m, id_lo + (id_hi - id_lo) * r.uni(), 0.02, 3.0, tmp);
+            if (tmp.empty()) tmp.push_back(M.residue(r));
+            members.push_back(tmp);
+            fam.push_back(f);
+        }
+        ++f;
+    }
+    // shuffle (Fisher-Yates)
+    std::vector<uint64_t> order(members.size());
+    for (uint64_t i = 0; i < order.size(); ++i) order[i] = i;
+    for (uint64_t i = order.size(); i > 1; --i) std::swap(order[i - 1], order[r.below(i)]);
+    for (uint64_t i : order) {
+        B->qres.insert(B->qres.end(), members[i].begin(), members[i].end());
+        B->qoff.push_back(B->qres.size());
+        B->source.push_back(fam[i]);
+    }
+    B->sres = B->qres;
+    B->soff = B->qoff;
+    for (uint64_t k = 0; k < ndecoy; ++k) {
+        const uint32_t n = lognormal_len(r, decoy_median, decoy_sigma, lmin, decoy_max);
+        tmp.clear();
+        for (uint32_t t = 0; t < n; ++t) tmp.push_back(M.residue(r));
+        B->sres.insert(B->sres.end(), tmp.rbegin(), tmp.rend());
+        B->soff.push_back(B->sres.size());
+    }
+    return B;
+}
+
+void wl_source(void* h, int32_t* out) {
+    auto* B = static_cast<Batch*>(h);
+    if (out && !B->source.empty()) std::memcpy(out, B->source.data(), 4 * B->source.size());
 }
 
 void wl_sizes(void* h, uint64_t* nqres, uint64_t* nq, uint64_t* nsres, uint64_t* ns) {

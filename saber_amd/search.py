@@ -1,0 +1,207 @@
+"""The scan schedule end to end on the GPU (C ABI saber_hip_search_*), under
+the reference's names: SeedAlphabet / SeedShape / ScanSeeds
+(src/alphabet.rs:280-384, src/search/engine/scan.rs:28-33), SearchParams
+(src/search/engine.rs:62-197, default preset src/search/presets.rs:129-140),
+Engine.search_scan (scan.rs:1261-1497) and Hit (engine.rs:312-357).
+
+search() returns, per query, the reported hits (subject, score, coordinates,
+CIGAR, E-value, exact flag), the capped candidate regions with their plans
+and scores, and the run's counters and stage timings.
+"""
+from dataclasses import dataclass, field
+import ctypes as C
+
+
+import numpy as np
+
+from . import _ffi
+from .matrices import ALPHABET
+
+MURPHY10_GROUPS = "A KR EDNQ C G H ILVM FYW P ST"
+# presets.rs:41-58 (DIAMOND's very-sensitive spaced seeds)
+DS16 = ["11110111", "11100100111", "110010101011", "11010001001011", "10101100001101", "110100100100011",
+        "1010010100010011", "1100101000001011", "11100000100010101", "11000100010010011", "11010000001000111",
+        "110001001000010011", "1010001000100001011", "1100010100000010011", "1100100000101000011",
+        "1101000010000001011"]
+
+
+def alphabet_map(groups):
+    """SeedAlphabet::from_groups (alphabet.rs:313-333): map[code] = group, 255 = not seedable."""
+    m = np.full(24, 255, np.uint8)
+    g = 0
+    for grp in groups.split():
+        for ch in grp:
+            code = ALPHABET.index(ch)
+            if code >= 20 or m[code] != 255:
+                raise ValueError(f"bad group letter {ch}")
+            m[code] = g
+        g += 1
+    if (m[:20] == 255).any():
+        raise ValueError("reduced alphabet must assign all 20 standard amino acids")
+    return m, g
+
+
+def full_alphabet():
+    """SeedAlphabet::full (alphabet.rs:291-297)."""
+    m = np.full(24, 255, np.uint8)
+    m[:20] = np.arange(20)
+    return m, 20
+
+
+@dataclass
+class ScanSeeds:
+    alpha_map: np.ndarray
+    alpha_size: int
+    shapes: list
+
+    @staticmethod
+    def default():
+        m, n = alphabet_map(MURPHY10_GROUPS)
+        return ScanSeeds(m, n, list(DS16))
+
+    def ffi(self):
+        s = _ffi.SeedScheme()
+        for i in range(24):
+            s.alphabet_map[i] = int(self.alpha_map[i])
+        s.alphabet_size = int(self.alpha_size)
+        s.nshapes = len(self.shapes)
+        for i, p in enumerate(self.shapes):
+            s.shapes[i].value = p.encode()
+        return s
+
+
+def default_params(**over):
+    """SearchParams of the default preset with the CLI defaults, overridden by keyword."""
+    p = _ffi.SearchParams()
+    _ffi.lib().saber_hip_search_defaults(C.byref(p))
+    for k, v in over.items():
+        if not hasattr(p, k):
+            raise KeyError(k)
+        setattr(p, k, v)
+    return p
+
+
+def composition_bias(query, matrix):
+    """composition_bias (src/scoring/bias.rs:80-122), the library's host code."""
+    q = np.ascontiguousarray(query, np.uint8)
+    m = np.ascontiguousarray(np.asarray(matrix, np.int32).reshape(576))
+    out = np.zeros(max(len(q), 1), np.int8)
+    rc = _ffi.lib().saber_hip_composition_bias(_ffi.ptr(q, C.c_uint8), len(q), _ffi.ptr(m, C.c_int32),
+                                               _ffi.ptr(out, C.c_int8))
+    if rc != 0:
+        raise ValueError("composition_bias: bad arguments")
+    return out[:len(q)].copy()
+
+
+@dataclass
+class Hit:
+    subject: int
+    score: int
+    bitscore: float
+    evalue: float
+    q_start: int
+    q_end: int
+    s_start: int
+    s_end: int
+    columns: int
+    matches: int
+    mismatches: int
+    positives: int
+    gap_opens: int
+    gap_columns: int
+    q_len: int
+    s_len: int
+    ops: list
+    exact_in_window: bool
+
+    def cigar(self):
+        return "".join(f"{l}{chr(o)}" for o, l in self.ops)
+
+    def key(self):
+        """The engine_consistency.rs:113-117 key."""
+        return (self.subject, self.score, self.q_start, self.q_end, self.s_start, self.s_end, self.cigar(),
+                self.exact_in_window)
+
+
+REGION_FIELDS = ["subject", "d_min", "d_max", "best", "n_hsp", "hsp_sum", "score", "win_lo", "win_hi", "band_lo",
+                 "band_hi", "banded", "flags", "evalue"]
+REGION_DTYPE = np.dtype([("subject", "<u8"), ("d_min", "<i8"), ("d_max", "<i8"), ("best", "<i4"), ("n_hsp", "<u4"),
+                         ("hsp_sum", "<i4"), ("score", "<i4"), ("win_lo", "<u8"), ("win_hi", "<u8"),
+                         ("band_lo", "<i8"), ("band_hi", "<i8"), ("banded", "<i4"), ("flags", "<u4"),
+                         ("evalue", "<f8")])
+assert REGION_DTYPE.itemsize == C.sizeof(_ffi.SearchRegion)
+
+
+@dataclass
+class SearchResult:
+    hits: list                  # per query: list of Hit
+    regions: list               # per query: REGION_DTYPE array
+    counters: dict
+    timing: dict = field(default_factory=dict)
+
+
+def run(ctx, qres, qoff, scheme, karlin, seeds=None, params=None, prefilter=None, keep_handle=False):
+    """Search queries (alignment codes, offsets) against ctx's uploaded database."""
+    seeds = seeds or ScanSeeds.default()
+    params = params or default_params()
+    qres = np.ascontiguousarray(qres, np.uint8)
+    qoff = np.ascontiguousarray(qoff, np.uint64)
+    nq = len(qoff) - 1
+    qs = _ffi.SeqSet(_ffi.ptr(qres, C.c_uint8), _ffi.ptr(qoff, C.c_uint64), nq, None)
+    sch = scheme.ffi()
+    ss = seeds.ffi()
+    h = C.c_void_p()
+    pf = None
+    if prefilter is not None:
+        pf = _ffi.Prefilter(prefilter["model"].h, _ffi.ptr(prefilter.get("mean"), C.c_float),
+                            _ffi.ptr(prefilter.get("inv_std"), C.c_float), float(prefilter["threshold"]), 0)
+    _ffi.check(_ffi.lib().saber_hip_search_run(ctx.h, C.byref(qs), C.byref(ss), C.byref(sch), C.byref(karlin._k),
+                                                C.byref(params), C.byref(pf) if pf is not None else None,
+                                                C.byref(h)))
+    L = _ffi.lib()
+    try:
+        hits, regions = [], []
+        for q in range(nq):
+            n = L.saber_hip_search_hit_count(h, q)
+            arr = (_ffi.Hit * max(n, 1))()
+            _ffi.check(L.saber_hip_search_hits(h, q, arr))
+            no = L.saber_hip_search_op_count(h, q)
+            oc = np.zeros(max(no, 1), np.uint8)
+            ol = np.zeros(max(no, 1), np.uint32)
+            _ffi.check(L.saber_hip_search_ops(h, q, _ffi.ptr(oc, C.c_uint8), _ffi.ptr(ol, C.c_uint32)))
+            hl = []
+            for k in range(n):
+                x = arr[k]
+                ops = [(int(oc[x.op_off + t]), int(ol[x.op_off + t])) for t in range(x.nops)]
+                hl.append(Hit(x.subject, x.score, x.bitscore, x.evalue, x.q_start, x.q_end, x.s_start, x.s_end,
+                              x.columns, x.matches, x.mismatches, x.positives, x.gap_opens, x.gap_columns, x.q_len,
+                              x.s_len, ops, bool(x.exact_in_window)))
+            hits.append(hl)
+            nr = L.saber_hip_search_region_count(h, q)
+            r = np.zeros(nr, REGION_DTYPE)
+            if nr:
+                _ffi.check(L.saber_hip_search_regions(h, q, r.ctypes.data_as(C.POINTER(_ffi.SearchRegion))))
+            regions.append(r)
+        c = _ffi.SearchCounters()
+        _ffi.check(L.saber_hip_search_get_counters(h, C.byref(c)))
+        t = _ffi.SearchTiming()
+        _ffi.check(L.saber_hip_search_get_timing(h, C.byref(t)))
+        res = SearchResult(hits, regions, {k: int(getattr(c, k)) for k in _ffi.SEARCH_COUNTERS},
+                           {k: float(getattr(t, k)) for k in _ffi.SEARCH_STAGES})
+        if keep_handle:
+            res.handle = h
+            h = None
+        return res
+    finally:
+        if h is not None and h.value:
+            L.saber_hip_search_destroy(h)
+
+
+def destroy(handle):
+    if handle is not None and handle.value:
+        _ffi.lib().saber_hip_search_destroy(handle)
+
+
+__all__ = ["ScanSeeds", "default_params", "composition_bias", "run", "Hit", "SearchResult", "alphabet_map",
+           "full_alphabet", "MURPHY10_GROUPS", "DS16", "REGION_DTYPE", "destroy"]
+

@@ -69,6 +69,9 @@ class QueryStatistics:
         return _ffi.lib().saber_hip_stats_floor_lower_bound(C.byref(self._q), float(evalue_cut), mb,
                                                              int(subject_len), int(db_floor))
 
+    def evalue_in(self, raw, space):
+        return _ffi.lib().saber_hip_stats_evalue_in(C.byref(self._q), int(raw), float(space))
+
     def evalue_for(self, raw, subject_len):
         return _ffi.lib().saber_hip_stats_evalue_for(C.byref(self._q), int(raw), int(subject_len))
 

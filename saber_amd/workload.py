@@ -89,3 +89,45 @@ def xdrop_hits(regions, seed, per_region=2):
     sp = np.where(on_diag, qp + c, (rng.random(n * per_region) * sl).astype(np.int64))
     sp = np.clip(sp, 0, sl - 1)
     return np.stack([q, s, qp, sp], axis=1).astype(np.uint32)
+
+
+def _take(h, nsrc):
+    W = _ffi.wlib()
+    try:
+        a, b, c, d = C.c_uint64(), C.c_uint64(), C.c_uint64(), C.c_uint64()
+        W.wl_sizes(h, C.byref(a), C.byref(b), C.byref(c), C.byref(d))
+        qres = np.zeros(max(a.value, 1), np.uint8)
+        qoff = np.zeros(b.value + 1, np.uint64)
+        sres = np.zeros(max(c.value, 1), np.uint8)
+        soff = np.zeros(d.value + 1, np.uint64)
+        W.wl_copy(h, _ffi.ptr(qres, C.c_uint8), _ffi.ptr(qoff, C.c_uint64), _ffi.ptr(sres, C.c_uint8),
+                  _ffi.ptr(soff, C.c_uint64), None, None, None)
+        src = np.zeros(max(nsrc(b.value, d.value), 1), np.int32)
+        W.wl_source(h, _ffi.ptr(src, C.c_int32))
+    finally:
+        W.wl_free(h)
+    return qres[:a.value], qoff, sres[:c.value], soff, src[:nsrc(b.value, d.value)]
+
+
+def search_set(seed=0, nq=100, nsub=5000, median=170.0, sigma=0.6, lmin=30, lmax=1000, homolog_frac=0.3,
+               id_lo=0.25, id_hi=0.60, matrix=BLOSUM62):
+    """The small blastp workload: nq queries, nsub subjects of which a fraction
+    are planted homologues of a query (source = query index + 1, 0 = none)."""
+    mat = _mat(matrix)
+    h = _ffi.wlib().wl_search_set(seed, nq, nsub, median, sigma, lmin, lmax, homolog_frac, id_lo, id_hi,
+                                  _ffi.ptr(mat, C.c_int32))
+    qres, qoff, sres, soff, src = _take(h, lambda nq_, ns_: ns_)
+    return dict(qres=qres, qoff=qoff, sres=sres, soff=soff, source=src)
+
+
+def family_set(seed=0, nq=11206, median=170.0, sigma=0.6, lmin=30, lmax=1000, fam_extra=3.6, fam_max=30,
+               id_lo=0.20, id_hi=0.50, ndecoy=320000, decoy_median=380.0, decoy_sigma=0.6, decoy_max=3000,
+               matrix=BLOSUM62):
+    """The self-search at scale: nq queries in planted families (family id per
+    query), searched against themselves plus ndecoy reversed unrelated
+    sequences (the database: queries first)."""
+    mat = _mat(matrix)
+    h = _ffi.wlib().wl_family_set(seed, nq, median, sigma, lmin, lmax, fam_extra, fam_max, id_lo, id_hi, ndecoy,
+                                  decoy_median, decoy_sigma, decoy_max, _ffi.ptr(mat, C.c_int32))
+    qres, qoff, sres, soff, src = _take(h, lambda nq_, ns_: nq_)
+    return dict(qres=qres, qoff=qoff, sres=sres, soff=soff, family=src)
