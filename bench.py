@@ -157,6 +157,132 @@ def prefilter_leg(ctx, args, sch):
     return out
 
 
+def cpu_info():
+    """CPU model and physical core count of the host (lscpu), as the baseline states them."""
+    model, sockets, cores, threads = None, 1, None, os.cpu_count()
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        kv = {k.strip(): v.strip() for k, v in (l.split(":", 1) for l in out.splitlines() if ":" in l)}
+        model = kv.get("Model name")
+        sockets = int(kv.get("Socket(s)", "1"))
+        cores = int(kv.get("Core(s) per socket", "0")) * sockets or None
+    except Exception:
+        pass
+    return {"model": model, "physical_cores": cores, "logical_cpus": threads}
+
+
+def search_leg(ctx, args, sch):
+    """The blastp workloads end to end on the GPU (saber_hip_search_run).
+
+    Small (100 queries x 5000 subjects): hit lists compared query by query
+    with the oracle's plain CPU search, which is also timed (one host core).
+    At scale (11,206 queries in ~2,000 planted families against themselves
+    plus 320,000 reversed decoys, ~148 M residues): queries/s end to end and
+    per stage, with size-independent checks -- every query's self hit is
+    reported with at least the score of the best ungapped segment of its main
+    diagonal, a sample of hits rescores to its score from its CIGAR, and a
+    repeated call gives the same hits.
+    """
+    from oracle import search as OS
+    from saber_amd import BLOSUM62, search, stats, workload
+    k = stats.gapped_protein(62, 11, 1)
+    out = {}
+    # small workload
+    d = workload.search_set(seed=0)
+    ns = len(d["soff"]) - 1
+    ctx.upload_db(d["sres"], d["soff"])
+    search.run(ctx, d["qres"], d["qoff"], sch, k)          # builds the database's seed-scan data
+    t0 = time.perf_counter()
+    g = search.run(ctx, d["qres"], d["qoff"], sch, k)
+    gw = time.perf_counter() - t0
+    st = stats.SearchStatistics(k, int(d["soff"][-1]), ns, stats.PER_SUBJECT)
+    sd = search.ScanSeeds.default()
+    t0 = time.perf_counter()
+    want, wctr = OS.search(d["qres"], d["qoff"], d["sres"], d["soff"], BLOSUM62, 11, 1, st, sd.alpha_map,
+                           sd.alpha_size, sd.shapes)
+    cw = time.perf_counter() - t0
+    bad = sum(int([h.key() for h in a] != [OS.hit_key(h) for h in b]) for a, b in zip(g.hits, want))
+    nq = len(d["qoff"]) - 1
+    out["small"] = {"workload": "100 queries x 5000 subjects, lognormal median 170 sigma 0.6 [30,1000], 30% planted "
+                                "homologues at 25-60% identity, default preset (Murphy-10, 16 weight-7 seeds, "
+                                "window filter 27), composition bias on, masking off, E <= 10",
+                    "queries_per_s": round(nq / gw, 1), "wall_ms": round(gw * 1e3, 2),
+                    "hits": int(sum(len(h) for h in g.hits)), "parity_queries": nq, "parity_mismatches": bad,
+                    "counters_equal": all(g.counters[c] == wctr[c] for c in wctr if c in g.counters),
+                    "cpu_plain_queries_per_s": round(nq / cw, 2), "cpu_plain": "oracle/search.py (scan schedule "
+                    "restated in C, scoring and traceback by the oracle), 1 host core"}
+    print(f"# search small: GPU {gw * 1e3:.1f} ms ({nq / gw:.0f} queries/s), oracle {cw:.1f} s, "
+          f"mismatching queries {bad}/{nq}", file=sys.stderr, flush=True)
+    if args.search_queries <= 0:
+        return out
+    # at scale
+    t0 = time.perf_counter()
+    d = workload.family_set(seed=0, nq=args.search_queries, ndecoy=args.search_decoys)
+    gen = time.perf_counter() - t0
+    nq, ns = len(d["qoff"]) - 1, len(d["soff"]) - 1
+    ctx.upload_db(d["sres"], d["soff"])
+    t0 = time.perf_counter()
+    g0 = search.run(ctx, d["qres"], d["qoff"], sch, k)
+    first = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    g = search.run(ctx, d["qres"], d["qoff"], sch, k)
+    wall = time.perf_counter() - t0
+    fam = d["family"]
+    # every query finds itself, with at least the best ungapped segment of
+    # its main diagonal (a lower bound of the local alignment score)
+    self_ok = 0
+    for qi in range(nq):
+        q = d["qres"][int(d["qoff"][qi]):int(d["qoff"][qi + 1])]
+        b = search.composition_bias(q, BLOSUM62)
+        best = run_ = 0
+        for c, bi in zip(q.tolist(), b.tolist()):
+            run_ = max(0, run_ + BLOSUM62[c][c] + int(bi))
+            best = max(best, run_)
+        h = [x for x in g.hits[qi] if x.subject == qi]
+        self_ok += int(bool(h) and h[0].score >= best)
+    rng = np.random.default_rng(3)
+    flat = [(qi, x) for qi, hl in enumerate(g.hits) for x in hl]
+    rs_bad = 0
+    for i in rng.choice(len(flat), size=min(300, len(flat)), replace=False).tolist():
+        qi, x = flat[i]
+        q = d["qres"][int(d["qoff"][qi]):int(d["qoff"][qi + 1])]
+        s = d["sres"][int(d["soff"][x.subject]):int(d["soff"][x.subject + 1])]
+        b = search.composition_bias(q, BLOSUM62)
+        sc, a, j = 0, x.q_start - 1, x.s_start - 1
+        for op, l in x.ops:
+            if op == ord("M"):
+                for _ in range(l):
+                    sc += BLOSUM62[int(q[a])][int(s[j])] + int(b[a])
+                    a += 1
+                    j += 1
+            else:
+                sc -= 11 + (l - 1)
+                a, j = (a + l, j) if op == ord("I") else (a, j + l)
+        rs_bad += int(sc != x.score or a != x.q_end or j != x.s_end)
+    same = sum(1 for qi, hl in enumerate(g.hits) for x in hl
+               if x.subject < nq and x.subject != qi and fam[x.subject] == fam[qi])
+    t = g.timing
+    scoring = sum(t[s_] for s_ in ("sort", "xdrop", "regions", "plan", "prefilter", "score", "finish"))
+    out["scale"] = {"workload": f"{nq} queries (lognormal median 170 sigma 0.6 [30,1000], {int(fam.max()) + 1} planted "
+                                f"families of 2-30 members at 20-50% identity) vs themselves + {args.search_decoys} "
+                                f"reversed decoys (lognormal median 380 sigma 0.6 [30,3000]), {ns} subjects, "
+                                f"{int(d['sres'].size)} residues; default preset, masking off",
+                    "queries_per_s": round(nq / wall, 1), "wall_s": round(wall, 3),
+                    "first_call_wall_s": round(first, 3), "candidate_scoring_ms": round(scoring, 1),
+                    "candidate_scoring_queries_per_s": round(nq / (scoring * 1e-3), 1),
+                    "stages_ms": {a: round(v, 1) for a, v in t.items()}, "counters": g.counters,
+                    "hits": len(flat), "same_family_hits": same, "self_hits": self_ok,
+                    "rescore_sample": min(300, len(flat)), "rescore_mismatches": rs_bad,
+                    "repeat_identical": [[x.key() for x in a] for a in g0.hits] == [[x.key() for x in a] for a in g.hits],
+                    "generate_s": round(gen, 1),
+                    "reference_published": "132 queries/s: 11,206 SCOP40 queries vs 146.8 M residues, 2 threads "
+                                           "Xeon @2.80 GHz (README.md:64, bench/results/scaling/times.jsonl)"}
+    print(f"# search at scale: {nq} x {ns}: {wall:.2f} s ({nq / wall:.0f} queries/s), first call {first:.2f} s; "
+          f"self hits {self_ok}/{nq}, rescore mismatches {rs_bad}", file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -183,6 +309,10 @@ def main():
     ap.add_argument("--pf-recall", type=float, default=0.995,
                     help="recall of the prefilter's cut on the validation set")
     ap.add_argument("--pf-epochs", type=int, default=12)
+    ap.add_argument("--search-queries", type=int, default=11206,
+                    help="queries of the at-scale blastp leg (0: small workload only)")
+    ap.add_argument("--search-decoys", type=int, default=320000)
+    ap.add_argument("--no-search", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only time a plain device copy of this many bytes and exit")
     args = ap.parse_args()
@@ -385,6 +515,7 @@ def main():
               f"mismatches {fbad}/40", file=sys.stderr, flush=True)
 
     pf = prefilter_leg(ctx, args, sch) if args.pf_train > 0 else None
+    srch = None if args.no_search else search_leg(ctx, args, sch)
 
     cpu = None
     if not args.no_cpu:
@@ -414,7 +545,15 @@ def main():
         dt = time.perf_counter() - t0
         sc = int(cell_arr[:nsample].sum()) * reps
         cpu_ok = bool(np.array_equal(cpu_scores, score[:nsample]))
+        # bands wider than 126 (the u8 kernel's limit, band_simd8.rs:38) take
+        # the int16 path in the reference's CPU code
+        sj = d["jobs"][:nsample]
+        m_ = (d["qoff"][sj["query"].astype(np.int64) + 1] - d["qoff"][sj["query"].astype(np.int64)]).astype(np.int64)
+        n_ = (sj["win_hi"].astype(np.int64) - sj["win_lo"].astype(np.int64))
+        w_ = np.minimum(sj["band_hi"].astype(np.int64), n_ - 1) - np.maximum(sj["band_lo"].astype(np.int64), -(m_ - 1)) + 1
+        i16_share = float(cell_arr[:nsample][w_ > 126].sum() / max(1, cell_arr[:nsample].sum()))
         cpu = {"value": round(sc / dt / 1e9, 3), "unit": "GCUPS", "cores": nthreads, "kind": "port",
+               "host": cpu_info(), "threads_used": nthreads, "int16_path_cell_share": round(i16_share, 4),
                "sample": f"first {nsample} regions ({nsample // args.per_query} whole queries) of the "
                          f"same batch x {reps} passes, {sc} cells in {dt:.1f} s; AVX2 restatement of score_banded_many "
                          f"(u8 32-lane batches + int16 banded kernel, band_simd8.rs:78-388, "
@@ -435,8 +574,8 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": "batched banded SW: 1e6 regions (query window, subject window, centre "
-                        "diagonal), lognormal lengths median 150 sigma 0.6 [20,2000], half-widths "
-                        "{8,16,32,64}, 40% planted homologues, BLOSUM62 11/1, no bias",
+                        "diagonal), 1000 regions per query, lognormal lengths median 150 sigma 0.6 "
+                        "[20,2000], half-widths {8,16,32,64}, 40% planted homologues, BLOSUM62 11/1, no bias",
             "regions": args.regions,
             "cells_per_step": int(cells),
             "parity_sample": int(len(idx)),
@@ -455,7 +594,7 @@ def main():
         "extra": {"autotune": tune,
                   "band_with_end_cells": {"ms_per_step": round(ems, 4), "launch_ms": round(eband_ms, 4),
                                           "gcups": round(cells / (ems * 1e-3) / 1e9, 2)},
-                  "xdrop": xd, "traceback": tb, "full_sw": fl, "prefilter": pf},
+                  "xdrop": xd, "traceback": tb, "full_sw": fl, "prefilter": pf, "search": srch},
     }
     print(json.dumps(out), flush=True)
     batch.close()

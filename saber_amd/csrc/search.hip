@@ -457,6 +457,11 @@ struct CoverArgs {
     unsigned long long* ctr;  // [0] extensions, [1] avoided, [2] steps, [3] (query, subject) groups
 };
 
+// One thread per diagonal: from the diagonal's first survivor, each
+// extension's q_hi covers the survivors before it on the diagonal, so the
+// next extended survivor is found by binary search (query positions rise
+// along a diagonal); the survivors jumped over are the avoided ones. The HSP
+// flags are cleared before the launch.
 __global__ __launch_bounds__(256) void coverage_kernel(CoverArgs a) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     uint64_t ext = 0, avoided = 0, steps = 0, groups = 0;
@@ -466,22 +471,37 @@ __global__ __launch_bounds__(256) void coverage_kernel(CoverArgs a) {
         const uint64_t prev = i ? a.keys[i - 1] : ~0ull;
         if (i == 0 || a.kl.pair_seg(prev) != a.kl.pair_seg(k)) groups = 1;
         if (i == 0 || a.kl.diag_seg(prev) != seg) {
-            uint32_t cover = 0;   // q_hi of the last extension on the diagonal
-            for (uint64_t j = i; j < a.n; ++j) {
-                const uint64_t kj = a.keys[j];
-                if (a.kl.diag_seg(kj) != seg) break;
-                const uint32_t qp = a.kl.qp(kj);
-                if (qp < cover) {
-                    ++avoided;
-                    a.hsp[j] = 0;
-                    continue;
-                }
+            // end of the diagonal: galloping search
+            uint64_t lo = i + 1, step = 1, hi = i + 1;
+            while (hi < a.n && a.kl.diag_seg(a.keys[hi]) == seg) {
+                lo = hi + 1;
+                hi = hi + step;
+                step <<= 1;
+            }
+            if (hi > a.n) hi = a.n;
+            while (lo < hi) {
+                const uint64_t mid = (lo + hi) >> 1;
+                if (a.kl.diag_seg(a.keys[mid]) == seg) lo = mid + 1;
+                else hi = mid;
+            }
+            const uint64_t end = lo;
+            uint64_t j = i;
+            while (j < end) {
                 const uint4 r = a.xres[j];
                 ++ext;
                 steps += r.w;
-                cover = r.z;
-                a.hsp[j] = (int32_t)r.x >= a.hsp_threshold ? 1 : 0;
+                if ((int32_t)r.x >= a.hsp_threshold) a.hsp[j] = 1;
+                const uint32_t cover = r.z;   // q_hi of this extension
+                // first survivor after j with query position >= cover
+                uint64_t l2 = j + 1, h2 = end;
+                while (l2 < h2) {
+                    const uint64_t mid = (l2 + h2) >> 1;
+                    if (a.kl.qp(a.keys[mid]) < cover) l2 = mid + 1;
+                    else h2 = mid;
+                }
+                j = l2;
             }
+            avoided = (end - i) - ext;
         }
     }
     ext = wave_sum64(ext);
@@ -1299,6 +1319,7 @@ int saber_hip_search_run(saber_hip_ctx* c, const saber_hip_seqset* queries, cons
         HIP_TRY(hspf.alloc(std::max<uint64_t>(nsurv, 1)));
         HIP_TRY(d_cctr.alloc(5));
         HIP_TRY(hipMemsetAsync(d_cctr.p, 0, sizeof(unsigned long long) * 5, st));
+        HIP_TRY(hipMemsetAsync(hspf.p, 0, std::max<uint64_t>(nsurv, 1), st));
         if (nsurv) {
             CoverArgs ca{};
             ca.keys = keys.p;
@@ -1375,52 +1396,56 @@ int saber_hip_search_run(saber_hip_ctx* c, const saber_hip_seqset* queries, cons
     keys.release();
     xres.release();
     sort_tmp.release();
-    // per query: region_order, the cap (scan.rs:1453-1466)
+    // per query: region_order, the cap (scan.rs:1453-1466); the regions of a
+    // query are contiguous (keys are query-major), sorted in place
     std::vector<uint64_t> rstart(nq + 1, 0);
     for (const auto& r : regs_all) rstart[r.reserved + 1]++;
     for (uint32_t q = 0; q < nq; ++q) rstart[q + 1] += rstart[q];
-    std::vector<std::vector<saber_hip_region>> regs(nq);
     parallel_for(nq, [&](uint64_t q) {
-        std::vector<saber_hip_region>& R = regs[q];
-        R.assign(regs_all.begin() + rstart[q], regs_all.begin() + rstart[q + 1]);
-        std::sort(R.begin(), R.end(), region_before);
+        std::sort(regs_all.begin() + rstart[q], regs_all.begin() + rstart[q + 1], region_before);
     });
-    regs_all.clear();
-    regs_all.shrink_to_fit();
+    std::vector<uint64_t> roff(nq + 1, 0);
     for (uint32_t q = 0; q < nq; ++q) {
-        if (regs[q].size() > P->max_candidates) {
-            ctr.regions_dropped_by_cap += regs[q].size() - P->max_candidates;
-            regs[q].resize(P->max_candidates);
-        }
-        ctr.regions += regs[q].size();
+        const uint64_t n = rstart[q + 1] - rstart[q];
+        if (n > P->max_candidates) ctr.regions_dropped_by_cap += n - P->max_candidates;
+        roff[q + 1] = roff[q] + std::min<uint64_t>(n, P->max_candidates);
     }
+    const uint64_t nreg = roff[nq];
+    ctr.regions = nreg;
+    std::vector<saber_hip_region> flat;
+    if (nreg == regs_all.size()) {
+        flat.swap(regs_all);
+    } else {
+        flat.resize(nreg);
+        for (uint32_t q = 0; q < nq; ++q)
+            std::copy(regs_all.begin() + rstart[q], regs_all.begin() + rstart[q] + (roff[q + 1] - roff[q]),
+                      flat.begin() + roff[q]);
+        regs_all.clear();
+        regs_all.shrink_to_fit();
+    }
+    auto region_count = [&](uint32_t q) { return roff[q + 1] - roff[q]; };
     tm.regions = ms_since(t0);
 
     // ---- plans (plan_windows) ----
     t0 = Clock::now();
-    std::vector<uint64_t> roff(nq + 1, 0);
-    for (uint32_t q = 0; q < nq; ++q) roff[q + 1] = roff[q] + regs[q].size();
-    const uint64_t nreg = roff[nq];
-    std::vector<saber_hip_region> flat(nreg);
     std::vector<saber_hip_plan> plans(nreg);
     std::vector<uint32_t> qidx(nreg);
     std::vector<double> thr(nreg, 0.0);
+    std::vector<uint64_t> slen_of(nreg);
     saber_hip_plan_params pp{P->window_slack, P->score_band, P->hsp_threshold, 0};
-    int plan_err = 0;
+    std::atomic<int> plan_err{0};
     parallel_for(nq, [&](uint64_t q) {
-        const uint64_t n = regs[q].size();
+        const uint64_t n = region_count((uint32_t)q), o = roff[q];
         if (!n) return;
-        std::vector<uint64_t> sl(n);
-        for (uint64_t i = 0; i < n; ++i) {
-            const saber_hip_region& r = regs[q][i];
-            sl[i] = c->db_off[r.subject + 1] - c->db_off[r.subject];
-            flat[roff[q] + i] = r;
-            qidx[roff[q] + i] = (uint32_t)q;
+        for (uint64_t i = o; i < o + n; ++i) {
+            const saber_hip_region& r = flat[i];
+            slen_of[i] = c->db_off[r.subject + 1] - c->db_off[r.subject];
+            qidx[i] = (uint32_t)q;
             // the two-stage cut (score_windows, engine.rs:1057-1068)
-            thr[roff[q] + i] = P->score_prefilter_frac *
-                               saber_hip_stats_floor_lower_bound(&prep[q].qs, region_e, NAN, sl[i], prep[q].region_floor);
+            thr[i] = P->score_prefilter_frac *
+                     saber_hip_stats_floor_lower_bound(&prep[q].qs, region_e, NAN, slen_of[i], prep[q].region_floor);
         }
-        if (saber_hip_plan_windows(regs[q].data(), n, qlen[q], sl.data(), &pp, plans.data() + roff[q]) != 0)
+        if (saber_hip_plan_windows(flat.data() + o, n, qlen[q], slen_of.data() + o, &pp, plans.data() + o) != 0)
             plan_err = 1;
     });
     if (plan_err) return fail(SABER_HIP_EINVAL, "plan_windows failed");
@@ -1475,7 +1500,7 @@ int saber_hip_search_run(saber_hip_ctx* c, const saber_hip_seqset* queries, cons
                 continue;
             }
             if (s <= 0) continue;
-            const uint64_t n = c->db_off[flat[i].subject + 1] - c->db_off[flat[i].subject];
+            const uint64_t n = slen_of[i];
             if (saber_hip_stats_evalue_in(&pq.qs, s, saber_hip_stats_space_lower_bound(&pq.qs, n)) > ecut) continue;
             const double e = saber_hip_stats_evalue_for(&pq.qs, s, n);
             if (e <= ecut) order[q].push_back(HitCand{e, i});
@@ -1487,7 +1512,7 @@ int saber_hip_search_run(saber_hip_ctx* c, const saber_hip_seqset* queries, cons
         });
         for (const HitCand& h : order[q]) reg_e[h.ri] = h.e;
     });
-    for (uint32_t q = 0; q < nq; ++q) ctr.regions_below_evalue += regs[q].size() - order[q].size();
+    for (uint32_t q = 0; q < nq; ++q) ctr.regions_below_evalue += region_count(q) - order[q].size();
 
     // The reporting loop per query, replayed over tracebacks computed in
     // rounds: each round traces, for every query still open, its next
