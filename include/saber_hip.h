@@ -307,6 +307,22 @@ int saber_hip_xdrop_batch_read(saber_hip_xdrop_batch* b, int32_t* score, uint32_
 int saber_hip_xdrop_batch_time(saber_hip_xdrop_batch* b, int warmup, int iters, double* ms_run,
                                uint64_t* steps);
 void saber_hip_xdrop_batch_destroy(saber_hip_xdrop_batch* b);
+/* No counterpart in the reference: saber_hip_xdrop for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid device allocation). Every input and the output are device
+ * pointers: residues as alignment codes (query and subject buffers, and the
+ * bias when given, readable 16 bytes before the first residue and 64 after
+ * the last, as the library's own uploads are), u64 offsets, hits as 4 u32
+ * (query, subject, qp, sp; not range-checked: they are device data), out as
+ * 4 u32 per hit (score, q_lo, q_hi, steps). Its scratch (claim counters, the
+ * score table, 32 bytes per hit) is a fixed array in device memory, one per
+ * GPU, held until the process ends: at most SABER_HIP_XDROP_NOALLOC_MAX_HITS
+ * hits per call (EINVAL above). Calls on one GPU share that array: calls
+ * that overlap on one GPU race. Runs on the context stream; blocking. */
+#define SABER_HIP_XDROP_NOALLOC_MAX_HITS (1u << 20)
+int saber_hip_xdrop_noalloc(saber_hip_ctx* ctx, const uint8_t* qres, const uint64_t* qoff, const int8_t* qbias,
+                            const uint8_t* sres, const uint64_t* soff, const uint32_t* hits, uint64_t nhits,
+                            const saber_hip_scheme* scheme, int32_t x, uint32_t* out);
 
 /* Kernel variant table of the packed band kernel: variant v uses L lanes per
  * job pair with C band cells per lane and covers clipped widths

@@ -160,6 +160,8 @@ class SearchRegion(C.Structure):
 
 P = C.c_void_p
 _SIGS = {
+    "saber_hip_xdrop_noalloc": (C.c_int, [P, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                          C.c_uint64, C.POINTER(Scheme), C.c_int32, C.c_void_p]),
     "saber_hip_search_defaults": (None, [C.POINTER(SearchParams)]),
     "saber_hip_seed_scheme_default": (None, [C.POINTER(SeedScheme)]),
     "saber_hip_search_run": (C.c_int, [P, C.POINTER(SeqSet), C.POINTER(SeedScheme), C.POINTER(Scheme),

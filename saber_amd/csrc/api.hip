@@ -262,6 +262,10 @@ int saber_hip_create(int device, const saber_hip_opts* opts, saber_hip_ctx** out
         saber_hip_destroy(c);
         return fail(SABER_HIP_EDEVICE, std::string("stream/event creation: ") + hipGetErrorString(e));
     }
+    if (int r = xdrop_noalloc_setup(device)) {
+        saber_hip_destroy(c);
+        return r;
+    }
     default_choice(c);
     *out = c;
     return SABER_HIP_OK;

@@ -108,6 +108,10 @@ int score_regions_impl(saber_hip_ctx* c, const saber_hip_seqset* queries, const 
                        const saber_hip_scheme* scheme, int64_t narrow_band, int64_t full_band, const double* thr,
                        int32_t* out, uint8_t* rescored, uint64_t* banded_cells, const uint8_t* skip);
 
+// Takes the address of saber_hip_xdrop_noalloc's scratch array on the
+// current device (xdrop_noalloc.hip); called when a context is created.
+int xdrop_noalloc_setup(int device);
+
 // Full-SW launches of one run are spread over this many streams (all at the
 // default priority) so small subject buckets and the tails of large ones
 // overlap.
