@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdlib>
 #include <atomic>
@@ -176,6 +177,34 @@ uint64_t band_cells(const BandJobDev& d) {
     // sum of max(0, i + lo): 0 below i = -lo
     const int64_t left = range_sum(std::max(i0, -lo) + lo, i1 + lo);
     return (uint64_t)(right - left + (i1 - i0 + 1));
+}
+
+// Stable LSD radix sort of v by the low `bits` bits of key[v[i]] (11-bit
+// digits): the planner's orderings over millions of jobs, without comparator
+// sorts that chase job records across memory.
+void radix_sort_by(std::vector<uint32_t>& v, const std::vector<uint64_t>& key, int bits) {
+    if (v.size() < 2 || bits <= 0) return;
+    std::vector<uint32_t> tmp(v.size());
+    std::vector<uint64_t> k(v.size()), ktmp(v.size());
+    for (size_t i = 0; i < v.size(); ++i) k[i] = key[v[i]];
+    for (int shift = 0; shift < bits; shift += 11) {
+        uint32_t cnt[2049] = {0};
+        for (size_t i = 0; i < v.size(); ++i) cnt[((k[i] >> shift) & 2047) + 1]++;
+        for (int d = 0; d < 2048; ++d) cnt[d + 1] += cnt[d];
+        for (size_t i = 0; i < v.size(); ++i) {
+            const uint32_t dst = cnt[(k[i] >> shift) & 2047]++;
+            tmp[dst] = v[i];
+            ktmp[dst] = k[i];
+        }
+        v.swap(tmp);
+        k.swap(ktmp);
+    }
+}
+
+int bits_of(uint64_t x) {
+    int b = 0;
+    while (b < 64 && (x >> b)) ++b;
+    return b;
 }
 
 void default_choice(saber_hip_ctx* c) {
@@ -524,6 +553,14 @@ int saber_hip_band_batch_create(saber_hip_ctx* c, const saber_hip_seqset* querie
     for (int i = 0; i < 576; ++i)
         if (scheme->matrix[i] < -128 || scheme->matrix[i] > 127) packed_ok = false;
 
+    static const bool trace = std::getenv("SABER_HIP_TRACE_HOST") != nullptr;
+    auto tp = std::chrono::steady_clock::now();
+    auto lap = [&](const char* what) {
+        if (!trace) return;
+        const auto t1 = std::chrono::steady_clock::now();
+        std::fprintf(stderr, "#   plan %s %.1f ms\n", what, std::chrono::duration<double, std::milli>(t1 - tp).count());
+        tp = t1;
+    };
     // Plan.
     b->host_jobs.assign(njobs, BandJobDev{});
     std::vector<uint64_t> host_cells(njobs, 0);
@@ -573,6 +610,7 @@ int saber_hip_band_batch_create(saber_hip_ctx* c, const saber_hip_seqset* querie
         thr_shape[k] = shape;
         est_work += 0.5 * (d.rows + c->shapes[shape].L + 3) * c->shapes[shape].pair_cost();
     }
+    lap("jobs");
     // Many-row jobs: a fixed row count, or the share of the estimated time.
     const double est_ms = est_work / kGpuLaneInstrPerMs;
     for (uint64_t k = 0; k < njobs; ++k) {
@@ -618,23 +656,51 @@ int saber_hip_band_batch_create(saber_hip_ctx* c, const saber_hip_seqset* querie
             chunk_est.push_back((double)steps * s.row_cost());
         }
     };
+    lap("buckets");
+    // sort keys: (query start, first row, rows descending) when they fit 64
+    // bits (they do unless the query blob is beyond 2^33 residues), and the
+    // rows descending alone
+    uint64_t max_qoff = 0, max_ifirst = 0;
+    for (uint64_t k = 0; k < njobs; ++k)
+        if (thr_shape[k] >= 0) {
+            max_qoff = std::max<uint64_t>(max_qoff, b->host_jobs[k].q_off);
+            max_ifirst = std::max<uint64_t>(max_ifirst, (uint64_t)b->host_jobs[k].i_first);
+        }
+    const int bq = bits_of(max_qoff), bi = bits_of(max_ifirst), br = 16;
+    const bool radix_ok = bq + bi + br <= 64;
+    std::vector<uint64_t> key_group, key_rows(njobs, 0);
+    if (radix_ok) key_group.assign(njobs, 0);
+    for (uint64_t k = 0; k < njobs; ++k) {
+        if (thr_shape[k] < 0) continue;
+        const uint64_t rdesc = 65535u - (uint64_t)rows_of[k];
+        key_rows[k] = rdesc;
+        if (radix_ok)
+            key_group[k] = (b->host_jobs[k].q_off << (bi + br)) | ((uint64_t)b->host_jobs[k].i_first << br) | rdesc;
+    }
     for (size_t bk = 0; bk < bucket.size(); ++bk) {
         auto& v = bucket[bk];
         if (v.empty()) continue;
         const int shape = (int)(bk / 2);
         // group by (query start, first row), longest first within a group
-        std::stable_sort(v.begin(), v.end(), [&](uint32_t x, uint32_t y) {
-            const BandJobDev &ja = b->host_jobs[x], &jb = b->host_jobs[y];
-            if (ja.q_off != jb.q_off) return ja.q_off < jb.q_off;
-            if (ja.i_first != jb.i_first) return ja.i_first < jb.i_first;
-            return rows_of[x] > rows_of[y];
-        });
+        if (radix_ok) {
+            radix_sort_by(v, key_group, bq + bi + br);
+        } else {
+            std::stable_sort(v.begin(), v.end(), [&](uint32_t x, uint32_t y) {
+                const BandJobDev &ja = b->host_jobs[x], &jb = b->host_jobs[y];
+                if (ja.q_off != jb.q_off) return ja.q_off < jb.q_off;
+                if (ja.i_first != jb.i_first) return ja.i_first < jb.i_first;
+                return rows_of[x] > rows_of[y];
+            });
+        }
         std::vector<uint32_t> shared, rest;
+        shared.reserve(v.size());
+        auto same_group = [&](uint32_t x, uint32_t y) {
+            if (radix_ok) return (key_group[x] >> br) == (key_group[y] >> br);
+            return b->host_jobs[x].q_off == b->host_jobs[y].q_off && b->host_jobs[x].i_first == b->host_jobs[y].i_first;
+        };
         for (size_t i = 0; i < v.size();) {
             size_t j = i;
-            while (j < v.size() && b->host_jobs[v[j]].q_off == b->host_jobs[v[i]].q_off &&
-                   b->host_jobs[v[j]].i_first == b->host_jobs[v[i]].i_first)
-                ++j;
+            while (j < v.size() && same_group(v[j], v[i])) ++j;
             const size_t even = i + ((j - i) & ~(size_t)1);
             shared.insert(shared.end(), v.begin() + i, v.begin() + even);
             if (even < j) rest.push_back(v[even]);
@@ -642,11 +708,13 @@ int saber_hip_band_batch_create(saber_hip_ctx* c, const saber_hip_seqset* querie
         }
         // shared pairs in order of their longer job's rows (pairs stay intact)
         {
-            std::vector<size_t> order(shared.size() / 2);
-            for (size_t k = 0; k < order.size(); ++k) order[k] = k;
-            std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) {
-                return rows_of[shared[2 * x]] > rows_of[shared[2 * y]];
-            });
+            std::vector<uint32_t> order(shared.size() / 2);
+            std::vector<uint64_t> pk(order.size());
+            for (size_t k = 0; k < order.size(); ++k) {
+                order[k] = (uint32_t)k;
+                pk[k] = key_rows[shared[2 * k]];
+            }
+            radix_sort_by(order, pk, br);
             std::vector<uint32_t> sorted(shared.size());
             for (size_t k = 0; k < order.size(); ++k) {
                 sorted[2 * k] = shared[2 * order[k]];
@@ -654,15 +722,22 @@ int saber_hip_band_batch_create(saber_hip_ctx* c, const saber_hip_seqset* querie
             }
             shared.swap(sorted);
         }
-        std::stable_sort(rest.begin(), rest.end(), [&](uint32_t x, uint32_t y) { return rows_of[x] > rows_of[y]; });
+        radix_sort_by(rest, key_rows, br);
         add_chunks(shared, shape, kChunkShared);
         add_chunks(rest, shape, 0);
     }
+    lap("pairs");
     if (pairs.size() / 2 >= 0xFFFFFFFFull) return fail(SABER_HIP_EINVAL, "too many job pairs for one batch");
     {
+        // longest estimated chunk first; the estimates are integers (row
+        // steps x the shape's integer per-row cost), ordered descending
         std::vector<uint32_t> order(b->host_blocks.size());
         for (uint32_t i = 0; i < order.size(); ++i) order[i] = i;
-        std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return chunk_est[x] > chunk_est[y]; });
+        double emax = 0.0;
+        for (double e : chunk_est) emax = std::max(emax, e);
+        std::vector<uint64_t> ek(order.size());
+        for (size_t i = 0; i < order.size(); ++i) ek[i] = (uint64_t)(emax - chunk_est[i]);
+        radix_sort_by(order, ek, bits_of((uint64_t)emax));
         std::vector<BandBlock> sorted(order.size());
         for (size_t i = 0; i < order.size(); ++i) sorted[i] = b->host_blocks[order[i]];
         b->host_blocks.swap(sorted);
@@ -682,6 +757,7 @@ int saber_hip_band_batch_create(saber_hip_ctx* c, const saber_hip_seqset* querie
     }
 
     const std::vector<uint32_t> ptable = build_pair_table(scheme->matrix);
+    lap("chunk order + tables");
 
     hipStream_t st = c->stream;
     const uint64_t nq = queries->count ? queries->offsets[queries->count] : 0;
@@ -741,6 +817,7 @@ int saber_hip_band_batch_create(saber_hip_ctx* c, const saber_hip_seqset* querie
     HIP_TRY(b->s_end.alloc(njobs ? njobs : 1));
     if (int r = upload(b->cells, host_cells.data(), njobs, st)) return r;
     HIP_TRY(hipStreamSynchronize(st));
+    lap("uploads");
     *out = b.release();
     return SABER_HIP_OK;
 }
@@ -954,10 +1031,24 @@ int saber_hip_score_bands(saber_hip_ctx* c, const saber_hip_seqset* queries,
                           uint32_t* q_end, uint32_t* s_end, uint64_t* cells) {
     if (njobs > 0 && !score) return fail(SABER_HIP_EINVAL, "score is null");
     saber_hip_band_batch* b = nullptr;
+    static const bool trace = std::getenv("SABER_HIP_TRACE_HOST") != nullptr;
+    auto t0 = std::chrono::steady_clock::now();
+    auto lap = [&](const char* what) {
+        if (!trace) return;
+        const auto t1 = std::chrono::steady_clock::now();
+        std::fprintf(stderr, "# score_bands %llu jobs: %s %.1f ms\n", (unsigned long long)njobs, what,
+                     std::chrono::duration<double, std::milli>(t1 - t0).count());
+        t0 = t1;
+    };
     if (int r = saber_hip_band_batch_create(c, queries, subjects, jobs, njobs, scheme, q_end || s_end, &b)) return r;
+    lap("plan + upload");
     int r = saber_hip_band_batch_run(b);
+    if (r == SABER_HIP_OK) r = saber_hip_band_batch_wait(b);
+    lap("run");
     if (r == SABER_HIP_OK) r = saber_hip_band_batch_read(b, score, q_end, s_end, cells);
+    lap("read");
     saber_hip_band_batch_destroy(b);
+    lap("destroy");
     return r;
 }
 

@@ -33,6 +33,8 @@
 // the device (score_regions_impl, saber_hip_traceback_ex).
 #include <hip/hip_runtime.h>
 
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 #include <rocprim/device/device_radix_sort.hpp>
@@ -319,11 +321,23 @@ struct ScanArgs {
 
 constexpr int kScanWaves = 4;
 
+constexpr uint32_t kStage = 256;   // survivors staged per wave before one append
+
+__device__ __forceinline__ void flush_staged(const ScanArgs& a, uint64_t* st, uint32_t n, uint32_t lane) {
+    unsigned long long base = 0;
+    if (lane == 0) base = atomicAdd(a.nout, (unsigned long long)n);
+    base = __shfl(base, 0, 64);
+    for (uint32_t k = lane; k < n; k += 64)
+        if (base + k < a.cap) a.out[base + k] = st[k];
+}
+
 __global__ __launch_bounds__(256) void scan_kernel(ScanArgs a) {
     __shared__ uint32_t s_excl[kScanWaves][64];
     __shared__ uint32_t s_ea[kScanWaves][64];
     __shared__ uint32_t s_sv[kScanWaves][64][8];
     __shared__ uint32_t s_sx[kScanWaves][64][8];
+    __shared__ uint64_t s_out[kScanWaves][kStage];
+    uint32_t staged = 0;   // wave-uniform
     const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
     uint64_t n_cand = 0, n_hits = 0, n_pass = 0;
@@ -397,17 +411,25 @@ __global__ __launch_bounds__(256) void scan_kernel(ScanArgs a) {
             const uint32_t cnt = (uint32_t)__popcll(bal);
             n_pass += cnt;
             if (cnt) {
-                unsigned long long base = 0;
-                if (lane == 0) base = atomicAdd(a.nout, (unsigned long long)cnt);
-                base = __shfl(base, 0, 64);
-                if (pass) {
-                    const uint64_t idx = base + __popcll(bal & ((1ull << lane) - 1ull));
-                    if (idx < a.cap) a.out[idx] = key;
+                if (staged + cnt > kStage) {
+                    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    flush_staged(a, s_out[wv], staged, lane);
+                    staged = 0;
+                    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
                 }
+                if (pass) s_out[wv][staged + __popcll(bal & ((1ull << lane) - 1ull))] = key;
+                staged += cnt;
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
+    }
+    if (staged) {
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        flush_staged(a, s_out[wv], staged, lane);
     }
     if (lane == 0) {
         if (n_cand) atomicAdd(&a.ctr[0], (unsigned long long)n_cand);
@@ -463,13 +485,13 @@ struct CoverArgs {
 // along a diagonal); the survivors jumped over are the avoided ones. The HSP
 // flags are cleared before the launch.
 __global__ __launch_bounds__(256) void coverage_kernel(CoverArgs a) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     uint64_t ext = 0, avoided = 0, steps = 0, groups = 0;
-    if (i < a.n) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += stride) {
         const uint64_t k = a.keys[i];
         const uint64_t seg = a.kl.diag_seg(k);
         const uint64_t prev = i ? a.keys[i - 1] : ~0ull;
-        if (i == 0 || a.kl.pair_seg(prev) != a.kl.pair_seg(k)) groups = 1;
+        if (i == 0 || a.kl.pair_seg(prev) != a.kl.pair_seg(k)) ++groups;
         if (i == 0 || a.kl.diag_seg(prev) != seg) {
             // end of the diagonal: galloping search
             uint64_t lo = i + 1, step = 1, hi = i + 1;
@@ -485,10 +507,11 @@ __global__ __launch_bounds__(256) void coverage_kernel(CoverArgs a) {
                 else hi = mid;
             }
             const uint64_t end = lo;
-            uint64_t j = i;
+            uint64_t j = i, seg_ext = 0;
             while (j < end) {
                 const uint4 r = a.xres[j];
                 ++ext;
+                ++seg_ext;
                 steps += r.w;
                 if ((int32_t)r.x >= a.hsp_threshold) a.hsp[j] = 1;
                 const uint32_t cover = r.z;   // q_hi of this extension
@@ -501,7 +524,7 @@ __global__ __launch_bounds__(256) void coverage_kernel(CoverArgs a) {
                 }
                 j = l2;
             }
-            avoided = (end - i) - ext;
+            avoided += (end - i) - seg_ext;
         }
     }
     ext = wave_sum64(ext);
@@ -1312,6 +1335,14 @@ int saber_hip_search_run(saber_hip_ctx* c, const saber_hip_seqset* queries, cons
 
     // ---- coverage, HSPs, regions ----
     t0 = Clock::now();
+    static const bool trace = std::getenv("SABER_HIP_TRACE_HOST") != nullptr;
+    auto tl = Clock::now();
+    auto lap = [&](const char* what) {
+        if (!trace) return;
+        (void)hipStreamSynchronize(st);
+        std::fprintf(stderr, "#   search %s %.1f ms\n", what, ms_since(tl));
+        tl = Clock::now();
+    };
     std::vector<saber_hip_region> regs_all;
     {
         DevBuf<uint8_t> hspf;
@@ -1329,9 +1360,11 @@ int saber_hip_search_run(saber_hip_ctx* c, const saber_hip_seqset* queries, cons
             ca.hsp_threshold = P->hsp_threshold;
             ca.hsp = hspf.p;
             ca.ctr = d_cctr.p;
-            hipLaunchKernelGGL(coverage_kernel, dim3(grid_for(nsurv, 256)), dim3(256), 0, st, ca);
+            const uint64_t cgrid = std::min<uint64_t>((uint64_t)c->cus * 16, (nsurv + 255) / 256);
+            hipLaunchKernelGGL(coverage_kernel, dim3((uint32_t)std::max<uint64_t>(cgrid, 1)), dim3(256), 0, st, ca);
             HIP_TRY(hipGetLastError());
         }
+        lap("coverage");
         // HSP indices in key order
         DevBuf<uint32_t> hidx;
         DevBuf<uint64_t> d_nh;
@@ -1348,6 +1381,7 @@ int saber_hip_search_run(saber_hip_ctx* c, const saber_hip_seqset* queries, cons
             HIP_TRY(hipStreamSynchronize(st));
         }
         ctr.hsps = nh;
+        lap("hsp select");
         DevBuf<saber_hip_region> rslot, rsel;
         DevBuf<uint8_t> rkeep;
         HIP_TRY(rslot.alloc(std::max<uint64_t>(nh, 1)));
@@ -1384,6 +1418,7 @@ int saber_hip_search_run(saber_hip_ctx* c, const saber_hip_seqset* queries, cons
             if (nr)
                 HIP_TRY(hipMemcpyAsync(regs_all.data(), rsel.p, sizeof(saber_hip_region) * nr, hipMemcpyDeviceToHost, st));
         }
+        lap("regions + download");
         unsigned long long cc[5];
         HIP_TRY(hipMemcpyAsync(cc, d_cctr.p, sizeof cc, hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
@@ -1404,6 +1439,7 @@ int saber_hip_search_run(saber_hip_ctx* c, const saber_hip_seqset* queries, cons
     parallel_for(nq, [&](uint64_t q) {
         std::sort(regs_all.begin() + rstart[q], regs_all.begin() + rstart[q + 1], region_before);
     });
+    lap("region sort");
     std::vector<uint64_t> roff(nq + 1, 0);
     for (uint32_t q = 0; q < nq; ++q) {
         const uint64_t n = rstart[q + 1] - rstart[q];

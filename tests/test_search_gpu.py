@@ -117,3 +117,37 @@ def test_empty_and_short_queries(ctx, sets):
     got, want, wctr = run_both(ctx, d)
     check(got, want, wctr)
     assert got.hits[0] == [] and got.hits[1] == []
+
+
+def test_many_survivors_regions_and_counters(ctx):
+    """More survivors than one pass of the device grids (no window filter,
+    30 000 subjects): the capped regions of every query and the scan-stage
+    counters against the oracle's scan (or_scan) directly."""
+    d = workload.search_set(seed=9, nq=60, nsub=30000)
+    seeds = ScanSeeds.default()
+    k = stats.gapped_protein(62, 11, 1)
+    ctx.upload_db(d["sres"], d["soff"])
+    params = search.default_params(hit_filter_min_ids=0, hit_filter_exact_weight=0)
+    got = search.run(ctx, d["qres"], d["qoff"], S.Scheme(BLOSUM62, 11, 1), k, seeds=seeds, params=params)
+    nq, ns = len(d["qoff"]) - 1, len(d["soff"]) - 1
+    st = stats.SearchStatistics(k, int(d["soff"][-1]), ns, stats.PER_SUBJECT)
+    re = 10.0
+    thr, bias = [], []
+    for q in range(nq):
+        qq = d["qres"][int(d["qoff"][q]):int(d["qoff"][q + 1])]
+        bias.append(search.composition_bias(qq, BLOSUM62))
+        rf = st.for_query(len(qq)).min_score(re)
+        thr.append(max(40, int(math.ceil(rf * 0.65))) if rf > 0 else 40)
+    regs, ctr, _ = OS.scan(d["qres"], d["qoff"], np.concatenate(bias), d["sres"], d["soff"], BLOSUM62,
+                           seeds.alpha_map, seeds.alpha_size, seeds.shapes, thr, min_ids=0, kx=0)
+    assert ctr["hits"] - ctr["hits_filtered"] > (1 << 20)
+    for key in ("seed_positions", "scan_candidates", "hits", "hits_filtered", "scan_groups", "ungapped_extensions",
+                "extensions_avoided", "xdrop_steps", "hsps", "regions_weak_dropped", "regions"):
+        assert got.counters[key] == ctr[key], (key, got.counters[key], ctr[key])
+    for q in range(nq):
+        g = got.regions[q]
+        w = regs[q]
+        assert [(int(r["subject"]), int(r["d_min"]), int(r["d_max"]), int(r["best"]), int(r["n_hsp"]),
+                 int(r["hsp_sum"])) for r in g] == \
+               [(int(r["subject"]), int(r["d_min"]), int(r["d_max"]), int(r["best"]), int(r["n_hsp"]),
+                 int(r["hsp_sum"])) for r in w], f"query {q}"
