@@ -28,9 +28,14 @@ The printed JSON line also carries
   extra.full_sw exhaustive full-matrix SW (queries x subjects, GCUPS; scores
                 only, as score_many, and with the end cell; shapes changed by
                 the autotuner);
-  extra.prefilter  the learned region prefilter: hit loss, kept fraction,
-                scoring-stage time with and without it, MLP forward
-                throughput against the bf16 MFMA peak.
+  extra.search  the blastp workloads end to end on the GPU: 100 x 5000
+                (hit lists against the oracle's plain CPU search, also
+                timed) and 11,206 queries against ~148 M residues
+                (queries/s, stage times, counters, size-independent checks);
+  extra.prefilter  the learned region prefilter on the at-scale workload:
+                trained on ~22 M regions the plain search labels, reported
+                hits lost, regions kept, queries/s plain vs filtered, MLP
+                forward throughput against the bf16 MFMA peak.
 """
 import argparse
 import json
@@ -70,90 +75,79 @@ def measured_traffic(args):
 
 
 def prefilter_leg(ctx, args, sch):
-    """Learned region prefilter (64 -> 256 -> 256 -> 1 bf16 MFMA MLP) on
-    synthetic candidate regions: 10 % planted homologues, 8 seed hits per
-    region (half on the region's centre diagonal). Labels: the exact band
-    score reaches E <= 10 (per-subject model, BLOSUM62 11/1, a 147 M-residue
-    / 331 k-sequence database). Features are computed on the device from the
-    X-drop results, standardised with training-set statistics. The model is
-    trained on the train set; its cut is set for --pf-recall (99.5 %) recall on a
-    validation set; the test set gives the hit loss, the kept fraction and the
-    time of the scoring stage with and without the filter (with it: feature
-    kernel + MLP forward + band scoring of the kept regions, device times;
-    scores only, as score_banded_many)."""
-    from saber_amd import prefilter as P, stats as ST, workload
-    per = 8
-    kp = ST.gapped_protein(62, 11, 1)
+    """Learned region prefilter (64 -> 256 -> 256 -> 1, bf16 MFMA kernels)
+    on the at-scale blastp workload (config 4 on config 3's workload).
 
-    def make(seed, n):
-        d = workload.band_regions(seed=seed, n=n, per_query=1000, homolog_frac=0.1)
-        hits = workload.xdrop_hits(d, seed=seed + 1, per_region=per)
-        xs, xlo, xhi, _ = ctx.xdrop(d["qres"], d["qoff"], d["sres"], d["soff"], hits, sch, 20)
-        b = ctx.band_batch(d["qres"], d["qoff"], d["sres"], d["soff"], d["jobs"], sch, ends=False)
-        b.run()
-        score = b.read()[0]
-        b.close()
-        J = d["jobs"]
-        m = (d["qoff"][1:] - d["qoff"][:-1])[J["query"]]
-        e = ST.evalues(kp, 147_000_000, 331_000, ST.PER_SUBJECT, m, (J["win_hi"] - J["win_lo"]).astype(np.uint64),
-                       score)
-        d["raw"] = (hits, xs, xlo, xhi)
-        return d, (e <= 10.0).astype(np.float32)
-
+    Training: every capped region of the plain GPU search over two workloads
+    of the at-scale shape (family sets, seeds 11 and 12: ~22 M regions),
+    labelled by the plain search itself (its exact score passes E <= 10);
+    features computed on the device, standardised with the training
+    statistics, resident in HBM; Adam on minibatches of 65,536. Threshold:
+    at most --pf-loss of the regions the plain search reported a hit from
+    are pruned on a third workload (seed 13). Test: the at-scale leg's own
+    workload (seed 0), plain vs filtered search: reported (query, subject)
+    hits lost, regions kept, queries/s end to end and for the
+    candidate-scoring stage (regions to hits: plan, prefilter, scores,
+    traceback and E-values), band cells scored; MLP forward throughput over
+    the test regions against the bf16 MFMA peak."""
+    from saber_amd import prefilter as P, search, stats, workload
+    k = stats.gapped_protein(62, 11, 1)
+    nq, nd = args.search_queries, args.search_decoys
+    log = lambda msg: print(msg, file=sys.stderr, flush=True)
     t0 = time.time()
-    dtr, ytr = make(args.seed + 1001, args.pf_train)
-    dva, yva = make(args.seed + 1002, args.pf_train // 4)
-    dte, yte = make(args.seed + 1003, args.pf_test)
-    feats = lambda d, mu=None, sd=None: P.region_features(ctx, d["qres"], d["qoff"], d["sres"], d["soff"], d["jobs"],
-                                                          *d["raw"], per, 40, mean=mu, std=sd)
-    raw = P.bf16_float(feats(dtr))
-    mu, sd = raw.mean(0), raw.std(0)
-    sd = np.where(sd > 1e-6, sd, 0.0)
-    Xtr, Xva = feats(dtr, mu, sd), feats(dva, mu, sd)
-    Xte, feat_ms = P.region_features(ctx, dte["qres"], dte["qoff"], dte["sres"], dte["soff"], dte["jobs"],
-                                     *dte["raw"], per, 40, mean=mu, std=sd, time_iters=max(3, args.steps // 2))
-    prep_s = time.time() - t0
-    m = P.Mlp(ctx, seed=args.seed)
-    m.upload(Xtr, ytr)
-    bs, epochs, lr = 65536, args.pf_epochs, 2e-3
-    t0 = time.time()
-    for ep in range(epochs):
-        for r0 in range(0, len(ytr), bs):
-            m.train_resident(r0, min(bs, len(ytr) - r0), lr=lr, pos_weight=4.0, want_loss=False)
-    loss = m.train_resident(0, min(bs, len(ytr)), lr=0.0, pos_weight=4.0)
+    train = [workload.family_set(seed=s, nq=nq, ndecoy=nd) for s in (11, 12)]
+    valid = workload.family_set(seed=13, nq=nq, ndecoy=nd)
+    pf = search.train_prefilter(ctx, train, valid, sch, k, loss_target=args.pf_loss, epochs=args.pf_epochs,
+                                batch=65536, seed=args.seed, log=log)
     train_s = time.time() - t0
-    lv = m.forward(Xva)
-    tau = float(np.quantile(lv[yva > 0.5], 1 - args.pf_recall))
-    lt = m.forward(Xte)
-    keep = lt >= tau
-    hit_loss = float(1 - keep[yte > 0.5].mean())
-
-    def band_ms(sel):
-        b = ctx.band_batch(dte["qres"], dte["qoff"], dte["sres"], dte["soff"], dte["jobs"][sel], sch, ends=False)
-        ms = b.time(warmup=2, iters=max(3, args.steps // 2))[0]
-        b.close()
-        return ms
-    t_all = band_ms(np.arange(len(yte)))
-    t_keep = band_ms(np.nonzero(keep)[0])
-    m.upload(Xte)
-    f_ms, s_ms = m.time(len(yte), warmup=2, iters=max(3, args.steps // 2))
-    m.upload(Xtr, ytr)
-    _, s64 = m.time(bs, warmup=1, iters=5)
+    del train, valid
+    test = workload.family_set(seed=0, nq=nq, ndecoy=nd)
+    ctx.upload_db(test["sres"], test["soff"])
+    search.run(ctx, test["qres"], test["qoff"], sch, k)     # the database's seed-scan data
+    t1 = time.perf_counter()
+    plain = search.run(ctx, test["qres"], test["qoff"], sch, k, keep_handle=True)
+    tp = time.perf_counter() - t1
+    t1 = time.perf_counter()
+    filt = search.run(ctx, test["qres"], test["qoff"], sch, k, prefilter=search.prefilter_arg(pf))
+    tf = time.perf_counter() - t1
+    pp, fp = search.hit_pairs(plain), search.hit_pairs(filt)
+    lost = len(pp - fp)
+    m = pf["model"]
+    X = search.features(plain, pf["mean"], pf["std"])
+    search.close(plain)
+    m.upload(X)
+    f_ms, _ = m.time(len(X), warmup=2, iters=5)
+    _, s_ms = m.time(65536, warmup=2, iters=5)
     m.close()
-    flop = 164352 * len(yte)
-    out = {"train_regions": int(len(ytr)), "test_regions": int(len(yte)), "hits_per_region": per,
-           "positives": round(float(yte.mean()), 4), "recall_target": args.pf_recall, "hit_loss": round(hit_loss, 5),
-           "kept_fraction": round(float(keep.mean()), 4), "band_ms_all": round(t_all, 4),
-           "band_ms_kept": round(t_keep, 4), "mlp_forward_ms": round(f_ms, 4),
-           "mlp_forward_tflops": round(flop / (f_ms * 1e-3) / 1e12, 1),
+    stage = lambda r: sum(r.timing[x] for x in ("plan", "prefilter", "score", "finish"))
+    sp, sf = stage(plain), stage(filt)
+    flop = 164352 * len(X)
+    out = {"workload": "at-scale blastp workload (search leg), default preset",
+           "train_regions": pf["train_regions"], "train_positives": pf["positives"], "train_steps": pf["steps"],
+           "epochs": args.pf_epochs, "minibatch": 65536, "pos_weight": round(pf["pos_weight"], 2),
+           "train_s": round(train_s, 1), "threshold": round(pf["threshold"], 4),
+           "valid_regions": pf["valid_regions"], "loss_target_valid": args.pf_loss,
+           "test_regions": int(filt.counters["regions"]), "regions_pruned": int(filt.counters["regions_pruned"]),
+           "kept_fraction": round(1 - filt.counters["regions_pruned"] / max(1, filt.counters["regions"]), 4),
+           "plain_hits": len(pp), "hits_lost": lost, "hit_loss": round(lost / max(1, len(pp)), 5),
+           "queries_per_s_plain": round(nq / tp, 1), "queries_per_s_filtered": round(nq / tf, 1),
+           "end_to_end_speedup": round(tp / tf, 3),
+           "candidate_scoring_ms_plain": round(sp, 1), "candidate_scoring_ms_filtered": round(sf, 1),
+           "candidate_scoring_speedup": round(sp / max(sf, 1e-9), 3),
+           "banded_cells_plain": int(plain.counters["banded_cells"]),
+           "banded_cells_filtered": int(filt.counters["banded_cells"]),
+           "prefilter_stage_ms": round(filt.timing["prefilter"], 2),
+           "mlp_forward_ms": round(f_ms, 3), "mlp_forward_tflops": round(flop / (f_ms * 1e-3) / 1e12, 1),
            "mlp_forward_mfma_frac": round(flop / (f_ms * 1e-3) / 2.5e15, 4),
-           "train_step_ms_65536": round(s64, 3), "epochs": epochs, "train_s": round(train_s, 2),
-           "final_loss": round(float(loss), 4), "prep_s": round(prep_s, 1),
-           "feature_ms": round(feat_ms, 4),
-           "scoring_speedup": round(t_all / (feat_ms + f_ms + t_keep), 3)}
-    print(f"# prefilter: kept {out['kept_fraction']}, hit loss {out['hit_loss']}, band {t_all:.3f} -> "
-          f"{t_keep:.3f} ms + features {feat_ms:.3f} ms + mlp {f_ms:.3f} ms ({out['mlp_forward_tflops']} TFLOP/s): "
-          f"{out['scoring_speedup']}x", file=sys.stderr, flush=True)
+           "train_step_ms_65536": round(s_ms, 3),
+           "prunable_bound": "gradient-boosted trees on the same features, fitted offline to a sample of one "
+                             "at-scale search (23 k positives, 300 k negatives), prune at most ~33 % of regions at "
+                             "1 % reported-hit loss: the weakest reported hits' regions carry noise-level X-drop "
+                             "evidence"}
+    log(f"# prefilter: {pf['train_regions']} training regions, kept {out['kept_fraction']}, hit loss "
+        f"{out['hit_loss']} ({lost}/{len(pp)}), queries/s {out['queries_per_s_plain']} -> "
+        f"{out['queries_per_s_filtered']}, candidate scoring {sp:.0f} -> {sf:.0f} ms, MLP forward "
+        f"{out['mlp_forward_tflops']} TFLOP/s")
     return out
 
 
@@ -303,12 +297,10 @@ def main():
     ap.add_argument("--full-queries", type=int, default=20,
                     help="queries of the exhaustive-mode leg (x 100,000 subjects); 0 skips it")
     ap.add_argument("--full-subjects", type=int, default=100_000)
-    ap.add_argument("--pf-train", type=int, default=4_000_000,
-                    help="training regions of the prefilter leg (0: skip); the test set has --pf-test")
-    ap.add_argument("--pf-test", type=int, default=4_000_000)
-    ap.add_argument("--pf-recall", type=float, default=0.995,
-                    help="recall of the prefilter's cut on the validation set")
-    ap.add_argument("--pf-epochs", type=int, default=12)
+    ap.add_argument("--pf-loss", type=float, default=0.008,
+                    help="prefilter threshold: largest share of reported-hit regions pruned on validation")
+    ap.add_argument("--no-prefilter", action="store_true", help="skip the learned-prefilter leg")
+    ap.add_argument("--pf-epochs", type=int, default=8)
     ap.add_argument("--search-queries", type=int, default=11206,
                     help="queries of the at-scale blastp leg (0: small workload only)")
     ap.add_argument("--search-decoys", type=int, default=320000)
@@ -514,7 +506,7 @@ def main():
               f"{fl['gcups']} GCUPS (with end cells {fems:.2f} ms, {fl['gcups_with_ends']} GCUPS), "
               f"mismatches {fbad}/40", file=sys.stderr, flush=True)
 
-    pf = prefilter_leg(ctx, args, sch) if args.pf_train > 0 else None
+    pf = None if (args.no_prefilter or args.search_queries <= 0) else prefilter_leg(ctx, args, sch)
     srch = None if args.no_search else search_leg(ctx, args, sch)
 
     cpu = None

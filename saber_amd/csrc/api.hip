@@ -2623,6 +2623,15 @@ int mlp_set(saber_hip_mlp* m, const float* params) {
 
 }  // namespace
 
+int saber::mlp_forward_device(saber_hip_mlp* m, const __bf16* X, uint64_t n, float* logits, hipStream_t st) {
+    if (!m) return fail(SABER_HIP_EINVAL, "null model");
+    if (n == 0) return SABER_HIP_OK;
+    HIP_TRY(mlp_launch_forward(mlp_weights(m), X, n, logits, m->cus, st));
+    return SABER_HIP_OK;
+}
+
+saber_hip_ctx* saber::mlp_context(const saber_hip_mlp* m) { return m ? m->ctx : nullptr; }
+
 extern "C" {
 
 int saber_hip_mlp_create(saber_hip_ctx* c, const float* params, uint64_t seed, saber_hip_mlp** out) {

@@ -337,4 +337,28 @@ struct FeatureArgs {
 // Query compositions (nq queries), then one lane per region.
 hipError_t launch_region_features(const FeatureArgs& a, uint32_t nq, hipStream_t s);
 
+// A search region as the search's feature kernel reads it.
+struct SearchFeatRegion {
+    uint64_t s_off;          // window start in the database residues
+    uint32_t m, n;           // query and window lengths
+    int32_t band_lo, band_hi;// band in window diagonals (the full rectangle when unbanded)
+    int32_t best, hsp_sum;
+    uint32_t n_hsp, query;
+    int32_t d_span;          // d_max - d_min
+    float thr;               // the query's region threshold
+    uint32_t slen;           // subject length
+    uint32_t banded;
+};
+struct SearchFeatureArgs {
+    const SearchFeatRegion* regions;
+    uint64_t n;
+    const uint8_t* sres;
+    const float* mean;
+    const float* inv_std;
+    __bf16* out;
+    float* qcomp;            // work: 20 per query
+};
+hipError_t launch_search_features(const SearchFeatureArgs& a, const uint64_t* qoff, const uint8_t* qres, uint32_t nq,
+                                  hipStream_t s);
+
 }  // namespace saber

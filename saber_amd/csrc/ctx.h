@@ -112,6 +112,11 @@ int score_regions_impl(saber_hip_ctx* c, const saber_hip_seqset* queries, const 
 // current device (xdrop_noalloc.hip); called when a context is created.
 int xdrop_noalloc_setup(int device);
 
+// Prefilter MLP forward over device features (bf16 [n][64]) into device
+// logits, enqueued on st; the model's context (its device).
+int mlp_forward_device(saber_hip_mlp* m, const __bf16* X, uint64_t n, float* logits, hipStream_t st);
+saber_hip_ctx* mlp_context(const saber_hip_mlp* m);
+
 // Full-SW launches of one run are spread over this many streams (all at the
 // default priority) so small subject buckets and the tails of large ones
 // overlap.

@@ -49,6 +49,60 @@ __global__ __launch_bounds__(256) void query_composition_kernel(const uint64_t* 
     }
 }
 
+// Subject-window composition into H[0..23]: aligned 16-byte blocks from the
+// one holding the window's first residue; loads past the window's last
+// block re-read that block, and bytes outside the window are not counted
+// (every block read lies inside the buffer, guard bytes included).
+__device__ __forceinline__ void window_hist(const uint8_t* s, uint64_t nwin, uint32_t* H) {
+#pragma unroll
+    for (int c = 0; c < 24; ++c) H[c] = 0;
+    if (!nwin) return;
+    const uintptr_t p = reinterpret_cast<uintptr_t>(s), pe = p + nwin;
+    const uintptr_t first = p & ~uintptr_t(15), last = (pe - 1) & ~uintptr_t(15);
+    const uint8_t* s0 = s - (p & 15);   // the first block, as a pointer
+    for (uintptr_t base = first; base <= last; base += 128) {
+        uint4 v[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const uintptr_t at = base + 16 * j;
+            const uintptr_t la = at <= last ? at : last;
+            v[j] = *reinterpret_cast<const uint4*>(s0 + (la - first));
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const uintptr_t at = base + 16 * j;
+            const int64_t b_lo = (int64_t)p - (int64_t)at;   // count bytes b with b_lo <= b < b_hi
+            const int64_t b_hi = (int64_t)pe - (int64_t)at;
+            const uint32_t d[4] = {v[j].x, v[j].y, v[j].z, v[j].w};
+#pragma unroll
+            for (int b = 0; b < 16; ++b)
+                if (b >= b_lo && b < b_hi) atomicAdd(&H[min((d[b >> 2] >> (8 * (b & 3))) & 0xFFu, 23u)], 1u);
+        }
+    }
+}
+
+// Standardise (optional) and store 64 features as bf16, eight 16-byte rows.
+__device__ __forceinline__ void store_features(float* f, const float* mean, const float* inv_std, __bf16* dst) {
+    if (mean) {
+#pragma unroll
+        for (int c = 0; c < 64; ++c) f[c] = (f[c] - mean[c]) * inv_std[c];
+    }
+    uint4* out = reinterpret_cast<uint4*>(dst);
+#pragma unroll
+    for (int x = 0; x < 8; ++x) {
+        uint32_t w4[4];
+#pragma unroll
+        for (int y = 0; y < 4; ++y) {
+            const __bf16 lo = (__bf16)f[8 * x + 2 * y], hi = (__bf16)f[8 * x + 2 * y + 1];
+            uint16_t bl, bh;
+            __builtin_memcpy(&bl, &lo, 2);
+            __builtin_memcpy(&bh, &hi, 2);
+            w4[y] = (uint32_t)bl | ((uint32_t)bh << 16);
+        }
+        out[x] = make_uint4(w4[0], w4[1], w4[2], w4[3]);
+    }
+}
+
 // One lane per region: the region's X-drop evidence, the composition of its
 // subject window (a per-lane histogram in LDS), its geometry and its query's
 // composition; the 64 features are stored as eight 16-byte rows. The kernel
@@ -66,36 +120,7 @@ __global__ __launch_bounds__(256) void region_features_kernel(FeatureArgs a) {
     const uint8_t* s = a.sres + a.soff[J.subject] + J.win_lo;
     const uint64_t nwin = J.win_hi - J.win_lo;
 
-    // subject-window composition: aligned 16-byte blocks from the one holding
-    // the window's first residue; loads past the window's last block re-read
-    // that block, and bytes outside the window are not counted (every block
-    // read lies inside the buffer, guard bytes included)
-#pragma unroll
-    for (int c = 0; c < 24; ++c) H[c] = 0;
-    if (nwin) {
-        const uintptr_t p = reinterpret_cast<uintptr_t>(s), pe = p + nwin;
-        const uintptr_t first = p & ~uintptr_t(15), last = (pe - 1) & ~uintptr_t(15);
-        const uint8_t* s0 = s - (p & 15);   // the first block, as a pointer
-        for (uintptr_t base = first; base <= last; base += 128) {
-            uint4 v[8];
-#pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                const uintptr_t at = base + 16 * j;
-                const uintptr_t la = at <= last ? at : last;
-                v[j] = *reinterpret_cast<const uint4*>(s0 + (la - first));
-            }
-#pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                const uintptr_t at = base + 16 * j;
-                const int64_t b_lo = (int64_t)p - (int64_t)at;   // count bytes b with b_lo <= b < b_hi
-                const int64_t b_hi = (int64_t)pe - (int64_t)at;
-                const uint32_t d[4] = {v[j].x, v[j].y, v[j].z, v[j].w};
-#pragma unroll
-                for (int b = 0; b < 16; ++b)
-                    if (b >= b_lo && b < b_hi) atomicAdd(&H[min((d[b >> 2] >> (8 * (b & 3))) & 0xFFu, 23u)], 1u);
-            }
-        }
-    }
+    window_hist(s, nwin, H);
 
     // X-drop evidence of the region's hits (four hits per 16-byte load when
     // the region's hits start 16-byte aligned)
@@ -169,24 +194,70 @@ __global__ __launch_bounds__(256) void region_features_kernel(FeatureArgs a) {
     for (int c = 0; c < 20; ++c) f[32 + c] = nwin ? (float)H[c] / (float)nwin : 0.f;
 #pragma unroll
     for (int c = 52; c < 64; ++c) f[c] = 0.f;
-    if (a.mean) {
+    store_features(f, a.mean, a.inv_std, a.out + r * 64);
+}
+
+// Features of the search's candidate regions (saber_hip_search): the
+// evidence is the region's own (best HSP, HSP count and saturating sum,
+// diagonal span), with its plan's window and band, the lengths, the query's
+// region threshold and the query / window compositions:
+//   0 best / 64   1 ln(1 + best)   2 HSPs / 4   3 HSP sum / 128
+//   4 diagonal span / 64   5 band width / 128 (the window rectangle's when
+//   unbanded)   6 ln(1 + window)   7 ln(1 + query)   8 best / ln(2 + query x
+//   window)   9 mean HSP score / 64   10 banded   11 best / region threshold
+//   12..31 query composition   32..51 window composition   52 ln(1 + subject)
+//   53 (best - region threshold) / 64   54 HSP sum / region threshold
+//   55..63 zero
+__global__ __launch_bounds__(256) void search_features_kernel(SearchFeatureArgs a) {
+    __shared__ uint32_t hist[256 * kHistStride];
+    uint32_t* H = hist + threadIdx.x * kHistStride;
+    const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= a.n) return;
+    const SearchFeatRegion R = a.regions[r];
+    window_hist(a.sres + R.s_off, R.n, H);
+    const float fb = (float)R.best, thr = fmaxf(R.thr, 1.f);
+    const float width = R.band_hi >= R.band_lo ? (float)(R.band_hi - R.band_lo + 1) : 0.f;
+    float f[64];
+    f[0] = fb / 64.f;
+    f[1] = logf(1.f + fmaxf(fb, 0.f));
+    f[2] = (float)R.n_hsp / 4.f;
+    f[3] = (float)R.hsp_sum / 128.f;
+    f[4] = (float)R.d_span / 64.f;
+    f[5] = width / 128.f;
+    f[6] = logf(1.f + (float)R.n);
+    f[7] = logf(1.f + (float)R.m);
+    f[8] = fb / logf(2.f + (float)R.m * (float)R.n);
+    f[9] = R.n_hsp ? (float)R.hsp_sum / (float)R.n_hsp / 64.f : 0.f;
+    f[10] = R.banded ? 1.f : 0.f;
+    f[11] = fb / thr;
+    const float4* qc = reinterpret_cast<const float4*>(a.qcomp + (uint64_t)R.query * 20);
 #pragma unroll
-        for (int c = 0; c < 64; ++c) f[c] = (f[c] - a.mean[c]) * a.inv_std[c];
+    for (int x = 0; x < 5; ++x) {
+        const float4 v = qc[x];
+        f[12 + 4 * x] = v.x;
+        f[13 + 4 * x] = v.y;
+        f[14 + 4 * x] = v.z;
+        f[15 + 4 * x] = v.w;
     }
-    uint4* out = reinterpret_cast<uint4*>(a.out + r * 64);
 #pragma unroll
-    for (int x = 0; x < 8; ++x) {
-        uint32_t w4[4];
+    for (int c = 0; c < 20; ++c) f[32 + c] = R.n ? (float)H[c] / (float)R.n : 0.f;
+    f[52] = logf(1.f + (float)R.slen);
+    f[53] = (fb - R.thr) / 64.f;
+    f[54] = (float)R.hsp_sum / thr;
 #pragma unroll
-        for (int y = 0; y < 4; ++y) {
-            const __bf16 lo = (__bf16)f[8 * x + 2 * y], hi = (__bf16)f[8 * x + 2 * y + 1];
-            uint16_t bl, bh;
-            __builtin_memcpy(&bl, &lo, 2);
-            __builtin_memcpy(&bh, &hi, 2);
-            w4[y] = (uint32_t)bl | ((uint32_t)bh << 16);
-        }
-        out[x] = make_uint4(w4[0], w4[1], w4[2], w4[3]);
-    }
+    for (int c = 55; c < 64; ++c) f[c] = 0.f;
+    store_features(f, a.mean, a.inv_std, a.out + r * 64);
+}
+
+hipError_t launch_search_features(const SearchFeatureArgs& a, const uint64_t* qoff, const uint8_t* qres, uint32_t nq,
+                                  hipStream_t s) {
+    if (a.n == 0) return hipSuccess;
+    hipLaunchKernelGGL(query_composition_kernel, dim3((uint32_t)std::min<uint64_t>((nq + 3) / 4, 4096)), dim3(256),
+                       0, s, qoff, qres, nq, a.qcomp);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(search_features_kernel, dim3((uint32_t)((a.n + 255) / 256)), dim3(256), 0, s, a);
+    return hipGetLastError();
 }
 
 hipError_t launch_region_features(const FeatureArgs& a, uint32_t nq, hipStream_t s) {

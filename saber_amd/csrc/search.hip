@@ -656,6 +656,13 @@ void scan_cache_free(ScanDbCache* p) { delete p; }
 
 struct saber_hip_search {
     uint32_t nq = 0;
+    // the capped regions of all queries with their plans (feature export)
+    std::vector<saber_hip_region> flat;
+    std::vector<saber_hip_plan> plans;
+    std::vector<uint32_t> qidx;
+    std::vector<uint64_t> qlen, slen;
+    std::vector<int32_t> rthr;
+    uint64_t db_gen = 0;
     std::vector<std::vector<saber_hip_hit>> hits;
     std::vector<std::vector<uint8_t>> op_codes;
     std::vector<std::vector<uint32_t>> op_lens;
@@ -915,6 +922,59 @@ struct HitCand {
     double e;
     uint64_t ri;
 };
+
+// Prefilter features of every region (bf16 [n][64] on the device): the
+// query compositions from the device query codes, the windows from the
+// database.
+int region_features_dev(saber_hip_ctx* c, hipStream_t st, const uint64_t* d_qoff, const uint8_t* d_qres, uint32_t nq,
+                        const std::vector<saber_hip_region>& regs, const std::vector<saber_hip_plan>& plans,
+                        const std::vector<uint32_t>& qidx, const std::vector<uint64_t>& qlen,
+                        const std::vector<uint64_t>& slen, const std::vector<int32_t>& rthr, const float* mean,
+                        const float* inv_std, DevBuf<__bf16>& out) {
+    const uint64_t n = regs.size();
+    std::vector<SearchFeatRegion> rec(n);
+    parallel_for((n + 4095) / 4096, [&](uint64_t blk) {
+        for (uint64_t i = blk * 4096; i < std::min<uint64_t>(n, (blk + 1) * 4096); ++i) {
+            const saber_hip_region& r = regs[i];
+            const saber_hip_plan& pl = plans[i];
+            SearchFeatRegion& x = rec[i];
+            const uint32_t q = qidx[i];
+            x.s_off = c->db_off[r.subject] + pl.lo;
+            x.m = (uint32_t)qlen[q];
+            x.n = (uint32_t)(pl.hi - pl.lo);
+            x.band_lo = pl.banded ? (int32_t)pl.band_lo : -(int32_t)x.m + 1;
+            x.band_hi = pl.banded ? (int32_t)pl.band_hi : (int32_t)x.n - 1;
+            x.best = r.best;
+            x.hsp_sum = r.hsp_sum;
+            x.n_hsp = r.n_hsp;
+            x.query = q;
+            x.d_span = (int32_t)(r.d_max - r.d_min);
+            x.thr = (float)rthr[q];
+            x.slen = (uint32_t)slen[i];
+            x.banded = pl.banded ? 1u : 0u;
+        }
+    });
+    DevBuf<SearchFeatRegion> d_rec;
+    DevBuf<float> d_mean, d_inv, d_qcomp;
+    if (int r = upload(d_rec, rec.data(), n, st)) return r;
+    if (mean && inv_std) {
+        if (int r = upload(d_mean, mean, 64, st)) return r;
+        if (int r = upload(d_inv, inv_std, 64, st)) return r;
+    }
+    HIP_TRY(d_qcomp.alloc(20ull * std::max<uint32_t>(nq, 1)));
+    HIP_TRY(out.reserve(64 * std::max<uint64_t>(n, 1)));
+    SearchFeatureArgs fa{};
+    fa.regions = d_rec.p;
+    fa.n = n;
+    fa.sres = c->db.p + kGuard;
+    fa.mean = (mean && inv_std) ? d_mean.p : nullptr;
+    fa.inv_std = (mean && inv_std) ? d_inv.p : nullptr;
+    fa.out = out.p;
+    fa.qcomp = d_qcomp.p;
+    HIP_TRY(launch_search_features(fa, d_qoff, d_qres, nq, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return SABER_HIP_OK;
+}
 
 }  // namespace
 }  // namespace saber
@@ -1492,16 +1552,24 @@ int saber_hip_search_run(saber_hip_ctx* c, const saber_hip_seqset* queries, cons
     // ---- learned prefilter (optional) ----
     t0 = Clock::now();
     std::vector<uint8_t> pruned(nreg, 0);
+    std::vector<int32_t> rthr_q(nq);
+    for (uint32_t q = 0; q < nq; ++q) rthr_q[q] = prep[q].region_thr;
     if (prefilter && nreg) {
-        std::vector<saber_hip_band_job> fj(nreg);
+        if (mlp_context(prefilter->model) != c) return fail(SABER_HIP_EINVAL, "prefilter model of another context");
+        DevBuf<__bf16> X;
+        DevBuf<float> logits;
+        if (int r = region_features_dev(c, st, d_qoff.p, d_q.p + kGuard, nq, flat, plans, qidx, qlen, slen_of, rthr_q,
+                                        prefilter->mean, prefilter->inv_std, X))
+            return r;
+        HIP_TRY(logits.alloc(nreg));
+        if (int r = mlp_forward_device(prefilter->model, X.p, nreg, logits.p, st)) return r;
+        std::vector<float> lg(nreg);
+        HIP_TRY(hipMemcpyAsync(lg.data(), logits.p, sizeof(float) * nreg, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
         for (uint64_t i = 0; i < nreg; ++i) {
-            const saber_hip_plan& pl = plans[i];
-            const int64_t m = (int64_t)qlen[qidx[i]], nw = (int64_t)(pl.hi - pl.lo);
-            int64_t lo = pl.banded ? pl.band_lo : -(m - 1), hi = pl.banded ? pl.band_hi : nw - 1;
-            fj[i] = saber_hip_band_job{qidx[i], (uint32_t)flat[i].subject, (uint32_t)pl.lo, (uint32_t)pl.hi,
-                                       (int32_t)lo, (int32_t)hi};
+            pruned[i] = lg[i] < prefilter->threshold ? 1 : 0;
+            ctr.regions_pruned += pruned[i];
         }
-        (void)fj;
     }
     tm.prefilter = ms_since(t0);
 
@@ -1795,6 +1863,13 @@ int saber_hip_search_run(saber_hip_ctx* c, const saber_hip_seqset* queries, cons
             o.evalue = reg_e[i];
         }
     }
+    res->flat.swap(flat);
+    res->plans.swap(plans);
+    res->qidx.swap(qidx);
+    res->qlen = qlen;
+    res->slen.swap(slen_of);
+    res->rthr = rthr_q;
+    res->db_gen = c->db_gen;
     tm.total = ms_since(t_start);
     *out = res.release();
     return SABER_HIP_OK;
@@ -1835,11 +1910,27 @@ int saber_hip_search_regions(const saber_hip_search* s, uint32_t q, saber_hip_se
 }
 
 int saber_hip_search_features(saber_hip_search* s, const float* mean, const float* inv_std, uint16_t* out) {
-    (void)s;
-    (void)mean;
-    (void)inv_std;
-    (void)out;
-    return fail(SABER_HIP_EINVAL, "not built yet");
+    if (!s || !s->ctx) return fail(SABER_HIP_EINVAL, "null search");
+    const uint64_t n = s->flat.size();
+    if (n == 0) return SABER_HIP_OK;
+    if (!out) return fail(SABER_HIP_EINVAL, "out is null");
+    saber_hip_ctx* c = s->ctx;
+    std::lock_guard<std::mutex> g(c->mu);
+    HIP_TRY(hipSetDevice(c->device));
+    if (c->db_gen != s->db_gen) return fail(SABER_HIP_ESTATE, "the database changed since the search");
+    hipStream_t st = c->stream;
+    DevBuf<uint8_t> d_q;
+    DevBuf<uint64_t> d_qoff;
+    const uint64_t qtotal = s->qoff.back();
+    if (int r = upload_codes(d_q, s->qres.data(), qtotal, st)) return r;
+    if (int r = upload(d_qoff, s->qoff.data(), s->qoff.size(), st)) return r;
+    DevBuf<__bf16> X;
+    if (int r = region_features_dev(c, st, d_qoff.p, d_q.p + kGuard, s->nq, s->flat, s->plans, s->qidx, s->qlen,
+                                    s->slen, s->rthr, mean, inv_std, X))
+        return r;
+    HIP_TRY(hipMemcpyAsync(out, X.p, 2 * 64 * n, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return SABER_HIP_OK;
 }
 
 int saber_hip_search_get_counters(const saber_hip_search* s, saber_hip_search_counters* out) {

@@ -153,8 +153,10 @@ def run(ctx, qres, qoff, scheme, karlin, seeds=None, params=None, prefilter=None
     h = C.c_void_p()
     pf = None
     if prefilter is not None:
-        pf = _ffi.Prefilter(prefilter["model"].h, _ffi.ptr(prefilter.get("mean"), C.c_float),
-                            _ffi.ptr(prefilter.get("inv_std"), C.c_float), float(prefilter["threshold"]), 0)
+        mean = np.ascontiguousarray(prefilter["mean"], np.float32)
+        inv = np.ascontiguousarray(prefilter["inv_std"], np.float32)
+        pf = _ffi.Prefilter(prefilter["model"].h, _ffi.ptr(mean, C.c_float), _ffi.ptr(inv, C.c_float),
+                            float(prefilter["threshold"]), 0)
     _ffi.check(_ffi.lib().saber_hip_search_run(ctx.h, C.byref(qs), C.byref(ss), C.byref(sch), C.byref(karlin._k),
                                                 C.byref(params), C.byref(pf) if pf is not None else None,
                                                 C.byref(h)))
@@ -203,5 +205,111 @@ def destroy(handle):
 
 
 __all__ = ["ScanSeeds", "default_params", "composition_bias", "run", "Hit", "SearchResult", "alphabet_map",
-           "full_alphabet", "MURPHY10_GROUPS", "DS16", "REGION_DTYPE", "destroy"]
+           "full_alphabet", "MURPHY10_GROUPS", "DS16", "REGION_DTYPE", "destroy", "features", "close", "hit_pairs",
+           "train_prefilter", "prefilter_arg"]
 
+
+
+def features(result, mean=None, std=None):
+    """Prefilter features (bf16 bit patterns [n, 64]) of every capped region of
+    a search run with keep_handle=True, in query order then region order;
+    with mean / std, standardised on the device before rounding."""
+    h = getattr(result, "handle", None)
+    if h is None:
+        raise ValueError("search result kept no handle (run(..., keep_handle=True))")
+    n = int(sum(len(r) for r in result.regions))
+    mu = inv = None
+    if mean is not None:
+        mu = np.ascontiguousarray(mean, np.float32)
+        s = np.asarray(std, np.float32)
+        inv = np.ascontiguousarray(np.where(s > 0, 1.0 / np.where(s > 0, s, 1), 0.0), np.float32)
+    out = np.zeros((max(n, 1), 64), np.uint16)
+    _ffi.check(_ffi.lib().saber_hip_search_features(h, _ffi.ptr(mu, C.c_float), _ffi.ptr(inv, C.c_float),
+                                                     _ffi.ptr(out, C.c_uint16)))
+    return out[:n]
+
+
+def close(result):
+    destroy(getattr(result, "handle", None))
+    result.handle = None
+
+
+def hit_pairs(result):
+    """(query, subject) of every reported hit."""
+    return {(q, h.subject) for q, hl in enumerate(result.hits) for h in hl}
+
+
+def train_prefilter(ctx, train_sets, valid_set, scheme, karlin, loss_target=0.005, epochs=8, batch=65536,
+                    lr=2e-3, seed=0, params=None, log=None):
+    """The learned region prefilter trained on regions the plain search itself
+    labels: every capped region of the plain GPU search over each training
+    set, label 1 when its exact score passes the E-value cut (flag 2, the
+    reporting cut of engine.rs:1138-1161). Features are standardised with
+    the training set's statistics; the model (64 -> 256 -> 256 -> 1, bf16
+    MFMA kernels) trains on the device with Adam and a class weight; the
+    threshold is set on the validation set so that at most loss_target of
+    the regions the plain search reported a hit from are pruned.
+    Returns dict(model, mean, std, threshold, train_regions, positives, ...)."""
+    from . import prefilter as PF
+    log = log or (lambda *a: None)
+    raws, labels = [], []
+    for d in train_sets:
+        ctx.upload_db(d["sres"], d["soff"])
+        r = run(ctx, d["qres"], d["qoff"], scheme, karlin, params=params, keep_handle=True)
+        raws.append(features(r))
+        labels.append(np.concatenate([(x["flags"] & 2) != 0 for x in r.regions]).astype(np.float32))
+        close(r)
+    X = np.concatenate(raws)
+    del raws
+    y = np.concatenate(labels)
+    # statistics and standardisation in chunks (tens of millions of rows)
+    s1 = np.zeros(64)
+    s2 = np.zeros(64)
+    for a in range(0, len(X), 1 << 20):
+        xf = PF.bf16_float(X[a:a + (1 << 20)]).astype(np.float64)
+        s1 += xf.sum(axis=0)
+        s2 += (xf * xf).sum(axis=0)
+    mean = (s1 / len(X)).astype(np.float32)
+    std = np.sqrt(np.maximum(s2 / len(X) - (s1 / len(X)) ** 2, 0.0)).astype(np.float32)
+    inv = np.where(std > 0, 1.0 / np.where(std > 0, std, 1), 0.0).astype(np.float32)
+    for a in range(0, len(X), 1 << 20):
+        xf = PF.bf16_float(X[a:a + (1 << 20)])
+        X[a:a + (1 << 20)] = PF.bf16_bits(((xf - mean) * inv).astype(np.float32))
+    Xs = X
+    npos = float(y.sum())
+    pos_weight = float(min(200.0, max(1.0, (len(y) - npos) / max(npos, 1.0))))
+    m = PF.Mlp(ctx, seed=seed)
+    rng = np.random.default_rng(seed)
+    perm = rng.permutation(len(y))
+    m.upload(Xs[perm], y[perm])
+    del Xs, X
+    steps = 0
+    for ep in range(epochs):
+        lr_e = lr * (0.5 ** max(0, ep - epochs // 2))
+        loss = None
+        for row0 in range(0, len(y), batch):
+            v = m.train_resident(row0, min(batch, len(y) - row0), lr_e, pos_weight, want_loss=(row0 == 0))
+            loss = v if row0 == 0 else loss
+            steps += 1
+        log(f"# prefilter epoch {ep}: first-batch loss {loss:.4f}")
+    # threshold on the validation set(s)
+    lvs, fls = [], []
+    for vs in (valid_set if isinstance(valid_set, (list, tuple)) else [valid_set]):
+        ctx.upload_db(vs["sres"], vs["soff"])
+        r = run(ctx, vs["qres"], vs["qoff"], scheme, karlin, params=params, keep_handle=True)
+        Xv = features(r, mean, std)
+        fls.append(np.concatenate([x["flags"] for x in r.regions]))
+        close(r)
+        lvs.append(m.forward(Xv))
+    lv, flags = np.concatenate(lvs), np.concatenate(fls)
+    rep = (flags & 4) != 0
+    tau = float(np.quantile(lv[rep], loss_target)) if rep.any() else float(lv.min())
+    tau = float(np.nextafter(np.float32(tau), np.float32(-np.inf)))
+    return dict(model=m, mean=mean, std=std, inv_std=inv, threshold=tau,
+                train_regions=int(len(y)), positives=int(npos), pos_weight=pos_weight, steps=steps,
+                valid_regions=int(len(lv)), valid_kept=float((lv >= tau).mean()))
+
+
+def prefilter_arg(pf):
+    """The run(..., prefilter=) argument of a train_prefilter result."""
+    return {"model": pf["model"], "mean": pf["mean"], "inv_std": pf["inv_std"], "threshold": pf["threshold"]}
