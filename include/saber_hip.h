@@ -347,8 +347,10 @@ int saber_hip_band_choose(saber_hip_ctx* ctx, int w_lo, int w_hi, int L, int C, 
  * into classes of widths covered by the same shapes; starting from the
  * modelled choice, classes holding at least 1 % of the sample's cells are
  * visited in decreasing share, each covering shape is tried for the class,
- * and it is kept when the sample's packed launch (scores only, HIP events,
- * 5 runs) gets faster by more than 1 %. The result becomes the context's
+ * and it is kept when the sample's packed launch (scores only, HIP events)
+ * gets clearly faster: the median of three timings (each the mean of three
+ * launches) lower than the incumbent's by more than 1 % and by more than the
+ * larger of the two spreads (max - min). The result becomes the context's
  * width -> shape table for batches created afterwards. report (may be NULL)
  * receives up to nreport visited classes (reserved = 1 where the shape
  * changed; gcups = the sample's rate after tuning); nclasses = their count. */
@@ -368,7 +370,8 @@ int saber_hip_band_autotune(saber_hip_ctx* ctx, const saber_hip_seqset* queries,
  * per group, in decreasing share of the residues, every shape covering the
  * group's longest subject up to twice its width is timed on a sample (whole
  * queries spread over the length order, up to target_cells cells against all
- * subjects; 0 = 5e10) and kept when the pass gets faster by more than 1 %.
+ * subjects; 0 = 5e10) and kept when the pass gets clearly faster (the rule
+ * of saber_hip_band_autotune: median of three, beyond 1 % and the spread).
  * The choice applies to full batches of this flavour (ends) created
  * afterwards on the context. report entries: w_lo / w_hi = the group's
  * subject lengths, L, C = the chosen shape (K = 0; reserved = 1 when it is

@@ -457,3 +457,275 @@ void cpu_banded_many_avx2(const uint8_t* qres, const uint8_t* sres, const int8_t
     for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
     free(grp);
 }
+
+/* ---- score_many (src/algorithm/sw_simd.rs:95-153): full-matrix SW scores
+ * of one query against many subjects. Subjects longest first (ties by
+ * index); an 8-bit pass over 32 subjects per register (sw_simd.rs:287-427:
+ * unsigned saturating lanes with the substitution offset, exact when the
+ * lane maximum stays below 255 - pmax); its other subjects re-scored by the
+ * 16-bit pass over 16 subjects per register (sw_simd.rs:157-277); a 16-bit
+ * score reaching 32767 repaired by the int32 full-band score. ---- */
+
+typedef struct {
+    const uint8_t* s;
+    int64_t n;
+    int64_t idx;
+} mjob;
+
+static int mjob_cmp(const void* a, const void* b) {
+    const mjob *x = (const mjob*)a, *y = (const mjob*)b;
+    if (x->n != y->n) return x->n > y->n ? -1 : 1;
+    return x->idx < y->idx ? -1 : (x->idx > y->idx);
+}
+
+/* 16-bit pass over the subjects of `jobs` (already in order) */
+static void many_i16(const uint8_t* q, int64_t m, const int8_t* bias, const mjob* jobs, int64_t nj,
+                     const int32_t* mat, int32_t go, int32_t ge, __m256i* H, __m256i* E, __m256i* B,
+                     int32_t* out) {
+    __m128i tlo[24], thi[24];
+    for (int a = 0; a < 24; ++a) {
+        int8_t lo[16], hi[16];
+        memset(hi, 0, sizeof hi);
+        for (int x = 0; x < 16; ++x) {
+            int v = mat[a * 24 + x];
+            lo[x] = (int8_t)(v < -128 ? -128 : v > 127 ? 127 : v);
+        }
+        for (int x = 16; x < 24; ++x) {
+            int v = mat[a * 24 + x];
+            hi[x - 16] = (int8_t)(v < -128 ? -128 : v > 127 ? 127 : v);
+        }
+        tlo[a] = _mm_loadu_si128((const __m128i*)lo);
+        thi[a] = _mm_loadu_si128((const __m128i*)hi);
+    }
+    for (int64_t r = 0; r < m; ++r) B[r] = _mm256_set1_epi16(bias ? bias[r] : 0);
+    const __m256i go_v = _mm256_set1_epi16((int16_t)go), ge_v = _mm256_set1_epi16((int16_t)ge);
+    const __m256i zero = _mm256_setzero_si256(), neg = _mm256_set1_epi16(-16384);
+    const __m128i fifteen = _mm_set1_epi8(15), low_nib = _mm_set1_epi8(0x0f);
+    for (int64_t start = 0; start < nj; start += 16) {
+        const int nl = (int)(nj - start < 16 ? nj - start : 16);
+        int64_t max_len = 0;
+        for (int l = 0; l < nl; ++l)
+            if (jobs[start + l].n > max_len) max_len = jobs[start + l].n;
+        for (int64_t r = 0; r < m; ++r) {
+            H[r] = zero;
+            E[r] = neg;
+        }
+        __m256i max_h = zero, profile[24];
+        for (int64_t col = 0; col < max_len; ++col) {
+            uint8_t resid[16];
+            int16_t act[16];
+            for (int l = 0; l < 16; ++l) {
+                if (l < nl && col < jobs[start + l].n) {
+                    resid[l] = (uint8_t)code(jobs[start + l].s[col]);
+                    act[l] = -1;
+                } else {
+                    resid[l] = 22;
+                    act[l] = 0;
+                }
+            }
+            const __m256i active = _mm256_loadu_si256((const __m256i*)act);
+            const __m128i rv = _mm_loadu_si128((const __m128i*)resid);
+            const __m128i idx = _mm_and_si128(rv, low_nib), is_hi = _mm_cmpgt_epi8(rv, fifteen);
+            for (int a = 0; a < 24; ++a)
+                profile[a] = _mm256_cvtepi8_epi16(_mm_blendv_epi8(_mm_shuffle_epi8(tlo[a], idx),
+                                                                  _mm_shuffle_epi8(thi[a], idx), is_hi));
+            __m256i u_f = neg, u_h = zero, ul_h = zero;
+            for (int64_t r = 0; r < m; ++r) {
+                const __m256i p = _mm256_adds_epi16(profile[code(q[r])], B[r]);
+                const __m256i ph = H[r];
+                const __m256i e = _mm256_max_epi16(_mm256_subs_epi16(ph, go_v), _mm256_subs_epi16(E[r], ge_v));
+                const __m256i f = _mm256_max_epi16(_mm256_subs_epi16(u_h, go_v), _mm256_subs_epi16(u_f, ge_v));
+                const __m256i diag = _mm256_adds_epi16(ul_h, p);
+                const __m256i h = _mm256_max_epi16(_mm256_max_epi16(e, f), _mm256_max_epi16(diag, zero));
+                max_h = _mm256_max_epi16(max_h, _mm256_and_si256(h, active));
+                u_f = f;
+                ul_h = ph;
+                u_h = h;
+                H[r] = h;
+                E[r] = e;
+            }
+        }
+        int16_t res[16];
+        _mm256_storeu_si256((__m256i*)res, max_h);
+        for (int l = 0; l < nl; ++l) out[jobs[start + l].idx] = res[l];
+    }
+}
+
+/* 8-bit pass; returns the number of subjects to re-score (their jobs are
+ * written to redo), or -1 when the scheme does not fit 8-bit lanes */
+static int64_t many_u8(const uint8_t* q, int64_t m, const int8_t* bias, const mjob* jobs, int64_t nj,
+                       const int32_t* mat, int32_t go, int32_t ge, __m256i* H, __m256i* E, __m256i* B,
+                       int32_t* out, mjob* redo) {
+    int smin = 1 << 30, smax = -(1 << 30), bmin = 0, bmax = 0;
+    for (int i = 0; i < 576; ++i) {
+        if (mat[i] < smin) smin = mat[i];
+        if (mat[i] > smax) smax = mat[i];
+    }
+    if (bias) {
+        bmin = 127;
+        bmax = -128;
+        for (int64_t r = 0; r < m; ++r) {
+            if (bias[r] < bmin) bmin = bias[r];
+            if (bias[r] > bmax) bmax = bias[r];
+        }
+    }
+    int t = smin + (bmin < 0 ? bmin : 0);
+    const int off = -(t < 0 ? t : 0);
+    const int pmax = smax + (bmax > 0 ? bmax : 0) + off;
+    if (pmax >= 128 || off > 127 || go < 0 || go > 255 || ge < 0 || ge > 255) return -1;
+    const int limit = 255 - pmax;
+    __m256i tlo[24], thi[24];
+    for (int a = 0; a < 24; ++a) {
+        uint8_t t0[16], t1[16];
+        memset(t1, 0, sizeof t1);
+        for (int x = 0; x < 16; ++x) t0[x] = (uint8_t)(mat[a * 24 + x] + off);
+        for (int x = 16; x < 24; ++x) t1[x - 16] = (uint8_t)(mat[a * 24 + x] + off);
+        tlo[a] = _mm256_broadcastsi128_si256(_mm_loadu_si128((const __m128i*)t0));
+        thi[a] = _mm256_broadcastsi128_si256(_mm_loadu_si128((const __m128i*)t1));
+    }
+    for (int64_t r = 0; r < m; ++r) B[r] = _mm256_set1_epi8(bias ? bias[r] : 0);
+    const __m256i go_v = _mm256_set1_epi8((char)go), ge_v = _mm256_set1_epi8((char)ge);
+    const __m256i offv = _mm256_set1_epi8((char)off), zero = _mm256_setzero_si256();
+    const __m256i nib = _mm256_set1_epi8(0x0f), fifteen = _mm256_set1_epi8(15);
+    int64_t nredo = 0;
+    for (int64_t start = 0; start < nj; start += 32) {
+        const int nl = (int)(nj - start < 32 ? nj - start : 32);
+        int64_t max_len = 0;
+        for (int l = 0; l < nl; ++l)
+            if (jobs[start + l].n > max_len) max_len = jobs[start + l].n;
+        for (int64_t r = 0; r < m; ++r) {
+            H[r] = zero;
+            E[r] = zero;
+        }
+        __m256i max_h = zero, profile[24];
+        for (int64_t col = 0; col < max_len; ++col) {
+            uint8_t resid[32], act[32];
+            for (int l = 0; l < 32; ++l) {
+                if (l < nl && col < jobs[start + l].n) {
+                    resid[l] = (uint8_t)code(jobs[start + l].s[col]);
+                    act[l] = 0xFF;
+                } else {
+                    resid[l] = 22;
+                    act[l] = 0;
+                }
+            }
+            const __m256i active = _mm256_loadu_si256((const __m256i*)act);
+            const __m256i rv = _mm256_loadu_si256((const __m256i*)resid);
+            const __m256i idx = _mm256_and_si256(rv, nib), ishi = _mm256_cmpgt_epi8(rv, fifteen);
+            for (int a = 0; a < 24; ++a)
+                profile[a] = _mm256_blendv_epi8(_mm256_shuffle_epi8(tlo[a], idx), _mm256_shuffle_epi8(thi[a], idx), ishi);
+            __m256i u_f = zero, u_h = zero, ul_h = zero;
+            for (int64_t r = 0; r < m; ++r) {
+                const __m256i p = _mm256_add_epi8(profile[code(q[r])], B[r]);
+                const __m256i ph = H[r];
+                const __m256i e = _mm256_max_epu8(_mm256_subs_epu8(ph, go_v), _mm256_subs_epu8(E[r], ge_v));
+                const __m256i f = _mm256_max_epu8(_mm256_subs_epu8(u_h, go_v), _mm256_subs_epu8(u_f, ge_v));
+                const __m256i diag = _mm256_subs_epu8(_mm256_adds_epu8(ul_h, p), offv);
+                const __m256i h = _mm256_max_epu8(_mm256_max_epu8(e, f), diag);
+                max_h = _mm256_max_epu8(max_h, _mm256_and_si256(h, active));
+                u_f = f;
+                ul_h = ph;
+                u_h = h;
+                H[r] = h;
+                E[r] = e;
+            }
+        }
+        uint8_t res[32];
+        _mm256_storeu_si256((__m256i*)res, max_h);
+        for (int l = 0; l < nl; ++l) {
+            if ((int)res[l] >= limit) redo[nredo++] = jobs[start + l];
+            else out[jobs[start + l].idx] = res[l];
+        }
+    }
+    return nredo;
+}
+
+typedef struct {
+    const uint8_t *qres, *sres;
+    const uint64_t *qoff, *soff;
+    const int8_t* bias;
+    uint32_t nq, ns;
+    const int32_t* mat;
+    int32_t go, ge;
+    int32_t* out;          /* [nq][ns] */
+    int64_t next;          /* next query (atomic) */
+    mjob* order;           /* subjects longest first, shared */
+} many_drv;
+
+static void* many_worker(void* p) {
+    many_drv* D = (many_drv*)p;
+    int64_t maxm = 1;
+    for (uint32_t i = 0; i < D->nq; ++i)
+        if ((int64_t)(D->qoff[i + 1] - D->qoff[i]) > maxm) maxm = (int64_t)(D->qoff[i + 1] - D->qoff[i]);
+    __m256i* H = (__m256i*)aligned_alloc(32, sizeof(__m256i) * maxm);
+    __m256i* E = (__m256i*)aligned_alloc(32, sizeof(__m256i) * maxm);
+    __m256i* B = (__m256i*)aligned_alloc(32, sizeof(__m256i) * maxm);
+    mjob* redo = (mjob*)malloc(sizeof(mjob) * (D->ns ? D->ns : 1));
+    int32_t* o16 = (int32_t*)malloc(sizeof(int32_t) * (D->ns ? D->ns : 1));
+    for (;;) {
+        const int64_t qi = __atomic_fetch_add(&D->next, 1, __ATOMIC_RELAXED);
+        if (qi >= (int64_t)D->nq) break;
+        const uint8_t* q = D->qres + D->qoff[qi];
+        const int64_t m = (int64_t)(D->qoff[qi + 1] - D->qoff[qi]);
+        const int8_t* b = D->bias ? D->bias + D->qoff[qi] : NULL;
+        int32_t* out = D->out + (uint64_t)qi * D->ns;
+        memset(out, 0, sizeof(int32_t) * D->ns);
+        if (m == 0) continue;
+        int64_t nr = many_u8(q, m, b, D->order, D->ns, D->mat, D->go, D->ge, H, E, B, out, redo);
+        if (nr < 0) {
+            many_i16(q, m, b, D->order, D->ns, D->mat, D->go, D->ge, H, E, B, out);
+        } else if (nr > 0) {
+            /* the redo list keeps the longest-first order */
+            many_i16(q, m, b, redo, nr, D->mat, D->go, D->ge, H, E, B, out);
+        }
+        for (uint32_t s = 0; s < D->ns; ++s)
+            if (out[s] >= 32767) {
+                int32_t sc;
+                int64_t qe, se;
+                uint64_t cells;
+                const int64_t n = (int64_t)(D->soff[s + 1] - D->soff[s]);
+                or_banded_score(q, m, D->sres + D->soff[s], n, b, D->mat, D->go, D->ge, -m, n, &sc, &qe, &se, &cells);
+                out[s] = sc;
+            }
+        (void)o16;
+    }
+    free(H);
+    free(E);
+    free(B);
+    free(redo);
+    free(o16);
+    return NULL;
+}
+
+/* score_many of every query against every subject, queries spread over
+ * nthreads threads; out [nq][ns]. */
+void cpu_score_many_avx2(const uint8_t* qres, const uint64_t* qoff, uint32_t nq, const int8_t* bias,
+                         const uint8_t* sres, const uint64_t* soff, uint32_t ns, const int32_t* mat, int32_t go,
+                         int32_t ge, int32_t* out, int nthreads) {
+    many_drv D;
+    D.qres = qres;
+    D.sres = sres;
+    D.qoff = qoff;
+    D.soff = soff;
+    D.bias = bias;
+    D.nq = nq;
+    D.ns = ns;
+    D.mat = mat;
+    D.go = go;
+    D.ge = ge;
+    D.out = out;
+    D.next = 0;
+    D.order = (mjob*)malloc(sizeof(mjob) * (ns ? ns : 1));
+    for (uint32_t s = 0; s < ns; ++s) {
+        D.order[s].s = sres + soff[s];
+        D.order[s].n = (int64_t)(soff[s + 1] - soff[s]);
+        D.order[s].idx = s;
+    }
+    qsort(D.order, ns, sizeof(mjob), mjob_cmp);
+    if (nthreads < 1) nthreads = 1;
+    pthread_t* th = (pthread_t*)malloc(sizeof(pthread_t) * nthreads);
+    for (int t = 0; t < nthreads; ++t) pthread_create(&th[t], NULL, many_worker, &D);
+    for (int t = 0; t < nthreads; ++t) pthread_join(th[t], NULL);
+    free(th);
+    free(D.order);
+}

@@ -136,6 +136,9 @@ def avx2_lib():
         L.cpu_banded_many_avx2.argtypes = [u8p, u8p, i8p, u64p, u32p, u64p, u32p, i32p, i32p, C.c_int64,
                                            i32p, C.c_int32, C.c_int32, i32p, C.c_int]
         L.cpu_banded_many_avx2.restype = None
+        L.cpu_score_many_avx2.argtypes = [u8p, u64p, C.c_uint32, i8p, u8p, u64p, C.c_uint32, i32p, C.c_int32,
+                                          C.c_int32, i32p, C.c_int]
+        L.cpu_score_many_avx2.restype = None
         _avx2 = L
     return _avx2
 
@@ -161,6 +164,22 @@ def banded_many_avx2(qres, qoff, sres, soff, jobs, matrix, go, ge, qbias=None, n
                                     _p(jsl, C.c_uint32), _p(lo, C.c_int32), _p(hi, C.c_int32), n,
                                     _p(mat, C.c_int32), go, ge, _p(score, C.c_int32), nthreads)
     return score
+
+
+def score_many_avx2(qres, qoff, sres, soff, matrix, go, ge, qbias=None, nthreads=8):
+    """Full-matrix SW scores [nq, ns] by the AVX2 restatement of score_many
+    (sw_simd.rs:95-153: 8-bit pass, 16-bit re-score, int32 repair), queries
+    spread over nthreads threads."""
+    qres, sres = _u8(qres), _u8(sres)
+    qoff = np.ascontiguousarray(qoff, np.uint64)
+    soff = np.ascontiguousarray(soff, np.uint64)
+    nq, ns = len(qoff) - 1, len(soff) - 1
+    out = np.zeros((max(nq, 1), max(ns, 1)), np.int32)
+    b = _bias(qbias)
+    avx2_lib().cpu_score_many_avx2(_p(qres, C.c_uint8), _p(qoff, C.c_uint64), nq, _p(b, C.c_int8),
+                                   _p(sres, C.c_uint8), _p(soff, C.c_uint64), ns, _p(_mat(matrix), C.c_int32), go, ge,
+                                   _p(out, C.c_int32), nthreads)
+    return out[:nq, :ns]
 
 
 def xdrop(q, s, bias, matrix, qp, sp, x):

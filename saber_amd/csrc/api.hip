@@ -201,6 +201,27 @@ void radix_sort_by(std::vector<uint32_t>& v, const std::vector<uint64_t>& key, i
     }
 }
 
+// Autotuner measurement: the median of three timings and their spread
+// (max - min). A candidate replaces the incumbent only when its median is
+// lower by more than both spreads and by more than 1 %, so run-to-run noise
+// does not flip the choice.
+struct Timing {
+    double median = 0.0, spread = 0.0;
+};
+template <class F>
+int timed3(F&& once, Timing* out) {
+    double t[3];
+    for (int i = 0; i < 3; ++i)
+        if (int r = once(&t[i])) return r;
+    std::sort(t, t + 3);
+    out->median = t[1];
+    out->spread = t[2] - t[0];
+    return SABER_HIP_OK;
+}
+bool clearly_faster(const Timing& cand, const Timing& inc) {
+    return cand.median < inc.median * 0.99 && inc.median - cand.median > std::max(cand.spread, inc.spread);
+}
+
 int bits_of(uint64_t x) {
     int b = 0;
     while (b < 64 && (x >> b)) ++b;
@@ -448,7 +469,8 @@ int saber_hip_band_autotune(saber_hip_ctx* c, const saber_hip_seqset* queries,
     }
     // Coordinate descent from the modelled choice: the classes in decreasing
     // share of the sample's cells; per class every covering shape, kept when
-    // the sample's packed launch gets faster by more than 1 %.
+    // the sample's packed launch gets clearly faster (clearly_faster: median
+    // of three timings lower by more than 1 % and by more than their spread).
     std::vector<int> order;
     for (size_t cl = 0; cl < classes.size(); ++cl)
         if (ccells[cl] * 100 >= tot && ccells[cl] > 0) order.push_back((int)cl);
@@ -460,17 +482,23 @@ int saber_hip_band_autotune(saber_hip_ctx* c, const saber_hip_seqset* queries,
         saved_rows = c->latency_rows;
         saved_lat = c->choice_lat;
     }
-    auto trial = [&](double* ms_out) -> int {
+    auto trial = [&](Timing* out) -> int {
         saber_hip_band_batch* b = nullptr;
         if (int r = saber_hip_band_batch_create(c, queries, subjects, jobs, nsample, scheme, 0, &b)) return r;
-        double ms = 0.0, band_ms = 0.0;
-        const int r = saber_hip_band_batch_time(b, 1, 5, &ms, &band_ms, nullptr, nullptr, nullptr);
+        bool warm = false;
+        const int r = timed3([&](double* ms_out) -> int {
+            double ms = 0.0, band_ms = 0.0;
+            const int rr = saber_hip_band_batch_time(b, warm ? 0 : 1, 3, &ms, &band_ms, nullptr, nullptr, nullptr);
+            warm = true;
+            *ms_out = band_ms;
+            return rr;
+        }, out);
         saber_hip_band_batch_destroy(b);
-        *ms_out = band_ms;
         return r;
     };
-    double best_ms = 0.0;
-    int status = trial(&best_ms);
+    Timing best;
+    int status = trial(&best);
+    double best_ms = best.median;
     std::vector<int> chosen(classes.size(), -1);
     for (int cl : order) {
         if (status != SABER_HIP_OK) break;
@@ -483,11 +511,12 @@ int saber_hip_band_autotune(saber_hip_ctx* c, const saber_hip_seqset* queries,
                 std::lock_guard<std::mutex> g(c->mu);
                 for (int w = w_lo; w <= w_hi; ++w) c->choice[w] = (int)k;
             }
-            double ms = 0.0;
-            status = trial(&ms);
+            Timing t;
+            status = trial(&t);
             if (status != SABER_HIP_OK) break;
-            if (ms < best_ms * 0.99) {
-                best_ms = ms;
+            if (clearly_faster(t, best)) {
+                best = t;
+                best_ms = t.median;
                 chosen[cl] = (int)k;
             }
         }
@@ -2387,22 +2416,28 @@ int saber_hip_full_autotune(saber_hip_ctx* c, const saber_hip_seqset* queries, c
     sq.count = nsq;
     sq.bias = queries->bias ? sq_bias.data() : nullptr;
     const uint64_t sample_cells = sq_res.size() * stot;
-    auto trial = [&](double* ms_out) -> int {
+    auto trial = [&](Timing* out) -> int {
         saber_hip_full_batch* b = nullptr;
         if (int r = saber_hip_full_batch_create(c, &sq, subjects, scheme, ends, &b)) return r;
-        const int r = saber_hip_full_batch_time(b, 1, 5, ms_out, nullptr, nullptr);
+        bool warm = false;
+        const int r = timed3([&](double* ms_out) -> int {
+            const int rr = saber_hip_full_batch_time(b, warm ? 0 : 1, 3, ms_out, nullptr, nullptr);
+            warm = true;
+            return rr;
+        }, out);
         saber_hip_full_batch_destroy(b);
         return r;
     };
     // Coordinate descent: classes in decreasing share of the residues; per
     // class every shape covering its longest subject up to twice its width,
-    // kept when the sample's pass gets faster by more than 1 %.
+    // kept when the sample's pass gets clearly faster (clearly_faster).
     std::vector<int> order;
     for (int v = 0; v < nshape; ++v)
         if (res[v] * 100 >= stot && res[v] > 0) order.push_back(v);
     std::sort(order.begin(), order.end(), [&](int x, int y) { return res[x] > res[y]; });
-    double best_ms = 0.0;
-    int status = trial(&best_ms);
+    Timing best;
+    int status = trial(&best);
+    double best_ms = best.median;
     std::vector<int> chosen(nshape, -1);
     for (int v : order) {
         if (status != SABER_HIP_OK) break;
@@ -2415,11 +2450,12 @@ int saber_hip_full_autotune(saber_hip_ctx* c, const saber_hip_seqset* queries, c
                 std::lock_guard<std::mutex> g(c->mu);
                 c->full_tuned[fl][v] = u;
             }
-            double ms = 0.0;
-            status = trial(&ms);
+            Timing t;
+            status = trial(&t);
             if (status != SABER_HIP_OK) break;
-            if (ms < best_ms * 0.99) {
-                best_ms = ms;
+            if (clearly_faster(t, best)) {
+                best = t;
+                best_ms = t.median;
                 chosen[v] = u;
             }
         }

@@ -64,3 +64,33 @@ def test_overflow_and_int16_decline():
     want = O.banded_many(qres, qoff, sres, soff, jobs, BLOSUM62, 11, 1)[0]
     assert got.tolist() == want.tolist()
     assert got[0] == 3300 and got[2] == 44000 and got[3] == 44000
+
+
+def test_score_many_avx2_equals_sw_reference():
+    """The AVX2 restatement of score_many (the exhaustive-mode CPU baseline)
+    against the oracle's full Gotoh on random, related and saturating pairs,
+    with and without a bias: the 8-bit pass, its 16-bit re-score and the
+    int32 repair are all exercised."""
+    from tests.gen import Lcg, flat, mutate, rand_seq
+    from oracle import oracle as O
+    from saber_amd import BLOSUM62
+    r = Lcg(2024)
+    qs = [rand_seq(r, 5 + r.next(300)) for _ in range(6)]
+    ss = [rand_seq(r, 1 + r.next(400)) for _ in range(40)] + [mutate(r, qs[i % 6]) for i in range(20)]
+    w = np.full(4000, 17, np.uint8)   # W x 4000 = 44 000 (banded.rs:719-728): the int32 repair
+    qs.append(w)
+    ss.append(w.copy())
+    ss.append(np.full(120, 17, np.uint8))   # 1320 > 255: the 16-bit re-score
+    qres, qoff = flat(qs)
+    sres, soff = flat(ss)
+    for bias in (None, ((np.arange(len(qres)) * 31) % 9 - 4).astype(np.int8)):
+        got = O.score_many_avx2(qres, qoff, sres, soff, BLOSUM62, 11, 1, qbias=bias, nthreads=4)
+        for i in range(len(qs)):
+            q = qres[int(qoff[i]):int(qoff[i + 1])]
+            b = None if bias is None else bias[int(qoff[i]):int(qoff[i + 1])]
+            for j in range(len(ss)):
+                s = sres[int(soff[j]):int(soff[j + 1])]
+                if len(q) * len(s) > 300_000 and not (len(q) == 4000 and len(s) == 4000):
+                    continue
+                assert got[i, j] == O.sw_reference(q, s, BLOSUM62, b, 11, 1)[0], (i, j)
+    assert got[len(qs) - 1, len(ss) - 2] == 44000 + int(bias[-4000:].sum())
