@@ -53,6 +53,10 @@ extern "C" {
 /* Context options. */
 #define SABER_HIP_FORCE_EXACT32 1u /* score every band job with the int32 kernel (testing) */
 #define SABER_HIP_NO_GRAPH 2u      /* enqueue band and full passes directly instead of replaying a captured graph */
+#define SABER_HIP_INJECT_FAULTS 4u /* every scoring entry point (score_windows, score_bands, score_full, xdrop,
+                                      band_align, traceback and the batch creators) fails with SABER_HIP_EDEVICE "injected fault" and writes
+                                      no result: exercises a host's fallback path (backend_fallbacks,
+                                      src/search/engine.rs:1107-1118) */
 typedef struct saber_hip_opts {
     uint32_t flags;
     uint32_t reserved[7];

@@ -291,10 +291,11 @@ def band_variants():
 class Context:
     """One device context (saber_hip_create)."""
 
-    def __init__(self, device=0, force_exact32=False, graphs=True):
+    def __init__(self, device=0, force_exact32=False, graphs=True, inject_faults=False):
         L = _ffi.lib()
         opts = _ffi.Opts()
-        opts.flags = (_ffi.FORCE_EXACT32 if force_exact32 else 0) | (0 if graphs else _ffi.NO_GRAPH)
+        opts.flags = ((_ffi.FORCE_EXACT32 if force_exact32 else 0) | (0 if graphs else _ffi.NO_GRAPH) |
+                      (_ffi.INJECT_FAULTS if inject_faults else 0))
         h = C.c_void_p()
         self._children = weakref.WeakSet()   # batches and models on this context
         _ffi.check(L.saber_hip_create(device, C.byref(opts), C.byref(h)))
@@ -638,8 +639,8 @@ class HipBackend:
     """ScoreBackend on the MI355X (src/gpu/mod.rs:45-59): exact i32 scores,
     identical to score_many; errors raise (the engine then falls back)."""
 
-    def __init__(self, device=0):
-        self.ctx = Context(device)
+    def __init__(self, device=0, inject_faults=False):
+        self.ctx = Context(device, inject_faults=inject_faults)
 
     def name(self):
         return "hip-gfx950"

@@ -13,6 +13,7 @@ WLIB_PATH = os.path.join(HERE, "libsaber_workload.so")
 OK, EINVAL, EDEVICE, ENOMEM, ESTATE = 0, 1, 2, 3, 4
 FORCE_EXACT32 = 1
 NO_GRAPH = 2
+INJECT_FAULTS = 4
 
 
 class Opts(C.Structure):

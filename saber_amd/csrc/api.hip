@@ -72,6 +72,12 @@ hipError_t fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s) {
     return hipGetLastError();
 }
 
+int injected(const saber_hip_ctx* c) {
+    if (c && (c->flags & SABER_HIP_INJECT_FAULTS))
+        return fail(SABER_HIP_EDEVICE, "injected fault (SABER_HIP_INJECT_FAULTS)");
+    return SABER_HIP_OK;
+}
+
 int check_seqset(const saber_hip_seqset* s, const char* what) {
     if (!s) return fail(SABER_HIP_EINVAL, std::string(what) + " is null");
     if (s->count > 0 && !s->offsets) return fail(SABER_HIP_EINVAL, std::string(what) + ": offsets null");
@@ -554,6 +560,7 @@ int saber_hip_band_batch_create(saber_hip_ctx* c, const saber_hip_seqset* querie
                                 const saber_hip_seqset* subjects, const saber_hip_band_job* jobs,
                                 uint64_t njobs, const saber_hip_scheme* scheme, int ends,
                                 saber_hip_band_batch** out) {
+    if (int r_ = injected(c)) return r_;
     if (!c || !out || !scheme) return fail(SABER_HIP_EINVAL, "null argument");
     *out = nullptr;
     if (njobs > 0 && !jobs) return fail(SABER_HIP_EINVAL, "jobs is null");
@@ -1058,6 +1065,7 @@ int saber_hip_score_bands(saber_hip_ctx* c, const saber_hip_seqset* queries,
                           const saber_hip_seqset* subjects, const saber_hip_band_job* jobs,
                           uint64_t njobs, const saber_hip_scheme* scheme, int32_t* score,
                           uint32_t* q_end, uint32_t* s_end, uint64_t* cells) {
+    if (int r_ = injected(c)) return r_;
     if (njobs > 0 && !score) return fail(SABER_HIP_EINVAL, "score is null");
     saber_hip_band_batch* b = nullptr;
     static const bool trace = std::getenv("SABER_HIP_TRACE_HOST") != nullptr;
@@ -1419,6 +1427,7 @@ int saber_hip_band_align(saber_hip_ctx* c, const saber_hip_seqset* queries, cons
                          const saber_hip_band_job* jobs, const int32_t* stop_at, uint64_t njobs,
                          const saber_hip_scheme* scheme, uint64_t max_dir_bytes, saber_hip_alignment* out,
                          uint8_t* op_codes, uint32_t* op_lens, uint64_t op_capacity) {
+    if (int r_ = injected(c)) return r_;
     return band_align_impl(c, queries, subjects, jobs, stop_at, njobs, scheme, max_dir_bytes, out, op_codes,
                            op_lens, op_capacity, 0, nullptr);
 }
@@ -1443,6 +1452,7 @@ int saber_hip_traceback_ex(saber_hip_ctx* c, const saber_hip_seqset* queries, co
                            int64_t band_pad, int exhaustive, uint64_t max_dir_bytes, saber_hip_alignment* out,
                            uint8_t* exact, uint8_t* op_codes, uint32_t* op_lens, uint64_t op_capacity,
                            saber_hip_traceback_counters* counters, saber_hip_traceback_counters* job_counters) {
+    if (int r_ = injected(c)) return r_;
     if (counters) *counters = saber_hip_traceback_counters{};
     if (!c || !scheme) return fail(SABER_HIP_EINVAL, "null argument");
     if (njobs == 0) return SABER_HIP_OK;
@@ -1730,6 +1740,7 @@ int saber_hip_score_windows(saber_hip_ctx* c, const uint8_t* query, uint32_t qle
                             const int8_t* bias, const uint8_t* const* windows,
                             const uint32_t* lens, uint32_t n, const saber_hip_scheme* scheme,
                             int32_t* out) {
+    if (int r_ = injected(c)) return r_;
     if (!c || !scheme) return fail(SABER_HIP_EINVAL, "null argument");
     if (n == 0) return SABER_HIP_OK;
     if (!out || !windows || !lens) return fail(SABER_HIP_EINVAL, "null argument");
@@ -1806,6 +1817,7 @@ int saber_hip_xdrop_batch_create(saber_hip_ctx* c, const saber_hip_seqset* queri
                                  const saber_hip_seqset* subjects, const saber_hip_xdrop_hit* hits,
                                  uint64_t nhits, const saber_hip_scheme* scheme, int32_t x,
                                  saber_hip_xdrop_batch** out) {
+    if (int r_ = injected(c)) return r_;
     if (!c || !out || !scheme) return fail(SABER_HIP_EINVAL, "null argument");
     *out = nullptr;
     if (nhits > 0 && !hits) return fail(SABER_HIP_EINVAL, "hits is null");
@@ -1952,6 +1964,7 @@ int saber_hip_xdrop(saber_hip_ctx* c, const saber_hip_seqset* queries,
                     const saber_hip_seqset* subjects, const saber_hip_xdrop_hit* hits,
                     uint64_t nhits, const saber_hip_scheme* scheme, int32_t x, int32_t* score,
                     uint32_t* q_lo, uint32_t* q_hi, uint32_t* steps) {
+    if (int r_ = injected(c)) return r_;
     if (nhits > 0 && !score) return fail(SABER_HIP_EINVAL, "score is null");
     saber_hip_xdrop_batch* b = nullptr;
     if (int r = saber_hip_xdrop_batch_create(c, queries, subjects, hits, nhits, scheme, x, &b)) return r;
@@ -2134,6 +2147,7 @@ extern "C" {
 int saber_hip_full_batch_create(saber_hip_ctx* c, const saber_hip_seqset* queries,
                                 const saber_hip_seqset* subjects, const saber_hip_scheme* scheme, int ends,
                                 saber_hip_full_batch** out) {
+    if (int r_ = injected(c)) return r_;
     if (!c || !out || !scheme) return fail(SABER_HIP_EINVAL, "null argument");
     *out = nullptr;
     if (int r = check_seqset(queries, "queries")) return r;
@@ -2493,6 +2507,7 @@ int saber_hip_full_autotune(saber_hip_ctx* c, const saber_hip_seqset* queries, c
 int saber_hip_score_full(saber_hip_ctx* c, const saber_hip_seqset* queries,
                          const saber_hip_seqset* subjects, const saber_hip_scheme* scheme,
                          int32_t* score, uint32_t* q_end, uint32_t* s_end) {
+    if (int r_ = injected(c)) return r_;
     saber_hip_full_batch* b = nullptr;
     if (int r = saber_hip_full_batch_create(c, queries, subjects, scheme, q_end || s_end, &b)) return r;
     int r = saber_hip_full_batch_run(b);

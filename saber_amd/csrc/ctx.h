@@ -72,6 +72,8 @@ struct Shape {
 constexpr uint64_t kGuard = 16, kGuardTail = 64;
 
 hipError_t fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
+// SABER_HIP_EDEVICE when the context injects faults (SABER_HIP_INJECT_FAULTS).
+int injected(const saber_hip_ctx* c);
 int check_seqset(const saber_hip_seqset* s, const char* what);
 
 template <class T>
