@@ -18,9 +18,12 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
 # Packed fp32 VALU ops are kept off: the compiler would otherwise form
-# v_pk_*_f32 from ordinary float code.
+# v_pk_*_f32 from ordinary float code. Calls are never marked as tail calls:
+# a non-inlined device function reached by a tail call has to save the
+# callee-saved registers it uses on the stack, one spill per register and
+# call (the per-shape bodies of the band and full-matrix kernels).
 FLAGS = [
-    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-fno-optimize-sibling-calls",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
 ]

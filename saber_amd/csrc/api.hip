@@ -265,15 +265,16 @@ void default_choice(saber_hip_ctx* c) {
 }
 
 // Pair table of the packed kernels: TP[q][a][b] = S(q,a) | S(q,b) << 16 (int16
-// halves) for query codes q < 24 and subject codes a, b <= 24, where code 24
-// (the sentinel column) scores -128 and the scores are clamped to int8 (the
-// packed kernels only run when every score fits).
+// halves) for query codes q <= 24 and subject codes a, b <= 24, where code 24
+// (the sentinel column, the dead query row) scores -128 and the scores are
+// clamped to int8 (the packed kernels only run when every score fits).
 std::vector<uint32_t> build_pair_table(const int32_t* matrix) {
     std::vector<uint32_t> t(kPairWords, 0);
     auto s8 = [&](int q, int x) -> int16_t {
-        return (int16_t)(x == kBandSentinel ? -128 : std::max(-128, std::min(127, matrix[q * 24 + x])));
+        if (q == kBandSentinel || x == kBandSentinel) return -128;
+        return (int16_t)std::max(-128, std::min(127, matrix[q * 24 + x]));
     };
-    for (int q = 0; q < 24; ++q)
+    for (int q = 0; q <= kBandSentinel; ++q)
         for (int x = 0; x <= kBandSentinel; ++x)
             for (int y = 0; y <= kBandSentinel; ++y)
                 t[(size_t)q * 625 + x * 25 + y] = (uint32_t)(uint16_t)s8(q, x) | ((uint32_t)(uint16_t)s8(q, y) << 16);
@@ -589,6 +590,15 @@ int saber_hip_band_batch_create(saber_hip_ctx* c, const saber_hip_seqset* querie
     for (int i = 0; i < 576; ++i)
         if (scheme->matrix[i] < -128 || scheme->matrix[i] > 127) packed_ok = false;
 
+    // Scores-only jobs without bias may run transposed (query and window
+    // exchanged, band negated: the same score under a symmetric matrix) when
+    // that gives them fewer rows, as it does for windows narrower than their
+    // band, whose band otherwise sweeps mostly outside the rectangle.
+    bool swap_ok = packed_ok && !b->ends && !b->has_bias;
+    for (int x = 0; x < 24 && swap_ok; ++x)
+        for (int y = 0; y < x; ++y)
+            if (scheme->matrix[x * 24 + y] != scheme->matrix[y * 24 + x]) swap_ok = false;
+
     static const bool trace = std::getenv("SABER_HIP_TRACE_HOST") != nullptr;
     auto tp = std::chrono::steady_clock::now();
     auto lap = [&](const char* what) {
@@ -632,9 +642,27 @@ int saber_hip_band_batch_create(saber_hip_ctx* c, const saber_hip_seqset* querie
         d.w = (int32_t)w;
         d.i_first = (int32_t)i_first;
         d.rows = (int32_t)(i_last - i_first + 1);
+        host_cells[k] = band_cells(d);
+        if (swap_ok) {
+            // transposed: rows over the window, band [-hi, -lo] (already
+            // inside [-(n - 1), m - 1] since lo >= -(m - 1) and hi <= n - 1)
+            const int64_t t_first = std::max<int64_t>(0, lo), t_last = std::min<int64_t>(n - 1, m - 1 + hi);
+            if (t_last >= t_first && t_last - t_first + 1 < d.rows) {
+                BandJobDev t{};
+                t.q_off = d.s_off;
+                t.s_off = d.q_off;
+                t.m = (uint32_t)n;
+                t.n = (uint32_t)m;
+                t.swapped = 1;
+                t.lo = (int32_t)-hi;
+                t.w = (int32_t)w;
+                t.i_first = (int32_t)t_first;
+                t.rows = (int32_t)(t_last - t_first + 1);
+                d = t;
+            }
+        }
         b->host_jobs[k] = d;
         rows_of[k] = d.rows;
-        host_cells[k] = band_cells(d);
         // The packed kernel keeps row indices in 16 bits and window columns
         // (times 8) in 32 bits.
         const bool fits = d.rows <= (int32_t)kMaxBandRows && m < kMaxPackedLen && n < kMaxPackedLen;
@@ -735,8 +763,12 @@ int saber_hip_band_batch_create(saber_hip_ctx* c, const saber_hip_seqset* querie
             return b->host_jobs[x].q_off == b->host_jobs[y].q_off && b->host_jobs[x].i_first == b->host_jobs[y].i_first;
         };
         for (size_t i = 0; i < v.size();) {
+            if (b->host_jobs[v[i]].swapped) {  // rows from the residue blob: never shared
+                rest.push_back(v[i++]);
+                continue;
+            }
             size_t j = i;
-            while (j < v.size() && same_group(v[j], v[i])) ++j;
+            while (j < v.size() && !b->host_jobs[v[j]].swapped && same_group(v[j], v[i])) ++j;
             const size_t even = i + ((j - i) & ~(size_t)1);
             shared.insert(shared.end(), v.begin() + i, v.begin() + even);
             if (even < j) rest.push_back(v[even]);
@@ -780,14 +812,17 @@ int saber_hip_band_batch_create(saber_hip_ctx* c, const saber_hip_seqset* querie
     }
 
     // Score table for the packed kernel: int8 T[a][c] = S(a, c) for codes
-    // a, c < 24 (stride 25), column 24 = sentinel (-128). Only built when
-    // every score fits int8 (packed_ok); otherwise no job uses it.
+    // a, c < 24 (stride 25), column 24 = sentinel and row 24 = dead row
+    // (-128). Only built when every score fits int8 (packed_ok); otherwise no
+    // job uses it.
     std::vector<uint32_t> table(kTable8Words, 0);
     {
         int8_t* t8 = reinterpret_cast<int8_t*>(table.data());
-        for (int a = 0; a < 24; ++a)
+        for (int a = 0; a <= kBandSentinel; ++a)
             for (int cc = 0; cc <= kBandSentinel; ++cc) {
-                const int v = cc == kBandSentinel ? -128 : std::max(-128, std::min(127, scheme->matrix[a * 24 + cc]));
+                const int v = (a == kBandSentinel || cc == kBandSentinel)
+                                  ? -128
+                                  : std::max(-128, std::min(127, scheme->matrix[a * 24 + cc]));
                 t8[a * kTable8Stride + cc] = (int8_t)v;
             }
     }

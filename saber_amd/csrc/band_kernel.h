@@ -121,13 +121,24 @@ __device__ __forceinline__ uint32_t pair_entries(uint32_t wa, uint32_t wb, int h
     return __builtin_bit_cast(uint32_t, e);
 }
 
-// One group of L lanes scoring job pair g (lane l of the group). Tb is the
-// int8 score table in LDS; Tp the pair table (SHARED only: both jobs of the
-// pair have the same query and first row, so a row's query code is common
-// and one LDS read of Tp[q][sA][sB] returns both scores already packed).
+// The kernel arguments as the workgroup's copy in LDS (a reference to the
+// kernel's own parameter would copy it onto every wave's stack).
+typedef const BandKernelArgs* band_args;
+
+// The pairs [pair_off, pair_off + npairs) of one chunk, L lanes per pair:
+// the group of lanes lane / L scores pair pair_off + lane / L (lane l = lane
+// % L of the group). Tb is the int8 score table in LDS; Tp the pair table
+// (SHARED only: both jobs of the pair have the same query and first row, so
+// a row's query code is common and one LDS read of Tp[q][sA][sB] returns
+// both scores already packed). Lane-dependent values are formed here, not in
+// the caller, so that the caller keeps no vector registers across the call.
 template <int L, int C, int K, bool BIAS, bool SHARED, bool ENDS>
-__device__ __attribute__((noinline)) void band_pair(const BandKernelArgs& a, const int8_t* Tb, const uint32_t* Tp, uint32_t g,
-                                          int l) {
+__device__ __attribute__((noinline)) void band_pair(band_args a, const int8_t* Tb, const uint32_t* Tp,
+                                                    uint32_t pair_off, uint32_t npairs) {
+    const uint32_t lane = __lane_id();
+    if (lane / L >= npairs) return;
+    const uint32_t g = pair_off + lane / L;
+    const int l = (int)(lane % L);
     static_assert(C >= 2 && K >= 1 && K <= C, "variant shape");
     static_assert(L == 1 || L == 2 || L == 4 || L == 8 || L == 16, "groups must tile a DPP row");
     constexpr int NW = (C + 6) / 4;               // byte window: dwords of 4 columns
@@ -141,11 +152,13 @@ __device__ __attribute__((noinline)) void band_pair(const BandKernelArgs& a, con
     int64_t qoff[2], soff[2];
     uint32_t n[2];
     int lo[2], w[2], ifirst[2], rows[2];
+    bool swapped[2];
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
-        jid[h] = a.pairs[2 * g + h];
+        jid[h] = a->pairs[2 * g + h];
         if (jid[h] != kNone) {
-            const BandJobDev J = a.jobs[jid[h]];
+            const BandJobDev J = a->jobs[jid[h]];
+            swapped[h] = J.swapped;
             qoff[h] = (int64_t)J.q_off;
             soff[h] = (int64_t)J.s_off;
             n[h] = J.n;
@@ -154,6 +167,7 @@ __device__ __attribute__((noinline)) void band_pair(const BandKernelArgs& a, con
             ifirst[h] = J.i_first;
             rows[h] = J.rows;
         } else {
+            swapped[h] = false;
             qoff[h] = 0;
             soff[h] = 0;
             n[h] = 0;
@@ -165,8 +179,8 @@ __device__ __attribute__((noinline)) void band_pair(const BandKernelArgs& a, con
     }
     const int R = rows[0] > rows[1] ? rows[0] : rows[1];
 
-    const v2s GO = {(short)a.go, (short)a.go};
-    const v2s GE = {(short)a.ge, (short)a.ge};
+    const v2s GO = {(short)a->go, (short)a->go};
+    const v2s GE = {(short)a->ge, (short)a->ge};
     const v2s ZERO = {0, 0};
 
     // Caps for the last K cells of this lane: cell k keeps its value if k < w,
@@ -201,7 +215,10 @@ __device__ __attribute__((noinline)) void band_pair(const BandKernelArgs& a, con
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
         jb[h] = ifirst[h] + lo[h] + l * C - l;
-        ss[h].init(a.sres, soff[h], a.sres_len);
+        if (swapped[h])
+            ss[h].init(a->qres, soff[h], a->qres_len);
+        else
+            ss[h].init(a->sres, soff[h], a->sres_len);
         if constexpr (!SHARED) {
 #pragma unroll
             for (int x = 0; x < NW; ++x) {
@@ -231,13 +248,16 @@ __device__ __attribute__((noinline)) void band_pair(const BandKernelArgs& a, con
 #pragma unroll
     for (int h = 0; h < NH; ++h) {
         qb[h] = ifirst[h] - l;
-        qs[h].init(a.qres, qoff[h], a.qres_len);
+        if (swapped[h])
+            qs[h].init(a->sres, qoff[h], a->sres_len);
+        else
+            qs[h].init(a->qres, qoff[h], a->qres_len);
         q_cur[h] = qs[h].load(qb[h]);
         q_n1[h] = qs[h].load(qb[h] + 4);
         q_n2[h] = qs[h].load(qb[h] + 8);
         b_cur[h] = b_n1[h] = b_n2[h] = 0;
         if constexpr (BIAS) {
-            bs[h].init(reinterpret_cast<const uint8_t*>(a.qbias), qoff[h], a.qres_len);
+            bs[h].init(reinterpret_cast<const uint8_t*>(a->qbias), qoff[h], a->qres_len);
             b_cur[h] = bs[h].load(qb[h]);
             b_n1[h] = bs[h].load(qb[h] + 4);
             b_n2[h] = bs[h].load(qb[h] + 8);
@@ -403,10 +423,10 @@ __device__ __attribute__((noinline)) void band_pair(const BandKernelArgs& a, con
         if (l == 0 && jid[h] != kNone) {
             const uint32_t id = jid[h];
             if (best >= kMax16) {
-                const uint32_t slot = atomicAdd(a.nflagged, 1u);
-                a.flagged[slot] = id;
+                const uint32_t slot = atomicAdd(a->nflagged, 1u);
+                a->flagged[slot] = id;
             } else {
-                a.out.score[id] = best;
+                a->out.score[id] = best;
                 if constexpr (ENDS) {
                     uint32_t qe = 0, se = 0;
                     if (best > 0) {
@@ -416,8 +436,8 @@ __device__ __attribute__((noinline)) void band_pair(const BandKernelArgs& a, con
                         qe = (uint32_t)i + 1;
                         se = (uint32_t)(i + lo[h] + (int)k + 1);
                     }
-                    a.out.q_end[id] = qe;
-                    a.out.s_end[id] = se;
+                    a->out.q_end[id] = qe;
+                    a->out.s_end[id] = se;
                 }
             }
         }
@@ -443,28 +463,33 @@ template <bool BIAS, bool ENDS>
 __global__ __launch_bounds__(768) void band_multi_kernel(BandKernelArgs a) {
     __shared__ uint32_t T4[kTable8Words];
     __shared__ uint32_t TP[kPairWords];
+    __shared__ BandKernelArgs SA;
     for (int t = threadIdx.x; t < kTable8Words; t += blockDim.x) T4[t] = a.table[t];
     for (int t = threadIdx.x; t < kPairWords; t += blockDim.x) TP[t] = a.pair_table[t];
+    if (threadIdx.x == 0) SA = a;
     __syncthreads();
     const int8_t* Tb = reinterpret_cast<const int8_t*>(T4);
+    const band_args ap = &SA;
     const int lane = threadIdx.x & 63;
     for (;;) {
         uint32_t ci = 0;
         if (lane == 0) ci = atomicAdd(a.next_chunk, 1u);
         ci = __builtin_amdgcn_readfirstlane(ci);
         if (ci >= a.nchunks) break;
+        // The chunk record in scalar registers: nothing of it is kept in
+        // vector registers across the call.
         const BandBlock blk = a.blocks[ci];
-        const bool shared = (blk.flags & kChunkShared) != 0;
+        const uint32_t shape = __builtin_amdgcn_readfirstlane(blk.shape);
+        const uint32_t pair_off = __builtin_amdgcn_readfirstlane(blk.pair_off);
+        const uint32_t npairs = __builtin_amdgcn_readfirstlane(blk.npairs);
+        const bool shared = (__builtin_amdgcn_readfirstlane(blk.flags) & kChunkShared) != 0;
         int id = 0;
 #define SABER_BAND_CASE(L_, C_, K_)                                                              \
-        if (blk.shape == (uint32_t)id) {                                                         \
-            const uint32_t q = (uint32_t)lane / (L_);                                            \
-            if (q < blk.npairs) {                                                                \
-                if (shared)                                                                      \
-                    band_pair<L_, C_, K_, BIAS, true, ENDS>(a, Tb, TP, blk.pair_off + q, lane % (L_)); \
-                else                                                                             \
-                    band_pair<L_, C_, K_, BIAS, false, ENDS>(a, Tb, TP, blk.pair_off + q, lane % (L_)); \
-            }                                                                                    \
+        if (shape == (uint32_t)id) {                                                             \
+            if (shared)                                                                          \
+                band_pair<L_, C_, K_, BIAS, true, ENDS>(ap, Tb, TP, pair_off, npairs);           \
+            else                                                                                 \
+                band_pair<L_, C_, K_, BIAS, false, ENDS>(ap, Tb, TP, pair_off, npairs);          \
             continue;                                                                            \
         }                                                                                        \
         ++id;

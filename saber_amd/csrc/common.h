@@ -30,20 +30,29 @@ __device__ __forceinline__ uint32_t lds_u32(lds_bytes p) {
 }
 constexpr short kMax16 = 32767;      // any job whose best reaches this is rescored in int32
 
-// Band kernel score table: int8, 24 query rows x 25 columns (subject codes
-// 0..23 and the sentinel column 24, scoring -128), staged into LDS as words.
+// Band kernel score table: int8, 25 query rows x 25 columns (subject codes
+// 0..23 and the sentinel column 24, scoring -128; query row 24 is the dead
+// row, scoring -128 against every column), staged into LDS as words. Code 24
+// stands for a subject column outside the window and for a query row outside
+// the query.
 constexpr int kBandSentinel = 24;
 constexpr uint32_t kSentinel4 = 0x18181818u;  // four sentinel bytes
 constexpr uint32_t kTable8Stride = 25;
-constexpr int kTable8Words = (24 * 25 + 3) / 4;
+constexpr int kTable8Words = (25 * 25 + 3) / 4;
 constexpr uint32_t kMaxBandRows = 32767;      // rows of one packed job (row index kept in 16 bits)
 
 // One banded job after planning (band clipped to its rectangle, rows that
 // hold band cells). Band cell k of row i is subject column j = i + lo + k.
+// A swapped job is the transposed problem of a scores-only job without bias
+// under a symmetric matrix (same score): its rows run over the subject window
+// (q_off is in the residue blob) and its columns over the query (s_off is in
+// the query blob), band negated.
 struct BandJobDev {
     uint64_t q_off;     // first query residue (and bias entry) in the query blob
     uint64_t s_off;     // first subject-window residue in the residue blob
-    uint32_t m, n;      // query / window lengths
+    uint32_t m : 31;    // query length
+    uint32_t swapped : 1;
+    uint32_t n;         // window length
     int32_t lo;         // clipped band lo (diagonal j - i)
     int32_t w;          // clipped band width
     int32_t i_first;    // first row holding a band cell
@@ -65,11 +74,11 @@ struct BandBlock {
 };
 constexpr uint32_t kChunkShared = 1u;
 
-// Pair table of the shared-query flavour: for each query code q < 24 and
-// subject codes a, b <= 24 (24 = sentinel), the dword (S(q, a), S(q, b)) as
-// two int16, at byte (q * 625 + a * 25 + b) * 4.
+// Pair table of the shared-query flavour: for each query code q <= 24 (24 =
+// the dead row) and subject codes a, b <= 24 (24 = sentinel), the dword
+// (S(q, a), S(q, b)) as two int16, at byte (q * 625 + a * 25 + b) * 4.
 constexpr uint32_t kPairRowBytes = 625 * 4;
-constexpr int kPairWords = 24 * 625;
+constexpr int kPairWords = 25 * 625;
 
 struct BandKernelArgs {
     const uint32_t* pairs;      // 2 job ids per pair (0xFFFFFFFF = none)

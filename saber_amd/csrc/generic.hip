@@ -32,9 +32,9 @@ __device__ JobResult generic_run(const GenericArgs& a, const BandJobDev& J, int3
         hc[k] = kNeg32;
         fc[k] = kNeg32;
     }
-    const uint8_t* q = a.qres + J.q_off;
-    const uint8_t* s = a.sres + J.s_off;
-    const int8_t* bias = a.qbias ? a.qbias + J.q_off : nullptr;
+    const uint8_t* q = (J.swapped ? a.sres : a.qres) + J.q_off;
+    const uint8_t* s = (J.swapped ? a.qres : a.sres) + J.s_off;
+    const int8_t* bias = a.qbias && !J.swapped ? a.qbias + J.q_off : nullptr;
     const int64_t m = J.m, n = J.n, lo = J.lo;
     const int32_t go = a.go, ge = a.ge;
     int32_t best = 0;

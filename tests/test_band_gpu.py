@@ -280,6 +280,37 @@ def test_region_workload_sample(ctx):
     check_scores_only(ctx, d["qres"], d["qoff"], d["sres"], d["soff"], d["jobs"], sch, qb, want)
 
 
+@pytest.mark.parametrize("bias", [False, True])
+def test_region_workload_at_scale(ctx, bias):
+    """The benchmark's batch at its full size (10^6 regions, 1000 per query):
+    the scores-only and end-cell builds agree on every job in two launches
+    each, and a 4000-job sample equals the oracle. Small batches never fill
+    the chip, so some timing-dependent faults only show at this size."""
+    d = workload.band_regions(seed=1, n=1_000_000, per_query=1000)
+    sch = S.Scheme(BLOSUM62, 11, 1)
+    qb = np.asarray((np.arange(len(d["qres"])) % 11) - 5, np.int8) if bias else None
+    runs = []
+    for ends in (True, False, True, False):
+        b = ctx.band_batch(d["qres"], d["qoff"], d["sres"], d["soff"], d["jobs"], sch, qbias=qb, ends=ends)
+        try:
+            b.run()
+            b.wait()
+            runs.append(tuple(np.asarray(x).copy() for x in b.read()))
+        finally:
+            b.close()
+    for k in range(1, 4):
+        bad = np.nonzero(runs[k][0] != runs[0][0])[0]
+        assert len(bad) == 0, f"launch {k}: {len(bad)} scores differ, jobs {bad[:10].tolist()}"
+    for f in range(1, 3):
+        assert np.array_equal(runs[2][f], runs[0][f])
+    idx = np.sort(np.random.default_rng(11).choice(len(d["jobs"]), 4000, replace=False))
+    want = O.banded_many(d["qres"], d["qoff"], d["sres"], d["soff"], d["jobs"][idx], BLOSUM62, 11, 1, qb)
+    for f, name in enumerate(("score", "q_end", "s_end", "cells")):
+        g = runs[0][f][idx].astype(np.int64)
+        bad = np.nonzero(g != np.asarray(want[f]).astype(np.int64))[0]
+        assert len(bad) == 0, f"{name} differs at sampled jobs {idx[bad[:10]].tolist()}"
+
+
 def test_score_windows_backend(ctx):
     # ScoreBackend contract (gpu/mod.rs:9-31) and doctor::check_gpu's shape
     # (doctor.rs:440-476): qlen 250, bias (i%5)-2, lengths 1+(13i)%700

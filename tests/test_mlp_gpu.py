@@ -49,6 +49,22 @@ def test_forward_matches_reference(ctx):
     m.close()
 
 
+def test_forward_deterministic_at_scale(ctx):
+    """4 M rows (every CU loops over many groups) through the forward kernel
+    three times: bit-identical logits each time, and a sample within the
+    reference tolerance."""
+    p = rand_params(4)
+    m = P.Mlp(ctx, p)
+    x = np.random.default_rng(5).normal(0, 1, (4_000_000, 64)).astype(np.float32)
+    first = m.forward(x)
+    for _ in range(2):
+        assert np.array_equal(m.forward(x), first)
+    idx = np.random.default_rng(6).choice(len(x), 20000, replace=False)
+    want, _ = mlp_ref.forward(p, x[idx])
+    assert np.all(np.abs(first[idx] - want) <= 2e-2 + 1e-2 * np.abs(want))
+    m.close()
+
+
 def test_init_and_params_roundtrip(ctx):
     m = P.Mlp(ctx, seed=7)
     p = m.params()
