@@ -21,7 +21,9 @@ The printed JSON line also carries
   cpu_baseline  the AVX2 restatement of the reference's score_banded_many
                 (oracle/cpu_avx2.c) on all host threads over a bounded sample
                 of whole queries of the same batch;
-  extra.xdrop   X-drop extension of 4 seed hits per region (steps/s);
+  extra.xdrop   X-drop extension of 4 seed hits per region (steps/s), with
+                the AVX2 restatement of xdrop_avx2 (oracle/cpu_avx2.c) timed
+                on all host threads over the first 2 M hits beside it;
   extra.traceback  banded traceback of the best-scoring regions, stopping at
                 their exact score (alignments/s, the oracle on one core), and
                 the engine's widening loop over them as hits (wall time);
@@ -461,8 +463,27 @@ def main():
               "gsteps_per_s": round(xsteps / (xms * 1e-3) / 1e9, 2),
               "hits_per_s": round(len(hits) / (xms * 1e-3), 1), "parity_sample": int(len(xi)),
               "parity_mismatches": xbad}
+        if not args.no_cpu:
+            # CPU baseline: the AVX2 restatement of xdrop_avx2 on every host
+            # thread over the batch's first hits (a bounded sample), checked
+            # equal to the GPU's outputs on that sample.
+            nth = os.cpu_count() or 1
+            ncpu = min(len(hits), 2_000_000)
+            t0 = time.time()
+            cs, clo, chi, cst = O.xdrop_many_avx2(d["qres"], d["qoff"], d["sres"], d["soff"], hits[:ncpu], BLOSUM62,
+                                                  20, nthreads=nth)
+            cpu_s = time.time() - t0
+            cbad = int(np.sum((cs != xs[:ncpu]) | (clo != xlo[:ncpu]) | (chi != xhi[:ncpu]) | (cst != xst[:ncpu])))
+            xd["cpu_baseline"] = {"kind": "port", "impl": "AVX2 restatement of xdrop_avx2 (scan.rs:934-1031)",
+                                  "hits": ncpu, "seconds": round(cpu_s, 3), "threads_used": nth,
+                                  "gsteps_per_s": round(float(cst.sum()) / cpu_s / 1e9, 3),
+                                  "hits_per_s": round(ncpu / cpu_s, 1), "mismatches_vs_gpu": cbad,
+                                  "host": cpu_info(), "sample": f"first {ncpu} hits of the batch"}
         print(f"# xdrop {len(hits)} hits: {xms:.3f} ms, {xsteps} steps, "
-              f"{xd['gsteps_per_s']} G steps/s, mismatches {xbad}/{len(xi)}", file=sys.stderr, flush=True)
+              f"{xd['gsteps_per_s']} G steps/s, mismatches {xbad}/{len(xi)}"
+              + (f"; CPU AVX2 {xd['cpu_baseline']['gsteps_per_s']} G steps/s on {xd['cpu_baseline']['threads_used']}"
+                 f" threads ({xd['cpu_baseline']['mismatches_vs_gpu']} differ)" if "cpu_baseline" in xd else ""),
+              file=sys.stderr, flush=True)
 
     # Secondary: exhaustive full-matrix SW, queries x subjects drawn like the
     # exhaustive workload (lognormal median 250, sigma 0.6, clamped 30..3000).

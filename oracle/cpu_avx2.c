@@ -729,3 +729,171 @@ void cpu_score_many_avx2(const uint8_t* qres, const uint64_t* qoff, uint32_t nq,
     free(th);
     free(D.order);
 }
+
+/* ---------------- ungapped X-drop, 16 steps per vector ------------------- */
+/*
+ * scan.rs:934-1031 (xdrop_dir_avx2 / xdrop_avx2), restated: each direction
+ * runs in blocks of 16 steps; a block's step scores (scalar table reads plus
+ * the query bias) are summed and maximised with in-register prefix scans
+ * relative to the score at the block start, and the first step whose sum
+ * falls more than x below the running best ends the extension. Outputs equal
+ * or_xdrop's (score, q_lo, q_hi, steps); hits whose scores could leave the
+ * int16 lanes (|substitution score| > 1000 or x > 16000) take or_xdrop.
+ */
+
+/* 16-lane inclusive prefix sum / prefix max of int16 (in each 128-bit half,
+ * then the low half's last lane carried into the high half). */
+static inline __m256i carry_lo_half(__m256i v) {
+    const __m256i b7 = _mm256_setr_epi8(14, 15, 14, 15, 14, 15, 14, 15, 14, 15, 14, 15, 14, 15, 14, 15, 14, 15, 14,
+                                        15, 14, 15, 14, 15, 14, 15, 14, 15, 14, 15, 14, 15);
+    return _mm256_shuffle_epi8(_mm256_permute2x128_si256(v, v, 0x08), b7);
+}
+static inline __m256i psum16(__m256i r) {
+    r = _mm256_add_epi16(r, _mm256_slli_si256(r, 2));
+    r = _mm256_add_epi16(r, _mm256_slli_si256(r, 4));
+    r = _mm256_add_epi16(r, _mm256_slli_si256(r, 8));
+    return _mm256_add_epi16(r, carry_lo_half(r));
+}
+static inline __m256i pmax16(__m256i m, __m256i floor) {
+    m = _mm256_max_epi16(m, _mm256_slli_si256(m, 2));
+    m = _mm256_max_epi16(m, _mm256_slli_si256(m, 4));
+    m = _mm256_max_epi16(m, _mm256_slli_si256(m, 8));
+    m = _mm256_max_epi16(m, carry_lo_half(m));
+    return _mm256_max_epi16(m, floor);
+}
+
+/* One direction: n steps available; step t pairs q[d*t + o], s[d*t + o] with
+ * (d, o) = (1, 0) rightwards and (-1, -1) leftwards. Returns the best sum,
+ * the steps to it and the steps taken. */
+static void xdrop_dir(const int16_t* T, const uint8_t* q, const int8_t* bias, const uint8_t* s, int64_t n, int left,
+                      int32_t x, int32_t* best_out, int64_t* end_out, int64_t* taken_out) {
+    int32_t cur = 0, best = 0;
+    int64_t end = 0, t0 = 0;
+    const __m256i xv = _mm256_set1_epi16((short)x);
+    while (t0 < n) {
+        int16_t sc[16] __attribute__((aligned(32)));
+        const int64_t vk = n - t0 < 16 ? n - t0 : 16;
+        for (int k = 0; k < 16; ++k) {
+            if (k < vk) {
+                const int64_t i = left ? -1 - (t0 + k) : t0 + k;
+                sc[k] = (int16_t)(T[code(q[i]) * 32 + code(s[i])] + (bias ? bias[i] : 0));
+            } else {
+                sc[k] = 0;
+            }
+        }
+        const __m256i r = psum16(_mm256_load_si256((const __m256i*)sc));
+        const int32_t base = best - cur;
+        const __m256i m = pmax16(r, _mm256_set1_epi16((short)base));
+        const uint32_t valid = vk == 16 ? 0xFFFFFFFFu : ((1u << (2 * vk)) - 1u);
+        const uint32_t stop = (uint32_t)_mm256_movemask_epi8(_mm256_cmpgt_epi16(_mm256_sub_epi16(m, xv), r)) & valid;
+        const int last = stop ? (int)(__builtin_ctz(stop) / 2) : (int)vk - 1;
+        int16_t ma[16], ra[16];
+        _mm256_storeu_si256((__m256i*)ma, m);
+        _mm256_storeu_si256((__m256i*)ra, r);
+        const int32_t bt = ma[last];
+        if (bt > base) {
+            const uint32_t eq = (uint32_t)_mm256_movemask_epi8(_mm256_cmpeq_epi16(r, _mm256_set1_epi16((short)bt)));
+            end = t0 + __builtin_ctz(eq) / 2 + 1;
+            best = cur + bt;
+        }
+        if (stop) {
+            t0 += last + 1;
+            break;
+        }
+        cur += ra[vk - 1];
+        t0 += vk;
+    }
+    *best_out = best;
+    *end_out = end;
+    *taken_out = t0;
+}
+
+typedef struct {
+    const uint8_t *qres, *sres;
+    const uint64_t *qoff, *soff;
+    const int8_t* bias;
+    const uint32_t* hits;
+    uint64_t nhits;
+    const int32_t* mat;
+    int16_t T[24 * 32];
+    int wide;
+    int32_t x;
+    int32_t* score;
+    uint32_t *lo, *hi;
+    uint64_t* steps;
+    int64_t next;
+} xdrop_drv;
+
+static void* xdrop_worker(void* p) {
+    xdrop_drv* D = (xdrop_drv*)p;
+    const int64_t chunk = 4096;
+    for (;;) {
+        const int64_t h0 = __atomic_fetch_add(&D->next, chunk, __ATOMIC_RELAXED);
+        if (h0 >= (int64_t)D->nhits) break;
+        const int64_t h1 = h0 + chunk < (int64_t)D->nhits ? h0 + chunk : (int64_t)D->nhits;
+        for (int64_t h = h0; h < h1; ++h) {
+            const uint32_t* H = D->hits + 4 * h;
+            const uint8_t* q = D->qres + D->qoff[H[0]];
+            const int64_t m = (int64_t)(D->qoff[H[0] + 1] - D->qoff[H[0]]);
+            const uint8_t* s = D->sres + D->soff[H[1]];
+            const int64_t n = (int64_t)(D->soff[H[1] + 1] - D->soff[H[1]]);
+            const int8_t* b = D->bias ? D->bias + D->qoff[H[0]] : NULL;
+            const int64_t qp = H[2], sp = H[3];
+            int32_t sc;
+            int64_t lo, hi;
+            uint64_t st;
+            if (D->wide) {
+                or_xdrop(q, m, b, s, n, D->mat, qp, sp, D->x, &sc, &lo, &hi, &st);
+            } else {
+                const int64_t nr = (m - qp) < (n - sp) ? (m - qp) : (n - sp);
+                const int64_t nl = qp < sp ? qp : sp;
+                int32_t br, bl;
+                int64_t er, el, tr, tl;
+                xdrop_dir(D->T, q + qp, b ? b + qp : NULL, s + sp, nr, 0, D->x, &br, &er, &tr);
+                xdrop_dir(D->T, q + qp, b ? b + qp : NULL, s + sp, nl, 1, D->x, &bl, &el, &tl);
+                sc = br + bl;
+                lo = qp - el;
+                hi = qp + er;
+                st = (uint64_t)(tr + tl);
+            }
+            D->score[h] = sc;
+            D->lo[h] = (uint32_t)lo;
+            D->hi[h] = (uint32_t)hi;
+            D->steps[h] = st;
+        }
+    }
+    return NULL;
+}
+
+/* X-drop of every hit (query, subject, q_pos, s_pos) over nthreads threads. */
+void cpu_xdrop_avx2(const uint8_t* qres, const uint64_t* qoff, const int8_t* bias, const uint8_t* sres,
+                    const uint64_t* soff, const uint32_t* hits, uint64_t nhits, const int32_t* mat, int32_t x,
+                    int32_t* score, uint32_t* lo, uint32_t* hi, uint64_t* steps, int nthreads) {
+    xdrop_drv* D = (xdrop_drv*)calloc(1, sizeof(xdrop_drv));
+    D->qres = qres;
+    D->qoff = qoff;
+    D->bias = bias;
+    D->sres = sres;
+    D->soff = soff;
+    D->hits = hits;
+    D->nhits = nhits;
+    D->mat = mat;
+    D->x = x;
+    D->score = score;
+    D->lo = lo;
+    D->hi = hi;
+    D->steps = steps;
+    D->wide = x > 16000 || x < 0;
+    for (int a = 0; a < 24; ++a)
+        for (int c = 0; c < 32; ++c) {
+            const int32_t v = mat[a * 24 + (c < 24 ? c : 22)];
+            if (v < -1000 || v > 1000) D->wide = 1;
+            D->T[a * 32 + c] = (int16_t)v;
+        }
+    if (nthreads < 1) nthreads = 1;
+    pthread_t* th = (pthread_t*)malloc(sizeof(pthread_t) * nthreads);
+    for (int t = 0; t < nthreads; ++t) pthread_create(&th[t], NULL, xdrop_worker, D);
+    for (int t = 0; t < nthreads; ++t) pthread_join(th[t], NULL);
+    free(th);
+    free(D);
+}

@@ -139,6 +139,9 @@ def avx2_lib():
         L.cpu_score_many_avx2.argtypes = [u8p, u64p, C.c_uint32, i8p, u8p, u64p, C.c_uint32, i32p, C.c_int32,
                                           C.c_int32, i32p, C.c_int]
         L.cpu_score_many_avx2.restype = None
+        L.cpu_xdrop_avx2.argtypes = [u8p, u64p, i8p, u8p, u64p, u32p, C.c_uint64, i32p, C.c_int32, i32p, u32p,
+                                     u32p, u64p, C.c_int]
+        L.cpu_xdrop_avx2.restype = None
         _avx2 = L
     return _avx2
 
@@ -180,6 +183,27 @@ def score_many_avx2(qres, qoff, sres, soff, matrix, go, ge, qbias=None, nthreads
                                    _p(sres, C.c_uint8), _p(soff, C.c_uint64), ns, _p(_mat(matrix), C.c_int32), go, ge,
                                    _p(out, C.c_int32), nthreads)
     return out[:nq, :ns]
+
+
+def xdrop_many_avx2(qres, qoff, sres, soff, hits, matrix, x, qbias=None, nthreads=8):
+    """X-drop of every hit row (query, subject, q_pos, s_pos) by the AVX2
+    restatement of xdrop_avx2 (scan.rs:934-1031), hits spread over nthreads
+    threads: (score, q_lo, q_hi, steps) arrays, equal to or_xdrop's."""
+    qres, sres = _u8(qres), _u8(sres)
+    qoff = np.ascontiguousarray(qoff, np.uint64)
+    soff = np.ascontiguousarray(soff, np.uint64)
+    hits = np.ascontiguousarray(hits, np.uint32).reshape(-1, 4)
+    n = len(hits)
+    score = np.zeros(max(n, 1), np.int32)
+    lo = np.zeros(max(n, 1), np.uint32)
+    hi = np.zeros(max(n, 1), np.uint32)
+    steps = np.zeros(max(n, 1), np.uint64)
+    b = _bias(qbias)
+    avx2_lib().cpu_xdrop_avx2(_p(qres, C.c_uint8), _p(qoff, C.c_uint64), _p(b, C.c_int8), _p(sres, C.c_uint8),
+                              _p(soff, C.c_uint64), _p(hits, C.c_uint32), n, _p(_mat(matrix), C.c_int32), x,
+                              _p(score, C.c_int32), _p(lo, C.c_uint32), _p(hi, C.c_uint32),
+                              _p(steps, C.c_uint64), nthreads)
+    return score[:n], lo[:n], hi[:n], steps[:n]
 
 
 def xdrop(q, s, bias, matrix, qp, sp, x):

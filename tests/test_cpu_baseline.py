@@ -94,3 +94,37 @@ def test_score_many_avx2_equals_sw_reference():
                     continue
                 assert got[i, j] == O.sw_reference(q, s, BLOSUM62, b, 11, 1)[0], (i, j)
     assert got[len(qs) - 1, len(ss) - 2] == 44000 + int(bias[-4000:].sum())
+
+
+@pytest.mark.parametrize("x", [1, 10, 20, 35, 300, 20000])
+def test_xdrop_avx2_equals_oracle(x):
+    """The AVX2 X-drop restatement (the X-drop leg's CPU baseline) against
+    or_xdrop: related and random pairs, seeds at both ends of the sequences
+    and in the middle, query bias of -128..127, blocks of 16 steps ending
+    exactly at a sequence end; x = 20000 takes the scalar route."""
+    r = Lcg(900 + x)
+    qs, ss, hits = [], [], []
+    for k in range(60):
+        q = rand_seq(r, 1 + r.next(120))
+        s = mutate(r, q) if k % 2 == 0 else rand_seq(r, 1 + r.next(120))
+        if len(s) == 0:
+            s = rand_seq(r, 1)
+        if k % 5 == 0:
+            s = np.concatenate([q, q])[:16 * (1 + r.next(4))] if len(q) else s
+            q = s.copy()
+        qs.append(q)
+        ss.append(s)
+        for _ in range(6):
+            hits.append((k, k, r.next(len(q)), r.next(len(s))))
+        hits.append((k, k, 0, 0))
+        hits.append((k, k, len(q) - 1, len(s) - 1))
+    qres, qoff = flat(qs)
+    sres, soff = flat(ss)
+    hits = np.array(hits, np.uint32)
+    bias = np.array([r.next(256) - 128 for _ in range(len(qres))], np.int8)
+    for mat, b in ((BLOSUM62, None), (PAM30, bias)):
+        got = O.xdrop_many_avx2(qres, qoff, sres, soff, hits, mat, x, b, nthreads=3)
+        for h, (qi, si, qp, sp) in enumerate(hits.tolist()):
+            bb = None if b is None else b[int(qoff[qi]):int(qoff[qi + 1])]
+            want = O.xdrop(qs[qi], ss[si], bb, mat, qp, sp, x)
+            assert tuple(int(a[h]) for a in got) == want, (x, h)
