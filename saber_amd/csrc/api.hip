@@ -1301,6 +1301,10 @@ int band_align_impl(saber_hip_ctx* c, const saber_hip_seqset* queries, const sab
             if (x < kAlignLaneClasses) cls[j] = (uint8_t)x;
         }
     }
+    // the lane kernel's 32-bit end-cell key must cover every row
+    for (const uint32_t j : order)
+        if (cls[j] < kAlignLaneClasses && !align_lane_key_fits((uint64_t)dj[j].rows, cls[j]))
+            cls[j] = (uint8_t)kAlignLaneClasses;
     std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
         return cls[x] != cls[y] ? cls[x] < cls[y] : dj[x].rows > dj[y].rows;
     });

@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include "../../include/saber_hip.h"
+#include "align_limits.h"
 
 namespace saber {
 
@@ -213,16 +214,6 @@ struct AlignArgs {
 
 // One thread per job, rows in the work buffer (band widths over 16 * 64).
 hipError_t launch_band_align(const AlignArgs& a, hipStream_t s);
-// Lane classes: x < 7: 1 << x lanes of 17 cells (band widths up to
-// 17 << x: the widths 2^k + 1 of half-width bands fit); x >= 7:
-// 4 << (x - 7) lanes of 5 cells (widths up to 20 << (x - 7)), for the jobs
-// that would otherwise run alone at the end of a launch: a quarter of the
-// cells per lane makes each row step about a quarter as long, at a somewhat
-// higher cost per cell.
-constexpr int kAlignLaneClasses = 12, kAlignWideClasses = 7;
-constexpr uint64_t kAlignMinLongSteps = 64;   // jobs of fewer row steps keep 17 cells a lane
-__host__ __device__ constexpr int align_class_lanes(int x) { return x < kAlignWideClasses ? 1 << x : 4 << (x - kAlignWideClasses); }
-__host__ __device__ constexpr int align_class_cells(int x) { return x < kAlignWideClasses ? 17 : 5; }
 // Lane classes of one launch: the launch's slots and blocks of each class, as
 // prefix offsets, and the block order: launch block b runs block
 // order[b] & 0xFFFFFF of class order[b] >> 24.
