@@ -170,9 +170,11 @@ int saber_hip_band_batch_time(saber_hip_band_batch* b, int warmup, int iters, do
  * length x clipped band width bytes) exceeds max_dir_bytes (Err). Query
  * coordinates and window coordinates (subject position - win_lo) are
  * half-open; the run-length operations ('M', 'I', 'D' with lengths, in
- * alignment order) of job j are written at out[j].op_off, which is the sum of
- * query + window lengths of the jobs before it: op_codes / op_lens need that
- * sum over all jobs (op_capacity). Runs on the context stream; blocking.
+ * alignment order) of job j are the out[j].nops entries at out[j].op_off:
+ * the jobs' operations are packed one after another (in no particular job
+ * order), and op_codes / op_lens need room for the sum over all jobs of
+ * query + window lengths (op_capacity), the most they can take. Runs on the
+ * context stream; blocking.
  * Jobs run in launches whose device scratch (direction bytes, work rows,
  * operation bytes) fits a budget: 2048 MiB, or the environment variable
  * SABER_HIP_ALIGN_SCRATCH_MB (read at each call). */
@@ -201,9 +203,10 @@ int saber_hip_band_align(saber_hip_ctx* ctx, const saber_hip_seqset* queries, co
  * diagonal up to max(end score, want). The attempts of all jobs in a round
  * go to one saber_hip_band_align call. out[j] is the alignment (status 0)
  * or none (1); exact[j] = 1 when its score reaches want. Operations as for
- * saber_hip_band_align (job j's at the sum of query + window lengths of the
- * jobs before it). counters may be NULL; else it receives the engine's
- * counters summed over the batch. Runs on the context stream; blocking. */
+ * saber_hip_band_align (packed, job j's out[j].nops entries at
+ * out[j].op_off; capacity the sum of query + window lengths). counters may
+ * be NULL; else it receives the engine's counters summed over the batch.
+ * Runs on the context stream; blocking. */
 typedef struct saber_hip_traceback_job {
     uint32_t query, subject;
     uint32_t win_lo, win_hi;     /* the window: subject residues [win_lo, win_hi) */

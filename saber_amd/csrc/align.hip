@@ -495,6 +495,33 @@ __global__ __launch_bounds__(256) void band_align_lanes_kernel(AlignArgs a, Alig
     }
 }
 
+// The operations of one launch packed densely for the copy to the host:
+// slot x's nops entries move from op_off[x] to dst_off[x] (the exclusive
+// prefix of nops over the launch's slots that aligned). One wave per slot.
+__global__ __launch_bounds__(256) void gather_ops_kernel(uint32_t n, const uint32_t* ids, const AlignRecord* recs,
+                                                         const uint64_t* op_off, const uint64_t* dst_off,
+                                                         const uint8_t* codes, const uint32_t* lens,
+                                                         uint8_t* dcodes, uint32_t* dlens) {
+    const uint32_t x = blockIdx.x * 4 + threadIdx.x / 64;
+    if (x >= n) return;
+    const AlignRecord& r = recs[ids[x]];
+    if (r.status != 0) return;
+    const uint64_t src = op_off[x], dst = dst_off[x];
+    for (uint32_t k = threadIdx.x % 64; k < r.nops; k += 64) {
+        dcodes[dst + k] = codes[src + k];
+        dlens[dst + k] = lens[src + k];
+    }
+}
+
+hipError_t launch_gather_ops(uint32_t n, const uint32_t* ids, const AlignRecord* recs, const uint64_t* op_off,
+                             const uint64_t* dst_off, const uint8_t* codes, const uint32_t* lens, uint8_t* dcodes,
+                             uint32_t* dlens, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(gather_ops_kernel, dim3((n + 3) / 4), dim3(256), 0, s, n, ids, recs, op_off, dst_off, codes,
+                       lens, dcodes, dlens);
+    return hipGetLastError();
+}
+
 hipError_t launch_band_align(const AlignArgs& a, hipStream_t s) {
     if (a.n == 0) return hipSuccess;
     hipLaunchKernelGGL(band_align_kernel, dim3((a.n + kWideThreads - 1) / kWideThreads), dim3(kWideThreads), 0, s, a);

@@ -1152,10 +1152,21 @@ struct AlignSeqs {
     DevBuf<int8_t> bias;
     bool ready = false;
     DevBuf<int32_t> work;
-    DevBuf<uint8_t> dir, codes;
-    DevBuf<uint32_t> lens;
+    DevBuf<uint8_t> dir, codes, dcodes;
+    DevBuf<uint32_t> lens, dlens;
+    DevBuf<uint64_t> dst_off;
     HostBuf<uint8_t> hcodes;
     HostBuf<uint32_t> hlens;
+    // per-call job data, refilled by every call of a caller's series
+    DevBuf<BandJobDev>& djobs = sq.djobs;
+    DevBuf<int32_t>& mat = sq.mat;
+    DevBuf<int32_t>& stops = sq.stops;
+    DevBuf<AlignRecord>& recs = sq.recs;
+    DevBuf<uint32_t>& dids = sq.dids;
+    DevBuf<uint32_t>& dorder = sq.dorder;
+    DevBuf<uint64_t>& dwoff = sq.dwoff;
+    DevBuf<uint64_t>& ddoff = sq.ddoff;
+    DevBuf<uint64_t>& dooff = sq.dooff;
 };
 
 // saber_hip_band_align; iters > 0 also times iters repetitions of every
@@ -1181,6 +1192,15 @@ int band_align_impl(saber_hip_ctx* c, const saber_hip_seqset* queries, const sab
     const uint64_t* soffs = subjects ? subjects->offsets : c->db_off.data();
     const uint32_t nsub = subjects ? subjects->count : (c->db_off.empty() ? 0 : (uint32_t)c->db_off.size() - 1);
     if (!subjects && !c->db.p) return fail(SABER_HIP_ESTATE, "no database uploaded");
+    static const bool trace = std::getenv("SABER_HIP_TRACE_HOST") != nullptr;
+    auto tp = std::chrono::steady_clock::now();
+    auto lap = [&](const char* what) {
+        if (!trace) return;
+        (void)hipStreamSynchronize(st);
+        const auto t1 = std::chrono::steady_clock::now();
+        std::fprintf(stderr, "#   align %s %.2f ms\n", what, std::chrono::duration<double, std::milli>(t1 - tp).count());
+        tp = t1;
+    };
     // Jobs: clipped bands (Band::clip, banded.rs:45-47), the reference's
     // memory check on m * w direction bytes, output operation offsets.
     std::vector<BandJobDev> dj(njobs);
@@ -1227,9 +1247,10 @@ int band_align_impl(saber_hip_ctx* c, const saber_hip_seqset* queries, const sab
     AlignSeqs& sq = seqs ? *seqs : local;
     DevBuf<uint8_t>& q = sq.q;
     DevBuf<int8_t>& bias = sq.bias;
-    DevBuf<BandJobDev> djobs;
-    DevBuf<int32_t> mat, stops;
-    DevBuf<AlignRecord> recs;
+    DevBuf<BandJobDev>& djobs = sq.djobs;
+    DevBuf<int32_t>& mat = sq.mat;
+    DevBuf<int32_t>& stops = sq.stops;
+    DevBuf<AlignRecord>& recs = sq.recs;
     if (!sq.ready) {
         const uint64_t nqres = queries->offsets[queries->count];
         if (int r = upload_codes(q, queries->residues, nqres, st)) return r;
@@ -1240,11 +1261,12 @@ int band_align_impl(saber_hip_ctx* c, const saber_hip_seqset* queries, const sab
         sq.ready = true;
     }
     const uint8_t* sres = subjects ? sq.sbuf.p + kGuard : c->db.p + kGuard;
-    if (int r = upload(djobs, dj.data(), njobs, st)) return r;
-    if (int r = upload(mat, scheme->matrix, 576, st)) return r;
+    lap("jobs + sequences");
+    if (int r = upload_keep(djobs, dj.data(), njobs, st)) return r;
+    if (int r = upload_keep(mat, scheme->matrix, 576, st)) return r;
     if (stop_at)
-        if (int r = upload(stops, stop_at, njobs, st)) return r;
-    HIP_TRY(recs.alloc(njobs));
+        if (int r = upload_keep(stops, stop_at, njobs, st)) return r;
+    HIP_TRY(recs.reserve(njobs));
     // Launches of jobs whose scratch (work rows, direction bytes, operation
     // bytes) fits a budget: the lane classes (band widths up to 64 x 17) in
     // one kind of launch, wider jobs on one thread each with their rows in
@@ -1257,10 +1279,13 @@ int band_align_impl(saber_hip_ctx* c, const saber_hip_seqset* queries, const sab
     std::vector<AlignRecord> hrec(njobs);
     std::vector<uint32_t> ids;
     std::vector<uint64_t> woff, doff, ooff;
-    DevBuf<uint32_t> dids, dorder;
+    DevBuf<uint32_t>& dids = sq.dids;
+    DevBuf<uint32_t>& dorder = sq.dorder;
     std::vector<uint32_t> border;
     std::vector<uint64_t> bkey;
-    DevBuf<uint64_t> dwoff, ddoff, dooff;
+    DevBuf<uint64_t>& dwoff = sq.dwoff;
+    DevBuf<uint64_t>& ddoff = sq.ddoff;
+    DevBuf<uint64_t>& dooff = sq.dooff;
     DevBuf<int32_t>& work = sq.work;
     DevBuf<uint8_t>& dir = sq.dir;
     DevBuf<uint8_t>& codes = sq.codes;
@@ -1309,6 +1334,7 @@ int band_align_impl(saber_hip_ctx* c, const saber_hip_seqset* queries, const sab
         return cls[x] != cls[y] ? cls[x] < cls[y] : dj[x].rows > dj[y].rows;
     });
     size_t pos = 0;
+    uint64_t ops_out = 0;   // operations written to the caller's buffers
     {
         while (pos < order.size()) {
             ids.clear();
@@ -1371,13 +1397,13 @@ int band_align_impl(saber_hip_ctx* c, const saber_hip_seqset* queries, const sab
                 std::vector<uint32_t> sorted(bidx.size());
                 for (size_t b = 0; b < bidx.size(); ++b) sorted[b] = border[bidx[b]];
                 border.swap(sorted);
-                if (int r = upload(dorder, border.data(), border.size(), st)) return r;
+                if (int r = upload_keep(dorder, border.data(), border.size(), st)) return r;
                 lc.order = dorder.p;
             }
-            if (int r = upload(dids, ids.data(), ids.size(), st)) return r;
-            if (int r = upload(dwoff, woff.data(), woff.size(), st)) return r;
-            if (int r = upload(ddoff, doff.data(), doff.size(), st)) return r;
-            if (int r = upload(dooff, ooff.data(), ooff.size(), st)) return r;
+            if (int r = upload_keep(dids, ids.data(), ids.size(), st)) return r;
+            if (int r = upload_keep(dwoff, woff.data(), woff.size(), st)) return r;
+            if (int r = upload_keep(ddoff, doff.data(), doff.size(), st)) return r;
+            if (int r = upload_keep(dooff, ooff.data(), ooff.size(), st)) return r;
             if (wt && work.n < wt / 4) HIP_TRY(work.alloc(wt / 4));
             if (dir.n < dt) HIP_TRY(dir.alloc(dt));
             if (codes.n < ot) {
@@ -1403,8 +1429,10 @@ int band_align_impl(saber_hip_ctx* c, const saber_hip_seqset* queries, const sab
             a.op_codes = codes.p;
             a.op_lens = lens.p;
             a.out = recs.p;
+            lap("plan + uploads");
             auto launch = [&]() { return lanes ? launch_band_align_lanes(a, lc, st) : launch_band_align(a, st); };
             HIP_TRY(launch());
+            lap("kernel");
             if (iters > 0 && kernel_ms) {
                 hipEvent_t e0, e1;
                 HIP_TRY(hipEventCreate(&e0));
@@ -1419,19 +1447,41 @@ int band_align_impl(saber_hip_ctx* c, const saber_hip_seqset* queries, const sab
                 (void)hipEventDestroy(e1);
                 *kernel_ms += ms / iters;
             }
-            uint8_t* hc = hcodes.get(ot);
-            uint32_t* hl = hlens.get(ot);
-            HIP_TRY(hipMemcpyAsync(hc, codes.p, ot, hipMemcpyDeviceToHost, st));
-            HIP_TRY(hipMemcpyAsync(hl, lens.p, 4 * ot, hipMemcpyDeviceToHost, st));
+            // Records first; then only the operations the alignments use,
+            // packed densely on the device (the slots hold m + n entries
+            // each, mostly unused).
             HIP_TRY(hipMemcpyAsync(hrec.data(), recs.p, sizeof(AlignRecord) * njobs, hipMemcpyDeviceToHost, st));
             HIP_TRY(hipStreamSynchronize(st));
+            std::vector<uint64_t> dense(ids.size());
+            uint64_t used = 0;
+            for (size_t x = 0; x < ids.size(); ++x) {
+                const AlignRecord& r = hrec[ids[x]];
+                dense[x] = used;
+                used += r.status == 0 ? r.nops : 0;
+            }
+            uint8_t* hc = hcodes.get(std::max<uint64_t>(used, 1));
+            uint32_t* hl = hlens.get(std::max<uint64_t>(used, 1));
+            if (used) {
+                if (int r = upload_keep(sq.dst_off, dense.data(), dense.size(), st)) return r;
+                if (sq.dcodes.n < used) {
+                    HIP_TRY(sq.dcodes.alloc(used));
+                    HIP_TRY(sq.dlens.alloc(used));
+                }
+                HIP_TRY(launch_gather_ops((uint32_t)ids.size(), dids.p, recs.p, dooff.p, sq.dst_off.p, codes.p,
+                                          lens.p, sq.dcodes.p, sq.dlens.p, st));
+                HIP_TRY(hipMemcpyAsync(hc, sq.dcodes.p, used, hipMemcpyDeviceToHost, st));
+                HIP_TRY(hipMemcpyAsync(hl, sq.dlens.p, 4 * used, hipMemcpyDeviceToHost, st));
+                HIP_TRY(hipStreamSynchronize(st));
+            }
+            lap("copy back");
+            // packed into the caller's buffers, launch after launch
+            if (used) {
+                std::memcpy(op_codes + ops_out, hc, used);
+                std::memcpy(op_lens + ops_out, hl, 4ull * used);
+            }
             for (size_t x = 0; x < ids.size(); ++x) {
                 const uint32_t id = ids[x];
                 const AlignRecord& r = hrec[id];
-                if (r.status == 0 && r.nops) {
-                    std::memcpy(op_codes + op_off[id], hc + ooff[x], r.nops);
-                    std::memcpy(op_lens + op_off[id], hl + ooff[x], 4ull * r.nops);
-                }
                 saber_hip_alignment& o = out[id];
                 o = saber_hip_alignment{};
                 o.status = r.status;
@@ -1441,9 +1491,10 @@ int band_align_impl(saber_hip_ctx* c, const saber_hip_seqset* queries, const sab
                 o.s_start = r.s_start;
                 o.s_end = r.s_end;
                 o.nops = r.status == 0 ? r.nops : 0;
-                o.op_off = op_off[id];
+                o.op_off = ops_out + dense[x];
                 o.cells = r.cells;
             }
+            ops_out += used;
         }
     }
     // jobs that did not run: capped (status 2) or without band cells (status 1)
@@ -1452,7 +1503,7 @@ int band_align_impl(saber_hip_ctx* c, const saber_hip_seqset* queries, const sab
         if (capped[j] || d.w <= 0 || d.rows <= 0) {
             out[j] = saber_hip_alignment{};
             out[j].status = capped[j] ? 2 : 1;
-            out[j].op_off = op_off[j];
+            out[j].op_off = ops_out;
         }
     }
     return SABER_HIP_OK;
@@ -1502,15 +1553,25 @@ int saber_hip_traceback_ex(saber_hip_ctx* c, const saber_hip_seqset* queries, co
         if (int r = check_seqset(subjects, "subjects")) return r;
     if (job_counters)
         for (uint64_t k = 0; k < njobs; ++k) job_counters[k] = saber_hip_traceback_counters{};
-    std::vector<uint64_t> slen;   // subject lengths
-    uint64_t gen = 0;             // the database this call reads (subjects == NULL)
+    static const bool trace = std::getenv("SABER_HIP_TRACE_HOST") != nullptr;
+    auto tp = std::chrono::steady_clock::now();
+    auto lap = [&](const char* what) {
+        if (!trace) return;
+        const auto t1 = std::chrono::steady_clock::now();
+        std::fprintf(stderr, "#   traceback %s %.2f ms\n", what, std::chrono::duration<double, std::milli>(t1 - tp).count());
+        tp = t1;
+    };
+    std::vector<uint64_t> slen(njobs);   // each job's subject length
+    uint64_t gen = 0;                    // the database this call reads (subjects == NULL)
     {
         std::lock_guard<std::mutex> g(c->mu);
         if (!subjects && !c->db.p) return fail(SABER_HIP_ESTATE, "no database uploaded");
         const uint64_t* so = subjects ? subjects->offsets : c->db_off.data();
         const uint32_t ns = subjects ? subjects->count : (uint32_t)c->db_off.size() - 1;
-        slen.resize(ns);
-        for (uint32_t k = 0; k < ns; ++k) slen[k] = so[k + 1] - so[k];
+        for (uint64_t k = 0; k < njobs; ++k) {
+            if (jobs[k].subject >= ns) return fail(SABER_HIP_EINVAL, "job subject out of range");
+            slen[k] = so[jobs[k].subject + 1] - so[jobs[k].subject];
+        }
         gen = c->db_gen;
     }
     // A database upload from another thread between two rounds would mix
@@ -1545,8 +1606,7 @@ int saber_hip_traceback_ex(saber_hip_ctx* c, const saber_hip_seqset* queries, co
     for (uint64_t k = 0; k < njobs; ++k) {
         const saber_hip_traceback_job& J = jobs[k];
         if (J.query >= queries->count) return fail(SABER_HIP_EINVAL, "job query out of range");
-        if (J.subject >= slen.size()) return fail(SABER_HIP_EINVAL, "job subject out of range");
-        if (J.win_hi < J.win_lo || J.win_hi > slen[J.subject]) return fail(SABER_HIP_EINVAL, "job window outside its subject");
+        if (J.win_hi < J.win_lo || J.win_hi > slen[k]) return fail(SABER_HIP_EINVAL, "job window outside its subject");
         Tb& t = T[k];
         t.m = (int64_t)(queries->offsets[J.query + 1] - queries->offsets[J.query]);
         t.n = (int64_t)J.win_hi - J.win_lo;
@@ -1574,6 +1634,7 @@ int saber_hip_traceback_ex(saber_hip_ctx* c, const saber_hip_seqset* queries, co
     }
     if (ops_total > op_capacity || (ops_total > 0 && (!op_codes || !op_lens)))
         return fail(SABER_HIP_EINVAL, "operation buffers need the sum over jobs of query + window length entries");
+    lap("set-up");
     saber_hip_traceback_counters cnt{};
     // a band in window diagonals as an int32 job band: values beyond the
     // rectangle move to its edge (one past it for an empty band), which
@@ -1582,15 +1643,18 @@ int saber_hip_traceback_ex(saber_hip_ctx* c, const saber_hip_seqset* queries, co
         o.band_lo = (int32_t)std::min<int64_t>(std::max<int64_t>(lo, -m), n + 1);
         o.band_hi = (int32_t)std::min<int64_t>(std::max<int64_t>(hi, -m - 1), n);
     };
+    // operations of finished jobs, packed in the order the jobs finish
+    uint64_t ops_out = 0;
     auto finish = [&](uint64_t k, const saber_hip_alignment* a, const uint8_t* oc, const uint32_t* ol, bool ex) {
         Tb& t = T[k];
         t.done = true;
         if (!a) return;
         out[k] = *a;
-        out[k].op_off = op_off[k];
+        out[k].op_off = ops_out;
         if (a->nops) {
-            std::memcpy(op_codes + op_off[k], oc, a->nops);
-            std::memcpy(op_lens + op_off[k], ol, 4ull * a->nops);
+            std::memcpy(op_codes + ops_out, oc, a->nops);
+            std::memcpy(op_lens + ops_out, ol, 4ull * a->nops);
+            ops_out += a->nops;
         }
         exact[k] = ex ? 1 : 0;
     };
@@ -1696,9 +1760,11 @@ int saber_hip_traceback_ex(saber_hip_ctx* c, const saber_hip_seqset* queries, co
         res.assign(ids.size(), saber_hip_alignment{});
         uint8_t* rcp = rc.get(std::max<uint64_t>(round_ops, 1));
         uint32_t* rlp = rl.get(std::max<uint64_t>(round_ops, 1));
+        lap("round set-up");
         if (int r = band_align_impl(c, queries, subjects, bj.data(), stops.data(), bj.size(), scheme, max_dir_bytes,
                                     res.data(), rcp, rlp, round_ops, 0, nullptr, &seqs))
             return r;
+        lap("band_align");
         for (size_t x = 0; x < ids.size(); ++x) {
             const uint64_t k = ids[x];
             Tb& t = T[k];
@@ -1709,14 +1775,15 @@ int saber_hip_traceback_ex(saber_hip_ctx* c, const saber_hip_seqset* queries, co
                 if (a.status == 0) {
                     cnt.banded_cells += a.cells;
                     if (auto* p = jc(k)) p->banded_cells += a.cells;
+                    if (a.score >= jobs[k].want || t.covers) {
+                        finish(k, &a, oc, ol, a.score >= jobs[k].want);
+                        continue;
+                    }
+                    // kept in case the band later exceeds the cap
                     t.have_best = true;
                     t.best = a;
                     t.bc.assign(oc, oc + a.nops);
                     t.bl.assign(ol, ol + a.nops);
-                    if (a.score >= jobs[k].want || t.covers) {
-                        finish(k, &t.best, t.bc.data(), t.bl.data(), a.score >= jobs[k].want);
-                        continue;
-                    }
                 } else if (a.status == 1) {
                     if (t.covers) {
                         finish(k, nullptr, nullptr, nullptr, false);
@@ -1749,6 +1816,7 @@ int saber_hip_traceback_ex(saber_hip_ctx* c, const saber_hip_seqset* queries, co
             t.pad *= 2;
         }
     }
+    lap("last round's results");
     if (int r = same_db()) return r;
     {
         // the call's device buffers, released on the context's device
@@ -1761,6 +1829,18 @@ int saber_hip_traceback_ex(saber_hip_ctx* c, const saber_hip_seqset* queries, co
         seqs.dir.release();
         seqs.codes.release();
         seqs.lens.release();
+        seqs.dcodes.release();
+        seqs.dlens.release();
+        seqs.dst_off.release();
+        seqs.djobs.release();
+        seqs.mat.release();
+        seqs.stops.release();
+        seqs.recs.release();
+        seqs.dids.release();
+        seqs.dorder.release();
+        seqs.dwoff.release();
+        seqs.ddoff.release();
+        seqs.dooff.release();
     }
     if (counters) *counters = cnt;
     return SABER_HIP_OK;

@@ -214,6 +214,9 @@ struct AlignArgs {
 
 // One thread per job, rows in the work buffer (band widths over 16 * 64).
 hipError_t launch_band_align(const AlignArgs& a, hipStream_t s);
+hipError_t launch_gather_ops(uint32_t n, const uint32_t* ids, const AlignRecord* recs, const uint64_t* op_off,
+                             const uint64_t* dst_off, const uint8_t* codes, const uint32_t* lens, uint8_t* dcodes,
+                             uint32_t* dlens, hipStream_t s);
 // Lane classes of one launch: the launch's slots and blocks of each class, as
 // prefix offsets, and the block order: launch block b runs block
 // order[b] & 0xFFFFFF of class order[b] >> 24.

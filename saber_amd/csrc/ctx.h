@@ -83,6 +83,15 @@ int upload(DevBuf<T>& d, const T* h, uint64_t n, hipStream_t s) {
     return SABER_HIP_OK;
 }
 
+// upload() into an allocation kept when it is large enough (buffers that a
+// caller refills call after call).
+template <class T>
+int upload_keep(DevBuf<T>& d, const T* h, uint64_t n, hipStream_t s) {
+    HIP_TRY(d.reserve(n == 0 ? 1 : n));
+    if (n) HIP_TRY(hipMemcpyAsync(d.p, h, sizeof(T) * n, hipMemcpyHostToDevice, s));
+    return SABER_HIP_OK;
+}
+
 template <class T>
 int upload_guarded(DevBuf<T>& d, const T* h, uint64_t n, hipStream_t s) {
     HIP_TRY(d.alloc(n + kGuard + kGuardTail));
