@@ -1128,10 +1128,6 @@ int saber_hip_score_bands(saber_hip_ctx* c, const saber_hip_seqset* queries,
 
 namespace {
 
-// The residues (and bias) of one band_align call on the device and its
-// scratch (work rows, direction bytes, operations), kept by a caller that
-// makes several calls over the same sequence sets: the residues are uploaded
-// once, the scratch grows to the largest call.
 // Host staging that grows to the largest request and is not initialised
 // (every byte read back was first written by a copy or the caller).
 template <class T>
@@ -1147,6 +1143,10 @@ struct HostBuf {
     }
 };
 
+// The residues (and bias) of one band_align call on the device and its
+// scratch (work rows, direction bytes, operations), kept by a caller that
+// makes several calls over the same sequence sets: the residues are uploaded
+// once, the scratch grows to the largest call.
 struct AlignSeqs {
     DevBuf<uint8_t> q, sbuf;
     DevBuf<int8_t> bias;
@@ -1158,15 +1158,11 @@ struct AlignSeqs {
     HostBuf<uint8_t> hcodes;
     HostBuf<uint32_t> hlens;
     // per-call job data, refilled by every call of a caller's series
-    DevBuf<BandJobDev>& djobs = sq.djobs;
-    DevBuf<int32_t>& mat = sq.mat;
-    DevBuf<int32_t>& stops = sq.stops;
-    DevBuf<AlignRecord>& recs = sq.recs;
-    DevBuf<uint32_t>& dids = sq.dids;
-    DevBuf<uint32_t>& dorder = sq.dorder;
-    DevBuf<uint64_t>& dwoff = sq.dwoff;
-    DevBuf<uint64_t>& ddoff = sq.ddoff;
-    DevBuf<uint64_t>& dooff = sq.dooff;
+    DevBuf<BandJobDev> djobs;
+    DevBuf<int32_t> mat, stops;
+    DevBuf<AlignRecord> recs;
+    DevBuf<uint32_t> dids, dorder;
+    DevBuf<uint64_t> dwoff, ddoff, dooff;
 };
 
 // saber_hip_band_align; iters > 0 also times iters repetitions of every
