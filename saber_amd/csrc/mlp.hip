@@ -409,18 +409,27 @@ __global__ __launch_bounds__(256) void mlp_backward_act_kernel(MlpWeights w, con
 }
 
 // out[u] += sum over rows of wgt[row] * M[row][u] (wgt == null: weight 1),
-// M bf16 [n][256]: one thread per column, rows split across blocks.
+// M bf16 [n][256]: one thread per column, rows split across blocks (enough
+// blocks to occupy the device; one atomic add per block and column).
 __global__ __launch_bounds__(256) void colsum_kernel(const __bf16* M, const float* wgt, uint64_t n,
                                                      uint64_t rows_per_block, float* out) {
     const int u = threadIdx.x;
     const uint64_t r0 = (uint64_t)blockIdx.x * rows_per_block;
     const uint64_t r1 = r0 + rows_per_block < n ? r0 + rows_per_block : n;
-    float acc = 0.f;
-    for (uint64_t row = r0; row < r1; ++row) {
-        const float v = (float)M[row * kHid + u];
-        acc += wgt ? wgt[row] * v : v;
+    float acc[4] = {0.f, 0.f, 0.f, 0.f};
+    uint64_t row = r0;
+    for (; row + 4 <= r1; row += 4) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const float v = (float)M[(row + k) * kHid + u];
+            acc[k] += wgt ? wgt[row + k] * v : v;
+        }
     }
-    atomicAdd(out + u, acc);
+    for (; row < r1; ++row) {
+        const float v = (float)M[row * kHid + u];
+        acc[0] += wgt ? wgt[row] * v : v;
+    }
+    atomicAdd(out + u, (acc[0] + acc[1]) + (acc[2] + acc[3]));
 }
 
 // C[a][b] += sum_k A[k][a] * B[k][b] over k in this block's chunk (A, B bf16
@@ -543,7 +552,8 @@ hipError_t mlp_launch_backward_act(const MlpWeights& w, const __bf16* X, const f
 
 hipError_t mlp_launch_colsum(const __bf16* M, const float* wgt, uint64_t n, float* out, hipStream_t s) {
     if (n == 0) return hipSuccess;
-    const uint64_t rpb = 2048;
+    // at most 4096 blocks (same-address atomics per column), at least 64 rows each
+    const uint64_t rpb = std::max<uint64_t>(64, (n + 4095) / 4096);
     hipLaunchKernelGGL(colsum_kernel, dim3((uint32_t)((n + rpb - 1) / rpb)), dim3(256), 0, s, M, wgt, n,
                        rpb, out);
     return hipGetLastError();
