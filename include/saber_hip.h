@@ -57,6 +57,9 @@ extern "C" {
                                       band_align, traceback and the batch creators) fails with SABER_HIP_EDEVICE "injected fault" and writes
                                       no result: exercises a host's fallback path (backend_fallbacks,
                                       src/search/engine.rs:1107-1118) */
+#define SABER_HIP_HOST_PLAN 8u   /* plan every band batch on the host (testing; batches of 65,536 jobs and more are
+                                    otherwise planned on the device) */
+#define SABER_HIP_DEVICE_PLAN 16u /* plan every band batch on the device, small ones too (testing) */
 typedef struct saber_hip_opts {
     uint32_t flags;
     uint32_t reserved[7];

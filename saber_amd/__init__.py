@@ -291,11 +291,16 @@ def band_variants():
 class Context:
     """One device context (saber_hip_create)."""
 
-    def __init__(self, device=0, force_exact32=False, graphs=True, inject_faults=False):
+    def __init__(self, device=0, force_exact32=False, graphs=True, inject_faults=False, plan=None):
+        """plan: None (band batches of 65,536 jobs and more planned on the
+        device, smaller ones on the host), "host" or "device" (every batch)."""
+        if plan not in (None, "host", "device"):
+            raise ValueError(f"plan must be None, 'host' or 'device', not {plan!r}")
         L = _ffi.lib()
         opts = _ffi.Opts()
         opts.flags = ((_ffi.FORCE_EXACT32 if force_exact32 else 0) | (0 if graphs else _ffi.NO_GRAPH) |
-                      (_ffi.INJECT_FAULTS if inject_faults else 0))
+                      (_ffi.INJECT_FAULTS if inject_faults else 0) |
+                      {None: 0, "host": _ffi.HOST_PLAN, "device": _ffi.DEVICE_PLAN}[plan])
         h = C.c_void_p()
         self._children = weakref.WeakSet()   # batches and models on this context
         _ffi.check(L.saber_hip_create(device, C.byref(opts), C.byref(h)))

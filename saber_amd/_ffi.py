@@ -14,6 +14,8 @@ OK, EINVAL, EDEVICE, ENOMEM, ESTATE = 0, 1, 2, 3, 4
 FORCE_EXACT32 = 1
 NO_GRAPH = 2
 INJECT_FAULTS = 4
+HOST_PLAN = 8
+DEVICE_PLAN = 16
 
 
 class Opts(C.Structure):

@@ -28,6 +28,7 @@
 #include "../../include/saber_hip.h"
 #include "common.h"
 #include "ctx.h"
+#include "band_plan.h"
 
 using namespace saber;
 
@@ -102,20 +103,6 @@ int upload_codes(DevBuf<uint8_t>& d, const uint8_t* h, uint64_t n, hipStream_t s
 
 }  // namespace saber
 
-// One lane steps through every row of its job, so a job's time is at least
-// rows x the lane's per-row time. Jobs whose time at their throughput shape
-// would exceed this fraction of the batch's estimated time are planned for
-// low per-row latency instead (short lanes, launched first). A context can
-// replace the rule by a fixed row count (saber_hip_band_tune).
-constexpr double kLatencyShare = 0.6;
-
-// Rates for the band planner's time estimates (lane VALU instructions per
-// ms): the whole GPU at about half its issue rate, and one lane of a wave
-// that shares its SIMD with another.
-constexpr double kGpuLaneInstrPerMs = 3.4e10;
-constexpr double kLaneInstrPerMs = 3.0e5;
-
-
 struct saber_hip_band_batch {
     saber_hip_ctx* ctx = nullptr;
     uint64_t njobs = 0;
@@ -131,13 +118,12 @@ struct saber_hip_band_batch {
     DevBuf<BandJobDev> jobs;
     DevBuf<uint32_t> pairs;
     // multi-shape launch: blocks in dispatch order (longest first)
-    std::vector<BandBlock> host_blocks;
     DevBuf<BandBlock> blocks;
+    uint64_t nchunks = 0;
     // exact int32 kernel: wide / unsuitable jobs known at plan time, plus the
     // packed kernel's saturated jobs known after it ran
-    std::vector<uint32_t> generic_static;
-    std::vector<BandJobDev> host_jobs;
-    DevBuf<uint32_t> gen_ids;      // static int32 jobs (known at plan time), uploaded once
+    uint32_t ngeneric = 0;
+    DevBuf<uint32_t> gen_ids;      // static int32 jobs (known at plan time), in job order
     DevBuf<uint64_t> gen_off;
     DevBuf<int32_t> gen_work;
     // saturated jobs are rescored by a fixed grid reading their count on the
@@ -160,30 +146,6 @@ struct saber_hip_band_batch {
 };
 
 namespace {
-
-// Query / window lengths the packed band kernel accepts (window columns are
-// handled as 8 * j in 32 bits).
-constexpr int64_t kMaxPackedLen = int64_t(1) << 27;
-
-// sum of x for x in [a, b] (0 when b < a)
-int64_t range_sum(int64_t a, int64_t b) { return b < a ? 0 : (a + b) * (b - a + 1) / 2; }
-
-// Band cells of a planned job: every row i in [i_first, i_first + rows) holds
-// the columns max(0, i + lo) .. min(n - 1, i + lo + w - 1), at least one
-// (the planner keeps exactly those rows), so the count is a sum of clamped
-// linear terms.
-uint64_t band_cells(const BandJobDev& d) {
-    if (d.rows <= 0) return 0;
-    const int64_t i0 = d.i_first, i1 = (int64_t)d.i_first + d.rows - 1;
-    const int64_t lo = d.lo, hi = (int64_t)d.lo + d.w - 1, n = d.n;
-    // sum of min(n - 1, i + hi): i + hi up to i = n - 1 - hi, then n - 1
-    const int64_t ic = n - 1 - hi;
-    int64_t right = range_sum(i0 + hi, std::min(i1, ic) + hi);
-    right += std::max<int64_t>(0, i1 - std::max(i0, ic + 1) + 1) * (n - 1);
-    // sum of max(0, i + lo): 0 below i = -lo
-    const int64_t left = range_sum(std::max(i0, -lo) + lo, i1 + lo);
-    return (uint64_t)(right - left + (i1 - i0 + 1));
-}
 
 // Stable LSD radix sort of v by the low `bits` bits of key[v[i]] (11-bit
 // digits): the planner's orderings over millions of jobs, without comparator
@@ -335,6 +297,7 @@ void saber_hip_destroy(saber_hip_ctx* c) {
     for (int k = 0; k < kStreams; ++k)
         if (c->work[k]) (void)hipStreamSynchronize(c->work[k]);
     c->scan.reset();
+    c->plan.reset();
     c->db.release();
     c->db_off_dev.release();
     for (int k = 0; k < kStreams; ++k) {
@@ -557,6 +520,255 @@ int saber_hip_band_autotune(saber_hip_ctx* c, const saber_hip_seqset* queries,
     return SABER_HIP_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// The host planner: the plan of band_plan_device (band_plan.hip) built by
+// one host thread, for small batches (and every batch of a context created
+// with SABER_HIP_HOST_PLAN), then uploaded into the batch's buffers.
+template <class Lap>
+int plan_on_host(saber_hip_ctx* c, hipStream_t st, const BandPlanInput& in, saber_hip_band_batch* b, BandPlan* out,
+                 Lap&& lap) {
+    const uint64_t njobs = in.njobs;
+    std::vector<BandJobDev> host_jobs(njobs);
+    std::vector<BandBlock> host_blocks;
+    std::vector<uint32_t> generic;  // int32-kernel jobs, in job order
+    std::vector<uint64_t> host_cells(njobs, 0);
+    std::vector<int> thr_shape(njobs, -1);  // throughput shape of each packed job
+    double est_work = 0.0;                  // lane instructions, all packed jobs
+    // bucket 2 * shape + 1 holds the shape's many-row jobs
+    std::vector<std::vector<uint32_t>> bucket(2 * c->shapes.size());
+    std::vector<int32_t> rows_of(njobs, 0);
+    for (uint64_t k = 0; k < njobs; ++k) {
+        const saber_hip_band_job& J = in.jobs[k];
+        if (J.query >= in.qcount) return fail(SABER_HIP_EINVAL, "job query index out of range");
+        if (J.subject >= in.nsub) return fail(SABER_HIP_EINVAL, "job subject index out of range");
+        const uint64_t slen = in.soffs[J.subject + 1] - in.soffs[J.subject];
+        if (J.win_hi < J.win_lo || J.win_hi > slen)
+            return fail(SABER_HIP_EINVAL, "job window outside its subject");
+        const int64_t m = (int64_t)(in.qoffs[J.query + 1] - in.qoffs[J.query]);
+        const int64_t n = (int64_t)J.win_hi - J.win_lo;
+        BandJobDev d{};
+        d.q_off = in.qoffs[J.query];
+        d.s_off = in.soffs[J.subject] + J.win_lo;
+        d.m = (uint32_t)m;
+        d.n = (uint32_t)n;
+        host_jobs[k] = d;
+        if (m == 0 || n == 0 || J.band_hi < J.band_lo) continue;
+        const int64_t lo = std::max<int64_t>(J.band_lo, -(m - 1));
+        const int64_t hi = std::min<int64_t>(J.band_hi, n - 1);
+        const int64_t w = hi - lo + 1;
+        if (w <= 0) continue;
+        const int64_t i_first = std::max<int64_t>(0, -hi);
+        const int64_t i_last = std::min<int64_t>(m - 1, n - 1 - lo);
+        if (i_last < i_first) continue;
+        d.lo = (int32_t)lo;
+        d.w = (int32_t)w;
+        d.i_first = (int32_t)i_first;
+        d.rows = (int32_t)(i_last - i_first + 1);
+        host_cells[k] = band_cells(d);
+        if (in.swap_ok) {
+            // transposed: rows over the window, band [-hi, -lo] (already
+            // inside [-(n - 1), m - 1] since lo >= -(m - 1) and hi <= n - 1)
+            const int64_t t_first = std::max<int64_t>(0, lo), t_last = std::min<int64_t>(n - 1, m - 1 + hi);
+            if (t_last >= t_first && t_last - t_first + 1 < d.rows) {
+                BandJobDev t{};
+                t.q_off = d.s_off;
+                t.s_off = d.q_off;
+                t.m = (uint32_t)n;
+                t.n = (uint32_t)m;
+                t.swapped = 1;
+                t.lo = (int32_t)-hi;
+                t.w = (int32_t)w;
+                t.i_first = (int32_t)t_first;
+                t.rows = (int32_t)(t_last - t_first + 1);
+                d = t;
+            }
+        }
+        host_jobs[k] = d;
+        rows_of[k] = d.rows;
+        // The packed kernel keeps row indices in 16 bits and window columns
+        // (times 8) in 32 bits.
+        const bool fits = d.rows <= (int32_t)kMaxBandRows && m < kMaxPackedLen && n < kMaxPackedLen;
+        const int shape = (in.packed_ok && fits && w < (int64_t)c->choice.size()) ? c->choice[w] : -1;
+        if (shape < 0) {
+            generic.push_back((uint32_t)k);
+            continue;
+        }
+        thr_shape[k] = shape;
+        est_work += 0.5 * (d.rows + c->shapes[shape].L + 3) * c->shapes[shape].pair_cost();
+    }
+    lap("jobs");
+    // Many-row jobs: a fixed row count, or the share of the estimated time.
+    const double est_ms = est_work / kGpuLaneInstrPerMs;
+    for (uint64_t k = 0; k < njobs; ++k) {
+        const int shape = thr_shape[k];
+        if (shape < 0) continue;
+        const Shape& s = c->shapes[shape];
+        bool lat;
+        if (c->latency_rows < 0)
+            lat = (rows_of[k] + s.L + 3) * s.row_cost() / kLaneInstrPerMs > kLatencyShare * est_ms;
+        else
+            lat = c->latency_rows > 0 && rows_of[k] > c->latency_rows;
+        const int w = host_jobs[k].w;
+        if (lat)
+            bucket[2 * c->choice_lat[w] + 1].push_back((uint32_t)k);
+        else
+            bucket[2 * shape].push_back((uint32_t)k);
+    }
+    // Pairs. A pair runs for its longer job's rows, so jobs are paired with
+    // the next longest of their bucket. Jobs of one query that start on the
+    // same query row are paired among themselves first: such a pair reads
+    // the same query code on every row and uses the pair-table flavour. The
+    // rest (different queries or first rows, odd ones out) pair by rows.
+    // Work chunks of 64 lanes (64 / L pairs) of one shape and flavour are
+    // claimed by the persistent kernel in decreasing order of their estimated
+    // time (the longest pair's row steps x the shape's per-row cost), longest
+    // processing time first.
+    std::vector<uint32_t> pairs;
+    std::vector<double> chunk_est;
+    auto add_chunks = [&](const std::vector<uint32_t>& v, int shape, uint32_t flags) {
+        if (v.empty()) return;
+        const Shape& s = c->shapes[shape];
+        const uint32_t per_chunk = 64u / (uint32_t)s.L;
+        const uint64_t first_pair = pairs.size() / 2;
+        for (size_t i = 0; i < v.size(); i += 2) {
+            pairs.push_back(v[i]);
+            pairs.push_back(i + 1 < v.size() ? v[i + 1] : kNone);
+        }
+        const uint64_t np = pairs.size() / 2 - first_pair;
+        for (uint64_t p0 = 0; p0 < np; p0 += per_chunk) {
+            const uint32_t cnt = (uint32_t)std::min<uint64_t>(per_chunk, np - p0);
+            const int steps = (rows_of[v[2 * p0]] + s.L - 1 + 3) & ~3;
+            host_blocks.push_back(BandBlock{(uint32_t)shape, (uint32_t)(first_pair + p0), cnt, flags});
+            chunk_est.push_back((double)steps * s.row_cost());
+        }
+    };
+    lap("buckets");
+    // sort keys: (query start, first row, rows descending) when they fit 64
+    // bits (they do unless the query blob is beyond 2^33 residues), and the
+    // rows descending alone
+    uint64_t max_qoff = 0, max_ifirst = 0;
+    for (uint64_t k = 0; k < njobs; ++k)
+        if (thr_shape[k] >= 0) {
+            max_qoff = std::max<uint64_t>(max_qoff, host_jobs[k].q_off);
+            max_ifirst = std::max<uint64_t>(max_ifirst, (uint64_t)host_jobs[k].i_first);
+        }
+    const int bq = bits_of(max_qoff), bi = bits_of(max_ifirst), br = 16;
+    const bool radix_ok = bq + bi + br <= 64;
+    std::vector<uint64_t> key_group, key_rows(njobs, 0);
+    if (radix_ok) key_group.assign(njobs, 0);
+    for (uint64_t k = 0; k < njobs; ++k) {
+        if (thr_shape[k] < 0) continue;
+        const uint64_t rdesc = 65535u - (uint64_t)rows_of[k];
+        key_rows[k] = rdesc;
+        if (radix_ok)
+            key_group[k] = (host_jobs[k].q_off << (bi + br)) | ((uint64_t)host_jobs[k].i_first << br) | rdesc;
+    }
+    for (size_t bk = 0; bk < bucket.size(); ++bk) {
+        auto& v = bucket[bk];
+        if (v.empty()) continue;
+        const int shape = (int)(bk / 2);
+        // group by (query start, first row), longest first within a group
+        if (radix_ok) {
+            radix_sort_by(v, key_group, bq + bi + br);
+        } else {
+            std::stable_sort(v.begin(), v.end(), [&](uint32_t x, uint32_t y) {
+                const BandJobDev &ja = host_jobs[x], &jb = host_jobs[y];
+                if (ja.q_off != jb.q_off) return ja.q_off < jb.q_off;
+                if (ja.i_first != jb.i_first) return ja.i_first < jb.i_first;
+                return rows_of[x] > rows_of[y];
+            });
+        }
+        std::vector<uint32_t> shared, rest;
+        shared.reserve(v.size());
+        auto same_group = [&](uint32_t x, uint32_t y) {
+            if (radix_ok) return (key_group[x] >> br) == (key_group[y] >> br);
+            return host_jobs[x].q_off == host_jobs[y].q_off && host_jobs[x].i_first == host_jobs[y].i_first;
+        };
+        for (size_t i = 0; i < v.size();) {
+            if (host_jobs[v[i]].swapped) {  // rows from the residue blob: never shared
+                rest.push_back(v[i++]);
+                continue;
+            }
+            size_t j = i;
+            while (j < v.size() && !host_jobs[v[j]].swapped && same_group(v[j], v[i])) ++j;
+            const size_t even = i + ((j - i) & ~(size_t)1);
+            shared.insert(shared.end(), v.begin() + i, v.begin() + even);
+            if (even < j) rest.push_back(v[even]);
+            i = j;
+        }
+        // shared pairs in order of their longer job's rows (pairs stay intact)
+        {
+            std::vector<uint32_t> order(shared.size() / 2);
+            std::vector<uint64_t> pk(order.size());
+            for (size_t k = 0; k < order.size(); ++k) {
+                order[k] = (uint32_t)k;
+                pk[k] = key_rows[shared[2 * k]];
+            }
+            radix_sort_by(order, pk, br);
+            std::vector<uint32_t> sorted(shared.size());
+            for (size_t k = 0; k < order.size(); ++k) {
+                sorted[2 * k] = shared[2 * order[k]];
+                sorted[2 * k + 1] = shared[2 * order[k] + 1];
+            }
+            shared.swap(sorted);
+        }
+        radix_sort_by(rest, key_rows, br);
+        add_chunks(shared, shape, kChunkShared);
+        add_chunks(rest, shape, 0);
+    }
+    lap("pairs");
+    if (pairs.size() / 2 >= 0xFFFFFFFFull) return fail(SABER_HIP_EINVAL, "too many job pairs for one batch");
+    {
+        // longest estimated chunk first; the estimates are integers (row
+        // steps x the shape's integer per-row cost), ordered descending
+        std::vector<uint32_t> order(host_blocks.size());
+        for (uint32_t i = 0; i < order.size(); ++i) order[i] = i;
+        double emax = 0.0;
+        for (double e : chunk_est) emax = std::max(emax, e);
+        std::vector<uint64_t> ek(order.size());
+        for (size_t i = 0; i < order.size(); ++i) ek[i] = (uint64_t)(emax - chunk_est[i]);
+        radix_sort_by(order, ek, bits_of((uint64_t)emax));
+        std::vector<BandBlock> sorted(order.size());
+        for (size_t i = 0; i < order.size(); ++i) sorted[i] = host_blocks[order[i]];
+        host_blocks.swap(sorted);
+    }
+
+    lap("chunk order");
+
+    if (int r = upload(b->jobs, host_jobs.data(), njobs, st)) return r;
+    if (int r = upload(b->cells, host_cells.data(), njobs, st)) return r;
+    if (int r = upload(b->pairs, pairs.data(), pairs.size(), st)) return r;
+    if (int r = upload(b->blocks, host_blocks.data(), host_blocks.size(), st)) return r;
+    out->npairs = pairs.size() / 2;
+    out->nchunks = host_blocks.size();
+    out->ngeneric = (uint32_t)generic.size();
+    // int32 jobs: ids and work offsets
+    if (!generic.empty()) {
+        std::vector<uint64_t> off(generic.size());
+        uint64_t tot = 0;
+        for (size_t i = 0; i < generic.size(); ++i) {
+            off[i] = tot;
+            tot += 4 * ((uint64_t)host_jobs[generic[i]].w + 1);
+        }
+        if (int r = upload(b->gen_ids, generic.data(), generic.size(), st)) return r;
+        if (int r = upload(b->gen_off, off.data(), off.size(), st)) return r;
+        out->gen_work = tot;
+    }
+    out->wmax = 0;
+    for (uint64_t k = 0; k < njobs; ++k)
+        if (thr_shape[k] >= 0) out->wmax = std::max(out->wmax, host_jobs[k].w);
+    // the host arrays stay alive until their copies are done
+    HIP_TRY(hipStreamSynchronize(st));
+    return SABER_HIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int saber_hip_band_batch_create(saber_hip_ctx* c, const saber_hip_seqset* queries,
                                 const saber_hip_seqset* subjects, const saber_hip_band_job* jobs,
                                 uint64_t njobs, const saber_hip_scheme* scheme, int ends,
@@ -607,209 +819,23 @@ int saber_hip_band_batch_create(saber_hip_ctx* c, const saber_hip_seqset* querie
         std::fprintf(stderr, "#   plan %s %.1f ms\n", what, std::chrono::duration<double, std::milli>(t1 - tp).count());
         tp = t1;
     };
-    // Plan.
-    b->host_jobs.assign(njobs, BandJobDev{});
-    std::vector<uint64_t> host_cells(njobs, 0);
-    std::vector<int> thr_shape(njobs, -1);  // throughput shape of each packed job
-    double est_work = 0.0;                  // lane instructions, all packed jobs
-    // bucket 2 * shape + 1 holds the shape's many-row jobs
-    std::vector<std::vector<uint32_t>> bucket(2 * c->shapes.size());
-    std::vector<int32_t> rows_of(njobs, 0);
-    for (uint64_t k = 0; k < njobs; ++k) {
-        const saber_hip_band_job& J = jobs[k];
-        if (J.query >= queries->count) return fail(SABER_HIP_EINVAL, "job query index out of range");
-        if (J.subject >= nsub) return fail(SABER_HIP_EINVAL, "job subject index out of range");
-        const uint64_t slen = soffs[J.subject + 1] - soffs[J.subject];
-        if (J.win_hi < J.win_lo || J.win_hi > slen)
-            return fail(SABER_HIP_EINVAL, "job window outside its subject");
-        const int64_t m = (int64_t)(queries->offsets[J.query + 1] - queries->offsets[J.query]);
-        const int64_t n = (int64_t)J.win_hi - J.win_lo;
-        BandJobDev d{};
-        d.q_off = queries->offsets[J.query];
-        d.s_off = soffs[J.subject] + J.win_lo;
-        d.m = (uint32_t)m;
-        d.n = (uint32_t)n;
-        b->host_jobs[k] = d;
-        if (m == 0 || n == 0 || J.band_hi < J.band_lo) continue;
-        const int64_t lo = std::max<int64_t>(J.band_lo, -(m - 1));
-        const int64_t hi = std::min<int64_t>(J.band_hi, n - 1);
-        const int64_t w = hi - lo + 1;
-        if (w <= 0) continue;
-        const int64_t i_first = std::max<int64_t>(0, -hi);
-        const int64_t i_last = std::min<int64_t>(m - 1, n - 1 - lo);
-        if (i_last < i_first) continue;
-        d.lo = (int32_t)lo;
-        d.w = (int32_t)w;
-        d.i_first = (int32_t)i_first;
-        d.rows = (int32_t)(i_last - i_first + 1);
-        host_cells[k] = band_cells(d);
-        if (swap_ok) {
-            // transposed: rows over the window, band [-hi, -lo] (already
-            // inside [-(n - 1), m - 1] since lo >= -(m - 1) and hi <= n - 1)
-            const int64_t t_first = std::max<int64_t>(0, lo), t_last = std::min<int64_t>(n - 1, m - 1 + hi);
-            if (t_last >= t_first && t_last - t_first + 1 < d.rows) {
-                BandJobDev t{};
-                t.q_off = d.s_off;
-                t.s_off = d.q_off;
-                t.m = (uint32_t)n;
-                t.n = (uint32_t)m;
-                t.swapped = 1;
-                t.lo = (int32_t)-hi;
-                t.w = (int32_t)w;
-                t.i_first = (int32_t)t_first;
-                t.rows = (int32_t)(t_last - t_first + 1);
-                d = t;
-            }
-        }
-        b->host_jobs[k] = d;
-        rows_of[k] = d.rows;
-        // The packed kernel keeps row indices in 16 bits and window columns
-        // (times 8) in 32 bits.
-        const bool fits = d.rows <= (int32_t)kMaxBandRows && m < kMaxPackedLen && n < kMaxPackedLen;
-        const int shape = (packed_ok && fits && w < (int64_t)c->choice.size()) ? c->choice[w] : -1;
-        if (shape < 0) {
-            b->generic_static.push_back((uint32_t)k);
-            continue;
-        }
-        thr_shape[k] = shape;
-        est_work += 0.5 * (d.rows + c->shapes[shape].L + 3) * c->shapes[shape].pair_cost();
+
+    // Plan: on the device for large batches, else on the host (the same plan).
+    hipStream_t st = c->stream;
+    const BandPlanInput in{jobs, njobs, queries->offsets, queries->count, soffs,
+                           subjects ? nullptr : c->db_off_dev.p, nsub, packed_ok, swap_ok};
+    const bool on_device = (c->flags & SABER_HIP_DEVICE_PLAN) ||
+                           (!(c->flags & SABER_HIP_HOST_PLAN) && njobs >= kDevicePlanMinJobs);
+    BandPlan plan;
+    if (on_device) {
+        if (int r = band_plan_device(c, st, in, b->jobs, b->cells, b->pairs, b->blocks, b->gen_ids, b->gen_off, &plan))
+            return r;
+        lap("on the device");
+    } else {
+        if (int r = plan_on_host(c, st, in, b.get(), &plan, lap)) return r;
     }
-    lap("jobs");
-    // Many-row jobs: a fixed row count, or the share of the estimated time.
-    const double est_ms = est_work / kGpuLaneInstrPerMs;
-    for (uint64_t k = 0; k < njobs; ++k) {
-        const int shape = thr_shape[k];
-        if (shape < 0) continue;
-        const Shape& s = c->shapes[shape];
-        bool lat;
-        if (c->latency_rows < 0)
-            lat = (rows_of[k] + s.L + 3) * s.row_cost() / kLaneInstrPerMs > kLatencyShare * est_ms;
-        else
-            lat = c->latency_rows > 0 && rows_of[k] > c->latency_rows;
-        const int w = b->host_jobs[k].w;
-        if (lat)
-            bucket[2 * c->choice_lat[w] + 1].push_back((uint32_t)k);
-        else
-            bucket[2 * shape].push_back((uint32_t)k);
-    }
-    // Pairs. A pair runs for its longer job's rows, so jobs are paired with
-    // the next longest of their bucket. Jobs of one query that start on the
-    // same query row are paired among themselves first: such a pair reads
-    // the same query code on every row and uses the pair-table flavour. The
-    // rest (different queries or first rows, odd ones out) pair by rows.
-    // Work chunks of 64 lanes (64 / L pairs) of one shape and flavour are
-    // claimed by the persistent kernel in decreasing order of their estimated
-    // time (the longest pair's row steps x the shape's per-row cost), longest
-    // processing time first.
-    std::vector<uint32_t> pairs;
-    std::vector<double> chunk_est;
-    auto add_chunks = [&](const std::vector<uint32_t>& v, int shape, uint32_t flags) {
-        if (v.empty()) return;
-        const Shape& s = c->shapes[shape];
-        const uint32_t per_chunk = 64u / (uint32_t)s.L;
-        const uint64_t first_pair = pairs.size() / 2;
-        for (size_t i = 0; i < v.size(); i += 2) {
-            pairs.push_back(v[i]);
-            pairs.push_back(i + 1 < v.size() ? v[i + 1] : kNone);
-        }
-        const uint64_t np = pairs.size() / 2 - first_pair;
-        for (uint64_t p0 = 0; p0 < np; p0 += per_chunk) {
-            const uint32_t cnt = (uint32_t)std::min<uint64_t>(per_chunk, np - p0);
-            const int steps = (rows_of[v[2 * p0]] + s.L - 1 + 3) & ~3;
-            b->host_blocks.push_back(BandBlock{(uint32_t)shape, (uint32_t)(first_pair + p0), cnt, flags});
-            chunk_est.push_back((double)steps * s.row_cost());
-        }
-    };
-    lap("buckets");
-    // sort keys: (query start, first row, rows descending) when they fit 64
-    // bits (they do unless the query blob is beyond 2^33 residues), and the
-    // rows descending alone
-    uint64_t max_qoff = 0, max_ifirst = 0;
-    for (uint64_t k = 0; k < njobs; ++k)
-        if (thr_shape[k] >= 0) {
-            max_qoff = std::max<uint64_t>(max_qoff, b->host_jobs[k].q_off);
-            max_ifirst = std::max<uint64_t>(max_ifirst, (uint64_t)b->host_jobs[k].i_first);
-        }
-    const int bq = bits_of(max_qoff), bi = bits_of(max_ifirst), br = 16;
-    const bool radix_ok = bq + bi + br <= 64;
-    std::vector<uint64_t> key_group, key_rows(njobs, 0);
-    if (radix_ok) key_group.assign(njobs, 0);
-    for (uint64_t k = 0; k < njobs; ++k) {
-        if (thr_shape[k] < 0) continue;
-        const uint64_t rdesc = 65535u - (uint64_t)rows_of[k];
-        key_rows[k] = rdesc;
-        if (radix_ok)
-            key_group[k] = (b->host_jobs[k].q_off << (bi + br)) | ((uint64_t)b->host_jobs[k].i_first << br) | rdesc;
-    }
-    for (size_t bk = 0; bk < bucket.size(); ++bk) {
-        auto& v = bucket[bk];
-        if (v.empty()) continue;
-        const int shape = (int)(bk / 2);
-        // group by (query start, first row), longest first within a group
-        if (radix_ok) {
-            radix_sort_by(v, key_group, bq + bi + br);
-        } else {
-            std::stable_sort(v.begin(), v.end(), [&](uint32_t x, uint32_t y) {
-                const BandJobDev &ja = b->host_jobs[x], &jb = b->host_jobs[y];
-                if (ja.q_off != jb.q_off) return ja.q_off < jb.q_off;
-                if (ja.i_first != jb.i_first) return ja.i_first < jb.i_first;
-                return rows_of[x] > rows_of[y];
-            });
-        }
-        std::vector<uint32_t> shared, rest;
-        shared.reserve(v.size());
-        auto same_group = [&](uint32_t x, uint32_t y) {
-            if (radix_ok) return (key_group[x] >> br) == (key_group[y] >> br);
-            return b->host_jobs[x].q_off == b->host_jobs[y].q_off && b->host_jobs[x].i_first == b->host_jobs[y].i_first;
-        };
-        for (size_t i = 0; i < v.size();) {
-            if (b->host_jobs[v[i]].swapped) {  // rows from the residue blob: never shared
-                rest.push_back(v[i++]);
-                continue;
-            }
-            size_t j = i;
-            while (j < v.size() && !b->host_jobs[v[j]].swapped && same_group(v[j], v[i])) ++j;
-            const size_t even = i + ((j - i) & ~(size_t)1);
-            shared.insert(shared.end(), v.begin() + i, v.begin() + even);
-            if (even < j) rest.push_back(v[even]);
-            i = j;
-        }
-        // shared pairs in order of their longer job's rows (pairs stay intact)
-        {
-            std::vector<uint32_t> order(shared.size() / 2);
-            std::vector<uint64_t> pk(order.size());
-            for (size_t k = 0; k < order.size(); ++k) {
-                order[k] = (uint32_t)k;
-                pk[k] = key_rows[shared[2 * k]];
-            }
-            radix_sort_by(order, pk, br);
-            std::vector<uint32_t> sorted(shared.size());
-            for (size_t k = 0; k < order.size(); ++k) {
-                sorted[2 * k] = shared[2 * order[k]];
-                sorted[2 * k + 1] = shared[2 * order[k] + 1];
-            }
-            shared.swap(sorted);
-        }
-        radix_sort_by(rest, key_rows, br);
-        add_chunks(shared, shape, kChunkShared);
-        add_chunks(rest, shape, 0);
-    }
-    lap("pairs");
-    if (pairs.size() / 2 >= 0xFFFFFFFFull) return fail(SABER_HIP_EINVAL, "too many job pairs for one batch");
-    {
-        // longest estimated chunk first; the estimates are integers (row
-        // steps x the shape's integer per-row cost), ordered descending
-        std::vector<uint32_t> order(b->host_blocks.size());
-        for (uint32_t i = 0; i < order.size(); ++i) order[i] = i;
-        double emax = 0.0;
-        for (double e : chunk_est) emax = std::max(emax, e);
-        std::vector<uint64_t> ek(order.size());
-        for (size_t i = 0; i < order.size(); ++i) ek[i] = (uint64_t)(emax - chunk_est[i]);
-        radix_sort_by(order, ek, bits_of((uint64_t)emax));
-        std::vector<BandBlock> sorted(order.size());
-        for (size_t i = 0; i < order.size(); ++i) sorted[i] = b->host_blocks[order[i]];
-        b->host_blocks.swap(sorted);
-    }
+    b->nchunks = plan.nchunks;
+    b->ngeneric = plan.ngeneric;
 
     // Score table for the packed kernel: int8 T[a][c] = S(a, c) for codes
     // a, c < 24 (stride 25), column 24 = sentinel and row 24 = dead row
@@ -826,11 +852,8 @@ int saber_hip_band_batch_create(saber_hip_ctx* c, const saber_hip_seqset* querie
                 t8[a * kTable8Stride + cc] = (int8_t)v;
             }
     }
-
     const std::vector<uint32_t> ptable = build_pair_table(scheme->matrix);
-    lap("chunk order + tables");
 
-    hipStream_t st = c->stream;
     const uint64_t nq = queries->count ? queries->offsets[queries->count] : 0;
     if (int r = upload_codes(b->q, queries->residues, nq, st)) return r;
     if (b->has_bias) {
@@ -851,42 +874,19 @@ int saber_hip_band_batch_create(saber_hip_ctx* c, const saber_hip_seqset* querie
     if (int r = upload(b->pair_table, ptable.data(), ptable.size(), st)) return r;
     HIP_TRY(b->next_chunk.alloc(1));
     if (int r = upload(b->mat, scheme->matrix, 576, st)) return r;
-    if (int r = upload(b->jobs, b->host_jobs.data(), njobs, st)) return r;
-    if (int r = upload(b->pairs, pairs.data(), pairs.size(), st)) return r;
-    if (int r = upload(b->blocks, b->host_blocks.data(), b->host_blocks.size(), st)) return r;
     HIP_TRY(b->flagged.alloc(njobs ? njobs : 1));
     HIP_TRY(b->nflagged.alloc(1));
-    {
-        // static int32 jobs: ids and work offsets, uploaded once
-        const std::vector<uint32_t>& ids = b->generic_static;
-        if (!ids.empty()) {
-            std::vector<uint64_t> off(ids.size());
-            uint64_t tot = 0;
-            for (size_t i = 0; i < ids.size(); ++i) {
-                off[i] = tot;
-                tot += 4 * ((uint64_t)b->host_jobs[ids[i]].w + 1);
-            }
-            if (int r = upload(b->gen_ids, ids.data(), ids.size(), st)) return r;
-            if (int r = upload(b->gen_off, off.data(), off.size(), st)) return r;
-            HIP_TRY(b->gen_work.alloc(tot));
-        }
-        // saturated-job slots: any packed job may saturate, so a slot holds
-        // the widest packed job's two band rows
-        int32_t wmax = 0;
-        std::vector<bool> is_static(njobs, false);
-        for (uint32_t k : ids) is_static[k] = true;
-        for (uint64_t k = 0; k < njobs; ++k)
-            if (!is_static[k] && b->host_jobs[k].rows > 0) wmax = std::max(wmax, b->host_jobs[k].w);
-        if (njobs > 0) {
-            b->flag_slot = 4u * ((uint32_t)wmax + 1u);
-            b->flag_threads = (uint32_t)std::min<uint64_t>(8192, std::max<uint64_t>(128, (njobs / 64 + 127) / 128 * 128));
-            HIP_TRY(b->flag_work.alloc((uint64_t)b->flag_threads * b->flag_slot));
-        }
+    if (plan.ngeneric) HIP_TRY(b->gen_work.alloc(plan.gen_work));
+    // saturated-job slots: any packed job may saturate, so a slot holds the
+    // widest packed job's two band rows
+    if (njobs > 0) {
+        b->flag_slot = 4u * ((uint32_t)plan.wmax + 1u);
+        b->flag_threads = (uint32_t)std::min<uint64_t>(8192, std::max<uint64_t>(128, (njobs / 64 + 127) / 128 * 128));
+        HIP_TRY(b->flag_work.alloc((uint64_t)b->flag_threads * b->flag_slot));
     }
     HIP_TRY(b->score.alloc(njobs ? njobs : 1));
     HIP_TRY(b->q_end.alloc(njobs ? njobs : 1));
     HIP_TRY(b->s_end.alloc(njobs ? njobs : 1));
-    if (int r = upload(b->cells, host_cells.data(), njobs, st)) return r;
     HIP_TRY(hipStreamSynchronize(st));
     lap("uploads");
     *out = b.release();
@@ -922,7 +922,7 @@ static int enqueue_step(saber_hip_band_batch* b, hipEvent_t ev_begin, hipEvent_t
     a.pairs = b->pairs.p;
     a.npairs = (uint32_t)(b->pairs.n / 2);
     a.blocks = b->blocks.p;
-    a.nchunks = (uint32_t)b->host_blocks.size();
+    a.nchunks = (uint32_t)b->nchunks;
     a.next_chunk = b->next_chunk.p;
     a.pair_table = b->pair_table.p;
     a.jobs = b->jobs.p;
@@ -939,13 +939,13 @@ static int enqueue_step(saber_hip_band_batch* b, hipEvent_t ev_begin, hipEvent_t
     a.nflagged = b->nflagged.p;
     if (ev_begin) HIP_TRY(hipEventRecord(ev_begin, st));
     // one persistent workgroup per CU (fewer when there is little work)
-    const uint32_t wg = (uint32_t)std::min<uint64_t>((uint64_t)b->ctx->cus, (b->host_blocks.size() + 11) / 12);
+    const uint32_t wg = (uint32_t)std::min<uint64_t>((uint64_t)b->ctx->cus, (b->nchunks + 11) / 12);
     if (a.nchunks) HIP_TRY(launch_band_multi(a, wg, b->has_bias, b->ends, st));
     if (ev_end) HIP_TRY(hipEventRecord(ev_end, st));
-    if (!b->generic_static.empty()) {
+    if (b->ngeneric) {
         GenericArgs g = generic_args(b);
         g.ids = b->gen_ids.p;
-        g.nids = (uint32_t)b->generic_static.size();
+        g.nids = b->ngeneric;
         g.work = b->gen_work.p;
         g.work_off = b->gen_off.p;
         HIP_TRY(launch_generic(g, st));
@@ -1084,8 +1084,11 @@ int saber_hip_band_batch_time(saber_hip_band_batch* b, int warmup, int iters, do
         if (variant_cells) {
             std::vector<uint32_t> pairs(b->pairs.n);
             if (!pairs.empty()) HIP_TRY(hipMemcpy(pairs.data(), b->pairs.p, 4 * pairs.size(), hipMemcpyDeviceToHost));
+            std::vector<BandBlock> blocks(b->nchunks);
+            if (!blocks.empty())
+                HIP_TRY(hipMemcpy(blocks.data(), b->blocks.p, sizeof(BandBlock) * blocks.size(), hipMemcpyDeviceToHost));
             for (int v = 0; v < cap; ++v) variant_cells[v] = 0;
-            for (const BandBlock& k : b->host_blocks) {
+            for (const BandBlock& k : blocks) {
                 const int v = 2 * (int)k.shape + (b->has_bias ? 1 : 0);
                 if (v >= cap) continue;
                 for (uint64_t x = 2 * (uint64_t)k.pair_off; x < 2 * ((uint64_t)k.pair_off + k.npairs); ++x)

@@ -111,6 +111,14 @@ struct ScanCacheDeleter {
     void operator()(ScanDbCache* p) const { scan_cache_free(p); }
 };
 
+// Device scratch of the band planner (band_plan.hip), kept across batches
+// and grown to the largest.
+struct BandPlanScratch;
+void plan_scratch_free(BandPlanScratch* p);
+struct PlanScratchDeleter {
+    void operator()(BandPlanScratch* p) const { plan_scratch_free(p); }
+};
+
 // The banded part of score_windows (saber_hip_score_regions) with the band
 // cells of the banded plans' passes summed into *banded_cells (may be null)
 // and regions with skip[i] != 0 (may be null) left unscored (score 0).
@@ -157,4 +165,5 @@ struct saber_hip_ctx {
     std::vector<saber::Shape> shapes;
     int latency_rows = -1;  // -1: kLatencyShare rule; 0: off; else a row count
     std::unique_ptr<saber::ScanDbCache, saber::ScanCacheDeleter> scan;
+    std::unique_ptr<saber::BandPlanScratch, saber::PlanScratchDeleter> plan;
 };
