@@ -29,6 +29,7 @@
 #include "common.h"
 #include "ctx.h"
 #include "band_plan.h"
+#include "host_par.h"
 
 using namespace saber;
 
@@ -3020,87 +3021,156 @@ int saber::score_regions_impl(saber_hip_ctx* c, const saber_hip_seqset* queries,
     if (int r = check_seqset(queries, "queries")) return r;
     const bool two_stage = narrow_band > 0 && narrow_band < full_band;
     if (two_stage && n && !thr) return fail(SABER_HIP_EINVAL, "thr is null");
-    std::vector<saber_hip_band_job> narrow, full;
-    std::vector<uint64_t> narrow_of, full_of;
-    std::vector<uint8_t> banded_job;   // full-pass job k comes from a banded plan
-    for (uint64_t i = 0; i < n; ++i) {
-        if (qidx[i] >= queries->count) return fail(SABER_HIP_EINVAL, "region query index out of range");
-        if (rescored) rescored[i] = 0;
-        if (skip && skip[i]) {
-            out[i] = 0;
-            continue;
+    static const bool trace = std::getenv("SABER_HIP_TRACE_HOST") != nullptr;
+    auto tp = std::chrono::steady_clock::now();
+    auto lap = [&](const char* what) {
+        if (!trace) return;
+        const auto t1 = std::chrono::steady_clock::now();
+        std::fprintf(stderr, "#   score_regions %s %.1f ms\n", what,
+                     std::chrono::duration<double, std::milli>(t1 - tp).count());
+        tp = t1;
+    };
+    // Jobs in region order, built in blocks of regions on the host threads:
+    // narrow-band jobs of banded plans (two-stage), full-band jobs of the
+    // rest. The first bad region (in region order) decides the error.
+    constexpr uint64_t kBlk = 1 << 15;
+    const uint64_t nblk = (n + kBlk - 1) / kBlk;
+    std::vector<uint64_t> n_narrow(nblk + 1, 0), n_full(nblk + 1, 0);
+    std::vector<uint64_t> bad_at(nblk, ~0ull);
+    std::vector<int> bad_kind(nblk, 0);
+    // 0 skipped, 1 narrow pass, 2 full pass (banded plan), 3 full pass (whole rectangle)
+    auto kind_of = [&](uint64_t i) -> int {
+        if (skip && skip[i]) return 0;
+        if (!plans[i].banded) return 3;
+        return two_stage ? 1 : 2;
+    };
+    parallel_for(nblk, [&](uint64_t blk) {
+        const uint64_t i1 = std::min(n, (blk + 1) * kBlk);
+        uint64_t a = 0, f = 0;
+        for (uint64_t i = blk * kBlk; i < i1; ++i) {
+            if (qidx[i] >= queries->count) {
+                bad_at[blk] = i;
+                bad_kind[blk] = 1;
+                return;
+            }
+            if (rescored) rescored[i] = 0;
+            const int k = kind_of(i);
+            if (k == 0) {
+                out[i] = 0;
+                continue;
+            }
+            if (plans[i].hi < plans[i].lo || regs[i].subject > 0xFFFFFFFFull) {
+                bad_at[blk] = i;
+                bad_kind[blk] = 2;
+                return;
+            }
+            (k == 1 ? a : f) += 1;
         }
-        const saber_hip_region& r = regs[i];
-        const saber_hip_plan& p = plans[i];
-        if (p.hi < p.lo || r.subject > 0xFFFFFFFFull) return fail(SABER_HIP_EINVAL, "bad plan");
-        const int64_t m = (int64_t)(queries->offsets[qidx[i] + 1] - queries->offsets[qidx[i]]);
-        const int64_t nw = (int64_t)(p.hi - p.lo);
-        saber_hip_band_job j{qidx[i], (uint32_t)r.subject, (uint32_t)p.lo, (uint32_t)p.hi, 0, 0};
-        if (p.banded) {
-            if (two_stage) {
+        n_narrow[blk + 1] = a;
+        n_full[blk + 1] = f;
+    });
+    for (uint64_t blk = 0; blk < nblk; ++blk)
+        if (bad_kind[blk] == 1) return fail(SABER_HIP_EINVAL, "region query index out of range");
+        else if (bad_kind[blk] == 2) return fail(SABER_HIP_EINVAL, "bad plan");
+    for (uint64_t blk = 0; blk < nblk; ++blk) {
+        n_narrow[blk + 1] += n_narrow[blk];
+        n_full[blk + 1] += n_full[blk];
+    }
+    // the full pass: full-band jobs, then the narrow jobs rescored
+    const uint64_t nn = n_narrow[nblk], nf0 = n_full[nblk];
+    auto narrow = raw_array<saber_hip_band_job>(nn), full = raw_array<saber_hip_band_job>(nf0 + nn);
+    auto narrow_of = raw_array<uint64_t>(nn), full_of = raw_array<uint64_t>(nf0 + nn);
+    auto banded_job = raw_array<uint8_t>(nf0 + nn);  // full-pass job k comes from a banded plan
+    uint64_t nfull = nf0;
+    parallel_for(nblk, [&](uint64_t blk) {
+        uint64_t a = n_narrow[blk], f = n_full[blk];
+        for (uint64_t i = blk * kBlk, i1 = std::min(n, (blk + 1) * kBlk); i < i1; ++i) {
+            const int k = kind_of(i);
+            if (k == 0) continue;
+            const saber_hip_region& r = regs[i];
+            const saber_hip_plan& p = plans[i];
+            const int64_t m = (int64_t)(queries->offsets[qidx[i] + 1] - queries->offsets[qidx[i]]);
+            const int64_t nw = (int64_t)(p.hi - p.lo);
+            saber_hip_band_job j{qidx[i], (uint32_t)r.subject, (uint32_t)p.lo, (uint32_t)p.hi, 0, 0};
+            if (k == 1) {
                 // narrow band around the region's own diagonals, clipped
                 j.band_lo = (int32_t)std::max<int64_t>(r.d_min - narrow_band - (int64_t)p.lo, -(m - 1));
                 j.band_hi = (int32_t)std::min<int64_t>(r.d_max + narrow_band - (int64_t)p.lo, nw - 1);
-                narrow.push_back(j);
-                narrow_of.push_back(i);
-            } else {
+                narrow[a] = j;
+                narrow_of[a++] = i;
+                continue;
+            }
+            if (k == 2) {
                 j.band_lo = (int32_t)p.band_lo;
                 j.band_hi = (int32_t)p.band_hi;
-                full.push_back(j);
-                full_of.push_back(i);
-                banded_job.push_back(1);
-            }
-        } else {
-            // the whole window rectangle: the full band of query x window
-            j.band_lo = (int32_t)(-(m - 1));
-            j.band_hi = (int32_t)(nw - 1);
-            full.push_back(j);
-            full_of.push_back(i);
-            banded_job.push_back(0);
-        }
-    }
-    std::vector<uint64_t> cells;
-    uint64_t* cells_p = nullptr;
-    if (!narrow.empty()) {
-        std::vector<int32_t> sc(narrow.size());
-        if (banded_cells) {
-            cells.assign(narrow.size(), 0);
-            cells_p = cells.data();
-        }
-        if (int r = saber_hip_score_bands(c, queries, subjects, narrow.data(), narrow.size(), scheme, sc.data(),
-                                          nullptr, nullptr, cells_p))
-            return r;
-        if (banded_cells)
-            for (uint64_t x : cells) *banded_cells += x;
-        for (size_t k = 0; k < narrow.size(); ++k) {
-            const uint64_t i = narrow_of[k];
-            if ((double)sc[k] >= thr[i]) {
-                saber_hip_band_job j = narrow[k];
-                j.band_lo = (int32_t)plans[i].band_lo;
-                j.band_hi = (int32_t)plans[i].band_hi;
-                full.push_back(j);
-                full_of.push_back(i);
-                banded_job.push_back(1);
-                if (rescored) rescored[i] = 1;
             } else {
-                out[i] = sc[k];
+                // the whole window rectangle: the full band of query x window
+                j.band_lo = (int32_t)(-(m - 1));
+                j.band_hi = (int32_t)(nw - 1);
             }
+            full[f] = j;
+            full_of[f] = i;
+            banded_job[f++] = k == 2 ? 1 : 0;
         }
+    });
+    lap("jobs");
+    if (nn) {
+        auto sc = raw_array<int32_t>(nn);
+        auto cells = raw_array<uint64_t>(banded_cells ? nn : 0);
+        if (int r = saber_hip_score_bands(c, queries, subjects, narrow.get(), nn, scheme, sc.get(), nullptr, nullptr,
+                                          banded_cells ? cells.get() : nullptr))
+            return r;
+        lap("narrow pass");
+        if (banded_cells)
+            for (uint64_t k = 0; k < nn; ++k) *banded_cells += cells[k];
+        // narrow scores at or above the region's cut are rescored over the
+        // full band (appended to the full pass in narrow-job order)
+        const uint64_t nb2 = (nn + kBlk - 1) / kBlk;
+        std::vector<uint64_t> n_res(nb2 + 1, 0);
+        parallel_for(nb2, [&](uint64_t blk) {
+            uint64_t x = 0;
+            for (uint64_t k = blk * kBlk, k1 = std::min(nn, (blk + 1) * kBlk); k < k1; ++k) {
+                const uint64_t i = narrow_of[k];
+                if ((double)sc[k] >= thr[i]) {
+                    ++x;
+                    if (rescored) rescored[i] = 1;
+                } else {
+                    out[i] = sc[k];
+                }
+            }
+            n_res[blk + 1] = x;
+        });
+        for (uint64_t blk = 0; blk < nb2; ++blk) n_res[blk + 1] += n_res[blk];
+        parallel_for(nb2, [&](uint64_t blk) {
+            uint64_t f = nf0 + n_res[blk];
+            for (uint64_t k = blk * kBlk, k1 = std::min(nn, (blk + 1) * kBlk); k < k1; ++k) {
+                const uint64_t i = narrow_of[k];
+                if ((double)sc[k] >= thr[i]) {
+                    saber_hip_band_job j = narrow[k];
+                    j.band_lo = (int32_t)plans[i].band_lo;
+                    j.band_hi = (int32_t)plans[i].band_hi;
+                    full[f] = j;
+                    banded_job[f] = 1;
+                    full_of[f++] = i;
+                }
+            }
+        });
+        nfull = nf0 + n_res[nb2];
+        lap("threshold");
     }
-    if (!full.empty()) {
-        std::vector<int32_t> sc(full.size());
-        cells_p = nullptr;
-        if (banded_cells) {
-            cells.assign(full.size(), 0);
-            cells_p = cells.data();
-        }
-        if (int r = saber_hip_score_bands(c, queries, subjects, full.data(), full.size(), scheme, sc.data(), nullptr,
-                                          nullptr, cells_p))
+    if (nfull) {
+        auto sc = raw_array<int32_t>(nfull);
+        auto cells = raw_array<uint64_t>(banded_cells ? nfull : 0);
+        if (int r = saber_hip_score_bands(c, queries, subjects, full.get(), nfull, scheme, sc.get(), nullptr, nullptr,
+                                          banded_cells ? cells.get() : nullptr))
             return r;
         if (banded_cells)
-            for (size_t k = 0; k < full.size(); ++k)
+            for (uint64_t k = 0; k < nfull; ++k)
                 if (banded_job[k]) *banded_cells += cells[k];
-        for (size_t k = 0; k < full.size(); ++k) out[full_of[k]] = sc[k];
+        parallel_for((nfull + kBlk - 1) / kBlk, [&](uint64_t blk) {
+            for (uint64_t k = blk * kBlk, k1 = std::min(nfull, (blk + 1) * kBlk); k < k1; ++k) out[full_of[k]] = sc[k];
+        });
+        lap("full pass");
     }
     return SABER_HIP_OK;
 }

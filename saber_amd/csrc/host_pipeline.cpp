@@ -7,7 +7,6 @@
 #include <algorithm>
 #include <cstdint>
 #include <limits>
-#include <map>
 #include <vector>
 
 #include "../../include/saber_hip.h"
@@ -129,19 +128,26 @@ int saber_hip_plan_windows(const saber_hip_region* regs, uint64_t n, uint64_t ql
     if (!p || (n && (!regs || !subject_len || !out))) return SABER_HIP_EINVAL;
     const int64_t q = (int64_t)qlen;
     // Chains of strong regions per subject (single linkage within kBandBridge).
+    // The regions are grouped by subject through one index sort (subject,
+    // then region index), without a per-subject container.
     std::vector<std::pair<int64_t, int64_t>> span(n);
-    std::map<uint64_t, std::vector<uint64_t>> by_subject;
+    std::vector<uint64_t> order(n), idx;
     for (uint64_t i = 0; i < n; ++i) {
         span[i] = {regs[i].d_min, regs[i].d_max};
-        by_subject[regs[i].subject].push_back(i);
+        order[i] = i;
     }
-    for (auto& kv : by_subject) {
-        std::vector<uint64_t>& idx = kv.second;
+    std::sort(order.begin(), order.end(), [&](uint64_t a, uint64_t b) {
+        return regs[a].subject != regs[b].subject ? regs[a].subject < regs[b].subject : a < b;
+    });
+    for (uint64_t g0 = 0, g1 = 0; g0 < n; g0 = g1) {
+        g1 = g0 + 1;
+        while (g1 < n && regs[order[g1]].subject == regs[order[g0]].subject) ++g1;
         int32_t top = 0;
-        for (uint64_t i : idx) top = std::max(top, regs[i].best);
+        for (uint64_t k = g0; k < g1; ++k) top = std::max(top, regs[order[k]].best);
         const int32_t strong = std::max(2 * p->hsp_threshold, top / 4);
-        idx.erase(std::remove_if(idx.begin(), idx.end(), [&](uint64_t i) { return regs[i].best < strong; }),
-                  idx.end());
+        idx.clear();
+        for (uint64_t k = g0; k < g1; ++k)
+            if (regs[order[k]].best >= strong) idx.push_back(order[k]);
         std::sort(idx.begin(), idx.end(), [&](uint64_t a, uint64_t b) {
             if (regs[a].d_min != regs[b].d_min) return regs[a].d_min < regs[b].d_min;
             if (regs[a].d_max != regs[b].d_max) return regs[a].d_max < regs[b].d_max;

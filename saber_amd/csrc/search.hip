@@ -56,6 +56,7 @@
 #include "../../include/saber_hip.h"
 #include "common.h"
 #include "ctx.h"
+#include "host_par.h"
 
 using namespace saber;
 
@@ -658,9 +659,10 @@ struct saber_hip_search {
     uint32_t nq = 0;
     // the capped regions of all queries with their plans (feature export)
     std::vector<saber_hip_region> flat;
-    std::vector<saber_hip_plan> plans;
-    std::vector<uint32_t> qidx;
-    std::vector<uint64_t> qlen, slen;
+    host_vec<saber_hip_plan> plans;
+    host_vec<uint32_t> qidx;
+    std::vector<uint64_t> qlen;
+    host_vec<uint64_t> slen;
     std::vector<int32_t> rthr;
     uint64_t db_gen = 0;
     std::vector<std::vector<saber_hip_hit>> hits;
@@ -757,26 +759,6 @@ uint64_t auto_occ_cap(uint64_t total, uint64_t ncodes, uint64_t db_residues, uin
         floor = (uint64_t)std::ceil((double)kOccCapFloor * (double)db_residues / (double)ref_residues);
     const uint64_t c = (uint64_t)std::ceil((double)total / (double)std::max<uint64_t>(ncodes, 1) * 64.0);
     return std::max(c, floor);
-}
-
-template <class F>
-void parallel_for(uint64_t n, F&& f) {
-    const unsigned nt = std::max(1u, std::min<unsigned>(std::thread::hardware_concurrency(), 16u));
-    if (n < 64 || nt == 1) {
-        for (uint64_t i = 0; i < n; ++i) f(i);
-        return;
-    }
-    std::vector<std::thread> th;
-    std::atomic<uint64_t> next{0};
-    for (unsigned t = 0; t < nt; ++t)
-        th.emplace_back([&]() {
-            for (;;) {
-                const uint64_t i = next.fetch_add(1);
-                if (i >= n) break;
-                f(i);
-            }
-        });
-    for (auto& x : th) x.join();
 }
 
 int grid_for(uint64_t n, uint32_t block) {
@@ -927,12 +909,10 @@ struct HitCand {
 // query compositions from the device query codes, the windows from the
 // database.
 int region_features_dev(saber_hip_ctx* c, hipStream_t st, const uint64_t* d_qoff, const uint8_t* d_qres, uint32_t nq,
-                        const std::vector<saber_hip_region>& regs, const std::vector<saber_hip_plan>& plans,
-                        const std::vector<uint32_t>& qidx, const std::vector<uint64_t>& qlen,
-                        const std::vector<uint64_t>& slen, const std::vector<int32_t>& rthr, const float* mean,
-                        const float* inv_std, DevBuf<__bf16>& out) {
-    const uint64_t n = regs.size();
-    std::vector<SearchFeatRegion> rec(n);
+                        const saber_hip_region* regs, uint64_t n, const saber_hip_plan* plans,
+                        const uint32_t* qidx, const uint64_t* qlen, const uint64_t* slen, const int32_t* rthr,
+                        const float* mean, const float* inv_std, DevBuf<__bf16>& out) {
+    auto rec = raw_array<SearchFeatRegion>(n);
     parallel_for((n + 4095) / 4096, [&](uint64_t blk) {
         for (uint64_t i = blk * 4096; i < std::min<uint64_t>(n, (blk + 1) * 4096); ++i) {
             const saber_hip_region& r = regs[i];
@@ -954,9 +934,17 @@ int region_features_dev(saber_hip_ctx* c, hipStream_t st, const uint64_t* d_qoff
             x.banded = pl.banded ? 1u : 0u;
         }
     });
+    static const bool trace = std::getenv("SABER_HIP_TRACE_HOST") != nullptr;
+    const auto t0 = std::chrono::steady_clock::now();
     DevBuf<SearchFeatRegion> d_rec;
     DevBuf<float> d_mean, d_inv, d_qcomp;
-    if (int r = upload(d_rec, rec.data(), n, st)) return r;
+    if (int r = upload(d_rec, rec.get(), n, st)) return r;
+    if (trace) {
+        HIP_TRY(hipStreamSynchronize(st));
+        std::fprintf(stderr, "#   features: %llu records of %zu bytes uploaded in %.1f ms\n", (unsigned long long)n,
+                     sizeof(SearchFeatRegion),
+                     std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+    }
     if (mean && inv_std) {
         if (int r = upload(d_mean, mean, 64, st)) return r;
         if (int r = upload(d_inv, inv_std, 64, st)) return r;
@@ -1524,10 +1512,11 @@ int saber_hip_search_run(saber_hip_ctx* c, const saber_hip_seqset* queries, cons
 
     // ---- plans (plan_windows) ----
     t0 = Clock::now();
-    std::vector<saber_hip_plan> plans(nreg);
-    std::vector<uint32_t> qidx(nreg);
-    std::vector<double> thr(nreg, 0.0);
-    std::vector<uint64_t> slen_of(nreg);
+    // every entry is written below (per query, by plan_windows)
+    host_vec<saber_hip_plan> plans(nreg);
+    host_vec<uint32_t> qidx(nreg);
+    host_vec<double> thr(nreg);
+    host_vec<uint64_t> slen_of(nreg);
     saber_hip_plan_params pp{P->window_slack, P->score_band, P->hsp_threshold, 0};
     std::atomic<int> plan_err{0};
     parallel_for(nq, [&](uint64_t q) {
@@ -1558,11 +1547,15 @@ int saber_hip_search_run(saber_hip_ctx* c, const saber_hip_seqset* queries, cons
         if (mlp_context(prefilter->model) != c) return fail(SABER_HIP_EINVAL, "prefilter model of another context");
         DevBuf<__bf16> X;
         DevBuf<float> logits;
-        if (int r = region_features_dev(c, st, d_qoff.p, d_q.p + kGuard, nq, flat, plans, qidx, qlen, slen_of, rthr_q,
-                                        prefilter->mean, prefilter->inv_std, X))
+        tl = Clock::now();
+        if (int r = region_features_dev(c, st, d_qoff.p, d_q.p + kGuard, nq, flat.data(), nreg, plans.data(),
+                                        qidx.data(), qlen.data(), slen_of.data(), rthr_q.data(), prefilter->mean,
+                                        prefilter->inv_std, X))
             return r;
+        lap("prefilter features");
         HIP_TRY(logits.alloc(nreg));
         if (int r = mlp_forward_device(prefilter->model, X.p, nreg, logits.p, st)) return r;
+        lap("prefilter forward");
         std::vector<float> lg(nreg);
         HIP_TRY(hipMemcpyAsync(lg.data(), logits.p, sizeof(float) * nreg, hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
@@ -1570,6 +1563,7 @@ int saber_hip_search_run(saber_hip_ctx* c, const saber_hip_seqset* queries, cons
             pruned[i] = lg[i] < prefilter->threshold ? 1 : 0;
             ctr.regions_pruned += pruned[i];
         }
+        lap("prefilter logits");
     }
     tm.prefilter = ms_since(t0);
 
@@ -1925,8 +1919,9 @@ int saber_hip_search_features(saber_hip_search* s, const float* mean, const floa
     if (int r = upload_codes(d_q, s->qres.data(), qtotal, st)) return r;
     if (int r = upload(d_qoff, s->qoff.data(), s->qoff.size(), st)) return r;
     DevBuf<__bf16> X;
-    if (int r = region_features_dev(c, st, d_qoff.p, d_q.p + kGuard, s->nq, s->flat, s->plans, s->qidx, s->qlen,
-                                    s->slen, s->rthr, mean, inv_std, X))
+    if (int r = region_features_dev(c, st, d_qoff.p, d_q.p + kGuard, s->nq, s->flat.data(), s->flat.size(),
+                                    s->plans.data(), s->qidx.data(), s->qlen.data(), s->slen.data(), s->rthr.data(),
+                                    mean, inv_std, X))
         return r;
     HIP_TRY(hipMemcpyAsync(out, X.p, 2 * 64 * n, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));

@@ -130,18 +130,29 @@ REGION_DTYPE = np.dtype([("subject", "<u8"), ("d_min", "<i8"), ("d_max", "<i8"),
                          ("band_lo", "<i8"), ("band_hi", "<i8"), ("banded", "<i4"), ("flags", "<u4"),
                          ("evalue", "<f8")])
 assert REGION_DTYPE.itemsize == C.sizeof(_ffi.SearchRegion)
+# saber_hip_hit as a numpy record (hits are read into one array per query)
+HIT_DTYPE = np.dtype([("subject", "<u8"), ("score", "<i4"), ("exact_in_window", "<u4"), ("bitscore", "<f8"),
+                      ("evalue", "<f8"), ("q_start", "<u8"), ("q_end", "<u8"), ("s_start", "<u8"), ("s_end", "<u8"),
+                      ("columns", "<u8"), ("matches", "<u8"), ("mismatches", "<u8"), ("positives", "<u8"),
+                      ("gap_opens", "<u8"), ("gap_columns", "<u8"), ("q_len", "<u8"), ("s_len", "<u8"),
+                      ("op_off", "<u8"), ("nops", "<u4"), ("reserved", "<u4")])
+assert HIT_DTYPE.itemsize == C.sizeof(_ffi.Hit)
 
 
 @dataclass
 class SearchResult:
     hits: list                  # per query: list of Hit
-    regions: list               # per query: REGION_DTYPE array
+    regions: list               # per query: REGION_DTYPE array (None unless asked for)
     counters: dict
     timing: dict = field(default_factory=dict)
 
 
-def run(ctx, qres, qoff, scheme, karlin, seeds=None, params=None, prefilter=None, keep_handle=False):
-    """Search queries (alignment codes, offsets) against ctx's uploaded database."""
+def run(ctx, qres, qoff, scheme, karlin, seeds=None, params=None, prefilter=None, keep_handle=False, regions=None):
+    """Search queries (alignment codes, offsets) against ctx's uploaded database.
+    regions: also return every query's capped regions (REGION_DTYPE arrays;
+    default: when keep_handle is set); otherwise result.regions is None."""
+    if regions is None:
+        regions = keep_handle
     seeds = seeds or ScanSeeds.default()
     params = params or default_params()
     qres = np.ascontiguousarray(qres, np.uint8)
@@ -161,34 +172,36 @@ def run(ctx, qres, qoff, scheme, karlin, seeds=None, params=None, prefilter=None
                                                 C.byref(params), C.byref(pf) if pf is not None else None,
                                                 C.byref(h)))
     L = _ffi.lib()
+    hit_ptr = C.POINTER(_ffi.Hit)
     try:
-        hits, regions = [], []
+        hits, regs = [], [] if regions else None
         for q in range(nq):
             n = L.saber_hip_search_hit_count(h, q)
-            arr = (_ffi.Hit * max(n, 1))()
-            _ffi.check(L.saber_hip_search_hits(h, q, arr))
-            no = L.saber_hip_search_op_count(h, q)
-            oc = np.zeros(max(no, 1), np.uint8)
-            ol = np.zeros(max(no, 1), np.uint32)
-            _ffi.check(L.saber_hip_search_ops(h, q, _ffi.ptr(oc, C.c_uint8), _ffi.ptr(ol, C.c_uint32)))
             hl = []
-            for k in range(n):
-                x = arr[k]
-                ops = [(int(oc[x.op_off + t]), int(ol[x.op_off + t])) for t in range(x.nops)]
-                hl.append(Hit(x.subject, x.score, x.bitscore, x.evalue, x.q_start, x.q_end, x.s_start, x.s_end,
-                              x.columns, x.matches, x.mismatches, x.positives, x.gap_opens, x.gap_columns, x.q_len,
-                              x.s_len, ops, bool(x.exact_in_window)))
+            if n:
+                arr = np.zeros(n, HIT_DTYPE)
+                _ffi.check(L.saber_hip_search_hits(h, q, arr.ctypes.data_as(hit_ptr)))
+                no = L.saber_hip_search_op_count(h, q)
+                oc = np.zeros(max(no, 1), np.uint8)
+                ol = np.zeros(max(no, 1), np.uint32)
+                _ffi.check(L.saber_hip_search_ops(h, q, _ffi.ptr(oc, C.c_uint8), _ffi.ptr(ol, C.c_uint32)))
+                codes, lens = oc.tolist(), ol.tolist()
+                for (subject, score, exact, bits, ev, q0, q1, s0, s1, cols, mat, mis, pos, gopen, gcols, ql, sl, oo,
+                     nops, _) in arr.tolist():
+                    hl.append(Hit(subject, score, bits, ev, q0, q1, s0, s1, cols, mat, mis, pos, gopen, gcols, ql, sl,
+                                  list(zip(codes[oo:oo + nops], lens[oo:oo + nops])), bool(exact)))
             hits.append(hl)
-            nr = L.saber_hip_search_region_count(h, q)
-            r = np.zeros(nr, REGION_DTYPE)
-            if nr:
-                _ffi.check(L.saber_hip_search_regions(h, q, r.ctypes.data_as(C.POINTER(_ffi.SearchRegion))))
-            regions.append(r)
+            if regions:
+                nr = L.saber_hip_search_region_count(h, q)
+                r = np.zeros(nr, REGION_DTYPE)
+                if nr:
+                    _ffi.check(L.saber_hip_search_regions(h, q, r.ctypes.data_as(C.POINTER(_ffi.SearchRegion))))
+                regs.append(r)
         c = _ffi.SearchCounters()
         _ffi.check(L.saber_hip_search_get_counters(h, C.byref(c)))
         t = _ffi.SearchTiming()
         _ffi.check(L.saber_hip_search_get_timing(h, C.byref(t)))
-        res = SearchResult(hits, regions, {k: int(getattr(c, k)) for k in _ffi.SEARCH_COUNTERS},
+        res = SearchResult(hits, regs, {k: int(getattr(c, k)) for k in _ffi.SEARCH_COUNTERS},
                            {k: float(getattr(t, k)) for k in _ffi.SEARCH_STAGES})
         if keep_handle:
             res.handle = h
@@ -204,9 +217,6 @@ def destroy(handle):
         _ffi.lib().saber_hip_search_destroy(handle)
 
 
-__all__ = ["ScanSeeds", "default_params", "composition_bias", "run", "Hit", "SearchResult", "alphabet_map",
-           "full_alphabet", "MURPHY10_GROUPS", "DS16", "REGION_DTYPE", "destroy", "features", "close", "hit_pairs",
-           "train_prefilter", "prefilter_arg"]
 
 
 
@@ -313,3 +323,8 @@ def train_prefilter(ctx, train_sets, valid_set, scheme, karlin, loss_target=0.00
 def prefilter_arg(pf):
     """The run(..., prefilter=) argument of a train_prefilter result."""
     return {"model": pf["model"], "mean": pf["mean"], "inv_std": pf["inv_std"], "threshold": pf["threshold"]}
+
+
+__all__ = ["ScanSeeds", "default_params", "composition_bias", "run", "Hit", "SearchResult", "alphabet_map",
+           "full_alphabet", "MURPHY10_GROUPS", "DS16", "REGION_DTYPE", "destroy", "features", "close", "hit_pairs",
+           "train_prefilter", "prefilter_arg"]

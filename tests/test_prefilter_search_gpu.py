@@ -70,8 +70,8 @@ def test_trained_prefilter_gain_and_loss(ctx, setup):
     print(f"# trained on {pf['train_regions']} regions ({pf['positives']} positive), threshold "
           f"{pf['threshold']:.3f}, validation kept {pf['valid_kept']:.3f}", flush=True)
     ctx.upload_db(test["sres"], test["soff"])
-    plain = search.run(ctx, test["qres"], test["qoff"], sch, k)
-    filt = search.run(ctx, test["qres"], test["qoff"], sch, k, prefilter=search.prefilter_arg(pf))
+    plain = search.run(ctx, test["qres"], test["qoff"], sch, k, regions=True)
+    filt = search.run(ctx, test["qres"], test["qoff"], sch, k, prefilter=search.prefilter_arg(pf), regions=True)
     pp, fp = search.hit_pairs(plain), search.hit_pairs(filt)
     loss = len(pp - fp) / max(1, len(pp))
     kept = 1 - filt.counters["regions_pruned"] / max(1, filt.counters["regions"])

@@ -48,7 +48,7 @@ def run_both(ctx, d, seeds=None, **kw):
                  prefilter_band="score_prefilter_band")
     params = search.default_params(**{names[a]: (int(v) if not isinstance(v, float) else v) for a, v in kw.items()})
     print(f"# search {len(d['qoff']) - 1} x {ns} {kw}", flush=True)
-    got = search.run(ctx, d["qres"], d["qoff"], S.Scheme(BLOSUM62, 11, 1), k, seeds=seeds, params=params)
+    got = search.run(ctx, d["qres"], d["qoff"], S.Scheme(BLOSUM62, 11, 1), k, seeds=seeds, params=params, regions=True)
     print("#   gpu stages (ms):", {a: round(b, 2) for a, b in got.timing.items()}, flush=True)
     want, wctr = OS.search(d["qres"], d["qoff"], d["sres"], d["soff"], BLOSUM62, 11, 1, st, seeds.alpha_map,
                            seeds.alpha_size, seeds.shapes, **kw)
@@ -128,7 +128,7 @@ def test_many_survivors_regions_and_counters(ctx):
     k = stats.gapped_protein(62, 11, 1)
     ctx.upload_db(d["sres"], d["soff"])
     params = search.default_params(hit_filter_min_ids=0, hit_filter_exact_weight=0)
-    got = search.run(ctx, d["qres"], d["qoff"], S.Scheme(BLOSUM62, 11, 1), k, seeds=seeds, params=params)
+    got = search.run(ctx, d["qres"], d["qoff"], S.Scheme(BLOSUM62, 11, 1), k, seeds=seeds, params=params, regions=True)
     nq, ns = len(d["qoff"]) - 1, len(d["soff"]) - 1
     st = stats.SearchStatistics(k, int(d["soff"][-1]), ns, stats.PER_SUBJECT)
     re = 10.0
