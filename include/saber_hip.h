@@ -642,6 +642,14 @@ int saber_hip_search_hits(const saber_hip_search* s, uint32_t query, saber_hip_h
 /* The query's operation list (all its hits), op_count entries. */
 uint64_t saber_hip_search_op_count(const saber_hip_search* s, uint32_t query);
 int saber_hip_search_ops(const saber_hip_search* s, uint32_t query, uint8_t* codes, uint32_t* lens);
+/* Every query's hits and operations in one call (host code; no counterpart
+ * in the reference, for a caller that reads a whole result set): query q's
+ * hits are out[hit_off[q] .. hit_off[q + 1]) and its operation list starts at
+ * codes / lens + op_base[q] (the hits' op_off are relative to it); hit_off and
+ * op_base hold nq + 1 offsets. Null arrays are skipped, so a first call with
+ * the offsets alone sizes the others. */
+int saber_hip_search_all_hits(const saber_hip_search* s, uint64_t* hit_off, saber_hip_hit* out, uint64_t* op_base,
+                              uint8_t* codes, uint32_t* lens);
 uint64_t saber_hip_search_region_count(const saber_hip_search* s, uint32_t query);
 int saber_hip_search_regions(const saber_hip_search* s, uint32_t query, saber_hip_search_region* out);
 /* Prefilter features (bf16 [n][64], the feature list of features.hip) of

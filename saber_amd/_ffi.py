@@ -174,6 +174,8 @@ _SIGS = {
     "saber_hip_search_hits": (C.c_int, [P, C.c_uint32, C.POINTER(Hit)]),
     "saber_hip_search_op_count": (C.c_uint64, [P, C.c_uint32]),
     "saber_hip_search_ops": (C.c_int, [P, C.c_uint32, C.POINTER(C.c_uint8), C.POINTER(C.c_uint32)]),
+    "saber_hip_search_all_hits": (C.c_int, [P, C.POINTER(C.c_uint64), C.POINTER(Hit), C.POINTER(C.c_uint64),
+                                            C.POINTER(C.c_uint8), C.POINTER(C.c_uint32)]),
     "saber_hip_search_region_count": (C.c_uint64, [P, C.c_uint32]),
     "saber_hip_search_regions": (C.c_int, [P, C.c_uint32, C.POINTER(SearchRegion)]),
     "saber_hip_search_features": (C.c_int, [P, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_uint16)]),

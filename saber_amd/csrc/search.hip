@@ -658,7 +658,7 @@ void scan_cache_free(ScanDbCache* p) { delete p; }
 struct saber_hip_search {
     uint32_t nq = 0;
     // the capped regions of all queries with their plans (feature export)
-    std::vector<saber_hip_region> flat;
+    host_vec<saber_hip_region> flat;
     host_vec<saber_hip_plan> plans;
     host_vec<uint32_t> qidx;
     std::vector<uint64_t> qlen;
@@ -1391,7 +1391,7 @@ int saber_hip_search_run(saber_hip_ctx* c, const saber_hip_seqset* queries, cons
         std::fprintf(stderr, "#   search %s %.1f ms\n", what, ms_since(tl));
         tl = Clock::now();
     };
-    std::vector<saber_hip_region> regs_all;
+    host_vec<saber_hip_region> regs_all;  // filled by the copy from the device
     {
         DevBuf<uint8_t> hspf;
         DevBuf<unsigned long long> d_cctr;
@@ -1496,7 +1496,7 @@ int saber_hip_search_run(saber_hip_ctx* c, const saber_hip_seqset* queries, cons
     }
     const uint64_t nreg = roff[nq];
     ctr.regions = nreg;
-    std::vector<saber_hip_region> flat;
+    host_vec<saber_hip_region> flat;
     if (nreg == regs_all.size()) {
         flat.swap(regs_all);
     } else {
@@ -1889,6 +1889,24 @@ int saber_hip_search_ops(const saber_hip_search* s, uint32_t q, uint8_t* codes, 
     if (!s->op_codes[q].empty() && (!codes || !lens)) return fail(SABER_HIP_EINVAL, "null output");
     std::copy(s->op_codes[q].begin(), s->op_codes[q].end(), codes);
     std::copy(s->op_lens[q].begin(), s->op_lens[q].end(), lens);
+    return SABER_HIP_OK;
+}
+
+int saber_hip_search_all_hits(const saber_hip_search* s, uint64_t* hit_off, saber_hip_hit* out, uint64_t* op_base,
+                              uint8_t* codes, uint32_t* lens) {
+    if (!s) return fail(SABER_HIP_EINVAL, "search is null");
+    uint64_t h = 0, o = 0;
+    for (uint32_t q = 0; q < s->nq; ++q) {
+        if (hit_off) hit_off[q] = h;
+        if (op_base) op_base[q] = o;
+        if (out) std::copy(s->hits[q].begin(), s->hits[q].end(), out + h);
+        if (codes) std::copy(s->op_codes[q].begin(), s->op_codes[q].end(), codes + o);
+        if (lens) std::copy(s->op_lens[q].begin(), s->op_lens[q].end(), lens + o);
+        h += s->hits[q].size();
+        o += s->op_codes[q].size();
+    }
+    if (hit_off) hit_off[s->nq] = h;
+    if (op_base) op_base[s->nq] = o;
     return SABER_HIP_OK;
 }
 

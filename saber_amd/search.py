@@ -172,25 +172,28 @@ def run(ctx, qres, qoff, scheme, karlin, seeds=None, params=None, prefilter=None
                                                 C.byref(params), C.byref(pf) if pf is not None else None,
                                                 C.byref(h)))
     L = _ffi.lib()
-    hit_ptr = C.POINTER(_ffi.Hit)
     try:
+        # all hits and operations in two calls (sizes, then contents)
+        hoff = np.zeros(nq + 1, np.uint64)
+        obase = np.zeros(nq + 1, np.uint64)
+        u64 = C.c_uint64
+        _ffi.check(L.saber_hip_search_all_hits(h, _ffi.ptr(hoff, u64), None, _ffi.ptr(obase, u64), None, None))
+        arr = np.zeros(max(int(hoff[-1]), 1), HIT_DTYPE)
+        oc = np.zeros(max(int(obase[-1]), 1), np.uint8)
+        ol = np.zeros(max(int(obase[-1]), 1), np.uint32)
+        _ffi.check(L.saber_hip_search_all_hits(h, _ffi.ptr(hoff, u64), arr.ctypes.data_as(C.POINTER(_ffi.Hit)),
+                                                _ffi.ptr(obase, u64), _ffi.ptr(oc, C.c_uint8),
+                                                _ffi.ptr(ol, C.c_uint32)))
+        rows = arr[:int(hoff[-1])].tolist()
+        codes, lens = oc.tolist(), ol.tolist()
+        hl_off, ob = hoff.tolist(), obase.tolist()
         hits, regs = [], [] if regions else None
         for q in range(nq):
-            n = L.saber_hip_search_hit_count(h, q)
-            hl = []
-            if n:
-                arr = np.zeros(n, HIT_DTYPE)
-                _ffi.check(L.saber_hip_search_hits(h, q, arr.ctypes.data_as(hit_ptr)))
-                no = L.saber_hip_search_op_count(h, q)
-                oc = np.zeros(max(no, 1), np.uint8)
-                ol = np.zeros(max(no, 1), np.uint32)
-                _ffi.check(L.saber_hip_search_ops(h, q, _ffi.ptr(oc, C.c_uint8), _ffi.ptr(ol, C.c_uint32)))
-                codes, lens = oc.tolist(), ol.tolist()
-                for (subject, score, exact, bits, ev, q0, q1, s0, s1, cols, mat, mis, pos, gopen, gcols, ql, sl, oo,
-                     nops, _) in arr.tolist():
-                    hl.append(Hit(subject, score, bits, ev, q0, q1, s0, s1, cols, mat, mis, pos, gopen, gcols, ql, sl,
-                                  list(zip(codes[oo:oo + nops], lens[oo:oo + nops])), bool(exact)))
-            hits.append(hl)
+            b = ob[q]
+            hits.append([Hit(subject, score, bits, ev, q0, q1, s0, s1, cols, mat, mis, pos, gopen, gcols, ql, sl,
+                             list(zip(codes[b + oo:b + oo + nops], lens[b + oo:b + oo + nops])), bool(exact))
+                         for (subject, score, exact, bits, ev, q0, q1, s0, s1, cols, mat, mis, pos, gopen, gcols, ql,
+                              sl, oo, nops, _) in rows[hl_off[q]:hl_off[q + 1]]])
             if regions:
                 nr = L.saber_hip_search_region_count(h, q)
                 r = np.zeros(nr, REGION_DTYPE)
