@@ -161,6 +161,15 @@ int saber_hip_band_batch_time(saber_hip_band_batch* b, int warmup, int iters, do
                               double* band_ms, uint64_t* variant_cells, int* nvar,
                               uint64_t* cells);
 
+/* The plan a batch runs (no counterpart in the reference: the planner is
+ * this library's own; tests compare the host and device planners with it).
+ * npairs / nchunks receive the sizes; pairs (2 * npairs job ids, 0xFFFFFFFF
+ * for none), blocks (4 * nchunks: shape, first pair, pairs, flags, in
+ * dispatch order) and geometry (5 per job: clipped band lo, width, first row,
+ * rows, transposed) are filled when not null. */
+int saber_hip_band_batch_plan(saber_hip_band_batch* b, uint64_t* npairs, uint64_t* nchunks, uint32_t* pairs,
+                              uint32_t* blocks, int32_t* geometry);
+
 /* Banded traceback: banded_align_until (src/algorithm/banded.rs:472-521)
  * for each job -- the alignment of the job's best band cell, found by the
  * scalar recurrence of banded_align_scalar (banded.rs:227-333), the row stop of

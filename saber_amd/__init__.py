@@ -180,6 +180,22 @@ class BandBatch:
         per = {v: kc[v] for v in range(min(nv.value, cap)) if kc[v] > 0}
         return ms.value, kms.value, per, cells.value
 
+    def plan(self):
+        """The batch's plan: (pairs [npairs, 2] job ids (0xFFFFFFFF = none),
+        blocks [nchunks, 4] (shape, first pair, pairs, flags) in dispatch
+        order, geometry [njobs, 5] (band lo, width, first row, rows,
+        transposed))."""
+        npairs, nchunks = C.c_uint64(), C.c_uint64()
+        lib = _ffi.lib()
+        _ffi.check(lib.saber_hip_band_batch_plan(self.h, C.byref(npairs), C.byref(nchunks), None, None, None))
+        pairs = np.zeros((npairs.value, 2), np.uint32)
+        blocks = np.zeros((nchunks.value, 4), np.uint32)
+        geom = np.zeros((self.njobs, 5), np.int32)
+        _ffi.check(lib.saber_hip_band_batch_plan(
+            self.h, C.byref(npairs), C.byref(nchunks), pairs.ctypes.data_as(C.POINTER(C.c_uint32)),
+            blocks.ctypes.data_as(C.POINTER(C.c_uint32)), geom.ctypes.data_as(C.POINTER(C.c_int32))))
+        return pairs, blocks, geom
+
     def close(self):
         if self.h:
             _ffi.lib().saber_hip_band_batch_destroy(self.h)

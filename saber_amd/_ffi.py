@@ -222,6 +222,9 @@ _SIGS = {
     "saber_hip_band_batch_read": (C.c_int, [P, C.POINTER(C.c_int32), C.POINTER(C.c_uint32),
                                             C.POINTER(C.c_uint32), C.POINTER(C.c_uint64)]),
     "saber_hip_band_batch_destroy": (None, [P]),
+    "saber_hip_band_batch_plan": (C.c_int, [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                            C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                            C.POINTER(C.c_int32)]),
     "saber_hip_band_batch_time": (C.c_int, [P, C.c_int, C.c_int, C.POINTER(C.c_double),
                                             C.POINTER(C.c_double), C.POINTER(C.c_uint64),
                                             C.POINTER(C.c_int), C.POINTER(C.c_uint64)]),

@@ -517,21 +517,21 @@ hipError_t launch_gather_ops(uint32_t n, const uint32_t* ids, const AlignRecord*
                              const uint64_t* dst_off, const uint8_t* codes, const uint32_t* lens, uint8_t* dcodes,
                              uint32_t* dlens, hipStream_t s) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(gather_ops_kernel, dim3((n + 3) / 4), dim3(256), 0, s, n, ids, recs, op_off, dst_off, codes,
+    SABER_LAUNCH(gather_ops_kernel, dim3((n + 3) / 4), dim3(256), 0, s, n, ids, recs, op_off, dst_off, codes,
                        lens, dcodes, dlens);
     return hipGetLastError();
 }
 
 hipError_t launch_band_align(const AlignArgs& a, hipStream_t s) {
     if (a.n == 0) return hipSuccess;
-    hipLaunchKernelGGL(band_align_kernel, dim3((a.n + kWideThreads - 1) / kWideThreads), dim3(kWideThreads), 0, s, a);
+    SABER_LAUNCH(band_align_kernel, dim3((a.n + kWideThreads - 1) / kWideThreads), dim3(kWideThreads), 0, s, a);
     return hipGetLastError();
 }
 
 hipError_t launch_band_align_lanes(const AlignArgs& a, const AlignLaneClasses& cls, hipStream_t s) {
     const uint32_t blocks = cls.block_start[kAlignLaneClasses];
     if (blocks == 0) return hipSuccess;
-    hipLaunchKernelGGL(band_align_lanes_kernel, dim3(blocks), dim3(256), 0, s, a, cls);
+    SABER_LAUNCH(band_align_lanes_kernel, dim3(blocks), dim3(256), 0, s, a, cls);
     return hipGetLastError();
 }
 

@@ -70,7 +70,7 @@ hipError_t fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s) {
     if (n == 0) return hipSuccess;
     const uint32_t block = 256;
     const uint64_t grid = (n + block - 1) / block;
-    hipLaunchKernelGGL(fill_u32_kernel, dim3((uint32_t)grid), dim3(block), 0, s, p, n, v);
+    SABER_LAUNCH(fill_u32_kernel, dim3((uint32_t)grid), dim3(block), 0, s, p, n, v);
     return hipGetLastError();
 }
 
@@ -96,7 +96,7 @@ int upload_codes(DevBuf<uint8_t>& d, const uint8_t* h, uint64_t n, hipStream_t s
     if (n) {
         const uint32_t block = 256;
         const uint64_t grid = (n + block - 1) / block;
-        hipLaunchKernelGGL(canon_codes_kernel, dim3((uint32_t)grid), dim3(block), 0, s, d.p + kGuard, n);
+        SABER_LAUNCH(canon_codes_kernel, dim3((uint32_t)grid), dim3(block), 0, s, d.p + kGuard, n);
         HIP_TRY(hipGetLastError());
     }
     return SABER_HIP_OK;
@@ -1095,6 +1095,30 @@ int saber_hip_band_batch_time(saber_hip_band_batch* b, int warmup, int iters, do
                 for (uint64_t x = 2 * (uint64_t)k.pair_off; x < 2 * ((uint64_t)k.pair_off + k.npairs); ++x)
                     if (pairs[x] != kNone) variant_cells[v] += c[pairs[x]];
             }
+        }
+    }
+    return SABER_HIP_OK;
+}
+
+int saber_hip_band_batch_plan(saber_hip_band_batch* b, uint64_t* npairs, uint64_t* nchunks, uint32_t* pairs,
+                              uint32_t* blocks, int32_t* geometry) {
+    if (!b || !npairs || !nchunks) return fail(SABER_HIP_EINVAL, "null argument");
+    std::lock_guard<std::mutex> g(b->ctx->mu);
+    HIP_TRY(hipSetDevice(b->ctx->device));
+    *npairs = b->nchunks ? b->pairs.n / 2 : 0;
+    *nchunks = b->nchunks;
+    if (pairs && *npairs) HIP_TRY(hipMemcpy(pairs, b->pairs.p, 8 * *npairs, hipMemcpyDeviceToHost));
+    if (blocks && b->nchunks) HIP_TRY(hipMemcpy(blocks, b->blocks.p, sizeof(BandBlock) * b->nchunks, hipMemcpyDeviceToHost));
+    if (geometry && b->njobs) {
+        std::vector<BandJobDev> j(b->njobs);
+        HIP_TRY(hipMemcpy(j.data(), b->jobs.p, sizeof(BandJobDev) * b->njobs, hipMemcpyDeviceToHost));
+        for (uint64_t k = 0; k < b->njobs; ++k) {
+            int32_t* o = geometry + 5 * k;
+            o[0] = j[k].lo;
+            o[1] = j[k].w;
+            o[2] = j[k].i_first;
+            o[3] = j[k].rows;
+            o[4] = (int32_t)j[k].swapped;
         }
     }
     return SABER_HIP_OK;
@@ -2669,10 +2693,10 @@ extern "C" int saber_hip_copy_probe(saber_hip_ctx* c, uint64_t bytes, int iters,
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
-    hipLaunchKernelGGL(copy_probe_kernel, dim3(grid), dim3(256), 0, st, a.p, b.p, n);
+    SABER_LAUNCH(copy_probe_kernel, dim3(grid), dim3(256), 0, st, a.p, b.p, n);
     HIP_TRY(hipEventRecord(e0, st));
     for (int i = 0; i < iters; ++i)
-        hipLaunchKernelGGL(copy_probe_kernel, dim3(grid), dim3(256), 0, st, a.p, b.p, n);
+        SABER_LAUNCH(copy_probe_kernel, dim3(grid), dim3(256), 0, st, a.p, b.p, n);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(e1, st));
     HIP_TRY(hipEventSynchronize(e1));

@@ -444,9 +444,15 @@ __device__ __attribute__((noinline)) void band_pair(band_args a, const int8_t* T
     }
 }
 
-// Every compiled shape (L, C, K), in variant-table order.
+// Every compiled shape (L, C, K), in variant-table order. There is no
+// (1, 13, 13): its body with bias, per-job tables and scores only returned
+// wrong scores on the device under load (a different value on each launch,
+// for most of a batch's jobs of that body; the other flavours of the same
+// shape and every other shape were exact), and the cause was not found;
+// widths up to 13 run on (1, 9, 9), (1, 17, 8), (1, 17, 17) and (2, 9, 9).
+// tests/test_band_stress_gpu.py runs every body and flavour under load.
 #define SABER_BAND_SHAPES(X)                                                            \
-    X(1, 9, 9) X(1, 13, 13) X(1, 17, 8) X(1, 17, 17)                                    \
+    X(1, 9, 9) X(1, 17, 8) X(1, 17, 17)                                                 \
     X(2, 9, 9) X(2, 13, 13) X(2, 17, 8) X(2, 17, 17)                                    \
     X(4, 9, 9) X(4, 13, 13) X(4, 17, 8) X(4, 17, 17)                                    \
     X(8, 9, 9) X(8, 13, 13) X(8, 17, 8) X(8, 17, 17)                                    \

@@ -5,7 +5,7 @@ namespace saber {
 
 hipError_t launch_band_multi_bias_scores(const BandKernelArgs& a, uint32_t nblocks, hipStream_t s) {
     if (nblocks == 0) return hipSuccess;
-    hipLaunchKernelGGL((band_multi_kernel<true, false>), dim3(nblocks), dim3(768), 0, s, a);
+    SABER_LAUNCH((band_multi_kernel<true, false>), dim3(nblocks), dim3(768), 0, s, a);
     return hipGetLastError();
 }
 

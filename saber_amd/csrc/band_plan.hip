@@ -464,7 +464,7 @@ int band_plan_device(saber_hip_ctx* c, hipStream_t st, const BandPlanInput& in, 
     HIP_TRY(S.pk_flag.reserve(n));
     HIP_TRY(S.gen_flag.reserve(n));
     HIP_TRY(S.count.reserve(1));
-    hipLaunchKernelGGL(init_totals_kernel, dim3(1), dim3(kThreads), 0, st, S.tot.p);
+    SABER_LAUNCH(init_totals_kernel, dim3(1), dim3(kThreads), 0, st, S.tot.p);
     HIP_TRY(hipGetLastError());
 
     // per-job pass and its totals
@@ -486,7 +486,7 @@ int band_plan_device(saber_hip_ctx* c, hipStream_t st, const BandPlanInput& in, 
     ja.pk_flag = S.pk_flag.p;
     ja.gen_flag = S.gen_flag.p;
     ja.tot = S.tot.p;
-    hipLaunchKernelGGL(plan_jobs_kernel, dim3(grid_for(n)), dim3(kThreads), 0, st, ja);
+    SABER_LAUNCH(plan_jobs_kernel, dim3(grid_for(n)), dim3(kThreads), 0, st, ja);
     HIP_TRY(hipGetLastError());
     PlanTotals tot{};
     HIP_TRY(hipMemcpyAsync(&tot, S.tot.p, sizeof(PlanTotals), hipMemcpyDeviceToHost, st));
@@ -516,7 +516,7 @@ int band_plan_device(saber_hip_ctx* c, hipStream_t st, const BandPlanInput& in, 
                 return rocprim::select(t, b, idx, gf, gi, cnt, (size_t)n, st);
             }))
             return r;
-        hipLaunchKernelGGL(plan_gen_size_kernel, dim3(grid_for(ng)), dim3(kThreads), 0, st, gen_ids.p, ng, jobs.p,
+        SABER_LAUNCH(plan_gen_size_kernel, dim3(grid_for(ng)), dim3(kThreads), 0, st, gen_ids.p, ng, jobs.p,
                            S.gen_sz.p);
         HIP_TRY(hipGetLastError());
         const uint64_t* gs = S.gen_sz.p;
@@ -564,7 +564,7 @@ int band_plan_device(saber_hip_ctx* c, hipStream_t st, const BandPlanInput& in, 
     const double est_ms = 0.5 * (double)tot.est / kGpuLaneInstrPerMs;
     ba.lat_limit = kLatencyShare * est_ms * kLaneInstrPerMs;
     ba.bucket_of = S.bucket_of.p;
-    hipLaunchKernelGGL(plan_bucket_kernel, dim3(grid_for(np)), dim3(kThreads), 0, st, ba);
+    SABER_LAUNCH(plan_bucket_kernel, dim3(grid_for(np)), dim3(kThreads), 0, st, ba);
     HIP_TRY(hipGetLastError());
 
     // grouping order: bucket, query start, first row, rows descending
@@ -579,7 +579,7 @@ int band_plan_device(saber_hip_ctx* c, hipStream_t st, const BandPlanInput& in, 
         ka.ids = ids;
         ka.mode = mode;
         ka.keys = keys;
-        hipLaunchKernelGGL(plan_key_kernel, dim3(grid_for(np)), dim3(kThreads), 0, st, ka);
+        SABER_LAUNCH(plan_key_kernel, dim3(grid_for(np)), dim3(kThreads), 0, st, ka);
         HIP_TRY(hipGetLastError());
         return sort_pairs(S, &keys, &keys_alt, &ids, &ids_alt, np, bits, st);
     };
@@ -592,7 +592,7 @@ int band_plan_device(saber_hip_ctx* c, hipStream_t st, const BandPlanInput& in, 
     }
 
     // runs, flavours and the (bucket, flavour, rows) order
-    hipLaunchKernelGGL(plan_runs_kernel, dim3(grid_for(np)), dim3(kThreads), 0, st, ids, np, jobs.p, S.bucket_of.p,
+    SABER_LAUNCH(plan_runs_kernel, dim3(grid_for(np)), dim3(kThreads), 0, st, ids, np, jobs.p, S.bucket_of.p,
                        S.start.p);
     HIP_TRY(hipGetLastError());
     {
@@ -603,10 +603,10 @@ int band_plan_device(saber_hip_ctx* c, hipStream_t st, const BandPlanInput& in, 
             }))
             return r;
     }
-    hipLaunchKernelGGL(plan_run_first_kernel, dim3(grid_for(np)), dim3(kThreads), 0, st, S.start.p, S.run.p, np,
+    SABER_LAUNCH(plan_run_first_kernel, dim3(grid_for(np)), dim3(kThreads), 0, st, S.start.p, S.run.p, np,
                        S.first.p);
     HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(plan_classify_kernel, dim3(grid_for(np)), dim3(kThreads), 0, st, ids, np, jobs.p, S.bucket_of.p,
+    SABER_LAUNCH(plan_classify_kernel, dim3(grid_for(np)), dim3(kThreads), 0, st, ids, np, jobs.p, S.bucket_of.p,
                        S.run.p, S.first.p, nsk, keys, S.tot.p);
     HIP_TRY(hipGetLastError());
     if (int r = sort_pairs(S, &keys, &keys_alt, &ids, &ids_alt, np, 16 + bits_for((uint64_t)(nsk - 1)), st)) return r;
@@ -646,18 +646,18 @@ int band_plan_device(saber_hip_ctx* c, hipStream_t st, const BandPlanInput& in, 
     // pair list
     HIP_TRY(pairs.alloc(2 * pair));
     HIP_TRY(fill_u32(pairs.p, 2 * pair, kNone, st));
-    hipLaunchKernelGGL(plan_pairs_kernel, dim3(grid_for(np)), dim3(kThreads), 0, st, ids, keys, np, S.segs.p, pairs.p);
+    SABER_LAUNCH(plan_pairs_kernel, dim3(grid_for(np)), dim3(kThreads), 0, st, ids, keys, np, S.segs.p, pairs.p);
     HIP_TRY(hipGetLastError());
 
     // chunks, longest estimated time first (stable)
     const uint32_t nch = (uint32_t)chunk;
     HIP_TRY(S.blk.reserve(nch));
-    hipLaunchKernelGGL(plan_chunks_kernel, dim3(grid_for(nch)), dim3(kThreads), 0, st, S.segs.p, nsk, nch, pairs.p,
+    SABER_LAUNCH(plan_chunks_kernel, dim3(grid_for(nch)), dim3(kThreads), 0, st, S.segs.p, nsk, nch, pairs.p,
                        jobs.p, ebound, S.blk.p, keys, ids);
     HIP_TRY(hipGetLastError());
     if (int r = sort_pairs(S, &keys, &keys_alt, &ids, &ids_alt, nch, bits_for(ebound), st)) return r;
     HIP_TRY(blocks.alloc(nch));
-    hipLaunchKernelGGL(plan_gather_blocks_kernel, dim3(grid_for(nch)), dim3(kThreads), 0, st, S.blk.p, ids, nch,
+    SABER_LAUNCH(plan_gather_blocks_kernel, dim3(grid_for(nch)), dim3(kThreads), 0, st, S.blk.p, ids, nch,
                        blocks.p);
     HIP_TRY(hipGetLastError());
     // the segment table's host copy stays alive until its upload is done

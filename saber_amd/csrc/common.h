@@ -29,6 +29,15 @@ typedef const __attribute__((address_space(3))) char* lds_bytes;
 __device__ __forceinline__ uint32_t lds_u32(lds_bytes p) {
     return *(const __attribute__((address_space(3))) uint32_t*)p;
 }
+// A kernel launch checked by the hipGetLastError that follows it: the error
+// state is cleared first, so that a failed HIP call tolerated here (or made by
+// other code on this thread) is not reported as the launch's failure.
+#define SABER_LAUNCH(...)                  \
+    do {                                   \
+        (void)hipGetLastError();           \
+        hipLaunchKernelGGL(__VA_ARGS__);   \
+    } while (0)
+
 constexpr short kMax16 = 32767;      // any job whose best reaches this is rescored in int32
 
 // Band kernel score table: int8, 25 query rows x 25 columns (subject codes

@@ -252,21 +252,21 @@ __global__ __launch_bounds__(256) void search_features_kernel(SearchFeatureArgs 
 hipError_t launch_search_features(const SearchFeatureArgs& a, const uint64_t* qoff, const uint8_t* qres, uint32_t nq,
                                   hipStream_t s) {
     if (a.n == 0) return hipSuccess;
-    hipLaunchKernelGGL(query_composition_kernel, dim3((uint32_t)std::min<uint64_t>((nq + 3) / 4, 4096)), dim3(256),
+    SABER_LAUNCH(query_composition_kernel, dim3((uint32_t)std::min<uint64_t>((nq + 3) / 4, 4096)), dim3(256),
                        0, s, qoff, qres, nq, a.qcomp);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(search_features_kernel, dim3((uint32_t)((a.n + 255) / 256)), dim3(256), 0, s, a);
+    SABER_LAUNCH(search_features_kernel, dim3((uint32_t)((a.n + 255) / 256)), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
 hipError_t launch_region_features(const FeatureArgs& a, uint32_t nq, hipStream_t s) {
     if (a.n == 0) return hipSuccess;
-    hipLaunchKernelGGL(query_composition_kernel, dim3((uint32_t)std::min<uint64_t>((nq + 3) / 4, 4096)), dim3(256),
+    SABER_LAUNCH(query_composition_kernel, dim3((uint32_t)std::min<uint64_t>((nq + 3) / 4, 4096)), dim3(256),
                        0, s, a.qoff, a.qres, nq, a.qcomp);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(region_features_kernel, dim3((uint32_t)((a.n + 255) / 256)), dim3(256), 0, s, a);
+    SABER_LAUNCH(region_features_kernel, dim3((uint32_t)((a.n + 255) / 256)), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

@@ -33,6 +33,8 @@
 // next column's E and the next row's F. Additions saturate at 32767: a job
 // whose best reaches it is flagged for the exact int32 kernel.
 #pragma once
+#include <atomic>
+
 #include "band_kernel.h"
 
 namespace saber {
@@ -334,16 +336,25 @@ __global__ __launch_bounds__(NT) void full_multi_kernel(FullKernelArgs a) {
 // full_multi_*.hip).
 template <bool BIAS, bool ENDS, int NT>
 hipError_t launch_full_class(const FullKernelArgs& a, uint32_t grid, hipStream_t s) {
-    hipLaunchKernelGGL((full_multi_kernel<BIAS, ENDS, NT>), dim3(grid), dim3(NT), 0, s, a);
+    SABER_LAUNCH((full_multi_kernel<BIAS, ENDS, NT>), dim3(grid), dim3(NT), 0, s, a);
     return hipGetLastError();
 }
 
+// Workgroups of one class kernel resident per CU (asked once; 1 when the
+// runtime cannot say, with its error state cleared).
 template <bool BIAS, bool ENDS, int NT>
 int full_class_blocks_per_cu() {
-    int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, full_multi_kernel<BIAS, ENDS, NT>, NT, 0) != hipSuccess)
+    static std::atomic<int> cached{0};
+    int nb = cached.load(std::memory_order_relaxed);
+    if (nb > 0) return nb;
+    nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, full_multi_kernel<BIAS, ENDS, NT>, NT, 0) != hipSuccess) {
+        (void)hipGetLastError();
         return 1;
-    return nb > 0 ? nb : 1;
+    }
+    nb = nb > 0 ? nb : 1;
+    cached.store(nb, std::memory_order_relaxed);
+    return nb;
 }
 
 }  // namespace saber

@@ -119,7 +119,7 @@ hipError_t launch_generic_slots(const GenericArgs& a, const uint32_t* count, uin
                                 hipStream_t s) {
     if (nthreads == 0) return hipSuccess;
     const uint32_t block = 128;
-    hipLaunchKernelGGL(generic_slots_kernel, dim3((nthreads + block - 1) / block), dim3(block), 0, s, a, count,
+    SABER_LAUNCH(generic_slots_kernel, dim3((nthreads + block - 1) / block), dim3(block), 0, s, a, count,
                        slot_ints);
     return hipGetLastError();
 }
@@ -162,7 +162,7 @@ hipError_t launch_full_int32(const GenericArgs& a, const uint32_t* pairs, const 
                              uint32_t nthreads, uint32_t slot_ints, hipStream_t s) {
     if (nthreads == 0) return hipSuccess;
     const uint32_t block = 128;
-    hipLaunchKernelGGL(full_int32_kernel, dim3((nthreads + block - 1) / block), dim3(block), 0, s, a, pairs, count,
+    SABER_LAUNCH(full_int32_kernel, dim3((nthreads + block - 1) / block), dim3(block), 0, s, a, pairs, count,
                        qoff, soff, nsub, score, q_end, s_end, slot_ints);
     return hipGetLastError();
 }
@@ -171,7 +171,7 @@ hipError_t launch_generic(const GenericArgs& a, hipStream_t s) {
     if (a.nids == 0) return hipSuccess;
     const uint32_t block = 128;
     const uint32_t grid = (a.nids + block - 1) / block;
-    hipLaunchKernelGGL(generic_band_kernel, dim3(grid), dim3(block), 0, s, a);
+    SABER_LAUNCH(generic_band_kernel, dim3(grid), dim3(block), 0, s, a);
     return hipGetLastError();
 }
 

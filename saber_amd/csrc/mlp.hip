@@ -535,7 +535,7 @@ hipError_t mlp_launch_forward(const MlpWeights& w, const __bf16* X, uint64_t n, 
     const uint64_t rows_per_block = 16 * kFwdR * kFwdWaves;
     const uint64_t blocks = (n + rows_per_block - 1) / rows_per_block;
     const int g = (int)std::max<uint64_t>(1, std::min<uint64_t>(blocks, (uint64_t)cus));
-    hipLaunchKernelGGL((mlp_forward_kernel<kFwdR, kFwdOC, kFwdWaves>), dim3(g), dim3(64 * kFwdWaves), 0, s, w, X,
+    SABER_LAUNCH((mlp_forward_kernel<kFwdR, kFwdOC, kFwdWaves>), dim3(g), dim3(64 * kFwdWaves), 0, s, w, X,
                        n, logits);
     return hipGetLastError();
 }
@@ -545,7 +545,7 @@ hipError_t mlp_launch_backward_act(const MlpWeights& w, const __bf16* X, const f
                                    __bf16* H1, __bf16* H2, __bf16* dH2, __bf16* dH1, float* dy,
                                    int grid, hipStream_t s) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(mlp_backward_act_kernel, dim3(grid), dim3(256), 0, s, w, X, labels, n,
+    SABER_LAUNCH(mlp_backward_act_kernel, dim3(grid), dim3(256), 0, s, w, X, labels, n,
                        pos_weight, inv_n, gr, H1, H2, dH2, dH1, dy);
     return hipGetLastError();
 }
@@ -554,7 +554,7 @@ hipError_t mlp_launch_colsum(const __bf16* M, const float* wgt, uint64_t n, floa
     if (n == 0) return hipSuccess;
     // at most 4096 blocks (same-address atomics per column), at least 64 rows each
     const uint64_t rpb = std::max<uint64_t>(64, (n + 4095) / 4096);
-    hipLaunchKernelGGL(colsum_kernel, dim3((uint32_t)((n + rpb - 1) / rpb)), dim3(256), 0, s, M, wgt, n,
+    SABER_LAUNCH(colsum_kernel, dim3((uint32_t)((n + rpb - 1) / rpb)), dim3(256), 0, s, M, wgt, n,
                        rpb, out);
     return hipGetLastError();
 }
@@ -565,27 +565,27 @@ hipError_t mlp_launch_gemm_tn(const __bf16* A, int lda, int a_dim, const __bf16*
     uint64_t kchunk = (K + kchunks - 1) / kchunks;
     kchunk = (kchunk + 31) / 32 * 32;
     const uint32_t nz = (uint32_t)((K + kchunk - 1) / kchunk);
-    hipLaunchKernelGGL(gemm_tn_kernel, dim3(a_dim / 64, b_dim / 64, nz), dim3(256), 0, s, A, lda, B, ldb,
+    SABER_LAUNCH(gemm_tn_kernel, dim3(a_dim / 64, b_dim / 64, nz), dim3(256), 0, s, A, lda, B, ldb,
                        C, b_dim, K, kchunk);
     return hipGetLastError();
 }
 
 hipError_t mlp_launch_adam(float* p, float* grad, float* m, float* v, uint32_t n, float lr, float b1,
                            float b2, float eps, float c1, float c2, hipStream_t s) {
-    hipLaunchKernelGGL(adam_kernel, dim3((n + 255) / 256), dim3(256), 0, s, p, grad, m, v, n, lr, b1, b2,
+    SABER_LAUNCH(adam_kernel, dim3((n + 255) / 256), dim3(256), 0, s, p, grad, m, v, n, lr, b1, b2,
                        eps, c1, c2);
     return hipGetLastError();
 }
 
 hipError_t mlp_launch_pack(const float* W1, const float* W2, __bf16* W1T, __bf16* W2P, __bf16* W2Q,
                            hipStream_t s) {
-    hipLaunchKernelGGL(pack_weights_kernel, dim3(kHid * kHid / 256), dim3(256), 0, s, W1, W2, W1T, W2P, W2Q);
+    SABER_LAUNCH(pack_weights_kernel, dim3(kHid * kHid / 256), dim3(256), 0, s, W1, W2, W1T, W2P, W2Q);
     return hipGetLastError();
 }
 
 hipError_t mlp_launch_to_bf16(const float* src, __bf16* dst, uint64_t n, hipStream_t s) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(f32_to_bf16_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, src, dst, n);
+    SABER_LAUNCH(f32_to_bf16_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, src, dst, n);
     return hipGetLastError();
 }
 

@@ -788,7 +788,7 @@ int build_scan_cache(saber_hip_ctx* c, const saber_hip_seed_scheme* seeds, const
         std::memcpy(la.pk, enc.code, 32);
         la.dlet = cache->dlet.p + kGuard;
         la.dpk = cache->dpk.p + kGuard;
-        hipLaunchKernelGGL(db_letters_kernel, dim3(grid_for(nres, 256)), dim3(256), 0, st, la);
+        SABER_LAUNCH(db_letters_kernel, dim3(grid_for(nres, 256)), dim3(256), 0, st, la);
         HIP_TRY(hipGetLastError());
     }
     // tiles of 64 positions, subject by subject
@@ -826,7 +826,7 @@ int build_scan_cache(saber_hip_ctx* c, const saber_hip_seed_scheme* seeds, const
             ca.kbits = enc.k;
             ca.counts = counts.p;
             ca.total = total.p;
-            hipLaunchKernelGGL(db_count_kernel, dim3(grid), dim3(256), 0, st, ca);
+            SABER_LAUNCH(db_count_kernel, dim3(grid), dim3(256), 0, st, ca);
             HIP_TRY(hipGetLastError());
         }
         unsigned long long tot = 0;
@@ -860,10 +860,10 @@ int build_scan_cache(saber_hip_ctx* c, const saber_hip_seed_scheme* seeds, const
             ca.kbits = enc.k;
             ca.counts = counts.p;
             ca.total = total.p;
-            hipLaunchKernelGGL(db_count_kernel, dim3(grid), dim3(256), 0, st, ca);
+            SABER_LAUNCH(db_count_kernel, dim3(grid), dim3(256), 0, st, ca);
             HIP_TRY(hipGetLastError());
             const uint64_t words = (sh.ncodes + 63) / 64;
-            hipLaunchKernelGGL(freq_bits_kernel, dim3(grid_for(words, 256)), dim3(256), 0, st, counts.p, sh.ncodes,
+            SABER_LAUNCH(freq_bits_kernel, dim3(grid_for(words, 256)), dim3(256), 0, st, counts.p, sh.ncodes,
                                cache->caps[si], cache->freq.p + cache->freq_off[si]);
             HIP_TRY(hipGetLastError());
         }
@@ -1186,7 +1186,7 @@ int saber_hip_search_run(saber_hip_ctx* c, const saber_hip_seqset* queries, cons
         qa.keys = tmp_keys.p;
         qa.vals = T->vals.p;
         qa.ctr = d_tctr.p + 2 * si;
-        hipLaunchKernelGGL(query_keys_kernel, dim3(grid_for(total, 256)), dim3(256), 0, st, qa);
+        SABER_LAUNCH(query_keys_kernel, dim3(grid_for(total, 256)), dim3(256), 0, st, qa);
         HIP_TRY(hipGetLastError());
         // stable sort by code; the values (batch positions) keep batch order per code
         DevBuf<uint32_t> vals_out;
@@ -1213,9 +1213,9 @@ int saber_hip_search_run(saber_hip_ctx* c, const saber_hip_seqset* queries, cons
         const uint64_t nc = shapes[si].ncodes;
         HIP_TRY(T.offs.alloc(nc + 1));
         HIP_TRY(T.bits.alloc((nc + 63) / 64));
-        hipLaunchKernelGGL(table_offsets_kernel, dim3(grid_for(nc + 1, 256)), dim3(256), 0, st, T.keys.p,
+        SABER_LAUNCH(table_offsets_kernel, dim3(grid_for(nc + 1, 256)), dim3(256), 0, st, T.keys.p,
                            (uint32_t)T.n, nc, T.offs.p);
-        hipLaunchKernelGGL(table_bits_kernel, dim3(grid_for((nc + 63) / 64, 256)), dim3(256), 0, st, T.offs.p, nc,
+        SABER_LAUNCH(table_bits_kernel, dim3(grid_for((nc + 63) / 64, 256)), dim3(256), 0, st, T.offs.p, nc,
                            T.bits.p);
         HIP_TRY(hipGetLastError());
         T.keys.release();
@@ -1276,7 +1276,7 @@ int saber_hip_search_run(saber_hip_ctx* c, const saber_hip_seqset* queries, cons
             sa.nout = d_nout.p;
             sa.ctr = d_sctr.p + 3 * si;
             const uint64_t waves = std::min<uint64_t>(cache->ntiles, (uint64_t)c->cus * 32);
-            hipLaunchKernelGGL(scan_kernel, dim3(grid_for(waves * 64, 256)), dim3(256), 0, st, sa);
+            SABER_LAUNCH(scan_kernel, dim3(grid_for(waves * 64, 256)), dim3(256), 0, st, sa);
             HIP_TRY(hipGetLastError());
             unsigned long long n = 0;
             HIP_TRY(hipMemcpyAsync(&n, d_nout.p, sizeof n, hipMemcpyDeviceToHost, st));
@@ -1356,7 +1356,7 @@ int saber_hip_search_run(saber_hip_ctx* c, const saber_hip_seqset* queries, cons
         ka.qoff = d_qoff.p;
         ka.soff = c->db_off_dev.p;
         ka.desc = desc.p;
-        hipLaunchKernelGGL(key_desc_kernel, dim3(grid_for(nsurv, 256)), dim3(256), 0, st, ka);
+        SABER_LAUNCH(key_desc_kernel, dim3(grid_for(nsurv, 256)), dim3(256), 0, st, ka);
         HIP_TRY(hipGetLastError());
         HIP_TRY(fill_u32(xcounter.p, kXdropGroups * kXdropCounterStride, 0, st));
         XdropArgs xa{};
@@ -1409,7 +1409,7 @@ int saber_hip_search_run(saber_hip_ctx* c, const saber_hip_seqset* queries, cons
             ca.hsp = hspf.p;
             ca.ctr = d_cctr.p;
             const uint64_t cgrid = std::min<uint64_t>((uint64_t)c->cus * 16, (nsurv + 255) / 256);
-            hipLaunchKernelGGL(coverage_kernel, dim3((uint32_t)std::max<uint64_t>(cgrid, 1)), dim3(256), 0, st, ca);
+            SABER_LAUNCH(coverage_kernel, dim3((uint32_t)std::max<uint64_t>(cgrid, 1)), dim3(256), 0, st, ca);
             HIP_TRY(hipGetLastError());
         }
         lap("coverage");
@@ -1454,7 +1454,7 @@ int saber_hip_search_run(saber_hip_ctx* c, const saber_hip_seqset* queries, cons
             ra.out = rslot.p;
             ra.keep = rkeep.p;
             ra.ctr = d_cctr.p + 4;
-            hipLaunchKernelGGL(region_kernel, dim3(grid_for(nh, 256)), dim3(256), 0, st, ra);
+            SABER_LAUNCH(region_kernel, dim3(grid_for(nh, 256)), dim3(256), 0, st, ra);
             HIP_TRY(hipGetLastError());
             size_t tb = 0;
             HIP_TRY(rocprim::select(nullptr, tb, rslot.p, rkeep.p, rsel.p, d_nh.p, (size_t)nh, st));

@@ -377,18 +377,18 @@ int xdrop_blocks_per_cu(bool bias, bool wide) {
 
 hipError_t launch_xdrop(const XdropArgs& a, bool bias, bool wide, int grid, hipStream_t s) {
     if (a.nhits == 0) return hipSuccess;
-    hipLaunchKernelGGL(xdrop_desc_kernel, dim3((a.nhits + 255) / 256), dim3(256), 0, s, a);
+    SABER_LAUNCH(xdrop_desc_kernel, dim3((a.nhits + 255) / 256), dim3(256), 0, s, a);
     return launch_xdrop_described(a, bias, wide, grid, s);
 }
 
 hipError_t launch_xdrop_described(const XdropArgs& a, bool bias, bool wide, int grid, hipStream_t s) {
     if (a.nhits == 0) return hipSuccess;
     if (bias) {
-        if (wide) hipLaunchKernelGGL((xdrop_kernel<true, true>), dim3(grid), dim3(256), 0, s, a);
-        else hipLaunchKernelGGL((xdrop_kernel<true, false>), dim3(grid), dim3(256), 0, s, a);
+        if (wide) SABER_LAUNCH((xdrop_kernel<true, true>), dim3(grid), dim3(256), 0, s, a);
+        else SABER_LAUNCH((xdrop_kernel<true, false>), dim3(grid), dim3(256), 0, s, a);
     } else {
-        if (wide) hipLaunchKernelGGL((xdrop_kernel<false, true>), dim3(grid), dim3(256), 0, s, a);
-        else hipLaunchKernelGGL((xdrop_kernel<false, false>), dim3(grid), dim3(256), 0, s, a);
+        if (wide) SABER_LAUNCH((xdrop_kernel<false, true>), dim3(grid), dim3(256), 0, s, a);
+        else SABER_LAUNCH((xdrop_kernel<false, false>), dim3(grid), dim3(256), 0, s, a);
     }
     return hipGetLastError();
 }

@@ -89,7 +89,7 @@ extern "C" int saber_hip_xdrop_noalloc(saber_hip_ctx* c, const uint8_t* qres, co
     short* table = reinterpret_cast<short*>(base + kCounterBytes);
     void* desc = base + kDescOffset;
     HIP_TRY(fill_u32(counters, kXdropGroups * kXdropCounterStride, 0, st));
-    hipLaunchKernelGGL(xdrop_table_kernel, dim3((kXdropTable + 255) / 256), dim3(256), 0, st, mat, table);
+    SABER_LAUNCH(xdrop_table_kernel, dim3((kXdropTable + 255) / 256), dim3(256), 0, st, mat, table);
     HIP_TRY(hipGetLastError());
     const bool bias = qbias != nullptr;
     XdropArgs a{};

@@ -113,6 +113,7 @@ def test_device_plan_database_subjects_and_latency_rule(ctxs):
     h, d = ctxs
     qres, qoff, sres, soff, jobs, qb = mixed_set(9)
     sch = S.Scheme(BLOSUM62, 11, 1)
+    want = O.banded_many(qres, qoff, sres, soff, jobs, BLOSUM62, 11, 1, qb)
     for c in (h, d):
         c.upload_db(sres, soff)
     try:
@@ -127,14 +128,19 @@ def test_device_plan_database_subjects_and_latency_rule(ctxs):
             try:
                 ph, pd = bh.time(warmup=0, iters=1), bd.time(warmup=0, iters=1)
                 assert ph[2] == pd[2], f"latency rows {rows}: {ph[2]} vs {pd[2]}"
-                same(bd.read(), bh.read(), f"scores only, latency rows {rows}")
+                rd, rh = bd.read(), bh.read()
+                # each planner's batch against the oracle first, so that a
+                # difference names the side that deviates
+                for name, r in (("host", rh), ("device", rd)):
+                    same((r[0], None, None, r[3]), (want[0], None, None, want[3]),
+                         f"{name} plan vs oracle, scores only, latency rows {rows}")
+                same(rd, rh, f"scores only, latency rows {rows}")
             finally:
                 bh.close()
                 bd.close()
     finally:
         h.tune(-1)
         d.tune(-1)
-    want = O.banded_many(qres, qoff, sres, soff, jobs, BLOSUM62, 11, 1, qb)
     same(gd, want, "database subjects vs oracle")
 
 
