@@ -571,7 +571,9 @@ typedef struct saber_hip_search_params {
     uint64_t ref_db_residues;          /* 40 000 000 */
     double min_identity, min_query_cover, min_subject_cover;   /* 0 */
     int32_t per_subject_evalues;       /* 1: EvalueModel::PerSubject (default), 0: Database */
-    uint32_t reserved[7];
+    uint32_t keep_regions;             /* 0 (default): only the hits; 1: also every capped region's record,
+                                          for saber_hip_search_regions / saber_hip_search_features */
+    uint32_t reserved[6];
 } saber_hip_search_params;
 void saber_hip_search_defaults(saber_hip_search_params* p);
 /* The seed scheme of the default preset: Murphy-10 and DIAMOND's 16 weight-7

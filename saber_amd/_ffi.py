@@ -117,7 +117,8 @@ class SearchParams(C.Structure):
                 ("max_candidates", C.c_uint64), ("max_regions_per_subject", C.c_uint32), ("comp_bias", C.c_int32),
                 ("seed_max_occ", C.c_uint64), ("traceback_max_bytes", C.c_uint64), ("ref_db_residues", C.c_uint64),
                 ("min_identity", C.c_double), ("min_query_cover", C.c_double), ("min_subject_cover", C.c_double),
-                ("per_subject_evalues", C.c_int32), ("reserved", C.c_uint32 * 7)]
+                ("per_subject_evalues", C.c_int32), ("keep_regions", C.c_uint32),
+                ("reserved", C.c_uint32 * 6)]
 
 
 class Hit(C.Structure):

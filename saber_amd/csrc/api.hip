@@ -141,6 +141,8 @@ struct saber_hip_band_batch {
     DevBuf<uint32_t> q_end, s_end;
     DevBuf<uint64_t> cells;
     const uint8_t* sres = nullptr;  // own copy or the context database (data start)
+    const uint8_t* qres = nullptr;  // own copy or the caller's device codes (data start)
+    const int8_t* qbias = nullptr;  // own copy, the caller's device biases, or none
     uint64_t qlen_total = 0, slen_total = 0;
     bool ran = false;
     uint32_t last_flagged = 0;
@@ -770,19 +772,16 @@ int plan_on_host(saber_hip_ctx* c, hipStream_t st, const BandPlanInput& in, sabe
 
 extern "C" {
 
-int saber_hip_band_batch_create(saber_hip_ctx* c, const saber_hip_seqset* queries,
-                                const saber_hip_seqset* subjects, const saber_hip_band_job* jobs,
-                                uint64_t njobs, const saber_hip_scheme* scheme, int ends,
-                                saber_hip_band_batch** out) {
-    if (int r_ = injected(c)) return r_;
-    if (!c || !out || !scheme) return fail(SABER_HIP_EINVAL, "null argument");
+}  // extern "C"
+
+// A band batch over host inputs (queries / subjects / jobs, the public entry
+// point) or over inputs already on the device (dev.jobs set: jobs, query
+// offsets, codes and biases in device memory, subjects from the context
+// database, planned on the device). The caller holds c->mu.
+int saber::band_batch_create_impl(saber_hip_ctx* c, const saber_hip_seqset* queries, const saber_hip_seqset* subjects,
+                                  const saber_hip_band_job* jobs, uint64_t njobs, const DevBandInput* dev,
+                                  const saber_hip_scheme* scheme, int ends, saber_hip_band_batch** out) {
     *out = nullptr;
-    if (njobs > 0 && !jobs) return fail(SABER_HIP_EINVAL, "jobs is null");
-    if (njobs >= (uint64_t)kNone) return fail(SABER_HIP_EINVAL, "too many jobs for one batch");
-    if (int r = check_seqset(queries, "queries")) return r;
-    if (subjects)
-        if (int r = check_seqset(subjects, "subjects")) return r;
-    std::lock_guard<std::mutex> g(c->mu);
     HIP_TRY(hipSetDevice(c->device));
     const uint64_t* soffs = subjects ? subjects->offsets : c->db_off.data();
     const uint32_t nsub = subjects ? subjects->count : (c->db_off.empty() ? 0 : (uint32_t)c->db_off.size() - 1);
@@ -790,7 +789,7 @@ int saber_hip_band_batch_create(saber_hip_ctx* c, const saber_hip_seqset* querie
     auto b = std::make_unique<saber_hip_band_batch>();
     b->ctx = c;
     b->njobs = njobs;
-    b->has_bias = queries->bias != nullptr;
+    b->has_bias = dev ? dev->qbias != nullptr : queries->bias != nullptr;
     b->ends = ends != 0;
     b->go = scheme->gap_open;
     b->ge = scheme->gap_extend;
@@ -823,9 +822,13 @@ int saber_hip_band_batch_create(saber_hip_ctx* c, const saber_hip_seqset* querie
 
     // Plan: on the device for large batches, else on the host (the same plan).
     hipStream_t st = c->stream;
-    const BandPlanInput in{jobs, njobs, queries->offsets, queries->count, soffs,
-                           subjects ? nullptr : c->db_off_dev.p, nsub, packed_ok, swap_ok};
-    const bool on_device = (c->flags & SABER_HIP_DEVICE_PLAN) ||
+    BandPlanInput in{jobs, njobs, dev ? nullptr : queries->offsets, dev ? dev->qcount : queries->count, soffs,
+                     subjects ? nullptr : c->db_off_dev.p, nsub, packed_ok, swap_ok};
+    if (dev) {
+        in.jobs_dev = dev->jobs;
+        in.qoffs_dev = dev->qoffs;
+    }
+    const bool on_device = dev || (c->flags & SABER_HIP_DEVICE_PLAN) ||
                            (!(c->flags & SABER_HIP_HOST_PLAN) && njobs >= kDevicePlanMinJobs);
     BandPlan plan;
     if (on_device) {
@@ -855,10 +858,19 @@ int saber_hip_band_batch_create(saber_hip_ctx* c, const saber_hip_seqset* querie
     }
     const std::vector<uint32_t> ptable = build_pair_table(scheme->matrix);
 
-    const uint64_t nq = queries->count ? queries->offsets[queries->count] : 0;
-    if (int r = upload_codes(b->q, queries->residues, nq, st)) return r;
-    if (b->has_bias) {
-        if (int r = upload_guarded(b->bias, queries->bias, nq, st)) return r;
+    uint64_t nq = 0;
+    if (dev) {
+        nq = dev->qlen_total;
+        b->qres = dev->qres;
+        b->qbias = dev->qbias;
+    } else {
+        nq = queries->count ? queries->offsets[queries->count] : 0;
+        if (int r = upload_codes(b->q, queries->residues, nq, st)) return r;
+        b->qres = b->q.p + kGuard;
+        if (b->has_bias) {
+            if (int r = upload_guarded(b->bias, queries->bias, nq, st)) return r;
+            b->qbias = b->bias.p + kGuard;
+        }
     }
     if (subjects) {
         const uint64_t ns = subjects->count ? subjects->offsets[subjects->count] : 0;
@@ -894,11 +906,29 @@ int saber_hip_band_batch_create(saber_hip_ctx* c, const saber_hip_seqset* querie
     return SABER_HIP_OK;
 }
 
+extern "C" {
+
+int saber_hip_band_batch_create(saber_hip_ctx* c, const saber_hip_seqset* queries,
+                                const saber_hip_seqset* subjects, const saber_hip_band_job* jobs,
+                                uint64_t njobs, const saber_hip_scheme* scheme, int ends,
+                                saber_hip_band_batch** out) {
+    if (int r_ = injected(c)) return r_;
+    if (!c || !out || !scheme) return fail(SABER_HIP_EINVAL, "null argument");
+    *out = nullptr;
+    if (njobs > 0 && !jobs) return fail(SABER_HIP_EINVAL, "jobs is null");
+    if (njobs >= (uint64_t)kNone) return fail(SABER_HIP_EINVAL, "too many jobs for one batch");
+    if (int r = check_seqset(queries, "queries")) return r;
+    if (subjects)
+        if (int r = check_seqset(subjects, "subjects")) return r;
+    std::lock_guard<std::mutex> g(c->mu);
+    return band_batch_create_impl(c, queries, subjects, jobs, njobs, nullptr, scheme, ends, out);
+}
+
 static GenericArgs generic_args(saber_hip_band_batch* b) {
     GenericArgs a{};
     a.jobs = b->jobs.p;
-    a.qres = b->q.p + kGuard;
-    a.qbias = b->has_bias ? b->bias.p + kGuard : nullptr;
+    a.qres = b->qres;
+    a.qbias = b->has_bias ? b->qbias : nullptr;
     a.sres = b->sres;
     a.mat = b->mat.p;
     a.go = b->go;
@@ -927,8 +957,8 @@ static int enqueue_step(saber_hip_band_batch* b, hipEvent_t ev_begin, hipEvent_t
     a.next_chunk = b->next_chunk.p;
     a.pair_table = b->pair_table.p;
     a.jobs = b->jobs.p;
-    a.qres = b->q.p + kGuard;
-    a.qbias = b->has_bias ? b->bias.p + kGuard : nullptr;
+    a.qres = b->qres;
+    a.qbias = b->has_bias ? b->qbias : nullptr;
     a.sres = b->sres;
     a.qres_len = b->qlen_total;
     a.sres_len = b->slen_total;
@@ -983,6 +1013,18 @@ static int enqueue_run(saber_hip_band_batch* b) {
     HIP_TRY(hipGraphLaunch(b->graph_exec, st));
     return SABER_HIP_OK;
 }
+
+}  // extern "C"
+
+int saber::band_batch_enqueue(saber_hip_band_batch* b) {
+    if (int r = enqueue_run(b)) return r;
+    b->ran = true;
+    return SABER_HIP_OK;
+}
+const int32_t* saber::band_batch_scores(const saber_hip_band_batch* b) { return b->score.p; }
+const uint64_t* saber::band_batch_cells(const saber_hip_band_batch* b) { return b->cells.p; }
+
+extern "C" {
 
 int saber_hip_band_batch_run(saber_hip_band_batch* b) {
     if (!b) return fail(SABER_HIP_EINVAL, "batch is null");

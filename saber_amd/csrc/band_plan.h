@@ -59,7 +59,9 @@ struct PlanShape {
 
 // The jobs of a batch as the planners take them. The offsets are host arrays
 // (count + 1 entries); soffs_dev, when set, is the same subject offsets
-// already on the device (the context database).
+// already on the device (the context database). jobs_dev / qoffs_dev, when
+// set, are the jobs and query offsets already on the device (then only the
+// device planner can take the batch, and jobs / qoffs may be null).
 struct BandPlanInput {
     const saber_hip_band_job* jobs;
     uint64_t njobs;
@@ -69,6 +71,8 @@ struct BandPlanInput {
     const uint64_t* soffs_dev;
     uint32_t nsub;
     bool packed_ok, swap_ok;
+    const saber_hip_band_job* jobs_dev = nullptr;
+    const uint64_t* qoffs_dev = nullptr;
 };
 
 // A plan as the band batch runs it. Device buffers: jobs and cells (one per

@@ -446,8 +446,16 @@ int band_plan_device(saber_hip_ctx* c, hipStream_t st, const BandPlanInput& in, 
     // inputs
     std::vector<PlanShape> shp(nshape);
     for (int k = 0; k < nshape; ++k) shp[k] = PlanShape{c->shapes[k].L, (uint32_t)c->shapes[k].row_cost()};
-    if (int r = upload_keep(S.in, in.jobs, n, st)) return r;
-    if (int r = upload_keep(S.qoffs, in.qoffs, (uint64_t)in.qcount + 1, st)) return r;
+    const saber_hip_band_job* jobs_in = in.jobs_dev;
+    if (!jobs_in) {
+        if (int r = upload_keep(S.in, in.jobs, n, st)) return r;
+        jobs_in = S.in.p;
+    }
+    const uint64_t* qoffs_in = in.qoffs_dev;
+    if (!qoffs_in) {
+        if (int r = upload_keep(S.qoffs, in.qoffs, (uint64_t)in.qcount + 1, st)) return r;
+        qoffs_in = S.qoffs.p;
+    }
     const uint64_t* soffs = in.soffs_dev;
     if (!soffs) {
         if (int r = upload_keep(S.soffs, in.soffs, (uint64_t)in.nsub + 1, st)) return r;
@@ -469,9 +477,9 @@ int band_plan_device(saber_hip_ctx* c, hipStream_t st, const BandPlanInput& in, 
 
     // per-job pass and its totals
     JobArgs ja{};
-    ja.jobs = S.in.p;
+    ja.jobs = jobs_in;
     ja.njobs = n;
-    ja.qoffs = S.qoffs.p;
+    ja.qoffs = qoffs_in;
     ja.qcount = in.qcount;
     ja.soffs = soffs;
     ja.nsub = in.nsub;

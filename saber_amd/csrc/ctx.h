@@ -127,6 +127,28 @@ int score_regions_impl(saber_hip_ctx* c, const saber_hip_seqset* queries, const 
                        const saber_hip_scheme* scheme, int64_t narrow_band, int64_t full_band, const double* thr,
                        int32_t* out, uint8_t* rescored, uint64_t* banded_cells, const uint8_t* skip);
 
+// Band jobs, query offsets (count + 1), codes and biases (data starts of
+// guarded buffers; qbias may be null) already in device memory; subjects are
+// the context database.
+struct DevBandInput {
+    const saber_hip_band_job* jobs;
+    const uint64_t* qoffs;
+    uint32_t qcount;
+    const uint8_t* qres;
+    uint64_t qlen_total;
+    const int8_t* qbias;
+};
+// saber_hip_band_batch_create over host inputs (dev == null) or device inputs
+// (queries == null, subjects == null). The caller holds c->mu.
+int band_batch_create_impl(saber_hip_ctx* c, const saber_hip_seqset* queries, const saber_hip_seqset* subjects,
+                           const saber_hip_band_job* jobs, uint64_t njobs, const DevBandInput* dev,
+                           const saber_hip_scheme* scheme, int ends, saber_hip_band_batch** out);
+// One scoring pass of a batch enqueued on the context stream (the caller
+// holds c->mu); its per-job scores and band cells stay in device memory.
+int band_batch_enqueue(saber_hip_band_batch* b);
+const int32_t* band_batch_scores(const saber_hip_band_batch* b);
+const uint64_t* band_batch_cells(const saber_hip_band_batch* b);
+
 // Takes the address of saber_hip_xdrop_noalloc's scratch array on the
 // current device (xdrop_noalloc.hip); called when a context is created.
 int xdrop_noalloc_setup(int device);

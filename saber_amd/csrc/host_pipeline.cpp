@@ -10,38 +10,18 @@
 #include <vector>
 
 #include "../../include/saber_hip.h"
+#include "plan_rules.h"
 
 namespace {
 
-constexpr uint64_t kBandScoreMinCells = 1ull << 20;  // engine.rs:401
-constexpr uint64_t kBandCostRatio = 12;              // engine.rs:402
-constexpr int64_t kBandBridge = 1024;                // engine.rs:404
-
-struct Band {
-    int64_t lo, hi;
-    int64_t width() const { return std::max<int64_t>(hi - lo + 1, 0); }
-    Band clip(int64_t m, int64_t n) const { return Band{std::max(lo, -(m - 1)), std::min(hi, n - 1)}; }
-    bool covers(int64_t m, int64_t n) const { return lo <= -(m - 1) && hi >= n - 1; }
-    Band widen(int64_t by) const { return Band{lo - by, hi + by}; }
-};
+using saber_plan::Band;
+using saber_plan::kBandBridge;
+using saber_plan::window;
 
 int32_t sat_add(int32_t a, int32_t b) {
     const int64_t s = (int64_t)a + b;
     return (int32_t)std::max<int64_t>(std::min<int64_t>(s, std::numeric_limits<int32_t>::max()),
                                       std::numeric_limits<int32_t>::min());
-}
-
-// Engine::window (engine.rs:1223-1234).
-void window(int64_t d_min, int64_t d_max, int64_t qlen, int64_t sl, int64_t slack, uint64_t* olo, uint64_t* ohi) {
-    const int64_t lo = std::min(std::max(d_min - slack, (int64_t)0), sl);
-    const int64_t hi = std::max(std::min(qlen + d_max + slack, sl), lo);
-    if ((hi - lo) * 4 >= sl * 3) {
-        *olo = 0;
-        *ohi = (uint64_t)sl;
-    } else {
-        *olo = (uint64_t)lo;
-        *ohi = (uint64_t)hi;
-    }
 }
 
 }  // namespace
@@ -144,7 +124,7 @@ int saber_hip_plan_windows(const saber_hip_region* regs, uint64_t n, uint64_t ql
         while (g1 < n && regs[order[g1]].subject == regs[order[g0]].subject) ++g1;
         int32_t top = 0;
         for (uint64_t k = g0; k < g1; ++k) top = std::max(top, regs[order[k]].best);
-        const int32_t strong = std::max(2 * p->hsp_threshold, top / 4);
+        const int32_t strong = saber_plan::strong_cut(p->hsp_threshold, top);
         idx.clear();
         for (uint64_t k = g0; k < g1; ++k)
             if (regs[order[k]].best >= strong) idx.push_back(order[k]);
@@ -167,42 +147,8 @@ int saber_hip_plan_windows(const saber_hip_region* regs, uint64_t n, uint64_t ql
     }
     for (uint64_t ri = 0; ri < n; ++ri) {
         const saber_hip_region& r = regs[ri];
-        const int64_t sl = (int64_t)subject_len[ri];
-        saber_hip_plan own{};
-        window(r.d_min, r.d_max, q, sl, p->window_slack, &own.lo, &own.hi);
-        own.banded = 0;
-        saber_hip_plan& o = out[ri];
-        o = own;
-        if (p->exhaustive || r.best == 0 || p->window_slack >= std::numeric_limits<int64_t>::max() / 8) continue;
-        const int64_t dmin = span[ri].first, dmax = span[ri].second;
-        if (p->score_band > 0) {
-            const int64_t w = p->score_band;
-            const int64_t lo = std::min(std::max(dmin - w, (int64_t)0), sl);
-            const int64_t hi = std::max(std::min(q + dmax + w, sl), lo);
-            const int64_t nn = hi - lo;
-            const Band band = Band{dmin - w - lo, dmax + w - lo}.clip(q, nn);
-            o.lo = (uint64_t)lo;
-            o.hi = (uint64_t)hi;
-            o.banded = (nn == 0 || band.covers(q, nn)) ? 0 : 1;
-            o.band_lo = o.banded ? band.lo : 0;
-            o.band_hi = o.banded ? band.hi : 0;
-            continue;
-        }
-        if ((uint64_t)q * (own.hi - own.lo) < kBandScoreMinCells) continue;
-        uint64_t ulo = 0, uhi = 0;
-        window(dmin, dmax, q, sl, p->window_slack, &ulo, &uhi);
-        const int64_t nn = (int64_t)(uhi - ulo);
-        const Band band = Band{dmin - (int64_t)ulo, dmax - (int64_t)ulo}.widen(p->window_slack).clip(q, nn);
-        if (band.covers(q, nn)) continue;
-        const uint64_t rows = (uint64_t)std::min<int64_t>(q, nn + band.width());
-        const uint64_t rect = (uint64_t)q * (uint64_t)nn;
-        if ((uint64_t)band.width() * rows * kBandCostRatio < rect) {
-            o.lo = ulo;
-            o.hi = uhi;
-            o.banded = 1;
-            o.band_lo = band.lo;
-            o.band_hi = band.hi;
-        }
+        out[ri] = saber_plan::plan_region(r.d_min, r.d_max, r.best, span[ri].first, span[ri].second, q,
+                                          (int64_t)subject_len[ri], *p);
     }
     return SABER_HIP_OK;
 }

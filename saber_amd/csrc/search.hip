@@ -57,6 +57,7 @@
 #include "common.h"
 #include "ctx.h"
 #include "host_par.h"
+#include "search_score.h"
 
 using namespace saber;
 
@@ -664,6 +665,7 @@ struct saber_hip_search {
     std::vector<uint64_t> qlen;
     host_vec<uint64_t> slen;
     std::vector<int32_t> rthr;
+    bool kept_regions = false;   // keep_regions: the arrays above and the region records are filled
     uint64_t db_gen = 0;
     std::vector<std::vector<saber_hip_hit>> hits;
     std::vector<std::vector<uint8_t>> op_codes;
@@ -1391,7 +1393,8 @@ int saber_hip_search_run(saber_hip_ctx* c, const saber_hip_seqset* queries, cons
         std::fprintf(stderr, "#   search %s %.1f ms\n", what, ms_since(tl));
         tl = Clock::now();
     };
-    host_vec<saber_hip_region> regs_all;  // filled by the copy from the device
+    DevBuf<saber_hip_region> rsel;   // region_kernel's selected regions, kept on the device
+    uint64_t nr = 0;
     {
         DevBuf<uint8_t> hspf;
         DevBuf<unsigned long long> d_cctr;
@@ -1430,7 +1433,7 @@ int saber_hip_search_run(saber_hip_ctx* c, const saber_hip_seqset* queries, cons
         }
         ctr.hsps = nh;
         lap("hsp select");
-        DevBuf<saber_hip_region> rslot, rsel;
+        DevBuf<saber_hip_region> rslot;
         DevBuf<uint8_t> rkeep;
         HIP_TRY(rslot.alloc(std::max<uint64_t>(nh, 1)));
         HIP_TRY(rsel.alloc(std::max<uint64_t>(nh, 1)));
@@ -1439,7 +1442,6 @@ int saber_hip_search_run(saber_hip_ctx* c, const saber_hip_seqset* queries, cons
         std::vector<int32_t> rthr(nq);
         for (uint32_t q = 0; q < nq; ++q) rthr[q] = prep[q].region_thr;
         if (int r = upload(d_rthr, rthr.data(), rthr.size(), st)) return r;
-        uint64_t nr = 0;
         if (nh) {
             RegionArgs ra{};
             ra.keys = keys.p;
@@ -1462,11 +1464,8 @@ int saber_hip_search_run(saber_hip_ctx* c, const saber_hip_seqset* queries, cons
             HIP_TRY(rocprim::select(sort_tmp.p, tb, rslot.p, rkeep.p, rsel.p, d_nh.p, (size_t)nh, st));
             HIP_TRY(hipMemcpyAsync(&nr, d_nh.p, sizeof nr, hipMemcpyDeviceToHost, st));
             HIP_TRY(hipStreamSynchronize(st));
-            regs_all.resize(nr);
-            if (nr)
-                HIP_TRY(hipMemcpyAsync(regs_all.data(), rsel.p, sizeof(saber_hip_region) * nr, hipMemcpyDeviceToHost, st));
         }
-        lap("regions + download");
+        lap("regions");
         unsigned long long cc[5];
         HIP_TRY(hipMemcpyAsync(cc, d_cctr.p, sizeof cc, hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
@@ -1479,136 +1478,84 @@ int saber_hip_search_run(saber_hip_ctx* c, const saber_hip_seqset* queries, cons
     keys.release();
     xres.release();
     sort_tmp.release();
-    // per query: region_order, the cap (scan.rs:1453-1466); the regions of a
-    // query are contiguous (keys are query-major), sorted in place
-    std::vector<uint64_t> rstart(nq + 1, 0);
-    for (const auto& r : regs_all) rstart[r.reserved + 1]++;
-    for (uint32_t q = 0; q < nq; ++q) rstart[q + 1] += rstart[q];
-    parallel_for(nq, [&](uint64_t q) {
-        std::sort(regs_all.begin() + rstart[q], regs_all.begin() + rstart[q + 1], region_before);
-    });
-    lap("region sort");
-    std::vector<uint64_t> roff(nq + 1, 0);
-    for (uint32_t q = 0; q < nq; ++q) {
-        const uint64_t n = rstart[q + 1] - rstart[q];
-        if (n > P->max_candidates) ctr.regions_dropped_by_cap += n - P->max_candidates;
-        roff[q + 1] = roff[q] + std::min<uint64_t>(n, P->max_candidates);
-    }
-    const uint64_t nreg = roff[nq];
-    ctr.regions = nreg;
-    host_vec<saber_hip_region> flat;
-    if (nreg == regs_all.size()) {
-        flat.swap(regs_all);
-    } else {
-        flat.resize(nreg);
-        for (uint32_t q = 0; q < nq; ++q)
-            std::copy(regs_all.begin() + rstart[q], regs_all.begin() + rstart[q] + (roff[q + 1] - roff[q]),
-                      flat.begin() + roff[q]);
-        regs_all.clear();
-        regs_all.shrink_to_fit();
-    }
-    auto region_count = [&](uint32_t q) { return roff[q + 1] - roff[q]; };
     tm.regions = ms_since(t0);
 
-    // ---- plans (plan_windows) ----
-    t0 = Clock::now();
-    // every entry is written below (per query, by plan_windows)
-    host_vec<saber_hip_plan> plans(nreg);
-    host_vec<uint32_t> qidx(nreg);
-    host_vec<double> thr(nreg);
-    host_vec<uint64_t> slen_of(nreg);
-    saber_hip_plan_params pp{P->window_slack, P->score_band, P->hsp_threshold, 0};
-    std::atomic<int> plan_err{0};
-    parallel_for(nq, [&](uint64_t q) {
-        const uint64_t n = region_count((uint32_t)q), o = roff[q];
-        if (!n) return;
-        for (uint64_t i = o; i < o + n; ++i) {
-            const saber_hip_region& r = flat[i];
-            slen_of[i] = c->db_off[r.subject + 1] - c->db_off[r.subject];
-            qidx[i] = (uint32_t)q;
-            // the two-stage cut (score_windows, engine.rs:1057-1068)
-            thr[i] = P->score_prefilter_frac *
-                     saber_hip_stats_floor_lower_bound(&prep[q].qs, region_e, NAN, slen_of[i], prep[q].region_floor);
-        }
-        if (saber_hip_plan_windows(flat.data() + o, n, qlen[q], slen_of.data() + o, &pp, plans.data() + o) != 0)
-            plan_err = 1;
-    });
-    if (plan_err) return fail(SABER_HIP_EINVAL, "plan_windows failed");
-    for (uint64_t i = 0; i < nreg; ++i)
-        if (plans[i].banded) ctr.band_scored_regions++;
-    tm.plan = ms_since(t0);
-
-    // ---- learned prefilter (optional) ----
-    t0 = Clock::now();
-    std::vector<uint8_t> pruned(nreg, 0);
+    // ---- cap, plans, thresholds, prefilter and both scoring passes on the device ----
+    std::vector<saber_hip_query_stats> qstats(nq);
+    std::vector<int64_t> floor_q(nq), rfloor_q(nq);
     std::vector<int32_t> rthr_q(nq);
-    for (uint32_t q = 0; q < nq; ++q) rthr_q[q] = prep[q].region_thr;
-    if (prefilter && nreg) {
-        if (mlp_context(prefilter->model) != c) return fail(SABER_HIP_EINVAL, "prefilter model of another context");
-        DevBuf<__bf16> X;
-        DevBuf<float> logits;
-        tl = Clock::now();
-        if (int r = region_features_dev(c, st, d_qoff.p, d_q.p + kGuard, nq, flat.data(), nreg, plans.data(),
-                                        qidx.data(), qlen.data(), slen_of.data(), rthr_q.data(), prefilter->mean,
-                                        prefilter->inv_std, X))
-            return r;
-        lap("prefilter features");
-        HIP_TRY(logits.alloc(nreg));
-        if (int r = mlp_forward_device(prefilter->model, X.p, nreg, logits.p, st)) return r;
-        lap("prefilter forward");
-        std::vector<float> lg(nreg);
-        HIP_TRY(hipMemcpyAsync(lg.data(), logits.p, sizeof(float) * nreg, hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipStreamSynchronize(st));
-        for (uint64_t i = 0; i < nreg; ++i) {
-            pruned[i] = lg[i] < prefilter->threshold ? 1 : 0;
-            ctr.regions_pruned += pruned[i];
-        }
-        lap("prefilter logits");
+    for (uint32_t q = 0; q < nq; ++q) {
+        qstats[q] = prep[q].qs;
+        floor_q[q] = prep[q].floor;
+        rfloor_q[q] = prep[q].region_floor;
+        rthr_q[q] = prep[q].region_thr;
     }
-    tm.prefilter = ms_since(t0);
-
-    // ---- scores (score_windows) ----
-    t0 = Clock::now();
-    std::vector<int32_t> scores(nreg, 0);
-    std::vector<uint8_t> rescored(nreg, 0);
+    const double ecut = P->evalue > 0.0 && std::isfinite(P->evalue) ? P->evalue : INFINITY;
+    SearchScoreIn si{};
+    si.st = st;
+    si.regions = rsel.p;
+    si.nregions = nr;
+    si.nq = nq;
+    si.d_qoff = d_qoff.p;
+    si.d_qlen = d_qlen.p;
+    si.d_qres = d_q.p + kGuard;
+    si.qtotal = qtotal;
+    si.d_qbias = any_bias ? d_bias.p + kGuard : nullptr;
+    si.qstats = qstats.data();
+    si.floor = floor_q.data();
+    si.region_floor = rfloor_q.data();
+    si.region_thr = rthr_q.data();
+    si.region_e = region_e;
+    si.ecut = ecut;
+    si.P = P;
+    si.scoring = scoring;
+    si.prefilter = prefilter;
+    si.keep_regions = P->keep_regions != 0;
+    SearchScoreOut so;
+    if (int r = search_score_device(c, si, &so)) return r;
+    rsel.release();
+    tm.regions += so.ms_regions;
+    tm.plan = so.ms_plan;
+    tm.prefilter = so.ms_prefilter;
+    tm.score = so.ms_score;
+    ctr.regions = so.regions;
+    ctr.regions_dropped_by_cap = so.dropped_by_cap;
+    ctr.band_scored_regions = so.band_scored;
+    ctr.band_rescored = so.rescored;
+    ctr.banded_cells = so.banded_cells;
+    ctr.regions_pruned = so.pruned;
+    const std::vector<uint64_t>& roff = so.roff;
+    const uint64_t nreg = so.regions;
+    auto region_count = [&](uint32_t q) { return roff[q + 1] - roff[q]; };
+    const std::vector<SearchCand>& cand = so.cands;   // in capped order, so grouped by query
+    std::vector<uint64_t> cstart(nq + 1, 0);
+    for (const SearchCand& x : cand) cstart[x.q + 1]++;
+    for (uint32_t q = 0; q < nq; ++q) cstart[q + 1] += cstart[q];
     saber_hip_seqset qs{res->qres.data(), queries->offsets, nq, any_bias ? bias_all.data() : nullptr};
-    c->mu.unlock();
-    uint64_t bcells = 0;
-    int sr = nreg ? score_regions_impl(c, &qs, nullptr, qidx.data(), flat.data(), plans.data(), nreg, scoring,
-                                       P->score_prefilter_band, P->score_band, thr.data(), scores.data(),
-                                       rescored.data(), &bcells, prefilter ? pruned.data() : nullptr)
-                  : SABER_HIP_OK;
-    c->mu.lock();
-    if (sr) return sr;
-    ctr.banded_cells = bcells;
-    for (uint64_t i = 0; i < nreg; ++i) ctr.band_rescored += rescored[i];
-    tm.score = ms_since(t0);
 
     // ---- finish (engine.rs:1127-1221) ----
     t0 = Clock::now();
-    const double ecut = P->evalue > 0.0 && std::isfinite(P->evalue) ? P->evalue : INFINITY;
+    // HitCand::ri indexes the candidates below
     std::vector<std::vector<HitCand>> order(nq);
-    std::vector<double> reg_e(nreg, INFINITY);
     parallel_for(nq, [&](uint64_t q) {
         const QPrep& pq = prep[q];
-        for (uint64_t i = roff[q]; i < roff[q + 1]; ++i) {
-            const int64_t s = scores[i];
+        for (uint64_t i = cstart[q]; i < cstart[q + 1]; ++i) {
+            const int64_t s = cand[i].score;
             if (!P->per_subject_evalues) {
                 if (s >= pq.floor) order[q].push_back(HitCand{saber_hip_stats_evalue(&pq.qs, s), i});
                 continue;
             }
             if (s <= 0) continue;
-            const uint64_t n = slen_of[i];
+            const uint64_t n = cand[i].slen;
             if (saber_hip_stats_evalue_in(&pq.qs, s, saber_hip_stats_space_lower_bound(&pq.qs, n)) > ecut) continue;
             const double e = saber_hip_stats_evalue_for(&pq.qs, s, n);
             if (e <= ecut) order[q].push_back(HitCand{e, i});
         }
         std::sort(order[q].begin(), order[q].end(), [&](const HitCand& a, const HitCand& b) {
             if (a.e != b.e) return a.e < b.e;
-            if (scores[a.ri] != scores[b.ri]) return scores[a.ri] > scores[b.ri];
-            return region_before(flat[a.ri], flat[b.ri]);
+            if (cand[a.ri].score != cand[b.ri].score) return cand[a.ri].score > cand[b.ri].score;
+            return region_before(cand[a.ri].r, cand[b.ri].r);
         });
-        for (const HitCand& h : order[q]) reg_e[h.ri] = h.e;
     });
     for (uint32_t q = 0; q < nq; ++q) ctr.regions_below_evalue += region_count(q) - order[q].size();
 
@@ -1632,7 +1579,7 @@ int saber_hip_search_run(saber_hip_ctx* c, const saber_hip_seqset* queries, cons
         std::map<uint64_t, Traced> traced;    // candidate index in order -> result
     };
     std::vector<LoopState> LS(nq);
-    std::vector<uint8_t> reported(nreg, 0);
+    std::vector<uint8_t> reported(cand.size(), 0);
     const uint64_t batch0 = (uint64_t)P->max_target_seqs * P->max_hsps;
     std::vector<uint64_t> want(nq, batch0);
     auto make_hit = [&](uint32_t q, const saber_hip_region& r, uint64_t lo, const Traced& T) {
@@ -1693,11 +1640,11 @@ int saber_hip_search_run(saber_hip_ctx* c, const saber_hip_seqset* queries, cons
             uint64_t taken = 0;
             for (size_t k = L.pos; k < order[q].size() && taken < want[q]; ++k) {
                 if (L.traced.count(k)) continue;
-                const saber_hip_region& r = flat[order[q][k].ri];
+                const saber_hip_region& r = cand[order[q][k].ri].r;
                 auto it = L.per_subject.find(r.subject);
                 if (it != L.per_subject.end() && it->second >= P->max_hsps) continue;
                 const uint64_t ri = order[q][k].ri;
-                const saber_hip_plan& pl = plans[ri];
+                const saber_hip_plan& pl = cand[ri].p;
                 saber_hip_traceback_job J{};
                 J.query = q;
                 J.subject = (uint32_t)r.subject;
@@ -1706,7 +1653,7 @@ int saber_hip_search_run(saber_hip_ctx* c, const saber_hip_seqset* queries, cons
                 J.d_min = r.d_min;
                 J.d_max = r.d_max;
                 J.best = r.best;
-                J.want = scores[ri];
+                J.want = cand[ri].score;
                 J.has_limit = pl.banded ? 1 : 0;
                 J.limit_lo = pl.band_lo;
                 J.limit_hi = pl.band_hi;
@@ -1749,7 +1696,7 @@ int saber_hip_search_run(saber_hip_ctx* c, const saber_hip_seqset* queries, cons
             auto& hits = res->hits[q];
             while (L.pos < order[q].size()) {
                 const uint64_t ri = order[q][L.pos].ri;
-                const saber_hip_region& r = flat[ri];
+                const saber_hip_region& r = cand[ri].r;
                 const uint64_t subj = r.subject;   // subject_group = 1
                 auto it = L.per_subject.find(subj);
                 const uint32_t cnt = it == L.per_subject.end() ? 0 : it->second;
@@ -1771,7 +1718,7 @@ int saber_hip_search_run(saber_hip_ctx* c, const saber_hip_seqset* queries, cons
                 ctr.banded_cells += T.tc.banded_cells;
                 ++L.pos;
                 if (T.a.status != 0) continue;   // no alignment
-                saber_hip_hit h = make_hit(q, r, plans[ri].lo, T);
+                saber_hip_hit h = make_hit(q, r, cand[ri].p.lo, T);
                 bool overlap = false;
                 if (cnt > 0)
                     for (const auto& o : hits)
@@ -1833,35 +1780,46 @@ int saber_hip_search_run(saber_hip_ctx* c, const saber_hip_seqset* queries, cons
     }
     tm.finish = ms_since(t0);
 
-    // ---- the regions as the caller may inspect them ----
-    for (uint32_t q = 0; q < nq; ++q) {
-        auto& R = res->regions[q];
-        R.resize(roff[q + 1] - roff[q]);
-        for (uint64_t i = roff[q]; i < roff[q + 1]; ++i) {
-            saber_hip_search_region& o = R[i - roff[q]];
-            const saber_hip_region& r = flat[i];
-            o.subject = r.subject;
-            o.d_min = r.d_min;
-            o.d_max = r.d_max;
-            o.best = r.best;
-            o.n_hsp = r.n_hsp;
-            o.hsp_sum = r.hsp_sum;
-            o.score = scores[i];
-            o.win_lo = plans[i].lo;
-            o.win_hi = plans[i].hi;
-            o.band_lo = plans[i].band_lo;
-            o.band_hi = plans[i].band_hi;
-            o.banded = plans[i].banded;
-            o.flags = (rescored[i] ? 1u : 0u) | (std::isfinite(reg_e[i]) ? 2u : 0u) | (reported[i] ? 4u : 0u) |
-                      (pruned[i] ? 8u : 0u);
-            o.evalue = reg_e[i];
+    // ---- the regions as the caller may inspect them (keep_regions) ----
+    if (P->keep_regions) {
+        std::vector<double> reg_e(nreg, INFINITY);
+        std::vector<uint8_t> rep(nreg, 0);
+        for (uint32_t q = 0; q < nq; ++q)
+            for (const HitCand& h : order[q]) {
+                reg_e[cand[h.ri].ri] = h.e;
+                rep[cand[h.ri].ri] = reported[h.ri];
+            }
+        for (uint32_t q = 0; q < nq; ++q) {
+            auto& R = res->regions[q];
+            R.resize(region_count(q));
+            for (uint64_t i = roff[q]; i < roff[q + 1]; ++i) {
+                saber_hip_search_region& o = R[i - roff[q]];
+                const saber_hip_region& r = so.flat[i];
+                const saber_hip_plan& pl = so.plans[i];
+                o.subject = r.subject;
+                o.d_min = r.d_min;
+                o.d_max = r.d_max;
+                o.best = r.best;
+                o.n_hsp = r.n_hsp;
+                o.hsp_sum = r.hsp_sum;
+                o.score = so.scores[i];
+                o.win_lo = pl.lo;
+                o.win_hi = pl.hi;
+                o.band_lo = pl.band_lo;
+                o.band_hi = pl.band_hi;
+                o.banded = pl.banded;
+                o.flags = (so.rescored_flag[i] ? 1u : 0u) | (std::isfinite(reg_e[i]) ? 2u : 0u) | (rep[i] ? 4u : 0u) |
+                          (so.pruned_flag[i] ? 8u : 0u);
+                o.evalue = reg_e[i];
+            }
         }
+        res->flat.assign(so.flat.begin(), so.flat.end());
+        res->plans.assign(so.plans.begin(), so.plans.end());
+        res->qidx.assign(so.qidx.begin(), so.qidx.end());
+        res->slen.assign(so.slen.begin(), so.slen.end());
     }
-    res->flat.swap(flat);
-    res->plans.swap(plans);
-    res->qidx.swap(qidx);
+    res->kept_regions = P->keep_regions != 0;
     res->qlen = qlen;
-    res->slen.swap(slen_of);
     res->rthr = rthr_q;
     res->db_gen = c->db_gen;
     tm.total = ms_since(t_start);
@@ -1916,6 +1874,7 @@ uint64_t saber_hip_search_region_count(const saber_hip_search* s, uint32_t q) {
 
 int saber_hip_search_regions(const saber_hip_search* s, uint32_t q, saber_hip_search_region* out) {
     if (!s || q >= s->nq) return fail(SABER_HIP_EINVAL, "bad search or query");
+    if (!s->kept_regions) return fail(SABER_HIP_ESTATE, "the search kept no regions (keep_regions = 0)");
     if (!s->regions[q].empty() && !out) return fail(SABER_HIP_EINVAL, "out is null");
     std::copy(s->regions[q].begin(), s->regions[q].end(), out);
     return SABER_HIP_OK;
@@ -1923,6 +1882,7 @@ int saber_hip_search_regions(const saber_hip_search* s, uint32_t q, saber_hip_se
 
 int saber_hip_search_features(saber_hip_search* s, const float* mean, const float* inv_std, uint16_t* out) {
     if (!s || !s->ctx) return fail(SABER_HIP_EINVAL, "null search");
+    if (!s->kept_regions) return fail(SABER_HIP_ESTATE, "the search kept no regions (keep_regions = 0)");
     const uint64_t n = s->flat.size();
     if (n == 0) return SABER_HIP_OK;
     if (!out) return fail(SABER_HIP_EINVAL, "out is null");

@@ -155,6 +155,9 @@ def run(ctx, qres, qoff, scheme, karlin, seeds=None, params=None, prefilter=None
         regions = keep_handle
     seeds = seeds or ScanSeeds.default()
     params = params or default_params()
+    if regions and not params.keep_regions:
+        params = _ffi.SearchParams.from_buffer_copy(params)
+        params.keep_regions = 1
     qres = np.ascontiguousarray(qres, np.uint8)
     qoff = np.ascontiguousarray(qoff, np.uint64)
     nq = len(qoff) - 1
