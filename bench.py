@@ -106,6 +106,7 @@ def prefilter_leg(ctx, args, sch):
     test = workload.family_set(seed=0, nq=nq, ndecoy=nd)
     ctx.upload_db(test["sres"], test["soff"])
     search.run(ctx, test["qres"], test["qoff"], sch, k)     # the database's seed-scan data
+    search.run(ctx, test["qres"], test["qoff"], sch, k, prefilter=search.prefilter_arg(pf))   # and the filter's buffers
     # both timed searches return the hits only (no region records)
     t1 = time.perf_counter()
     plain = search.run(ctx, test["qres"], test["qoff"], sch, k)
@@ -142,6 +143,9 @@ def prefilter_leg(ctx, args, sch):
            "banded_cells_plain": int(plain.counters["banded_cells"]),
            "banded_cells_filtered": int(filt.counters["banded_cells"]),
            "prefilter_stage_ms": round(filt.timing["prefilter"], 2),
+           "stages_ms_plain": {a: round(b, 1) for a, b in plain.timing.items()},
+           "stages_ms_filtered": {a: round(b, 1) for a, b in filt.timing.items()},
+           "wall_ms_plain": round(tp * 1e3, 1), "wall_ms_filtered": round(tf * 1e3, 1),
            "mlp_forward_ms": round(f_ms, 3), "mlp_forward_tflops": round(flop / (f_ms * 1e-3) / 1e12, 1),
            "mlp_forward_mfma_frac": round(flop / (f_ms * 1e-3) / 2.5e15, 4),
            "train_step_ms_65536": round(s_ms, 3),

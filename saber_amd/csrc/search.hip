@@ -1082,18 +1082,20 @@ int saber_hip_search_run(saber_hip_ctx* c, const saber_hip_seqset* queries, cons
     }();
     uint64_t maxq = 1;
     bool any_bias = false;
-    for (uint32_t q = 0; q < nq; ++q) {
+    // per query, on the host threads: composition bias, statistics, floors
+    std::atomic<int> prep_err{SABER_HIP_OK};
+    parallel_for(nq, [&](uint64_t q) {
         const uint64_t m = queries->offsets[q + 1] - queries->offsets[q];
-        maxq = std::max(maxq, m);
         QPrep& pp = prep[q];
         pp.bias.assign(m, 0);
         if (P->comp_bias && m)
             saber_hip_composition_bias(queries->residues + queries->offsets[q], m, scoring->matrix, pp.bias.data());
         for (int8_t b : pp.bias) pp.has_bias = pp.has_bias || b != 0;
-        any_bias = any_bias || pp.has_bias;
         if (int r = saber_hip_stats_for_query(karlin, db_len, ns, P->per_subject_evalues ? 1 : 0,
-                                              std::max<uint64_t>(m, 1), &pp.qs))
-            return r;
+                                              std::max<uint64_t>(m, 1), &pp.qs)) {
+            prep_err = r;
+            return;
+        }
         if (m == 0) {
             pp.floor = pp.region_floor = std::numeric_limits<int64_t>::max();
         } else {
@@ -1108,6 +1110,11 @@ int saber_hip_search_run(saber_hip_ctx* c, const saber_hip_seqset* queries, cons
         } else {
             pp.region_thr = P->hsp_threshold;
         }
+    });
+    if (prep_err != SABER_HIP_OK) return fail(prep_err, "query statistics");
+    for (uint32_t q = 0; q < nq; ++q) {
+        maxq = std::max<uint64_t>(maxq, queries->offsets[q + 1] - queries->offsets[q]);
+        any_bias = any_bias || prep[q].has_bias;
     }
     std::vector<int8_t> bias_all(qtotal, 0);
     for (uint32_t q = 0; q < nq; ++q)
