@@ -109,6 +109,31 @@ void saber_hip_destroy(saber_hip_ctx* ctx);
  * SeedIndex, src/search/index.rs:644-703). Replaces any previous upload. */
 int saber_hip_upload_db(saber_hip_ctx* ctx, const saber_hip_seqset* subjects);
 
+/* A SABER index file, format v4 (src/search/index.rs:1-52), opened for its
+ * residues and subject offsets: the sections `enc` and `enc_off` that
+ * SeedIndex::residues() / subject_offsets() return (index.rs:644-706), read
+ * through a read-only memory map. Open checks what SeedIndex::open
+ * (index.rs:947-1009) and validate_structure (:822-849) check of them --
+ * magic, version, byte order, header and metadata CRC-32, file size,
+ * section bounds and alignment, the offsets against the residue section --
+ * with the same messages; verify_sections also checks every section's
+ * CRC-32 (verify_checksums, :925-934). Postings are not read. */
+typedef struct saber_hip_index {
+    void* mapping;                   /* internal */
+    const uint8_t* residues;         /* total_residues alignment codes, every subject concatenated */
+    const uint64_t* offsets;         /* num_subjects + 1 subject starts */
+    uint64_t num_subjects, total_residues;
+    int32_t protein;                 /* kind: 1 Protein, 0 Nucleotide */
+    int32_t translated;              /* six-frame translated nucleotide database */
+} saber_hip_index;
+int saber_hip_index_open(const char* path, int verify_sections, saber_hip_index** out);
+void saber_hip_index_close(saber_hip_index* idx);
+/* saber_hip_upload_db of an opened index's residues and offsets. */
+int saber_hip_upload_index(saber_hip_ctx* ctx, const saber_hip_index* idx);
+/* CRC-32 of the index file (src/search/crc32.rs): IEEE, reflected, initial
+ * and final xor 0xFFFFFFFF. */
+uint32_t saber_hip_crc32(const uint8_t* data, uint64_t n);
+
 /* ScoreBackend::score_windows: exact full-window SW scores of one query
  * against n windows (windows[k] has lens[k] residues), in input order.
  * bias may be NULL. An empty query gives all zeros. */

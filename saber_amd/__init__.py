@@ -19,6 +19,7 @@ from dataclasses import dataclass
 import atexit
 import time
 import ctypes as C
+import os
 import weakref
 
 import numpy as np
@@ -293,6 +294,47 @@ class XdropBatch:
             pass
 
 
+class IndexFile:
+    """A SABER index file, format v4, opened for its residues and subject
+    offsets (saber_hip_index_open: the checks of SeedIndex::open,
+    src/search/index.rs:947-1009; verify also checks every section CRC-32).
+    residues / offsets are read-only views of the file's memory map."""
+
+    def __init__(self, path, verify=False):
+        h = C.POINTER(_ffi.IndexFile)()
+        _ffi.check(_ffi.lib().saber_hip_index_open(os.fsencode(path), int(bool(verify)), C.byref(h)))
+        self.h = h
+        x = h.contents
+        self.num_subjects, self.total_residues = int(x.num_subjects), int(x.total_residues)
+        self.protein, self.translated = bool(x.protein), bool(x.translated)
+        self.residues = np.ctypeslib.as_array(x.residues, shape=(max(self.total_residues, 1),))[:self.total_residues]
+        self.offsets = np.ctypeslib.as_array(x.offsets, shape=(self.num_subjects + 1,))
+
+    def close(self):
+        if self.h:
+            self.residues = self.offsets = None
+            _ffi.lib().saber_hip_index_close(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def crc32(data):
+    """CRC-32 of the index file format (saber_hip_crc32, src/search/crc32.rs)."""
+    b = np.frombuffer(bytes(data), np.uint8) if not isinstance(data, np.ndarray) else np.ascontiguousarray(data, np.uint8)
+    return int(_ffi.lib().saber_hip_crc32(b.ctypes.data_as(C.POINTER(C.c_uint8)), len(b)))
+
+
 def band_variants():
     """[(L, C, K, bias)] for every compiled packed-kernel variant."""
     out = []
@@ -463,6 +505,11 @@ class Context:
             int(target_cells), rep, 64, C.byref(n)))
         return [{"len_lo": e.w_lo, "len_hi": e.w_hi, "shape": (e.L, e.C), "changed": bool(e.reserved),
                  "cells": int(e.cells), "gcups": round(e.gcups, 1)} for e in rep[:min(n.value, 64)]]
+
+    def upload_index(self, index):
+        """Make an opened v4 index file's subjects the device database
+        (saber_hip_upload_index)."""
+        _ffi.check(_ffi.lib().saber_hip_upload_index(self.h, index.h))
 
     def upload_db(self, qres, qoff):
         off = np.ascontiguousarray(qoff, dtype=np.uint64)

@@ -26,6 +26,12 @@ class Scheme(C.Structure):
     _fields_ = [("matrix", C.c_int32 * 576), ("gap_open", C.c_int32), ("gap_extend", C.c_int32)]
 
 
+class IndexFile(C.Structure):
+    _fields_ = [("mapping", C.c_void_p), ("residues", C.POINTER(C.c_uint8)), ("offsets", C.POINTER(C.c_uint64)),
+                ("num_subjects", C.c_uint64), ("total_residues", C.c_uint64), ("protein", C.c_int32),
+                ("translated", C.c_int32)]
+
+
 class SeqSet(C.Structure):
     _fields_ = [
         ("residues", C.POINTER(C.c_uint8)),
@@ -196,6 +202,10 @@ _SIGS = {
     "saber_hip_create": (C.c_int, [C.c_int, C.POINTER(Opts), C.POINTER(P)]),
     "saber_hip_destroy": (None, [P]),
     "saber_hip_upload_db": (C.c_int, [P, C.POINTER(SeqSet)]),
+    "saber_hip_index_open": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.POINTER(IndexFile))]),
+    "saber_hip_index_close": (None, [C.POINTER(IndexFile)]),
+    "saber_hip_upload_index": (C.c_int, [P, C.POINTER(IndexFile)]),
+    "saber_hip_crc32": (C.c_uint32, [C.POINTER(C.c_uint8), C.c_uint64]),
     "saber_hip_score_windows": (C.c_int, [P, C.POINTER(C.c_uint8), C.c_uint32, C.POINTER(C.c_int8),
                                           C.POINTER(C.POINTER(C.c_uint8)), C.POINTER(C.c_uint32),
                                           C.c_uint32, C.POINTER(Scheme), C.POINTER(C.c_int32)]),

@@ -332,6 +332,13 @@ int saber_hip_upload_db(saber_hip_ctx* c, const saber_hip_seqset* subjects) {
     return SABER_HIP_OK;
 }
 
+int saber_hip_upload_index(saber_hip_ctx* c, const saber_hip_index* idx) {
+    if (!c || !idx) return fail(SABER_HIP_EINVAL, "null argument");
+    if (idx->num_subjects >= 0xFFFFFFFFull) return fail(SABER_HIP_EINVAL, "more than 2^32 - 1 subjects");
+    const saber_hip_seqset s{idx->residues, idx->offsets, (uint32_t)idx->num_subjects, nullptr};
+    return saber_hip_upload_db(c, &s);
+}
+
 int saber_hip_band_variant(int v, int* L, int* C, int* K, int* bias) {
     int nv = 0;
     const BandVariant* t = band_variants(&nv);
