@@ -63,12 +63,12 @@ __device__ __forceinline__ uint32_t full_from_prev_lane(uint32_t v) {
     X(16, 32) X(32, 24) X(32, 32) X(64, 24) X(64, 32) X(64, 40) X(64, 48)
 
 // Workgroup size of a shape's register class: 12 waves (3 per SIMD: 168
-// registers) unless the shape needs more registers (wide lanes; end-cell
-// snapshots): 8 waves (256 registers), or 4 waves for end-cell snapshots of
-// the widest lanes (which still spill a little; they only serve subjects
-// longer than 2048).
+// registers) for lanes of up to 32 columns, 8 waves (256 registers) for the
+// wider lanes; the end-cell build keeps a row and a column per half instead
+// of a copy of the lane's cells, so it needs no more registers than the
+// scores-only build.
 template <int C, bool ENDS>
-constexpr int kFullThreads = ENDS ? (C <= 24 ? 768 : C <= 32 ? 512 : 256) : (C <= 32 ? 768 : 512);
+constexpr int kFullThreads = C <= 32 ? 768 : 512;
 
 // One subject pair (pair index g of a.pairs, lane l of its L) against query
 // qid: the whole matrix, outputs at row qid of the [query][subject] arrays.
@@ -152,9 +152,7 @@ __device__ __attribute__((noinline)) void full_group(const FullKernelArgs& a, ld
 
     v2s best = ZERO;
     int brow[2] = {-1, -1};   // ENDS: row where the half's best last rose
-    uint32_t snap[ENDS ? C : 1];
-#pragma unroll
-    for (int c = 0; c < (ENDS ? C : 1); ++c) snap[c] = 0;
+    int bcol[2] = {C, C};     // ENDS: and the leftmost of the lane's cells holding it in that row
     uint32_t outH = 0u, outE = 0u;   // lane 0's left border: H = 0, E' = 0
     uint32_t diagIn = 0u;            // H(i-1, l*C-1) for the current row
 
@@ -199,14 +197,26 @@ __device__ __attribute__((noinline)) void full_group(const FullKernelArgs& a, ld
             outE = bits(E);
             diagIn = bits(inH);
             if constexpr (ENDS) {
-                // halves whose best rose: remember the row and its cells
+                // halves whose best rose: remember the row and the leftmost
+                // cell of the lane holding the new best (rare after the first
+                // rows, so behind a branch)
                 const bool r0 = rm.x > best.x, r1 = rm.y > best.y;
-                uint32_t rose = (r0 ? 0x0000FFFFu : 0u) | (r1 ? 0xFFFF0000u : 0u);
-                asm volatile("" : "+v"(rose));
-                brow[0] = r0 ? r : brow[0];
-                brow[1] = r1 ? r : brow[1];
+                if (r0 || r1) {
+                    int c0 = C, c1 = C;
 #pragma unroll
-                for (int c = 0; c < C; ++c) snap[c] = (bits(H[c]) & rose) | (snap[c] & ~rose);
+                    for (int c = C - 1; c >= 0; --c) {
+                        c0 = H[c].x == rm.x ? c : c0;
+                        c1 = H[c].y == rm.y ? c : c1;
+                    }
+                    if (r0) {
+                        brow[0] = r;
+                        bcol[0] = c0;
+                    }
+                    if (r1) {
+                        brow[1] = r;
+                        bcol[1] = c1;
+                    }
+                }
             }
             best = vmax(best, rm);
             // next row's query code and bias
@@ -248,16 +258,8 @@ __device__ __attribute__((noinline)) void full_group(const FullKernelArgs& a, ld
         }
         uint32_t key = 0xFFFFFFFFu;
         if constexpr (ENDS) {
-            if (top > 0 && top < kMax16 && mine == top) {
-                for (int c = 0; c < C; ++c) {
-                    const uint32_t v = snap[c];
-                    const int hv = h == 0 ? (int)(short)(v & 0xFFFFu) : (int)(short)(v >> 16);
-                    if (hv == top) {
-                        key = (uint32_t)brow[h] * (uint32_t)(L * C) + (uint32_t)(l * C + c);
-                        break;
-                    }
-                }
-            }
+            if (top > 0 && top < kMax16 && mine == top && bcol[h] < C)
+                key = (uint32_t)brow[h] * (uint32_t)(L * C) + (uint32_t)(l * C + bcol[h]);
 #pragma unroll
             for (int d = 1; d < L; d <<= 1) {
                 const uint32_t o = (uint32_t)__shfl_xor((int)key, d, L);

@@ -6,14 +6,12 @@ namespace saber {
 hipError_t launch_full_multi_bias_ends(int nt, const FullKernelArgs& a, uint32_t grid, hipStream_t s) {
     if (nt == 768) return launch_full_class<true, true, 768>(a, grid, s);
     if (nt == 512) return launch_full_class<true, true, 512>(a, grid, s);
-    if (nt == 256) return launch_full_class<true, true, 256>(a, grid, s);
     return hipErrorInvalidValue;
 }
 
 int full_multi_blocks_bias_ends(int nt) {
     if (nt == 768) return full_class_blocks_per_cu<true, true, 768>();
     if (nt == 512) return full_class_blocks_per_cu<true, true, 512>();
-    if (nt == 256) return full_class_blocks_per_cu<true, true, 256>();
     return 1;
 }
 
