@@ -40,6 +40,7 @@ The printed JSON line also carries
                 forward throughput against the bf16 MFMA peak.
 """
 import argparse
+import gc
 import json
 import os
 import sys
@@ -107,13 +108,17 @@ def prefilter_leg(ctx, args, sch):
     ctx.upload_db(test["sres"], test["soff"])
     search.run(ctx, test["qres"], test["qoff"], sch, k)     # the database's seed-scan data
     search.run(ctx, test["qres"], test["qoff"], sch, k, prefilter=search.prefilter_arg(pf))   # and the filter's buffers
-    # both timed searches return the hits only (no region records)
+    # both timed searches return the hits only (no region records); Python's
+    # cyclic garbage collector is paused around them (the engine has none)
+    gc.collect()
+    gc.disable()
     t1 = time.perf_counter()
     plain = search.run(ctx, test["qres"], test["qoff"], sch, k)
     tp = time.perf_counter() - t1
     t1 = time.perf_counter()
     filt = search.run(ctx, test["qres"], test["qoff"], sch, k, prefilter=search.prefilter_arg(pf))
     tf = time.perf_counter() - t1
+    gc.enable()
     # the plain search again, keeping its regions for their features (untimed)
     kept = search.run(ctx, test["qres"], test["qoff"], sch, k, keep_handle=True)
     pp, fp = search.hit_pairs(plain), search.hit_pairs(filt)
@@ -228,9 +233,12 @@ def search_leg(ctx, args, sch):
     t0 = time.perf_counter()
     g0 = search.run(ctx, d["qres"], d["qoff"], sch, k)
     first = time.perf_counter() - t0
+    gc.collect()
+    gc.disable()
     t0 = time.perf_counter()
     g = search.run(ctx, d["qres"], d["qoff"], sch, k)
     wall = time.perf_counter() - t0
+    gc.enable()
     fam = d["family"]
     # every query finds itself, with at least the best ungapped segment of
     # its main diagonal (a lower bound of the local alignment score)
