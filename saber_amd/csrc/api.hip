@@ -209,23 +209,27 @@ void default_choice(saber_hip_ctx* c) {
     }
     int wmax = 0;
     for (auto& s : c->shapes) wmax = std::max(wmax, s.W());
-    c->choice.assign((size_t)wmax + 1, -1);
-    c->choice_lat.assign((size_t)wmax + 1, -1);
-    for (int w = 1; w <= wmax; ++w) {
-        int best = -1, fast = -1;
-        for (int k = 0; k < (int)c->shapes.size(); ++k) {
-            const Shape& s = c->shapes[k];
-            if (!s.covers(w)) continue;
-            // throughput: least work per pair-row; latency: least per-row time,
-            // and among equals the widest shape, so that the few many-row
-            // jobs of all widths share one launch
-            if (best < 0 || s.pair_cost() < c->shapes[best].pair_cost()) best = k;
-            if (fast < 0 || s.row_cost() < c->shapes[fast].row_cost() ||
-                (s.row_cost() == c->shapes[fast].row_cost() && s.W() > c->shapes[fast].W()))
-                fast = k;
+    for (int e = 0; e < 2; ++e) {
+        std::vector<int>& ch = e ? c->choice_ends : c->choice;
+        std::vector<int>& lat = e ? c->choice_lat_ends : c->choice_lat;
+        ch.assign((size_t)wmax + 1, -1);
+        lat.assign((size_t)wmax + 1, -1);
+        for (int w = 1; w <= wmax; ++w) {
+            int best = -1, fast = -1;
+            for (int k = 0; k < (int)c->shapes.size(); ++k) {
+                const Shape& s = c->shapes[k];
+                if (!s.covers(w) || (e && !s.ends_ok())) continue;
+                // throughput: least work per pair-row; latency: least per-row
+                // time, and among equals the widest shape, so that the few
+                // many-row jobs of all widths share one launch
+                if (best < 0 || s.pair_cost() < c->shapes[best].pair_cost()) best = k;
+                if (fast < 0 || s.row_cost() < c->shapes[fast].row_cost() ||
+                    (s.row_cost() == c->shapes[fast].row_cost() && s.W() > c->shapes[fast].W()))
+                    fast = k;
+            }
+            ch[w] = best;
+            lat[w] = fast;
         }
-        c->choice[w] = best;
-        c->choice_lat[w] = fast;
     }
 }
 
@@ -367,7 +371,10 @@ int saber_hip_band_choose(saber_hip_ctx* c, int w_lo, int w_hi, int L, int C, in
         if (L != 0) return fail(SABER_HIP_EINVAL, "width above the widest variant");
         w_hi = (int)c->choice.size() - 1;
     }
-    for (int w = w_lo; w <= w_hi; ++w) c->choice[w] = c->choice_lat[w] = shape;
+    for (int w = w_lo; w <= w_hi; ++w) {
+        c->choice[w] = c->choice_lat[w] = shape;
+        if (c->shapes[shape].ends_ok()) c->choice_ends[w] = c->choice_lat_ends[w] = shape;
+    }
     return SABER_HIP_OK;
 }
 
@@ -601,7 +608,8 @@ int plan_on_host(saber_hip_ctx* c, hipStream_t st, const BandPlanInput& in, sabe
         // The packed kernel keeps row indices in 16 bits and window columns
         // (times 8) in 32 bits.
         const bool fits = d.rows <= (int32_t)kMaxBandRows && m < kMaxPackedLen && n < kMaxPackedLen;
-        const int shape = (in.packed_ok && fits && w < (int64_t)c->choice.size()) ? c->choice[w] : -1;
+        const std::vector<int>& choice = in.ends ? c->choice_ends : c->choice;
+        const int shape = (in.packed_ok && fits && w < (int64_t)choice.size()) ? choice[w] : -1;
         if (shape < 0) {
             generic.push_back((uint32_t)k);
             continue;
@@ -623,7 +631,7 @@ int plan_on_host(saber_hip_ctx* c, hipStream_t st, const BandPlanInput& in, sabe
             lat = c->latency_rows > 0 && rows_of[k] > c->latency_rows;
         const int w = host_jobs[k].w;
         if (lat)
-            bucket[2 * c->choice_lat[w] + 1].push_back((uint32_t)k);
+            bucket[2 * (in.ends ? c->choice_lat_ends : c->choice_lat)[w] + 1].push_back((uint32_t)k);
         else
             bucket[2 * shape].push_back((uint32_t)k);
     }
@@ -831,6 +839,7 @@ int saber::band_batch_create_impl(saber_hip_ctx* c, const saber_hip_seqset* quer
     hipStream_t st = c->stream;
     BandPlanInput in{jobs, njobs, dev ? nullptr : queries->offsets, dev ? dev->qcount : queries->count, soffs,
                      subjects ? nullptr : c->db_off_dev.p, nsub, packed_ok, swap_ok};
+    in.ends = b->ends;
     if (dev) {
         in.jobs_dev = dev->jobs;
         in.qoffs_dev = dev->qoffs;

@@ -456,7 +456,8 @@ __device__ __attribute__((noinline)) void band_pair(band_args a, const int8_t* T
     X(2, 9, 9) X(2, 13, 13) X(2, 17, 8) X(2, 17, 17)                                    \
     X(4, 9, 9) X(4, 13, 13) X(4, 17, 8) X(4, 17, 17)                                    \
     X(8, 9, 9) X(8, 13, 13) X(8, 17, 8) X(8, 17, 17)                                    \
-    X(16, 9, 9) X(16, 13, 13) X(16, 17, 16) X(16, 17, 17) X(16, 21, 21)
+    X(16, 9, 9) X(16, 13, 13) X(16, 17, 16) X(16, 17, 17) X(16, 21, 21)                \
+    X(2, 33, 2) X(4, 33, 4)
 
 // All shapes in one persistent launch: one workgroup of 12 waves per CU
 // stages both score tables in LDS once, then each wave claims work chunks in

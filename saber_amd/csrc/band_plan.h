@@ -73,6 +73,7 @@ struct BandPlanInput {
     bool packed_ok, swap_ok;
     const saber_hip_band_job* jobs_dev = nullptr;
     const uint64_t* qoffs_dev = nullptr;
+    bool ends = false;   // end cells: the shape choice of end-cell batches
 };
 
 // A plan as the band batch runs it. Device buffers: jobs and cells (one per

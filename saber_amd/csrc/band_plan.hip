@@ -461,9 +461,10 @@ int band_plan_device(saber_hip_ctx* c, hipStream_t st, const BandPlanInput& in, 
         if (int r = upload_keep(S.soffs, in.soffs, (uint64_t)in.nsub + 1, st)) return r;
         soffs = S.soffs.p;
     }
-    if (int r = upload_keep(S.choice, reinterpret_cast<const int32_t*>(c->choice.data()), c->choice.size(), st)) return r;
-    if (int r = upload_keep(S.choice_lat, reinterpret_cast<const int32_t*>(c->choice_lat.data()), c->choice_lat.size(),
-                            st))
+    const std::vector<int>& choice = in.ends ? c->choice_ends : c->choice;
+    const std::vector<int>& choice_lat = in.ends ? c->choice_lat_ends : c->choice_lat;
+    if (int r = upload_keep(S.choice, reinterpret_cast<const int32_t*>(choice.data()), choice.size(), st)) return r;
+    if (int r = upload_keep(S.choice_lat, reinterpret_cast<const int32_t*>(choice_lat.data()), choice_lat.size(), st))
         return r;
     if (int r = upload_keep(S.shapes, shp.data(), shp.size(), st)) return r;
     HIP_TRY(S.tot.reserve(1));
@@ -484,7 +485,7 @@ int band_plan_device(saber_hip_ctx* c, hipStream_t st, const BandPlanInput& in, 
     ja.soffs = soffs;
     ja.nsub = in.nsub;
     ja.choice = S.choice.p;
-    ja.nchoice = (uint32_t)c->choice.size();
+    ja.nchoice = (uint32_t)choice.size();
     ja.shapes = S.shapes.p;
     ja.packed_ok = in.packed_ok;
     ja.swap_ok = in.swap_ok;

@@ -60,6 +60,9 @@ struct Shape {
     // Every lane caps its last K cells, so a shape serves widths W-K+1 .. W,
     // and every width up to W when K = C.
     bool covers(int64_t w) const { return w >= 1 && w <= W() && (K == C || w > W() - K); }
+    // The end-cell build keeps a register copy of a lane's cells of the row
+    // where its best last rose; lanes of more than 21 cells would spill it.
+    bool ends_ok() const { return C <= 21; }
     // VALU instructions of one lane per row: about 13 per packed cell, one per
     // capped cell and a fixed part for the row and stream bookkeeping.
     double row_cost() const { return 13.0 * C + K + 45.0; }
@@ -184,6 +187,7 @@ struct saber_hip_ctx {
     std::map<int, int> full_tuned[2];
     std::vector<int> choice;      // width -> shape index (-1: exact int32 kernel)
     std::vector<int> choice_lat;  // the same for jobs with many rows
+    std::vector<int> choice_ends, choice_lat_ends;  // the same for end-cell batches
     std::vector<saber::Shape> shapes;
     int latency_rows = -1;  // -1: kLatencyShare rule; 0: off; else a row count
     std::unique_ptr<saber::ScanDbCache, saber::ScanCacheDeleter> scan;
