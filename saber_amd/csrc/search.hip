@@ -227,17 +227,20 @@ struct QueryKeyArgs {
     uint64_t total;
     ShapeDev sh;
     uint32_t asize;
-    uint32_t inv;            // the invalid code (ncodes)
+    uint32_t inv;            // ncodes: the key of an over-frequent seed; ncodes + 1: no seed
     const uint64_t* freq;    // over-frequent codes (null: none)
     uint32_t* keys;
     uint32_t* vals;
-    unsigned long long* ctr; // [0] seeds kept, [1] over-frequent seeds skipped
 };
 
+// The seed code of every batch position as a sort key. The counts of kept
+// and over-frequent seeds are read off the sorted keys afterwards (the table
+// offsets of codes ncodes and ncodes + 1), not counted here: one counter
+// update per wave, from every wave of a short kernel, would serialise on the
+// counter's address.
 __global__ __launch_bounds__(256) void query_keys_kernel(QueryKeyArgs a) {
     const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t key = a.inv;
-    uint32_t kept = 0, skipped = 0;
+    uint32_t key = a.inv + 1;
     if (g < a.total) {
         const uint32_t q = a.qid[g];
         if (q != kNone) {
@@ -253,30 +256,26 @@ __global__ __launch_bounds__(256) void query_keys_kernel(QueryKeyArgs a) {
                         c = c * a.asize + (ok ? r : 0);
                     }
                 }
-                if (ok) {
-                    if (a.freq && ((a.freq[c >> 6] >> (c & 63)) & 1)) {
-                        skipped = 1;
-                    } else {
-                        key = c;
-                        kept = 1;
-                    }
-                }
+                if (ok) key = (a.freq && ((a.freq[c >> 6] >> (c & 63)) & 1)) ? a.inv : c;
             }
         }
         a.keys[g] = key;
         a.vals[g] = (uint32_t)g;
     }
-    const uint64_t k2 = wave_sum64(kept), s2 = wave_sum64(skipped);
-    if ((threadIdx.x & 63) == 0) {
-        if (k2) atomicAdd(&a.ctr[0], (unsigned long long)k2);
-        if (s2) atomicAdd(&a.ctr[1], (unsigned long long)s2);
+}
+
+// seeds kept (offs[nc]) and over-frequent seeds skipped (offs[nc + 1] - offs[nc])
+__global__ void table_counts_kernel(const uint32_t* offs, uint64_t nc, unsigned long long* ctr) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        ctr[0] = offs[nc];
+        ctr[1] = offs[nc + 1] - offs[nc];
     }
 }
 
-// offs[c] = first sorted entry with code >= c, for c in [0, ncodes]
-__global__ void table_offsets_kernel(const uint32_t* keys, uint32_t n, uint64_t ncodes, uint32_t* offs) {
+// offs[c] = first sorted entry with code >= c, for c in [0, last]
+__global__ void table_offsets_kernel(const uint32_t* keys, uint32_t n, uint64_t last, uint32_t* offs) {
     const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c > ncodes) return;
+    if (c > last) return;
     uint32_t lo = 0, hi = n;
     while (lo < hi) {
         const uint32_t mid = (lo + hi) >> 1;
@@ -334,6 +333,7 @@ __device__ __forceinline__ void flush_staged(const ScanArgs& a, uint64_t* st, ui
 }
 
 __global__ __launch_bounds__(256) void scan_kernel(ScanArgs a) {
+    __shared__ unsigned long long s_ctr[3];
     __shared__ uint32_t s_excl[kScanWaves][64];
     __shared__ uint32_t s_ea[kScanWaves][64];
     __shared__ uint32_t s_sv[kScanWaves][64][8];
@@ -341,6 +341,8 @@ __global__ __launch_bounds__(256) void scan_kernel(ScanArgs a) {
     __shared__ uint64_t s_out[kScanWaves][kStage];
     uint32_t staged = 0;   // wave-uniform
     const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (threadIdx.x < 3) s_ctr[threadIdx.x] = 0;
+    __syncthreads();
     const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
     uint64_t n_cand = 0, n_hits = 0, n_pass = 0;
     for (uint64_t t = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; t < a.ntiles; t += nw) {
@@ -433,11 +435,15 @@ __global__ __launch_bounds__(256) void scan_kernel(ScanArgs a) {
         __builtin_amdgcn_wave_barrier();
         flush_staged(a, s_out[wv], staged, lane);
     }
+    // counters: summed in the workgroup first (one global update per
+    // workgroup and counter instead of one per wave)
     if (lane == 0) {
-        if (n_cand) atomicAdd(&a.ctr[0], (unsigned long long)n_cand);
-        if (n_hits) atomicAdd(&a.ctr[1], (unsigned long long)n_hits);
-        if (n_hits - n_pass) atomicAdd(&a.ctr[2], (unsigned long long)(n_hits - n_pass));
+        atomicAdd(&s_ctr[0], (unsigned long long)n_cand);
+        atomicAdd(&s_ctr[1], (unsigned long long)n_hits);
+        atomicAdd(&s_ctr[2], (unsigned long long)(n_hits - n_pass));
     }
+    __syncthreads();
+    if (threadIdx.x < 3 && s_ctr[threadIdx.x]) atomicAdd(&a.ctr[threadIdx.x], s_ctr[threadIdx.x]);
 }
 
 // ---- survivors -> X-drop descriptors ----
@@ -1173,7 +1179,6 @@ int saber_hip_search_run(saber_hip_ctx* c, const saber_hip_seqset* queries, cons
     std::vector<std::unique_ptr<Table>> tabs;
     DevBuf<unsigned long long> d_tctr;
     HIP_TRY(d_tctr.alloc(2 * shapes.size()));
-    HIP_TRY(hipMemsetAsync(d_tctr.p, 0, sizeof(unsigned long long) * 2 * shapes.size(), st));
     DevBuf<uint32_t> tmp_keys;
     HIP_TRY(tmp_keys.alloc(total));
     DevBuf<uint8_t> sort_tmp;
@@ -1194,14 +1199,13 @@ int saber_hip_search_run(saber_hip_ctx* c, const saber_hip_seqset* queries, cons
         qa.freq = cache->freq_off[si] >= 0 ? cache->freq.p + cache->freq_off[si] : nullptr;
         qa.keys = tmp_keys.p;
         qa.vals = T->vals.p;
-        qa.ctr = d_tctr.p + 2 * si;
         SABER_LAUNCH(query_keys_kernel, dim3(grid_for(total, 256)), dim3(256), 0, st, qa);
         HIP_TRY(hipGetLastError());
         // stable sort by code; the values (batch positions) keep batch order per code
         DevBuf<uint32_t> vals_out;
         HIP_TRY(vals_out.alloc(total));
         size_t tb = 0;
-        const unsigned end_bit = bits_for(sh.ncodes);
+        const unsigned end_bit = bits_for(sh.ncodes + 1);
         HIP_TRY(rocprim::radix_sort_pairs(nullptr, tb, tmp_keys.p, T->keys.p, T->vals.p, vals_out.p, (size_t)total,
                                           0, end_bit, st));
         HIP_TRY(sort_tmp.reserve(tb));
@@ -1209,6 +1213,17 @@ int saber_hip_search_run(saber_hip_ctx* c, const saber_hip_seqset* queries, cons
                                           (size_t)total, 0, end_bit, st));
         std::swap(T->vals.p, vals_out.p);
         std::swap(T->vals.n, vals_out.n);
+        // code offsets over all the batch's keys (those of over-frequent
+        // seeds and of positions without a seed sort last)
+        const uint64_t nc = sh.ncodes;
+        HIP_TRY(T->offs.alloc(nc + 2));
+        HIP_TRY(T->bits.alloc((nc + 63) / 64));
+        SABER_LAUNCH(table_offsets_kernel, dim3(grid_for(nc + 2, 256)), dim3(256), 0, st, T->keys.p, (uint32_t)total,
+                     nc + 1, T->offs.p);
+        SABER_LAUNCH(table_bits_kernel, dim3(grid_for((nc + 63) / 64, 256)), dim3(256), 0, st, T->offs.p, nc,
+                     T->bits.p);
+        SABER_LAUNCH(table_counts_kernel, dim3(1), dim3(64), 0, st, T->offs.p, nc, d_tctr.p + 2 * si);
+        HIP_TRY(hipGetLastError());
         tabs.push_back(std::move(T));
     }
     std::vector<unsigned long long> tctr(2 * shapes.size());
@@ -1219,18 +1234,9 @@ int saber_hip_search_run(saber_hip_ctx* c, const saber_hip_seqset* queries, cons
         T.n = tctr[2 * si];
         ctr.seed_positions += tctr[2 * si];
         ctr.seeds_frequent_skipped += tctr[2 * si + 1];
-        const uint64_t nc = shapes[si].ncodes;
-        HIP_TRY(T.offs.alloc(nc + 1));
-        HIP_TRY(T.bits.alloc((nc + 63) / 64));
-        SABER_LAUNCH(table_offsets_kernel, dim3(grid_for(nc + 1, 256)), dim3(256), 0, st, T.keys.p,
-                           (uint32_t)T.n, nc, T.offs.p);
-        SABER_LAUNCH(table_bits_kernel, dim3(grid_for((nc + 63) / 64, 256)), dim3(256), 0, st, T.offs.p, nc,
-                           T.bits.p);
-        HIP_TRY(hipGetLastError());
         T.keys.release();
     }
     tmp_keys.release();
-    HIP_TRY(hipStreamSynchronize(st));
     tm.seed_tables = ms_since(t0);
 
     // ---- the scan ----
