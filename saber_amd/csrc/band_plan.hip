@@ -92,6 +92,13 @@ struct JobArgs {
 // Per job: validation, clipped band, rows, cells, transposition, shape
 // (the host planner's job loop), and the totals.
 __global__ __launch_bounds__(kThreads) void plan_jobs_kernel(JobArgs a) {
+    __shared__ PlanTotals bt;   // the workgroup's totals (seg_count unused)
+    if (threadIdx.x == 0) {
+        bt.first_err = kNoErr;
+        bt.est = bt.max_qoff = bt.gen_work = 0;
+        bt.max_ifirst = bt.wmax = bt.npacked = bt.ngeneric = 0;
+    }
+    __syncthreads();
     unsigned long long err = kNoErr, est = 0, qmax = 0, gw = 0;
     unsigned int imax = 0, wmax = 0, npk = 0, ngen = 0;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -183,16 +190,29 @@ __global__ __launch_bounds__(kThreads) void plan_jobs_kernel(JobArgs a) {
         npk += __shfl_xor(npk, o);
         ngen += __shfl_xor(ngen, o);
     }
+    // the waves' totals into the workgroup's, then one global update per
+    // workgroup and field (updates of one address serialise)
     if (__lane_id() == 0) {
+        if (err != kNoErr) atomicMin(&bt.first_err, err);
+        if (est) atomicAdd(&bt.est, est);
+        if (qmax) atomicMax(&bt.max_qoff, qmax);
+        if (gw) atomicAdd(&bt.gen_work, gw);
+        if (imax) atomicMax(&bt.max_ifirst, imax);
+        if (wmax) atomicMax(&bt.wmax, wmax);
+        if (npk) atomicAdd(&bt.npacked, npk);
+        if (ngen) atomicAdd(&bt.ngeneric, ngen);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
         PlanTotals* t = a.tot;
-        if (err != kNoErr) atomicMin(&t->first_err, err);
-        if (est) atomicAdd(&t->est, est);
-        if (qmax) atomicMax(&t->max_qoff, qmax);
-        if (gw) atomicAdd(&t->gen_work, gw);
-        if (imax) atomicMax(&t->max_ifirst, imax);
-        if (wmax) atomicMax(&t->wmax, wmax);
-        if (npk) atomicAdd(&t->npacked, npk);
-        if (ngen) atomicAdd(&t->ngeneric, ngen);
+        if (bt.first_err != kNoErr) atomicMin(&t->first_err, bt.first_err);
+        if (bt.est) atomicAdd(&t->est, bt.est);
+        if (bt.max_qoff) atomicMax(&t->max_qoff, bt.max_qoff);
+        if (bt.gen_work) atomicAdd(&t->gen_work, bt.gen_work);
+        if (bt.max_ifirst) atomicMax(&t->max_ifirst, bt.max_ifirst);
+        if (bt.wmax) atomicMax(&t->wmax, bt.wmax);
+        if (bt.npacked) atomicAdd(&t->npacked, bt.npacked);
+        if (bt.ngeneric) atomicAdd(&t->ngeneric, bt.ngeneric);
     }
 }
 
@@ -360,7 +380,9 @@ __global__ __launch_bounds__(kThreads) void plan_gen_size_kernel(const uint32_t*
         sz[i] = 4ull * ((uint64_t)jobs[ids[i]].w + 1);
 }
 
-uint32_t grid_for(uint64_t n) { return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + kThreads - 1) / kThreads, 8192)); }
+// Every kernel here loops over its range with the grid's stride; 2048
+// workgroups fill the device once.
+uint32_t grid_for(uint64_t n) { return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + kThreads - 1) / kThreads, 2048)); }
 
 int bits_for(uint64_t x) {
     int b = 0;

@@ -493,6 +493,9 @@ struct CoverArgs {
 // along a diagonal); the survivors jumped over are the avoided ones. The HSP
 // flags are cleared before the launch.
 __global__ __launch_bounds__(256) void coverage_kernel(CoverArgs a) {
+    __shared__ unsigned long long s_ctr[4];
+    if (threadIdx.x < 4) s_ctr[threadIdx.x] = 0;
+    __syncthreads();
     uint64_t ext = 0, avoided = 0, steps = 0, groups = 0;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += stride) {
@@ -539,12 +542,15 @@ __global__ __launch_bounds__(256) void coverage_kernel(CoverArgs a) {
     avoided = wave_sum64(avoided);
     steps = wave_sum64(steps);
     groups = wave_sum64(groups);
+    // summed per workgroup, then one global update per workgroup and counter
     if ((threadIdx.x & 63) == 0) {
-        if (ext) atomicAdd(&a.ctr[0], (unsigned long long)ext);
-        if (avoided) atomicAdd(&a.ctr[1], (unsigned long long)avoided);
-        if (steps) atomicAdd(&a.ctr[2], (unsigned long long)steps);
-        if (groups) atomicAdd(&a.ctr[3], (unsigned long long)groups);
+        atomicAdd(&s_ctr[0], (unsigned long long)ext);
+        atomicAdd(&s_ctr[1], (unsigned long long)avoided);
+        atomicAdd(&s_ctr[2], (unsigned long long)steps);
+        atomicAdd(&s_ctr[3], (unsigned long long)groups);
     }
+    __syncthreads();
+    if (threadIdx.x < 4 && s_ctr[threadIdx.x]) atomicAdd(&a.ctr[threadIdx.x], s_ctr[threadIdx.x]);
 }
 
 // ---- HSPs -> regions (finish_subject_into) ----
@@ -570,10 +576,15 @@ struct RegAcc {
     uint32_t n;
 };
 
+// Grid-stride over the HSPs; the weak-region count is summed per workgroup
+// and added to the counter once per workgroup.
 __global__ __launch_bounds__(256) void region_kernel(RegionArgs a) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ unsigned long long s_weak;
+    if (threadIdx.x == 0) s_weak = 0;
+    __syncthreads();
     uint64_t weak = 0;
-    if (i < a.nh) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.nh;
+         i += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t k0 = a.keys[a.hidx[i]];
         const uint64_t pair = a.kl.pair_seg(k0);
         if (i == 0 || a.kl.pair_seg(a.keys[a.hidx[i - 1]]) != pair) {
@@ -639,7 +650,9 @@ __global__ __launch_bounds__(256) void region_kernel(RegionArgs a) {
         }
     }
     weak = wave_sum64(weak);
-    if ((threadIdx.x & 63) == 0 && weak) atomicAdd(&a.ctr[0], (unsigned long long)weak);
+    if ((threadIdx.x & 63) == 0 && weak) atomicAdd(&s_weak, (unsigned long long)weak);
+    __syncthreads();
+    if (threadIdx.x == 0 && s_weak) atomicAdd(&a.ctr[0], s_weak);
 }
 
 }  // namespace
@@ -1469,7 +1482,7 @@ int saber_hip_search_run(saber_hip_ctx* c, const saber_hip_seqset* queries, cons
             ra.out = rslot.p;
             ra.keep = rkeep.p;
             ra.ctr = d_cctr.p + 4;
-            SABER_LAUNCH(region_kernel, dim3(grid_for(nh, 256)), dim3(256), 0, st, ra);
+            SABER_LAUNCH(region_kernel, dim3(std::min(grid_for(nh, 256), c->cus * 16)), dim3(256), 0, st, ra);
             HIP_TRY(hipGetLastError());
             size_t tb = 0;
             HIP_TRY(rocprim::select(nullptr, tb, rslot.p, rkeep.p, rsel.p, d_nh.p, (size_t)nh, st));

@@ -46,7 +46,10 @@ double ms_since(Clock::time_point t) { return std::chrono::duration<double, std:
 constexpr uint32_t kT = 256;
 constexpr uint32_t kMaxGroup = 8;  // regions of one (query, subject): at most max_regions_per_subject <= 8
 
-uint32_t grid_of(uint64_t n) { return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + kT - 1) / kT, 1u << 20)); }
+// Every kernel here loops over its range with the grid's stride: 2048
+// workgroups fill the device once and keep the per-wave counter updates of
+// the counting kernels few (updates of one address serialise).
+uint32_t grid_of(uint64_t n) { return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + kT - 1) / kT, 2048)); }
 int bits_of(uint64_t x) {
     int b = 0;
     while (b < 64 && (x >> b)) ++b;
