@@ -1714,11 +1714,14 @@ int saber_hip_search_run(saber_hip_ctx* c, const saber_hip_seqset* queries, cons
                 LS[job_of[x].first].traced.emplace(job_of[x].second, std::move(T));
             }
         }
-        // replay the loops
-        bool open = false;
-        for (uint32_t q = 0; q < nq; ++q) {
+        // replay the loops: each query's loop, hits and counters are its own
+        std::vector<uint8_t> qopen(nq, 0);
+        std::vector<saber_hip_search_counters> qc(nq);
+        parallel_for(nq, [&](uint64_t qq) {
+            const uint32_t q = (uint32_t)qq;
             LoopState& L = LS[q];
-            if (L.done) continue;
+            if (L.done) return;
+            saber_hip_search_counters& ctr = qc[q];
             auto& hits = res->hits[q];
             while (L.pos < order[q].size()) {
                 const uint64_t ri = order[q][L.pos].ri;
@@ -1770,13 +1773,25 @@ int saber_hip_search_run(saber_hip_ctx* c, const saber_hip_seqset* queries, cons
             }
             if (L.pos >= order[q].size()) L.done = true;
             if (!L.done) {
-                open = true;
+                qopen[q] = 1;
                 want[q] *= 2;
             }
+        });
+        bool open = false;
+        for (uint32_t q = 0; q < nq; ++q) {
+            const saber_hip_search_counters& x = qc[q];
+            ctr.tracebacks += x.tracebacks;
+            ctr.band_widenings += x.band_widenings;
+            ctr.band_capped += x.band_capped;
+            ctr.end_anchored_tracebacks += x.end_anchored_tracebacks;
+            ctr.banded_cells += x.banded_cells;
+            ctr.filtered_identity_coverage += x.filtered_identity_coverage;
+            open = open || qopen[q];
         }
         if (!open) break;
     }
-    for (uint32_t q = 0; q < nq; ++q) {
+    parallel_for(nq, [&](uint64_t qq) {
+        const uint32_t q = (uint32_t)qq;
         auto& hits = res->hits[q];
         // the operations move with their hits
         std::vector<size_t> idx(hits.size());
@@ -1802,8 +1817,8 @@ int saber_hip_search_run(saber_hip_ctx* c, const saber_hip_seqset* queries, cons
         hits.swap(sorted);
         res->op_codes[q].swap(oc);
         res->op_lens[q].swap(ol);
-        ctr.alignments += hits.size();
-    }
+    });
+    for (uint32_t q = 0; q < nq; ++q) ctr.alignments += res->hits[q].size();
     tm.finish = ms_since(t0);
 
     // ---- the regions as the caller may inspect them (keep_regions) ----
