@@ -381,7 +381,9 @@ int saber_hip_copy_probe(saber_hip_ctx* ctx, uint64_t bytes, int iters, double* 
 
 /* Width -> variant choice used by the planner (the autotuner's output):
  * widths lo..hi (inclusive) use variant shape (L, C, K); L = 0 routes them to
- * the exact int32 kernel. */
+ * the exact int32 kernel. End-cell batches follow the choice only for shapes
+ * of at most 21 cells per lane (C <= 21); for wider ones they keep their own
+ * modelled choice. */
 int saber_hip_band_choose(saber_hip_ctx* ctx, int w_lo, int w_hi, int L, int C, int K);
 
 /* Measured shape choice (no counterpart in the reference; the autotuner of

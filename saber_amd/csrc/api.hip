@@ -371,9 +371,12 @@ int saber_hip_band_choose(saber_hip_ctx* c, int w_lo, int w_hi, int L, int C, in
         if (L != 0) return fail(SABER_HIP_EINVAL, "width above the widest variant");
         w_hi = (int)c->choice.size() - 1;
     }
+    // end-cell batches take the choice too unless the shape's lanes are too
+    // wide for their register copy of a row
+    const bool ends_too = shape < 0 || c->shapes[shape].ends_ok();
     for (int w = w_lo; w <= w_hi; ++w) {
         c->choice[w] = c->choice_lat[w] = shape;
-        if (c->shapes[shape].ends_ok()) c->choice_ends[w] = c->choice_lat_ends[w] = shape;
+        if (ends_too) c->choice_ends[w] = c->choice_lat_ends[w] = shape;
     }
     return SABER_HIP_OK;
 }

@@ -97,3 +97,31 @@ def test_every_shape_under_load(shape):
                         b.close()
     finally:
         ctx.close()
+
+
+def test_choice_of_the_int32_kernel_and_of_wide_shapes_for_end_cells():
+    """saber_hip_band_choose with L = 0 routes a width range to the exact
+    int32 kernel for both builds; a 33-cell shape chosen for a range serves
+    the scores-only build while end-cell batches keep a shape of at most 21
+    cells. Scores and end cells equal the oracle either way."""
+    sch = S.Scheme(BLOSUM62, 11, 1)
+    ctx = S.Context(0)
+    try:
+        qres, qoff, sres, soff, jobs, qb = shape_set(4242, 129, 132, 200, 2)
+        ctx.upload_db(sres, soff)
+        want = O.banded_many(qres, qoff, sres, soff, jobs, BLOSUM62, 11, 1, None)
+        for L, C, K in ((0, 0, 0), (4, 33, 4)):
+            ctx.choose(129, 132, L, C, K)
+            for ends in (False, True):
+                b = ctx.band_batch(qres, qoff, None, None, jobs, sch, ends=ends)
+                try:
+                    b.run()
+                    b.wait()
+                    got = b.read()
+                    for f in ((0, 1, 2) if ends else (0,)):
+                        assert np.array_equal(np.asarray(got[f]).astype(np.int64),
+                                              np.asarray(want[f]).astype(np.int64)), (L, C, K, ends, f)
+                finally:
+                    b.close()
+    finally:
+        ctx.close()
