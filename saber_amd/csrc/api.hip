@@ -124,6 +124,7 @@ struct saber_hip_band_batch {
     // exact int32 kernel: wide / unsuitable jobs known at plan time, plus the
     // packed kernel's saturated jobs known after it ran
     uint32_t ngeneric = 0;
+    uint64_t npacked = 0;      // jobs of the packed kernel
     DevBuf<uint32_t> gen_ids;      // static int32 jobs (known at plan time), in job order
     DevBuf<uint64_t> gen_off;
     DevBuf<int32_t> gen_work;
@@ -766,6 +767,7 @@ int plan_on_host(saber_hip_ctx* c, hipStream_t st, const BandPlanInput& in, sabe
     out->npairs = pairs.size() / 2;
     out->nchunks = host_blocks.size();
     out->ngeneric = (uint32_t)generic.size();
+    out->npacked = (uint64_t)std::count_if(thr_shape.begin(), thr_shape.end(), [](int s) { return s >= 0; });
     // int32 jobs: ids and work offsets
     if (!generic.empty()) {
         std::vector<uint64_t> off(generic.size());
@@ -859,6 +861,7 @@ int saber::band_batch_create_impl(saber_hip_ctx* c, const saber_hip_seqset* quer
     }
     b->nchunks = plan.nchunks;
     b->ngeneric = plan.ngeneric;
+    b->npacked = plan.npacked;
 
     // Score table for the packed kernel: int8 T[a][c] = S(a, c) for codes
     // a, c < 24 (stride 25), column 24 = sentinel and row 24 = dead row
@@ -963,9 +966,14 @@ static GenericArgs generic_args(saber_hip_band_batch* b) {
 static int enqueue_step(saber_hip_band_batch* b, hipEvent_t ev_begin, hipEvent_t ev_end) {
     hipStream_t st = b->ctx->stream;
     const uint64_t n = b->njobs;
-    HIP_TRY(fill_u32(reinterpret_cast<uint32_t*>(b->score.p), n, 0, st));
-    HIP_TRY(fill_u32(b->q_end.p, n, 0, st));
-    HIP_TRY(fill_u32(b->s_end.p, n, 0, st));
+    // Every packed and int32 job's outputs are written by its kernel (a
+    // saturated packed job's by the rescore); only jobs without band cells
+    // need the zeros.
+    if (b->npacked + b->ngeneric < n) {
+        HIP_TRY(fill_u32(reinterpret_cast<uint32_t*>(b->score.p), n, 0, st));
+        HIP_TRY(fill_u32(b->q_end.p, n, 0, st));
+        HIP_TRY(fill_u32(b->s_end.p, n, 0, st));
+    }
     HIP_TRY(fill_u32(b->nflagged.p, 1, 0, st));
     HIP_TRY(fill_u32(b->next_chunk.p, 1, 0, st));
     BandKernelArgs a{};

@@ -83,6 +83,8 @@ struct BandPlanInput {
 struct BandPlan {
     uint64_t npairs = 0, nchunks = 0;
     uint32_t ngeneric = 0;
+    uint64_t npacked = 0;   // jobs of the packed kernel (the rest of the batch
+                            // beyond npacked + ngeneric has no band cells)
     uint64_t gen_work = 0;  // int32 work elements of the generic jobs
     int32_t wmax = 0;       // widest packed job (0: none)
 };

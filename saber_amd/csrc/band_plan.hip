@@ -531,6 +531,7 @@ int band_plan_device(saber_hip_ctx* c, hipStream_t st, const BandPlanInput& in, 
     }
     out->wmax = (int32_t)tot.wmax;
     out->ngeneric = tot.ngeneric;
+    out->npacked = tot.npacked;
     out->gen_work = tot.gen_work;
 
     // int32-kernel jobs in job order, and their work offsets
