@@ -28,6 +28,12 @@ const char* g_err_cstr();
 
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 
+// Device memory of the library's buffers (dev_cache.hip): blocks released
+// with dev_free are cached per device after the device has finished its
+// work, and handed out again by dev_alloc.
+hipError_t dev_alloc(void** p, uint64_t bytes);
+void dev_free(void* p);
+
 template <class T>
 struct DevBuf {
     T* p = nullptr;
@@ -36,7 +42,7 @@ struct DevBuf {
         release();
         n = count;
         if (count == 0) return hipSuccess;
-        return hipMalloc(reinterpret_cast<void**>(&p), sizeof(T) * count);
+        return dev_alloc(reinterpret_cast<void**>(&p), sizeof(T) * count);
     }
     // Keeps the allocation when it already holds count elements.
     hipError_t reserve(uint64_t count) {
@@ -44,7 +50,7 @@ struct DevBuf {
         return alloc(count < 1 ? 1 : count);
     }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) dev_free(p);
         p = nullptr;
         n = 0;
     }
